@@ -1,0 +1,1653 @@
+/*
+ * snap_oracle.c -- CPU restatement (FP64) of snapy's finite-volume RHS and
+ * vertically implicit step.  TEST INFRASTRUCTURE ONLY (see snap_oracle.h).
+ *
+ * The reference library cannot be compiled here: its headers include a
+ * CMake-generated configure.h and Eigen/yaml-cpp/kintera/harp, none of which
+ * exist in this image.  This file is therefore a restatement from reading the
+ * reference sources; every function names the file:line it follows.  What
+ * pins each function (tests under tests/):
+ *
+ *   interpolants      reference KATs (weno5(1..5)=2.5000000000000004,
+ *                     weno3(1,2,3)=1.5, plm(1,2,3)=1.5), WENO3 smooth limit,
+ *                     exactness on polynomials, mirror symmetry, table form ==
+ *                     closed form                      (test_oracle_recon.py)
+ *   reconstruct       dummy-face fill rules, shock/CP5 split, constant and
+ *                     linear data                      (test_oracle_recon.py)
+ *   riemann           reference face-pressure closed forms (seeds 77/78/79),
+ *                     Roe equal states, consistency F(w,w)=Euler flux,
+ *                     mirror symmetry, Sod vs exact Riemann solution
+ *                                                     (test_oracle_riemann.py)
+ *   cons2prim         W->U->W round trip, closed forms (test_oracle_hydro.py)
+ *   divergence        telescoping / uniform state      (test_oracle_hydro.py)
+ *   gravity revision  isentropic closed form           (test_oracle_hydro.py)
+ *   VIC               dense brute-force solve of the assembled block system,
+ *                     3x3 inverse vs LU, reference MS-VIC redistribution test
+ *                     (tests/test_backward_substitution.cpp), column mass
+ *                     conservation                     (test_oracle_vic.py)
+ *   RK3 (harp, not vendored: parity unpinned at the integrator boundary)
+ *                     Shu-Osher closed form, third-order convergence
+ *                                                     (test_oracle_hydro.py)
+ *   Rgas (kintera, not vendored: parity unpinned) CODATA 8.31446261815324.
+ *
+ * Built with -ffp-contract=off so the arithmetic order written here is the
+ * order executed.
+ */
+#include "snap_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define SQR(x) ((x) * (x))
+#define RGAS 8.31446261815324
+
+static int g_threads = 1;
+
+void oc_set_threads(int n) { g_threads = n > 0 ? n : 1; }
+int oc_get_threads(void) { return g_threads; }
+
+/* ------------------------------------------------------------------------ */
+/* grid                                                                      */
+/* ------------------------------------------------------------------------ */
+
+/* src/coord/coordinate.hpp:58-60 */
+int oc_nc1(const oc_config* c) { return c->nx1 > 1 ? c->nx1 + 2 * c->nghost : 1; }
+int oc_nc2(const oc_config* c) { return c->nx2 > 1 ? c->nx2 + 2 * c->nghost : 1; }
+int oc_nc3(const oc_config* c) { return c->nx3 > 1 ? c->nx3 + 2 * c->nghost : 1; }
+
+/* src/coord/cartesian.cpp:19-36: dx?f = ones(nc?) * (xmax - xmin) / nx */
+void oc_cartesian_widths(const oc_config* c, double* dx1f, double* dx2f,
+                         double* dx3f) {
+  int n1 = oc_nc1(c), n2 = oc_nc2(c), n3 = oc_nc3(c);
+  double d1 = (c->x1max - c->x1min) / c->nx1;
+  double d2 = (c->x2max - c->x2min) / c->nx2;
+  double d3 = (c->x3max - c->x3min) / c->nx3;
+  for (int i = 0; i < n1; ++i) dx1f[i] = 1.0 * d1;
+  for (int i = 0; i < n2; ++i) dx2f[i] = 1.0 * d2;
+  for (int i = 0; i < n3; ++i) dx3f[i] = 1.0 * d3;
+}
+
+/* stride (in elements) and length of tensor axis dim (3=x1, 2=x2, 1=x3) */
+static void axis_info(int n3, int n2, int n1, int dim, long* stride, int* len) {
+  if (dim == 3) {
+    *stride = 1;
+    *len = n1;
+  } else if (dim == 2) {
+    *stride = n1;
+    *len = n2;
+  } else {
+    *stride = (long)n2 * n1;
+    *len = n3;
+  }
+}
+
+/* offsets of the first cell of every pencil along dim */
+static long pencil_count(int n3, int n2, int n1, int dim) {
+  if (dim == 3) return (long)n3 * n2;
+  if (dim == 2) return (long)n3 * n1;
+  return (long)n2 * n1;
+}
+
+static long pencil_base(int n3, int n2, int n1, int dim, long p) {
+  if (dim == 3) return p * n1; /* p = k*n2 + j */
+  if (dim == 2) {
+    long k = p / n1, i = p % n1;
+    return k * (long)n2 * n1 + i;
+  }
+  return p; /* p = j*n1 + i */
+}
+
+/* ------------------------------------------------------------------------ */
+/* scalar interpolants: src/recon/interp_simple.hpp                          */
+/* ------------------------------------------------------------------------ */
+
+/* interp_simple.hpp:76-79 */
+double oc_interp_cp3(double phim1, double phi, double phip1) {
+  return 1. / 3. * phim1 + 5. / 6. * phi - 1. / 6. * phip1;
+}
+
+/* interp_simple.hpp:109-114 */
+double oc_interp_cp5(double phim2, double phim1, double phi, double phip1,
+                     double phip2) {
+  return -1. / 20. * phim2 + 9. / 20. * phim1 + 47. / 60. * phi -
+         13. / 60. * phip1 + 1. / 30. * phip2;
+}
+
+/* interp_simple.hpp:91-104 */
+double oc_interp_weno3(double phim1, double phi, double phip1) {
+  double p0 = (1.0 / 2.0) * phi + (1.0 / 2.0) * phim1;
+  double p1 = (-1.0 / 2.0) * phip1 + (3.0 / 2.0) * phi;
+  double beta0 = (phim1 - phi) * (phim1 - phi);
+  double beta1 = (phi - phip1) * (phi - phip1);
+  double alpha0 = (2.0 / 3.0) / SQR(beta0 + 1e-6);
+  double alpha1 = (1.0 / 3.0) / SQR(beta1 + 1e-6);
+  return (alpha0 * p0 + alpha1 * p1) / (alpha0 + alpha1);
+}
+
+/* interp_simple.hpp:133-153 */
+double oc_interp_weno5(double phim2, double phim1, double phi, double phip1,
+                       double phip2) {
+  double p0 = (1. / 3.) * phi + (5. / 6.) * phim1 - (1. / 6.) * phim2;
+  double p1 = (-1. / 6.) * phip1 + (5. / 6.) * phi + (1. / 3.) * phim1;
+  double p2 = (1. / 3.) * phip2 - (7. / 6.) * phip1 + (11. / 6.) * phi;
+  double beta0 = 13. / 12. * SQR(phi - 2. * phim1 + phim2) +
+                 .25 * SQR(3. * phi - 4. * phim1 + phim2);
+  double beta1 =
+      13. / 12. * SQR(phip1 - 2. * phi + phim1) + .25 * SQR(phip1 - phim1);
+  double beta2 = 13. / 12. * SQR(phip2 - 2. * phip1 + phi) +
+                 .25 * SQR(phip2 - 4. * phip1 + 3. * phi);
+  double alpha0 = .3 / SQR(beta0 + 1e-6);
+  double alpha1 = .6 / SQR(beta1 + 1e-6);
+  double alpha2 = .1 / SQR(beta2 + 1e-6);
+  return (alpha0 * p0 + alpha1 * p1 + alpha2 * p2) / (alpha0 + alpha1 + alpha2);
+}
+
+/* interp_simple.hpp:57-63 */
+double oc_interp_plm(double phim1, double phi, double phip1) {
+  double dwl = phi - phim1;
+  double dwr = phip1 - phi;
+  double dw2 = dwl * dwr;
+  double dwm = dw2 < 0. ? 0. : 2. * dw2 / (dwl + dwr);
+  return phi - 0.5 * dwm;
+}
+
+/* ------------------------------------------------------------------------ */
+/* table-driven window kernels: src/recon/interp_impl.h (the CPU module path) */
+/* ------------------------------------------------------------------------ */
+
+/* coefficient tables: weno5 src/recon/weno5.cpp:10-24 (cp = cm.flip(1)),
+   weno3 src/recon/weno3.cpp:10-18, cp5 src/recon/cp5.cpp:12-17,
+   cp3 src/recon/cp3.cpp:10-14 */
+static const double W5CM[9][5] = {{-1. / 6., 5. / 6., 1. / 3., 0., 0.},
+                                  {0., 1. / 3., 5. / 6., -1. / 6., 0.},
+                                  {0., 0., 11. / 6., -7. / 6., 1. / 3.},
+                                  {1., -2., 1., 0., 0.},
+                                  {1., -4., 3., 0., 0.},
+                                  {0., 1., -2., 1., 0.},
+                                  {0., -1., 0., 1., 0.},
+                                  {0., 0., 1., -2., 1.},
+                                  {0., 0., 3., -4., 1.}};
+static const double W3CM[4][3] = {{1. / 2., 1. / 2., 0.},
+                                  {0., 3. / 2., -1. / 2.},
+                                  {1., -1., 0.},
+                                  {0., 1., -1.}};
+static const double C5CM[5] = {-1. / 20., 9. / 20., 47. / 60., -13. / 60.,
+                               1. / 30.};
+static const double C3CM[3] = {1. / 3., 5. / 6., -1. / 6.};
+
+static double W5CP[9][5], W3CP[4][3], C5CP[5], C3CP[3];
+static int g_tables_ready = 0;
+
+static void init_tables(void) {
+  if (g_tables_ready) return;
+  for (int r = 0; r < 9; ++r)
+    for (int k = 0; k < 5; ++k) W5CP[r][k] = W5CM[r][4 - k];
+  for (int r = 0; r < 4; ++r)
+    for (int k = 0; k < 3; ++k) W3CP[r][k] = W3CM[r][2 - k];
+  for (int k = 0; k < 5; ++k) C5CP[k] = C5CM[4 - k];
+  for (int k = 0; k < 3; ++k) C3CP[k] = C3CM[2 - k];
+  g_tables_ready = 1;
+}
+
+/* interp_impl.h:12-19 (_vvdot) */
+static double vvdot(const double* a, const double* b, int n) {
+  double out = 0.;
+  for (int i = 0; i < n; ++i) out += a[i] * b[i];
+  return out;
+}
+
+/* interp_impl.h:22-31 */
+static double poly_win(const double* in, long s, const double* c, int n) {
+  double out = 0.;
+  for (int i = 0; i < n; ++i) out += c[i] * in[i * s];
+  return out;
+}
+
+/* interp_impl.h:34-70 */
+static double weno3_win(const double* in, long s, const double (*c)[3],
+                        int scale) {
+  double phi[3];
+  double vscale =
+      scale ? (fabs(in[0]) + fabs(in[s]) + fabs(in[2 * s])) / 3.0 : 1.0;
+  if (vscale == 0.0) return 0.0;
+  for (int k = 0; k < 3; ++k) phi[k] = in[k * s] / vscale;
+  double p0 = vvdot(phi, c[0], 3);
+  double p1 = vvdot(phi, c[1], 3);
+  double beta0 = SQR(vvdot(phi, c[2], 3));
+  double beta1 = SQR(vvdot(phi, c[3], 3));
+  double alpha0 = (2.0 / 3.0) / SQR(beta0 + 1e-6);
+  double alpha1 = (1.0 / 3.0) / SQR(beta1 + 1e-6);
+  return (alpha0 * p0 + alpha1 * p1) / (alpha0 + alpha1) * vscale;
+}
+
+/* interp_impl.h:73-122 */
+static double weno5_win(const double* in, long s, const double (*c)[5],
+                        int scale) {
+  double phi[5];
+  double vscale = scale ? (fabs(in[0]) + fabs(in[s]) + fabs(in[2 * s]) +
+                           fabs(in[3 * s]) + fabs(in[4 * s])) /
+                              5.0
+                        : 1.0;
+  if (vscale == 0.0) return 0.0;
+  for (int k = 0; k < 5; ++k) phi[k] = in[k * s] / vscale;
+  double p0 = vvdot(phi, c[0], 5);
+  double p1 = vvdot(phi, c[1], 5);
+  double p2 = vvdot(phi, c[2], 5);
+  double beta0 =
+      13. / 12. * SQR(vvdot(phi, c[3], 5)) + .25 * SQR(vvdot(phi, c[4], 5));
+  double beta1 =
+      13. / 12. * SQR(vvdot(phi, c[5], 5)) + .25 * SQR(vvdot(phi, c[6], 5));
+  double beta2 =
+      13. / 12. * SQR(vvdot(phi, c[7], 5)) + .25 * SQR(vvdot(phi, c[8], 5));
+  double alpha0 = .3 / SQR(beta0 + 1e-6);
+  double alpha1 = .6 / SQR(beta1 + 1e-6);
+  double alpha2 = .1 / SQR(beta2 + 1e-6);
+  return vscale * (alpha0 * p0 + alpha1 * p1 + alpha2 * p2) /
+         (alpha0 + alpha1 + alpha2);
+}
+
+static int interp_stencils(int type) {
+  switch (type) {
+    case OC_INTERP_DC:
+      return 1;
+    case OC_INTERP_PLM:
+    case OC_INTERP_CP3:
+    case OC_INTERP_WENO3:
+      return 3;
+    default:
+      return 5;
+  }
+}
+
+/* the companion interpolant used for velocity/pressure when shock == false
+   (src/recon/interpolation.cpp:21-36) */
+static int interp2_type(int type) {
+  if (type == OC_INTERP_WENO5) return OC_INTERP_CP5;
+  if (type == OC_INTERP_WENO3) return OC_INTERP_CP3;
+  return type;
+}
+
+/* InterpImpl::forward for one variable pencil: writes the lower-face value of
+   every cell into irt[il-1 .. iu] and the upper-face value into ilt[il ..
+   iu+1] (faces indexed like cells; src/recon/reconstruct.cpp:46-61 and the
+   left/right kernels) */
+static void interp_pencil(int type, int scale, const double* w, long s, int n,
+                          double* irt, double* ilt) {
+  int N = interp_stencils(type);
+  int il = N / 2 + 1;
+  int iu = n - il;
+  int nout = iu - il + 2; /* = n - N + 1 windows for N > 1 */
+  (void)nout;
+  if (type == OC_INTERP_DC) {
+    /* DonorCellInterpImpl::left/right (src/recon/interpolation.hpp:103-108) */
+    for (int f = 0; f < n; ++f) irt[f * s] = w[f * s];
+    for (int f = 1; f < n; ++f) ilt[f * s] = w[(f - 1) * s];
+    return;
+  }
+  if (type == OC_INTERP_PLM) {
+    /* PLMInterpImpl::forward (src/recon/plm.cpp:14-40) */
+    for (int c = 1; c <= n - 2; ++c) {
+      double dwl = w[c * s] - w[(c - 1) * s];
+      double dwr = w[(c + 1) * s] - w[c * s];
+      double dw2 = dwl * dwr;
+      double dwm = 2. * dw2 / (dwl + dwr + (double)FLT_MIN);
+      dwm *= (double)(dw2 >= 0);
+      irt[c * s] = w[c * s] - 0.5 * dwm;
+      ilt[(c + 1) * s] = w[c * s] + 0.5 * dwm;
+    }
+    return;
+  }
+  /* windowed kernels: window m starts at cell m, centre c = m + N/2 */
+  int half = N / 2;
+  for (int m = 0; m + N <= n; ++m) {
+    const double* win = w + m * s;
+    double lo, hi;
+    switch (type) {
+      case OC_INTERP_CP3:
+        lo = poly_win(win, s, C3CM, 3);
+        hi = poly_win(win, s, C3CP, 3);
+        break;
+      case OC_INTERP_CP5:
+        lo = poly_win(win, s, C5CM, 5);
+        hi = poly_win(win, s, C5CP, 5);
+        break;
+      case OC_INTERP_WENO3:
+        lo = weno3_win(win, s, W3CM, scale);
+        hi = weno3_win(win, s, W3CP, scale);
+        break;
+      default:
+        lo = weno5_win(win, s, W5CM, scale);
+        hi = weno5_win(win, s, W5CP, scale);
+        break;
+    }
+    int c = m + half;
+    irt[c * s] = lo;
+    ilt[(c + 1) * s] = hi;
+  }
+}
+
+/* one window of the table-driven Interp modules: right = 0 gives left()
+   (value at the lower face of the window centre, cm tables), right = 1 gives
+   right() (upper face, cp tables); src/recon/{weno5,weno3,cp5,cp3}.cpp */
+double oc_interp_window(int type, int right, int scale, const double* phi) {
+  init_tables();
+  switch (type) {
+    case OC_INTERP_CP3:
+      return poly_win(phi, 1, right ? C3CP : C3CM, 3);
+    case OC_INTERP_CP5:
+      return poly_win(phi, 1, right ? C5CP : C5CM, 5);
+    case OC_INTERP_WENO3:
+      return weno3_win(phi, 1, right ? W3CP : W3CM, scale);
+    case OC_INTERP_WENO5:
+      return weno5_win(phi, 1, right ? W5CP : W5CM, scale);
+    default:
+      return phi[0];
+  }
+}
+
+/* ReconstructImpl::forward (src/recon/reconstruct.cpp:75-154) with
+   _apply_inplace (:46-61).  wlr is [2][nvar][n3][n2][n1], zeroed here. */
+void oc_reconstruct(const double* w, double* wlr, int nvar, int n3, int n2,
+                    int n1, int dim, int type, int scale, int shock,
+                    int has_parent, int limiter, double dfloor,
+                    double pfloor) {
+  init_tables();
+  long ncell = (long)n3 * n2 * n1;
+  memset(wlr, 0, sizeof(double) * 2 * nvar * ncell);
+  long s;
+  int n;
+  axis_info(n3, n2, n1, dim, &s, &n);
+  int N = interp_stencils(type);
+  int il = N / 2 + 1;
+  int iu = n - il;
+  long np = pencil_count(n3, n2, n1, dim);
+  double* ilt_all = wlr;
+  double* irt_all = wlr + nvar * ncell;
+
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+  for (long p = 0; p < np; ++p) {
+    long base = pencil_base(n3, n2, n1, dim, p);
+    for (int v = 0; v < nvar; ++v) {
+      int t = type;
+      if (has_parent && !shock && v >= OC_IVX && v <= OC_IPR)
+        t = interp2_type(type);
+      const double* wp = w + v * ncell + base;
+      double* irt = irt_all + v * ncell + base;
+      double* ilt = ilt_all + v * ncell + base;
+      interp_pencil(t, scale, wp, s, n, irt, ilt);
+      /* dummy regions (reconstruct.cpp:55-60) */
+      double r_il = irt[il * s], r_iu = irt[iu * s];
+      double l_il = ilt[il * s], l_iu = ilt[iu * s];
+      for (int f = 0; f < il; ++f) {
+        irt[f * s] = r_il;
+        ilt[f * s] = l_il;
+      }
+      for (int f = iu + 1; f < n; ++f) {
+        irt[f * s] = r_iu;
+        ilt[f * s] = l_iu;
+      }
+    }
+  }
+
+  if (!has_parent) {
+    /* reconstruct.cpp:89-93 */
+    for (long q = 0; q < 2 * nvar * ncell; ++q)
+      if (wlr[q] < 0.) wlr[q] = 0.;
+    return;
+  }
+  if (shock || !limiter) return;
+  /* reconstruct.cpp:128-151 */
+  for (int side = 0; side < 2; ++side) {
+    double* base = wlr + side * nvar * ncell;
+    for (long q = 0; q < ncell; ++q) {
+      if (base[OC_IDN * ncell + q] < dfloor) base[OC_IDN * ncell + q] = dfloor;
+      if (nvar > OC_IPR && base[OC_IPR * ncell + q] < pfloor)
+        base[OC_IPR * ncell + q] = pfloor;
+      for (int v = OC_ICY; v < nvar; ++v)
+        if (base[v * ncell + q] < 0.) base[v * ncell + q] = 0.;
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Riemann solvers                                                           */
+/* ------------------------------------------------------------------------ */
+
+#define WL(n) wl[(n) * sw]
+#define WR(n) wr[(n) * sw]
+#define FLX(n) flx[(n) * sf]
+
+/* src/riemann/lmars_impl.h:16-78; hl/hr enter as W->I / rho
+   (src/riemann/lmars.cpp:34-44) */
+static void lmars_face(double* flx, const double* wl, const double* wr,
+                       double hl, double hr, double gammal, double gammar,
+                       int dim, int ny, long sw, long sf, double* fp) {
+  int ivx = OC_IPR - dim;
+  int ivy = OC_IVX + ((ivx - OC_IVX) + 1) % 3;
+  int ivz = OC_IVX + ((ivx - OC_IVX) + 2) % 3;
+  double wli[5] = {WL(OC_IDN), WL(ivx), WL(ivy), WL(ivz), WL(OC_IPR)};
+  double wri[5] = {WR(OC_IDN), WR(ivx), WR(ivy), WR(ivz), WR(OC_IPR)};
+  hl += 0.5 * (SQR(wli[1]) + SQR(wli[2]) + SQR(wli[3])) + wli[4] / wli[0];
+  hr += 0.5 * (SQR(wri[1]) + SQR(wri[2]) + SQR(wri[3])) + wri[4] / wri[0];
+  double rhobar = 0.5 * (wli[0] + wri[0]);
+  double gamma_bar = 0.5 * (gammal + gammar);
+  double cbar = sqrt(0.5 * gamma_bar * (wli[4] + wri[4]) / rhobar);
+  double pbar = 0.5 * (wli[4] + wri[4]) + 0.5 * (rhobar * cbar) * (wli[1] - wri[1]);
+  if (fp) *fp = pbar;
+  double ubar =
+      0.5 * (wli[1] + wri[1]) + 0.5 / (rhobar * cbar) * (wli[4] - wri[4]);
+  double rd = 1.0;
+  if (ubar > 0.0) {
+    for (int n = 0; n < ny; n++) rd -= WL(OC_ICY + n);
+    FLX(OC_IDN) = ubar * wli[0] * rd;
+    for (int n = 0; n < ny; n++) FLX(OC_ICY + n) = ubar * wli[0] * WL(OC_ICY + n);
+    FLX(ivx) = ubar * wli[0] * wli[1] + pbar;
+    FLX(ivy) = ubar * wli[0] * wli[2];
+    FLX(ivz) = ubar * wli[0] * wli[3];
+    FLX(OC_IPR) = ubar * wli[0] * hl;
+  } else {
+    for (int n = 0; n < ny; n++) rd -= WR(OC_ICY + n);
+    FLX(OC_IDN) = ubar * wri[0] * rd;
+    for (int n = 0; n < ny; n++) FLX(OC_ICY + n) = ubar * wri[0] * WR(OC_ICY + n);
+    FLX(ivx) = ubar * wri[0] * wri[1] + pbar;
+    FLX(ivy) = ubar * wri[0] * wri[2];
+    FLX(ivz) = ubar * wri[0] * wri[3];
+    FLX(OC_IPR) = ubar * wri[0] * hr;
+  }
+}
+
+/* src/riemann/hllc_impl.h:16-130; el/er = W->I, cl/cr = WA->L
+   (src/riemann/hllc.cpp:34-54) */
+static void hllc_face(double* flx, const double* wl, const double* wr,
+                      double el, double er, double gammal, double gammar,
+                      double cl, double cr, int dim, int ny, long sw, long sf,
+                      double* fp) {
+  double TINY_NUMBER = 1.0e-10;
+  int ivx = OC_IPR - dim;
+  int ivy = OC_IVX + ((ivx - OC_IVX) + 1) % 3;
+  int ivz = OC_IVX + ((ivx - OC_IVX) + 2) % 3;
+
+  el += 0.5 * WL(OC_IDN) * (SQR(WL(OC_IVX)) + SQR(WL(OC_IVY)) + SQR(WL(OC_IVZ)));
+  er += 0.5 * WR(OC_IDN) * (SQR(WR(OC_IVX)) + SQR(WR(OC_IVY)) + SQR(WR(OC_IVZ)));
+
+  double rhoa = .5 * (WL(OC_IDN) + WR(OC_IDN));
+  double ca = .5 * (cl + cr);
+  double pmid = .5 * (WL(OC_IPR) + WR(OC_IPR) + (WL(ivx) - WR(ivx)) * rhoa * ca);
+  double umid = .5 * (WL(ivx) + WR(ivx) + (WL(OC_IPR) - WR(OC_IPR)) / (rhoa * ca));
+  (void)umid;
+
+  double ql = (pmid <= WL(OC_IPR))
+                  ? 1.0
+                  : sqrt(1.0 + (gammal + 1) / (2 * gammal) * (pmid / WL(OC_IPR) - 1.0));
+  double qr = (pmid <= WR(OC_IPR))
+                  ? 1.0
+                  : sqrt(1.0 + (gammar + 1) / (2 * gammar) * (pmid / WR(OC_IPR) - 1.0));
+
+  double al = WL(ivx) - cl * ql;
+  double ar = WR(ivx) + cr * qr;
+  double bp = ar > 0.0 ? ar : (TINY_NUMBER);
+  double bm = al < 0.0 ? al : -(TINY_NUMBER);
+
+  double vxl = WL(ivx) - al;
+  double vxr = WR(ivx) - ar;
+  double tl = WL(OC_IPR) + vxl * WL(OC_IDN) * WL(ivx);
+  double tr = WR(OC_IPR) + vxr * WR(OC_IDN) * WR(ivx);
+  double ml = WL(OC_IDN) * vxl;
+  double mr = -(WR(OC_IDN) * vxr);
+  double am = (tl - tr) / (ml + mr);
+  double cp = (ml * tr + mr * tl) / (ml + mr);
+  cp = cp > 0.0 ? cp : 0.0;
+  if (fp) *fp = cp;
+
+  vxl = WL(ivx) - bm;
+  vxr = WR(ivx) - bp;
+
+  double rdl = 1., rdr = 1.;
+  for (int n = 0; n < ny; ++n) {
+    rdl -= WL(OC_ICY + n);
+    rdr -= WR(OC_ICY + n);
+  }
+  double fl[5], fr[5];
+  fl[OC_IDN] = WL(OC_IDN) * vxl * rdl;
+  fr[OC_IDN] = WR(OC_IDN) * vxr * rdr;
+  fl[ivx] = WL(OC_IDN) * WL(ivx) * vxl + WL(OC_IPR);
+  fr[ivx] = WR(OC_IDN) * WR(ivx) * vxr + WR(OC_IPR);
+  fl[ivy] = WL(OC_IDN) * WL(ivy) * vxl;
+  fr[ivy] = WR(OC_IDN) * WR(ivy) * vxr;
+  fl[ivz] = WL(OC_IDN) * WL(ivz) * vxl;
+  fr[ivz] = WR(OC_IDN) * WR(ivz) * vxr;
+  fl[OC_IPR] = el * vxl + WL(OC_IPR) * WL(ivx);
+  fr[OC_IPR] = er * vxr + WR(OC_IPR) * WR(ivx);
+
+  double sl, sr, sm;
+  if (am >= 0.0) {
+    sl = am / (am - bm);
+    sr = 0.0;
+    sm = -bm / (am - bm);
+  } else {
+    sl = 0.0;
+    sr = -am / (bp - am);
+    sm = bp / (bp - am);
+  }
+  FLX(OC_IDN) = sl * fl[OC_IDN] + sr * fr[OC_IDN];
+  FLX(ivx) = sl * fl[ivx] + sr * fr[ivx] + sm * cp;
+  FLX(ivy) = sl * fl[ivy] + sr * fr[ivy];
+  FLX(ivz) = sl * fl[ivz] + sr * fr[ivz];
+  FLX(OC_IPR) = sl * fl[OC_IPR] + sr * fr[OC_IPR] + sm * cp * am;
+  for (int n = 0; n < ny; ++n) {
+    double fln = WL(OC_IDN) * WL(OC_ICY + n) * vxl;
+    double frn = WR(OC_IDN) * WR(OC_ICY + n) * vxr;
+    FLX(OC_ICY + n) = sl * fln + sr * frn;
+  }
+}
+
+/* src/riemann/roe_impl.h:23-211 (dry ideal-gas branch: ideal_moist false);
+   el/er = W->I, cl/cr = WA->L (src/riemann/roe.cpp:32-41) */
+static void roe_face(double* flx, const double* wl, const double* wr,
+                     double el, double er, double gammal, double gammar,
+                     double cl, double cr, int dim, int ny, long sw, long sf,
+                     double* fp) {
+  double TINY_NUMBER = 1.0e-10;
+  int ivx = OC_IPR - dim;
+  int ivy = OC_IVX + ((ivx - OC_IVX) + 1) % 3;
+  int ivz = OC_IVX + ((ivx - OC_IVX) + 2) % 3;
+
+  double etl = el + 0.5 * WL(OC_IDN) * (SQR(WL(OC_IVX)) + SQR(WL(OC_IVY)) + SQR(WL(OC_IVZ)));
+  double etr = er + 0.5 * WR(OC_IDN) * (SQR(WR(OC_IVX)) + SQR(WR(OC_IVY)) + SQR(WR(OC_IVZ)));
+
+  double sqrtdl = sqrt(WL(OC_IDN));
+  double sqrtdr = sqrt(WR(OC_IDN));
+  double isdlpdr = 1.0 / (sqrtdl + sqrtdr);
+
+  double rhobar = sqrtdl * sqrtdr;
+  double v1 = (sqrtdl * WL(ivx) + sqrtdr * WR(ivx)) * isdlpdr;
+  double v2 = (sqrtdl * WL(ivy) + sqrtdr * WR(ivy)) * isdlpdr;
+  double v3 = (sqrtdl * WL(ivz) + sqrtdr * WR(ivz)) * isdlpdr;
+  double hl = (etl + WL(OC_IPR)) / WL(OC_IDN);
+  double hr = (etr + WR(OC_IPR)) / WR(OC_IDN);
+  double h = (hl * sqrtdl + hr * sqrtdr) * isdlpdr;
+
+  double dryl = 1.0, dryr = 1.0;
+  for (int n = 0; n < ny; ++n) {
+    dryl -= WL(OC_ICY + n);
+    dryr -= WR(OC_ICY + n);
+  }
+  double rhol0 = WL(OC_IDN) * dryl;
+  double rhor0 = WR(OC_IDN) * dryr;
+
+  double fl0 = rhol0 * WL(ivx);
+  double fr0 = rhor0 * WR(ivx);
+  double fl1 = WL(OC_IDN) * WL(ivx) * WL(ivx) + WL(OC_IPR);
+  double fr1 = WR(OC_IDN) * WR(ivx) * WR(ivx) + WR(OC_IPR);
+  double fl2 = WL(OC_IDN) * WL(ivx) * WL(ivy);
+  double fr2 = WR(OC_IDN) * WR(ivx) * WR(ivy);
+  double fl3 = WL(OC_IDN) * WL(ivx) * WL(ivz);
+  double fr3 = WR(OC_IDN) * WR(ivx) * WR(ivz);
+  double fl4 = (etl + WL(OC_IPR)) * WL(ivx);
+  double fr4 = (etr + WR(OC_IPR)) * WR(ivx);
+
+  double qbar[64];
+  qbar[0] = (rhol0 / sqrtdl + rhor0 / sqrtdr) * isdlpdr;
+  for (int n = 0; n < ny; ++n) {
+    double rholn = WL(OC_IDN) * WL(OC_ICY + n);
+    double rhorn = WR(OC_IDN) * WR(OC_ICY + n);
+    qbar[1 + n] = (rholn / sqrtdl + rhorn / sqrtdr) * isdlpdr;
+  }
+
+  double gamma_roe = 0.5 * (gammal + gammar);
+  double offset = 0.0;
+
+  double du1 = WR(OC_IDN) * WR(ivx) - WL(OC_IDN) * WL(ivx);
+  double du2 = WR(OC_IDN) * WR(ivy) - WL(OC_IDN) * WL(ivy);
+  double du3 = WR(OC_IDN) * WR(ivz) - WL(OC_IDN) * WL(ivz);
+  double du4 = etr - etl;
+
+  FLX(OC_IDN) = 0.5 * (fl0 + fr0);
+  FLX(ivx) = 0.5 * (fl1 + fr1);
+  FLX(ivy) = 0.5 * (fl2 + fr2);
+  FLX(ivz) = 0.5 * (fl3 + fr3);
+  FLX(OC_IPR) = 0.5 * (fl4 + fr4);
+  for (int n = 0; n < ny; ++n) {
+    double rholn = WL(OC_IDN) * WL(OC_ICY + n);
+    double rhorn = WR(OC_IDN) * WR(OC_ICY + n);
+    FLX(OC_ICY + n) = 0.5 * (rholn * WL(ivx) + rhorn * WR(ivx));
+  }
+
+  double vsq = v1 * v1 + v2 * v2 + v3 * v3;
+  double gm1_roe = gamma_roe - 1.0;
+  double q = h - 0.5 * vsq - offset;
+  /* roe_max(a, b) = a > b ? a : b (roe_impl.h:19-22) */
+  double qm = q > TINY_NUMBER ? q : TINY_NUMBER;
+  double cs_sq = gm1_roe * qm > TINY_NUMBER ? gm1_roe * qm : TINY_NUMBER;
+  double cs = sqrt(cs_sq);
+  if (fp) *fp = 0.5 * (WL(OC_IPR) + WR(OC_IPR) + rhobar * cs * (WL(ivx) - WR(ivx)));
+
+  double lam_m = v1 - cs;
+  double lam_0 = v1;
+  double lam_p = v1 + cs;
+  double spd_m = fabs(lam_m);
+  double spd_0 = fabs(lam_0);
+  double spd_p = fabs(lam_p);
+
+  int llf_flag = 0;
+  double du = WR(ivx) - WL(ivx);
+  double dv = WR(ivy) - WL(ivy);
+  double dw = WR(ivz) - WL(ivz);
+  double dp = WR(OC_IPR) - WL(OC_IPR);
+
+  double alpha_m = -0.5 * rhobar / cs * du + 0.5 * dp / cs_sq;
+  double alpha_p = +0.5 * rhobar / cs * du + 0.5 * dp / cs_sq;
+  double alpha_v = rhobar * dv;
+  double alpha_w = rhobar * dw;
+
+  double alpha0 = 0.0;
+  double alpha_comp0 = rhor0 - rhol0 - dp / cs_sq * qbar[0];
+  double rho_first = rhol0 + alpha_m * qbar[0];
+  if (rho_first < 0.0) llf_flag = 1;
+  double rho_second = rho_first + alpha_comp0;
+  if (rho_second < 0.0) llf_flag = 1;
+  alpha0 += alpha_comp0;
+  FLX(OC_IDN) -= 0.5 * (spd_m * alpha_m * qbar[0] + spd_0 * alpha_comp0 +
+                        spd_p * alpha_p * qbar[0]);
+  for (int n = 0; n < ny; ++n) {
+    double rholn = WL(OC_IDN) * WL(OC_ICY + n);
+    double rhorn = WR(OC_IDN) * WR(OC_ICY + n);
+    double alpha_n = rhorn - rholn - dp / cs_sq * qbar[1 + n];
+    double rho1 = rholn + alpha_m * qbar[1 + n];
+    if (rho1 < 0.0) llf_flag = 1;
+    double rho2 = rho1 + alpha_n;
+    if (rho2 < 0.0) llf_flag = 1;
+    alpha0 += alpha_n;
+    FLX(OC_ICY + n) -= 0.5 * (spd_m * alpha_m * qbar[1 + n] + spd_0 * alpha_n +
+                              spd_p * alpha_p * qbar[1 + n]);
+  }
+
+  FLX(ivx) -= 0.5 * (spd_m * alpha_m * (v1 - cs) + spd_0 * (v1 * alpha0) +
+                     spd_p * alpha_p * (v1 + cs));
+  FLX(ivy) -= 0.5 * (spd_m * alpha_m * v2 + spd_0 * (v2 * alpha0 + alpha_v) +
+                     spd_p * alpha_p * v2);
+  FLX(ivz) -= 0.5 * (spd_m * alpha_m * v3 + spd_0 * (v3 * alpha0 + alpha_w) +
+                     spd_p * alpha_p * v3);
+  FLX(OC_IPR) -= 0.5 * (spd_m * alpha_m * (h - v1 * cs) +
+                        spd_0 * (rhobar * (hr - hl - v1 * du) + alpha0 * h - dp) +
+                        spd_p * alpha_p * (h + v1 * cs));
+
+  if (lam_m > 0.0) {
+    FLX(OC_IDN) = fl0;
+    FLX(ivx) = fl1;
+    FLX(ivy) = fl2;
+    FLX(ivz) = fl3;
+    FLX(OC_IPR) = fl4;
+    for (int n = 0; n < ny; ++n) FLX(OC_ICY + n) = WL(OC_IDN) * WL(OC_ICY + n) * WL(ivx);
+  }
+  if (lam_p < 0.0) {
+    FLX(OC_IDN) = fr0;
+    FLX(ivx) = fr1;
+    FLX(ivy) = fr2;
+    FLX(ivz) = fr3;
+    FLX(OC_IPR) = fr4;
+    for (int n = 0; n < ny; ++n) FLX(OC_ICY + n) = WR(OC_IDN) * WR(OC_ICY + n) * WR(ivx);
+  }
+  if (llf_flag) {
+    double x = fabs(WL(ivx)) + cl, y = fabs(WR(ivx)) + cr;
+    double a = 0.5 * (x > y ? x : y);
+    FLX(OC_IDN) = 0.5 * (fl0 + fr0) - a * (rhor0 - rhol0);
+    FLX(ivx) = 0.5 * (fl1 + fr1) - a * du1;
+    FLX(ivy) = 0.5 * (fl2 + fr2) - a * du2;
+    FLX(ivz) = 0.5 * (fl3 + fr3) - a * du3;
+    FLX(OC_IPR) = 0.5 * (fl4 + fr4) - a * du4;
+    for (int n = 0; n < ny; ++n) {
+      double rholn = WL(OC_IDN) * WL(OC_ICY + n);
+      double rhorn = WR(OC_IDN) * WR(OC_ICY + n);
+      FLX(OC_ICY + n) = 0.5 * (rholn * WL(ivx) + rhorn * WR(ivx)) - a * (rhorn - rholn);
+    }
+  }
+}
+
+#undef WL
+#undef WR
+#undef FLX
+
+/* RiemannSolverImpl::forward for LMARS/HLLC/Roe (src/riemann/lmars.cpp:30-103,
+   src/riemann/hllc.cpp:30-106, src/riemann/roe.cpp:14-59): the EOS helpers
+   W->I, W->A, WA->L of the ideal gas (src/eos/ideal_gas.cpp:30-65) feed the
+   per-face kernel over every face of the [n3, n2, n1] block. */
+void oc_riemann(int solver, double* wl, double* wr, double* flx, double* fp,
+                int nvar, int n3, int n2, int n1, int dim, double gammad) {
+  long ncell = (long)n3 * n2 * n1;
+  int ny = nvar - OC_ICY;
+  if (ny < 0) ny = 0;
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+  for (long q = 0; q < ncell; ++q) {
+    const double* a = wl + q;
+    const double* b = wr + q;
+    double* f = flx + q;
+    double* fq = fp ? fp + q : NULL;
+    double pl = a[OC_IPR * ncell], pr = b[OC_IPR * ncell];
+    double dl = a[OC_IDN * ncell], dr = b[OC_IDN * ncell];
+    double gl = gammad * 1.0, gr = gammad * 1.0; /* W->A */
+    double eil = pl / (gammad - 1), eir = pr / (gammad - 1); /* W->I */
+    if (solver == OC_RIEMANN_LMARS) {
+      lmars_face(f, a, b, eil / dl, eir / dr, gl, gr, dim, ny, ncell, ncell, fq);
+    } else {
+      double cl = sqrt(gl * pl / dl), cr = sqrt(gr * pr / dr); /* WA->L */
+      if (solver == OC_RIEMANN_HLLC)
+        hllc_face(f, a, b, eil, eir, gl, gr, cl, cr, dim, ny, ncell, ncell, fq);
+      else
+        roe_face(f, a, b, eil, eir, gl, gr, cl, cr, dim, ny, ncell, ncell, fq);
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* EOS                                                                       */
+/* ------------------------------------------------------------------------ */
+
+/* EquationOfStateImpl::apply_conserved_limiter_
+   (src/eos/equation_of_state.cpp:110-164), dry ideal gas */
+static void conserved_limiter(const oc_config* c, double* u, int64_t ncell) {
+  if (!c->limiter) return;
+  double cv = RGAS / c->weight / (c->gammad - 1.);
+  for (int64_t q = 0; q < 5 * ncell; ++q)
+    if (isnan(u[q])) u[q] = 0.;
+  for (int64_t q = 0; q < ncell; ++q) {
+    double* d = u + OC_IDN * ncell + q;
+    if (*d < c->density_floor) *d = c->density_floor;
+    double m1 = u[OC_IVX * ncell + q], m2 = u[OC_IVY * ncell + q],
+           m3 = u[OC_IVZ * ncell + q];
+    double rho = *d;
+    double ke = 0.5 * (m1 * m1 + m2 * m2 + m3 * m3) / rho;
+    double min_ie = rho * cv * c->temperature_floor;
+    double* e = u + OC_IPR * ncell + q;
+    double lo = ke + min_ie;
+    if (*e < lo) *e = lo;
+  }
+}
+
+/* EquationOfStateImpl::apply_primitive_limiter_
+   (src/eos/equation_of_state.cpp:166-179), dry ideal gas */
+static void primitive_limiter(const oc_config* c, double* w, int64_t ncell) {
+  if (!c->limiter) return;
+  for (int64_t q = 0; q < 5 * ncell; ++q)
+    if (isnan(w[q])) w[q] = 0.;
+  for (int64_t q = 0; q < ncell; ++q) {
+    if (w[OC_IDN * ncell + q] < c->density_floor) w[OC_IDN * ncell + q] = c->density_floor;
+    if (w[OC_IPR * ncell + q] < c->pressure_floor) w[OC_IPR * ncell + q] = c->pressure_floor;
+  }
+}
+
+/* IdealGasImpl::_cons2prim (src/eos/ideal_gas.cpp:90-109) with the kernel
+   ideal_gas_cons2prim (src/eos/ideal_gas_impl.h:15-35).  The contravariant
+   raise with cos(theta) = 0 (Cartesian, src/coord/cartesian.cpp:12-14) is an
+   exact identity and is omitted. */
+void oc_cons2prim(const oc_config* c, double* u, double* w, int64_t ncell) {
+  conserved_limiter(c, u, ncell);
+  double gammad = c->gammad;
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+  for (int64_t q = 0; q < ncell; ++q) {
+    const double* U = u + q;
+    double* W = w + q;
+    long s = ncell;
+    W[OC_IDN * s] = U[OC_IDN * s];
+    W[OC_IVX * s] = U[OC_IVX * s] / W[OC_IDN * s];
+    W[OC_IVY * s] = U[OC_IVY * s] / W[OC_IDN * s];
+    W[OC_IVZ * s] = U[OC_IVZ * s] / W[OC_IDN * s];
+    double ke = 0.5 * (W[OC_IVX * s] * U[OC_IVX * s] + W[OC_IVY * s] * U[OC_IVY * s] +
+                       W[OC_IVZ * s] * U[OC_IVZ * s]);
+    double ie = U[OC_IPR * s] - ke;
+    W[OC_IPR * s] = (gammad - 1.) * ie;
+  }
+  primitive_limiter(c, w, ncell);
+}
+
+/* IdealGasImpl::_prim2cons (src/eos/ideal_gas.cpp:67-88) */
+void oc_prim2cons(const oc_config* c, double* w, double* u, int64_t ncell) {
+  primitive_limiter(c, w, ncell);
+  double gammad = c->gammad;
+  for (int64_t q = 0; q < ncell; ++q) {
+    const double* W = w + q;
+    double* U = u + q;
+    long s = ncell;
+    U[OC_IDN * s] = W[OC_IDN * s];
+    U[OC_IVX * s] = W[OC_IVX * s] * W[OC_IDN * s];
+    U[OC_IVY * s] = W[OC_IVY * s] * W[OC_IDN * s];
+    U[OC_IVZ * s] = W[OC_IVZ * s] * W[OC_IDN * s];
+    double ke = 0.5 * ((W[OC_IVX * s] * U[OC_IVX * s] + W[OC_IVY * s] * U[OC_IVY * s]) +
+                       W[OC_IVZ * s] * U[OC_IVZ * s]);
+    U[OC_IPR * s] = W[OC_IPR * s] / (gammad - 1);
+    U[OC_IPR * s] += ke;
+  }
+  conserved_limiter(c, u, ncell);
+}
+
+/* ------------------------------------------------------------------------ */
+/* divergence: CoordinateImpl::divergence (src/coord/coordinate.cpp:352-396)  */
+/* ------------------------------------------------------------------------ */
+
+void oc_divergence(const double* f1, const double* f2, const double* f3,
+                   double* div, int nvar, int n3, int n2, int n1, int nghost,
+                   const double* dx1f, const double* dx2f, const double* dx3f) {
+  long ncell = (long)n3 * n2 * n1;
+  int si = n1 > 1 ? nghost : 0, ei = n1 > 1 ? n1 - nghost : 1;
+  int sj = n2 > 1 ? nghost : 0, ej = n2 > 1 ? n2 - nghost : 1;
+  int sk = n3 > 1 ? nghost : 0, ek = n3 > 1 ? n3 - nghost : 1;
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+  for (long q = 0; q < ncell; ++q) {
+    int i = q % n1;
+    int j = (q / n1) % n2;
+    int k = q / ((long)n1 * n2);
+    double vol = dx3f[k] * (dx2f[j] * dx1f[i]); /* cell_volume (:328-331) */
+    for (int v = 0; v < nvar; ++v) {
+      long o = v * ncell + q;
+      double d = 0.;
+      if (f1 && i >= si && i < ei) {
+        double a = dx3f[k] * dx2f[j]; /* face_area1 (:316-318) */
+        d += a * f1[o + 1] - a * f1[o];
+      }
+      if (f2 && j >= sj && j < ej) {
+        double a = dx3f[k] * dx1f[i]; /* face_area2 (:320-322) */
+        d += a * f2[o + n1] - a * f2[o];
+      }
+      if (f3 && k >= sk && k < ek) {
+        double a = dx2f[j] * dx1f[i]; /* face_area3 (:324-326) */
+        d += a * f3[o + (long)n1 * n2] - a * f3[o];
+      }
+      div[o] = d / vol;
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* gravity boundary revisions (src/hydro/hydro.cpp:217-269)                  */
+/* ------------------------------------------------------------------------ */
+
+/* _revise_x1inner_ghost / _revise_x1outer_ghost: isentropic hydrostatic
+   refill of the x1 ghost cells; dx1v has length n1 */
+void oc_revise_x1_ghost(double* w, int nvar, int n3, int n2, int n1,
+                        int nghost, double grav1, double gammad,
+                        const double* dx1v) {
+  (void)nvar;
+  long ncell = (long)n3 * n2 * n1;
+  int is = nghost, ie = n1 - nghost - 1;
+  double grav = -grav1;
+  double gamma = gammad * 1.0;
+  double gm = gamma - 1.;
+  double a = gm / gamma;
+  double inv_g = 1. / gamma, inv_a = 1. / a;
+  for (long p = 0; p < (long)n3 * n2; ++p) {
+    double* rho = w + OC_IDN * ncell + p * n1;
+    double* pre = w + OC_IPR * ncell + p * n1;
+    double K = pre[is] / pow(rho[is], gamma);
+    for (int n = 0; n < nghost; ++n) {
+      double dz = dx1v[is - n - 1];
+      double h = pow(pre[is - n], a) + a * grav * dz / pow(K, inv_g);
+      pre[is - n - 1] = pow(h, inv_a);
+      rho[is - n - 1] = pow(pre[is - n - 1] / K, inv_g);
+    }
+    K = pre[ie] / pow(rho[ie], gamma);
+    for (int n = 0; n < nghost; ++n) {
+      double dz = dx1v[ie + n];
+      double h = pow(pre[ie + n], a) - a * grav * dz / pow(K, inv_g);
+      pre[ie + n + 1] = pow(h, inv_a);
+      rho[ie + n + 1] = pow(pre[ie + n + 1] / K, inv_g);
+    }
+  }
+}
+
+/* _revise_x1inner_lr / _revise_x1outer_lr: copy rho, p of the inner state
+   across the physical x1 boundary faces (is and ie+1) */
+void oc_revise_x1_lr(double* wlr, int nvar, int n3, int n2, int n1,
+                     int nghost) {
+  long ncell = (long)n3 * n2 * n1;
+  double* wl = wlr;
+  double* wr = wlr + nvar * ncell;
+  int is = nghost, ie = n1 - nghost - 1;
+  for (long p = 0; p < (long)n3 * n2; ++p) {
+    long b = p * n1;
+    wl[OC_IPR * ncell + b + is] = wr[OC_IPR * ncell + b + is];
+    wl[OC_IDN * ncell + b + is] = wr[OC_IDN * ncell + b + is];
+    wr[OC_IPR * ncell + b + ie + 1] = wl[OC_IPR * ncell + b + ie + 1];
+    wr[OC_IDN * ncell + b + ie + 1] = wl[OC_IDN * ncell + b + ie + 1];
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* vertically implicit correction                                            */
+/* ------------------------------------------------------------------------ */
+
+/* 3x3 inverse by cofactors (adjugate / determinant), the closed form used for
+   the partial scheme's small blocks (src/implicit/forward_sweep_impl.h:66-68,
+   101-103) */
+void oc_inverse3(const double* m, double* inv) {
+#define M(r, c) m[(r) * 3 + (c)]
+#define COF(i, j)                                                        \
+  (M(((i) + 1) % 3, ((j) + 1) % 3) * M(((i) + 2) % 3, ((j) + 2) % 3) - \
+   M(((i) + 1) % 3, ((j) + 2) % 3) * M(((i) + 2) % 3, ((j) + 1) % 3))
+  double c00 = COF(0, 0), c10 = COF(1, 0), c20 = COF(2, 0);
+  double det = c00 * M(0, 0) + c10 * M(1, 0) + c20 * M(2, 0);
+  double invdet = 1.0 / det;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) inv[i * 3 + j] = COF(j, i) * invdet;
+#undef COF
+#undef M
+}
+
+static void matmul(const double* a, const double* b, double* c, int N) {
+  for (int i = 0; i < N; ++i)
+    for (int j = 0; j < N; ++j) {
+      double s = 0.;
+      for (int k = 0; k < N; ++k) s += a[i * N + k] * b[k * N + j];
+      c[i * N + j] = s;
+    }
+}
+
+static void matvec(const double* a, const double* x, double* y, int N) {
+  for (int i = 0; i < N; ++i) {
+    double s = 0.;
+    for (int k = 0; k < N; ++k) s += a[i * N + k] * x[k];
+    y[i] = s;
+  }
+}
+
+/* flux_decomposition_impl.h:21-37 (CopyPrimitives, ny = 0); the column view
+   w points at the first interior cell, stride1 = variable stride */
+static void copy_prim(double* wl, double* wr, const double* w, int i,
+                      long stride1) {
+  wl[OC_IDN] = w[i - 1];
+  wr[OC_IDN] = w[i];
+  for (int n = OC_IVX; n <= OC_IPR; ++n) {
+    wl[n] = w[n * stride1 + (i - 1)];
+    wr[n] = w[n * stride1 + i];
+  }
+}
+
+/* flux_decomposition_impl.h:39-70 */
+static void roe_average(double* prim, double gm1, const double* wl,
+                        const double* wr) {
+  double sqrtdl = sqrt(wl[OC_IDN]);
+  double sqrtdr = sqrt(wr[OC_IDN]);
+  double isdlpdr = 1.0 / (sqrtdl + sqrtdr);
+  prim[OC_IDN] = sqrtdl * sqrtdr;
+  prim[OC_IVX] = (sqrtdl * wl[OC_IVX] + sqrtdr * wr[OC_IVX]) * isdlpdr;
+  prim[OC_IVY] = (sqrtdl * wl[OC_IVY] + sqrtdr * wr[OC_IVY]) * isdlpdr;
+  prim[OC_IVZ] = (sqrtdl * wl[OC_IVZ] + sqrtdr * wr[OC_IVZ]) * isdlpdr;
+  double el = wl[OC_IPR] / gm1 +
+              0.5 * wl[OC_IDN] * (SQR(wl[OC_IVX]) + SQR(wl[OC_IVY]) + SQR(wl[OC_IVZ]));
+  double er = wr[OC_IPR] / gm1 +
+              0.5 * wr[OC_IDN] * (SQR(wr[OC_IVX]) + SQR(wr[OC_IVY]) + SQR(wr[OC_IVZ]));
+  double hbar = ((el + wl[OC_IPR]) / sqrtdl + (er + wr[OC_IPR]) / sqrtdr) * isdlpdr;
+  prim[OC_IPR] = (hbar - 0.5 * (SQR(prim[OC_IVX]) + SQR(prim[OC_IVY]) + SQR(prim[OC_IVZ]))) *
+                 gm1 / (gm1 + 1.) * prim[OC_IDN];
+}
+
+/* flux_decomposition_impl.h:16-19 */
+static double sound_speed(const double* prim, double gm1) {
+  return sqrt(prim[OC_IPR] * (gm1 + 1.) / prim[OC_IDN]);
+}
+
+/* |A| = R |Lambda| R^-1 at a Roe-averaged interface state
+   (flux_decomposition_impl.h:72-105, dir = 0) as a 5x5 row-major matrix */
+static void abs_jacobian(const double* prim, double cs, double gm1, double* A) {
+  double r = prim[OC_IDN];
+  double u = prim[OC_IVX];
+  double v = prim[OC_IVY];
+  double w = prim[OC_IVZ];
+  double p = prim[OC_IPR];
+  double ke = 0.5 * (u * u + v * v + w * w);
+  double hp = (gm1 + 1.) / gm1 * p / r;
+  double h = hp + ke;
+  double R[25] = {1., 1., 1., 0., 0.,                     /**/
+                  u - cs, u, u + cs, 0., 0.,              /**/
+                  v, v, v, 1., 0.,                        /**/
+                  w, w, w, 0., 1.,                        /**/
+                  h - u * cs, ke, h + u * cs, v, w};
+  double Ri[25] = {(cs * ke + u * hp) / (2. * cs * hp),
+                   (-hp - cs * u) / (2. * cs * hp),
+                   -v / (2. * hp),
+                   -w / (2. * hp),
+                   1. / (2. * hp),
+                   (hp - ke) / hp,
+                   u / hp,
+                   v / hp,
+                   w / hp,
+                   -1. / hp,
+                   (cs * ke - u * hp) / (2. * cs * hp),
+                   (hp - cs * u) / (2. * cs * hp),
+                   -v / (2. * hp),
+                   -w / (2. * hp),
+                   1. / (2. * hp),
+                   -v, 0., 1., 0., 0.,
+                   -w, 0., 0., 1., 0.};
+  double lam[5] = {fabs(u - cs), fabs(u), fabs(u + cs), fabs(u), fabs(u)};
+  for (int i = 0; i < 5; ++i)
+    for (int j = 0; j < 5; ++j) {
+      double s = 0.;
+      for (int k = 0; k < 5; ++k) s += (R[i * 5 + k] * lam[k]) * Ri[k * 5 + j];
+      A[i * 5 + j] = s;
+    }
+}
+
+/* flux_decomposition_impl.h:107-128 (dir = 0) as a 5x5 row-major matrix */
+static void flux_jacobian(double gm1, const double* w, double* J) {
+  double v1 = w[OC_IVX], v2 = w[OC_IVY], v3 = w[OC_IVZ];
+  double rho = w[OC_IDN], pres = w[OC_IPR];
+  double s2 = v1 * v1 + v2 * v2 + v3 * v3;
+  double c1 = ((gm1 - 1) * s2 / 2 - (gm1 + 1) / gm1 * pres / rho) * v1;
+  double c2 = (gm1 + 1) / gm1 * pres / rho + s2 / 2 - gm1 * v1 * v1;
+  double m[25] = {0, 1., 0., 0., 0.,                                        /**/
+                  gm1 * s2 / 2 - v1 * v1, (2. - gm1) * v1, -gm1 * v2, -gm1 * v3, gm1, /**/
+                  -v1 * v2, v2, v1, 0., 0.,                                 /**/
+                  -v1 * v3, v3, 0., v1, 0.,                                 /**/
+                  c1, c2, -gm1 * v2 * v1, -gm1 * v3 * v1, (gm1 + 1) * v1};
+  memcpy(J, m, sizeof(m));
+}
+
+static const int SUB3[3] = {OC_IDN, OC_IVX, OC_IPR};
+
+static void sub3(const double* A5, double* A3) {
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) A3[r * 3 + c] = A5[SUB3[r] * 5 + SUB3[c]];
+}
+
+/* Block coefficients a,b,c of one column (N = 3: vic_assemble_partial_impl,
+   src/implicit/vic_assemble_partial_impl.h:28-111; N = 5:
+   vic_assemble_full_impl, src/implicit/vic_assemble_full_impl.h:15-106).
+   w: interior view of the column (w[v*stride1 + i], i = -nghost..nl+nghost-1
+   valid), area: face areas (index i and i+1), vol: cell volumes. */
+static void vic_assemble(int N, const double* w, long stride1, int nl,
+                         double gammad, const double* area, const double* vol,
+                         double dt, double grav, double* a, double* b,
+                         double* c) {
+  double wl[5], wr[5], prim[5];
+  double J5p[25], J5c[25], J5n[25], Am5[25], Ap5[25];
+  double gamma = gammad;
+  for (int i = 0; i < nl; ++i) {
+    double gm1;
+    /* interface i-1/2: cells i-1, i */
+    copy_prim(wl, wr, w, i, stride1);
+    gm1 = gamma - 1.;
+    flux_jacobian(gm1, wl, J5p);
+    flux_jacobian(gm1, wr, J5c);
+    gm1 = 0.5 * (gamma + gamma) - 1.;
+    roe_average(prim, gm1, wl, wr);
+    double cs = sound_speed(prim, gm1);
+    abs_jacobian(prim, cs, gm1, Am5);
+    /* interface i+1/2: cells i, i+1 */
+    copy_prim(wl, wr, w, i + 1, stride1);
+    gm1 = gamma - 1.;
+    flux_jacobian(gm1, wr, J5n);
+    gm1 = 0.5 * (gamma + gamma) - 1.;
+    roe_average(prim, gm1, wl, wr);
+    cs = sound_speed(prim, gm1);
+    abs_jacobian(prim, cs, gm1, Ap5);
+
+    double Am[25], Ap[25], Jp[25], Jc[25], Jn[25];
+    if (N == 3) {
+      sub3(Am5, Am);
+      sub3(Ap5, Ap);
+      sub3(J5p, Jp);
+      sub3(J5c, Jc);
+      sub3(J5n, Jn);
+    } else {
+      memcpy(Am, Am5, sizeof(Am5));
+      memcpy(Ap, Ap5, sizeof(Ap5));
+      memcpy(Jp, J5p, sizeof(J5p));
+      memcpy(Jc, J5c, sizeof(J5c));
+      memcpy(Jn, J5n, sizeof(J5n));
+    }
+    double area_i = area[i], area_ip1 = area[i + 1];
+    double half_inv_vol = 0.5 / vol[i];
+    double* ai = a + i * N * N;
+    double* bi = b + i * N * N;
+    double* ci = c + i * N * N;
+    for (int r = 0; r < N; ++r)
+      for (int q = 0; q < N; ++q) {
+        int o = r * N + q;
+        /* Phi: (IVX, IDN) = (IPR, IVX) = grav */
+        double phi = 0.;
+        if (N == 3) {
+          if ((r == 1 && q == 0) || (r == 2 && q == 1)) phi = grav;
+        } else {
+          if ((r == OC_IVX && q == OC_IDN) || (r == OC_IPR && q == OC_IVX)) phi = grav;
+        }
+        ai[o] = (Am[o] * area_i + Ap[o] * area_ip1 + (area_ip1 - area_i) * Jc[o]) *
+                    half_inv_vol -
+                phi;
+        if (r == q) ai[o] += 1. / dt;
+        bi[o] = -(Am[o] + Jp[o]) * area_i * half_inv_vol;
+        ci[o] = -(Ap[o] - Jn[o]) * area_ip1 * half_inv_vol;
+      }
+  }
+  /* reflecting column ends: Bnd = diag(1, -1 (normal velocity), 1 ...) */
+  for (int r = 0; r < N; ++r)
+    for (int q = 0; q < N; ++q) {
+      double bnd = (q == 1) ? -1. : 1.;
+      a[r * N + q] += b[r * N + q] * bnd;
+      a[(nl - 1) * N * N + r * N + q] += c[(nl - 1) * N * N + r * N + q] * bnd;
+    }
+}
+
+/* ludcmp with partial pivoting (src/math/ludcmp.h:25-76) on a row-major NxN */
+static int ludcmp(double* a, int N, int* indx) {
+  int i, imax = 0, j, k, d = 1;
+  double big, dum, sum, temp, vv[8];
+  for (i = 0; i < N; i++) {
+    big = 0.0;
+    for (j = 0; j < N; j++)
+      if ((temp = fabs(a[i * N + j])) > big) big = temp;
+    if (big == 0.0) return 1;
+    vv[i] = 1.0 / big;
+  }
+  for (j = 0; j < N; j++) {
+    for (i = 0; i < j; i++) {
+      sum = a[i * N + j];
+      for (k = 0; k < i; k++) sum -= a[i * N + k] * a[k * N + j];
+      a[i * N + j] = sum;
+    }
+    big = 0.0;
+    for (i = j; i < N; i++) {
+      sum = a[i * N + j];
+      for (k = 0; k < j; k++) sum -= a[i * N + k] * a[k * N + j];
+      a[i * N + j] = sum;
+      if ((dum = vv[i] * fabs(sum)) >= big) {
+        big = dum;
+        imax = i;
+      }
+    }
+    if (j != imax) {
+      for (k = 0; k < N; k++) {
+        dum = a[imax * N + k];
+        a[imax * N + k] = a[j * N + k];
+        a[j * N + k] = dum;
+      }
+      d = -d;
+      vv[imax] = vv[j];
+    }
+    indx[j] = imax;
+    if (j != N - 1) {
+      dum = (1.0 / a[j * N + j]);
+      for (i = j + 1; i < N; i++) a[i * N + j] *= dum;
+    }
+  }
+  return d;
+}
+
+/* lubksb, matrix right-hand side (src/math/lubksb.h:49-71); b is N x M
+   row-major */
+static void lubksb_mat(const double* a, int N, const int* indx, double* b,
+                       int M) {
+  for (int i = 0; i < N; ++i) {
+    int ip = indx[i];
+    if (ip != i)
+      for (int m = 0; m < M; ++m) {
+        double t = b[ip * M + m];
+        b[ip * M + m] = b[i * M + m];
+        b[i * M + m] = t;
+      }
+    for (int j = 0; j < i; ++j)
+      for (int m = 0; m < M; ++m) b[i * M + m] -= a[i * N + j] * b[j * M + m];
+  }
+  for (int i = N - 1; i >= 0; --i) {
+    for (int j = i + 1; j < N; ++j)
+      for (int m = 0; m < M; ++m) b[i * M + m] -= a[i * N + j] * b[j * M + m];
+    for (int m = 0; m < M; ++m) b[i * M + m] /= a[i * N + i];
+  }
+}
+
+/* ForwardSweep (src/implicit/forward_sweep_impl.h:20-109); du is the column
+   view (du[v*stride1 + i]) */
+static void forward_sweep(int N, double* a, const double* b, const double* c,
+                          double* delta, const double* du, long stride1,
+                          int nl, double dt) {
+  double rhs[5], tmp[25], tmp2[25], v5[5];
+  for (int i = 0; i < nl; ++i) {
+    rhs[0] = du[OC_IDN * stride1 + i];
+    rhs[0] /= dt;
+    rhs[1] = du[OC_IVX * stride1 + i] / dt;
+    if (N == 3) {
+      rhs[2] = du[OC_IPR * stride1 + i] / dt;
+    } else {
+      rhs[2] = du[OC_IVY * stride1 + i] / dt;
+      rhs[3] = du[OC_IVZ * stride1 + i] / dt;
+      rhs[4] = du[OC_IPR * stride1 + i] / dt;
+    }
+    double* ai = a + i * N * N;
+    const double* bi = b + i * N * N;
+    const double* ci = c + i * N * N;
+    double* di = delta + i * N;
+    if (i > 0) {
+      /* a[i] -= b[i] a[i-1];  rhs -= b[i] delta[i-1] */
+      matmul(bi, a + (i - 1) * N * N, tmp, N);
+      for (int q = 0; q < N * N; ++q) ai[q] -= tmp[q];
+      matvec(bi, delta + (i - 1) * N, v5, N);
+      for (int q = 0; q < N; ++q) rhs[q] = rhs[q] - v5[q];
+    }
+    if (N == 3) {
+      oc_inverse3(ai, tmp);
+      matvec(tmp, rhs, di, 3);
+      matmul(tmp, ci, tmp2, 3);
+      memcpy(ai, tmp2, sizeof(double) * 9);
+    } else {
+      int indx[5];
+      memcpy(tmp, ai, sizeof(double) * 25);
+      ludcmp(tmp, 5, indx);
+      double solved[30];
+      for (int r = 0; r < 5; ++r) {
+        for (int q = 0; q < 5; ++q) solved[r * 6 + q] = ci[r * 5 + q];
+        solved[r * 6 + 5] = rhs[r];
+      }
+      lubksb_mat(tmp, 5, indx, solved, 6);
+      for (int r = 0; r < 5; ++r) {
+        for (int q = 0; q < 5; ++q) ai[r * 5 + q] = solved[r * 6 + q];
+        di[r] = solved[r * 6 + 5];
+      }
+    }
+  }
+}
+
+/* vic_backward_reduce (src/implicit/vic_redistribute_impl.h:26-62) */
+void oc_vic_backward_reduce(int N, double* du, const double* w, double* a,
+                            double* delta, const double* vol, double* scratch,
+                            int il, int iu, int dir, int ny, int stride1,
+                            int stride2) {
+  (void)dir;
+  double v5[5];
+  for (int i = iu - 1; i >= il; --i) {
+    matvec(a + i * N * N, delta + (i + 1) * N, v5, N);
+    for (int q = 0; q < N; ++q) delta[i * N + q] -= v5[q];
+  }
+#define DU(n, i) du[(n) * stride1 + (i) * stride2]
+#define W(n, i) w[(n) * stride1 + (i) * stride2]
+#define VOL(n) vol[(n) * stride2]
+  double sum_a2 = 0, b_dry = 0;
+  for (int i = il; i <= iu; ++i) {
+    double explicit_total = DU(OC_IDN, i);
+    double dryfrac = 1;
+    for (int n = 0; n < ny; ++n) {
+      explicit_total += DU(OC_ICY + n, i);
+      dryfrac -= W(OC_ICY + n, i);
+    }
+    double a_i = W(OC_IDN, i) * VOL(i);
+    sum_a2 += a_i * a_i;
+    b_dry += (explicit_total - delta[i * N + 0]) * dryfrac * VOL(i);
+  }
+  scratch[0] = sum_a2;
+  scratch[1] = b_dry;
+  for (int n = 0; n < ny; ++n) {
+    double b_species = 0;
+    for (int i = il; i <= iu; ++i) {
+      double explicit_total = DU(OC_IDN, i);
+      for (int m = 0; m < ny; ++m) explicit_total += DU(OC_ICY + m, i);
+      b_species += (explicit_total - delta[i * N + 0]) * W(OC_ICY + n, i) * VOL(i);
+    }
+    scratch[2 + n] = b_species;
+  }
+}
+
+/* vic_redistribute_cell (src/implicit/vic_redistribute_impl.h:66-106) */
+void oc_vic_redistribute_cell(int N, double* du, const double* w,
+                              double* mass_fix, const double* delta,
+                              const double* vol, const double* scratch, int i,
+                              int dir, int ny, int stride1, int stride2) {
+  (void)dir;
+  double sum_a2 = scratch[0];
+  double a_i = W(OC_IDN, i) * VOL(i);
+  double dryfrac = 1;
+  for (int n = 0; n < ny; ++n) dryfrac -= W(OC_ICY + n, i);
+  double explicit_total = DU(OC_IDN, i);
+  for (int n = 0; n < ny; ++n) explicit_total += DU(OC_ICY + n, i);
+  double diffusion_fix = delta[i * N + 0] - explicit_total;
+  if (mass_fix) mass_fix[OC_IDN * stride1 + i * stride2] = diffusion_fix;
+  double dry_structural = W(OC_IDN, i) * a_i * scratch[1] / sum_a2;
+  DU(OC_IDN, i) = DU(OC_IDN, i) + diffusion_fix * dryfrac + dry_structural;
+  if (N == 3) {
+    DU(OC_IVX, i) = delta[i * N + 1];
+    DU(OC_IPR, i) = delta[i * N + 2];
+  } else {
+    DU(OC_IVX, i) = delta[i * N + 1];
+    DU(OC_IVY, i) = delta[i * N + 2];
+    DU(OC_IVZ, i) = delta[i * N + 3];
+    DU(OC_IPR, i) = delta[i * N + 4];
+  }
+  for (int n = 0; n < ny; ++n)
+    DU(OC_ICY + n, i) = DU(OC_ICY + n, i) + diffusion_fix * W(OC_ICY + n, i);
+}
+#undef DU
+#undef W
+#undef VOL
+
+static void column_geometry(const oc_config* c, int k, int j, double* area,
+                            double* vol, double* dx1f) {
+  int n1 = oc_nc1(c), n2 = oc_nc2(c), n3 = oc_nc3(c);
+  double* dx2f = malloc(sizeof(double) * n2);
+  double* dx3f = malloc(sizeof(double) * n3);
+  oc_cartesian_widths(c, dx1f, dx2f, dx3f);
+  int il = c->nghost;
+  for (int i = 0; i <= c->nx1; ++i) area[i] = dx3f[k] * dx2f[j];
+  for (int i = 0; i < c->nx1; ++i) vol[i] = dx3f[k] * (dx2f[j] * dx1f[il + i]);
+  free(dx2f);
+  free(dx3f);
+}
+
+/* expose the assembled block matrices of one interior column (k, j are full
+   cell indices) for brute-force checks */
+void oc_vic_column_matrices(const oc_config* c, double dt, const double* w,
+                            int k, int j, double* a, double* b, double* cc) {
+  int n1 = oc_nc1(c), n2 = oc_nc2(c), n3 = oc_nc3(c);
+  long ncell = (long)n3 * n2 * n1;
+  int N = ((c->implicit_scheme >> 3) & 1) ? 5 : 3;
+  int nl = c->nx1;
+  double* area = malloc(sizeof(double) * (nl + 1));
+  double* vol = malloc(sizeof(double) * nl);
+  double* dx1f = malloc(sizeof(double) * n1);
+  column_geometry(c, k, j, area, vol, dx1f);
+  const double* wc = w + ((long)k * n2 + j) * n1 + c->nghost;
+  vic_assemble(N, wc, ncell, nl, c->gammad, area, vol, dt, c->grav1, a, b, cc);
+  free(area);
+  free(vol);
+  free(dx1f);
+}
+
+/* HydroImpl::_apply_implicit_correction + ImplicitHydroImpl::forward
+   (src/hydro/hydro.cpp:186-215, src/implicit/implicit_hydro.cpp:99-189) for a
+   Cartesian block (the local-frame projections are exact identities). */
+void oc_implicit_correction(const oc_config* c, double dt, double* du,
+                            const double* w) {
+  if (c->implicit_scheme == 0) return;
+  int n1 = oc_nc1(c), n2 = oc_nc2(c), n3 = oc_nc3(c);
+  long ncell = (long)n3 * n2 * n1;
+  int N = ((c->implicit_scheme >> 3) & 1) ? 5 : 3;
+  int nl = c->nx1;
+  int jl = n2 > 1 ? c->nghost : 0, ju = n2 > 1 ? n2 - c->nghost : 1;
+  int kl = n3 > 1 ? c->nghost : 0, ku = n3 > 1 ? n3 - c->nghost : 1;
+  double* dx1f = malloc(sizeof(double) * n1);
+  double* dx2f = malloc(sizeof(double) * n2);
+  double* dx3f = malloc(sizeof(double) * n3);
+  oc_cartesian_widths(c, dx1f, dx2f, dx3f);
+  long ncol = (long)(ku - kl) * (ju - jl);
+#pragma omp parallel num_threads(g_threads)
+  {
+    double* a = malloc(sizeof(double) * nl * N * N);
+    double* b = malloc(sizeof(double) * nl * N * N);
+    double* cc = malloc(sizeof(double) * nl * N * N);
+    double* delta = malloc(sizeof(double) * nl * N);
+    double* area = malloc(sizeof(double) * (nl + 1));
+    double* vol = malloc(sizeof(double) * nl);
+#pragma omp for schedule(static)
+    for (long col = 0; col < ncol; ++col) {
+      int k = kl + col / (ju - jl);
+      int j = jl + col % (ju - jl);
+      for (int i = 0; i <= nl; ++i) area[i] = dx3f[k] * dx2f[j];
+      for (int i = 0; i < nl; ++i) vol[i] = dx3f[k] * (dx2f[j] * dx1f[c->nghost + i]);
+      long off = ((long)k * n2 + j) * n1 + c->nghost;
+      const double* wc = w + off;
+      double* duc = du + off;
+      vic_assemble(N, wc, ncell, nl, c->gammad, area, vol, dt, c->grav1, a, b, cc);
+      forward_sweep(N, a, b, cc, delta, duc, ncell, nl, dt);
+      double scratch[2];
+      oc_vic_backward_reduce(N, duc, wc, a, delta, vol, scratch, 0, nl - 1, 0,
+                             0, (int)ncell, 1);
+      for (int i = 0; i < nl; ++i)
+        oc_vic_redistribute_cell(N, duc, wc, NULL, delta, vol, scratch, i, 0,
+                                 0, (int)ncell, 1);
+    }
+    free(a);
+    free(b);
+    free(cc);
+    free(delta);
+    free(area);
+    free(vol);
+  }
+  free(dx1f);
+  free(dx2f);
+  free(dx3f);
+}
+
+/* ------------------------------------------------------------------------ */
+/* hydro RHS: HydroImpl::forward (src/hydro/hydro_forward.cpp:14-217)         */
+/* ------------------------------------------------------------------------ */
+
+void oc_hydro_forward(const oc_config* c, double dt, double* u, double* w,
+                      double* du, double* f1out, double* f2out,
+                      double* f3out) {
+  int nvar = 5;
+  int n1 = oc_nc1(c), n2 = oc_nc2(c), n3 = oc_nc3(c);
+  long ncell = (long)n3 * n2 * n1;
+  int ng = c->nghost;
+  double* dx1f = malloc(sizeof(double) * n1);
+  double* dx2f = malloc(sizeof(double) * n2);
+  double* dx3f = malloc(sizeof(double) * n3);
+  oc_cartesian_widths(c, dx1f, dx2f, dx3f);
+  double* wlr = malloc(sizeof(double) * 2 * nvar * ncell);
+  double* F1 = n1 > 1 ? calloc(nvar * ncell, sizeof(double)) : NULL;
+  double* F2 = n2 > 1 ? calloc(nvar * ncell, sizeof(double)) : NULL;
+  double* F3 = n3 > 1 ? calloc(nvar * ncell, sizeof(double)) : NULL;
+  double* div = malloc(sizeof(double) * nvar * ncell);
+  double* rho_grav = calloc(ncell, sizeof(double));
+
+  /* (1) primitives */
+  oc_cons2prim(c, u, w, ncell);
+
+  /* (2) x1 */
+  if (n1 > 1) {
+    int grav1 = c->grav1 != 0.;
+    if (grav1) oc_revise_x1_ghost(w, nvar, n3, n2, n1, ng, c->grav1, c->gammad, dx1f);
+    oc_reconstruct(w, wlr, nvar, n3, n2, n1, 3, c->recon_type[0], c->recon_scale[0],
+                   c->recon_shock[0], 1, c->limiter, c->density_floor,
+                   c->pressure_floor);
+    if (grav1) oc_revise_x1_lr(wlr, nvar, n3, n2, n1, ng);
+    if (grav1 && c->non_hydrostatic < 1.) {
+      /* hydro_forward.cpp:67-76 */
+      const double* pl = wlr + OC_IPR * ncell;
+      const double* pr = wlr + nvar * ncell + OC_IPR * ncell;
+      for (long p = 0; p < (long)n3 * n2; ++p)
+        for (int i = ng; i < n1 - ng; ++i)
+          rho_grav[p * n1 + i] = (pl[p * n1 + i + 1] - pr[p * n1 + i]) / dx1f[i];
+    }
+    if (!c->disable_flux[0])
+      oc_riemann(c->riemann[0], wlr, wlr + nvar * ncell, F1, NULL, nvar, n3, n2, n1, 3,
+                 c->gammad);
+  }
+  /* (3, 4) x2, x3 */
+  if (n2 > 1) {
+    oc_reconstruct(w, wlr, nvar, n3, n2, n1, 2, c->recon_type[1], c->recon_scale[1],
+                   c->recon_shock[1], 1, c->limiter, c->density_floor,
+                   c->pressure_floor);
+    if (!c->disable_flux[1])
+      oc_riemann(c->riemann[1], wlr, wlr + nvar * ncell, F2, NULL, nvar, n3, n2, n1, 2,
+                 c->gammad);
+  }
+  if (n3 > 1) {
+    oc_reconstruct(w, wlr, nvar, n3, n2, n1, 1, c->recon_type[1], c->recon_scale[1],
+                   c->recon_shock[1], 1, c->limiter, c->density_floor,
+                   c->pressure_floor);
+    if (!c->disable_flux[2])
+      oc_riemann(c->riemann[2], wlr, wlr + nvar * ncell, F3, NULL, nvar, n3, n2, n1, 1,
+                 c->gammad);
+  }
+
+  /* (5) divergence */
+  oc_divergence(F1, F2, F3, div, nvar, n3, n2, n1, ng, dx1f, dx2f, dx3f);
+
+  /* (6) du = -dt D on the interior, then forcings on every cell */
+  memset(du, 0, sizeof(double) * nvar * ncell);
+  int il = n1 > 1 ? ng : 0, iu = n1 > 1 ? n1 - ng : 1;
+  int jl = n2 > 1 ? ng : 0, ju = n2 > 1 ? n2 - ng : 1;
+  int kl = n3 > 1 ? ng : 0, ku = n3 > 1 ? n3 - ng : 1;
+  for (int v = 0; v < nvar; ++v)
+    for (int k = kl; k < ku; ++k)
+      for (int j = jl; j < ju; ++j)
+        for (int i = il; i < iu; ++i) {
+          long o = v * ncell + ((long)k * n2 + j) * n1 + i;
+          du[o] = -dt * div[o];
+        }
+  /* ConstGravityImpl::forward (src/forcing/const_gravity.cpp:26-45) */
+  double nh = c->non_hydrostatic;
+  for (long q = 0; q < ncell; ++q) {
+    double rho = w[OC_IDN * ncell + q];
+    if (c->grav1 != 0.) {
+      du[OC_IVX * ncell + q] += dt * rho * c->grav1 * nh;
+      du[OC_IPR * ncell + q] += dt * rho * w[OC_IVX * ncell + q] * c->grav1 * nh;
+    }
+    if (c->grav2 != 0.) {
+      du[OC_IVY * ncell + q] += dt * rho * c->grav2;
+      du[OC_IPR * ncell + q] += dt * rho * w[OC_IVY * ncell + q] * c->grav2;
+    }
+    if (c->grav3 != 0.) {
+      du[OC_IVZ * ncell + q] += dt * rho * c->grav3;
+      du[OC_IPR * ncell + q] += dt * rho * w[OC_IVZ * ncell + q] * c->grav3;
+    }
+  }
+  /* hydrostatic correction (hydro_forward.cpp:191-195) */
+  if (c->grav1 != 0. && nh < 1.) {
+    for (long q = 0; q < ncell; ++q) {
+      du[OC_IVX * ncell + q] += dt * rho_grav[q] * (1. - nh);
+      du[OC_IPR * ncell + q] += dt * w[OC_IVX * ncell + q] * rho_grav[q] * (1. - nh);
+    }
+  }
+
+  /* (7) implicit correction */
+  if (c->implicit_scheme) oc_implicit_correction(c, dt, du, w);
+
+  if (f1out && F1) memcpy(f1out, F1, sizeof(double) * nvar * ncell);
+  if (f2out && F2) memcpy(f2out, F2, sizeof(double) * nvar * ncell);
+  if (f3out && F3) memcpy(f3out, F3, sizeof(double) * nvar * ncell);
+  free(dx1f);
+  free(dx2f);
+  free(dx3f);
+  free(wlr);
+  free(F1);
+  free(F2);
+  free(F3);
+  free(div);
+  free(rho_grav);
+}
+
+/* ------------------------------------------------------------------------ */
+/* boundary conditions (src/bc/bc_func.cpp:7-63), applied in the order x1-in, */
+/* x1-out, x2-in, x2-out, x3-in, x3-out (src/mesh/meshblock.cpp:717-722)     */
+/* ------------------------------------------------------------------------ */
+
+static void bc_axis(double* var, int nvar, int n3, int n2, int n1, int ng,
+                    int dim, int outer, int type, int var_type) {
+  long s;
+  int n;
+  axis_info(n3, n2, n1, dim, &s, &n);
+  if (n == 1 || type == OC_BC_NONE) return;
+  long ncell = (long)n3 * n2 * n1;
+  long np = pencil_count(n3, n2, n1, dim);
+  int vnorm = 4 - dim;
+  for (int v = 0; v < nvar; ++v)
+    for (long p = 0; p < np; ++p) {
+      double* x = var + v * ncell + pencil_base(n3, n2, n1, dim, p);
+      for (int g = 0; g < ng; ++g) {
+        int dst, src;
+        if (!outer) {
+          dst = g;
+          if (type == OC_BC_REFLECTING)
+            src = 2 * ng - 1 - g;
+          else if (type == OC_BC_OUTFLOW)
+            src = ng;
+          else
+            src = n - 2 * ng + g;
+        } else {
+          dst = n - ng + g;
+          if (type == OC_BC_REFLECTING)
+            src = n - ng - 1 - g;
+          else if (type == OC_BC_OUTFLOW)
+            src = n - ng - 1;
+          else
+            src = ng + g;
+        }
+        double val = x[src * s];
+        if (type == OC_BC_REFLECTING && v == vnorm &&
+            (var_type == OC_CONSERVED || var_type == OC_PRIMITIVE))
+          val = val * -1;
+        x[dst * s] = val;
+      }
+    }
+}
+
+void oc_apply_bc(const oc_config* c, double* var, int nvar, int var_type) {
+  int n1 = oc_nc1(c), n2 = oc_nc2(c), n3 = oc_nc3(c);
+  for (int i = 0; i < 6; ++i)
+    bc_axis(var, nvar, n3, n2, n1, c->nghost, 3 - i / 2, i % 2, c->bc[i], var_type);
+}
+
+/* ------------------------------------------------------------------------ */
+/* time stepping                                                             */
+/* ------------------------------------------------------------------------ */
+
+/* HydroImpl::max_time_step (src/hydro/hydro.cpp:118-180) times
+   cfl (src/mesh/meshblock.cpp:511-530, redo = 0) */
+double oc_max_time_step(const oc_config* c, const double* w, double cfl) {
+  int n1 = oc_nc1(c), n2 = oc_nc2(c), n3 = oc_nc3(c);
+  long ncell = (long)n3 * n2 * n1;
+  double* dx1f = malloc(sizeof(double) * n1);
+  double* dx2f = malloc(sizeof(double) * n2);
+  double* dx3f = malloc(sizeof(double) * n3);
+  oc_cartesian_widths(c, dx1f, dx2f, dx3f);
+  int sch = c->implicit_scheme;
+  int use1 = n1 > 1 && (!(sch & 1) || (n3 == 1 && n2 == 1));
+  int use2 = n2 > 1 && !((sch >> 1) & 1);
+  int use3 = n3 > 1 && !((sch >> 2) & 1);
+  double d1 = 1.e9, d2 = 1.e9, d3 = 1.e9;
+  int il = n1 > 1 ? c->nghost : 0, iu = n1 > 1 ? n1 - c->nghost : 1;
+  int jl = n2 > 1 ? c->nghost : 0, ju = n2 > 1 ? n2 - c->nghost : 1;
+  int kl = n3 > 1 ? c->nghost : 0, ku = n3 > 1 ? n3 - c->nghost : 1;
+  for (int k = kl; k < ku; ++k)
+    for (int j = jl; j < ju; ++j)
+      for (int i = il; i < iu; ++i) {
+        long q = ((long)k * n2 + j) * n1 + i;
+        double cs = sqrt(c->gammad * 1.0 * w[OC_IPR * ncell + q] / w[OC_IDN * ncell + q]);
+        double t;
+        if (use1 && (t = dx1f[i] / (fabs(w[OC_IVX * ncell + q]) + cs)) < d1) d1 = t;
+        if (use2 && (t = dx2f[j] / (fabs(w[OC_IVY * ncell + q]) + cs)) < d2) d2 = t;
+        if (use3 && (t = dx3f[k] / (fabs(w[OC_IVZ * ncell + q]) + cs)) < d3) d3 = t;
+      }
+  free(dx1f);
+  free(dx2f);
+  free(dx3f);
+  double dt = d1 < d2 ? d1 : d2;
+  dt = dt < d3 ? dt : d3;
+  return 1.0 * cfl * dt;
+}
+
+/* MeshBlockImpl::advance_local over the three Shu-Osher SSP-RK3 stages
+   (src/mesh/meshblock.cpp:577-786); the stage weights (w0, w1, w2) of
+   u <- w0 u0 + w1 u1 + w2 du are those of the harp integrator (not vendored;
+   pinned here by the SSP-RK3 closed form). */
+void oc_rk3_step(const oc_config* c, double dt, double* u, double* w) {
+  static const double W[3][3] = {
+      {0., 1., 1.}, {3. / 4., 1. / 4., 1. / 4.}, {1. / 3., 2. / 3., 2. / 3.}};
+  int n1 = oc_nc1(c), n2 = oc_nc2(c), n3 = oc_nc3(c);
+  long n = 5L * n3 * n2 * n1;
+  double* u0 = malloc(sizeof(double) * n);
+  double* u1 = malloc(sizeof(double) * n);
+  double* du = malloc(sizeof(double) * n);
+  memcpy(u0, u, sizeof(double) * n);
+  memcpy(u1, u, sizeof(double) * n);
+  for (int s = 0; s < 3; ++s) {
+    oc_hydro_forward(c, dt, u, w, du, NULL, NULL, NULL);
+    for (long q = 0; q < n; ++q) u[q] = W[s][0] * u0[q] + W[s][1] * u1[q] + W[s][2] * du[q];
+    conserved_limiter(c, u, n / 5);
+    oc_apply_bc(c, u, 5, OC_CONSERVED);
+    memcpy(u1, u, sizeof(double) * n);
+  }
+  free(u0);
+  free(u1);
+  free(du);
+}
+
+void oc_initialize(const oc_config* c, double* w, double* u) {
+  int n1 = oc_nc1(c), n2 = oc_nc2(c), n3 = oc_nc3(c);
+  oc_apply_bc(c, w, 5, OC_PRIMITIVE);
+  oc_prim2cons(c, w, u, (int64_t)n3 * n2 * n1);
+  oc_apply_bc(c, u, 5, OC_CONSERVED);
+}
