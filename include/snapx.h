@@ -1,0 +1,219 @@
+/*
+ * snapx.h -- C ABI of the MI355X-native finite-volume RHS and vertically
+ * implicit step (libsnapx.so).
+ *
+ * Conventions
+ *  - Every entry point returns 0 on success or a negative SNAPX_E* code;
+ *    snapx_last_error() then holds a message (thread-local).  No exception
+ *    crosses the ABI.
+ *  - All array arguments are device pointers (hipMalloc / torch device
+ *    memory), laid out like the reference tensors: a field is
+ *    [nvar][nc3][nc2][nc1], x1 (vertical) fastest; an interface-state array
+ *    is [2][nvar][nc3][nc2][nc1] with [0] = left (ILT), [1] = right (IRT).
+ *    nc? = nx? + 2*nghost on active axes, 1 otherwise
+ *    (src/coord/coordinate.hpp:58-60).
+ *  - dim is the reference tensor axis: 3 = x1, 2 = x2, 1 = x3
+ *    (src/hydro/hydro_forward.cpp:16).
+ *  - dtype selects float (SNAPX_F32) or double (SNAPX_F64) element storage.
+ *  - stream is a hipStream_t passed as void* (0 = default stream); every
+ *    launch goes to it and no entry point synchronises the device unless it
+ *    says so.
+ *  - The caller owns all memory.  Entry points that need scratch take a
+ *    workspace pointer and its size; *_workspace_bytes() returns the size.
+ *  - Entry points are reentrant: there is no global mutable state apart from
+ *    the thread-local error string.
+ *  - Geometry is uniform Cartesian with face widths (x?max - x?min)/nx?
+ *    (src/coord/cartesian.cpp:8-47).
+ */
+#ifndef SNAPX_H
+#define SNAPX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SNAPX_VERSION 1
+
+/* status codes */
+#define SNAPX_OK 0
+#define SNAPX_EINVAL -1   /* invalid argument / unsupported option */
+#define SNAPX_EHIP -2     /* HIP runtime error */
+#define SNAPX_ESPACE -3   /* workspace too small */
+
+/* element types */
+#define SNAPX_F32 0
+#define SNAPX_F64 1
+
+/* interpolation types (src/recon/interpolation.cpp:6-40) */
+#define SNAPX_INTERP_DC 0
+#define SNAPX_INTERP_PLM 1
+#define SNAPX_INTERP_CP3 2
+#define SNAPX_INTERP_CP5 3
+#define SNAPX_INTERP_WENO3 4
+#define SNAPX_INTERP_WENO5 5
+
+/* Riemann solvers (src/riemann/riemann_solver.cpp:47-68) */
+#define SNAPX_RIEMANN_ROE 0
+#define SNAPX_RIEMANN_LMARS 1
+#define SNAPX_RIEMANN_HLLC 2
+
+/* external boundary types (src/bc/bc_func.cpp:7-63) */
+#define SNAPX_BC_NONE (-1)
+#define SNAPX_BC_REFLECTING 0
+#define SNAPX_BC_OUTFLOW 1
+#define SNAPX_BC_PERIODIC 2
+
+/* variable kinds for boundary functions (src/snap.h:62-71) */
+#define SNAPX_PRIMITIVE 0
+#define SNAPX_CONSERVED 1
+#define SNAPX_SCALAR 2
+
+/*
+ * Block configuration: the hot-path YAML options of one MeshBlock
+ * (dynamics.*, forcing.const-gravity.*, integration.implicit-scheme,
+ * geometry.*, boundary-condition.external.*).  Plain data, no pointers.
+ */
+typedef struct snapx_config {
+  int nx1, nx2, nx3, nghost;
+  double x1min, x1max, x2min, x2max, x3min, x3max;
+  double gammad;
+  /* reconstruction: [0] = dynamics.reconstruct.vertical (x1),
+                     [1] = dynamics.reconstruct.horizontal (x2, x3) */
+  int recon_type[2], recon_scale[2], recon_shock[2];
+  /* Riemann solver per direction: [0] = x1, [1] = x2, [2] = x3 */
+  int riemann[3];
+  int limiter;
+  double density_floor, pressure_floor, temperature_floor, weight;
+  double grav1, grav2, grav3, non_hydrostatic;
+  int implicit_scheme; /* 0 none, 1 vic-partial (3x3), 9 vic-full (5x5) */
+  int disable_flux[3];
+  /* x1-inner, x1-outer, x2-inner, x2-outer, x3-inner, x3-outer */
+  int bc[6];
+} snapx_config;
+
+const char* snapx_last_error(void);
+int snapx_version(void);
+/* number of HIP devices visible (0 on a host without GPUs) */
+int snapx_device_count(void);
+
+/* ---------------------------------------------------------------------- */
+/* Module kernels of the plain path (one call = one reference module call) */
+/* ---------------------------------------------------------------------- */
+
+/* IdealGasImpl::_cons2prim (src/eos/ideal_gas.cpp:90-109) with the limiters
+   of src/eos/equation_of_state.cpp:110-179; replaces the dispatch stub
+   ideal_gas_cons2prim (src/eos/eos_dispatch.hpp:9-13).  u may be modified by
+   the conserved limiter (as in the reference). */
+int snapx_ideal_gas_cons2prim(const snapx_config* cfg, void* u, void* w,
+                              int dtype, void* stream);
+
+/* IdealGasImpl::_prim2cons (src/eos/ideal_gas.cpp:67-88), "W->U". */
+int snapx_ideal_gas_prim2cons(const snapx_config* cfg, void* w, void* u,
+                              int dtype, void* stream);
+
+/* ReconstructImpl::forward (src/recon/reconstruct.cpp:75-154): w
+   [nvar][n3][n2][n1] -> wlr [2][nvar][n3][n2][n1], including the dummy-face
+   fill and limiter clamps; replaces call_weno5 / call_poly5 / call_weno3 /
+   call_poly3 (src/recon/recon_dispatch.hpp:9-18).  revise_x1_lr = 1 applies
+   HydroImpl::_revise_x1inner_lr/_revise_x1outer_lr
+   (src/hydro/hydro.cpp:217-229) at the physical x1 faces (dim = 3 only). */
+int snapx_reconstruct(const void* w, void* wlr, int nvar, int n3, int n2,
+                      int n1, int dim, int interp_type, int scale, int shock,
+                      int has_parent, int limiter, double density_floor,
+                      double pressure_floor, int revise_x1_lr, int nghost,
+                      int dtype, void* stream);
+
+/* RiemannSolverImpl::forward for LMARS / HLLC / Roe
+   (src/riemann/lmars.cpp:30-103, src/riemann/hllc.cpp:30-106,
+   src/riemann/roe.cpp:14-59) over every face of the block, ideal gas;
+   replaces call_lmars / call_hllc / call_roe
+   (src/riemann/riemann_dispatch.hpp:10-23).  face_pressure may be NULL. */
+int snapx_riemann(int solver, const void* wl, const void* wr, void* flx,
+                  void* face_pressure, int nvar, int n3, int n2, int n1,
+                  int dim, double gammad, int dtype, void* stream);
+
+/* CoordinateImpl::divergence (src/coord/coordinate.cpp:352-396), Cartesian:
+   div = sum_d (A F_{+} - A F_{-}) / V on the interior ranges of each active
+   axis.  f1/f2/f3 may be NULL for inactive or disabled axes. */
+int snapx_divergence(const snapx_config* cfg, const void* f1, const void* f2,
+                     const void* f3, void* div, int dtype, void* stream);
+
+/* HydroImpl::_revise_x1inner_ghost / _revise_x1outer_ghost
+   (src/hydro/hydro.cpp:231-269): isentropic hydrostatic refill of the x1
+   ghost cells of w (density and pressure). */
+int snapx_revise_x1_ghost(const snapx_config* cfg, void* w, int dtype,
+                          void* stream);
+
+/* External boundary functions (src/bc/bc_func.cpp:7-63) applied in the order
+   of src/mesh/meshblock.cpp:717-722. */
+int snapx_apply_bc(const snapx_config* cfg, void* var, int nvar, int var_kind,
+                   int dtype, void* stream);
+
+/* Stage combine of the SSP-RK3 integrator, u = w0*u0 + w1*u1 + w2*du
+   (src/mesh/meshblock.cpp:675). */
+int snapx_rk_combine(void* u, const void* u0, const void* u1, const void* du,
+                     double w0, double w1, double w2, int64_t n, int dtype,
+                     void* stream);
+
+/* ImplicitHydroImpl::forward (src/implicit/implicit_hydro.cpp:99-189) via
+   HydroImpl::_apply_implicit_correction (src/hydro/hydro.cpp:186-215):
+   block-tridiagonal assembly, block Thomas sweep and MS-VIC redistribution
+   over every interior column; replaces vic_assemble_* / vic_solve_* /
+   vic_redistribute_* (src/implicit/implicit_dispatch.hpp:9-19).  du is
+   modified in place; w must hold the revised x1 ghosts.  The sweep runs in
+   FP64 whatever dtype is. */
+size_t snapx_implicit_workspace_bytes(const snapx_config* cfg);
+int snapx_implicit_correction(const snapx_config* cfg, double dt, void* du,
+                              const void* w, void* workspace,
+                              size_t workspace_bytes, int dtype, void* stream);
+
+/* ---------------------------------------------------------------------- */
+/* Composed plain path                                                     */
+/* ---------------------------------------------------------------------- */
+
+/* HydroImpl::forward (src/hydro/hydro_forward.cpp:14-217): u -> du (dt
+   factor included); w (hydro_w) is written, ghosts revised as in the
+   reference.  f1/f2/f3 (optional, may be NULL) receive the face fluxes. */
+size_t snapx_hydro_workspace_bytes(const snapx_config* cfg, int dtype);
+int snapx_hydro_forward(const snapx_config* cfg, double dt, void* u, void* w,
+                        void* du, void* f1, void* f2, void* f3,
+                        void* workspace, size_t workspace_bytes, int dtype,
+                        void* stream);
+
+/* One SSP-RK3 step of MeshBlockImpl::advance_local (three stages of
+   hydro forward, stage combine, external boundaries;
+   src/mesh/meshblock.cpp:577-786) on the plain path.  u is advanced in
+   place; w receives the last stage's primitives. */
+size_t snapx_rk3_workspace_bytes(const snapx_config* cfg, int dtype);
+int snapx_rk3_step(const snapx_config* cfg, double dt, void* u, void* w,
+                   void* workspace, size_t workspace_bytes, int dtype,
+                   void* stream);
+
+/* HydroImpl::max_time_step (src/hydro/hydro.cpp:118-180) times cfl
+   (src/mesh/meshblock.cpp:511-530): writes the CFL time step to the device
+   double *dt_out (no host synchronisation). */
+int snapx_max_time_step(const snapx_config* cfg, const void* w, double cfl,
+                        double* dt_out, int dtype, void* stream);
+
+/* ---------------------------------------------------------------------- */
+/* Fused FP32 path (no reference counterpart; same results as the plain    */
+/* path to FP32 rounding)                                                  */
+/* ---------------------------------------------------------------------- */
+
+/* One SSP-RK3 step with FP32 state: per stage one fused kernel (cons->prim,
+   recon, Riemann, divergence, gravity; interface states stay on chip) and
+   one column kernel (implicit solve in FP64, stage combine, boundaries).
+   u is float [5][nc3][nc2][nc1], advanced in place.  Requires nvar = 5,
+   nghost = 3 and the WENO5/CP5 family. */
+size_t snapx_fused_workspace_bytes(const snapx_config* cfg);
+int snapx_fused_rk3_step(const snapx_config* cfg, double dt, float* u,
+                         void* workspace, size_t workspace_bytes,
+                         void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SNAPX_H */

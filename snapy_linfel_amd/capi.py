@@ -1,0 +1,266 @@
+"""ctypes binding of the C ABI in include/snapx.h (libsnapx.so).
+
+This is the Python-side FFI a caller binds to the drop-in library; arrays are
+torch device tensors (the data carrier), passed as raw pointers together with
+torch's current HIP stream.  Every call checks the returned status and raises
+``SnapxError`` with ``snapx_last_error()`` on failure.  There is no CPU
+fallback: without the built library or without a GPU the calls fail loudly.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libsnapx.so")
+
+F32, F64 = 0, 1
+INTERP = {"dc": 0, "plm": 1, "cp3": 2, "cp5": 3, "weno3": 4, "weno5": 5}
+RIEMANN = {"roe": 0, "lmars": 1, "hllc": 2}
+BC = {None: -1, "none": -1, "reflecting": 0, "outflow": 1, "periodic": 2}
+PRIMITIVE, CONSERVED, SCALAR = 0, 1, 2
+
+
+class SnapxError(RuntimeError):
+    pass
+
+
+class Config(C.Structure):
+    """snapx_config (include/snapx.h)."""
+    _fields_ = [
+        ("nx1", C.c_int), ("nx2", C.c_int), ("nx3", C.c_int), ("nghost", C.c_int),
+        ("x1min", C.c_double), ("x1max", C.c_double),
+        ("x2min", C.c_double), ("x2max", C.c_double),
+        ("x3min", C.c_double), ("x3max", C.c_double),
+        ("gammad", C.c_double),
+        ("recon_type", C.c_int * 2), ("recon_scale", C.c_int * 2),
+        ("recon_shock", C.c_int * 2),
+        ("riemann", C.c_int * 3),
+        ("limiter", C.c_int),
+        ("density_floor", C.c_double), ("pressure_floor", C.c_double),
+        ("temperature_floor", C.c_double), ("weight", C.c_double),
+        ("grav1", C.c_double), ("grav2", C.c_double), ("grav3", C.c_double),
+        ("non_hydrostatic", C.c_double),
+        ("implicit_scheme", C.c_int),
+        ("disable_flux", C.c_int * 3),
+        ("bc", C.c_int * 6),
+    ]
+
+    @property
+    def shape(self):
+        g = self.nghost
+        n1 = self.nx1 + 2 * g if self.nx1 > 1 else 1
+        n2 = self.nx2 + 2 * g if self.nx2 > 1 else 1
+        n3 = self.nx3 + 2 * g if self.nx3 > 1 else 1
+        return (5, n3, n2, n1)
+
+    @property
+    def n_interior(self):
+        return self.nx1 * self.nx2 * self.nx3
+
+
+def make_config(**kw) -> Config:
+    """Config from keyword options; names follow the hot-path YAML keys."""
+    c = Config()
+    c.nx1, c.nx2, c.nx3 = kw.get("nx1", 1), kw.get("nx2", 1), kw.get("nx3", 1)
+    c.nghost = kw.get("nghost", 3)
+    c.x1min, c.x1max = kw.get("x1min", 0.0), kw.get("x1max", 1.0)
+    c.x2min, c.x2max = kw.get("x2min", 0.0), kw.get("x2max", 1.0)
+    c.x3min, c.x3max = kw.get("x3min", 0.0), kw.get("x3max", 1.0)
+    c.gammad = kw.get("gammad", 1.4)
+    rv = kw.get("recon_vertical", ("weno5", False, False))
+    rh = kw.get("recon_horizontal", rv)
+    for r, (t, sc, sh) in enumerate((rv, rh)):
+        c.recon_type[r], c.recon_scale[r], c.recon_shock[r] = INTERP[t], int(sc), int(sh)
+    rs = kw.get("riemann", "lmars")
+    if isinstance(rs, str):
+        rs = (rs, rs, rs)
+    for d in range(3):
+        c.riemann[d] = RIEMANN[rs[d]]
+    c.limiter = int(kw.get("limiter", False))
+    c.density_floor = kw.get("density_floor", 1e-10)
+    c.pressure_floor = kw.get("pressure_floor", 1e-10)
+    c.temperature_floor = kw.get("temperature_floor", 20.0)
+    c.weight = kw.get("weight", 29e-3)
+    c.grav1, c.grav2, c.grav3 = kw.get("grav1", 0.0), kw.get("grav2", 0.0), kw.get("grav3", 0.0)
+    c.non_hydrostatic = kw.get("non_hydrostatic", 1.0)
+    c.implicit_scheme = kw.get("implicit_scheme", 0)
+    df = kw.get("disable_flux", (0, 0, 0))
+    for d in range(3):
+        c.disable_flux[d] = int(df[d])
+    bcs = kw.get("bc", ("reflecting",) * 6)
+    active = [c.nx1 > 1, c.nx1 > 1, c.nx2 > 1, c.nx2 > 1, c.nx3 > 1, c.nx3 > 1]
+    for d in range(6):
+        c.bc[d] = BC[bcs[d]] if active[d] else -1
+    return c
+
+
+_lib = None
+
+# name -> (restype, argtypes)
+_P, _I, _D, _Z, _L = C.c_void_p, C.c_int, C.c_double, C.c_size_t, C.c_int64
+_CFG = C.POINTER(Config)
+SIGNATURES = {
+    "snapx_last_error": (C.c_char_p, []),
+    "snapx_version": (_I, []),
+    "snapx_device_count": (_I, []),
+    "snapx_ideal_gas_cons2prim": (_I, [_CFG, _P, _P, _I, _P]),
+    "snapx_ideal_gas_prim2cons": (_I, [_CFG, _P, _P, _I, _P]),
+    "snapx_reconstruct": (_I, [_P, _P, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _D, _D, _I, _I,
+                               _I, _P]),
+    "snapx_riemann": (_I, [_I, _P, _P, _P, _P, _I, _I, _I, _I, _I, _D, _I, _P]),
+    "snapx_divergence": (_I, [_CFG, _P, _P, _P, _P, _I, _P]),
+    "snapx_revise_x1_ghost": (_I, [_CFG, _P, _I, _P]),
+    "snapx_apply_bc": (_I, [_CFG, _P, _I, _I, _I, _P]),
+    "snapx_rk_combine": (_I, [_P, _P, _P, _P, _D, _D, _D, _L, _I, _P]),
+    "snapx_implicit_workspace_bytes": (_Z, [_CFG]),
+    "snapx_implicit_correction": (_I, [_CFG, _D, _P, _P, _P, _Z, _I, _P]),
+    "snapx_hydro_workspace_bytes": (_Z, [_CFG, _I]),
+    "snapx_hydro_forward": (_I, [_CFG, _D, _P, _P, _P, _P, _P, _P, _P, _Z, _I, _P]),
+    "snapx_rk3_workspace_bytes": (_Z, [_CFG, _I]),
+    "snapx_rk3_step": (_I, [_CFG, _D, _P, _P, _P, _Z, _I, _P]),
+    "snapx_max_time_step": (_I, [_CFG, _P, _D, _P, _I, _P]),
+    "snapx_fused_workspace_bytes": (_Z, [_CFG]),
+    "snapx_fused_rk3_step": (_I, [_CFG, _D, _P, _P, _Z, _P]),
+}
+
+
+def lib():
+    """Load libsnapx.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SnapxError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status: int, what: str):
+    if status != 0:
+        msg = lib().snapx_last_error().decode()
+        raise SnapxError(f"{what} failed ({status}): {msg}")
+
+
+# ---- torch helpers ----
+
+def _torch():
+    import torch
+    return torch
+
+
+def stream_ptr():
+    torch = _torch()
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def ptr(t):
+    if t is None:
+        return None
+    torch = _torch()
+    assert t.is_cuda, "snapx expects device tensors"
+    assert t.is_contiguous(), "snapx expects contiguous tensors"
+    return C.c_void_p(t.data_ptr())
+
+
+def dtype_code(t):
+    torch = _torch()
+    if t.dtype == torch.float64:
+        return F64
+    if t.dtype == torch.float32:
+        return F32
+    raise SnapxError(f"unsupported dtype {t.dtype}")
+
+
+def workspace(nbytes, device="cuda"):
+    torch = _torch()
+    return torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)
+
+
+def cons2prim(cfg, u, w):
+    check(lib().snapx_ideal_gas_cons2prim(C.byref(cfg), ptr(u), ptr(w), dtype_code(u),
+                                          stream_ptr()), "cons2prim")
+
+
+def prim2cons(cfg, w, u):
+    check(lib().snapx_ideal_gas_prim2cons(C.byref(cfg), ptr(w), ptr(u), dtype_code(w),
+                                          stream_ptr()), "prim2cons")
+
+
+def reconstruct(w, wlr, dim, type="weno5", scale=False, shock=True, has_parent=True,
+                limiter=False, dfloor=1e-10, pfloor=1e-10, revise_x1_lr=False, nghost=3):
+    nvar, n3, n2, n1 = w.shape
+    check(lib().snapx_reconstruct(ptr(w), ptr(wlr), nvar, n3, n2, n1, dim, INTERP[type],
+                                  int(scale), int(shock), int(has_parent), int(limiter), dfloor,
+                                  pfloor, int(revise_x1_lr), nghost, dtype_code(w),
+                                  stream_ptr()), "reconstruct")
+
+
+def riemann(solver, wl, wr, flx, fp, dim, gammad=1.4):
+    nvar, n3, n2, n1 = wl.shape
+    check(lib().snapx_riemann(RIEMANN[solver], ptr(wl), ptr(wr), ptr(flx), ptr(fp), nvar, n3,
+                              n2, n1, dim, gammad, dtype_code(wl), stream_ptr()), "riemann")
+
+
+def divergence(cfg, f1, f2, f3, div):
+    check(lib().snapx_divergence(C.byref(cfg), ptr(f1), ptr(f2), ptr(f3), ptr(div),
+                                 dtype_code(div), stream_ptr()), "divergence")
+
+
+def revise_x1_ghost(cfg, w):
+    check(lib().snapx_revise_x1_ghost(C.byref(cfg), ptr(w), dtype_code(w), stream_ptr()),
+          "revise_x1_ghost")
+
+
+def apply_bc(cfg, var, kind=CONSERVED):
+    check(lib().snapx_apply_bc(C.byref(cfg), ptr(var), var.shape[0], kind, dtype_code(var),
+                               stream_ptr()), "apply_bc")
+
+
+def rk_combine(u, u0, u1, du, w0, w1, w2):
+    check(lib().snapx_rk_combine(ptr(u), ptr(u0), ptr(u1), ptr(du), w0, w1, w2, u.numel(),
+                                 dtype_code(u), stream_ptr()), "rk_combine")
+
+
+def implicit_correction(cfg, dt, du, w, ws=None):
+    n = lib().snapx_implicit_workspace_bytes(C.byref(cfg))
+    ws = workspace(n, du.device) if ws is None else ws
+    check(lib().snapx_implicit_correction(C.byref(cfg), dt, ptr(du), ptr(w), ptr(ws), n,
+                                          dtype_code(du), stream_ptr()), "implicit_correction")
+
+
+def hydro_forward(cfg, dt, u, w, du, f1=None, f2=None, f3=None, ws=None):
+    n = lib().snapx_hydro_workspace_bytes(C.byref(cfg), dtype_code(u))
+    ws = workspace(n, u.device) if ws is None else ws
+    check(lib().snapx_hydro_forward(C.byref(cfg), dt, ptr(u), ptr(w), ptr(du), ptr(f1), ptr(f2),
+                                    ptr(f3), ptr(ws), ws.numel(), dtype_code(u), stream_ptr()),
+          "hydro_forward")
+
+
+def rk3_workspace(cfg, dtype, device="cuda"):
+    return workspace(lib().snapx_rk3_workspace_bytes(C.byref(cfg), dtype), device)
+
+
+def rk3_step(cfg, dt, u, w, ws=None):
+    ws = rk3_workspace(cfg, dtype_code(u), u.device) if ws is None else ws
+    check(lib().snapx_rk3_step(C.byref(cfg), dt, ptr(u), ptr(w), ptr(ws), ws.numel(),
+                               dtype_code(u), stream_ptr()), "rk3_step")
+
+
+def max_time_step(cfg, w, cfl, out):
+    check(lib().snapx_max_time_step(C.byref(cfg), ptr(w), cfl, ptr(out), dtype_code(w),
+                                    stream_ptr()), "max_time_step")
+
+
+def fused_workspace(cfg, device="cuda"):
+    return workspace(lib().snapx_fused_workspace_bytes(C.byref(cfg)), device)
+
+
+def fused_rk3_step(cfg, dt, u, ws=None):
+    ws = fused_workspace(cfg, u.device) if ws is None else ws
+    check(lib().snapx_fused_rk3_step(C.byref(cfg), dt, ptr(u), ptr(ws), ws.numel(),
+                                     stream_ptr()), "fused_rk3_step")

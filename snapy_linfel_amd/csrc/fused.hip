@@ -1,0 +1,650 @@
+// Fused FP32 SSP-RK3 stage on MI355X.
+//
+// Per stage two kernels replace the ~15 module launches of the plain path:
+//
+//  K1 fused_rhs_kernel (FP32): one workgroup per (x1, x2, x3) tile.  The tile
+//     plus a 3-cell halo of the conserved state is read from HBM once
+//     (coalesced along x1), converted to primitives into LDS, the x1 ghost
+//     cells get the isentropic hydrostatic refill, and then for each
+//     direction the interface states (WENO5/CP5), Riemann fluxes and the flux
+//     differences are computed from LDS; interface states and fluxes never
+//     leave the chip.  Output: the explicit tendency du (dt included) of the
+//     tile's cells, gravity added.
+//  K2 fused_column_kernel (FP64 arithmetic): one lane per vertical column.
+//     Rebuilds the column's primitives in FP64, assembles the block rows with
+//     rolling interface Jacobians, runs the block Thomas sweep and the MS-VIC
+//     redistribution, and applies the SSP-RK3 stage combine; writes the new
+//     conserved state.
+//
+// Ghost cells are never read during the step: halo loads map outside indices
+// through the external boundary functions (reflecting: mirror and flip the
+// normal momentum; outflow: copy the edge cell; periodic: wrap), which gives
+// the values the reference's boundary pass would have written.  One boundary
+// kernel at the end of the step fills the stored ghosts.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "common.hpp"
+#include "interp_impl.hpp"
+#include "riemann_impl.hpp"
+#include "vic_impl.hpp"
+
+namespace snapx {
+
+// implemented in plain.hip
+int launch_bc_f32(const snapx_config* c, float* var, hipStream_t st);
+
+constexpr int kRhsThreads = 256;
+constexpr int kMaxCellsPerThread = 4;  // tile cells <= 1024
+constexpr int kColThreads = 64;
+
+struct FusedArgs {
+  int n1, n2, n3, ng;
+  int nx1, nx2, nx3;
+  int tx1, tx2, tx3;  // tile extents (interior cells)
+  int h2, h3;         // halo widths along x2 / x3 (3 or 0)
+  int nt1, nt2, nt3;  // tiles per axis
+  int recon_type[2], recon_scale[2], recon_shock[2];
+  int riemann[3];
+  int bc[6];
+  float gammad, dx1, dx2, dx3, dt, grav1, grav2, grav3;
+};
+
+// Maps an interior-relative index x (may lie in the ghost range) of an axis
+// with nx interior cells through that axis' boundary functions.  Returns the
+// source index and sets flip when the normal momentum changes sign.
+__device__ __forceinline__ int bc_map(int x, int nx, int bin, int bout, bool& flip) {
+  if (x < 0) {
+    if (bin == SNAPX_BC_REFLECTING) {
+      flip = !flip;
+      return -1 - x;
+    }
+    if (bin == SNAPX_BC_PERIODIC) return x + nx;
+    return 0;  // outflow (and no boundary function)
+  }
+  if (x >= nx) {
+    if (bout == SNAPX_BC_REFLECTING) {
+      flip = !flip;
+      return 2 * nx - 1 - x;
+    }
+    if (bout == SNAPX_BC_PERIODIC) return x - nx;
+    return nx - 1;
+  }
+  return x;
+}
+
+// cell index in the full array of interior-relative (i, j, k), mapping
+// through the boundaries; flips[a] reports a sign change of the axis-a
+// momentum
+__device__ __forceinline__ int mapped_cell(const FusedArgs& a, int i, int j, int k, bool* flips) {
+  flips[0] = flips[1] = flips[2] = false;
+  int lim = a.nx1 + a.ng - 1;
+  i = i > lim ? lim : i;
+  i = bc_map(i, a.nx1, a.bc[0], a.bc[1], flips[0]);
+  int qi = i + a.ng, qj = 0, qk = 0;
+  if (a.n2 > 1) {
+    int l2 = a.nx2 + a.ng - 1;
+    j = j > l2 ? l2 : j;
+    j = bc_map(j, a.nx2, a.bc[2], a.bc[3], flips[1]);
+    qj = j + a.ng;
+  }
+  if (a.n3 > 1) {
+    int l3 = a.nx3 + a.ng - 1;
+    k = k > l3 ? l3 : k;
+    k = bc_map(k, a.nx3, a.bc[4], a.bc[5], flips[2]);
+    qk = k + a.ng;
+  }
+  return (qk * a.n2 + qj) * a.n1 + qi;
+}
+
+template <typename T>
+__device__ __forceinline__ void cons_to_prim(const T* U, T gammad, T* W) {
+  W[IDN] = U[IDN];
+  T inv = T(1) / U[IDN];
+  W[IVX] = U[IVX] * inv;
+  W[IVY] = U[IVY] * inv;
+  W[IVZ] = U[IVZ] * inv;
+  T ke = T(0.5) * ((W[IVX] * U[IVX] + W[IVY] * U[IVY]) + W[IVZ] * U[IVZ]);
+  W[IPR] = (gammad - T(1.)) * (U[IPR] - ke);
+}
+
+__global__ void __launch_bounds__(kRhsThreads)
+    fused_rhs_kernel(const float* __restrict__ u, float* __restrict__ du, FusedArgs a) {
+  extern __shared__ float smem[];
+  const int H1 = a.tx1 + 6, H2 = a.tx2 + 2 * a.h2, H3 = a.tx3 + 2 * a.h3;
+  const int H = H1 * H2 * H3;
+  float* ws = smem;          // primitives [5][H3][H2][H1]
+  float* fs = smem + 5 * H;  // face fluxes of one direction [5][faces]
+  const int ncell = a.n1 * a.n2 * a.n3;
+
+  int b = blockIdx.x;
+  int t1 = b % a.nt1;
+  int t2 = (b / a.nt1) % a.nt2;
+  int t3 = b / (a.nt1 * a.nt2);
+  int o1 = t1 * a.tx1, o2 = t2 * a.tx2, o3 = t3 * a.tx3;
+  int e1 = min(a.tx1, a.nx1 - o1);
+  int e2 = a.n2 > 1 ? min(a.tx2, a.nx2 - o2) : 1;
+  int e3 = a.n3 > 1 ? min(a.tx3, a.nx3 - o3) : 1;
+  const float gammad = a.gammad;
+
+  // ---- 1. halo'd tile: conserved -> primitive into LDS ----
+  for (int l = threadIdx.x; l < H; l += blockDim.x) {
+    int x = l % H1;
+    int y = (l / H1) % H2;
+    int z = l / (H1 * H2);
+    bool fl[3];
+    int q = mapped_cell(a, o1 + x - 3, o2 + y - a.h2, o3 + z - a.h3, fl);
+    float U[5], W[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) U[v] = u[v * ncell + q];
+    if (fl[0]) U[IVX] = -U[IVX];
+    if (fl[1]) U[IVY] = -U[IVY];
+    if (fl[2]) U[IVZ] = -U[IVZ];
+    cons_to_prim(U, gammad, W);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) ws[v * H + l] = W[v];
+  }
+  __syncthreads();
+
+  // ---- 2. isentropic refill of the physical x1 ghosts (hydro.cpp:231-269) ----
+  if (a.grav1 != 0.f) {
+    bool bot = o1 == 0, top = o1 + e1 == a.nx1;
+    int ncolt = e2 * e3;
+    for (int c = threadIdx.x; c < 2 * ncolt; c += blockDim.x) {
+      bool upper = c >= ncolt;
+      if (upper ? !top : !bot) continue;
+      int cc = upper ? c - ncolt : c;
+      int y = a.h2 + cc % e2, z = a.h3 + cc / e2;
+      float* rho = ws + IDN * H + (z * H2 + y) * H1;
+      float* pre = ws + IPR * H + (z * H2 + y) * H1;
+      float gamma = gammad, gm = gamma - 1.f, ae = gm / gamma;
+      float inv_g = 1.f / gamma, inv_a = 1.f / ae;
+      float grav = -a.grav1;
+      int is = upper ? 3 + e1 - 1 : 3;
+      float K = pre[is] / powf(rho[is], gamma);
+      float Kg = powf(K, inv_g);
+      for (int n = 0; n < 3; ++n) {
+        int src = upper ? is + n : is - n;
+        int dst = upper ? is + n + 1 : is - n - 1;
+        float h = upper ? powf(pre[src], ae) - ae * grav * a.dx1 / Kg
+                        : powf(pre[src], ae) + ae * grav * a.dx1 / Kg;
+        pre[dst] = powf(h, inv_a);
+        rho[dst] = powf(pre[dst] / K, inv_g);
+      }
+    }
+    __syncthreads();
+  }
+
+  // ---- 3. per direction: interface states + fluxes in LDS, differences ----
+  float acc[kMaxCellsPerThread][5];
+#pragma unroll
+  for (int r = 0; r < kMaxCellsPerThread; ++r)
+#pragma unroll
+    for (int v = 0; v < 5; ++v) acc[r][v] = 0.f;
+  const int ncells_t = e1 * e2 * e3;
+
+  for (int d = 0; d < 3; ++d) {
+    int nd = d == 0 ? a.nx1 : (d == 1 ? a.nx2 : a.nx3);
+    int ncd = d == 0 ? a.n1 : (d == 1 ? a.n2 : a.n3);
+    if (ncd == 1) continue;
+    // face space: extent along d grows by one
+    int f1 = e1 + (d == 0), f2 = e2 + (d == 1), f3 = e3 + (d == 2);
+    int nf = f1 * f2 * f3;
+    int sd = d == 0 ? 1 : (d == 1 ? H1 : H1 * H2);  // LDS stride along d
+    int r = d == 0 ? 0 : 1;
+    int type = a.recon_type[r], scale = a.recon_scale[r], shock = a.recon_shock[r];
+    int type2 = shock ? type : interp2_type(type);
+    int dimc = 3 - d;  // reference tensor axis
+    int solver = a.riemann[d];
+    for (int f = threadIdx.x; f < nf; f += blockDim.x) {
+      int fx = f % f1, fy = (f / f1) % f2, fz = f / (f1 * f2);
+      // LDS coordinates of the cell above the face along d
+      int x = fx + 3, y = fy + a.h2, z = fz + a.h3;
+      int lin = (z * H2 + y) * H1 + x;
+      int c = d == 0 ? x : (d == 1 ? y : z);
+      int base = lin - c * sd;  // pencil start along d
+      float wl[5], wr[5];
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        int tv = (v == IDN) ? type : type2;
+        const float* p = ws + v * H + base;
+        wr[v] = interp_face<float>(tv, 0, scale, p, sd, c);
+        wl[v] = interp_face<float>(tv, 1, scale, p, sd, c - 1);
+      }
+      if (d == 0 && a.grav1 != 0.f) {
+        // _revise_x1inner_lr / _revise_x1outer_lr (hydro.cpp:217-229)
+        int gi = o1 + fx;
+        if (gi == 0) {
+          wl[IDN] = wr[IDN];
+          wl[IPR] = wr[IPR];
+        }
+        if (gi == nd) {
+          wr[IDN] = wl[IDN];
+          wr[IPR] = wl[IPR];
+        }
+      }
+      float flx[5];
+      riemann_flux<float>(solver, wl, wr, gammad, dimc, flx, nullptr);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) fs[v * nf + f] = flx[v];
+    }
+    __syncthreads();
+    float area = d == 0 ? a.dx3 * a.dx2 : (d == 1 ? a.dx3 * a.dx1 : a.dx2 * a.dx1);
+#pragma unroll
+    for (int rr = 0; rr < kMaxCellsPerThread; ++rr) {
+      int cid = threadIdx.x + rr * kRhsThreads;
+      if (cid < ncells_t) {
+        int cx = cid % e1, cy = (cid / e1) % e2, cz = cid / (e1 * e2);
+        int lo = (cz * f2 + cy) * f1 + cx;
+        int hi = lo + (d == 0 ? 1 : (d == 1 ? f1 : f1 * f2));
+#pragma unroll
+        for (int v = 0; v < 5; ++v) acc[rr][v] += area * (fs[v * nf + hi] - fs[v * nf + lo]);
+      }
+    }
+    __syncthreads();
+  }
+
+  // ---- 4. du = -dt D + gravity ----
+  float vol = a.dx3 * (a.dx2 * a.dx1);
+  float dt = a.dt;
+#pragma unroll
+  for (int rr = 0; rr < kMaxCellsPerThread; ++rr) {
+    int cid = threadIdx.x + rr * kRhsThreads;
+    if (cid >= ncells_t) continue;
+    int cx = cid % e1, cy = (cid / e1) % e2, cz = cid / (e1 * e2);
+    int lin = ((cz + a.h3) * H2 + cy + a.h2) * H1 + cx + 3;
+    float d[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) d[v] = -dt * (acc[rr][v] / vol);
+    float rho = ws[IDN * H + lin];
+    if (a.grav1 != 0.f) {
+      d[IVX] += dt * rho * a.grav1;
+      d[IPR] += dt * rho * ws[IVX * H + lin] * a.grav1;
+    }
+    if (a.grav2 != 0.f) {
+      d[IVY] += dt * rho * a.grav2;
+      d[IPR] += dt * rho * ws[IVY * H + lin] * a.grav2;
+    }
+    if (a.grav3 != 0.f) {
+      d[IVZ] += dt * rho * a.grav3;
+      d[IPR] += dt * rho * ws[IVZ * H + lin] * a.grav3;
+    }
+    int qi = o1 + cx + a.ng;
+    int qj = a.n2 > 1 ? o2 + cy + a.ng : 0;
+    int qk = a.n3 > 1 ? o3 + cz + a.ng : 0;
+    int q = (qk * a.n2 + qj) * a.n1 + qi;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) du[v * ncell + q] = d[v];
+  }
+}
+
+struct ColArgs {
+  int n1, n2, n3, ng;
+  int nl, ncol, ncol2;
+  int bc_in, bc_out;  // x1 boundary functions
+  int scheme;         // 0 none, 1 partial, 9 full
+  int stage;
+  double gammad, dx1, area, vol, dt, grav1;
+  double w0, w1, w2;  // stage weights
+};
+
+// primitive state (FP64) of interior-relative layer i of a column whose
+// interior cell 0 is at full index q0, through the x1 boundary functions
+__device__ __forceinline__ void column_prim(const float* __restrict__ u, int ncell, int q0, int i,
+                                            const ColArgs& a, double* W) {
+  bool flip = false;
+  int ii = bc_map(i, a.nl, a.bc_in, a.bc_out, flip);
+  double U[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) U[v] = (double)u[v * ncell + q0 + ii];
+  if (flip) U[IVX] = -U[IVX];
+  cons_to_prim<double>(U, a.gammad, W);
+}
+
+// isentropic refill of the ghost next to boundary cell wb (dir -1: below,
+// +1: above) (src/hydro/hydro.cpp:231-269, first ghost)
+__device__ __forceinline__ void refill_ghost(const double* wb, double* wg, const ColArgs& a,
+                                             int dir) {
+  double gamma = a.gammad, gm = gamma - 1., ae = gm / gamma;
+  double grav = -a.grav1;
+  double K = wb[IPR] / pow(wb[IDN], gamma);
+  double h = dir < 0 ? pow(wb[IPR], ae) + ae * grav * a.dx1 / pow(K, 1. / gamma)
+                     : pow(wb[IPR], ae) - ae * grav * a.dx1 / pow(K, 1. / gamma);
+  wg[IPR] = pow(h, 1. / ae);
+  wg[IDN] = pow(wg[IPR] / K, 1. / gamma);
+}
+
+template <int N>
+__global__ void __launch_bounds__(kColThreads)
+    fused_column_kernel(float* __restrict__ u, float* __restrict__ u0,
+                        const float* __restrict__ du, double* __restrict__ scr, ColArgs a) {
+  int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= a.ncol) return;
+  const int ncell = a.n1 * a.n2 * a.n3;
+  int j = a.n2 > 1 ? a.ng + col % a.ncol2 : 0;
+  int k = a.n3 > 1 ? a.ng + col / a.ncol2 : 0;
+  int q0 = (k * a.n2 + j) * a.n1 + a.ng;
+  const int nl = a.nl;
+  const int NN = N * N;
+  const int S = NN + N;  // scratch doubles per layer
+  const int64_t sstride = a.ncol;
+
+  double sum_a2 = 0.0;
+  if (N > 0 && a.scheme) {
+    double wm[5], wc[5], wp[5];
+    column_prim(u, ncell, q0, 0, a, wc);
+    column_prim(u, ncell, q0, -1, a, wm);
+    refill_ghost(wc, wm, a, -1);
+    double gm1 = a.gammad - 1.;
+    double gm1r = 0.5 * (a.gammad + a.gammad) - 1.;
+    double Jp[NN], Jc[NN], Jn[NN], Am[NN], Ap[NN];
+    flux_jacobian<double, N>(gm1, wm, Jp);
+    flux_jacobian<double, N>(gm1, wc, Jc);
+    abs_jacobian<double, N>(gm1r, wm, wc, Am);
+    double aprev[NN], dprev[N];
+    double hv = 0.5 / a.vol;
+    for (int i = 0; i < nl; ++i) {
+      if (i + 1 < nl) {
+        column_prim(u, ncell, q0, i + 1, a, wp);
+      } else {
+        column_prim(u, ncell, q0, nl, a, wp);
+        refill_ghost(wc, wp, a, +1);
+      }
+      flux_jacobian<double, N>(gm1, wp, Jn);
+      abs_jacobian<double, N>(gm1r, wc, wp, Ap);
+      double A[NN], B[NN], C[NN];
+#pragma unroll
+      for (int r = 0; r < N; ++r)
+#pragma unroll
+        for (int c = 0; c < N; ++c) {
+          int o = r * N + c;
+          double phi = ((r == 1 && c == 0) || (vic_row(N, r) == IPR && c == 1)) ? a.grav1 : 0.;
+          A[o] = (Am[o] * a.area + Ap[o] * a.area + (a.area - a.area) * Jc[o]) * hv - phi;
+          if (r == c) A[o] += 1. / a.dt;
+          B[o] = -(Am[o] + Jp[o]) * a.area * hv;
+          C[o] = -(Ap[o] - Jn[o]) * a.area * hv;
+        }
+#pragma unroll
+      for (int r = 0; r < N; ++r)
+#pragma unroll
+        for (int c = 0; c < N; ++c) {
+          double bnd = c == 1 ? -1. : 1.;
+          if (i == 0) A[r * N + c] += B[r * N + c] * bnd;
+          if (i == nl - 1) A[r * N + c] += C[r * N + c] * bnd;
+        }
+      double rhs[N];
+      int q = q0 + i;
+      rhs[0] = (double)du[IDN * ncell + q] / a.dt;
+      rhs[1] = (double)du[IVX * ncell + q] / a.dt;
+      if (N == 3) {
+        rhs[2] = (double)du[IPR * ncell + q] / a.dt;
+      } else {
+        rhs[2] = (double)du[IVY * ncell + q] / a.dt;
+        rhs[3] = (double)du[IVZ * ncell + q] / a.dt;
+        rhs[4] = (double)du[IPR * ncell + q] / a.dt;
+      }
+      if (i > 0) {
+        double tmp[NN], v[N];
+        matmul<double, N>(B, aprev, tmp);
+#pragma unroll
+        for (int e = 0; e < NN; ++e) A[e] -= tmp[e];
+        matvec<double, N>(B, dprev, v);
+#pragma unroll
+        for (int e = 0; e < N; ++e) rhs[e] -= v[e];
+      }
+      if (N == 3) {
+        double inv[9];
+        inverse3<double>(A, inv);
+        matvec<double, 3>(inv, rhs, dprev);
+        matmul<double, 3>(inv, C, aprev);
+      } else {
+        int indx[5];
+        ludcmp5<double>(A, indx);
+        double solved[30];
+#pragma unroll
+        for (int r = 0; r < 5; ++r) {
+#pragma unroll
+          for (int c = 0; c < 5; ++c) solved[r * 6 + c] = C[r * 5 + c];
+          solved[r * 6 + 5] = rhs[r];
+        }
+        lubksb5<double, 6>(A, indx, solved);
+#pragma unroll
+        for (int r = 0; r < 5; ++r) {
+#pragma unroll
+          for (int c = 0; c < 5; ++c) aprev[r * 5 + c] = solved[r * 6 + c];
+          dprev[r] = solved[r * 6 + 5];
+        }
+      }
+      double* sp = scr + (int64_t)i * S * sstride + col;
+#pragma unroll
+      for (int e = 0; e < NN; ++e) sp[e * sstride] = aprev[e];
+#pragma unroll
+      for (int e = 0; e < N; ++e) sp[(NN + e) * sstride] = dprev[e];
+      double ai = wc[IDN] * a.vol;
+      sum_a2 += ai * ai;
+      // roll the interface data up one layer
+#pragma unroll
+      for (int e = 0; e < NN; ++e) {
+        Jp[e] = Jc[e];
+        Jc[e] = Jn[e];
+        Am[e] = Ap[e];
+      }
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        wm[v] = wc[v];
+        wc[v] = wp[v];
+      }
+    }
+    // back substitution; delta_i overwrites the scratch slot of layer i
+    double b_dry = 0.0;
+    {
+      int q = q0 + nl - 1;
+      b_dry += ((double)du[IDN * ncell + q] - dprev[0]) * 1.0 * a.vol;
+    }
+    for (int i = nl - 2; i >= 0; --i) {
+      double* sp = scr + (int64_t)i * S * sstride + col;
+      double Ai[NN], d[N], v[N];
+#pragma unroll
+      for (int e = 0; e < NN; ++e) Ai[e] = sp[e * sstride];
+#pragma unroll
+      for (int e = 0; e < N; ++e) d[e] = sp[(NN + e) * sstride];
+      matvec<double, N>(Ai, dprev, v);
+#pragma unroll
+      for (int e = 0; e < N; ++e) {
+        dprev[e] = d[e] - v[e];
+        sp[(NN + e) * sstride] = dprev[e];
+      }
+      int q = q0 + i;
+      b_dry += ((double)du[IDN * ncell + q] - dprev[0]) * 1.0 * a.vol;
+    }
+    // redistribution + stage combine
+    for (int i = 0; i < nl; ++i) {
+      int q = q0 + i;
+      double* sp = scr + (int64_t)i * S * sstride + col;
+      double d[N];
+#pragma unroll
+      for (int e = 0; e < N; ++e) d[e] = sp[(NN + e) * sstride];
+      double rho = (double)u[IDN * ncell + q];
+      double a_i = rho * a.vol;
+      double dd[5];
+#pragma unroll
+      for (int v = 0; v < 5; ++v) dd[v] = (double)du[v * ncell + q];
+      double fix = d[0] - dd[IDN];
+      dd[IDN] = (dd[IDN] + fix) + rho * a_i * b_dry / sum_a2;
+      dd[IVX] = d[1];
+      if (N == 3) {
+        dd[IPR] = d[2];
+      } else {
+        dd[IVY] = d[2];
+        dd[IVZ] = d[3];
+        dd[IPR] = d[4];
+      }
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        float uc = u[v * ncell + q];
+        float ub = a.stage == 0 ? uc : u0[v * ncell + q];
+        if (a.stage == 0) u0[v * ncell + q] = uc;
+        u[v * ncell + q] = (float)((a.w0 * ub + a.w1 * uc) + a.w2 * dd[v]);
+      }
+    }
+  } else {
+    for (int i = 0; i < nl; ++i) {
+      int q = q0 + i;
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        float uc = u[v * ncell + q];
+        float ub = a.stage == 0 ? uc : u0[v * ncell + q];
+        if (a.stage == 0) u0[v * ncell + q] = uc;
+        u[v * ncell + q] =
+            (float)((a.w0 * ub + a.w1 * uc) + a.w2 * (double)du[v * ncell + q]);
+      }
+    }
+  }
+}
+
+// ----------------------------------------------------------- host side ----
+
+struct Tile {
+  int tx1, tx2, tx3;
+};
+
+// Tile shape: x1 extent as long as possible (coalesced loads, halo ratio),
+// then enough tiles to fill the 256 CUs; the LDS footprint stays below 64 KB
+// so that two workgroups fit one CU.
+static Tile choose_tile(const Grid& g) {
+  Tile t;
+  if (g.n3 > 1) {
+    t = {16, 4, 4};
+  } else if (g.n2 > 1) {
+    t = {32, 8, 1};
+    int64_t ntile = (int64_t)((g.nx1 + 31) / 32) * ((g.nx2 + 7) / 8);
+    if (ntile < 512) t = {16, 4, 1};
+  } else {
+    t = {256, 1, 1};
+  }
+  t.tx1 = std::min(t.tx1, g.nx1);
+  t.tx2 = g.n2 > 1 ? std::min(t.tx2, g.nx2) : 1;
+  t.tx3 = g.n3 > 1 ? std::min(t.tx3, g.nx3) : 1;
+  return t;
+}
+
+static size_t rhs_smem_bytes(const FusedArgs& a) {
+  size_t H = (size_t)(a.tx1 + 6) * (a.tx2 + 2 * a.h2) * (a.tx3 + 2 * a.h3);
+  size_t f1 = (size_t)(a.tx1 + 1) * a.tx2 * a.tx3;
+  size_t f2 = (size_t)a.tx1 * (a.tx2 + 1) * a.tx3;
+  size_t f3 = (size_t)a.tx1 * a.tx2 * (a.tx3 + 1);
+  size_t fm = std::max(f1, std::max(f2, f3));
+  return sizeof(float) * 5 * (H + fm);
+}
+
+static int check_fused(const snapx_config* c) {
+  if (int e = validate_config(c)) return e;
+  SNAPX_CHECK_ARG(c->nghost == 3, "fused path: nghost must be 3");
+  SNAPX_CHECK_ARG(c->nx1 > 1, "fused path: x1 must be active");
+  SNAPX_CHECK_ARG(!c->limiter, "fused path: the EOS limiter is not supported (use the plain path)");
+  SNAPX_CHECK_ARG(c->non_hydrostatic == 1.0,
+                  "fused path: non-hydrostatic < 1 is not supported (use the plain path)");
+  for (int d = 0; d < 3; ++d)
+    SNAPX_CHECK_ARG(!c->disable_flux[d], "fused path: disable-flux is not supported");
+  return 0;
+}
+
+static size_t col_count(const snapx_config* c) {
+  size_t n2 = c->nx2 > 1 ? c->nx2 : 1, n3 = c->nx3 > 1 ? c->nx3 : 1;
+  return n2 * n3;
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+static size_t scratch_bytes(const snapx_config* c) {
+  int N = ((c->implicit_scheme >> 3) & 1) ? 5 : 3;
+  return c->implicit_scheme ? sizeof(double) * (size_t)(N * N + N) * c->nx1 * col_count(c) : 0;
+}
+
+}  // namespace snapx
+
+using namespace snapx;
+
+extern "C" size_t snapx_fused_workspace_bytes(const snapx_config* cfg) {
+  if (check_fused(cfg)) return 0;
+  Grid g = make_grid(cfg);
+  size_t field = align256(sizeof(float) * 5 * (size_t)g.ncell);
+  return 2 * field + align256(scratch_bytes(cfg));
+}
+
+extern "C" int snapx_fused_rk3_step(const snapx_config* cfg, double dt, float* u, void* workspace,
+                                    size_t workspace_bytes, void* stream) {
+  if (int e = check_fused(cfg)) return e;
+  SNAPX_CHECK_ARG(u && workspace, "snapx_fused_rk3_step: null pointer");
+  SNAPX_CHECK_ARG(workspace_bytes >= snapx_fused_workspace_bytes(cfg),
+                  "snapx_fused_rk3_step: workspace too small");
+  hipStream_t st = as_stream(stream);
+  Grid g = make_grid(cfg);
+  size_t field = align256(sizeof(float) * 5 * (size_t)g.ncell);
+  char* base = reinterpret_cast<char*>(workspace);
+  float* du = reinterpret_cast<float*>(base);
+  float* u0 = reinterpret_cast<float*>(base + field);
+  double* scr = reinterpret_cast<double*>(base + 2 * field);
+
+  Tile t = choose_tile(g);
+  FusedArgs a;
+  a.n1 = g.n1, a.n2 = g.n2, a.n3 = g.n3, a.ng = g.ng;
+  a.nx1 = cfg->nx1, a.nx2 = cfg->nx2, a.nx3 = cfg->nx3;
+  a.tx1 = t.tx1, a.tx2 = t.tx2, a.tx3 = t.tx3;
+  a.h2 = g.n2 > 1 ? 3 : 0;
+  a.h3 = g.n3 > 1 ? 3 : 0;
+  a.nt1 = (a.nx1 + a.tx1 - 1) / a.tx1;
+  a.nt2 = g.n2 > 1 ? (a.nx2 + a.tx2 - 1) / a.tx2 : 1;
+  a.nt3 = g.n3 > 1 ? (a.nx3 + a.tx3 - 1) / a.tx3 : 1;
+  for (int r = 0; r < 2; ++r) {
+    a.recon_type[r] = cfg->recon_type[r];
+    a.recon_scale[r] = cfg->recon_scale[r];
+    a.recon_shock[r] = cfg->recon_shock[r];
+  }
+  for (int d = 0; d < 3; ++d) a.riemann[d] = cfg->riemann[d];
+  for (int b = 0; b < 6; ++b) a.bc[b] = cfg->bc[b];
+  a.gammad = (float)cfg->gammad;
+  a.dx1 = (float)g.dx1, a.dx2 = (float)g.dx2, a.dx3 = (float)g.dx3;
+  a.dt = (float)dt;
+  a.grav1 = (float)cfg->grav1, a.grav2 = (float)cfg->grav2, a.grav3 = (float)cfg->grav3;
+  SNAPX_CHECK_ARG((int64_t)a.tx1 * a.tx2 * a.tx3 <= kRhsThreads * kMaxCellsPerThread,
+                  "fused path: tile too large");
+  size_t smem = rhs_smem_bytes(a);
+  int nblk = a.nt1 * a.nt2 * a.nt3;
+
+  ColArgs c;
+  c.n1 = g.n1, c.n2 = g.n2, c.n3 = g.n3, c.ng = g.ng;
+  c.nl = cfg->nx1;
+  c.ncol2 = g.n2 > 1 ? cfg->nx2 : 1;
+  c.ncol = (int)col_count(cfg);
+  c.bc_in = cfg->bc[0], c.bc_out = cfg->bc[1];
+  c.scheme = cfg->implicit_scheme;
+  c.gammad = cfg->gammad;
+  c.dx1 = g.dx1;
+  c.area = g.dx3 * g.dx2;
+  c.vol = g.dx3 * (g.dx2 * g.dx1);
+  c.dt = dt;
+  c.grav1 = cfg->grav1;
+  SNAPX_CHECK_ARG(!c.scheme || c.grav1 != 0., "[ImplicitHydro] forcing does not have const-gravity");
+  int N = ((cfg->implicit_scheme >> 3) & 1) ? 5 : 3;
+  int cblk = (c.ncol + kColThreads - 1) / kColThreads;
+
+  static const double W[3][3] = {
+      {0., 1., 1.}, {3. / 4., 1. / 4., 1. / 4.}, {1. / 3., 2. / 3., 2. / 3.}};
+  for (int s = 0; s < 3; ++s) {
+    hipLaunchKernelGGL(fused_rhs_kernel, dim3(nblk), dim3(kRhsThreads), smem, st, u, du, a);
+    SNAPX_LAUNCH_CHECK();
+    c.stage = s;
+    c.w0 = W[s][0], c.w1 = W[s][1], c.w2 = W[s][2];
+    if (N == 3)
+      hipLaunchKernelGGL(fused_column_kernel<3>, dim3(cblk), dim3(kColThreads), 0, st, u, u0, du,
+                         scr, c);
+    else
+      hipLaunchKernelGGL(fused_column_kernel<5>, dim3(cblk), dim3(kColThreads), 0, st, u, u0, du,
+                         scr, c);
+    SNAPX_LAUNCH_CHECK();
+  }
+  return launch_bc_f32(cfg, u, st);
+}
