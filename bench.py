@@ -1,0 +1,252 @@
+"""Benchmark: cell-updates/s per SSP-RK3 step on one MI355X.
+
+Workload (the configuration the metric is quoted on): 2-D Straka density
+current, 512 x 64 cells (x2 horizontal, periodic, 25.6 km; x1 vertical,
+reflecting, 6.4 km), dry ideal gas cp = 1004, R = 287, theta0 = 300 K,
+-15 K cosine bubble, gravity 9.81, WENO5 (shock: false) + LMARS, vertically
+implicit 3x3 solve, SSP-RK3 with dt = 0.1 s.  Synthetic initial state (the
+problem set-up of examples/straka.cpp), inputs resident in HBM.
+
+Reported:
+  value        FP32 fused path, interior cells x steps / wall seconds
+  fp64_plain   the FP64 plain (module-by-module) GPU path on the same steps
+  error        per-field L2 / Linf relative error of FP32 fused vs FP64 plain
+               after the timed steps, and the FP32 mass drift
+  roofline     dominant kernel of the fused step, algorithmic bytes per
+               launch / its average device time (HIP events on the stream)
+  cpu_baseline the FP64 CPU oracle on a bounded sample of the workload
+  step_hbm_frac  160 B per cell-step (FP32) x CU/s / 8 TB/s
+Run ``python bench.py`` (N = 1); more than one GPU is not part of this build.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+HBM_PEAK_GBS = 8000.0
+NV = 5
+
+
+def straka_cfg(capi, nx2=512, nx1=64):
+    R, cp = 287.0, 1004.0
+    return capi.make_config(nx1=nx1, nx2=nx2, nghost=3, x1min=0.0, x1max=6.4e3, x2min=0.0,
+                            x2max=25.6e3, x3min=0.0, x3max=25.6e3, gammad=cp / (cp - R),
+                            weight=8.31446261815324 / R,
+                            recon_vertical=("weno5", False, False), riemann="lmars",
+                            grav1=-9.81, implicit_scheme=1,
+                            bc=("reflecting", "reflecting", "periodic", "periodic",
+                                "periodic", "periodic"))
+
+
+def box3d_cfg(capi, n=(128, 256, 256)):
+    return capi.make_config(nx1=n[0], nx2=n[1], nx3=n[2], nghost=3, x1min=0.0,
+                            x1max=100.0 * n[0], x2min=0.0, x2max=100.0 * n[1], x3min=0.0,
+                            x3max=100.0 * n[2], gammad=1.4, weight=8.31446261815324 / 287.0,
+                            recon_vertical=("weno5", False, False),
+                            riemann=("lmars", "roe", "roe"), grav1=-9.81, implicit_scheme=1,
+                            bc=("reflecting", "reflecting", "periodic", "periodic", "periodic",
+                                "periodic"))
+
+
+def errors(got, ref, interior):
+    import numpy as np
+    out = {}
+    names = ["rho", "mom1", "mom2", "mom3", "E"]
+    rho_max = np.abs(ref[0][interior]).max()
+    for v, name in enumerate(names):
+        g, r = got[v][interior], ref[v][interior]
+        if np.abs(r).max() == 0:
+            continue
+        scale = max(np.abs(r).max(), rho_max * 300.0 if v in (1, 2, 3) else 0.0)
+        out[name] = {
+            "linf_rel": float(np.abs(g - r).max() / scale),
+            "l2_rel": float(np.linalg.norm(g - r) / max(np.linalg.norm(r), 1e-300)),
+        }
+    return out
+
+
+def cpu_baseline(steps_cap=40, seconds=12.0):
+    import numpy as np
+    import oracle as O
+    O.set_threads(1)
+    oc = O.straka_config(nx2=512, nx1=64, implicit_scheme=1)
+    w, u = O.initialize(oc, O.straka_init(oc))
+    t0 = time.perf_counter()
+    n = 0
+    while n < steps_cap and time.perf_counter() - t0 < seconds:
+        u, w = O.rk3_step(oc, 0.1, u)
+        n += 1
+    t = time.perf_counter() - t0
+    return {"value": 512 * 64 * n / t, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": f"FP64 C oracle, Straka 512x64, {n} RK3 steps from the initial state, "
+                      f"1 thread ({t:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-3d", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from snapy_linfel_amd import capi
+    import oracle as O
+
+    dev = torch.device("cuda:0")
+    cfg = straka_cfg(capi)
+    oc = O.straka_config(nx2=512, nx1=64, implicit_scheme=1)
+    w0, u0 = O.initialize(oc, O.straka_init(oc))
+    I = O.interior(oc)[1:]
+    dt = 0.1
+    K, W = args.steps, args.warmup
+
+    # ---- FP32 fused: warm-up, then K timed steps (launches only) ----
+    u32 = torch.from_numpy(u0).to(dev, torch.float32)
+    ws32 = capi.fused_workspace(cfg)
+    capi.fused_rk3_steps(cfg, dt, W, u32.clone(), ws32)
+    torch.cuda.synchronize()
+    u32.copy_(torch.from_numpy(u0).to(dev, torch.float32))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    capi.fused_rk3_steps(cfg, dt, K, u32, ws32)
+    torch.cuda.synchronize()
+    t32 = time.perf_counter() - t0
+    cus32 = cfg.n_interior * K / t32
+
+    # ---- per-kernel device times (HIP events around every launch) ----
+    tim = capi.Timing()
+    ut = torch.from_numpy(u0).to(dev, torch.float32)
+    nk = min(K, 200)
+    capi.fused_rk3_steps(cfg, dt, nk, ut, ws32, timing=tim)
+    torch.cuda.synchronize()
+    rhs_avg = tim.rhs_ms / max(tim.rhs_launches, 1)
+    col_avg = tim.column_ms / max(tim.column_launches, 1)
+
+    # ---- FP64 plain path on the same steps ----
+    u64 = torch.from_numpy(u0).to(dev, torch.float64)
+    w64 = torch.empty_like(u64)
+    ws64 = capi.rk3_workspace(cfg, capi.F64)
+    for _ in range(W):
+        capi.rk3_step(cfg, dt, u64.clone(), w64, ws64)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(K):
+        capi.rk3_step(cfg, dt, u64, w64, ws64)
+    torch.cuda.synchronize()
+    t64 = time.perf_counter() - t0
+    cus64 = cfg.n_interior * K / t64
+
+    got = u32.double().cpu().numpy()
+    ref = u64.cpu().numpy()
+    err = errors(got, ref, (slice(None),) + I)
+    m0 = u0[0][I].sum()
+    mass_drift = float(abs(got[0][I].sum() - m0) / m0)
+    mass_drift64 = float(abs(ref[0][I].sum() - m0) / m0)
+
+    # ---- roofline of the dominant kernel ----
+    # K1 (fused RHS) algorithmic bytes: read the stage state + write the
+    # tendency, 2 * NV * 4 B per interior cell; K2 (column solve + combine):
+    # read tendency, stage state, u0 and write the state, 4 * NV * 4 B.
+    n = cfg.n_interior
+    k1_bytes, k2_bytes = 2 * NV * 4 * n, 4 * NV * 4 * n
+    if rhs_avg >= col_avg:
+        name, avg_ms, bytes_ = "fused_rhs_kernel", rhs_avg, k1_bytes
+    else:
+        name, avg_ms, bytes_ = "fused_column_kernel<3>", col_avg, k2_bytes
+    achieved = bytes_ / (avg_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+            "kernel": name, "avg_launch_ms": round(avg_ms, 5),
+            "bytes_per_launch": bytes_}
+
+    extra = {"fused_rhs_avg_ms": round(rhs_avg, 5), "fused_column_avg_ms": round(col_avg, 5),
+             "timed_launches": int(tim.rhs_launches + tim.column_launches + tim.bc_launches)}
+
+    # ---- 3-D box (256 x 256 x 128, configuration-3 shape) secondary line ----
+    if not args.no_3d:
+        c3 = box3d_cfg(capi)
+        shp = c3.shape
+        z = (torch.arange(shp[3], dtype=torch.float64) - 3 + 0.5) * 100.0
+        T = 250.0 - 9.81 * z / 1004.5
+        p = 1e5 * (T / 250.0) ** (1004.5 / 287.0)
+        w3 = torch.zeros(shp, dtype=torch.float64)
+        w3[4] = p
+        g = torch.Generator().manual_seed(1234)
+        th = (torch.rand(shp[1:], generator=g, dtype=torch.float64) - 0.5)
+        th[..., 23:] = 0.0
+        w3[0] = p / (287.0 * (T + th))
+        wd = w3.to(dev)
+        ud = torch.empty_like(wd)
+        capi.prim2cons(c3, wd, ud)
+        capi.apply_bc(c3, ud)
+        u3 = ud.float()
+        ws3 = capi.fused_workspace(c3)
+        capi.fused_rk3_steps(c3, 0.5, 2, u3.clone(), ws3)
+        torch.cuda.synchronize()
+        k3 = 10
+        t0 = time.perf_counter()
+        capi.fused_rk3_steps(c3, 0.5, k3, u3, ws3)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter() - t0
+        tim3 = capi.Timing()
+        capi.fused_rk3_steps(c3, 0.5, 3, u3, ws3, timing=tim3)
+        cus3 = c3.n_interior * k3 / t3
+        r3 = tim3.rhs_ms / max(tim3.rhs_launches, 1)
+        c3ms = tim3.column_ms / max(tim3.column_launches, 1)
+        extra["box3d_256x256x128_fp32"] = {
+            "cell_updates_per_s": round(cus3, 1),
+            "ms_per_step": round(t3 / k3 * 1e3, 4),
+            "step_hbm_frac": round(cus3 * 160 / (HBM_PEAK_GBS * 1e9), 4),
+            "fused_rhs_avg_ms": round(r3, 4), "fused_column_avg_ms": round(c3ms, 4),
+            "rhs_hbm_frac": round(2 * NV * 4 * c3.n_interior / (r3 * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                  4),
+        }
+
+    cpu = None if args.no_cpu else cpu_baseline()
+
+    line = {
+        "metric": "cell-updates/s per RK3 step, FP32 fused vs FP64 plain at stated err; "
+                  "adjoint/fwd",
+        "value": round(cus32, 1),
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(t32 / K * 1e3, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (FP64 implicit column solve)",
+        "data": "synthetic (Straka initial state of examples/straka.cpp)",
+        "config": {"workload": "straka_2d_512x64_weno5_lmars_vic3_dt0.1",
+                   "cells": [512, 64], "dt_s": dt, "steps": K},
+        "fp64_plain": {"value": round(cus64, 1), "ms_per_step": round(t64 / K * 1e3, 5)},
+        "fp32_fused_over_fp64_plain": round(cus32 / cus64, 3),
+        "error_fp32_vs_fp64_after_steps": err,
+        "mass_drift_fp32": mass_drift,
+        "mass_drift_fp64": mass_drift64,
+        "adjoint_over_forward": None,
+        "step_hbm_frac": round(cus32 * 160 / (HBM_PEAK_GBS * 1e9), 6),
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -213,6 +213,22 @@ int snapx_fused_rk3_step(const snapx_config* cfg, double dt, float* u,
                          void* workspace, size_t workspace_bytes,
                          void* stream);
 
+/* Per-kernel device time accumulated by snapx_fused_rk3_steps. */
+typedef struct snapx_timing {
+  double rhs_ms;    /* fused RHS kernels (K1) */
+  double column_ms; /* implicit column + stage combine kernels (K2) */
+  double bc_ms;     /* end-of-step boundary kernels */
+  int64_t rhs_launches, column_launches, bc_launches;
+} snapx_timing;
+
+/* nsteps fused SSP-RK3 steps with a fixed dt.  With timing == NULL the
+   launches are only enqueued; with a timing record every kernel launch is
+   bracketed by HIP events on the stream, the stream is synchronised after
+   each step and the device times are accumulated into *timing. */
+int snapx_fused_rk3_steps(const snapx_config* cfg, double dt, int nsteps,
+                          float* u, void* workspace, size_t workspace_bytes,
+                          snapx_timing* timing, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -11,4 +11,17 @@ Native parts:
 """
 from . import capi  # noqa: F401
 
-__all__ = ["capi"]
+__all__ = ["capi", "host"]
+
+
+def __getattr__(name):
+    if name == "host":
+        return _load_host()
+    raise AttributeError(name)
+
+
+def _load_host():
+    """The C++ host modules (Hydro, Reconstruct, ...); raises if not built."""
+    import importlib
+    import torch  # noqa: F401  (libtorch must be loaded first)
+    return importlib.import_module("snapy_linfel_amd._snapy")

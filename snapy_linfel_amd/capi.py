@@ -122,7 +122,15 @@ SIGNATURES = {
     "snapx_max_time_step": (_I, [_CFG, _P, _D, _P, _I, _P]),
     "snapx_fused_workspace_bytes": (_Z, [_CFG]),
     "snapx_fused_rk3_step": (_I, [_CFG, _D, _P, _P, _Z, _P]),
+    "snapx_fused_rk3_steps": (_I, [_CFG, _D, _I, _P, _P, _Z, _P, _P]),
 }
+
+
+class Timing(C.Structure):
+    """snapx_timing (include/snapx.h)."""
+    _fields_ = [("rhs_ms", C.c_double), ("column_ms", C.c_double), ("bc_ms", C.c_double),
+                ("rhs_launches", C.c_int64), ("column_launches", C.c_int64),
+                ("bc_launches", C.c_int64)]
 
 
 def lib():
@@ -258,6 +266,13 @@ def max_time_step(cfg, w, cfl, out):
 
 def fused_workspace(cfg, device="cuda"):
     return workspace(lib().snapx_fused_workspace_bytes(C.byref(cfg)), device)
+
+
+def fused_rk3_steps(cfg, dt, nsteps, u, ws=None, timing=None):
+    ws = fused_workspace(cfg, u.device) if ws is None else ws
+    tp = C.byref(timing) if timing is not None else None
+    check(lib().snapx_fused_rk3_steps(C.byref(cfg), dt, nsteps, ptr(u), ptr(ws), ws.numel(), tp,
+                                      stream_ptr()), "fused_rk3_steps")
 
 
 def fused_rk3_step(cfg, dt, u, ws=None):

@@ -563,24 +563,14 @@ static size_t scratch_bytes(const snapx_config* c) {
   return c->implicit_scheme ? sizeof(double) * (size_t)(N * N + N) * c->nx1 * col_count(c) : 0;
 }
 
-}  // namespace snapx
+struct EventPair {
+  hipEvent_t a, b;
+};
 
-using namespace snapx;
-
-extern "C" size_t snapx_fused_workspace_bytes(const snapx_config* cfg) {
-  if (check_fused(cfg)) return 0;
-  Grid g = make_grid(cfg);
-  size_t field = align256(sizeof(float) * 5 * (size_t)g.ncell);
-  return 2 * field + align256(scratch_bytes(cfg));
-}
-
-extern "C" int snapx_fused_rk3_step(const snapx_config* cfg, double dt, float* u, void* workspace,
-                                    size_t workspace_bytes, void* stream) {
-  if (int e = check_fused(cfg)) return e;
-  SNAPX_CHECK_ARG(u && workspace, "snapx_fused_rk3_step: null pointer");
-  SNAPX_CHECK_ARG(workspace_bytes >= snapx_fused_workspace_bytes(cfg),
-                  "snapx_fused_rk3_step: workspace too small");
-  hipStream_t st = as_stream(stream);
+// One fused RK3 step; with ev != nullptr the 7 launches are bracketed by the
+// event pairs ev[0..6] (K1, K2 per stage, then the boundary kernel).
+static int fused_step(const snapx_config* cfg, double dt, float* u, void* workspace,
+                      hipStream_t st, EventPair* ev) {
   Grid g = make_grid(cfg);
   size_t field = align256(sizeof(float) * 5 * (size_t)g.ncell);
   char* base = reinterpret_cast<char*>(workspace);
@@ -634,10 +624,13 @@ extern "C" int snapx_fused_rk3_step(const snapx_config* cfg, double dt, float* u
   static const double W[3][3] = {
       {0., 1., 1.}, {3. / 4., 1. / 4., 1. / 4.}, {1. / 3., 2. / 3., 2. / 3.}};
   for (int s = 0; s < 3; ++s) {
+    if (ev) SNAPX_HIP(hipEventRecord(ev[2 * s].a, st));
     hipLaunchKernelGGL(fused_rhs_kernel, dim3(nblk), dim3(kRhsThreads), smem, st, u, du, a);
     SNAPX_LAUNCH_CHECK();
+    if (ev) SNAPX_HIP(hipEventRecord(ev[2 * s].b, st));
     c.stage = s;
     c.w0 = W[s][0], c.w1 = W[s][1], c.w2 = W[s][2];
+    if (ev) SNAPX_HIP(hipEventRecord(ev[2 * s + 1].a, st));
     if (N == 3)
       hipLaunchKernelGGL(fused_column_kernel<3>, dim3(cblk), dim3(kColThreads), 0, st, u, u0, du,
                          scr, c);
@@ -645,6 +638,79 @@ extern "C" int snapx_fused_rk3_step(const snapx_config* cfg, double dt, float* u
       hipLaunchKernelGGL(fused_column_kernel<5>, dim3(cblk), dim3(kColThreads), 0, st, u, u0, du,
                          scr, c);
     SNAPX_LAUNCH_CHECK();
+    if (ev) SNAPX_HIP(hipEventRecord(ev[2 * s + 1].b, st));
   }
-  return launch_bc_f32(cfg, u, st);
+  if (ev) SNAPX_HIP(hipEventRecord(ev[6].a, st));
+  if (int e = launch_bc_f32(cfg, u, st)) return e;
+  if (ev) SNAPX_HIP(hipEventRecord(ev[6].b, st));
+  return 0;
+}
+
+}  // namespace snapx
+
+using namespace snapx;
+
+extern "C" size_t snapx_fused_workspace_bytes(const snapx_config* cfg) {
+  if (check_fused(cfg)) return 0;
+  Grid g = make_grid(cfg);
+  size_t field = align256(sizeof(float) * 5 * (size_t)g.ncell);
+  return 2 * field + align256(scratch_bytes(cfg));
+}
+
+extern "C" int snapx_fused_rk3_step(const snapx_config* cfg, double dt, float* u, void* workspace,
+                                    size_t workspace_bytes, void* stream) {
+  if (int e = check_fused(cfg)) return e;
+  SNAPX_CHECK_ARG(u && workspace, "snapx_fused_rk3_step: null pointer");
+  SNAPX_CHECK_ARG(workspace_bytes >= snapx_fused_workspace_bytes(cfg),
+                  "snapx_fused_rk3_step: workspace too small");
+  return fused_step(cfg, dt, u, workspace, as_stream(stream), nullptr);
+}
+
+extern "C" int snapx_fused_rk3_steps(const snapx_config* cfg, double dt, int nsteps, float* u,
+                                     void* workspace, size_t workspace_bytes,
+                                     snapx_timing* timing, void* stream) {
+  if (int e = check_fused(cfg)) return e;
+  SNAPX_CHECK_ARG(u && workspace && nsteps >= 0, "snapx_fused_rk3_steps: bad arguments");
+  SNAPX_CHECK_ARG(workspace_bytes >= snapx_fused_workspace_bytes(cfg),
+                  "snapx_fused_rk3_steps: workspace too small");
+  hipStream_t st = as_stream(stream);
+  if (!timing) {
+    for (int n = 0; n < nsteps; ++n)
+      if (int e = fused_step(cfg, dt, u, workspace, st, nullptr)) return e;
+    return 0;
+  }
+  EventPair ev[7];
+  for (auto& e : ev) {
+    SNAPX_HIP(hipEventCreate(&e.a));
+    SNAPX_HIP(hipEventCreate(&e.b));
+  }
+  int status = 0;
+  for (int n = 0; n < nsteps && !status; ++n) {
+    status = fused_step(cfg, dt, u, workspace, st, ev);
+    if (status) break;
+    if (hipEventSynchronize(ev[6].b) != hipSuccess) {
+      set_error("snapx_fused_rk3_steps: event synchronisation failed");
+      status = SNAPX_EHIP;
+      break;
+    }
+    for (int k = 0; k < 7; ++k) {
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, ev[k].a, ev[k].b);
+      if (k == 6) {
+        timing->bc_ms += ms;
+        timing->bc_launches += 1;
+      } else if (k % 2 == 0) {
+        timing->rhs_ms += ms;
+        timing->rhs_launches += 1;
+      } else {
+        timing->column_ms += ms;
+        timing->column_launches += 1;
+      }
+    }
+  }
+  for (auto& e : ev) {
+    hipEventDestroy(e.a);
+    hipEventDestroy(e.b);
+  }
+  return status;
 }
