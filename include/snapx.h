@@ -229,6 +229,41 @@ int snapx_fused_rk3_steps(const snapx_config* cfg, double dt, int nsteps,
                           float* u, void* workspace, size_t workspace_bytes,
                           snapx_timing* timing, void* stream);
 
+/* ---------------------------------------------------------------------- */
+/* Batched members (ensembles / blocks of uneven size; no reference        */
+/* counterpart: the reference runs one block per thread of                 */
+/* src/mesh/mesh.cpp:37-122)                                               */
+/* ---------------------------------------------------------------------- */
+
+/* Plan of a batch, filled by snapx_batch_plan (plain data). */
+typedef struct snapx_batch_info {
+  int nmembers;
+  int tile_x1, tile_x2, tile_x3; /* K1 tile shape shared by all members */
+  int total_tiles, total_cols;   /* work-list lengths of the persistent kernels */
+  int vic_size;                  /* 3 or 5 */
+  int64_t total_elems;           /* floats of the concatenated state */
+  int64_t scratch_doubles;
+  size_t rhs_smem;
+  double dt;
+} snapx_batch_info;
+
+/* Members are independent blocks, each with its own snapx_config (same
+   implicit scheme); their FP32 states are concatenated in member order, each
+   [5][nc3][nc2][nc1].  snapx_batch_plan writes the member table into the
+   workspace (synchronous copy) for a fixed dt and tile shape (0 = default);
+   snapx_batch_rk3_steps then advances every member nsteps SSP-RK3 steps with
+   one persistent launch per kernel and stage for the whole batch (it only
+   enqueues kernels, so a caller may capture it in a HIP graph when timing is
+   NULL). */
+size_t snapx_batch_workspace_bytes(const snapx_config* members, int nmembers);
+int snapx_batch_plan(const snapx_config* members, int nmembers, double dt,
+                     int tile_x1, int tile_x2, int tile_x3, void* workspace,
+                     size_t workspace_bytes, snapx_batch_info* info);
+int snapx_batch_rk3_steps(const snapx_config* members,
+                          const snapx_batch_info* info, int nsteps, float* u,
+                          void* workspace, size_t workspace_bytes,
+                          snapx_timing* timing, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

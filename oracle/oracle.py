@@ -97,6 +97,14 @@ def lib():
     return _lib
 
 
+def _check_field(c, a, nvar=None):
+    """Shape guard: the C oracle trusts its sizes, so every field passed with a
+    config must be [nvar, n3, n2, n1] of that config."""
+    assert a.ndim == 4 and tuple(a.shape[1:]) == tuple(shape(c)), (a.shape, shape(c))
+    if nvar is not None:
+        assert a.shape[0] == nvar, (a.shape, nvar)
+
+
 def _p(a):
     if a is None:
         return None
@@ -196,6 +204,7 @@ def interp_window(type, right, phi, scale=False):
 
 
 def cons2prim(c, u):
+    _check_field(c, u, 5)
     u = np.ascontiguousarray(u, dtype=np.float64).copy()
     w = np.empty_like(u)
     lib().oc_cons2prim(C.byref(c), _p(u), _p(w), u[0].size)
@@ -203,6 +212,7 @@ def cons2prim(c, u):
 
 
 def prim2cons(c, w):
+    _check_field(c, w, 5)
     w = np.ascontiguousarray(w, dtype=np.float64).copy()
     u = np.empty_like(w)
     lib().oc_prim2cons(C.byref(c), _p(w), _p(u), w[0].size)
@@ -232,6 +242,9 @@ def riemann(solver, wl, wr, dim, gammad=1.4):
 
 
 def divergence(c, f1, f2, f3):
+    for f in (f1, f2, f3):
+        if f is not None:
+            _check_field(c, f, 5)
     n3, n2, n1 = shape(c)
     d1, d2, d3 = cartesian_widths(c)
     div = np.zeros((5, n3, n2, n1))
@@ -243,6 +256,7 @@ def divergence(c, f1, f2, f3):
 def hydro_forward(c, dt, u, w=None):
     """Returns (du, w, (F1, F2, F3)); u and w are copied (u is limited in place
     by the reference only when the EOS limiter is on)."""
+    _check_field(c, u, 5)
     u = np.ascontiguousarray(u, dtype=np.float64).copy()
     w = np.empty_like(u) if w is None else np.ascontiguousarray(w, dtype=np.float64).copy()
     du = np.empty_like(u)
@@ -253,6 +267,7 @@ def hydro_forward(c, dt, u, w=None):
 
 
 def implicit_correction(c, dt, du, w):
+    _check_field(c, du, 5)
     du = np.ascontiguousarray(du, dtype=np.float64).copy()
     w = np.ascontiguousarray(w, dtype=np.float64)
     lib().oc_implicit_correction(C.byref(c), dt, _p(du), _p(w))
@@ -260,6 +275,7 @@ def implicit_correction(c, dt, du, w):
 
 
 def vic_column_matrices(c, dt, w, k, j):
+    _check_field(c, w, 5)
     N = 5 if (c.implicit_scheme >> 3) & 1 else 3
     nl = c.nx1
     a = np.zeros((nl, N, N))
@@ -278,17 +294,20 @@ def inverse3(m):
 
 
 def apply_bc(c, var, var_type=CONSERVED):
+    _check_field(c, var, None)
     var = np.ascontiguousarray(var, dtype=np.float64).copy()
     lib().oc_apply_bc(C.byref(c), _p(var), var.shape[0], var_type)
     return var
 
 
 def max_time_step(c, w, cfl):
+    _check_field(c, w, 5)
     w = np.ascontiguousarray(w, dtype=np.float64)
     return lib().oc_max_time_step(C.byref(c), _p(w), cfl)
 
 
 def initialize(c, w):
+    _check_field(c, w, 5)
     w = np.ascontiguousarray(w, dtype=np.float64).copy()
     u = np.empty_like(w)
     lib().oc_initialize(C.byref(c), _p(w), _p(u))
@@ -296,6 +315,7 @@ def initialize(c, w):
 
 
 def rk3_step(c, dt, u, w=None):
+    _check_field(c, u, 5)
     u = np.ascontiguousarray(u, dtype=np.float64).copy()
     w = np.empty_like(u) if w is None else np.ascontiguousarray(w, dtype=np.float64).copy()
     lib().oc_rk3_step(C.byref(c), dt, _p(u), _p(w))

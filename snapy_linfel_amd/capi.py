@@ -123,7 +123,18 @@ SIGNATURES = {
     "snapx_fused_workspace_bytes": (_Z, [_CFG]),
     "snapx_fused_rk3_step": (_I, [_CFG, _D, _P, _P, _Z, _P]),
     "snapx_fused_rk3_steps": (_I, [_CFG, _D, _I, _P, _P, _Z, _P, _P]),
+    "snapx_batch_workspace_bytes": (_Z, [_CFG, _I]),
+    "snapx_batch_plan": (_I, [_CFG, _I, _D, _I, _I, _I, _P, _Z, _P]),
+    "snapx_batch_rk3_steps": (_I, [_CFG, _P, _I, _P, _P, _Z, _P, _P]),
 }
+
+
+class BatchInfo(C.Structure):
+    """snapx_batch_info (include/snapx.h)."""
+    _fields_ = [("nmembers", C.c_int), ("tile_x1", C.c_int), ("tile_x2", C.c_int),
+                ("tile_x3", C.c_int), ("total_tiles", C.c_int), ("total_cols", C.c_int),
+                ("vic_size", C.c_int), ("total_elems", C.c_int64),
+                ("scratch_doubles", C.c_int64), ("rhs_smem", C.c_size_t), ("dt", C.c_double)]
 
 
 class Timing(C.Structure):
@@ -279,3 +290,43 @@ def fused_rk3_step(cfg, dt, u, ws=None):
     ws = fused_workspace(cfg, u.device) if ws is None else ws
     check(lib().snapx_fused_rk3_step(C.byref(cfg), dt, ptr(u), ptr(ws), ws.numel(),
                                      stream_ptr()), "fused_rk3_step")
+
+
+class Batch:
+    """A planned batch of independent members (configs may differ in size);
+    their FP32 states are concatenated in member order."""
+
+    def __init__(self, configs, dt, tile=(0, 0, 0), device="cuda"):
+        self.n = len(configs)
+        self.cfgs = (Config * self.n)(*configs)
+        nb = lib().snapx_batch_workspace_bytes(self.cfgs, self.n)
+        if nb == 0:
+            raise SnapxError("batch: " + lib().snapx_last_error().decode())
+        self.ws = workspace(nb, device)
+        self.info = BatchInfo()
+        check(lib().snapx_batch_plan(self.cfgs, self.n, dt, tile[0], tile[1], tile[2],
+                                     ptr(self.ws), self.ws.numel(), C.byref(self.info)),
+              "batch_plan")
+        self.offsets = []
+        off = 0
+        for c in configs:
+            self.offsets.append(off)
+            n = 1
+            for d in c.shape:
+                n *= d
+            off += n
+        self.total = off
+
+    def member(self, u, m):
+        """View of member m's [5, nc3, nc2, nc1] state inside the batch array."""
+        c = self.cfgs[m]
+        n = 1
+        for d in c.shape:
+            n *= d
+        return u[self.offsets[m]:self.offsets[m] + n].view(*c.shape)
+
+    def steps(self, u, nsteps, timing=None):
+        tp = C.byref(timing) if timing is not None else None
+        check(lib().snapx_batch_rk3_steps(self.cfgs, C.byref(self.info), nsteps, ptr(u),
+                                          ptr(self.ws), self.ws.numel(), tp, stream_ptr()),
+              "batch_rk3_steps")

@@ -25,6 +25,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <vector>
 
 #include "common.hpp"
 #include "interp_impl.hpp"
@@ -110,16 +111,16 @@ __device__ __forceinline__ void cons_to_prim(const T* U, T gammad, T* W) {
   W[IPR] = (gammad - T(1.)) * (U[IPR] - ke);
 }
 
-__global__ void __launch_bounds__(kRhsThreads)
-    fused_rhs_kernel(const float* __restrict__ u, float* __restrict__ du, FusedArgs a) {
-  extern __shared__ float smem[];
+// K1 body for one tile; every thread of the workgroup calls it with the same
+// tile (it synchronises the workgroup internally)
+__device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __restrict__ du,
+                                         const FusedArgs& a, int b, float* smem) {
   const int H1 = a.tx1 + 6, H2 = a.tx2 + 2 * a.h2, H3 = a.tx3 + 2 * a.h3;
   const int H = H1 * H2 * H3;
   float* ws = smem;          // primitives [5][H3][H2][H1]
   float* fs = smem + 5 * H;  // face fluxes of one direction [5][faces]
   const int ncell = a.n1 * a.n2 * a.n3;
 
-  int b = blockIdx.x;
   int t1 = b % a.nt1;
   int t2 = (b / a.nt1) % a.nt2;
   int t3 = b / (a.nt1 * a.nt2);
@@ -280,6 +281,12 @@ __global__ void __launch_bounds__(kRhsThreads)
   }
 }
 
+__global__ void __launch_bounds__(kRhsThreads)
+    fused_rhs_kernel(const float* __restrict__ u, float* __restrict__ du, FusedArgs a) {
+  extern __shared__ float smem[];
+  rhs_tile(u, du, a, blockIdx.x, smem);
+}
+
 struct ColArgs {
   int n1, n2, n3, ng;
   int nl, ncol, ncol2;
@@ -317,11 +324,9 @@ __device__ __forceinline__ void refill_ghost(const double* wb, double* wg, const
 }
 
 template <int N>
-__global__ void __launch_bounds__(kColThreads)
-    fused_column_kernel(float* __restrict__ u, float* __restrict__ u0,
-                        const float* __restrict__ du, double* __restrict__ scr, ColArgs a) {
-  int col = blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= a.ncol) return;
+__device__ __forceinline__ void column_work(float* __restrict__ u, float* __restrict__ u0,
+                                            const float* __restrict__ du,
+                                            double* __restrict__ scr, const ColArgs& a, int col) {
   const int ncell = a.n1 * a.n2 * a.n3;
   int j = a.n2 > 1 ? a.ng + col % a.ncol2 : 0;
   int k = a.n3 > 1 ? a.ng + col / a.ncol2 : 0;
@@ -504,6 +509,77 @@ __global__ void __launch_bounds__(kColThreads)
   }
 }
 
+template <int N>
+__global__ void __launch_bounds__(kColThreads)
+    fused_column_kernel(float* __restrict__ u, float* __restrict__ u0,
+                        const float* __restrict__ du, double* __restrict__ scr, ColArgs a) {
+  int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= a.ncol) return;
+  column_work<N>(u, u0, du, scr, a, col);
+}
+
+// ------------------------------------------------------- batched members ----
+
+// One member of a batch: its block geometry / options, where its fields start
+// in the concatenated arrays, and where its work items start in the
+// persistent kernels' work lists.
+struct BatchMember {
+  FusedArgs a;
+  ColArgs c;
+  int64_t off;      // element offset of the member's [5][n3][n2][n1] field
+  int64_t scr_off;  // double offset of the member's column scratch
+  int tile0;        // first K1 work item
+  int col0;         // first K2 work item (multiple of the wavefront size)
+};
+
+__device__ __forceinline__ int find_member(const BatchMember* m, int nm, int item, bool tiles) {
+  int lo = 0, hi = nm - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    int start = tiles ? m[mid].tile0 : m[mid].col0;
+    if (start <= item)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// persistent K1 over the (member, tile) work list
+__global__ void __launch_bounds__(kRhsThreads)
+    batch_rhs_kernel(const float* __restrict__ u, float* __restrict__ du,
+                     const BatchMember* __restrict__ mem, int nm, int total) {
+  extern __shared__ float smem[];
+  for (int item = blockIdx.x; item < total; item += gridDim.x) {
+    int m = find_member(mem, nm, item, true);
+    FusedArgs a = mem[m].a;
+    rhs_tile(u + mem[m].off, du + mem[m].off, a, item - mem[m].tile0, smem);
+    __syncthreads();
+  }
+}
+
+// persistent K2 over the (member, column) work list; member column ranges are
+// padded to whole wavefronts so that a wavefront serves one member
+template <int N>
+__global__ void __launch_bounds__(kColThreads)
+    batch_column_kernel(float* __restrict__ u, float* __restrict__ u0,
+                        const float* __restrict__ du, double* __restrict__ scr,
+                        const BatchMember* __restrict__ mem, int nm, int total, int stage) {
+  for (int item = blockIdx.x * blockDim.x + threadIdx.x; item < total;
+       item += gridDim.x * blockDim.x) {
+    int m = find_member(mem, nm, item, false);
+    ColArgs c = mem[m].c;
+    int col = item - mem[m].col0;
+    if (col >= c.ncol) continue;
+    static const double W[3][3] = {
+        {0., 1., 1.}, {3. / 4., 1. / 4., 1. / 4.}, {1. / 3., 2. / 3., 2. / 3.}};
+    c.stage = stage;
+    c.w0 = W[stage][0], c.w1 = W[stage][1], c.w2 = W[stage][2];
+    int64_t o = mem[m].off;
+    column_work<N>(u + o, u0 + o, du + o, scr + mem[m].scr_off, c, col);
+  }
+}
+
 // ----------------------------------------------------------- host side ----
 
 struct Tile {
@@ -563,23 +639,12 @@ static size_t scratch_bytes(const snapx_config* c) {
   return c->implicit_scheme ? sizeof(double) * (size_t)(N * N + N) * c->nx1 * col_count(c) : 0;
 }
 
-struct EventPair {
-  hipEvent_t a, b;
-};
-
-// One fused RK3 step; with ev != nullptr the 7 launches are bracketed by the
-// event pairs ev[0..6] (K1, K2 per stage, then the boundary kernel).
-static int fused_step(const snapx_config* cfg, double dt, float* u, void* workspace,
-                      hipStream_t st, EventPair* ev) {
+static int make_fused_args(const snapx_config* cfg, Tile t, double dt, FusedArgs* out) {
   Grid g = make_grid(cfg);
-  size_t field = align256(sizeof(float) * 5 * (size_t)g.ncell);
-  char* base = reinterpret_cast<char*>(workspace);
-  float* du = reinterpret_cast<float*>(base);
-  float* u0 = reinterpret_cast<float*>(base + field);
-  double* scr = reinterpret_cast<double*>(base + 2 * field);
-
-  Tile t = choose_tile(g);
   FusedArgs a;
+  t.tx1 = std::min(t.tx1, g.nx1);
+  t.tx2 = g.n2 > 1 ? std::min(t.tx2, g.nx2) : 1;
+  t.tx3 = g.n3 > 1 ? std::min(t.tx3, g.nx3) : 1;
   a.n1 = g.n1, a.n2 = g.n2, a.n3 = g.n3, a.ng = g.ng;
   a.nx1 = cfg->nx1, a.nx2 = cfg->nx2, a.nx3 = cfg->nx3;
   a.tx1 = t.tx1, a.tx2 = t.tx2, a.tx3 = t.tx3;
@@ -599,11 +664,16 @@ static int fused_step(const snapx_config* cfg, double dt, float* u, void* worksp
   a.dx1 = (float)g.dx1, a.dx2 = (float)g.dx2, a.dx3 = (float)g.dx3;
   a.dt = (float)dt;
   a.grav1 = (float)cfg->grav1, a.grav2 = (float)cfg->grav2, a.grav3 = (float)cfg->grav3;
+  SNAPX_CHECK_ARG(a.tx1 >= 1 && a.tx2 >= 1 && a.tx3 >= 1, "fused path: empty tile");
   SNAPX_CHECK_ARG((int64_t)a.tx1 * a.tx2 * a.tx3 <= kRhsThreads * kMaxCellsPerThread,
                   "fused path: tile too large");
-  size_t smem = rhs_smem_bytes(a);
-  int nblk = a.nt1 * a.nt2 * a.nt3;
+  SNAPX_CHECK_ARG(rhs_smem_bytes(a) <= 64 * 1024, "fused path: tile needs more than 64 KB LDS");
+  *out = a;
+  return 0;
+}
 
+static int make_col_args(const snapx_config* cfg, double dt, ColArgs* out) {
+  Grid g = make_grid(cfg);
   ColArgs c;
   c.n1 = g.n1, c.n2 = g.n2, c.n3 = g.n3, c.ng = g.ng;
   c.nl = cfg->nx1;
@@ -611,13 +681,42 @@ static int fused_step(const snapx_config* cfg, double dt, float* u, void* worksp
   c.ncol = (int)col_count(cfg);
   c.bc_in = cfg->bc[0], c.bc_out = cfg->bc[1];
   c.scheme = cfg->implicit_scheme;
+  c.stage = 0;
   c.gammad = cfg->gammad;
   c.dx1 = g.dx1;
   c.area = g.dx3 * g.dx2;
   c.vol = g.dx3 * (g.dx2 * g.dx1);
   c.dt = dt;
   c.grav1 = cfg->grav1;
-  SNAPX_CHECK_ARG(!c.scheme || c.grav1 != 0., "[ImplicitHydro] forcing does not have const-gravity");
+  c.w0 = 0., c.w1 = 1., c.w2 = 1.;
+  SNAPX_CHECK_ARG(!c.scheme || c.grav1 != 0.,
+                  "[ImplicitHydro] forcing does not have const-gravity");
+  *out = c;
+  return 0;
+}
+
+struct EventPair {
+  hipEvent_t a, b;
+};
+
+// One fused RK3 step; with ev != nullptr the 7 launches are bracketed by the
+// event pairs ev[0..6] (K1, K2 per stage, then the boundary kernel).
+static int fused_step(const snapx_config* cfg, double dt, float* u, void* workspace,
+                      hipStream_t st, EventPair* ev, bool bc = true) {
+  Grid g = make_grid(cfg);
+  size_t field = align256(sizeof(float) * 5 * (size_t)g.ncell);
+  char* base = reinterpret_cast<char*>(workspace);
+  float* du = reinterpret_cast<float*>(base);
+  float* u0 = reinterpret_cast<float*>(base + field);
+  double* scr = reinterpret_cast<double*>(base + 2 * field);
+
+  Tile t = choose_tile(g);
+  FusedArgs a;
+  if (int e = make_fused_args(cfg, t, dt, &a)) return e;
+  size_t smem = rhs_smem_bytes(a);
+  int nblk = a.nt1 * a.nt2 * a.nt3;
+  ColArgs c;
+  if (int e = make_col_args(cfg, dt, &c)) return e;
   int N = ((cfg->implicit_scheme >> 3) & 1) ? 5 : 3;
   int cblk = (c.ncol + kColThreads - 1) / kColThreads;
 
@@ -641,7 +740,8 @@ static int fused_step(const snapx_config* cfg, double dt, float* u, void* worksp
     if (ev) SNAPX_HIP(hipEventRecord(ev[2 * s + 1].b, st));
   }
   if (ev) SNAPX_HIP(hipEventRecord(ev[6].a, st));
-  if (int e = launch_bc_f32(cfg, u, st)) return e;
+  if (bc)
+    if (int e = launch_bc_f32(cfg, u, st)) return e;
   if (ev) SNAPX_HIP(hipEventRecord(ev[6].b, st));
   return 0;
 }
@@ -675,8 +775,10 @@ extern "C" int snapx_fused_rk3_steps(const snapx_config* cfg, double dt, int nst
                   "snapx_fused_rk3_steps: workspace too small");
   hipStream_t st = as_stream(stream);
   if (!timing) {
+    // the fused kernels never read ghost cells, so the stored ghosts are
+    // refreshed once, after the last step
     for (int n = 0; n < nsteps; ++n)
-      if (int e = fused_step(cfg, dt, u, workspace, st, nullptr)) return e;
+      if (int e = fused_step(cfg, dt, u, workspace, st, nullptr, n == nsteps - 1)) return e;
     return 0;
   }
   EventPair ev[7];
@@ -713,4 +815,149 @@ extern "C" int snapx_fused_rk3_steps(const snapx_config* cfg, double dt, int nst
     hipEventDestroy(e.b);
   }
   return status;
+}
+
+// ---------------------------------------------------- batched C ABI ----
+
+namespace snapx {
+
+static size_t batch_table_bytes(int nm) { return align256(sizeof(BatchMember) * (size_t)nm); }
+
+static int batch_layout(const snapx_config* members, int nm, double dt, int tx1, int tx2, int tx3,
+                        std::vector<BatchMember>* table, snapx_batch_info* info) {
+  SNAPX_CHECK_ARG(members && nm >= 1, "batch: no members");
+  int scheme = members[0].implicit_scheme;
+  bool three_d = false, two_d = false;
+  for (int m = 0; m < nm; ++m) {
+    if (int e = check_fused(&members[m])) return e;
+    SNAPX_CHECK_ARG(members[m].implicit_scheme == scheme,
+                    "batch: every member needs the same implicit scheme");
+    three_d |= members[m].nx3 > 1;
+    two_d |= members[m].nx2 > 1;
+  }
+  Tile def = three_d ? Tile{16, 4, 4} : (two_d ? Tile{16, 4, 1} : Tile{256, 1, 1});
+  Tile t = {tx1 > 0 ? tx1 : def.tx1, tx2 > 0 ? tx2 : def.tx2, tx3 > 0 ? tx3 : def.tx3};
+  table->resize(nm);
+  int64_t off = 0, soff = 0;
+  int tiles = 0, cols = 0;
+  size_t smem = 0;
+  for (int m = 0; m < nm; ++m) {
+    BatchMember& b = (*table)[m];
+    if (int e = make_fused_args(&members[m], t, dt, &b.a)) return e;
+    if (int e = make_col_args(&members[m], dt, &b.c)) return e;
+    smem = std::max(smem, rhs_smem_bytes(b.a));
+    b.off = off;
+    b.scr_off = soff;
+    b.tile0 = tiles;
+    b.col0 = cols;
+    Grid g = make_grid(&members[m]);
+    off += 5 * (int64_t)g.ncell;
+    soff += (int64_t)scratch_bytes(&members[m]) / sizeof(double);
+    tiles += b.a.nt1 * b.a.nt2 * b.a.nt3;
+    cols += (b.c.ncol + kColThreads - 1) / kColThreads * kColThreads;
+  }
+  info->nmembers = nm;
+  info->tile_x1 = t.tx1, info->tile_x2 = t.tx2, info->tile_x3 = t.tx3;
+  info->total_tiles = tiles;
+  info->total_cols = cols;
+  info->total_elems = off;
+  info->scratch_doubles = soff;
+  info->rhs_smem = smem;
+  info->vic_size = ((scheme >> 3) & 1) ? 5 : 3;
+  info->dt = dt;
+  return 0;
+}
+
+}  // namespace snapx
+
+extern "C" size_t snapx_batch_workspace_bytes(const snapx_config* members, int nmembers) {
+  std::vector<BatchMember> table;
+  snapx_batch_info info;
+  if (batch_layout(members, nmembers, 1.0, 0, 0, 0, &table, &info)) return 0;
+  size_t field = align256(sizeof(float) * (size_t)info.total_elems);
+  return batch_table_bytes(nmembers) + 2 * field + align256(sizeof(double) * info.scratch_doubles);
+}
+
+extern "C" int snapx_batch_plan(const snapx_config* members, int nmembers, double dt,
+                                int tile_x1, int tile_x2, int tile_x3, void* workspace,
+                                size_t workspace_bytes, snapx_batch_info* info) {
+  SNAPX_CHECK_ARG(workspace && info, "snapx_batch_plan: null pointer");
+  std::vector<BatchMember> table;
+  if (int e = batch_layout(members, nmembers, dt, tile_x1, tile_x2, tile_x3, &table, info))
+    return e;
+  SNAPX_CHECK_ARG(workspace_bytes >= snapx_batch_workspace_bytes(members, nmembers),
+                  "snapx_batch_plan: workspace too small");
+  SNAPX_HIP(hipMemcpy(workspace, table.data(), sizeof(BatchMember) * table.size(),
+                      hipMemcpyHostToDevice));
+  return 0;
+}
+
+extern "C" int snapx_batch_rk3_steps(const snapx_config* members, const snapx_batch_info* info,
+                                     int nsteps, float* u, void* workspace,
+                                     size_t workspace_bytes, snapx_timing* timing, void* stream) {
+  SNAPX_CHECK_ARG(members && info && u && workspace && nsteps >= 0,
+                  "snapx_batch_rk3_steps: bad arguments");
+  SNAPX_CHECK_ARG(workspace_bytes >= snapx_batch_workspace_bytes(members, info->nmembers),
+                  "snapx_batch_rk3_steps: workspace too small");
+  hipStream_t st = as_stream(stream);
+  int nm = info->nmembers;
+  char* base = reinterpret_cast<char*>(workspace);
+  const BatchMember* table = reinterpret_cast<const BatchMember*>(base);
+  size_t field = align256(sizeof(float) * (size_t)info->total_elems);
+  float* du = reinterpret_cast<float*>(base + batch_table_bytes(nm));
+  float* u0 = reinterpret_cast<float*>(base + batch_table_bytes(nm) + field);
+  double* scr = reinterpret_cast<double*>(base + batch_table_bytes(nm) + 2 * field);
+  // persistent grids: two workgroups per CU for K1, enough wavefronts for K2
+  int rhs_grid = std::min(info->total_tiles, 512);
+  int col_grid = std::min((info->total_cols + kColThreads - 1) / kColThreads, 2048);
+  EventPair ev[2];
+  if (timing)
+    for (auto& e : ev) {
+      SNAPX_HIP(hipEventCreate(&e.a));
+      SNAPX_HIP(hipEventCreate(&e.b));
+    }
+  for (int n = 0; n < nsteps; ++n) {
+    for (int s = 0; s < 3; ++s) {
+      if (timing) SNAPX_HIP(hipEventRecord(ev[0].a, st));
+      hipLaunchKernelGGL(batch_rhs_kernel, dim3(rhs_grid), dim3(kRhsThreads), info->rhs_smem, st,
+                         u, du, table, nm, info->total_tiles);
+      SNAPX_LAUNCH_CHECK();
+      if (timing) {
+        SNAPX_HIP(hipEventRecord(ev[0].b, st));
+        SNAPX_HIP(hipEventRecord(ev[1].a, st));
+      }
+      if (info->vic_size == 3)
+        hipLaunchKernelGGL(batch_column_kernel<3>, dim3(col_grid), dim3(kColThreads), 0, st, u,
+                           u0, du, scr, table, nm, info->total_cols, s);
+      else
+        hipLaunchKernelGGL(batch_column_kernel<5>, dim3(col_grid), dim3(kColThreads), 0, st, u,
+                           u0, du, scr, table, nm, info->total_cols, s);
+      SNAPX_LAUNCH_CHECK();
+      if (timing) {
+        SNAPX_HIP(hipEventRecord(ev[1].b, st));
+        SNAPX_HIP(hipEventSynchronize(ev[1].b));
+        float a = 0.f, b = 0.f;
+        hipEventElapsedTime(&a, ev[0].a, ev[0].b);
+        hipEventElapsedTime(&b, ev[1].a, ev[1].b);
+        timing->rhs_ms += a;
+        timing->column_ms += b;
+        timing->rhs_launches += 1;
+        timing->column_launches += 1;
+      }
+    }
+  }
+  if (timing)
+    for (auto& e : ev) {
+      hipEventDestroy(e.a);
+      hipEventDestroy(e.b);
+    }
+  // refresh the stored ghosts of every member once
+  if (nsteps > 0) {
+    int64_t off = 0;
+    for (int m = 0; m < nm; ++m) {
+      if (int e = launch_bc_f32(&members[m], u + off, st)) return e;
+      off += 5 * (int64_t)make_grid(&members[m]).ncell;
+    }
+  }
+  return 0;
 }

@@ -1,0 +1,59 @@
+"""GPU: batched members of uneven size in one persistent launch per kernel
+reproduce the single-block fused path exactly, and the oracle to the FP32
+bar."""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from snapy_linfel_amd import capi
+from test_gpu_plain import cfg_of, host, to_dev, var_errors
+
+pytestmark = pytest.mark.gpu
+
+
+def _members(seed=7, n=5, riemann="hllc"):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        nx2 = int(rng.choice([16, 24, 40]))
+        nx1 = int(rng.choice([8, 12, 20]))
+        oc = O.straka_config(nx2=nx2, nx1=nx1, implicit_scheme=1, riemann=riemann)
+        oc.x2max = 50.0 * nx2
+        oc.x1max = 50.0 * nx1
+        w = O.straka_init(oc, xc=float(rng.uniform(0, 50.0 * nx2)), xr=float(rng.uniform(250, 600)),
+                          zc=25.0 * nx1, zr=float(rng.uniform(150, 300)), dT=0.5)
+        w, u = O.initialize(oc, w)
+        out.append((oc, u))
+    return out
+
+
+def test_batch_matches_single_block_fused_exactly():
+    mem = _members()
+    cfgs = [cfg_of(oc) for oc, _ in mem]
+    b = capi.Batch(cfgs, 0.25)
+    u = torch.cat([to_dev(u0, torch.float32).reshape(-1) for _, u0 in mem])
+    b.steps(u, 3)
+    for m, (oc, u0) in enumerate(mem):
+        single = to_dev(u0, torch.float32)
+        capi.fused_rk3_steps(cfgs[m], 0.25, 3, single)
+        assert torch.equal(b.member(u, m), single), m
+
+
+def test_batch_vs_oracle_one_step():
+    mem = _members(seed=11, n=4)
+    cfgs = [cfg_of(oc) for oc, _ in mem]
+    b = capi.Batch(cfgs, 0.25, tile=(8, 4, 1))
+    u = torch.cat([to_dev(u0, torch.float32).reshape(-1) for _, u0 in mem])
+    b.steps(u, 1)
+    for m, (oc, u0) in enumerate(mem):
+        ref, _ = O.rk3_step(oc, 0.25, u0)
+        e = var_errors(host(b.member(u, m)), ref, O.interior(oc))
+        assert (e < 1e-5).all(), (m, e)
+
+
+def test_batch_rejects_mixed_schemes():
+    a, _ = _members(n=1)[0]
+    b = O.straka_config(nx2=16, nx1=8, implicit_scheme=9)
+    with pytest.raises(capi.SnapxError, match="same implicit scheme"):
+        capi.Batch([cfg_of(a), cfg_of(b)], 0.25)

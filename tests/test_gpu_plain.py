@@ -133,7 +133,7 @@ def test_boundary_functions_exact():
     oc = O.make_config(nx1=7, nx2=6, nx3=5, nghost=3,
                        bc=("reflecting", "outflow", "periodic", "periodic", "outflow",
                            "reflecting"))
-    v = np.random.default_rng(4).normal(size=O.shape(oc))
+    v = np.random.default_rng(4).normal(size=(5,) + O.shape(oc))
     ref = O.apply_bc(oc, v, O.CONSERVED)
     d = to_dev(v)
     capi.apply_bc(cfg_of(oc), d, capi.CONSERVED)
