@@ -152,7 +152,7 @@ def main():
 
     got = u32.double().cpu().numpy()
     ref = u64.cpu().numpy()
-    err = errors(got, ref, (slice(None),) + I)
+    err = errors(got, ref, I)
     m0 = u0[0][I].sum()
     mass_drift = float(abs(got[0][I].sum() - m0) / m0)
     mass_drift64 = float(abs(ref[0][I].sum() - m0) / m0)

@@ -230,6 +230,21 @@ int snapx_fused_rk3_steps(const snapx_config* cfg, double dt, int nsteps,
                           snapx_timing* timing, void* stream);
 
 /* ---------------------------------------------------------------------- */
+/* Adjoint (no reference counterpart)                                      */
+/* ---------------------------------------------------------------------- */
+
+/* Discrete adjoint of one FP64 plain-path SSP-RK3 step
+   (snapx_rk3_step): on entry lambda (double [5][nc3][nc2][nc1]) holds
+   dL/du_out; on exit it holds dL/du_in for the step input u_in.  The
+   forward stages are recomputed from u_in.  Requires limiter off,
+   non-hydrostatic = 1 and no disable-flux. */
+size_t snapx_adjoint_workspace_bytes(const snapx_config* cfg);
+int snapx_adjoint_rk3_step(const snapx_config* cfg, double dt,
+                           const double* u_in, double* lambda,
+                           void* workspace, size_t workspace_bytes,
+                           void* stream);
+
+/* ---------------------------------------------------------------------- */
 /* Batched members (ensembles / blocks of uneven size; no reference        */
 /* counterpart: the reference runs one block per thread of                 */
 /* src/mesh/mesh.cpp:37-122)                                               */

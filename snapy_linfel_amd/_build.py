@@ -34,6 +34,7 @@ HIP_UNITS = {
     "plain.hip": ["-ffp-contract=off"],
     "vic.hip": ["-ffp-contract=off"],
     "fused.hip": [],
+    "adjoint.hip": ["-ffp-contract=off"],
 }
 
 HIP_FLAGS = [

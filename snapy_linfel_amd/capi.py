@@ -123,6 +123,8 @@ SIGNATURES = {
     "snapx_fused_workspace_bytes": (_Z, [_CFG]),
     "snapx_fused_rk3_step": (_I, [_CFG, _D, _P, _P, _Z, _P]),
     "snapx_fused_rk3_steps": (_I, [_CFG, _D, _I, _P, _P, _Z, _P, _P]),
+    "snapx_adjoint_workspace_bytes": (_Z, [_CFG]),
+    "snapx_adjoint_rk3_step": (_I, [_CFG, _D, _P, _P, _P, _Z, _P]),
     "snapx_batch_workspace_bytes": (_Z, [_CFG, _I]),
     "snapx_batch_plan": (_I, [_CFG, _I, _D, _I, _I, _I, _P, _Z, _P]),
     "snapx_batch_rk3_steps": (_I, [_CFG, _P, _I, _P, _P, _Z, _P, _P]),
@@ -273,6 +275,17 @@ def rk3_step(cfg, dt, u, w, ws=None):
 def max_time_step(cfg, w, cfl, out):
     check(lib().snapx_max_time_step(C.byref(cfg), ptr(w), cfl, ptr(out), dtype_code(w),
                                     stream_ptr()), "max_time_step")
+
+
+def adjoint_workspace(cfg, device="cuda"):
+    return workspace(lib().snapx_adjoint_workspace_bytes(C.byref(cfg)), device)
+
+
+def adjoint_rk3_step(cfg, dt, u_in, lam, ws=None):
+    """lam <- (d u_out / d u_in)^T lam for one FP64 plain-path RK3 step."""
+    ws = adjoint_workspace(cfg, u_in.device) if ws is None else ws
+    check(lib().snapx_adjoint_rk3_step(C.byref(cfg), dt, ptr(u_in), ptr(lam), ptr(ws),
+                                       ws.numel(), stream_ptr()), "adjoint_rk3_step")
 
 
 def fused_workspace(cfg, device="cuda"):

@@ -1,0 +1,784 @@
+// Hand-written discrete adjoint of one SSP-RK3 step of the FP64 plain path.
+//
+// Reverse of u_{s+1} = BC(a u0 + b u_s + c du(u_s)), stage by stage, with the
+// forward stage inputs recomputed from the step input.  du(u_s) is
+//   VIC( -dt D(F(recon(w))) + gravity(w) , w ),  w = revise_ghost(cons2prim(u_s)).
+// Every local Jacobian (interpolant window, Riemann face, VIC block row,
+// ghost refill, cons2prim) is the exact derivative of the forward device code,
+// obtained by evaluating that same templated code on Dual<K> numbers; the
+// reverse sweep applies them transposed:
+//   boundaries   ghost adjoints fold into their source cells
+//   VIC          transposed redistribution, transposed block-tridiagonal
+//                solve (block Thomas on a^T, c^T, b^T), and the block-row
+//                derivatives contracted with -lambda_r delta^T
+//   divergence   lambda_F = (-dt A / V)(lambda_du[left] - lambda_du[right])
+//   Riemann      J^T lambda_F per face (Dual<10>)
+//   recon        gathered per cell over the faces whose window holds it
+// All arithmetic is double.  Deterministic: no atomics.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "common.hpp"
+#include "interp_impl.hpp"
+#include "riemann_impl.hpp"
+#include "vic_impl.hpp"
+// dual.hpp last: its overloads must not hide the double ones at the template
+// definitions above
+#include "dual.hpp"
+
+namespace snapx {
+
+constexpr int kT = 128;
+
+struct AdjArgs {
+  int n1, n2, n3, ng;
+  int nx1, nx2, nx3;
+  double dx1, dx2, dx3, dt, gammad, grav1, grav2, grav3, nh;
+  int recon_type[2], recon_scale[2], recon_shock[2];
+  int riemann[3];
+  int bc[6];
+  int scheme;
+};
+
+__device__ __forceinline__ void decode(const AdjArgs& a, int q, int& i, int& j, int& k) {
+  i = q % a.n1;
+  j = (q / a.n1) % a.n2;
+  k = q / (a.n1 * a.n2);
+}
+
+__device__ __forceinline__ bool interior(const AdjArgs& a, int i, int j, int k) {
+  return (a.n1 == 1 || (i >= a.ng && i < a.n1 - a.ng)) &&
+         (a.n2 == 1 || (j >= a.ng && j < a.n2 - a.ng)) &&
+         (a.n3 == 1 || (k >= a.ng && k < a.n3 - a.ng));
+}
+
+// ------------------------------------------------ boundary-function adjoint
+
+// reverse of one axis of bc_kernel: one thread per (variable, pencil) folds
+// the ghost adjoints into their source cells, then zeroes the ghosts
+__global__ void bc_adj_kernel(double* __restrict__ lam, AdjArgs a, int dim, int tin, int tout,
+                              int flip) {
+  int s, n, np;
+  if (dim == 3) {
+    s = 1, n = a.n1, np = a.n2 * a.n3;
+  } else if (dim == 2) {
+    s = a.n1, n = a.n2, np = a.n1 * a.n3;
+  } else {
+    s = a.n1 * a.n2, n = a.n3, np = a.n1 * a.n2;
+  }
+  int ncell = a.n1 * a.n2 * a.n3;
+  int ng = a.ng;
+  int vnorm = 4 - dim;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < 5 * np; t += gridDim.x * blockDim.x) {
+    int p = t % np, v = t / np;
+    int base = dim == 3 ? p * a.n1 : (dim == 2 ? (p / a.n1) * a.n1 * a.n2 + p % a.n1 : p);
+    double* x = lam + v * ncell + base;
+    for (int side = 0; side < 2; ++side) {
+      int type = side ? tout : tin;
+      if (type < 0) continue;
+      for (int g = 0; g < ng; ++g) {
+        int dst, src;
+        if (!side) {
+          dst = g;
+          src = type == SNAPX_BC_REFLECTING ? 2 * ng - 1 - g
+                : type == SNAPX_BC_OUTFLOW  ? ng
+                                            : n - 2 * ng + g;
+        } else {
+          dst = n - ng + g;
+          src = type == SNAPX_BC_REFLECTING ? n - ng - 1 - g
+                : type == SNAPX_BC_OUTFLOW  ? n - ng - 1
+                                            : ng + g;
+        }
+        double sign = (type == SNAPX_BC_REFLECTING && v == vnorm && flip) ? -1. : 1.;
+        x[src * s] += sign * x[dst * s];
+        x[dst * s] = 0.;
+      }
+    }
+  }
+}
+
+// --------------------------------------------------------- stage combine
+
+__global__ void combine_adj_kernel(const double* __restrict__ lam, double* __restrict__ lam_du,
+                                   double* __restrict__ lam_us, double* __restrict__ lam_u0,
+                                   double wa, double wb, double wc, int64_t n) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    double l = lam[q];
+    lam_du[q] = wc * l;
+    lam_us[q] = wb * l;
+    lam_u0[q] += wa * l;
+  }
+}
+
+// ------------------------------------------------------------------- VIC
+
+__device__ __forceinline__ int col_q0(const AdjArgs& a, int col) {
+  int ncol2 = a.n2 > 1 ? a.nx2 : 1;
+  int j = a.n2 > 1 ? a.ng + col % ncol2 : 0;
+  int k = a.n3 > 1 ? a.ng + col / ncol2 : 0;
+  return (k * a.n2 + j) * a.n1 + a.ng;
+}
+
+template <typename T>
+__device__ __forceinline__ void load_w(const double* w, int ncell, int q, T* out) {
+#pragma unroll
+  for (int v = 0; v < 5; ++v) out[v] = T(w[v * ncell + q]);
+}
+
+// One thread per column: forward assembly + block Thomas (delta), MS-VIC
+// redistribution adjoint, transposed block solve.  Writes
+//   lam_w   (rho part of the redistribution)  += ...
+//   lam_dx  (adjoint of the explicit tendency) = ...
+//   dl      [col][layer][2N]: delta and lambda_r for the row-derivative kernel
+template <int N>
+__global__ void __launch_bounds__(64)
+    vic_adj_column_kernel(const double* __restrict__ w, const double* __restrict__ dux,
+                          const double* __restrict__ lam_du, double* __restrict__ lam_dx,
+                          double* __restrict__ lam_w, double* __restrict__ abc,
+                          double* __restrict__ pr, double* __restrict__ dl, AdjArgs a, int ncol) {
+  const int NN = N * N;
+  const int ncell = a.n1 * a.n2 * a.n3;
+  const int nl = a.nx1;
+  const double area = a.dx3 * a.dx2, vol = a.dx3 * (a.dx2 * a.dx1);
+  for (int col = blockIdx.x * blockDim.x + threadIdx.x; col < ncol;
+       col += gridDim.x * blockDim.x) {
+    int q0 = col_q0(a, col);
+    double* cabc = abc + (int64_t)col * nl * 3 * NN;
+    double* cpr = pr + (int64_t)col * nl * (NN + N);
+    double* cdl = dl + (int64_t)col * nl * 2 * N;
+    // ---- assembly of every block row ----
+    for (int i = 0; i < nl; ++i) {
+      double wm[5], w0[5], wp[5];
+      load_w(w, ncell, q0 + i - 1, wm);
+      load_w(w, ncell, q0 + i, w0);
+      load_w(w, ncell, q0 + i + 1, wp);
+      vic_assemble_cell<double, N>(wm, w0, wp, a.gammad, area, area, vol, a.dt, a.grav1, i == 0,
+                                   i == nl - 1, cabc + i * 3 * NN, cabc + i * 3 * NN + NN,
+                                   cabc + i * 3 * NN + 2 * NN);
+    }
+    // ---- forward sweep + back substitution -> delta ----
+    double aprev[NN], dprev[N];
+    for (int i = 0; i < nl; ++i) {
+      int q = q0 + i;
+      double rhs[N];
+      rhs[0] = dux[IDN * ncell + q] / a.dt;
+      rhs[1] = dux[IVX * ncell + q] / a.dt;
+      if (N == 3) {
+        rhs[2] = dux[IPR * ncell + q] / a.dt;
+      } else {
+        rhs[2] = dux[IVY * ncell + q] / a.dt;
+        rhs[3] = dux[IVZ * ncell + q] / a.dt;
+        rhs[4] = dux[IPR * ncell + q] / a.dt;
+      }
+      double A[NN], B[NN], C[NN];
+      for (int e = 0; e < NN; ++e) {
+        A[e] = cabc[i * 3 * NN + e];
+        B[e] = cabc[i * 3 * NN + NN + e];
+        C[e] = cabc[i * 3 * NN + 2 * NN + e];
+      }
+      if (i > 0) {
+        double tmp[NN], v[N];
+        matmul<double, N>(B, aprev, tmp);
+        for (int e = 0; e < NN; ++e) A[e] -= tmp[e];
+        matvec<double, N>(B, dprev, v);
+        for (int e = 0; e < N; ++e) rhs[e] -= v[e];
+      }
+      if (N == 3) {
+        double inv[9];
+        inverse3<double>(A, inv);
+        matvec<double, 3>(inv, rhs, dprev);
+        matmul<double, 3>(inv, C, aprev);
+      } else {
+        int indx[5];
+        ludcmp5<double>(A, indx);
+        double solved[30];
+        for (int r = 0; r < 5; ++r) {
+          for (int c = 0; c < 5; ++c) solved[r * 6 + c] = C[r * 5 + c];
+          solved[r * 6 + 5] = rhs[r];
+        }
+        lubksb5<double, 6>(A, indx, solved);
+        for (int r = 0; r < 5; ++r) {
+          for (int c = 0; c < 5; ++c) aprev[r * 5 + c] = solved[r * 6 + c];
+          dprev[r] = solved[r * 6 + 5];
+        }
+      }
+      for (int e = 0; e < NN; ++e) cpr[i * (NN + N) + e] = aprev[e];
+      for (int e = 0; e < N; ++e) cpr[i * (NN + N) + NN + e] = dprev[e];
+    }
+    for (int e = 0; e < N; ++e) cdl[(nl - 1) * 2 * N + e] = dprev[e];
+    for (int i = nl - 2; i >= 0; --i) {
+      double v[N];
+      matvec<double, N>(cpr + i * (NN + N), dprev, v);
+      for (int e = 0; e < N; ++e) {
+        dprev[e] = cpr[i * (NN + N) + NN + e] - v[e];
+        cdl[i * 2 * N + e] = dprev[e];
+      }
+    }
+    // ---- redistribution adjoint ----
+    double S = 0., Bd = 0., G = 0.;
+    for (int i = 0; i < nl; ++i) {
+      int q = q0 + i;
+      double rho = w[IDN * ncell + q];
+      double ai = rho * vol;
+      S += ai * ai;
+      Bd += (dux[IDN * ncell + q] - cdl[i * 2 * N]) * 1.0 * vol;
+      G += lam_du[IDN * ncell + q] * rho * ai;
+    }
+    G /= S;
+    double lamS = -G * Bd / S;
+    // lambda_delta into the second half of dl (temporarily), direct lam_dx
+    for (int i = 0; i < nl; ++i) {
+      int q = q0 + i;
+      double rho = w[IDN * ncell + q];
+      double ln[5];
+      for (int v = 0; v < 5; ++v) ln[v] = lam_du[v * ncell + q];
+      double ld[N];
+      ld[0] = ln[IDN] - G * vol;
+      ld[1] = ln[IVX];
+      if (N == 3) {
+        ld[2] = ln[IPR];
+        lam_dx[IVY * ncell + q] = ln[IVY];
+        lam_dx[IVZ * ncell + q] = ln[IVZ];
+      } else {
+        ld[2] = ln[IVY];
+        ld[3] = ln[IVZ];
+        ld[4] = ln[IPR];
+      }
+      lam_dx[IDN * ncell + q] = G * vol;
+      lam_w[IDN * ncell + q] += ln[IDN] * 2. * rho * vol * Bd / S + lamS * 2. * rho * vol * vol;
+      for (int e = 0; e < N; ++e) cdl[i * 2 * N + N + e] = ld[e];
+    }
+    // ---- transposed block solve  M^T y = lambda_delta ----
+    // rows of M^T: diagonal a_i^T, lower c_{i-1}^T, upper b_{i+1}^T
+    double yprev[N];
+    for (int i = 0; i < nl; ++i) {
+      double A[NN], Bt[NN], Ct[NN], rhs[N];
+      const double* ai = cabc + i * 3 * NN;
+      for (int r = 0; r < N; ++r)
+        for (int c = 0; c < N; ++c) {
+          A[r * N + c] = ai[c * N + r];
+          Bt[r * N + c] = i > 0 ? cabc[(i - 1) * 3 * NN + 2 * NN + c * N + r] : 0.;
+          Ct[r * N + c] = i < nl - 1 ? cabc[(i + 1) * 3 * NN + NN + c * N + r] : 0.;
+        }
+      for (int e = 0; e < N; ++e) rhs[e] = cdl[i * 2 * N + N + e];
+      if (i > 0) {
+        double tmp[NN], v[N];
+        matmul<double, N>(Bt, aprev, tmp);
+        for (int e = 0; e < NN; ++e) A[e] -= tmp[e];
+        matvec<double, N>(Bt, yprev, v);
+        for (int e = 0; e < N; ++e) rhs[e] -= v[e];
+      }
+      if (N == 3) {
+        double inv[9];
+        inverse3<double>(A, inv);
+        matvec<double, 3>(inv, rhs, yprev);
+        matmul<double, 3>(inv, Ct, aprev);
+      } else {
+        int indx[5];
+        ludcmp5<double>(A, indx);
+        double solved[30];
+        for (int r = 0; r < 5; ++r) {
+          for (int c = 0; c < 5; ++c) solved[r * 6 + c] = Ct[r * 5 + c];
+          solved[r * 6 + 5] = rhs[r];
+        }
+        lubksb5<double, 6>(A, indx, solved);
+        for (int r = 0; r < 5; ++r) {
+          for (int c = 0; c < 5; ++c) aprev[r * 5 + c] = solved[r * 6 + c];
+          yprev[r] = solved[r * 6 + 5];
+        }
+      }
+      for (int e = 0; e < NN; ++e) cpr[i * (NN + N) + e] = aprev[e];
+      for (int e = 0; e < N; ++e) cpr[i * (NN + N) + NN + e] = yprev[e];
+    }
+    for (int e = 0; e < N; ++e) cdl[(nl - 1) * 2 * N + N + e] = yprev[e];
+    for (int i = nl - 2; i >= 0; --i) {
+      double v[N];
+      matvec<double, N>(cpr + i * (NN + N), yprev, v);
+      for (int e = 0; e < N; ++e) {
+        yprev[e] = cpr[i * (NN + N) + NN + e] - v[e];
+        cdl[i * 2 * N + N + e] = yprev[e];
+      }
+    }
+    // lambda of the explicit tendency through r = du / dt
+    for (int i = 0; i < nl; ++i) {
+      int q = q0 + i;
+      const double* y = cdl + i * 2 * N + N;
+      lam_dx[IDN * ncell + q] += y[0] / a.dt;
+      lam_dx[IVX * ncell + q] = y[1] / a.dt;
+      if (N == 3) {
+        lam_dx[IPR * ncell + q] = y[2] / a.dt;
+      } else {
+        lam_dx[IVY * ncell + q] = y[2] / a.dt;
+        lam_dx[IVZ * ncell + q] = y[3] / a.dt;
+        lam_dx[IPR * ncell + q] = y[4] / a.dt;
+      }
+    }
+  }
+}
+
+// d(a_i delta_i + b_i delta_{i-1} + c_i delta_{i+1}) / d w_j contracted with
+// -lambda_r_i, summed over the rows i = j-1, j, j+1 that hold cell j; one
+// thread per (column, layer j), j in [-1, nl] (the ghost layers enter rows 0
+// and nl-1)
+template <int N>
+__global__ void __launch_bounds__(kT)
+    vic_adj_rows_kernel(const double* __restrict__ w, const double* __restrict__ dl,
+                        double* __restrict__ lam_w, AdjArgs a, int ncol) {
+  const int NN = N * N;
+  const int ncell = a.n1 * a.n2 * a.n3;
+  const int nl = a.nx1;
+  const double area = a.dx3 * a.dx2, vol = a.dx3 * (a.dx2 * a.dx1);
+  int64_t total = (int64_t)ncol * (nl + 2);
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    int col = (int)(t / (nl + 2));
+    int j = (int)(t - (int64_t)col * (nl + 2)) - 1;
+    int q0 = col_q0(a, col);
+    const double* cdl = dl + (int64_t)col * nl * 2 * N;
+    double acc[5] = {0., 0., 0., 0., 0.};
+    for (int i = j - 1; i <= j + 1; ++i) {
+      if (i < 0 || i >= nl) continue;
+      Dual<5> wm[5], w0[5], wp[5];
+      load_w(w, ncell, q0 + i - 1, wm);
+      load_w(w, ncell, q0 + i, w0);
+      load_w(w, ncell, q0 + i + 1, wp);
+      Dual<5>* seeded = (j == i - 1) ? wm : (j == i ? w0 : wp);
+      for (int v = 0; v < 5; ++v) seeded[v] = Dual<5>::seed(seeded[v].v, v);
+      Dual<5> A[NN], B[NN], C[NN];
+      vic_assemble_cell<Dual<5>, N>(wm, w0, wp, Dual<5>(a.gammad), Dual<5>(area),
+                                    Dual<5>(area), Dual<5>(vol), Dual<5>(a.dt),
+                                    Dual<5>(a.grav1), i == 0, i == nl - 1, A, B, C);
+      const double* d0 = cdl + i * 2 * N;
+      const double* y = cdl + i * 2 * N + N;
+      for (int r = 0; r < N; ++r) {
+        Dual<5> row = Dual<5>(0.);
+        for (int c = 0; c < N; ++c) {
+          row += A[r * N + c] * Dual<5>(d0[c]);
+          if (i > 0) row += B[r * N + c] * Dual<5>(cdl[(i - 1) * 2 * N + c]);
+          if (i < nl - 1) row += C[r * N + c] * Dual<5>(cdl[(i + 1) * 2 * N + c]);
+        }
+        for (int v = 0; v < 5; ++v) acc[v] -= y[r] * row.d[v];
+      }
+    }
+    int q = q0 + j;
+    for (int v = 0; v < 5; ++v) lam_w[v * ncell + q] += acc[v];
+  }
+}
+
+// ------------------------------------------------------- explicit part
+
+// interface states of face q along the axis with stride s (face f along the
+// axis); the same arithmetic as recon_kernel for the faces whose adjoint is
+// nonzero (f in [ng, n - ng]: never a dummy face)
+template <typename T>
+__device__ __forceinline__ void face_states(const double* w, int ncell, int q, int s, int f,
+                                            const AdjArgs& a, int d, T* wl, T* wr) {
+  int r = d == 0 ? 0 : 1;
+  int type = a.recon_type[r], scale = a.recon_scale[r], shock = a.recon_shock[r];
+  for (int v = 0; v < 5; ++v) {
+    int tv = (!shock && v >= IVX && v <= IPR) ? interp2_type(type) : type;
+    T win[7];
+    for (int m = -3; m <= 3; ++m) {
+      int c = f + m;
+      int n = d == 0 ? a.n1 : (d == 1 ? a.n2 : a.n3);
+      c = c < 0 ? 0 : (c >= n ? n - 1 : c);
+      win[m + 3] = T(w[v * ncell + q + (c - f) * s]);
+    }
+    // window centred on cell f (index 3) for the right state and cell f-1
+    // (index 2) for the left state
+    wr[v] = interp_face<T>(tv, 0, scale, win, 1, 3);
+    wl[v] = interp_face<T>(tv, 1, scale, win, 1, 2);
+  }
+}
+
+__device__ __forceinline__ int axis_len(const AdjArgs& a, int d) {
+  return d == 0 ? a.n1 : (d == 1 ? a.n2 : a.n3);
+}
+__device__ __forceinline__ int axis_stride(const AdjArgs& a, int d) {
+  return d == 0 ? 1 : (d == 1 ? a.n1 : a.n1 * a.n2);
+}
+
+// lambda_F at each face, Riemann adjoint (Dual<10>), x1 LR-revision adjoint;
+// writes lam_wlr [2][5][ncell] (ILT, IRT) for axis d
+__global__ void __launch_bounds__(kT)
+    flux_adj_kernel(const double* __restrict__ w, const double* __restrict__ lam_dx,
+                    double* __restrict__ lam_wlr, AdjArgs a, int d) {
+  const int ncell = a.n1 * a.n2 * a.n3;
+  const int s = axis_stride(a, d), n = axis_len(a, d);
+  double area = d == 0 ? a.dx3 * a.dx2 : (d == 1 ? a.dx3 * a.dx1 : a.dx2 * a.dx1);
+  double vol = a.dx3 * (a.dx2 * a.dx1);
+  double coef = -a.dt * area / vol;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < ncell; q += gridDim.x * blockDim.x) {
+    int i, j, k;
+    decode(a, q, i, j, k);
+    int f = d == 0 ? i : (d == 1 ? j : k);
+    double lf[5] = {0., 0., 0., 0., 0.};
+    bool any = false;
+    if (f >= 1) {
+      int il = i - (d == 0), jl = j - (d == 1), kl = k - (d == 2);
+      if (interior(a, il, jl, kl))
+        for (int v = 0; v < 5; ++v) lf[v] += coef * lam_dx[v * ncell + q - s], any = true;
+    }
+    if (interior(a, i, j, k))
+      for (int v = 0; v < 5; ++v) lf[v] -= coef * lam_dx[v * ncell + q], any = true;
+    double lwl[5] = {0., 0., 0., 0., 0.}, lwr[5] = {0., 0., 0., 0., 0.};
+    if (any && f >= a.ng && f <= n - a.ng) {
+      Dual<10> wl[5], wr[5];
+      face_states<Dual<10>>(w, ncell, q, s, f, a, d, wl, wr);
+      bool rev = d == 0 && a.grav1 != 0.;
+      if (rev && f == a.ng) {
+        wl[IDN] = wr[IDN];
+        wl[IPR] = wr[IPR];
+      }
+      if (rev && f == n - a.ng) {
+        wr[IDN] = wl[IDN];
+        wr[IPR] = wl[IPR];
+      }
+      // re-seed on the (revised) face states: derivative w.r.t. wl / wr
+      Dual<10> sl[5], sr[5];
+      for (int v = 0; v < 5; ++v) {
+        sl[v] = Dual<10>::seed(wl[v].v, v);
+        sr[v] = Dual<10>::seed(wr[v].v, 5 + v);
+      }
+      Dual<10> flx[5];
+      riemann_flux<Dual<10>>(a.riemann[d], sl, sr, Dual<10>(a.gammad), 3 - d, flx, nullptr);
+      for (int v = 0; v < 5; ++v)
+        for (int m = 0; m < 5; ++m) {
+          lwl[v] += lf[m] * flx[m].d[v];
+          lwr[v] += lf[m] * flx[m].d[5 + v];
+        }
+      // LR revision adjoint: the copied states take the adjoint of the copies
+      if (rev && f == a.ng) {
+        lwr[IDN] += lwl[IDN], lwl[IDN] = 0.;
+        lwr[IPR] += lwl[IPR], lwl[IPR] = 0.;
+      }
+      if (rev && f == n - a.ng) {
+        lwl[IDN] += lwr[IDN], lwr[IDN] = 0.;
+        lwl[IPR] += lwr[IPR], lwr[IPR] = 0.;
+      }
+    }
+    for (int v = 0; v < 5; ++v) {
+      lam_wlr[v * ncell + q] = lwl[v];
+      lam_wlr[(5 + v) * ncell + q] = lwr[v];
+    }
+  }
+}
+
+// lambda_w[c] += sum over faces whose interpolation window holds cell c
+__global__ void __launch_bounds__(kT)
+    recon_adj_kernel(const double* __restrict__ w, const double* __restrict__ lam_wlr,
+                     double* __restrict__ lam_w, AdjArgs a, int d) {
+  const int ncell = a.n1 * a.n2 * a.n3;
+  const int s = axis_stride(a, d), n = axis_len(a, d);
+  int r = d == 0 ? 0 : 1;
+  int type = a.recon_type[r], scale = a.recon_scale[r], shock = a.recon_shock[r];
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < 5 * ncell;
+       t += gridDim.x * blockDim.x) {
+    int v = t / ncell, q = t - v * ncell;
+    int i, j, k;
+    decode(a, q, i, j, k);
+    int c = d == 0 ? i : (d == 1 ? j : k);
+    int tv = (!shock && v >= IVX && v <= IPR) ? interp2_type(type) : type;
+    double acc = 0.;
+    for (int f = c - 3; f <= c + 3; ++f) {
+      if (f < a.ng || f > n - a.ng) continue;
+      double lr = lam_wlr[(5 + v) * ncell + q + (f - c) * s];
+      double ll = lam_wlr[v * ncell + q + (f - c) * s];
+      if (lr == 0. && ll == 0.) continue;
+      Dual<1> win[7];
+      for (int m = -3; m <= 3; ++m) {
+        int cc = f + m;
+        cc = cc < 0 ? 0 : (cc >= n ? n - 1 : cc);
+        win[m + 3] = cc == c ? Dual<1>::seed(w[v * ncell + q + (cc - c) * s], 0)
+                             : Dual<1>(w[v * ncell + q + (cc - c) * s]);
+      }
+      if (lr != 0.) acc += lr * interp_face<Dual<1>>(tv, 0, scale, win, 1, 3).d[0];
+      if (ll != 0.) acc += ll * interp_face<Dual<1>>(tv, 1, scale, win, 1, 2).d[0];
+    }
+    lam_w[t] += acc;
+  }
+}
+
+// constant gravity (src/forcing/const_gravity.cpp:26-45) adjoint
+__global__ void grav_adj_kernel(const double* __restrict__ w, const double* __restrict__ lam_dx,
+                                double* __restrict__ lam_w, AdjArgs a) {
+  const int ncell = a.n1 * a.n2 * a.n3;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < ncell; q += gridDim.x * blockDim.x) {
+    double rho = w[IDN * ncell + q];
+    double l1 = lam_dx[IVX * ncell + q], l2 = lam_dx[IVY * ncell + q];
+    double l3 = lam_dx[IVZ * ncell + q], le = lam_dx[IPR * ncell + q];
+    double lr = 0.;
+    if (a.grav1 != 0.) {
+      lr += a.dt * a.grav1 * a.nh * (l1 + w[IVX * ncell + q] * le);
+      lam_w[IVX * ncell + q] += a.dt * rho * a.grav1 * a.nh * le;
+    }
+    if (a.grav2 != 0.) {
+      lr += a.dt * a.grav2 * (l2 + w[IVY * ncell + q] * le);
+      lam_w[IVY * ncell + q] += a.dt * rho * a.grav2 * le;
+    }
+    if (a.grav3 != 0.) {
+      lr += a.dt * a.grav3 * (l3 + w[IVZ * ncell + q] * le);
+      lam_w[IVZ * ncell + q] += a.dt * rho * a.grav3 * le;
+    }
+    lam_w[IDN * ncell + q] += lr;
+  }
+}
+
+// isentropic ghost refill (src/hydro/hydro.cpp:231-269) adjoint, one thread
+// per column; wpre holds the primitives before the refill (only rho, p of
+// the boundary cells are read)
+__global__ void ghost_adj_kernel(const double* __restrict__ w, double* __restrict__ lam_w,
+                                 AdjArgs a) {
+  const int ncell = a.n1 * a.n2 * a.n3;
+  int ncol = a.n2 * a.n3;
+  for (int col = blockIdx.x * blockDim.x + threadIdx.x; col < ncol;
+       col += gridDim.x * blockDim.x) {
+    for (int side = 0; side < 2; ++side) {
+      int is = side ? a.n1 - a.ng - 1 : a.ng;
+      int base = col * a.n1;
+      Dual<2> rho_b = Dual<2>::seed(w[IDN * ncell + base + is], 0);
+      Dual<2> pre_b = Dual<2>::seed(w[IPR * ncell + base + is], 1);
+      Dual<2> gamma = Dual<2>(a.gammad), gm = gamma - Dual<2>(1.), ae = gm / gamma;
+      Dual<2> inv_g = Dual<2>(1.) / gamma, inv_a = Dual<2>(1.) / ae;
+      Dual<2> grav = Dual<2>(-a.grav1), dz = Dual<2>(a.dx1);
+      Dual<2> K = pre_b / pow(rho_b, gamma);
+      Dual<2> prev = pre_b;
+      double lr = 0., lp = 0.;
+      for (int n = 0; n < a.ng; ++n) {
+        Dual<2> h = side ? pow(prev, ae) - ae * grav * dz / pow(K, inv_g)
+                         : pow(prev, ae) + ae * grav * dz / pow(K, inv_g);
+        Dual<2> pg = pow(h, inv_a);
+        Dual<2> rg = pow(pg / K, inv_g);
+        int g = side ? is + n + 1 : is - n - 1;
+        double lrg = lam_w[IDN * ncell + base + g], lpg = lam_w[IPR * ncell + base + g];
+        lr += lrg * rg.d[0] + lpg * pg.d[0];
+        lp += lrg * rg.d[1] + lpg * pg.d[1];
+        lam_w[IDN * ncell + base + g] = 0.;
+        lam_w[IPR * ncell + base + g] = 0.;
+        prev = pg;
+      }
+      lam_w[IDN * ncell + base + is] += lr;
+      lam_w[IPR * ncell + base + is] += lp;
+    }
+  }
+}
+
+// cons2prim (src/eos/ideal_gas_impl.h:15-35) adjoint: lam_u += J^T lam_w
+__global__ void c2p_adj_kernel(const double* __restrict__ u, const double* __restrict__ lam_w,
+                               double* __restrict__ lam_u, AdjArgs a) {
+  const int ncell = a.n1 * a.n2 * a.n3;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < ncell; q += gridDim.x * blockDim.x) {
+    Dual<5> U[5], W[5];
+    for (int v = 0; v < 5; ++v) U[v] = Dual<5>::seed(u[v * ncell + q], v);
+    W[IDN] = U[IDN];
+    W[IVX] = U[IVX] / W[IDN];
+    W[IVY] = U[IVY] / W[IDN];
+    W[IVZ] = U[IVZ] / W[IDN];
+    Dual<5> ke = Dual<5>(0.5) * ((W[IVX] * U[IVX] + W[IVY] * U[IVY]) + W[IVZ] * U[IVZ]);
+    W[IPR] = (Dual<5>(a.gammad) - Dual<5>(1.)) * (U[IPR] - ke);
+    for (int v = 0; v < 5; ++v) {
+      double acc = 0.;
+      for (int m = 0; m < 5; ++m) acc += lam_w[m * ncell + q] * W[m].d[v];
+      lam_u[v * ncell + q] += acc;
+    }
+  }
+}
+
+__global__ void zero_kernel(double* __restrict__ x, int64_t n) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n;
+       q += (int64_t)gridDim.x * blockDim.x)
+    x[q] = 0.;
+}
+
+__global__ void axpy_kernel(double* __restrict__ y, const double* __restrict__ x, int64_t n) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n;
+       q += (int64_t)gridDim.x * blockDim.x)
+    y[q] += x[q];
+}
+
+static AdjArgs adj_args(const snapx_config* c, double dt) {
+  Grid g = make_grid(c);
+  AdjArgs a;
+  a.n1 = g.n1, a.n2 = g.n2, a.n3 = g.n3, a.ng = g.ng;
+  a.nx1 = c->nx1, a.nx2 = c->nx2, a.nx3 = c->nx3;
+  a.dx1 = g.dx1, a.dx2 = g.dx2, a.dx3 = g.dx3;
+  a.dt = dt, a.gammad = c->gammad;
+  a.grav1 = c->grav1, a.grav2 = c->grav2, a.grav3 = c->grav3, a.nh = c->non_hydrostatic;
+  for (int r = 0; r < 2; ++r) {
+    a.recon_type[r] = c->recon_type[r];
+    a.recon_scale[r] = c->recon_scale[r];
+    a.recon_shock[r] = c->recon_shock[r];
+  }
+  for (int d = 0; d < 3; ++d) a.riemann[d] = c->riemann[d];
+  for (int b = 0; b < 6; ++b) a.bc[b] = c->bc[b];
+  a.scheme = c->implicit_scheme;
+  return a;
+}
+
+static size_t al(size_t x) { return (x + 255) & ~size_t(255); }
+
+struct AdjSpace {
+  double *us[3], *w, *dux, *lam, *lam_us, *lam_u0, *lam_du, *lam_dx, *lam_w, *lam_wlr, *abc,
+      *pr, *dl;
+  char* hydro_ws;
+  size_t hydro_bytes, bytes;
+};
+
+static AdjSpace adj_layout(const snapx_config* c, char* base) {
+  Grid g = make_grid(c);
+  size_t f = al(sizeof(double) * 5 * (size_t)g.ncell);
+  int N = ((c->implicit_scheme >> 3) & 1) ? 5 : 3;
+  size_t ncol = (size_t)(g.n2 > 1 ? c->nx2 : 1) * (g.n3 > 1 ? c->nx3 : 1);
+  size_t cells = ncol * c->nx1;
+  AdjSpace s;
+  size_t off = 0;
+  auto take = [&](size_t b) {
+    char* p = base + off;
+    off += al(b);
+    return reinterpret_cast<double*>(p);
+  };
+  for (int k = 0; k < 3; ++k) s.us[k] = take(f);
+  s.w = take(f);
+  s.dux = take(f);
+  s.lam = take(f);
+  s.lam_us = take(f);
+  s.lam_u0 = take(f);
+  s.lam_du = take(f);
+  s.lam_dx = take(f);
+  s.lam_w = take(f);
+  s.lam_wlr = take(2 * f);
+  s.abc = take(sizeof(double) * cells * 3 * N * N);
+  s.pr = take(sizeof(double) * cells * (N * N + N));
+  s.dl = take(sizeof(double) * cells * 2 * N);
+  s.hydro_bytes = snapx_hydro_workspace_bytes(c, SNAPX_F64);
+  s.hydro_ws = reinterpret_cast<char*>(take(s.hydro_bytes));
+  s.bytes = off;
+  return s;
+}
+
+static int bc_adj(const snapx_config* c, const AdjArgs& a, double* lam, hipStream_t st) {
+  int dims[3] = {3, 2, 1};
+  int ns[3] = {a.n1, a.n2, a.n3};
+  for (int ax = 2; ax >= 0; --ax) {  // reverse of the forward x1, x2, x3 order
+    if (ns[ax] == 1) continue;
+    int tin = c->bc[2 * ax], tout = c->bc[2 * ax + 1];
+    if (tin < 0 && tout < 0) continue;
+    int np = a.n1 * a.n2 * a.n3 / ns[ax];
+    hipLaunchKernelGGL(bc_adj_kernel, blocks_for(5 * np, kT), dim3(kT), 0, st, lam, a, dims[ax],
+                       tin, tout, 1);
+    SNAPX_LAUNCH_CHECK();
+  }
+  return 0;
+}
+
+}  // namespace snapx
+
+using namespace snapx;
+
+extern "C" size_t snapx_adjoint_workspace_bytes(const snapx_config* cfg) {
+  if (validate_config(cfg)) return 0;
+  return adj_layout(cfg, nullptr).bytes;
+}
+
+extern "C" int snapx_adjoint_rk3_step(const snapx_config* cfg, double dt, const double* u_in,
+                                      double* lambda, void* workspace, size_t workspace_bytes,
+                                      void* stream) {
+  if (int e = validate_config(cfg)) return e;
+  SNAPX_CHECK_ARG(u_in && lambda && workspace, "snapx_adjoint_rk3_step: null pointer");
+  SNAPX_CHECK_ARG(!cfg->limiter, "adjoint: the EOS limiter is not differentiable (turn it off)");
+  SNAPX_CHECK_ARG(cfg->non_hydrostatic == 1.0, "adjoint: non-hydrostatic < 1 is not supported");
+  for (int d = 0; d < 3; ++d)
+    SNAPX_CHECK_ARG(!cfg->disable_flux[d], "adjoint: disable-flux is not supported");
+  AdjSpace s = adj_layout(cfg, reinterpret_cast<char*>(workspace));
+  SNAPX_CHECK_ARG(workspace_bytes >= s.bytes, "snapx_adjoint_rk3_step: workspace too small");
+  hipStream_t st = as_stream(stream);
+  Grid g = make_grid(cfg);
+  AdjArgs a = adj_args(cfg, dt);
+  int64_t n = 5 * (int64_t)g.ncell;
+  size_t fb = sizeof(double) * n;
+  snapx_config c0 = *cfg;
+  c0.implicit_scheme = 0;
+  static const double W[3][3] = {
+      {0., 1., 1.}, {3. / 4., 1. / 4., 1. / 4.}, {1. / 3., 2. / 3., 2. / 3.}};
+  dim3 bn = blocks_for(n, kT), bc = blocks_for(g.ncell, kT);
+
+  // ---- forward recompute of the stage inputs ----
+  SNAPX_HIP(hipMemcpyAsync(s.us[0], u_in, fb, hipMemcpyDeviceToDevice, st));
+  for (int k = 0; k < 2; ++k) {
+    if (int e = snapx_hydro_forward(cfg, dt, s.us[k], s.w, s.lam_du, nullptr, nullptr, nullptr,
+                                    s.hydro_ws, s.hydro_bytes, SNAPX_F64, st))
+      return e;
+    if (int e = snapx_rk_combine(s.us[k + 1], s.us[0], s.us[k], s.lam_du, W[k][0], W[k][1],
+                                 W[k][2], n, SNAPX_F64, st))
+      return e;
+    if (int e = snapx_apply_bc(cfg, s.us[k + 1], 5, SNAPX_CONSERVED, SNAPX_F64, st)) return e;
+  }
+
+  // ---- reverse sweep ----
+  SNAPX_HIP(hipMemcpyAsync(s.lam, lambda, fb, hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL(zero_kernel, bn, dim3(kT), 0, st, s.lam_u0, n);
+  SNAPX_LAUNCH_CHECK();
+  int N = ((cfg->implicit_scheme >> 3) & 1) ? 5 : 3;
+  int ncol = (g.n2 > 1 ? cfg->nx2 : 1) * (g.n3 > 1 ? cfg->nx3 : 1);
+  int ns[3] = {g.n1, g.n2, g.n3};
+  for (int stg = 2; stg >= 0; --stg) {
+    if (int e = bc_adj(cfg, a, s.lam, st)) return e;
+    hipLaunchKernelGGL(combine_adj_kernel, bn, dim3(kT), 0, st, s.lam, s.lam_du, s.lam_us,
+                       s.lam_u0, W[stg][0], W[stg][1], W[stg][2], n);
+    SNAPX_LAUNCH_CHECK();
+    // primitives (revised ghosts) and the explicit tendency of this stage
+    if (int e = snapx_hydro_forward(&c0, dt, s.us[stg], s.w, s.dux, nullptr, nullptr, nullptr,
+                                    s.hydro_ws, s.hydro_bytes, SNAPX_F64, st))
+      return e;
+    hipLaunchKernelGGL(zero_kernel, bn, dim3(kT), 0, st, s.lam_w, n);
+    hipLaunchKernelGGL(zero_kernel, bn, dim3(kT), 0, st, s.lam_dx, n);
+    SNAPX_LAUNCH_CHECK();
+    // the column solve rewrites the interior of du only; ghost entries (the
+    // gravity forcing) pass through
+    hipLaunchKernelGGL(axpy_kernel, bn, dim3(kT), 0, st, s.lam_dx, s.lam_du, n);
+    SNAPX_LAUNCH_CHECK();
+    if (cfg->implicit_scheme) {
+      if (N == 3) {
+        hipLaunchKernelGGL(vic_adj_column_kernel<3>, blocks_for(ncol, 64), dim3(64), 0, st, s.w,
+                           s.dux, s.lam_du, s.lam_dx, s.lam_w, s.abc, s.pr, s.dl, a, ncol);
+        SNAPX_LAUNCH_CHECK();
+        hipLaunchKernelGGL(vic_adj_rows_kernel<3>, blocks_for((int64_t)ncol * (cfg->nx1 + 2), kT),
+                           dim3(kT), 0, st, s.w, s.dl, s.lam_w, a, ncol);
+      } else {
+        hipLaunchKernelGGL(vic_adj_column_kernel<5>, blocks_for(ncol, 64), dim3(64), 0, st, s.w,
+                           s.dux, s.lam_du, s.lam_dx, s.lam_w, s.abc, s.pr, s.dl, a, ncol);
+        SNAPX_LAUNCH_CHECK();
+        hipLaunchKernelGGL(vic_adj_rows_kernel<5>, blocks_for((int64_t)ncol * (cfg->nx1 + 2), kT),
+                           dim3(kT), 0, st, s.w, s.dl, s.lam_w, a, ncol);
+      }
+      SNAPX_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(grav_adj_kernel, bc, dim3(kT), 0, st, s.w, s.lam_dx, s.lam_w, a);
+    SNAPX_LAUNCH_CHECK();
+    for (int d = 0; d < 3; ++d) {
+      if (ns[d] == 1) continue;
+      hipLaunchKernelGGL(flux_adj_kernel, bc, dim3(kT), 0, st, s.w, s.lam_dx, s.lam_wlr, a, d);
+      SNAPX_LAUNCH_CHECK();
+      hipLaunchKernelGGL(recon_adj_kernel, bn, dim3(kT), 0, st, s.w, s.lam_wlr, s.lam_w, a, d);
+      SNAPX_LAUNCH_CHECK();
+    }
+    if (g.n1 > 1 && cfg->grav1 != 0.) {
+      hipLaunchKernelGGL(ghost_adj_kernel, blocks_for(g.n2 * g.n3, kT), dim3(kT), 0, st, s.w,
+                         s.lam_w, a);
+      SNAPX_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(c2p_adj_kernel, bc, dim3(kT), 0, st, s.us[stg], s.lam_w, s.lam_us, a);
+    SNAPX_LAUNCH_CHECK();
+    // lam_us is now dL/du_stg; the next iteration folds its ghosts (u_stg,
+    // stg > 0, left a boundary pass); u_0 = u_in keeps its ghost adjoints
+    SNAPX_HIP(hipMemcpyAsync(s.lam, s.lam_us, fb, hipMemcpyDeviceToDevice, st));
+  }
+  hipLaunchKernelGGL(axpy_kernel, bn, dim3(kT), 0, st, s.lam, s.lam_u0, n);
+  SNAPX_LAUNCH_CHECK();
+  SNAPX_HIP(hipMemcpyAsync(lambda, s.lam, fb, hipMemcpyDeviceToDevice, st));
+  return 0;
+}

@@ -1,0 +1,186 @@
+"""GPU checks of the hand-written adjoint (snapx_adjoint_rk3_step).
+
+The adjoint has no counterpart in the reference; what pins it is the forward
+map it differentiates (the FP64 plain path, itself pinned to the oracle):
+
+* dot-product test  <lambda, (S(u + e v) - S(u - e v)) / 2e>  ==  <J^T lambda, v>
+  for random directions v and weights lambda, per configuration (central
+  differences of the FP64 step; bar 1e-6 relative);
+* torch.autograd.gradcheck of the RK3Step node on tiny grids (every column of
+  the Jacobian by finite differences against every row by the adjoint);
+* the same gradient through a 3-step ``integrate`` chain as the product of
+  single-step adjoints (checkpointing restores the stage states exactly).
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from snapy_linfel_amd import capi
+from snapy_linfel_amd import adjoint as A
+from test_gpu_plain import _box3d, _straka_small, cfg_of, to_dev
+
+pytestmark = pytest.mark.gpu
+
+
+def _step(cfg, dt, u):
+    out = u.clone()
+    capi.rk3_step(cfg, dt, out, torch.empty_like(out))
+    return out
+
+
+def _scales(u):
+    s = u.abs().amax(dim=(1, 2, 3))
+    return torch.maximum(s, torch.full_like(s, 1e-3)).view(5, 1, 1, 1)
+
+
+def _case(name):
+    if name.startswith("straka"):
+        oc, w, u = _straka_small(int(name[6]), nx2=24, nx1=16, wind=10.0)
+        return oc, u, 0.2
+    if name == "box_lmars_roe":
+        oc, w, u = _box3d(n=(10, 8, 6))
+        return oc, u, 0.2
+    if name == "box_hllc":
+        oc, w, u = _box3d(riemann="hllc", n=(10, 8, 6))
+        return oc, u, 0.2
+    if name == "box_nograv":
+        oc, w, u = _box3d(riemann=("hllc", "lmars", "roe"), scheme=0, n=(10, 8, 6))
+        oc.grav1 = 0.0
+        w, u = O.initialize(oc, w)
+        return oc, u, 0.2
+    if name == "sod":
+        oc = O.sod_config(64)
+        w, u = O.initialize(oc, _sod_moving(oc))
+        return oc, u, O.max_time_step(oc, w, 0.5)
+    raise KeyError(name)
+
+
+def _sod_moving(oc, seed=11):
+    """Sod state advected at 0.1 (c ~ 1.2) with 1 % seeded noise.  HLLC's
+    branches are continuous but not differentiable where the contact speed
+    is zero or where the PVRS star pressure equals a side pressure; every
+    face of a uniform region at rest sits on both, and every face of a
+    uniform region in motion on the second.  The noise moves the faces off
+    them by far more than the finite-difference step."""
+    rng = np.random.default_rng(seed)
+    w = O.sod_init(oc)
+    w[0] *= 1.0 + 1e-2 * rng.uniform(-1, 1, w[0].shape)
+    w[4] *= 1.0 + 1e-2 * rng.uniform(-1, 1, w[4].shape)
+    w[1] = 0.1 + 1e-2 * rng.uniform(-1, 1, w[1].shape)
+    return w
+
+
+def _bc_consistent(cfg, v):
+    """Directions are taken in the space of states that satisfy the boundary
+    conditions (every state the step sees does): at a reflecting wall the
+    face velocity is then exactly zero for all of them, and the upwind branch
+    of the solver never switches."""
+    capi.apply_bc(cfg, v, capi.CONSERVED)
+    return v
+
+
+CASES = ["straka0", "straka1", "straka9", "box_lmars_roe", "box_hllc", "box_nograv", "sod"]
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_adjoint_dot_product_vs_central_differences(case):
+    oc, u_np, dt = _case(case)
+    cfg = cfg_of(oc)
+    u = to_dev(u_np)
+    g = torch.Generator(device="cpu").manual_seed(7)
+    sc = _scales(u)
+    ws = capi.adjoint_workspace(cfg)
+    for trial in range(2):
+        v = _bc_consistent(cfg, torch.randn(u.shape, generator=g, dtype=torch.float64)
+                           .to(u.device) * sc)
+        lam = torch.randn(u.shape, generator=g, dtype=torch.float64).to(u.device) / sc
+        eps = 1e-6
+        fd = ((_step(cfg, dt, u + eps * v) - _step(cfg, dt, u - eps * v)) * lam).sum() / (2 * eps)
+        gl = lam.clone()
+        capi.adjoint_rk3_step(cfg, dt, u, gl, ws)
+        ad = (gl * v).sum()
+        rel = abs(float(ad - fd)) / max(abs(float(fd)), 1e-300)
+        print(f"{case} trial {trial}: adjoint {float(ad):.12e} fd {float(fd):.12e} rel {rel:.2e}")
+        assert rel < 1e-6, (case, trial, float(ad), float(fd), rel)
+
+
+def _bc_box_tiny(xi, ng=3):
+    """Boundary pass of the tiny box as torch ops (x1 reflecting with the
+    normal momentum flipped, then x2 periodic over whole rows, the order of
+    the library's boundary pass)."""
+    sgn = torch.tensor([1.0, -1.0, 1.0, 1.0, 1.0], dtype=xi.dtype, device=xi.device)
+    sgn = sgn.view(5, 1, 1, 1)
+    lo = xi[..., :ng].flip(-1) * sgn
+    hi = xi[..., -ng:].flip(-1) * sgn
+    x = torch.cat([lo, xi, hi], dim=-1)
+    return torch.cat([x[:, :, -ng:], x, x[:, :, :ng]], dim=2)
+
+
+def test_adjoint_gradcheck_sod_tiny():
+    oc = O.sod_config(12)
+    w, u_np = O.initialize(oc, _sod_moving(oc))
+    dt = O.max_time_step(oc, w, 0.5)
+    cfg = cfg_of(oc)
+    u = to_dev(u_np)
+    sc = _scales(u)
+    # every entry (ghosts included) in scaled variables, so that all are O(1)
+    x = (u / sc).detach().requires_grad_(True)
+
+    def f(x):
+        return A.rk3_step(x * sc, cfg, dt) / sc
+
+    assert torch.autograd.gradcheck(f, (x,), eps=1e-6, atol=1e-7, rtol=1e-5)
+
+
+def test_adjoint_gradcheck_box_tiny():
+    """x1 reflecting + gravity + 3x3 column solve, x2 periodic, LMARS / Roe;
+    the inputs are the interior cells, the boundary pass is part of f."""
+    oc, w, u_np = _box3d(n=(6, 4, 1), scheme=1)
+    dt = 0.2
+    cfg = cfg_of(oc)
+    u = to_dev(u_np)
+    assert u.shape == (5, 1, 10, 12)
+    sc = _scales(u)
+    xi = (u[:, :, 3:-3, 3:-3] / sc).detach().requires_grad_(True)
+
+    def f(xi):
+        out = A.rk3_step(_bc_box_tiny(xi * sc), cfg, dt)
+        return out[:, :, 3:-3, 3:-3] / sc
+
+    # the torch boundary pass reproduces the library's
+    full = _bc_box_tiny(u[:, :, 3:-3, 3:-3])
+    ref = full.clone()
+    capi.apply_bc(cfg, ref, capi.CONSERVED)
+    assert torch.equal(full, ref)
+    assert torch.autograd.gradcheck(f, (xi,), eps=1e-6, atol=1e-7, rtol=1e-5)
+
+
+def test_integrate_chain_matches_stepwise_adjoints():
+    oc, u_np, dt = _case("straka1")
+    cfg = cfg_of(oc)
+    u0 = to_dev(u_np)
+    g = torch.Generator(device="cpu").manual_seed(3)
+    lam = torch.randn(u0.shape, generator=g, dtype=torch.float64).to(u0.device) / _scales(u0)
+    x = u0.clone().requires_grad_(True)
+    y = A.integrate(x, cfg, dt, 3)
+    (grad,) = torch.autograd.grad((y * lam).sum(), x)
+    # by hand: forward states, then adjoints in reverse
+    us = [u0]
+    for _ in range(2):
+        us.append(_step(cfg, dt, us[-1]))
+    l = lam.clone()
+    for k in (2, 1, 0):
+        capi.adjoint_rk3_step(cfg, dt, us[k], l)
+    assert torch.equal(grad, l)
+    # forward values are the plain path's
+    assert torch.equal(y.detach(), _step(cfg, dt, us[2]))
+
+
+def test_adjoint_rejects_limiter():
+    oc, u_np, dt = _case("sod")
+    oc.limiter = 1
+    cfg = cfg_of(oc)
+    u = to_dev(u_np)
+    with pytest.raises(capi.SnapxError, match="limiter"):
+        capi.adjoint_rk3_step(cfg, dt, u, u.clone())

@@ -140,16 +140,35 @@ def test_boundary_functions_exact():
     assert np.array_equal(host(d), ref)
 
 
-def _straka_small(scheme, nx2=48, nx1=24, riemann="lmars"):
+def _straka_small(scheme, nx2=48, nx1=24, riemann="lmars", wind=0.0):
+    """Reduced Straka block (dx1 = 6.4 km / nx1, dx2 = 100 m).  ``wind`` adds
+    a smooth velocity field of that amplitude (m/s), so that every tendency
+    is O(dt u / dx) instead of the roundoff residual of a state at rest."""
     oc = O.straka_config(nx2=nx2, nx1=nx1, implicit_scheme=scheme, riemann=riemann)
     oc.x2max = 25.6e3 * nx2 / 512 * 2
-    w, u = O.initialize(oc, O.straka_init(oc))
+    wp = O.straka_init(oc)
+    if wind:
+        n3, n2, n1 = O.shape(oc)
+        z = (np.arange(n1) - oc.nghost + 0.5) / oc.nx1
+        x = (np.arange(n2) - oc.nghost + 0.5) / oc.nx2
+        X, Z = np.meshgrid(x, z, indexing="ij")
+        wp[1, 0] = wind * np.sin(np.pi * Z) * np.sin(2 * np.pi * X)
+        wp[2, 0] = wind * np.cos(np.pi * Z) * np.cos(2 * np.pi * X)
+    w, u = O.initialize(oc, wp)
     return oc, w, u
+
+
+def _du_tol(scheme):
+    """Bar for du vs the oracle, relative to the largest tendency.  Host and
+    device pow/sqrt differ by an ulp; the 5x5 block solve (scheme 9, partial
+    pivoting) amplifies that by the condition number of its blocks (acoustic
+    CFL ~2 in the vertical), so it gets 1e-9 instead of 1e-11."""
+    return 1e-9 if scheme == 9 else 1e-11
 
 
 @pytest.mark.parametrize("scheme", [0, 1, 9])
 def test_hydro_forward_straka(scheme):
-    oc, w, u = _straka_small(scheme)
+    oc, w, u = _straka_small(scheme, wind=10.0)
     du_ref, w_ref, F = O.hydro_forward(oc, 0.5, u)
     cfg = cfg_of(oc)
     ud = to_dev(u)
@@ -161,7 +180,7 @@ def test_hydro_forward_straka(scheme):
     # du itself: scale by the largest tendency of the variable
     scale = np.abs(du_ref[I]).max(axis=(1, 2, 3)) + 1e-300
     rel = np.abs(host(dud)[I] - du_ref[I]).max(axis=(1, 2, 3)) / scale
-    assert (rel < 1e-11).all(), rel
+    assert (rel < _du_tol(scheme)).all(), rel
     assert np.allclose(host(wd), w_ref, rtol=1e-14, atol=0)
 
 
@@ -197,7 +216,7 @@ def test_hydro_forward_3d_per_direction_solvers(scheme):
     I = O.interior(oc)
     scale = np.abs(du_ref[I]).max(axis=(1, 2, 3)) + 1e-300
     rel = np.abs(host(dud)[I] - du_ref[I]).max(axis=(1, 2, 3)) / scale
-    assert (rel < 1e-11).all(), rel
+    assert (rel < _du_tol(scheme)).all(), rel
 
 
 @pytest.mark.parametrize("case", ["straka1", "straka9", "box_lmars_roe", "box_hllc"])
