@@ -260,6 +260,8 @@ typedef struct snapx_batch_info {
   int64_t scratch_doubles;
   size_t rhs_smem;
   double dt;
+  int col_groups; /* work items of the cell-parallel column kernel (0: the
+                     lane-per-column kernel serves the batch) */
 } snapx_batch_info;
 
 /* Members are independent blocks, each with its own snapx_config (same

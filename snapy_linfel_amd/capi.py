@@ -136,7 +136,8 @@ class BatchInfo(C.Structure):
     _fields_ = [("nmembers", C.c_int), ("tile_x1", C.c_int), ("tile_x2", C.c_int),
                 ("tile_x3", C.c_int), ("total_tiles", C.c_int), ("total_cols", C.c_int),
                 ("vic_size", C.c_int), ("total_elems", C.c_int64),
-                ("scratch_doubles", C.c_int64), ("rhs_smem", C.c_size_t), ("dt", C.c_double)]
+                ("scratch_doubles", C.c_int64), ("rhs_smem", C.c_size_t), ("dt", C.c_double),
+                ("col_groups", C.c_int)]
 
 
 class Timing(C.Structure):

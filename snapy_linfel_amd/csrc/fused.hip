@@ -10,11 +10,14 @@
 //     differences are computed from LDS; interface states and fluxes never
 //     leave the chip.  Output: the explicit tendency du (dt included) of the
 //     tile's cells, gravity added.
-//  K2 fused_column_kernel (FP64 arithmetic): one lane per vertical column.
-//     Rebuilds the column's primitives in FP64, assembles the block rows with
-//     rolling interface Jacobians, runs the block Thomas sweep and the MS-VIC
-//     redistribution, and applies the SSP-RK3 stage combine; writes the new
-//     conserved state.
+//  K2 fused_column_pcr_kernel (FP64 arithmetic): one lane per cell, whole
+//     columns per workgroup.  Rebuilds the primitives in FP64, assembles the
+//     block rows (interface |A| shared through LDS), solves the
+//     block-tridiagonal system by parallel cyclic reduction, applies the
+//     MS-VIC redistribution (column sums by wave reductions) and the SSP-RK3
+//     stage combine; writes the new conserved state.  Columns taller than
+//     the workgroup use fused_column_kernel (one lane per column, rolling
+//     interface Jacobians, serial block Thomas sweep).
 //
 // Ghost cells are never read during the step: halo loads map outside indices
 // through the external boundary functions (reflecting: mirror and flip the
@@ -293,6 +296,7 @@ struct ColArgs {
   int bc_in, bc_out;  // x1 boundary functions
   int scheme;         // 0 none, 1 partial, 9 full
   int stage;
+  int lanes, lshift;  // cell-parallel kernel: lanes per column (power of two), log2
   double gammad, dx1, area, vol, dt, grav1;
   double w0, w1, w2;  // stage weights
 };
@@ -518,6 +522,368 @@ __global__ void __launch_bounds__(kColThreads)
   column_work<N>(u, u0, du, scr, a, col);
 }
 
+// ------------------------------------------- cell-parallel column solve ----
+//
+// One lane per cell: a workgroup holds T / L whole columns, L = lanes per
+// column (power of two >= nl).  Primitives and interface |A| are exchanged
+// through LDS, every lane assembles its own block row, and the
+// block-tridiagonal system is solved by parallel cyclic reduction (log2 nl
+// levels; at each level row i eliminates its couplings to rows i -+ s with
+// the normalised rows of its neighbours).  The MS-VIC column sums are wave
+// reductions, then the lane applies the redistribution and the stage
+// combine to its own cell.  Arithmetic in FP64, as in the serial sweep.
+
+template <int N>
+struct PcrShape {
+  static constexpr int kThreads = N == 3 ? 256 : 128;
+  static constexpr int kPub = 2 * N * N + N;  // doubles published per lane
+};
+
+// Solves A X = R for the N x M right-hand side R (row-major, overwritten),
+// partial pivoting with fully unrolled row selection (no dynamic indexing).
+template <int N, int M>
+__device__ __forceinline__ void solve_rows(double* A, double* R) {
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    // pivot row p >= k of largest |A[., k]|
+    int p = k;
+    double big = fabs(A[k * N + k]);
+#pragma unroll
+    for (int r = k + 1; r < N; ++r) {
+      double v = fabs(A[r * N + k]);
+      if (v > big) {
+        big = v;
+        p = r;
+      }
+    }
+#pragma unroll
+    for (int r = k + 1; r < N; ++r) {
+      if (p == r) {
+#pragma unroll
+        for (int c = 0; c < N; ++c) {
+          double t = A[k * N + c];
+          A[k * N + c] = A[r * N + c];
+          A[r * N + c] = t;
+        }
+#pragma unroll
+        for (int c = 0; c < M; ++c) {
+          double t = R[k * M + c];
+          R[k * M + c] = R[r * M + c];
+          R[r * M + c] = t;
+        }
+      }
+    }
+    double inv = 1.0 / A[k * N + k];
+#pragma unroll
+    for (int r = 0; r < N; ++r) {
+      if (r == k) continue;
+      double f = A[r * N + k] * inv;
+#pragma unroll
+      for (int c = k; c < N; ++c) A[r * N + c] -= f * A[k * N + c];
+#pragma unroll
+      for (int c = 0; c < M; ++c) R[r * M + c] -= f * R[k * M + c];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    double inv = 1.0 / A[k * N + k];
+#pragma unroll
+    for (int c = 0; c < M; ++c) R[k * M + c] *= inv;
+  }
+}
+
+// normalised row of lane t: Pb = A^-1 B, Pc = A^-1 C, pr = A^-1 r, written
+// to LDS (structure of arrays, stride T)
+template <int N>
+__device__ __forceinline__ void pcr_publish(const double* A, const double* B, const double* C,
+                                            const double* r, double* sh, int T, int t) {
+  const int NN = N * N;
+  if (N == 3) {
+    double inv[9], x[9];
+    inverse3<double>(A, inv);
+    matmul<double, 3>(inv, B, x);
+#pragma unroll
+    for (int e = 0; e < 9; ++e) sh[e * T + t] = x[e];
+    matmul<double, 3>(inv, C, x);
+#pragma unroll
+    for (int e = 0; e < 9; ++e) sh[(9 + e) * T + t] = x[e];
+    matvec<double, 3>(inv, r, x);
+#pragma unroll
+    for (int e = 0; e < 3; ++e) sh[(18 + e) * T + t] = x[e];
+  } else {
+    const int M = 2 * N + 1;
+    double a[NN], R[N * M];
+#pragma unroll
+    for (int e = 0; e < NN; ++e) a[e] = A[e];
+#pragma unroll
+    for (int row = 0; row < N; ++row) {
+#pragma unroll
+      for (int c = 0; c < N; ++c) {
+        R[row * M + c] = B[row * N + c];
+        R[row * M + N + c] = C[row * N + c];
+      }
+      R[row * M + 2 * N] = r[row];
+    }
+    solve_rows<N, M>(a, R);
+#pragma unroll
+    for (int row = 0; row < N; ++row) {
+#pragma unroll
+      for (int c = 0; c < N; ++c) {
+        sh[(row * N + c) * T + t] = R[row * M + c];
+        sh[(NN + row * N + c) * T + t] = R[row * M + N + c];
+      }
+      sh[(2 * NN + row) * T + t] = R[row * M + 2 * N];
+    }
+  }
+}
+
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) x += __shfl_xor(x, m);
+  return x;
+}
+
+// Every thread of the workgroup calls this with the same ColArgs; lane t
+// serves layer i = t mod L of column col (col < a.ncol checked here).
+template <int N>
+__device__ __forceinline__ void column_pcr(float* __restrict__ u, float* __restrict__ u0,
+                                           const float* __restrict__ du, const ColArgs& a,
+                                           int col, double* __restrict__ sh) {
+  constexpr int NN = N * N;
+  const int T = blockDim.x, t = threadIdx.x;
+  const int ncell = a.n1 * a.n2 * a.n3;
+  const int nl = a.nl;
+  const int i = t & (a.lanes - 1);
+  const bool colok = col < a.ncol;
+  const bool act = colok && i < nl;
+  int q = 0;
+  if (colok) {
+    int j = a.n2 > 1 ? a.ng + col % a.ncol2 : 0;
+    int k = a.n3 > 1 ? a.ng + col / a.ncol2 : 0;
+    q = (k * a.n2 + j) * a.n1 + a.ng + i;
+  }
+  const int q0 = q - i;
+
+  double dd[5];
+  if (a.scheme) {
+    // ---- primitives of the cell and of its neighbours ----
+    double wc[5] = {1., 0., 0., 0., 1.}, wm[5], wp[5];
+    if (act) column_prim(u, ncell, q0, i, a, wc);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) sh[v * T + t] = wc[v];
+    __syncthreads();
+#pragma unroll
+    for (int v = 0; v < 5; ++v) wm[v] = wp[v] = wc[v];
+    if (act) {
+      if (i > 0) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) wm[v] = sh[v * T + t - 1];
+      } else {
+        column_prim(u, ncell, q0, -1, a, wm);
+        refill_ghost(wc, wm, a, -1);
+      }
+      if (i < nl - 1) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) wp[v] = sh[v * T + t + 1];
+      } else {
+        column_prim(u, ncell, q0, nl, a, wp);
+        refill_ghost(wc, wp, a, +1);
+      }
+    }
+    __syncthreads();
+    // ---- interface |A|: lane i owns the face below its cell ----
+    const double gm1 = a.gammad - 1.;
+    const double gm1r = 0.5 * (a.gammad + a.gammad) - 1.;
+    double Am[NN], Ap[NN];
+    abs_jacobian<double, N>(gm1r, wm, wc, Am);
+#pragma unroll
+    for (int e = 0; e < NN; ++e) sh[e * T + t] = Am[e];
+    __syncthreads();
+    if (act && i < nl - 1) {
+#pragma unroll
+      for (int e = 0; e < NN; ++e) Ap[e] = sh[e * T + t + 1];
+    } else {
+      abs_jacobian<double, N>(gm1r, wc, wp, Ap);
+    }
+    __syncthreads();
+    // ---- block row i (vic_assemble_cell; uniform Cartesian areas) ----
+    double A[NN], B[NN], C[NN], r[N];
+    {
+      double Jp[NN], Jn[NN];
+      flux_jacobian<double, N>(gm1, wm, Jp);
+      flux_jacobian<double, N>(gm1, wp, Jn);
+      const double hv = 0.5 / a.vol;
+#pragma unroll
+      for (int rr = 0; rr < N; ++rr)
+#pragma unroll
+        for (int c = 0; c < N; ++c) {
+          int o = rr * N + c;
+          double phi = ((rr == 1 && c == 0) || (vic_row(N, rr) == IPR && c == 1)) ? a.grav1 : 0.;
+          A[o] = (Am[o] * a.area + Ap[o] * a.area) * hv - phi;
+          if (rr == c) A[o] += 1. / a.dt;
+          B[o] = -(Am[o] + Jp[o]) * a.area * hv;
+          C[o] = -(Ap[o] - Jn[o]) * a.area * hv;
+        }
+      // reflecting column ends fold the ghost coupling into the diagonal
+#pragma unroll
+      for (int rr = 0; rr < N; ++rr)
+#pragma unroll
+        for (int c = 0; c < N; ++c) {
+          double bnd = c == 1 ? -1. : 1.;
+          if (i == 0) A[rr * N + c] += B[rr * N + c] * bnd;
+          if (i == nl - 1) A[rr * N + c] += C[rr * N + c] * bnd;
+        }
+      if (i == 0)
+#pragma unroll
+        for (int e = 0; e < NN; ++e) B[e] = 0.;
+      if (i == nl - 1)
+#pragma unroll
+        for (int e = 0; e < NN; ++e) C[e] = 0.;
+    }
+#pragma unroll
+    for (int v = 0; v < 5; ++v) dd[v] = act ? (double)du[v * ncell + q] : 0.;
+    r[0] = dd[IDN] / a.dt;
+    r[1] = dd[IVX] / a.dt;
+    if (N == 3) {
+      r[2] = dd[IPR] / a.dt;
+    } else {
+      r[2] = dd[IVY] / a.dt;
+      r[3] = dd[IVZ] / a.dt;
+      r[4] = dd[IPR] / a.dt;
+    }
+    if (!act) {
+#pragma unroll
+      for (int e = 0; e < NN; ++e) {
+        A[e] = (e % (N + 1)) == 0 ? 1. : 0.;
+        B[e] = C[e] = 0.;
+      }
+#pragma unroll
+      for (int e = 0; e < N; ++e) r[e] = 0.;
+    }
+    // ---- parallel cyclic reduction ----
+    for (int sft = 1; sft < nl; sft <<= 1) {
+      pcr_publish<N>(A, B, C, r, sh, T, t);
+      __syncthreads();
+      const bool hm = act && i - sft >= 0;
+      const bool hp = act && i + sft < nl;
+      double nB[NN], nC[NN];
+#pragma unroll
+      for (int e = 0; e < NN; ++e) nB[e] = nC[e] = 0.;
+      if (hm) {
+        const int tm = t - sft;
+        double Pb[NN], Pc[NN], pr[N], x[NN];
+#pragma unroll
+        for (int e = 0; e < NN; ++e) {
+          Pb[e] = sh[e * T + tm];
+          Pc[e] = sh[(NN + e) * T + tm];
+        }
+#pragma unroll
+        for (int e = 0; e < N; ++e) pr[e] = sh[(2 * NN + e) * T + tm];
+        matmul<double, N>(B, Pb, x);
+#pragma unroll
+        for (int e = 0; e < NN; ++e) nB[e] = -x[e];
+        matmul<double, N>(B, Pc, x);
+#pragma unroll
+        for (int e = 0; e < NN; ++e) A[e] -= x[e];
+        matvec<double, N>(B, pr, x);
+#pragma unroll
+        for (int e = 0; e < N; ++e) r[e] -= x[e];
+      }
+      if (hp) {
+        const int tp = t + sft;
+        double Pb[NN], Pc[NN], pr[N], x[NN];
+#pragma unroll
+        for (int e = 0; e < NN; ++e) {
+          Pb[e] = sh[e * T + tp];
+          Pc[e] = sh[(NN + e) * T + tp];
+        }
+#pragma unroll
+        for (int e = 0; e < N; ++e) pr[e] = sh[(2 * NN + e) * T + tp];
+        matmul<double, N>(C, Pc, x);
+#pragma unroll
+        for (int e = 0; e < NN; ++e) nC[e] = -x[e];
+        matmul<double, N>(C, Pb, x);
+#pragma unroll
+        for (int e = 0; e < NN; ++e) A[e] -= x[e];
+        matvec<double, N>(C, pr, x);
+#pragma unroll
+        for (int e = 0; e < N; ++e) r[e] -= x[e];
+      }
+#pragma unroll
+      for (int e = 0; e < NN; ++e) {
+        B[e] = nB[e];
+        C[e] = nC[e];
+      }
+      __syncthreads();
+    }
+    // ---- decoupled rows: delta = A^-1 r ----
+    double d[N];
+    if (N == 3) {
+      double inv[9];
+      inverse3<double>(A, inv);
+      matvec<double, 3>(inv, r, d);
+    } else {
+      solve_rows<N, 1>(A, r);
+#pragma unroll
+      for (int e = 0; e < N; ++e) d[e] = r[e];
+    }
+    // ---- MS-VIC column sums (wave reductions, then across the column's waves) ----
+    double ai = wc[IDN] * a.vol;
+    double s_a2 = act ? ai * ai : 0.;
+    double s_b = act ? (dd[IDN] - d[0]) * 1.0 * a.vol : 0.;
+    s_a2 = wave_sum(s_a2);
+    s_b = wave_sum(s_b);
+    if (a.lanes > 64) {
+      const int wv = t >> 6, nwv = a.lanes >> 6;
+      if ((t & 63) == 0) {
+        sh[2 * wv] = s_a2;
+        sh[2 * wv + 1] = s_b;
+      }
+      __syncthreads();
+      const int w0 = (t >> a.lshift) * nwv;
+      s_a2 = 0.;
+      s_b = 0.;
+      for (int k = 0; k < nwv; ++k) {
+        s_a2 += sh[2 * (w0 + k)];
+        s_b += sh[2 * (w0 + k) + 1];
+      }
+    }
+    double rho = wc[IDN];
+    double fix = d[0] - dd[IDN];
+    dd[IDN] = (dd[IDN] + fix) + rho * ai * s_b / s_a2;
+    dd[IVX] = d[1];
+    if (N == 3) {
+      dd[IPR] = d[2];
+    } else {
+      dd[IVY] = d[2];
+      dd[IVZ] = d[3];
+      dd[IPR] = d[4];
+    }
+  } else {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) dd[v] = act ? (double)du[v * ncell + q] : 0.;
+  }
+  // ---- stage combine ----
+  if (act) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      float uc = u[v * ncell + q];
+      float ub = a.stage == 0 ? uc : u0[v * ncell + q];
+      if (a.stage == 0) u0[v * ncell + q] = uc;
+      u[v * ncell + q] = (float)((a.w0 * ub + a.w1 * uc) + a.w2 * dd[v]);
+    }
+  }
+}
+
+template <int N>
+__global__ void __launch_bounds__(PcrShape<N>::kThreads)
+    fused_column_pcr_kernel(float* __restrict__ u, float* __restrict__ u0,
+                            const float* __restrict__ du, ColArgs a) {
+  extern __shared__ double sh[];
+  int col = blockIdx.x * (blockDim.x >> a.lshift) + (threadIdx.x >> a.lshift);
+  column_pcr<N>(u, u0, du, a, col, sh);
+}
+
 // ------------------------------------------------------- batched members ----
 
 // One member of a batch: its block geometry / options, where its fields start
@@ -529,14 +895,21 @@ struct BatchMember {
   int64_t off;      // element offset of the member's [5][n3][n2][n1] field
   int64_t scr_off;  // double offset of the member's column scratch
   int tile0;        // first K1 work item
-  int col0;         // first K2 work item (multiple of the wavefront size)
+  int col0;         // first K2 work item of the lane-per-column kernel
+                    // (multiple of the wavefront size)
+  int grp0;         // first K2 work item of the cell-parallel kernel (one
+                    // workgroup of whole columns)
 };
 
-__device__ __forceinline__ int find_member(const BatchMember* m, int nm, int item, bool tiles) {
+enum { kListTiles = 0, kListCols = 1, kListGroups = 2 };
+
+__device__ __forceinline__ int find_member(const BatchMember* m, int nm, int item, int list) {
   int lo = 0, hi = nm - 1;
   while (lo < hi) {
     int mid = (lo + hi + 1) >> 1;
-    int start = tiles ? m[mid].tile0 : m[mid].col0;
+    int start = list == kListTiles ? m[mid].tile0
+                : list == kListCols ? m[mid].col0
+                                    : m[mid].grp0;
     if (start <= item)
       lo = mid;
     else
@@ -551,7 +924,7 @@ __global__ void __launch_bounds__(kRhsThreads)
                      const BatchMember* __restrict__ mem, int nm, int total) {
   extern __shared__ float smem[];
   for (int item = blockIdx.x; item < total; item += gridDim.x) {
-    int m = find_member(mem, nm, item, true);
+    int m = find_member(mem, nm, item, kListTiles);
     FusedArgs a = mem[m].a;
     rhs_tile(u + mem[m].off, du + mem[m].off, a, item - mem[m].tile0, smem);
     __syncthreads();
@@ -567,7 +940,7 @@ __global__ void __launch_bounds__(kColThreads)
                         const BatchMember* __restrict__ mem, int nm, int total, int stage) {
   for (int item = blockIdx.x * blockDim.x + threadIdx.x; item < total;
        item += gridDim.x * blockDim.x) {
-    int m = find_member(mem, nm, item, false);
+    int m = find_member(mem, nm, item, kListCols);
     ColArgs c = mem[m].c;
     int col = item - mem[m].col0;
     if (col >= c.ncol) continue;
@@ -577,6 +950,27 @@ __global__ void __launch_bounds__(kColThreads)
     c.w0 = W[stage][0], c.w1 = W[stage][1], c.w2 = W[stage][2];
     int64_t o = mem[m].off;
     column_work<N>(u + o, u0 + o, du + o, scr + mem[m].scr_off, c, col);
+  }
+}
+
+// persistent cell-parallel K2 over the (member, column group) work list
+template <int N>
+__global__ void __launch_bounds__(PcrShape<N>::kThreads)
+    batch_column_pcr_kernel(float* __restrict__ u, float* __restrict__ u0,
+                            const float* __restrict__ du, const BatchMember* __restrict__ mem,
+                            int nm, int total, int stage) {
+  extern __shared__ double sh[];
+  static const double W[3][3] = {
+      {0., 1., 1.}, {3. / 4., 1. / 4., 1. / 4.}, {1. / 3., 2. / 3., 2. / 3.}};
+  for (int item = blockIdx.x; item < total; item += gridDim.x) {
+    int m = find_member(mem, nm, item, kListGroups);
+    ColArgs c = mem[m].c;
+    c.stage = stage;
+    c.w0 = W[stage][0], c.w1 = W[stage][1], c.w2 = W[stage][2];
+    int col = (item - mem[m].grp0) * (blockDim.x >> c.lshift) + (threadIdx.x >> c.lshift);
+    int64_t o = mem[m].off;
+    column_pcr<N>(u + o, u0 + o, du + o, c, col, sh);
+    __syncthreads();  // LDS is reused by the next item
   }
 }
 
@@ -689,10 +1083,27 @@ static int make_col_args(const snapx_config* cfg, double dt, ColArgs* out) {
   c.dt = dt;
   c.grav1 = cfg->grav1;
   c.w0 = 0., c.w1 = 1., c.w2 = 1.;
+  c.lanes = 64, c.lshift = 6;
+  while (c.lanes < c.nl) c.lanes <<= 1, c.lshift += 1;
   SNAPX_CHECK_ARG(!c.scheme || c.grav1 != 0.,
                   "[ImplicitHydro] forcing does not have const-gravity");
   *out = c;
   return 0;
+}
+
+// The cell-parallel column kernel holds whole columns in one workgroup:
+// usable when the lanes per column fit the workgroup of that block size.
+static bool pcr_ok(const ColArgs& c, int N) {
+  return c.lanes <= (N == 3 ? PcrShape<3>::kThreads : PcrShape<5>::kThreads);
+}
+static int pcr_threads(int N) { return N == 3 ? PcrShape<3>::kThreads : PcrShape<5>::kThreads; }
+static size_t pcr_smem(int N) {
+  return sizeof(double) * (size_t)pcr_threads(N) *
+         (N == 3 ? PcrShape<3>::kPub : PcrShape<5>::kPub);
+}
+static int pcr_groups(const ColArgs& c, int N) {
+  int cpw = pcr_threads(N) / c.lanes;
+  return (c.ncol + cpw - 1) / cpw;
 }
 
 struct EventPair {
@@ -730,12 +1141,21 @@ static int fused_step(const snapx_config* cfg, double dt, float* u, void* worksp
     c.stage = s;
     c.w0 = W[s][0], c.w1 = W[s][1], c.w2 = W[s][2];
     if (ev) SNAPX_HIP(hipEventRecord(ev[2 * s + 1].a, st));
-    if (N == 3)
+    if (pcr_ok(c, N)) {
+      dim3 grid(pcr_groups(c, N)), block(pcr_threads(N));
+      if (N == 3)
+        hipLaunchKernelGGL(fused_column_pcr_kernel<3>, grid, block, pcr_smem(3), st, u, u0, du,
+                           c);
+      else
+        hipLaunchKernelGGL(fused_column_pcr_kernel<5>, grid, block, pcr_smem(5), st, u, u0, du,
+                           c);
+    } else if (N == 3) {
       hipLaunchKernelGGL(fused_column_kernel<3>, dim3(cblk), dim3(kColThreads), 0, st, u, u0, du,
                          scr, c);
-    else
+    } else {
       hipLaunchKernelGGL(fused_column_kernel<5>, dim3(cblk), dim3(kColThreads), 0, st, u, u0, du,
                          scr, c);
+    }
     SNAPX_LAUNCH_CHECK();
     if (ev) SNAPX_HIP(hipEventRecord(ev[2 * s + 1].b, st));
   }
@@ -839,7 +1259,8 @@ static int batch_layout(const snapx_config* members, int nm, double dt, int tx1,
   Tile t = {tx1 > 0 ? tx1 : def.tx1, tx2 > 0 ? tx2 : def.tx2, tx3 > 0 ? tx3 : def.tx3};
   table->resize(nm);
   int64_t off = 0, soff = 0;
-  int tiles = 0, cols = 0;
+  int tiles = 0, cols = 0, groups = 0;
+  bool pcr = true;
   size_t smem = 0;
   for (int m = 0; m < nm; ++m) {
     BatchMember& b = (*table)[m];
@@ -850,16 +1271,21 @@ static int batch_layout(const snapx_config* members, int nm, double dt, int tx1,
     b.scr_off = soff;
     b.tile0 = tiles;
     b.col0 = cols;
+    b.grp0 = groups;
     Grid g = make_grid(&members[m]);
     off += 5 * (int64_t)g.ncell;
     soff += (int64_t)scratch_bytes(&members[m]) / sizeof(double);
     tiles += b.a.nt1 * b.a.nt2 * b.a.nt3;
     cols += (b.c.ncol + kColThreads - 1) / kColThreads * kColThreads;
+    int N = ((scheme >> 3) & 1) ? 5 : 3;
+    pcr &= pcr_ok(b.c, N);
+    if (pcr_ok(b.c, N)) groups += pcr_groups(b.c, N);
   }
   info->nmembers = nm;
   info->tile_x1 = t.tx1, info->tile_x2 = t.tx2, info->tile_x3 = t.tx3;
   info->total_tiles = tiles;
   info->total_cols = cols;
+  info->col_groups = pcr ? groups : 0;
   info->total_elems = off;
   info->scratch_doubles = soff;
   info->rhs_smem = smem;
@@ -926,12 +1352,22 @@ extern "C" int snapx_batch_rk3_steps(const snapx_config* members, const snapx_ba
         SNAPX_HIP(hipEventRecord(ev[0].b, st));
         SNAPX_HIP(hipEventRecord(ev[1].a, st));
       }
-      if (info->vic_size == 3)
+      if (info->col_groups > 0) {
+        int N = info->vic_size;
+        dim3 grid(std::min(info->col_groups, 4096)), block(pcr_threads(N));
+        if (N == 3)
+          hipLaunchKernelGGL(batch_column_pcr_kernel<3>, grid, block, pcr_smem(3), st, u, u0, du,
+                             table, nm, info->col_groups, s);
+        else
+          hipLaunchKernelGGL(batch_column_pcr_kernel<5>, grid, block, pcr_smem(5), st, u, u0, du,
+                             table, nm, info->col_groups, s);
+      } else if (info->vic_size == 3) {
         hipLaunchKernelGGL(batch_column_kernel<3>, dim3(col_grid), dim3(kColThreads), 0, st, u,
                            u0, du, scr, table, nm, info->total_cols, s);
-      else
+      } else {
         hipLaunchKernelGGL(batch_column_kernel<5>, dim3(col_grid), dim3(kColThreads), 0, st, u,
                            u0, du, scr, table, nm, info->total_cols, s);
+      }
       SNAPX_LAUNCH_CHECK();
       if (timing) {
         SNAPX_HIP(hipEventRecord(ev[1].b, st));
