@@ -25,6 +25,7 @@
 #include "interp_impl.hpp"
 #include "riemann_impl.hpp"
 #include "vic_impl.hpp"
+#include "pcr.hpp"
 // dual.hpp last: its overloads must not hide the double ones at the template
 // definitions above
 #include "dual.hpp"
@@ -41,15 +42,16 @@ struct AdjArgs {
   int riemann[3];
   int bc[6];
   int scheme;
+  int lanes, lshift;  // cell-parallel column kernels: lanes per column, log2
 };
 
-__device__ __forceinline__ void decode(const AdjArgs& a, int q, int& i, int& j, int& k) {
+__device__ __forceinline__ void decode(const AdjArgs a, int q, int& i, int& j, int& k) {
   i = q % a.n1;
   j = (q / a.n1) % a.n2;
   k = q / (a.n1 * a.n2);
 }
 
-__device__ __forceinline__ bool interior(const AdjArgs& a, int i, int j, int k) {
+__device__ __forceinline__ bool interior(const AdjArgs a, int i, int j, int k) {
   return (a.n1 == 1 || (i >= a.ng && i < a.n1 - a.ng)) &&
          (a.n2 == 1 || (j >= a.ng && j < a.n2 - a.ng)) &&
          (a.n3 == 1 || (k >= a.ng && k < a.n3 - a.ng));
@@ -116,7 +118,7 @@ __global__ void combine_adj_kernel(const double* __restrict__ lam, double* __res
 
 // ------------------------------------------------------------------- VIC
 
-__device__ __forceinline__ int col_q0(const AdjArgs& a, int col) {
+__device__ __forceinline__ int col_q0(const AdjArgs a, int col) {
   int ncol2 = a.n2 > 1 ? a.nx2 : 1;
   int j = a.n2 > 1 ? a.ng + col % ncol2 : 0;
   int k = a.n3 > 1 ? a.ng + col / ncol2 : 0;
@@ -320,73 +322,376 @@ __global__ void __launch_bounds__(64)
   }
 }
 
-// d(a_i delta_i + b_i delta_{i-1} + c_i delta_{i+1}) / d w_j contracted with
-// -lambda_r_i, summed over the rows i = j-1, j, j+1 that hold cell j; one
-// thread per (column, layer j), j in [-1, nl] (the ghost layers enter rows 0
-// and nl-1)
+// ---------------------------------------- cell-parallel VIC (forward+adjoint)
+
+// block rows a_i, b_i, c_i of every interior cell, abc [col][layer][3][NN]
 template <int N>
 __global__ void __launch_bounds__(kT)
-    vic_adj_rows_kernel(const double* __restrict__ w, const double* __restrict__ dl,
-                        double* __restrict__ lam_w, AdjArgs a, int ncol) {
+    vic_asm_kernel(const double* __restrict__ w, double* __restrict__ abc, AdjArgs a, int ncol) {
   const int NN = N * N;
   const int ncell = a.n1 * a.n2 * a.n3;
   const int nl = a.nx1;
   const double area = a.dx3 * a.dx2, vol = a.dx3 * (a.dx2 * a.dx1);
+  int64_t total = (int64_t)ncol * nl;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    int col = (int)(t / nl), i = (int)(t - (int64_t)col * nl);
+    int q = col_q0(a, col) + i;
+    double wm[5], w0[5], wp[5];
+    load_w(w, ncell, q - 1, wm);
+    load_w(w, ncell, q, w0);
+    load_w(w, ncell, q + 1, wp);
+    double* row = abc + t * 3 * NN;
+    vic_assemble_cell<double, N>(wm, w0, wp, a.gammad, area, area, vol, a.dt, a.grav1, i == 0,
+                                 i == nl - 1, row, row + NN, row + 2 * NN);
+  }
+}
+
+// rhs layout of the column solve: (rho, v1, p) or (rho, v1, v2, v3, p)
+template <int N>
+__device__ __forceinline__ void vic_pick(const double* x, int ncell, int q, double* r) {
+  r[0] = x[IDN * ncell + q];
+  r[1] = x[IVX * ncell + q];
+  if (N == 3) {
+    r[2] = x[IPR * ncell + q];
+  } else {
+    r[2] = x[IVY * ncell + q];
+    r[3] = x[IVZ * ncell + q];
+    r[4] = x[IPR * ncell + q];
+  }
+}
+
+// Lane geometry of the cell-parallel column kernels: lane t of the
+// workgroup serves layer i of column col.
+struct ColLane {
+  int col, i, q;
+  bool act;
+};
+
+__device__ __forceinline__ ColLane col_lane(const AdjArgs a, int ncol) {
+  ColLane L;
+  L.col = blockIdx.x * (blockDim.x >> a.lshift) + (threadIdx.x >> a.lshift);
+  L.i = threadIdx.x & (a.lanes - 1);
+  L.act = L.col < ncol && L.i < a.nx1;
+  L.q = L.act ? col_q0(a, L.col) + L.i : 0;
+  return L;
+}
+
+// row i of M (transpose = false) or of M^T (transpose = true) from abc; the
+// folded end rows carry no coupling outside the column
+template <int N>
+__device__ __forceinline__ void vic_row_blocks(const double* __restrict__ abc, int64_t col, int i,
+                                               int nl, bool transpose, double* A, double* B,
+                                               double* C) {
+  const int NN = N * N;
+  const double* row = abc + (col * nl + i) * 3 * NN;
+  if (!transpose) {
+#pragma unroll
+    for (int e = 0; e < NN; ++e) {
+      A[e] = row[e];
+      B[e] = i > 0 ? row[NN + e] : 0.;
+      C[e] = i < nl - 1 ? row[2 * NN + e] : 0.;
+    }
+    return;
+  }
+  // M^T: diagonal a_i^T, lower c_{i-1}^T, upper b_{i+1}^T
+  const double* lo = row - 3 * NN + 2 * NN;
+  const double* up = row + 3 * NN + NN;
+#pragma unroll
+  for (int r = 0; r < N; ++r)
+#pragma unroll
+    for (int c = 0; c < N; ++c) {
+      A[r * N + c] = row[c * N + r];
+      B[r * N + c] = i > 0 ? lo[c * N + r] : 0.;
+      C[r * N + c] = i < nl - 1 ? up[c * N + r] : 0.;
+    }
+}
+
+// Forward column solve + MS-VIC redistribution of one stage (the recompute
+// of the stage states): du (interior) from the explicit tendency dux.
+template <int N>
+__global__ void __launch_bounds__(PcrShape<N>::kThreads)
+    vic_fwd_solve_kernel(const double* __restrict__ w, const double* __restrict__ dux,
+                         const double* __restrict__ abc, double* __restrict__ du, AdjArgs a,
+                         int ncol) {
+  extern __shared__ double sh[];
+  const int NN = N * N;
+  const int ncell = a.n1 * a.n2 * a.n3, nl = a.nx1;
+  const double vol = a.dx3 * (a.dx2 * a.dx1);
+  ColLane L = col_lane(a, ncol);
+  double A[NN], B[NN], C[NN], r[N], d[N];
+  if (L.act) {
+    vic_row_blocks<N>(abc, L.col, L.i, nl, false, A, B, C);
+    vic_pick<N>(dux, ncell, L.q, r);
+#pragma unroll
+    for (int e = 0; e < N; ++e) r[e] /= a.dt;
+  }
+  pcr_solve<N>(A, B, C, r, d, sh, blockDim.x, threadIdx.x, L.i, nl, L.act);
+  double rho = L.act ? w[IDN * ncell + L.q] : 0.;
+  double xr = L.act ? dux[IDN * ncell + L.q] : 0.;
+  double ai = rho * vol;
+  double sums[2] = {ai * ai, L.act ? (xr - d[0]) * 1.0 * vol : 0.};
+  column_sums<2>(sums, sh, threadIdx.x, a.lanes, a.lshift);
+  if (!L.act) return;
+  du[IDN * ncell + L.q] = (xr + (d[0] - xr)) + rho * ai * sums[1] / sums[0];
+  du[IVX * ncell + L.q] = d[1];
+  if (N == 3) {
+    du[IPR * ncell + L.q] = d[2];
+  } else {
+    du[IVY * ncell + L.q] = d[2];
+    du[IVZ * ncell + L.q] = d[3];
+    du[IPR * ncell + L.q] = d[4];
+  }
+}
+
+// Adjoint of the column solve + redistribution: delta = M^-1 (dux / dt),
+// redistribution adjoint (column sums S, Bd, G), y = M^-T lambda_delta.
+// Writes dl [col][layer][2N] = (delta, y), lam_dx (interior) and the rho
+// part of lam_w; lam_dx enters holding lam_du.
+template <int N>
+__global__ void __launch_bounds__(PcrShape<N>::kThreads)
+    vic_adj_solve_kernel(const double* __restrict__ w, const double* __restrict__ dux,
+                         const double* __restrict__ lam_du, const double* __restrict__ abc,
+                         double* __restrict__ lam_dx, double* __restrict__ lam_w,
+                         double* __restrict__ dl, AdjArgs a, int ncol) {
+  extern __shared__ double sh[];
+  const int NN = N * N;
+  const int ncell = a.n1 * a.n2 * a.n3, nl = a.nx1;
+  const double vol = a.dx3 * (a.dx2 * a.dx1);
+  ColLane L = col_lane(a, ncol);
+  double A[NN], B[NN], C[NN], r[N], d[N];
+  if (L.act) {
+    vic_row_blocks<N>(abc, L.col, L.i, nl, false, A, B, C);
+    vic_pick<N>(dux, ncell, L.q, r);
+#pragma unroll
+    for (int e = 0; e < N; ++e) r[e] /= a.dt;
+  }
+  pcr_solve<N>(A, B, C, r, d, sh, blockDim.x, threadIdx.x, L.i, nl, L.act);
+  double rho = L.act ? w[IDN * ncell + L.q] : 0.;
+  double lr = L.act ? lam_du[IDN * ncell + L.q] : 0.;
+  double ai = rho * vol;
+  double sums[3] = {ai * ai, L.act ? (dux[IDN * ncell + L.q] - d[0]) * 1.0 * vol : 0.,
+                    lr * rho * ai};
+  column_sums<3>(sums, sh, threadIdx.x, a.lanes, a.lshift);
+  const double S = sums[0], Bd = sums[1], G = sums[2] / S;
+  const double lamS = -G * Bd / S;
+  double ld[N];
+  if (L.act) {
+    vic_pick<N>(lam_du, ncell, L.q, ld);
+    ld[0] = lr - G * vol;
+    lam_w[IDN * ncell + L.q] += lr * 2. * rho * vol * Bd / S + lamS * 2. * rho * vol * vol;
+    vic_row_blocks<N>(abc, L.col, L.i, nl, true, A, B, C);
+  }
+  double y[N];
+  pcr_solve<N>(A, B, C, ld, y, sh, blockDim.x, threadIdx.x, L.i, nl, L.act);
+  if (!L.act) return;
+  double* cdl = dl + ((int64_t)L.col * nl + L.i) * 2 * N;
+#pragma unroll
+  for (int e = 0; e < N; ++e) {
+    cdl[e] = d[e];
+    cdl[N + e] = y[e];
+  }
+  lam_dx[IDN * ncell + L.q] = G * vol + y[0] / a.dt;
+  lam_dx[IVX * ncell + L.q] = y[1] / a.dt;
+  if (N == 3) {
+    lam_dx[IPR * ncell + L.q] = y[2] / a.dt;
+  } else {
+    lam_dx[IVY * ncell + L.q] = y[2] / a.dt;
+    lam_dx[IVZ * ncell + L.q] = y[3] / a.dt;
+    lam_dx[IPR * ncell + L.q] = y[4] / a.dt;
+  }
+}
+
+// delta and y of layer k extended past the column ends: the folded end rows
+// act as delta_{-1} = Bnd delta_0, delta_nl = Bnd delta_{nl-1}, with no row
+// (y = 0) outside [0, nl)
+template <int N>
+__device__ __forceinline__ void ext_delta_y(const double* cdl, int k, int nl, double* d,
+                                            double* y) {
+  int kk = k < 0 ? 0 : (k >= nl ? nl - 1 : k);
+  bool out = kk != k;
+#pragma unroll
+  for (int e = 0; e < N; ++e) {
+    d[e] = cdl[kk * 2 * N + e] * ((out && e == 1) ? -1. : 1.);
+    y[e] = out ? 0. : cdl[kk * 2 * N + N + e];
+  }
+}
+
+// Roe-averaged (r, u, v, w, p) of an interface, the arithmetic of
+// abs_jacobian's averaging step
+template <typename T>
+__device__ __forceinline__ void roe_state(T gm1, const T* wl, const T* wr, T* z) {
+  T sqrtdl = sqrt(wl[IDN]);
+  T sqrtdr = sqrt(wr[IDN]);
+  T isdlpdr = T(1.0) / (sqrtdl + sqrtdr);
+  z[0] = sqrtdl * sqrtdr;
+  z[1] = (sqrtdl * wl[IVX] + sqrtdr * wr[IVX]) * isdlpdr;
+  z[2] = (sqrtdl * wl[IVY] + sqrtdr * wr[IVY]) * isdlpdr;
+  z[3] = (sqrtdl * wl[IVZ] + sqrtdr * wr[IVZ]) * isdlpdr;
+  T el = wl[IPR] / gm1 + T(0.5) * wl[IDN] * ((sq(wl[IVX]) + sq(wl[IVY])) + sq(wl[IVZ]));
+  T er = wr[IPR] / gm1 + T(0.5) * wr[IDN] * ((sq(wr[IVX]) + sq(wr[IVY])) + sq(wr[IVZ]));
+  T hbar = ((el + wl[IPR]) / sqrtdl + (er + wr[IPR]) / sqrtdr) * isdlpdr;
+  z[4] = (hbar - T(0.5) * ((sq(z[1]) + sq(z[2])) + sq(z[3]))) * gm1 / (gm1 + T(1.)) * z[0];
+}
+
+// P^T |A| Q with |A| = R |Lambda| R^-1 at the Roe state z (the matrices of
+// abs_jacobian, contracted without forming them); P, Q are 5-vectors
+template <typename T>
+__device__ __forceinline__ T abs_contract(T gm1, const T* z, const double* P, const double* Q) {
+  T r = z[0], u = z[1], v = z[2], w = z[3], p = z[4];
+  T cs = sqrt(p * (gm1 + T(1.)) / r);
+  T ke = T(0.5) * ((u * u + v * v) + w * w);
+  T hp = (gm1 + T(1.)) / gm1 * p / r;
+  T h = hp + ke;
+  T lam[5] = {fabs(u - cs), fabs(u), fabs(u + cs), fabs(u), fabs(u)};
+  T base = (T(P[0]) + T(P[2]) * v) + T(P[3]) * w;
+  T PR[5] = {base + T(P[1]) * (u - cs) + T(P[4]) * (h - u * cs),
+             base + T(P[1]) * u + T(P[4]) * ke,
+             base + T(P[1]) * (u + cs) + T(P[4]) * (h + u * cs),
+             T(P[2]) + T(P[4]) * v, T(P[3]) + T(P[4]) * w};
+  T two_cs_hp = T(2.) * cs * hp, two_hp = T(2.) * hp;
+  T tang = (-v * T(Q[2]) - w * T(Q[3]) + T(Q[4])) / two_hp;
+  T RQ[5] = {((cs * ke + u * hp) * T(Q[0]) + (-hp - cs * u) * T(Q[1])) / two_cs_hp + tang,
+             ((hp - ke) * T(Q[0]) + u * T(Q[1]) + v * T(Q[2]) + w * T(Q[3]) - T(Q[4])) / hp,
+             ((cs * ke - u * hp) * T(Q[0]) + (hp - cs * u) * T(Q[1])) / two_cs_hp + tang,
+             -v * T(Q[0]) + T(Q[2]), -w * T(Q[0]) + T(Q[3])};
+  T phi = T(0.);
+#pragma unroll
+  for (int k = 0; k < 5; ++k) phi += PR[k] * lam[k] * RQ[k];
+  return phi;
+}
+
+// Face f (between layers f-1 and f, f in [0, nl]) of the row-derivative
+// contraction: phi_f = (y_{f-1} - y_f)^T |A|_f (delta_{f-1} - delta_f);
+// G[col][f][0..4] = d phi / d w_{f-1}, G[col][f][5..9] = d phi / d w_f.
+// The Roe state is differentiated with Dual<10>, phi(z) with Dual<5>.
+template <int N>
+__global__ void __launch_bounds__(kT)
+    vic_adj_face_kernel(const double* __restrict__ w, const double* __restrict__ dl,
+                        double* __restrict__ G, AdjArgs a, int ncol) {
+  const int ncell = a.n1 * a.n2 * a.n3;
+  const int nl = a.nx1;
+  const double gm1 = a.gammad - 1.;
+  int64_t total = (int64_t)ncol * (nl + 1);
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    int col = (int)(t / (nl + 1)), f = (int)(t - (int64_t)col * (nl + 1));
+    const double* cdl = dl + (int64_t)col * nl * 2 * N;
+    double dm[N], ym[N], dp[N], yp[N];
+    ext_delta_y<N>(cdl, f - 1, nl, dm, ym);
+    ext_delta_y<N>(cdl, f, nl, dp, yp);
+    double P[5] = {0., 0., 0., 0., 0.}, Q[5] = {0., 0., 0., 0., 0.};
+#pragma unroll
+    for (int e = 0; e < N; ++e) {
+      P[vic_row(N, e)] = ym[e] - yp[e];
+      Q[vic_row(N, e)] = dm[e] - dp[e];
+    }
+    int q = col_q0(a, col) + f;  // the cell above the face
+    Dual<10> wl[5], wr[5], z[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      wl[v] = Dual<10>::seed(w[v * ncell + q - 1], v);
+      wr[v] = Dual<10>::seed(w[v * ncell + q], 5 + v);
+    }
+    roe_state<Dual<10>>(Dual<10>(gm1), wl, wr, z);
+    Dual<5> zs[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) zs[k] = Dual<5>::seed(z[k].v, k);
+    Dual<5> phi = abs_contract<Dual<5>>(Dual<5>(gm1), zs, P, Q);
+    double* g = G + t * 10;
+#pragma unroll
+    for (int m = 0; m < 10; ++m) {
+      double s = 0.;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) s += phi.d[k] * z[k].d[m];
+      g[m] = s;
+    }
+  }
+}
+
+// Cell j in [-1, nl] of a column: lam_w[j] -= (area / 2V) d/dw_j of
+//   phi_j + phi_{j+1} + (y_{j-1} - y_{j+1})^T J(w_j) delta_j
+// (the faces below / above and the flux Jacobian of the cell, which enters
+// row j+1 as its lower and row j-1 as its upper coupling).
+template <int N>
+__global__ void __launch_bounds__(kT)
+    vic_adj_cell_kernel(const double* __restrict__ w, const double* __restrict__ dl,
+                        const double* __restrict__ G, double* __restrict__ lam_w, AdjArgs a,
+                        int ncol) {
+  const int NN = N * N;
+  const int ncell = a.n1 * a.n2 * a.n3;
+  const int nl = a.nx1;
+  const double ah = a.dx3 * a.dx2 * (0.5 / (a.dx3 * (a.dx2 * a.dx1)));
   int64_t total = (int64_t)ncol * (nl + 2);
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
        t += (int64_t)gridDim.x * blockDim.x) {
-    int col = (int)(t / (nl + 2));
-    int j = (int)(t - (int64_t)col * (nl + 2)) - 1;
-    int q0 = col_q0(a, col);
+    int col = (int)(t / (nl + 2)), j = (int)(t - (int64_t)col * (nl + 2)) - 1;
     const double* cdl = dl + (int64_t)col * nl * 2 * N;
+    const double* gc = G + (int64_t)col * (nl + 1) * 10;
     double acc[5] = {0., 0., 0., 0., 0.};
-    for (int i = j - 1; i <= j + 1; ++i) {
-      if (i < 0 || i >= nl) continue;
-      Dual<5> wm[5], w0[5], wp[5];
-      load_w(w, ncell, q0 + i - 1, wm);
-      load_w(w, ncell, q0 + i, w0);
-      load_w(w, ncell, q0 + i + 1, wp);
-      Dual<5>* seeded = (j == i - 1) ? wm : (j == i ? w0 : wp);
-      for (int v = 0; v < 5; ++v) seeded[v] = Dual<5>::seed(seeded[v].v, v);
-      Dual<5> A[NN], B[NN], C[NN];
-      vic_assemble_cell<Dual<5>, N>(wm, w0, wp, Dual<5>(a.gammad), Dual<5>(area),
-                                    Dual<5>(area), Dual<5>(vol), Dual<5>(a.dt),
-                                    Dual<5>(a.grav1), i == 0, i == nl - 1, A, B, C);
-      const double* d0 = cdl + i * 2 * N;
-      const double* y = cdl + i * 2 * N + N;
-      for (int r = 0; r < N; ++r) {
-        Dual<5> row = Dual<5>(0.);
-        for (int c = 0; c < N; ++c) {
-          row += A[r * N + c] * Dual<5>(d0[c]);
-          if (i > 0) row += B[r * N + c] * Dual<5>(cdl[(i - 1) * 2 * N + c]);
-          if (i < nl - 1) row += C[r * N + c] * Dual<5>(cdl[(i + 1) * 2 * N + c]);
-        }
-        for (int v = 0; v < 5; ++v) acc[v] -= y[r] * row.d[v];
-      }
-    }
-    int q = q0 + j;
-    for (int v = 0; v < 5; ++v) lam_w[v * ncell + q] += acc[v];
+    if (j >= 0)
+#pragma unroll
+      for (int v = 0; v < 5; ++v) acc[v] += gc[j * 10 + 5 + v];
+    if (j + 1 <= nl)
+#pragma unroll
+      for (int v = 0; v < 5; ++v) acc[v] += gc[(j + 1) * 10 + v];
+    double dj[N], yj[N], dlo[N], ylo[N], dhi[N], yhi[N];
+    ext_delta_y<N>(cdl, j, nl, dj, yj);
+    ext_delta_y<N>(cdl, j - 1, nl, dlo, ylo);
+    ext_delta_y<N>(cdl, j + 1, nl, dhi, yhi);
+    int q = col_q0(a, col) + j;
+    Dual<5> wj[5], J[NN];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) wj[v] = Dual<5>::seed(w[v * ncell + q], v);
+    flux_jacobian<Dual<5>, N>(Dual<5>(a.gammad - 1.), wj, J);
+    Dual<5> psi = Dual<5>(0.);
+#pragma unroll
+    for (int r = 0; r < N; ++r)
+#pragma unroll
+      for (int c = 0; c < N; ++c) psi += J[r * N + c] * Dual<5>((ylo[r] - yhi[r]) * dj[c]);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) lam_w[v * ncell + q] -= ah * (acc[v] + psi.d[v]);
   }
 }
 
 // ------------------------------------------------------- explicit part
 
+// Slot v of the face frame of axis d holds variable vmap[v]: density,
+// normal velocity, the two tangential velocities in cyclic order, pressure.
+// Kernels that work in this frame index their register arrays by constants
+// only (a runtime slot index would put the arrays in scratch memory) and
+// move data in and out of the frame through memory addresses.
+__device__ __forceinline__ void face_frame(int d, int* vmap) {
+  vmap[0] = IDN;
+  vmap[1] = IVX + d;
+  vmap[2] = IVX + (d + 1) % 3;
+  vmap[3] = IVX + (d + 2) % 3;
+  vmap[4] = IPR;
+}
+
 // interface states of face q along the axis with stride s (face f along the
-// axis); the same arithmetic as recon_kernel for the faces whose adjoint is
-// nonzero (f in [ng, n - ng]: never a dummy face)
+// axis) in the face frame; the same arithmetic as recon_kernel for the faces
+// whose adjoint is nonzero (f in [ng, n - ng]: never a dummy face)
 template <typename T>
 __device__ __forceinline__ void face_states(const double* w, int ncell, int q, int s, int f,
-                                            const AdjArgs& a, int d, T* wl, T* wr) {
+                                            const AdjArgs a, int d, const int* vmap, T* wl,
+                                            T* wr) {
   int r = d == 0 ? 0 : 1;
-  int type = a.recon_type[r], scale = a.recon_scale[r], shock = a.recon_shock[r];
+  // selects, not a runtime index into the argument struct (which would
+  // place it in scratch memory)
+  int type = r ? a.recon_type[1] : a.recon_type[0];
+  int scale = r ? a.recon_scale[1] : a.recon_scale[0];
+  int shock = r ? a.recon_shock[1] : a.recon_shock[0];
+#pragma unroll
   for (int v = 0; v < 5; ++v) {
     int tv = (!shock && v >= IVX && v <= IPR) ? interp2_type(type) : type;
     T win[7];
+#pragma unroll
     for (int m = -3; m <= 3; ++m) {
       int c = f + m;
       int n = d == 0 ? a.n1 : (d == 1 ? a.n2 : a.n3);
       c = c < 0 ? 0 : (c >= n ? n - 1 : c);
-      win[m + 3] = T(w[v * ncell + q + (c - f) * s]);
+      win[m + 3] = T(w[vmap[v] * ncell + q + (c - f) * s]);
     }
     // window centred on cell f (index 3) for the right state and cell f-1
     // (index 2) for the left state
@@ -395,14 +700,14 @@ __device__ __forceinline__ void face_states(const double* w, int ncell, int q, i
   }
 }
 
-__device__ __forceinline__ int axis_len(const AdjArgs& a, int d) {
+__device__ __forceinline__ int axis_len(const AdjArgs a, int d) {
   return d == 0 ? a.n1 : (d == 1 ? a.n2 : a.n3);
 }
-__device__ __forceinline__ int axis_stride(const AdjArgs& a, int d) {
+__device__ __forceinline__ int axis_stride(const AdjArgs a, int d) {
   return d == 0 ? 1 : (d == 1 ? a.n1 : a.n1 * a.n2);
 }
 
-// lambda_F at each face, Riemann adjoint (Dual<10>), x1 LR-revision adjoint;
+// lambda_F at each face, Riemann adjoint (Dual<1> per input), x1 LR-revision adjoint;
 // writes lam_wlr [2][5][ncell] (ILT, IRT) for axis d
 __global__ void __launch_bounds__(kT)
     flux_adj_kernel(const double* __restrict__ w, const double* __restrict__ lam_dx,
@@ -412,23 +717,33 @@ __global__ void __launch_bounds__(kT)
   double area = d == 0 ? a.dx3 * a.dx2 : (d == 1 ? a.dx3 * a.dx1 : a.dx2 * a.dx1);
   double vol = a.dx3 * (a.dx2 * a.dx1);
   double coef = -a.dt * area / vol;
+  const int solver = d == 0 ? a.riemann[0] : (d == 1 ? a.riemann[1] : a.riemann[2]);
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < ncell; q += gridDim.x * blockDim.x) {
     int i, j, k;
     decode(a, q, i, j, k);
     int f = d == 0 ? i : (d == 1 ? j : k);
+    // lambda_F in the face frame
+    int vmap[5];
+    face_frame(d, vmap);
     double lf[5] = {0., 0., 0., 0., 0.};
     bool any = false;
     if (f >= 1) {
       int il = i - (d == 0), jl = j - (d == 1), kl = k - (d == 2);
-      if (interior(a, il, jl, kl))
-        for (int v = 0; v < 5; ++v) lf[v] += coef * lam_dx[v * ncell + q - s], any = true;
+      if (interior(a, il, jl, kl)) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) lf[v] += coef * lam_dx[vmap[v] * ncell + q - s];
+        any = true;
+      }
     }
-    if (interior(a, i, j, k))
-      for (int v = 0; v < 5; ++v) lf[v] -= coef * lam_dx[v * ncell + q], any = true;
+    if (interior(a, i, j, k)) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) lf[v] -= coef * lam_dx[vmap[v] * ncell + q];
+      any = true;
+    }
     double lwl[5] = {0., 0., 0., 0., 0.}, lwr[5] = {0., 0., 0., 0., 0.};
     if (any && f >= a.ng && f <= n - a.ng) {
-      Dual<10> wl[5], wr[5];
-      face_states<Dual<10>>(w, ncell, q, s, f, a, d, wl, wr);
+      double wl[5], wr[5];
+      face_states<double>(w, ncell, q, s, f, a, d, vmap, wl, wr);
       bool rev = d == 0 && a.grav1 != 0.;
       if (rev && f == a.ng) {
         wl[IDN] = wr[IDN];
@@ -438,19 +753,29 @@ __global__ void __launch_bounds__(kT)
         wr[IDN] = wl[IDN];
         wr[IPR] = wl[IPR];
       }
-      // re-seed on the (revised) face states: derivative w.r.t. wl / wr
-      Dual<10> sl[5], sr[5];
-      for (int v = 0; v < 5; ++v) {
-        sl[v] = Dual<10>::seed(wl[v].v, v);
-        sr[v] = Dual<10>::seed(wr[v].v, 5 + v);
-      }
-      Dual<10> flx[5];
-      riemann_flux<Dual<10>>(a.riemann[d], sl, sr, Dual<10>(a.gammad), 3 - d, flx, nullptr);
-      for (int v = 0; v < 5; ++v)
-        for (int m = 0; m < 5; ++m) {
-          lwl[v] += lf[m] * flx[m].d[v];
-          lwr[v] += lf[m] * flx[m].d[5 + v];
+      // J^T lambda_F one input direction at a time (Dual<1>): the Roe solver
+      // on wider duals does not fit the register file
+#pragma unroll 1
+      for (int kk = 0; kk < 10; ++kk) {
+        Dual<1> sl[5], sr[5], flx[5];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+          sl[v] = Dual<1>(wl[v]);
+          sr[v] = Dual<1>(wr[v]);
+          sl[v].d[0] = kk == v ? 1. : 0.;
+          sr[v].d[0] = kk == 5 + v ? 1. : 0.;
         }
+        // x1 frame: the solver's velocity slots are constants
+        riemann_flux<Dual<1>>(solver, sl, sr, Dual<1>(a.gammad), 3, flx, (Dual<1>*)nullptr);
+        double acc = 0.;
+#pragma unroll
+        for (int m = 0; m < 5; ++m) acc += lf[m] * flx[m].d[0];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+          if (kk == v) lwl[v] += acc;
+          if (kk == 5 + v) lwr[v] += acc;
+        }
+      }
       // LR revision adjoint: the copied states take the adjoint of the copies
       if (rev && f == a.ng) {
         lwr[IDN] += lwl[IDN], lwl[IDN] = 0.;
@@ -461,9 +786,10 @@ __global__ void __launch_bounds__(kT)
         lwl[IPR] += lwr[IPR], lwr[IPR] = 0.;
       }
     }
+#pragma unroll
     for (int v = 0; v < 5; ++v) {
-      lam_wlr[v * ncell + q] = lwl[v];
-      lam_wlr[(5 + v) * ncell + q] = lwr[v];
+      lam_wlr[vmap[v] * ncell + q] = lwl[v];
+      lam_wlr[(5 + vmap[v]) * ncell + q] = lwr[v];
     }
   }
 }
@@ -475,7 +801,11 @@ __global__ void __launch_bounds__(kT)
   const int ncell = a.n1 * a.n2 * a.n3;
   const int s = axis_stride(a, d), n = axis_len(a, d);
   int r = d == 0 ? 0 : 1;
-  int type = a.recon_type[r], scale = a.recon_scale[r], shock = a.recon_shock[r];
+  // selects, not a runtime index into the argument struct (which would
+  // place it in scratch memory)
+  int type = r ? a.recon_type[1] : a.recon_type[0];
+  int scale = r ? a.recon_scale[1] : a.recon_scale[0];
+  int shock = r ? a.recon_shock[1] : a.recon_shock[0];
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < 5 * ncell;
        t += gridDim.x * blockDim.x) {
     int v = t / ncell, q = t - v * ncell;
@@ -616,14 +946,16 @@ static AdjArgs adj_args(const snapx_config* c, double dt) {
   for (int d = 0; d < 3; ++d) a.riemann[d] = c->riemann[d];
   for (int b = 0; b < 6; ++b) a.bc[b] = c->bc[b];
   a.scheme = c->implicit_scheme;
+  a.lanes = 64, a.lshift = 6;
+  while (a.lanes < c->nx1) a.lanes <<= 1, a.lshift += 1;
   return a;
 }
 
 static size_t al(size_t x) { return (x + 255) & ~size_t(255); }
 
 struct AdjSpace {
-  double *us[3], *w, *dux, *lam, *lam_us, *lam_u0, *lam_du, *lam_dx, *lam_w, *lam_wlr, *abc,
-      *pr, *dl;
+  double *us[3], *w[3], *dux[3], *lam, *lam_us, *lam_u0, *lam_du, *lam_dx, *lam_w, *lam_wlr,
+      *abc, *pr, *dl, *G;
   char* hydro_ws;
   size_t hydro_bytes, bytes;
 };
@@ -641,9 +973,11 @@ static AdjSpace adj_layout(const snapx_config* c, char* base) {
     off += al(b);
     return reinterpret_cast<double*>(p);
   };
-  for (int k = 0; k < 3; ++k) s.us[k] = take(f);
-  s.w = take(f);
-  s.dux = take(f);
+  for (int k = 0; k < 3; ++k) {
+    s.us[k] = take(f);
+    s.w[k] = take(f);
+    s.dux[k] = take(f);
+  }
   s.lam = take(f);
   s.lam_us = take(f);
   s.lam_u0 = take(f);
@@ -654,13 +988,14 @@ static AdjSpace adj_layout(const snapx_config* c, char* base) {
   s.abc = take(sizeof(double) * cells * 3 * N * N);
   s.pr = take(sizeof(double) * cells * (N * N + N));
   s.dl = take(sizeof(double) * cells * 2 * N);
+  s.G = take(sizeof(double) * ncol * (c->nx1 + 1) * 10);
   s.hydro_bytes = snapx_hydro_workspace_bytes(c, SNAPX_F64);
   s.hydro_ws = reinterpret_cast<char*>(take(s.hydro_bytes));
   s.bytes = off;
   return s;
 }
 
-static int bc_adj(const snapx_config* c, const AdjArgs& a, double* lam, hipStream_t st) {
+static int bc_adj(const snapx_config* c, const AdjArgs a, double* lam, hipStream_t st) {
   int dims[3] = {3, 2, 1};
   int ns[3] = {a.n1, a.n2, a.n3};
   for (int ax = 2; ax >= 0; --ax) {  // reverse of the forward x1, x2, x3 order
@@ -684,6 +1019,54 @@ extern "C" size_t snapx_adjoint_workspace_bytes(const snapx_config* cfg) {
   return adj_layout(cfg, nullptr).bytes;
 }
 
+namespace snapx {
+
+static bool adj_pcr(const AdjArgs a, int N) {
+  return a.lanes <= (N == 3 ? PcrShape<3>::kThreads : PcrShape<5>::kThreads);
+}
+
+template <int N>
+static int launch_vic_fwd(const AdjArgs a, const AdjSpace& s, int k, double* du, int ncol,
+                          hipStream_t st) {
+  const int T = PcrShape<N>::kThreads;
+  hipLaunchKernelGGL(vic_asm_kernel<N>, blocks_for((int64_t)ncol * a.nx1, kT), dim3(kT), 0, st,
+                     s.w[k], s.abc, a, ncol);
+  SNAPX_LAUNCH_CHECK();
+  int cpw = T >> a.lshift;
+  hipLaunchKernelGGL(vic_fwd_solve_kernel<N>, dim3((ncol + cpw - 1) / cpw), dim3(T),
+                     sizeof(double) * T * PcrShape<N>::kPub, st, s.w[k], s.dux[k], s.abc, du, a,
+                     ncol);
+  SNAPX_LAUNCH_CHECK();
+  return 0;
+}
+
+template <int N>
+static int launch_vic_adj(const AdjArgs a, const AdjSpace& s, int k, int ncol, hipStream_t st) {
+  const int T = PcrShape<N>::kThreads;
+  if (adj_pcr(a, N)) {
+    hipLaunchKernelGGL(vic_asm_kernel<N>, blocks_for((int64_t)ncol * a.nx1, kT), dim3(kT), 0,
+                       st, s.w[k], s.abc, a, ncol);
+    SNAPX_LAUNCH_CHECK();
+    int cpw = T >> a.lshift;
+    hipLaunchKernelGGL(vic_adj_solve_kernel<N>, dim3((ncol + cpw - 1) / cpw), dim3(T),
+                       sizeof(double) * T * PcrShape<N>::kPub, st, s.w[k], s.dux[k], s.lam_du,
+                       s.abc, s.lam_dx, s.lam_w, s.dl, a, ncol);
+  } else {
+    hipLaunchKernelGGL(vic_adj_column_kernel<N>, blocks_for(ncol, 64), dim3(64), 0, st, s.w[k],
+                       s.dux[k], s.lam_du, s.lam_dx, s.lam_w, s.abc, s.pr, s.dl, a, ncol);
+  }
+  SNAPX_LAUNCH_CHECK();
+  hipLaunchKernelGGL(vic_adj_face_kernel<N>, blocks_for((int64_t)ncol * (a.nx1 + 1), kT),
+                     dim3(kT), 0, st, s.w[k], s.dl, s.G, a, ncol);
+  SNAPX_LAUNCH_CHECK();
+  hipLaunchKernelGGL(vic_adj_cell_kernel<N>, blocks_for((int64_t)ncol * (a.nx1 + 2), kT),
+                     dim3(kT), 0, st, s.w[k], s.dl, s.G, s.lam_w, a, ncol);
+  SNAPX_LAUNCH_CHECK();
+  return 0;
+}
+
+}  // namespace snapx
+
 extern "C" int snapx_adjoint_rk3_step(const snapx_config* cfg, double dt, const double* u_in,
                                       double* lambda, void* workspace, size_t workspace_bytes,
                                       void* stream) {
@@ -705,15 +1088,34 @@ extern "C" int snapx_adjoint_rk3_step(const snapx_config* cfg, double dt, const 
   static const double W[3][3] = {
       {0., 1., 1.}, {3. / 4., 1. / 4., 1. / 4.}, {1. / 3., 2. / 3., 2. / 3.}};
   dim3 bn = blocks_for(n, kT), bc = blocks_for(g.ncell, kT);
+  const int N = ((cfg->implicit_scheme >> 3) & 1) ? 5 : 3;
+  const int ncol = (g.n2 > 1 ? cfg->nx2 : 1) * (g.n3 > 1 ? cfg->nx3 : 1);
+  const int ns[3] = {g.n1, g.n2, g.n3};
 
-  // ---- forward recompute of the stage inputs ----
+  // ---- forward recompute: stage inputs, their primitives (revised ghosts)
+  //      and explicit tendencies ----
   SNAPX_HIP(hipMemcpyAsync(s.us[0], u_in, fb, hipMemcpyDeviceToDevice, st));
-  for (int k = 0; k < 2; ++k) {
-    if (int e = snapx_hydro_forward(cfg, dt, s.us[k], s.w, s.lam_du, nullptr, nullptr, nullptr,
-                                    s.hydro_ws, s.hydro_bytes, SNAPX_F64, st))
+  for (int k = 0; k < 3; ++k) {
+    if (int e = snapx_hydro_forward(&c0, dt, s.us[k], s.w[k], s.dux[k], nullptr, nullptr,
+                                    nullptr, s.hydro_ws, s.hydro_bytes, SNAPX_F64, st))
       return e;
-    if (int e = snapx_rk_combine(s.us[k + 1], s.us[0], s.us[k], s.lam_du, W[k][0], W[k][1],
-                                 W[k][2], n, SNAPX_F64, st))
+    if (k == 2) break;
+    // du of the stage: the column solve on a copy of the explicit tendency
+    double* du = s.lam_du;
+    SNAPX_HIP(hipMemcpyAsync(du, s.dux[k], fb, hipMemcpyDeviceToDevice, st));
+    if (cfg->implicit_scheme) {
+      if (adj_pcr(a, N)) {
+        if (int e = N == 3 ? launch_vic_fwd<3>(a, s, k, du, ncol, st)
+                           : launch_vic_fwd<5>(a, s, k, du, ncol, st))
+          return e;
+      } else if (int e = snapx_hydro_forward(cfg, dt, s.us[k], s.lam_w, du, nullptr, nullptr,
+                                             nullptr, s.hydro_ws, s.hydro_bytes, SNAPX_F64,
+                                             st)) {
+        return e;
+      }
+    }
+    if (int e = snapx_rk_combine(s.us[k + 1], s.us[0], s.us[k], du, W[k][0], W[k][1], W[k][2],
+                                 n, SNAPX_F64, st))
       return e;
     if (int e = snapx_apply_bc(cfg, s.us[k + 1], 5, SNAPX_CONSERVED, SNAPX_F64, st)) return e;
   }
@@ -722,18 +1124,11 @@ extern "C" int snapx_adjoint_rk3_step(const snapx_config* cfg, double dt, const 
   SNAPX_HIP(hipMemcpyAsync(s.lam, lambda, fb, hipMemcpyDeviceToDevice, st));
   hipLaunchKernelGGL(zero_kernel, bn, dim3(kT), 0, st, s.lam_u0, n);
   SNAPX_LAUNCH_CHECK();
-  int N = ((cfg->implicit_scheme >> 3) & 1) ? 5 : 3;
-  int ncol = (g.n2 > 1 ? cfg->nx2 : 1) * (g.n3 > 1 ? cfg->nx3 : 1);
-  int ns[3] = {g.n1, g.n2, g.n3};
   for (int stg = 2; stg >= 0; --stg) {
     if (int e = bc_adj(cfg, a, s.lam, st)) return e;
     hipLaunchKernelGGL(combine_adj_kernel, bn, dim3(kT), 0, st, s.lam, s.lam_du, s.lam_us,
                        s.lam_u0, W[stg][0], W[stg][1], W[stg][2], n);
     SNAPX_LAUNCH_CHECK();
-    // primitives (revised ghosts) and the explicit tendency of this stage
-    if (int e = snapx_hydro_forward(&c0, dt, s.us[stg], s.w, s.dux, nullptr, nullptr, nullptr,
-                                    s.hydro_ws, s.hydro_bytes, SNAPX_F64, st))
-      return e;
     hipLaunchKernelGGL(zero_kernel, bn, dim3(kT), 0, st, s.lam_w, n);
     hipLaunchKernelGGL(zero_kernel, bn, dim3(kT), 0, st, s.lam_dx, n);
     SNAPX_LAUNCH_CHECK();
@@ -741,33 +1136,22 @@ extern "C" int snapx_adjoint_rk3_step(const snapx_config* cfg, double dt, const 
     // gravity forcing) pass through
     hipLaunchKernelGGL(axpy_kernel, bn, dim3(kT), 0, st, s.lam_dx, s.lam_du, n);
     SNAPX_LAUNCH_CHECK();
-    if (cfg->implicit_scheme) {
-      if (N == 3) {
-        hipLaunchKernelGGL(vic_adj_column_kernel<3>, blocks_for(ncol, 64), dim3(64), 0, st, s.w,
-                           s.dux, s.lam_du, s.lam_dx, s.lam_w, s.abc, s.pr, s.dl, a, ncol);
-        SNAPX_LAUNCH_CHECK();
-        hipLaunchKernelGGL(vic_adj_rows_kernel<3>, blocks_for((int64_t)ncol * (cfg->nx1 + 2), kT),
-                           dim3(kT), 0, st, s.w, s.dl, s.lam_w, a, ncol);
-      } else {
-        hipLaunchKernelGGL(vic_adj_column_kernel<5>, blocks_for(ncol, 64), dim3(64), 0, st, s.w,
-                           s.dux, s.lam_du, s.lam_dx, s.lam_w, s.abc, s.pr, s.dl, a, ncol);
-        SNAPX_LAUNCH_CHECK();
-        hipLaunchKernelGGL(vic_adj_rows_kernel<5>, blocks_for((int64_t)ncol * (cfg->nx1 + 2), kT),
-                           dim3(kT), 0, st, s.w, s.dl, s.lam_w, a, ncol);
-      }
-      SNAPX_LAUNCH_CHECK();
-    }
-    hipLaunchKernelGGL(grav_adj_kernel, bc, dim3(kT), 0, st, s.w, s.lam_dx, s.lam_w, a);
+    if (cfg->implicit_scheme)
+      if (int e = N == 3 ? launch_vic_adj<3>(a, s, stg, ncol, st)
+                         : launch_vic_adj<5>(a, s, stg, ncol, st))
+        return e;
+    const double* w = s.w[stg];
+    hipLaunchKernelGGL(grav_adj_kernel, bc, dim3(kT), 0, st, w, s.lam_dx, s.lam_w, a);
     SNAPX_LAUNCH_CHECK();
     for (int d = 0; d < 3; ++d) {
       if (ns[d] == 1) continue;
-      hipLaunchKernelGGL(flux_adj_kernel, bc, dim3(kT), 0, st, s.w, s.lam_dx, s.lam_wlr, a, d);
+      hipLaunchKernelGGL(flux_adj_kernel, bc, dim3(kT), 0, st, w, s.lam_dx, s.lam_wlr, a, d);
       SNAPX_LAUNCH_CHECK();
-      hipLaunchKernelGGL(recon_adj_kernel, bn, dim3(kT), 0, st, s.w, s.lam_wlr, s.lam_w, a, d);
+      hipLaunchKernelGGL(recon_adj_kernel, bn, dim3(kT), 0, st, w, s.lam_wlr, s.lam_w, a, d);
       SNAPX_LAUNCH_CHECK();
     }
     if (g.n1 > 1 && cfg->grav1 != 0.) {
-      hipLaunchKernelGGL(ghost_adj_kernel, blocks_for(g.n2 * g.n3, kT), dim3(kT), 0, st, s.w,
+      hipLaunchKernelGGL(ghost_adj_kernel, blocks_for(g.n2 * g.n3, kT), dim3(kT), 0, st, w,
                          s.lam_w, a);
       SNAPX_LAUNCH_CHECK();
     }

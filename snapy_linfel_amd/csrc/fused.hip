@@ -34,6 +34,7 @@
 #include "interp_impl.hpp"
 #include "riemann_impl.hpp"
 #include "vic_impl.hpp"
+#include "pcr.hpp"
 
 namespace snapx {
 
@@ -115,9 +116,11 @@ __device__ __forceinline__ void cons_to_prim(const T* U, T gammad, T* W) {
 }
 
 // K1 body for one tile; every thread of the workgroup calls it with the same
-// tile (it synchronises the workgroup internally)
+// tile (it synchronises the workgroup internally).  The arguments come by
+// value: a reference to the kernel argument makes the compiler keep a copy
+// of the whole struct in scratch memory.
 __device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __restrict__ du,
-                                         const FusedArgs& a, int b, float* smem) {
+                                         const FusedArgs a, int b, float* smem) {
   const int H1 = a.tx1 + 6, H2 = a.tx2 + 2 * a.h2, H3 = a.tx3 + 2 * a.h3;
   const int H = H1 * H2 * H3;
   float* ws = smem;          // primitives [5][H3][H2][H1]
@@ -198,10 +201,13 @@ __device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __r
     int nf = f1 * f2 * f3;
     int sd = d == 0 ? 1 : (d == 1 ? H1 : H1 * H2);  // LDS stride along d
     int r = d == 0 ? 0 : 1;
-    int type = a.recon_type[r], scale = a.recon_scale[r], shock = a.recon_shock[r];
+    // selects, not a runtime index into the argument struct (which would
+    // place it in scratch memory)
+    int type = r ? a.recon_type[1] : a.recon_type[0];
+    int scale = r ? a.recon_scale[1] : a.recon_scale[0];
+    int shock = r ? a.recon_shock[1] : a.recon_shock[0];
     int type2 = shock ? type : interp2_type(type);
-    int dimc = 3 - d;  // reference tensor axis
-    int solver = a.riemann[d];
+    int solver = d == 0 ? a.riemann[0] : (d == 1 ? a.riemann[1] : a.riemann[2]);
     for (int f = threadIdx.x; f < nf; f += blockDim.x) {
       int fx = f % f1, fy = (f / f1) % f2, fz = f / (f1 * f2);
       // LDS coordinates of the cell above the face along d
@@ -209,11 +215,15 @@ __device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __r
       int lin = (z * H2 + y) * H1 + x;
       int c = d == 0 ? x : (d == 1 ? y : z);
       int base = lin - c * sd;  // pencil start along d
+      // face frame: slot v holds variable vmap[v] (normal velocity in slot
+      // IVX); register arrays are indexed by constants only, the frame is
+      // entered and left through LDS addresses
+      const int vmap[5] = {IDN, IVX + d, IVX + (d + 1) % 3, IVX + (d + 2) % 3, IPR};
       float wl[5], wr[5];
 #pragma unroll
       for (int v = 0; v < 5; ++v) {
         int tv = (v == IDN) ? type : type2;
-        const float* p = ws + v * H + base;
+        const float* p = ws + vmap[v] * H + base;
         wr[v] = interp_face<float>(tv, 0, scale, p, sd, c);
         wl[v] = interp_face<float>(tv, 1, scale, p, sd, c - 1);
       }
@@ -230,9 +240,9 @@ __device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __r
         }
       }
       float flx[5];
-      riemann_flux<float>(solver, wl, wr, gammad, dimc, flx, nullptr);
+      riemann_flux<float>(solver, wl, wr, gammad, 3, flx, (float*)nullptr);
 #pragma unroll
-      for (int v = 0; v < 5; ++v) fs[v * nf + f] = flx[v];
+      for (int v = 0; v < 5; ++v) fs[vmap[v] * nf + f] = flx[v];
     }
     __syncthreads();
     float area = d == 0 ? a.dx3 * a.dx2 : (d == 1 ? a.dx3 * a.dx1 : a.dx2 * a.dx1);
@@ -533,116 +543,6 @@ __global__ void __launch_bounds__(kColThreads)
 // reductions, then the lane applies the redistribution and the stage
 // combine to its own cell.  Arithmetic in FP64, as in the serial sweep.
 
-template <int N>
-struct PcrShape {
-  static constexpr int kThreads = N == 3 ? 256 : 128;
-  static constexpr int kPub = 2 * N * N + N;  // doubles published per lane
-};
-
-// Solves A X = R for the N x M right-hand side R (row-major, overwritten),
-// partial pivoting with fully unrolled row selection (no dynamic indexing).
-template <int N, int M>
-__device__ __forceinline__ void solve_rows(double* A, double* R) {
-#pragma unroll
-  for (int k = 0; k < N; ++k) {
-    // pivot row p >= k of largest |A[., k]|
-    int p = k;
-    double big = fabs(A[k * N + k]);
-#pragma unroll
-    for (int r = k + 1; r < N; ++r) {
-      double v = fabs(A[r * N + k]);
-      if (v > big) {
-        big = v;
-        p = r;
-      }
-    }
-#pragma unroll
-    for (int r = k + 1; r < N; ++r) {
-      if (p == r) {
-#pragma unroll
-        for (int c = 0; c < N; ++c) {
-          double t = A[k * N + c];
-          A[k * N + c] = A[r * N + c];
-          A[r * N + c] = t;
-        }
-#pragma unroll
-        for (int c = 0; c < M; ++c) {
-          double t = R[k * M + c];
-          R[k * M + c] = R[r * M + c];
-          R[r * M + c] = t;
-        }
-      }
-    }
-    double inv = 1.0 / A[k * N + k];
-#pragma unroll
-    for (int r = 0; r < N; ++r) {
-      if (r == k) continue;
-      double f = A[r * N + k] * inv;
-#pragma unroll
-      for (int c = k; c < N; ++c) A[r * N + c] -= f * A[k * N + c];
-#pragma unroll
-      for (int c = 0; c < M; ++c) R[r * M + c] -= f * R[k * M + c];
-    }
-  }
-#pragma unroll
-  for (int k = 0; k < N; ++k) {
-    double inv = 1.0 / A[k * N + k];
-#pragma unroll
-    for (int c = 0; c < M; ++c) R[k * M + c] *= inv;
-  }
-}
-
-// normalised row of lane t: Pb = A^-1 B, Pc = A^-1 C, pr = A^-1 r, written
-// to LDS (structure of arrays, stride T)
-template <int N>
-__device__ __forceinline__ void pcr_publish(const double* A, const double* B, const double* C,
-                                            const double* r, double* sh, int T, int t) {
-  const int NN = N * N;
-  if (N == 3) {
-    double inv[9], x[9];
-    inverse3<double>(A, inv);
-    matmul<double, 3>(inv, B, x);
-#pragma unroll
-    for (int e = 0; e < 9; ++e) sh[e * T + t] = x[e];
-    matmul<double, 3>(inv, C, x);
-#pragma unroll
-    for (int e = 0; e < 9; ++e) sh[(9 + e) * T + t] = x[e];
-    matvec<double, 3>(inv, r, x);
-#pragma unroll
-    for (int e = 0; e < 3; ++e) sh[(18 + e) * T + t] = x[e];
-  } else {
-    const int M = 2 * N + 1;
-    double a[NN], R[N * M];
-#pragma unroll
-    for (int e = 0; e < NN; ++e) a[e] = A[e];
-#pragma unroll
-    for (int row = 0; row < N; ++row) {
-#pragma unroll
-      for (int c = 0; c < N; ++c) {
-        R[row * M + c] = B[row * N + c];
-        R[row * M + N + c] = C[row * N + c];
-      }
-      R[row * M + 2 * N] = r[row];
-    }
-    solve_rows<N, M>(a, R);
-#pragma unroll
-    for (int row = 0; row < N; ++row) {
-#pragma unroll
-      for (int c = 0; c < N; ++c) {
-        sh[(row * N + c) * T + t] = R[row * M + c];
-        sh[(NN + row * N + c) * T + t] = R[row * M + N + c];
-      }
-      sh[(2 * NN + row) * T + t] = R[row * M + 2 * N];
-    }
-  }
-}
-
-__device__ __forceinline__ double wave_sum(double x) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) x += __shfl_xor(x, m);
-  return x;
-}
-
 // Every thread of the workgroup calls this with the same ColArgs; lane t
 // serves layer i = t mod L of column col (col < a.ncol checked here).
 template <int N>
@@ -760,94 +660,14 @@ __device__ __forceinline__ void column_pcr(float* __restrict__ u, float* __restr
 #pragma unroll
       for (int e = 0; e < N; ++e) r[e] = 0.;
     }
-    // ---- parallel cyclic reduction ----
-    for (int sft = 1; sft < nl; sft <<= 1) {
-      pcr_publish<N>(A, B, C, r, sh, T, t);
-      __syncthreads();
-      const bool hm = act && i - sft >= 0;
-      const bool hp = act && i + sft < nl;
-      double nB[NN], nC[NN];
-#pragma unroll
-      for (int e = 0; e < NN; ++e) nB[e] = nC[e] = 0.;
-      if (hm) {
-        const int tm = t - sft;
-        double Pb[NN], Pc[NN], pr[N], x[NN];
-#pragma unroll
-        for (int e = 0; e < NN; ++e) {
-          Pb[e] = sh[e * T + tm];
-          Pc[e] = sh[(NN + e) * T + tm];
-        }
-#pragma unroll
-        for (int e = 0; e < N; ++e) pr[e] = sh[(2 * NN + e) * T + tm];
-        matmul<double, N>(B, Pb, x);
-#pragma unroll
-        for (int e = 0; e < NN; ++e) nB[e] = -x[e];
-        matmul<double, N>(B, Pc, x);
-#pragma unroll
-        for (int e = 0; e < NN; ++e) A[e] -= x[e];
-        matvec<double, N>(B, pr, x);
-#pragma unroll
-        for (int e = 0; e < N; ++e) r[e] -= x[e];
-      }
-      if (hp) {
-        const int tp = t + sft;
-        double Pb[NN], Pc[NN], pr[N], x[NN];
-#pragma unroll
-        for (int e = 0; e < NN; ++e) {
-          Pb[e] = sh[e * T + tp];
-          Pc[e] = sh[(NN + e) * T + tp];
-        }
-#pragma unroll
-        for (int e = 0; e < N; ++e) pr[e] = sh[(2 * NN + e) * T + tp];
-        matmul<double, N>(C, Pc, x);
-#pragma unroll
-        for (int e = 0; e < NN; ++e) nC[e] = -x[e];
-        matmul<double, N>(C, Pb, x);
-#pragma unroll
-        for (int e = 0; e < NN; ++e) A[e] -= x[e];
-        matvec<double, N>(C, pr, x);
-#pragma unroll
-        for (int e = 0; e < N; ++e) r[e] -= x[e];
-      }
-#pragma unroll
-      for (int e = 0; e < NN; ++e) {
-        B[e] = nB[e];
-        C[e] = nC[e];
-      }
-      __syncthreads();
-    }
-    // ---- decoupled rows: delta = A^-1 r ----
+    // ---- parallel cyclic reduction, then delta = A^-1 r ----
     double d[N];
-    if (N == 3) {
-      double inv[9];
-      inverse3<double>(A, inv);
-      matvec<double, 3>(inv, r, d);
-    } else {
-      solve_rows<N, 1>(A, r);
-#pragma unroll
-      for (int e = 0; e < N; ++e) d[e] = r[e];
-    }
-    // ---- MS-VIC column sums (wave reductions, then across the column's waves) ----
+    pcr_solve<N>(A, B, C, r, d, sh, T, t, i, nl, act);
+    // ---- MS-VIC column sums ----
     double ai = wc[IDN] * a.vol;
-    double s_a2 = act ? ai * ai : 0.;
-    double s_b = act ? (dd[IDN] - d[0]) * 1.0 * a.vol : 0.;
-    s_a2 = wave_sum(s_a2);
-    s_b = wave_sum(s_b);
-    if (a.lanes > 64) {
-      const int wv = t >> 6, nwv = a.lanes >> 6;
-      if ((t & 63) == 0) {
-        sh[2 * wv] = s_a2;
-        sh[2 * wv + 1] = s_b;
-      }
-      __syncthreads();
-      const int w0 = (t >> a.lshift) * nwv;
-      s_a2 = 0.;
-      s_b = 0.;
-      for (int k = 0; k < nwv; ++k) {
-        s_a2 += sh[2 * (w0 + k)];
-        s_b += sh[2 * (w0 + k) + 1];
-      }
-    }
+    double sums[2] = {act ? ai * ai : 0., act ? (dd[IDN] - d[0]) * 1.0 * a.vol : 0.};
+    column_sums<2>(sums, sh, t, a.lanes, a.lshift);
+    const double s_a2 = sums[0], s_b = sums[1];
     double rho = wc[IDN];
     double fix = d[0] - dd[IDN];
     dd[IDN] = (dd[IDN] + fix) + rho * ai * s_b / s_a2;
