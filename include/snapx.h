@@ -281,6 +281,33 @@ int snapx_batch_rk3_steps(const snapx_config* members,
                           void* workspace, size_t workspace_bytes,
                           snapx_timing* timing, void* stream);
 
+/* Tile autotuner of the batched path (no reference counterpart).  Times
+   `trials` launches of the RHS kernel (it reads u, writes only workspace) for
+   every candidate tile that fits the members, then plans the batch with the
+   fastest one (as snapx_batch_plan; *best_ms = its average launch time). */
+int snapx_batch_autotune(const snapx_config* members, int nmembers, double dt,
+                         const float* u, void* workspace, size_t workspace_bytes,
+                         int trials, void* stream, snapx_batch_info* info,
+                         double* best_ms);
+
+/* ---------------------------------------------------------------------- */
+/* HIP graphs (no reference counterpart)                                   */
+/* ---------------------------------------------------------------------- */
+
+/* Capture nsteps fused steps (snapx_fused_rk3_steps) or batched steps
+   (snapx_batch_rk3_steps) on the given buffers into an executable graph;
+   snapx_graph_launch replays it on a stream.  The caller keeps u and the
+   workspace alive and unchanged in place while the graph exists. */
+int snapx_fused_graph_create(const snapx_config* cfg, double dt, int nsteps,
+                             float* u, void* workspace, size_t workspace_bytes,
+                             void** graph);
+int snapx_batch_graph_create(const snapx_config* members,
+                             const snapx_batch_info* info, int nsteps, float* u,
+                             void* workspace, size_t workspace_bytes,
+                             void** graph);
+int snapx_graph_launch(void* graph, void* stream);
+int snapx_graph_destroy(void* graph);
+
 #ifdef __cplusplus
 }
 #endif

@@ -33,7 +33,9 @@ HIP_UNITS = {
     "abi.hip": [],
     "plain.hip": ["-ffp-contract=off"],
     "vic.hip": ["-ffp-contract=off"],
-    "fused.hip": [],
+    # FP32 fused path: hardware reciprocal / square root (1-2 ulp) instead of
+    # the correctly rounded division and sqrt sequences
+    "fused.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt"],
     "adjoint.hip": ["-ffp-contract=off"],
 }
 

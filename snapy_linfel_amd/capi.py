@@ -128,6 +128,11 @@ SIGNATURES = {
     "snapx_batch_workspace_bytes": (_Z, [_CFG, _I]),
     "snapx_batch_plan": (_I, [_CFG, _I, _D, _I, _I, _I, _P, _Z, _P]),
     "snapx_batch_rk3_steps": (_I, [_CFG, _P, _I, _P, _P, _Z, _P, _P]),
+    "snapx_batch_autotune": (_I, [_CFG, _I, _D, _P, _P, _Z, _I, _P, _P, _P]),
+    "snapx_fused_graph_create": (_I, [_CFG, _D, _I, _P, _P, _Z, _P]),
+    "snapx_batch_graph_create": (_I, [_CFG, _P, _I, _P, _P, _Z, _P]),
+    "snapx_graph_launch": (_I, [_P, _P]),
+    "snapx_graph_destroy": (_I, [_P]),
 }
 
 
@@ -344,3 +349,45 @@ class Batch:
         check(lib().snapx_batch_rk3_steps(self.cfgs, C.byref(self.info), nsteps, ptr(u),
                                           ptr(self.ws), self.ws.numel(), tp, stream_ptr()),
               "batch_rk3_steps")
+
+    def autotune(self, u, dt, trials=3):
+        """Re-plan with the fastest RHS tile for these members (u unchanged);
+        returns (tile, ms per RHS launch)."""
+        ms = C.c_double(0.0)
+        check(lib().snapx_batch_autotune(self.cfgs, self.n, dt, ptr(u), ptr(self.ws),
+                                         self.ws.numel(), trials, stream_ptr(),
+                                         C.byref(self.info), C.byref(ms)), "batch_autotune")
+        return (self.info.tile_x1, self.info.tile_x2, self.info.tile_x3), ms.value
+
+    def graph(self, u, nsteps):
+        """A Graph replaying nsteps batched steps on u."""
+        h = C.c_void_p()
+        check(lib().snapx_batch_graph_create(self.cfgs, C.byref(self.info), nsteps, ptr(u),
+                                             ptr(self.ws), self.ws.numel(), C.byref(h)),
+              "batch_graph_create")
+        return Graph(h, (u, self.ws))
+
+
+class Graph:
+    """An executable HIP graph of captured steps; keeps its buffers alive."""
+
+    def __init__(self, handle, keep):
+        self.h = handle
+        self._keep = keep
+
+    def launch(self):
+        check(lib().snapx_graph_launch(self.h, stream_ptr()), "graph_launch")
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            lib().snapx_graph_destroy(self.h)
+            self.h = None
+
+
+def fused_graph(cfg, dt, nsteps, u, ws=None):
+    """A Graph replaying nsteps fused FP32 steps on u."""
+    ws = fused_workspace(cfg, u.device) if ws is None else ws
+    h = C.c_void_p()
+    check(lib().snapx_fused_graph_create(C.byref(cfg), dt, nsteps, ptr(u), ptr(ws), ws.numel(),
+                                         C.byref(h)), "fused_graph_create")
+    return Graph(h, (u, ws))

@@ -115,6 +115,23 @@ __device__ __forceinline__ void cons_to_prim(const T* U, T gammad, T* W) {
   W[IPR] = (gammad - T(1.)) * (U[IPR] - ke);
 }
 
+// Quotient and remainder of a small non-negative index by a positive
+// divisor through the FP32 reciprocal and one correction step (an integer
+// division by a runtime divisor costs some forty instructions).  Exact for
+// l < 2^22.
+struct DivMod {
+  int d;
+  float inv;
+  __device__ __forceinline__ explicit DivMod(int dd) : d(dd), inv(1.0f / (float)dd) {}
+  __device__ __forceinline__ int div(int l, int& r) const {
+    int q = (int)((float)l * inv);
+    r = l - q * d;
+    if (r < 0) r += d, q -= 1;
+    if (r >= d) r -= d, q += 1;
+    return q;
+  }
+};
+
 // K1 body for one tile; every thread of the workgroup calls it with the same
 // tile (it synchronises the workgroup internally).  The arguments come by
 // value: a reference to the kernel argument makes the compiler keep a copy
@@ -137,10 +154,10 @@ __device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __r
   const float gammad = a.gammad;
 
   // ---- 1. halo'd tile: conserved -> primitive into LDS ----
+  const DivMod dH1(H1), dH2(H2);
   for (int l = threadIdx.x; l < H; l += blockDim.x) {
-    int x = l % H1;
-    int y = (l / H1) % H2;
-    int z = l / (H1 * H2);
+    int x, y;
+    int z = dH2.div(dH1.div(l, x), y);
     bool fl[3];
     int q = mapped_cell(a, o1 + x - 3, o2 + y - a.h2, o3 + z - a.h3, fl);
     float U[5], W[5];
@@ -191,6 +208,7 @@ __device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __r
 #pragma unroll
     for (int v = 0; v < 5; ++v) acc[r][v] = 0.f;
   const int ncells_t = e1 * e2 * e3;
+  const DivMod de1(e1), de2(e2);
 
   for (int d = 0; d < 3; ++d) {
     int nd = d == 0 ? a.nx1 : (d == 1 ? a.nx2 : a.nx3);
@@ -208,8 +226,10 @@ __device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __r
     int shock = r ? a.recon_shock[1] : a.recon_shock[0];
     int type2 = shock ? type : interp2_type(type);
     int solver = d == 0 ? a.riemann[0] : (d == 1 ? a.riemann[1] : a.riemann[2]);
+    const DivMod df1(f1), df2(f2);
     for (int f = threadIdx.x; f < nf; f += blockDim.x) {
-      int fx = f % f1, fy = (f / f1) % f2, fz = f / (f1 * f2);
+      int fx, fy;
+      int fz = df2.div(df1.div(f, fx), fy);
       // LDS coordinates of the cell above the face along d
       int x = fx + 3, y = fy + a.h2, z = fz + a.h3;
       int lin = (z * H2 + y) * H1 + x;
@@ -250,7 +270,8 @@ __device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __r
     for (int rr = 0; rr < kMaxCellsPerThread; ++rr) {
       int cid = threadIdx.x + rr * kRhsThreads;
       if (cid < ncells_t) {
-        int cx = cid % e1, cy = (cid / e1) % e2, cz = cid / (e1 * e2);
+        int cx, cy;
+        int cz = de2.div(de1.div(cid, cx), cy);
         int lo = (cz * f2 + cy) * f1 + cx;
         int hi = lo + (d == 0 ? 1 : (d == 1 ? f1 : f1 * f2));
 #pragma unroll
@@ -267,7 +288,8 @@ __device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __r
   for (int rr = 0; rr < kMaxCellsPerThread; ++rr) {
     int cid = threadIdx.x + rr * kRhsThreads;
     if (cid >= ncells_t) continue;
-    int cx = cid % e1, cy = (cid / e1) % e2, cz = cid / (e1 * e2);
+    int cx, cy;
+    int cz = de2.div(de1.div(cid, cx), cy);
     int lin = ((cz + a.h3) * H2 + cy + a.h2) * H1 + cx + 3;
     float d[5];
 #pragma unroll
@@ -325,16 +347,21 @@ __device__ __forceinline__ void column_prim(const float* __restrict__ u, int nce
 }
 
 // isentropic refill of the ghost next to boundary cell wb (dir -1: below,
-// +1: above) (src/hydro/hydro.cpp:231-269, first ghost)
+// +1: above) (src/hydro/hydro.cpp:231-269, first ghost).  FP32, the
+// arithmetic of the refill in the RHS kernel, which computes the fluxes of
+// the same stage from these ghost values.
 __device__ __forceinline__ void refill_ghost(const double* wb, double* wg, const ColArgs& a,
                                              int dir) {
-  double gamma = a.gammad, gm = gamma - 1., ae = gm / gamma;
-  double grav = -a.grav1;
-  double K = wb[IPR] / pow(wb[IDN], gamma);
-  double h = dir < 0 ? pow(wb[IPR], ae) + ae * grav * a.dx1 / pow(K, 1. / gamma)
-                     : pow(wb[IPR], ae) - ae * grav * a.dx1 / pow(K, 1. / gamma);
-  wg[IPR] = pow(h, 1. / ae);
-  wg[IDN] = pow(wg[IPR] / K, 1. / gamma);
+  float gamma = (float)a.gammad, gm = gamma - 1.f, ae = gm / gamma;
+  float inv_g = 1.f / gamma, inv_a = 1.f / ae;
+  float grav = -(float)a.grav1, dz = (float)a.dx1;
+  float rho = (float)wb[IDN], pre = (float)wb[IPR];
+  float K = pre / powf(rho, gamma);
+  float Kg = powf(K, inv_g);
+  float h = dir < 0 ? powf(pre, ae) + ae * grav * dz / Kg : powf(pre, ae) - ae * grav * dz / Kg;
+  float pg = powf(h, inv_a);
+  wg[IPR] = (double)pg;
+  wg[IDN] = (double)powf(pg / K, inv_g);
 }
 
 template <int N>
@@ -595,7 +622,7 @@ __device__ __forceinline__ void column_pcr(float* __restrict__ u, float* __restr
     const double gm1 = a.gammad - 1.;
     const double gm1r = 0.5 * (a.gammad + a.gammad) - 1.;
     double Am[NN], Ap[NN];
-    abs_jacobian<double, N>(gm1r, wm, wc, Am);
+    abs_jacobian_fast<double, N>(gm1r, wm, wc, Am);
 #pragma unroll
     for (int e = 0; e < NN; ++e) sh[e * T + t] = Am[e];
     __syncthreads();
@@ -603,7 +630,7 @@ __device__ __forceinline__ void column_pcr(float* __restrict__ u, float* __restr
 #pragma unroll
       for (int e = 0; e < NN; ++e) Ap[e] = sh[e * T + t + 1];
     } else {
-      abs_jacobian<double, N>(gm1r, wc, wp, Ap);
+      abs_jacobian_fast<double, N>(gm1r, wc, wp, Ap);
     }
     __syncthreads();
     // ---- block row i (vic_assemble_cell; uniform Cartesian areas) ----
@@ -1216,4 +1243,170 @@ extern "C" int snapx_batch_rk3_steps(const snapx_config* members, const snapx_ba
     }
   }
   return 0;
+}
+
+// ------------------------------------------------------------ HIP graphs ----
+
+namespace snapx {
+
+// Captures whatever enqueue(stream) launches on a private capture stream
+// into an instantiated graph.  The capture stream has the default priority.
+template <typename F>
+static int capture_graph(F enqueue, void** graph_out) {
+  SNAPX_CHECK_ARG(graph_out, "graph: null output pointer");
+  hipStream_t cs;
+  SNAPX_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    (void)hipStreamDestroy(cs);
+    set_error("graph: stream capture could not start");
+    return SNAPX_EHIP;
+  }
+  int status = enqueue(cs);
+  hipGraph_t g = nullptr;
+  hipError_t ec = hipStreamEndCapture(cs, &g);
+  (void)hipStreamDestroy(cs);
+  if (status) {
+    if (g) (void)hipGraphDestroy(g);
+    return status;
+  }
+  if (ec != hipSuccess || !g) {
+    set_error("graph: stream capture failed");
+    return SNAPX_EHIP;
+  }
+  hipGraphExec_t exec = nullptr;
+  hipError_t ei = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ei != hipSuccess) {
+    set_error("graph: instantiation failed");
+    return SNAPX_EHIP;
+  }
+  *graph_out = exec;
+  return 0;
+}
+
+}  // namespace snapx
+
+extern "C" int snapx_fused_graph_create(const snapx_config* cfg, double dt, int nsteps, float* u,
+                                        void* workspace, size_t workspace_bytes,
+                                        void** graph) {
+  if (int e = check_fused(cfg)) return e;
+  SNAPX_CHECK_ARG(u && workspace && nsteps >= 1, "snapx_fused_graph_create: bad arguments");
+  SNAPX_CHECK_ARG(workspace_bytes >= snapx_fused_workspace_bytes(cfg),
+                  "snapx_fused_graph_create: workspace too small");
+  return capture_graph(
+      [&](hipStream_t cs) {
+        return snapx_fused_rk3_steps(cfg, dt, nsteps, u, workspace, workspace_bytes, nullptr,
+                                     cs);
+      },
+      graph);
+}
+
+extern "C" int snapx_batch_graph_create(const snapx_config* members,
+                                        const snapx_batch_info* info, int nsteps, float* u,
+                                        void* workspace, size_t workspace_bytes, void** graph) {
+  SNAPX_CHECK_ARG(members && info && u && workspace && nsteps >= 1,
+                  "snapx_batch_graph_create: bad arguments");
+  return capture_graph(
+      [&](hipStream_t cs) {
+        return snapx_batch_rk3_steps(members, info, nsteps, u, workspace, workspace_bytes,
+                                     nullptr, cs);
+      },
+      graph);
+}
+
+extern "C" int snapx_graph_launch(void* graph, void* stream) {
+  SNAPX_CHECK_ARG(graph, "snapx_graph_launch: null graph");
+  SNAPX_HIP(hipGraphLaunch(reinterpret_cast<hipGraphExec_t>(graph), as_stream(stream)));
+  return 0;
+}
+
+extern "C" int snapx_graph_destroy(void* graph) {
+  if (graph) SNAPX_HIP(hipGraphExecDestroy(reinterpret_cast<hipGraphExec_t>(graph)));
+  return 0;
+}
+
+// ------------------------------------------------------ tile autotuner ----
+
+namespace snapx {
+
+// Candidate K1 tiles: x1 extents that keep rows coalesced, then x2 / x3
+// extents, within the kernel limits (<= 1024 cells, <= 64 KB LDS).
+static std::vector<Tile> tile_candidates(bool two_d, bool three_d) {
+  std::vector<Tile> out;
+  const int t1s[] = {16, 32, 64, 128, 256};
+  const int t2s[] = {1, 2, 4, 8, 16};
+  const int t3s[] = {1, 2, 4, 8};
+  for (int t1 : t1s)
+    for (int t2 : t2s)
+      for (int t3 : t3s) {
+        if (!two_d && !three_d && (t2 > 1 || t3 > 1)) continue;
+        if (two_d && !three_d && (t2 == 1 || t3 > 1)) continue;
+        if (three_d && (t2 == 1 || t3 == 1)) continue;
+        if (t1 * t2 * t3 > kRhsThreads * kMaxCellsPerThread) continue;
+        out.push_back({t1, t2, t3});
+      }
+  return out;
+}
+
+}  // namespace snapx
+
+extern "C" int snapx_batch_autotune(const snapx_config* members, int nmembers, double dt,
+                                    const float* u, void* workspace, size_t workspace_bytes,
+                                    int trials, void* stream, snapx_batch_info* info,
+                                    double* best_ms) {
+  SNAPX_CHECK_ARG(members && nmembers >= 1 && u && workspace && info && trials >= 1,
+                  "snapx_batch_autotune: bad arguments");
+  SNAPX_CHECK_ARG(workspace_bytes >= snapx_batch_workspace_bytes(members, nmembers),
+                  "snapx_batch_autotune: workspace too small");
+  hipStream_t st = as_stream(stream);
+  bool two_d = false, three_d = false;
+  for (int m = 0; m < nmembers; ++m) {
+    three_d |= members[m].nx3 > 1;
+    two_d |= members[m].nx2 > 1;
+  }
+  char* base = reinterpret_cast<char*>(workspace);
+  const BatchMember* table = reinterpret_cast<const BatchMember*>(base);
+  float* du = reinterpret_cast<float*>(base + batch_table_bytes(nmembers));
+  hipEvent_t e0, e1;
+  SNAPX_HIP(hipEventCreate(&e0));
+  SNAPX_HIP(hipEventCreate(&e1));
+  double best = 1e300;
+  Tile best_t = {0, 0, 0};
+  int status = 0;
+  for (Tile t : tile_candidates(two_d, three_d)) {
+    std::vector<BatchMember> tab;
+    snapx_batch_info inf;
+    // candidates that do not fit some member (LDS) are skipped
+    if (batch_layout(members, nmembers, dt, t.tx1, t.tx2, t.tx3, &tab, &inf)) continue;
+    if (hipMemcpy(workspace, tab.data(), sizeof(BatchMember) * tab.size(),
+                  hipMemcpyHostToDevice) != hipSuccess) {
+      set_error("snapx_batch_autotune: table copy failed");
+      status = SNAPX_EHIP;
+      break;
+    }
+    int grid = std::min(inf.total_tiles, 512);
+    // K1 reads u and writes du only: the state is not changed
+    hipLaunchKernelGGL(batch_rhs_kernel, dim3(grid), dim3(kRhsThreads), inf.rhs_smem, st, u, du,
+                       table, nmembers, inf.total_tiles);
+    (void)hipEventRecord(e0, st);
+    for (int k = 0; k < trials; ++k)
+      hipLaunchKernelGGL(batch_rhs_kernel, dim3(grid), dim3(kRhsThreads), inf.rhs_smem, st, u,
+                         du, table, nmembers, inf.total_tiles);
+    (void)hipEventRecord(e1, st);
+    if (hipEventSynchronize(e1) != hipSuccess || hipGetLastError() != hipSuccess) {
+      set_error("snapx_batch_autotune: trial launch failed");
+      status = SNAPX_EHIP;
+      break;
+    }
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (ms / trials < best) best = ms / trials, best_t = t;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (status) return status;
+  SNAPX_CHECK_ARG(best_t.tx1 > 0, "snapx_batch_autotune: no tile fits the members");
+  if (best_ms) *best_ms = best;
+  return snapx_batch_plan(members, nmembers, dt, best_t.tx1, best_t.tx2, best_t.tx3, workspace,
+                          workspace_bytes, info);
 }

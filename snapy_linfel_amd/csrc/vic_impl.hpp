@@ -100,6 +100,57 @@ __device__ __forceinline__ void abs_jacobian(T gm1, const T* wl, const T* wr, T*
   }
 }
 
+// abs_jacobian with the shared divisors inverted once (6 divisions instead
+// of 18); the same matrix up to rounding.  Used by the fused path, whose
+// state is FP32; the FP64 plain path keeps the reference's arithmetic.
+template <typename T, int N>
+__device__ __forceinline__ void abs_jacobian_fast(T gm1, const T* wl, const T* wr, T* A) {
+  T sqrtdl = sqrt(wl[IDN]);
+  T sqrtdr = sqrt(wr[IDN]);
+  T isdlpdr = T(1.0) / (sqrtdl + sqrtdr);
+  T r = sqrtdl * sqrtdr;
+  T u = (sqrtdl * wl[IVX] + sqrtdr * wr[IVX]) * isdlpdr;
+  T v = (sqrtdl * wl[IVY] + sqrtdr * wr[IVY]) * isdlpdr;
+  T w = (sqrtdl * wl[IVZ] + sqrtdr * wr[IVZ]) * isdlpdr;
+  T igm1 = T(1.0) / gm1;
+  T el = wl[IPR] * igm1 + T(0.5) * wl[IDN] * ((sq(wl[IVX]) + sq(wl[IVY])) + sq(wl[IVZ]));
+  T er = wr[IPR] * igm1 + T(0.5) * wr[IDN] * ((sq(wr[IVX]) + sq(wr[IVY])) + sq(wr[IVZ]));
+  // (e + p) / sqrt(rho_l) = (e + p) sqrt(rho_r) / r
+  T ir = T(1.0) / r;
+  T hbar = ((el + wl[IPR]) * sqrtdr + (er + wr[IPR]) * sqrtdl) * isdlpdr * ir;
+  T p = (hbar - T(0.5) * ((sq(u) + sq(v)) + sq(w))) * gm1 / (gm1 + T(1.)) * r;
+  T cs = sqrt(p * (gm1 + T(1.)) * ir);
+  T ke = T(0.5) * ((u * u + v * v) + w * w);
+  T hp = (gm1 + T(1.)) * igm1 * p * ir;
+  T h = hp + ke;
+  T lam[5] = {fabs(u - cs), fabs(u), fabs(u + cs), fabs(u), fabs(u)};
+  T R[25] = {T(1.), T(1.), T(1.), T(0.), T(0.),
+             u - cs, u, u + cs, T(0.), T(0.),
+             v, v, v, T(1.), T(0.),
+             w, w, w, T(0.), T(1.),
+             h - u * cs, ke, h + u * cs, v, w};
+  T ihp = T(1.) / hp;
+  T i2hp = T(0.5) * ihp;
+  T i2cshp = i2hp / cs;
+  T Ri[25] = {(cs * ke + u * hp) * i2cshp, (-hp - cs * u) * i2cshp, -v * i2hp, -w * i2hp, i2hp,
+              (hp - ke) * ihp, u * ihp, v * ihp, w * ihp, -ihp,
+              (cs * ke - u * hp) * i2cshp, (hp - cs * u) * i2cshp, -v * i2hp, -w * i2hp, i2hp,
+              -v, T(0.), T(1.), T(0.), T(0.),
+              -w, T(0.), T(0.), T(1.), T(0.)};
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    int ri = vic_row(N, i);
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      int cj = vic_row(N, j);
+      T s = T(0.);
+#pragma unroll
+      for (int k = 0; k < 5; ++k) s += (R[ri * 5 + k] * lam[k]) * Ri[k * 5 + cj];
+      A[i * N + j] = s;
+    }
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ void inverse3(const T* m, T* inv) {
 #define M_(r, c) m[(r) * 3 + (c)]
