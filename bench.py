@@ -54,6 +54,42 @@ def box3d_cfg(capi, n=(128, 256, 256)):
                                 "periodic"))
 
 
+def ensemble_members(capi, n=96, seed=2024):
+    """Batched warm-bubble ensemble: grid sizes drawn by seed from
+    {128,192,256,384,512} x {64,96,128}, 50 m cells, dry ideal gas with
+    theta0 = 300 K, a Gaussian +0.5 K bubble of radius 250-2000 m at a seeded
+    position, WENO5 + HLLC, implicit 3x3 column solve, reflecting x1,
+    periodic x2.  Returns (configs, primitive states as numpy arrays)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    R, cp, g, p0, th0 = 287.0, 1004.0, 9.81, 1e5, 300.0
+    cfgs, ws = [], []
+    for _ in range(n):
+        nx2 = int(rng.choice([128, 192, 256, 384, 512]))
+        nx1 = int(rng.choice([64, 96, 128]))
+        c = capi.make_config(nx1=nx1, nx2=nx2, nghost=3, x1min=0.0, x1max=50.0 * nx1,
+                             x2min=0.0, x2max=50.0 * nx2, gammad=cp / (cp - R),
+                             weight=8.31446261815324 / R, recon_vertical=("weno5", False, False),
+                             riemann="hllc", grav1=-g, implicit_scheme=1,
+                             bc=("reflecting", "reflecting", "periodic", "periodic",
+                                 "periodic", "periodic"))
+        shp = c.shape
+        z = (np.arange(shp[3]) - 3 + 0.5) * 50.0
+        x = (np.arange(shp[2]) - 3 + 0.5) * 50.0
+        X, Z = np.meshgrid(x, z, indexing="ij")
+        rad = rng.uniform(250.0, 2000.0)
+        xc, zc = rng.uniform(0.0, 50.0 * nx2), rng.uniform(0.2, 0.5) * 50.0 * nx1
+        T = th0 - g * Z / cp
+        p = p0 * (T / th0) ** (cp / R)
+        th = 0.5 * np.exp(-((X - xc) ** 2 + (Z - zc) ** 2) / rad ** 2)
+        w = np.zeros(shp)
+        w[4, 0] = p
+        w[0, 0] = p / (R * T * (1.0 + th / th0))
+        cfgs.append(c)
+        ws.append(w)
+    return cfgs, ws
+
+
 def errors(got, ref, interior):
     import numpy as np
     out = {}
@@ -95,6 +131,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-3d", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -231,6 +268,87 @@ def main():
             "rhs_hbm_frac": round(2 * NV * 4 * c3.n_interior / (r3 * 1e-3) / 1e9 / HBM_PEAK_GBS,
                                   4),
         }
+
+    # ---- batched ensemble (96 members of uneven size), autotuned tiles,
+    #      HIP-graph replays ----
+    if not args.no_ensemble:
+        cfgs_e, ws_e = ensemble_members(capi)
+        parts = []
+        for c, w in zip(cfgs_e, ws_e):
+            wd = torch.from_numpy(w).to(dev)
+            ud = torch.empty_like(wd)
+            capi.prim2cons(c, wd, ud)
+            capi.apply_bc(c, ud)
+            parts.append(ud.float().reshape(-1))
+        ue = torch.cat(parts)
+        del parts
+        dte = 0.1
+        b = capi.Batch(cfgs_e, dte)
+        tile, rhs_ms = b.autotune(ue, dte, trials=3)
+        steps_per_graph, ngraph = 10, 3
+        g = b.graph(ue, steps_per_graph)
+        g.launch()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(ngraph):
+            g.launch()
+        torch.cuda.synchronize()
+        te = time.perf_counter() - t0
+        ncells_e = sum(c.n_interior for c in cfgs_e)
+        extra["ensemble_96_fp32_graph"] = {
+            "members": len(cfgs_e), "cells": int(ncells_e),
+            "cell_updates_per_s": round(ncells_e * steps_per_graph * ngraph / te, 1),
+            "ms_per_step": round(te / (steps_per_graph * ngraph) * 1e3, 4),
+            "autotuned_tile": list(tile), "rhs_ms_at_tile": round(rhs_ms, 4),
+            "finite": bool(torch.isfinite(ue).all()),
+        }
+        del g, b, ue
+
+    # ---- adjoint at scale: 3-D box 128 x 128 x 96 of 200 m, FP64 ----
+    if not args.no_3d:
+        ca = box3d_cfg(capi, n=(96, 128, 128))
+        ca.x1max, ca.x2max, ca.x3max = 200.0 * 96, 200.0 * 128, 200.0 * 128
+        shp = ca.shape
+        z = (torch.arange(shp[3], dtype=torch.float64) - 3 + 0.5) * 200.0
+        T = 250.0 - 9.81 * z / 1004.5
+        p = 1e5 * (T / 250.0) ** (1004.5 / 287.0)
+        wa = torch.zeros(shp, dtype=torch.float64)
+        wa[4] = p
+        g2 = torch.Generator().manual_seed(99)
+        th = (torch.rand(shp[1:], generator=g2, dtype=torch.float64) - 0.5)
+        th[..., 23:] = 0.0
+        wa[0] = p / (287.0 * (T + th))
+        wad = wa.to(dev)
+        ua = torch.empty_like(wad)
+        capi.prim2cons(ca, wad, ua)
+        capi.apply_bc(ca, ua)
+        dta = 0.4
+        wsf = capi.rk3_workspace(ca, capi.F64)
+        wwa = torch.empty_like(ua)
+        capi.rk3_step(ca, dta, ua.clone(), wwa, wsf)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        uf = ua.clone()
+        for _ in range(3):
+            capi.rk3_step(ca, dta, uf, wwa, wsf)
+        torch.cuda.synchronize()
+        tf = (time.perf_counter() - t0) / 3
+        del wsf
+        wsa = capi.adjoint_workspace(ca)
+        lam_a = torch.randn(ua.shape, dtype=torch.float64, generator=g2).to(dev)
+        capi.adjoint_rk3_step(ca, dta, ua, lam_a.clone(), wsa)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            capi.adjoint_rk3_step(ca, dta, ua, lam_a, wsa)
+        torch.cuda.synchronize()
+        tadj = (time.perf_counter() - t0) / 3
+        extra["adjoint_box3d_128x128x96_fp64"] = {
+            "forward_ms_per_step": round(tf * 1e3, 3), "adjoint_ms_per_step": round(tadj * 1e3, 3),
+            "adjoint_over_forward": round(tadj / tf, 3),
+            "finite": bool(torch.isfinite(lam_a).all()),
+        }
+        del wsa, lam_a
 
     cpu = None if args.no_cpu else cpu_baseline()
 

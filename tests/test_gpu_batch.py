@@ -57,3 +57,54 @@ def test_batch_rejects_mixed_schemes():
     b = O.straka_config(nx2=16, nx1=8, implicit_scheme=9)
     with pytest.raises(capi.SnapxError, match="same implicit scheme"):
         capi.Batch([cfg_of(a), cfg_of(b)], 0.25)
+
+
+def test_batch_autotune_keeps_state_and_results():
+    """The autotuner times RHS launches only (u unchanged) and the batch it
+    plans reproduces the single-block fused path exactly."""
+    mem = _members(seed=5, n=4)
+    cfgs = [cfg_of(oc) for oc, _ in mem]
+    b = capi.Batch(cfgs, 0.25)
+    u = torch.cat([to_dev(u0, torch.float32).reshape(-1) for _, u0 in mem])
+    u_before = u.clone()
+    tile, ms = b.autotune(u, 0.25, trials=2)
+    assert torch.equal(u, u_before)
+    assert ms > 0 and tile[0] >= 16 and tile[2] == 1
+    b.steps(u, 2)
+    for m, (oc, u0) in enumerate(mem):
+        single = to_dev(u0, torch.float32)
+        capi.fused_rk3_steps(cfgs[m], 0.25, 2, single)
+        e = var_errors(host(b.member(u, m)), host(single), O.interior(oc))
+        # a different tile changes no arithmetic of a cell
+        assert (e == 0).all(), (m, tile, e)
+
+
+def test_batch_graph_replays_match_eager():
+    """Replays of a captured 2-step batch graph, with eager work and
+    synchronisations in between, each equal the eager steps."""
+    mem = _members(seed=9, n=3)
+    cfgs = [cfg_of(oc) for oc, _ in mem]
+    b = capi.Batch(cfgs, 0.25)
+    u0 = torch.cat([to_dev(u, torch.float32).reshape(-1) for _, u in mem])
+    ug = u0.clone()
+    g = b.graph(ug, 2)
+    ue = u0.clone()
+    for rep in range(3):
+        g.launch()
+        b.steps(ue, 2)
+        torch.cuda.synchronize()
+        assert torch.equal(ug, ue), rep
+        _ = (ue * 2.0).sum().item()  # unrelated eager work between replays
+
+
+def test_fused_graph_replays_match_eager():
+    oc, u_np = _members(seed=3, n=1)[0]
+    cfg = cfg_of(oc)
+    ug = to_dev(u_np, torch.float32)
+    ue = ug.clone()
+    g = capi.fused_graph(cfg, 0.25, 3, ug)
+    for rep in range(3):
+        g.launch()
+        capi.fused_rk3_steps(cfg, 0.25, 3, ue)
+        torch.cuda.synchronize()
+        assert torch.equal(ug, ue), rep
