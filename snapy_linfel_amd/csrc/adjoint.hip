@@ -33,6 +33,7 @@
 namespace snapx {
 
 constexpr int kT = 128;
+constexpr int kFluxDirs = 1;  // input directions per Riemann-adjoint pass
 
 struct AdjArgs {
   int n1, n2, n3, ng;
@@ -707,83 +708,130 @@ __device__ __forceinline__ int axis_stride(const AdjArgs a, int d) {
   return d == 0 ? 1 : (d == 1 ? a.n1 : a.n1 * a.n2);
 }
 
-// lambda_F at each face, Riemann adjoint (Dual<1> per input), x1 LR-revision adjoint;
-// writes lam_wlr [2][5][ncell] (ILT, IRT) for axis d
-__global__ void __launch_bounds__(kT)
-    flux_adj_kernel(const double* __restrict__ w, const double* __restrict__ lam_dx,
-                    double* __restrict__ lam_wlr, AdjArgs a, int d) {
-  const int ncell = a.n1 * a.n2 * a.n3;
-  const int s = axis_stride(a, d), n = axis_len(a, d);
+// Faces of axis d whose flux reaches an interior cell (a face f in
+// [ng, n - ng], never a dummy face), and lambda_F there in the face frame:
+// lambda_F = (-dt A / V)(lambda_dx[left cell] - lambda_dx[right cell]).
+__device__ __forceinline__ bool face_lambda(const AdjArgs a, int d, const double* lam_dx,
+                                            int ncell, int q, int s, const int* vmap,
+                                            int& f, double* lf) {
+  const DivMod dn1(a.n1), dn2(a.n2);
+  int i, j;
+  int k = dn2.div(dn1.div(q, i), j);
+  const int n = axis_len(a, d);
+  f = d == 0 ? i : (d == 1 ? j : k);
   double area = d == 0 ? a.dx3 * a.dx2 : (d == 1 ? a.dx3 * a.dx1 : a.dx2 * a.dx1);
-  double vol = a.dx3 * (a.dx2 * a.dx1);
-  double coef = -a.dt * area / vol;
-  const int solver = d == 0 ? a.riemann[0] : (d == 1 ? a.riemann[1] : a.riemann[2]);
-  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < ncell; q += gridDim.x * blockDim.x) {
-    int i, j, k;
-    decode(a, q, i, j, k);
-    int f = d == 0 ? i : (d == 1 ? j : k);
-    // lambda_F in the face frame
-    int vmap[5];
-    face_frame(d, vmap);
-    double lf[5] = {0., 0., 0., 0., 0.};
-    bool any = false;
-    if (f >= 1) {
-      int il = i - (d == 0), jl = j - (d == 1), kl = k - (d == 2);
-      if (interior(a, il, jl, kl)) {
+  double coef = -a.dt * area / (a.dx3 * (a.dx2 * a.dx1));
 #pragma unroll
-        for (int v = 0; v < 5; ++v) lf[v] += coef * lam_dx[vmap[v] * ncell + q - s];
-        any = true;
-      }
-    }
-    if (interior(a, i, j, k)) {
+  for (int v = 0; v < 5; ++v) lf[v] = 0.;
+  bool any = false;
+  if (f >= 1) {
+    int il = i - (d == 0), jl = j - (d == 1), kl = k - (d == 2);
+    if (interior(a, il, jl, kl)) {
 #pragma unroll
-      for (int v = 0; v < 5; ++v) lf[v] -= coef * lam_dx[vmap[v] * ncell + q];
+      for (int v = 0; v < 5; ++v) lf[v] += coef * lam_dx[vmap[v] * ncell + q - s];
       any = true;
     }
+  }
+  if (interior(a, i, j, k)) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) lf[v] -= coef * lam_dx[vmap[v] * ncell + q];
+    any = true;
+  }
+  return any && f >= a.ng && f <= n - a.ng;
+}
+
+// Interface states (face frame, x1 LR revision applied) of the faces that
+// carry an adjoint, into buf [2][5][ncell] (slot order; left, then right).
+__global__ void __launch_bounds__(kT)
+    face_state_kernel(const double* __restrict__ w, const double* __restrict__ lam_dx,
+                      double* __restrict__ buf, AdjArgs a, int d) {
+  const int ncell = a.n1 * a.n2 * a.n3;
+  const int s = axis_stride(a, d), n = axis_len(a, d);
+  int vmap[5];
+  face_frame(d, vmap);
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < ncell; q += gridDim.x * blockDim.x) {
+    int f;
+    double lf[5];
+    if (!face_lambda(a, d, lam_dx, ncell, q, s, vmap, f, lf)) continue;
+    double wl[5], wr[5];
+    face_states<double>(w, ncell, q, s, f, a, d, vmap, wl, wr);
+    if (d == 0 && a.grav1 != 0.) {
+      if (f == a.ng) wl[IDN] = wr[IDN], wl[IPR] = wr[IPR];
+      if (f == n - a.ng) wr[IDN] = wl[IDN], wr[IPR] = wl[IPR];
+    }
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      buf[v * ncell + q] = wl[v];
+      buf[(5 + v) * ncell + q] = wr[v];
+    }
+  }
+}
+
+// Riemann adjoint (Dual<kFluxDirs> passes) and x1 LR-revision adjoint of
+// every face of axis d: lam_wlr [2][5][ncell] (ILT, IRT; variable order)
+// replaces the face states face_state_kernel left there (same index).
+// One instantiation per solver: each holds only its own solver's registers.
+template <int SOLVER>
+__global__ void __launch_bounds__(kT)
+    flux_adj_kernel(const double* __restrict__ lam_dx, double* __restrict__ lam_wlr, AdjArgs a,
+                    int d) {
+  const int ncell = a.n1 * a.n2 * a.n3;
+  const int s = axis_stride(a, d), n = axis_len(a, d);
+  const int solver = SOLVER;
+  int vmap[5];
+  face_frame(d, vmap);
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < ncell; q += gridDim.x * blockDim.x) {
+    int f;
+    double lf[5];
+    bool act = face_lambda(a, d, lam_dx, ncell, q, s, vmap, f, lf);
     double lwl[5] = {0., 0., 0., 0., 0.}, lwr[5] = {0., 0., 0., 0., 0.};
-    if (any && f >= a.ng && f <= n - a.ng) {
+    if (act) {
       double wl[5], wr[5];
-      face_states<double>(w, ncell, q, s, f, a, d, vmap, wl, wr);
-      bool rev = d == 0 && a.grav1 != 0.;
-      if (rev && f == a.ng) {
-        wl[IDN] = wr[IDN];
-        wl[IPR] = wr[IPR];
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        wl[v] = lam_wlr[v * ncell + q];
+        wr[v] = lam_wlr[(5 + v) * ncell + q];
       }
-      if (rev && f == n - a.ng) {
-        wr[IDN] = wl[IDN];
-        wr[IPR] = wl[IPR];
-      }
-      // J^T lambda_F one input direction at a time (Dual<1>): the Roe solver
-      // on wider duals does not fit the register file
+      // J^T lambda_F, kFluxDirs input directions per pass: the solvers on
+      // wider duals do not fit the register file
 #pragma unroll 1
-      for (int kk = 0; kk < 10; ++kk) {
-        Dual<1> sl[5], sr[5], flx[5];
+      for (int p0 = 0; p0 < 10; p0 += kFluxDirs) {
+        Dual<kFluxDirs> sl[5], sr[5], flx[5];
 #pragma unroll
         for (int v = 0; v < 5; ++v) {
-          sl[v] = Dual<1>(wl[v]);
-          sr[v] = Dual<1>(wr[v]);
-          sl[v].d[0] = kk == v ? 1. : 0.;
-          sr[v].d[0] = kk == 5 + v ? 1. : 0.;
+          sl[v] = Dual<kFluxDirs>(wl[v]);
+          sr[v] = Dual<kFluxDirs>(wr[v]);
+#pragma unroll
+          for (int j = 0; j < kFluxDirs; ++j) {
+            sl[v].d[j] = p0 + j == v ? 1. : 0.;
+            sr[v].d[j] = p0 + j == 5 + v ? 1. : 0.;
+          }
         }
         // x1 frame: the solver's velocity slots are constants
-        riemann_flux<Dual<1>>(solver, sl, sr, Dual<1>(a.gammad), 3, flx, (Dual<1>*)nullptr);
-        double acc = 0.;
+        riemann_flux<Dual<kFluxDirs>>(solver, sl, sr, Dual<kFluxDirs>(a.gammad), 3, flx,
+                                      (Dual<kFluxDirs>*)nullptr);
 #pragma unroll
-        for (int m = 0; m < 5; ++m) acc += lf[m] * flx[m].d[0];
+        for (int j = 0; j < kFluxDirs; ++j) {
+          double acc = 0.;
 #pragma unroll
-        for (int v = 0; v < 5; ++v) {
-          if (kk == v) lwl[v] += acc;
-          if (kk == 5 + v) lwr[v] += acc;
+          for (int m = 0; m < 5; ++m) acc += lf[m] * flx[m].d[j];
+#pragma unroll
+          for (int v = 0; v < 5; ++v) {
+            if (p0 + j == v) lwl[v] += acc;
+            if (p0 + j == 5 + v) lwr[v] += acc;
+          }
         }
       }
       // LR revision adjoint: the copied states take the adjoint of the copies
-      if (rev && f == a.ng) {
-        lwr[IDN] += lwl[IDN], lwl[IDN] = 0.;
-        lwr[IPR] += lwl[IPR], lwl[IPR] = 0.;
-      }
-      if (rev && f == n - a.ng) {
-        lwl[IDN] += lwr[IDN], lwr[IDN] = 0.;
-        lwl[IPR] += lwr[IPR], lwr[IPR] = 0.;
+      if (d == 0 && a.grav1 != 0.) {
+        if (f == a.ng) {
+          lwr[IDN] += lwl[IDN], lwl[IDN] = 0.;
+          lwr[IPR] += lwl[IPR], lwl[IPR] = 0.;
+        }
+        if (f == n - a.ng) {
+          lwl[IDN] += lwr[IDN], lwr[IDN] = 0.;
+          lwl[IPR] += lwr[IPR], lwr[IPR] = 0.;
+        }
       }
     }
 #pragma unroll
@@ -794,7 +842,11 @@ __global__ void __launch_bounds__(kT)
   }
 }
 
-// lambda_w[c] += sum over faces whose interpolation window holds cell c
+// lambda_w[c] += sum over the faces whose interpolation window holds cell c
+// (faces f = c - m, m in [-3, 3]).  The 13 cells c-6..c+6 that those windows
+// span are loaded once; linear interpolants (DC, CP3, CP5) contribute their
+// constant weights, the others are differentiated with Dual<1> seeded at
+// cell c.
 __global__ void __launch_bounds__(kT)
     recon_adj_kernel(const double* __restrict__ w, const double* __restrict__ lam_wlr,
                      double* __restrict__ lam_w, AdjArgs a, int d) {
@@ -806,25 +858,55 @@ __global__ void __launch_bounds__(kT)
   int type = r ? a.recon_type[1] : a.recon_type[0];
   int scale = r ? a.recon_scale[1] : a.recon_scale[0];
   int shock = r ? a.recon_shock[1] : a.recon_shock[0];
+  const DivMod dn1(a.n1), dn2(a.n2), dnc(ncell);
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < 5 * ncell;
        t += gridDim.x * blockDim.x) {
-    int v = t / ncell, q = t - v * ncell;
-    int i, j, k;
-    decode(a, q, i, j, k);
+    int q;
+    int v = dnc.div(t, q);
+    int i, j;
+    int k = dn2.div(dn1.div(q, i), j);
     int c = d == 0 ? i : (d == 1 ? j : k);
     int tv = (!shock && v >= IVX && v <= IPR) ? interp2_type(type) : type;
+    bool linear = tv == SNAPX_INTERP_DC || tv == SNAPX_INTERP_CP3 || tv == SNAPX_INTERP_CP5;
+    const double* wv = w + (int64_t)v * ncell + q;
+    double cells[13];
+#pragma unroll
+    for (int o = -6; o <= 6; ++o) {
+      int cc = c + o;
+      cc = cc < 0 ? 0 : (cc >= n ? n - 1 : cc);
+      cells[o + 6] = wv[(cc - c) * s];
+    }
+    // constant weights of a linear interpolant on its 7-cell window
+    double clo[7], chi[7];
+    if (linear) {
+      Dual<7> win[7];
+#pragma unroll
+      for (int m = 0; m < 7; ++m) win[m] = Dual<7>::seed(0.0, m);
+      Dual<7> lo = interp_face<Dual<7>>(tv, 0, 0, win, 1, 3);
+      Dual<7> hi = interp_face<Dual<7>>(tv, 1, 0, win, 1, 2);
+#pragma unroll
+      for (int m = 0; m < 7; ++m) clo[m] = lo.d[m], chi[m] = hi.d[m];
+    }
     double acc = 0.;
-    for (int f = c - 3; f <= c + 3; ++f) {
+#pragma unroll
+    for (int m = -3; m <= 3; ++m) {
+      int f = c - m;  // cell c sits at window position m + 3 of face f
       if (f < a.ng || f > n - a.ng) continue;
-      double lr = lam_wlr[(5 + v) * ncell + q + (f - c) * s];
-      double ll = lam_wlr[v * ncell + q + (f - c) * s];
+      double lr = lam_wlr[(5 + v) * ncell + q - m * s];
+      double ll = lam_wlr[v * ncell + q - m * s];
       if (lr == 0. && ll == 0.) continue;
+      if (linear) {
+        acc += lr * clo[m + 3] + ll * chi[m + 3];
+        continue;
+      }
+      // window of face f: cells f-3..f+3 = cells[3 - m .. 9 - m]; the
+      // clamped copies of an edge cell are the same cell
       Dual<1> win[7];
-      for (int m = -3; m <= 3; ++m) {
-        int cc = f + m;
-        cc = cc < 0 ? 0 : (cc >= n ? n - 1 : cc);
-        win[m + 3] = cc == c ? Dual<1>::seed(w[v * ncell + q + (cc - c) * s], 0)
-                             : Dual<1>(w[v * ncell + q + (cc - c) * s]);
+#pragma unroll
+      for (int p = 0; p < 7; ++p) {
+        int cc = f - 3 + p;
+        int ccl = cc < 0 ? 0 : (cc >= n ? n - 1 : cc);
+        win[p] = ccl == c ? Dual<1>::seed(cells[3 - m + p], 0) : Dual<1>(cells[3 - m + p]);
       }
       if (lr != 0.) acc += lr * interp_face<Dual<1>>(tv, 0, scale, win, 1, 3).d[0];
       if (ll != 0.) acc += ll * interp_face<Dual<1>>(tv, 1, scale, win, 1, 2).d[0];
@@ -1145,7 +1227,18 @@ extern "C" int snapx_adjoint_rk3_step(const snapx_config* cfg, double dt, const 
     SNAPX_LAUNCH_CHECK();
     for (int d = 0; d < 3; ++d) {
       if (ns[d] == 1) continue;
-      hipLaunchKernelGGL(flux_adj_kernel, bc, dim3(kT), 0, st, w, s.lam_dx, s.lam_wlr, a, d);
+      hipLaunchKernelGGL(face_state_kernel, bc, dim3(kT), 0, st, w, s.lam_dx, s.lam_wlr, a, d);
+      SNAPX_LAUNCH_CHECK();
+      int solver = cfg->riemann[d];
+      if (solver == SNAPX_RIEMANN_LMARS)
+        hipLaunchKernelGGL(flux_adj_kernel<SNAPX_RIEMANN_LMARS>, bc, dim3(kT), 0, st, s.lam_dx,
+                           s.lam_wlr, a, d);
+      else if (solver == SNAPX_RIEMANN_HLLC)
+        hipLaunchKernelGGL(flux_adj_kernel<SNAPX_RIEMANN_HLLC>, bc, dim3(kT), 0, st, s.lam_dx,
+                           s.lam_wlr, a, d);
+      else
+        hipLaunchKernelGGL(flux_adj_kernel<SNAPX_RIEMANN_ROE>, bc, dim3(kT), 0, st, s.lam_dx,
+                           s.lam_wlr, a, d);
       SNAPX_LAUNCH_CHECK();
       hipLaunchKernelGGL(recon_adj_kernel, bn, dim3(kT), 0, st, w, s.lam_wlr, s.lam_w, a, d);
       SNAPX_LAUNCH_CHECK();

@@ -86,4 +86,21 @@ inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s);
 // Validates a config for the kernels of this library.
 int validate_config(const snapx_config* c);
 
+// Quotient and remainder of a small non-negative index by a positive
+// divisor through the FP32 reciprocal and one correction step (an integer
+// division by a runtime divisor costs some forty instructions).  Exact for
+// 0 <= l < 2^22.
+struct DivMod {
+  int d;
+  float inv;
+  __device__ __forceinline__ explicit DivMod(int dd) : d(dd), inv(1.0f / (float)dd) {}
+  __device__ __forceinline__ int div(int l, int& r) const {
+    int q = (int)((float)l * inv);
+    r = l - q * d;
+    if (r < 0) r += d, q -= 1;
+    if (r >= d) r -= d, q += 1;
+    return q;
+  }
+};
+
 }  // namespace snapx

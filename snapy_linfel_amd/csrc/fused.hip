@@ -115,23 +115,6 @@ __device__ __forceinline__ void cons_to_prim(const T* U, T gammad, T* W) {
   W[IPR] = (gammad - T(1.)) * (U[IPR] - ke);
 }
 
-// Quotient and remainder of a small non-negative index by a positive
-// divisor through the FP32 reciprocal and one correction step (an integer
-// division by a runtime divisor costs some forty instructions).  Exact for
-// l < 2^22.
-struct DivMod {
-  int d;
-  float inv;
-  __device__ __forceinline__ explicit DivMod(int dd) : d(dd), inv(1.0f / (float)dd) {}
-  __device__ __forceinline__ int div(int l, int& r) const {
-    int q = (int)((float)l * inv);
-    r = l - q * d;
-    if (r < 0) r += d, q -= 1;
-    if (r >= d) r -= d, q += 1;
-    return q;
-  }
-};
-
 // K1 body for one tile; every thread of the workgroup calls it with the same
 // tile (it synchronises the workgroup internally).  The arguments come by
 // value: a reference to the kernel argument makes the compiler keep a copy
