@@ -409,12 +409,13 @@ __device__ __forceinline__ void vic_row_blocks(const double* __restrict__ abc, i
 }
 
 // Forward column solve + MS-VIC redistribution of one stage (the recompute
-// of the stage states): du (interior) from the explicit tendency dux.
+// of the stage states): du (interior) from the explicit tendency dux; delta
+// is kept in the first half of dl [col][layer][2N] for the reverse sweep.
 template <int N>
 __global__ void __launch_bounds__(PcrShape<N>::kThreads)
     vic_fwd_solve_kernel(const double* __restrict__ w, const double* __restrict__ dux,
-                         const double* __restrict__ abc, double* __restrict__ du, AdjArgs a,
-                         int ncol) {
+                         const double* __restrict__ abc, double* __restrict__ du,
+                         double* __restrict__ dl, AdjArgs a, int ncol) {
   extern __shared__ double sh[];
   const int NN = N * N;
   const int ncell = a.n1 * a.n2 * a.n3, nl = a.nx1;
@@ -434,6 +435,9 @@ __global__ void __launch_bounds__(PcrShape<N>::kThreads)
   double sums[2] = {ai * ai, L.act ? (xr - d[0]) * 1.0 * vol : 0.};
   column_sums<2>(sums, sh, threadIdx.x, a.lanes, a.lshift);
   if (!L.act) return;
+  double* cdl = dl + ((int64_t)L.col * nl + L.i) * 2 * N;
+#pragma unroll
+  for (int e = 0; e < N; ++e) cdl[e] = d[e];
   du[IDN * ncell + L.q] = (xr + (d[0] - xr)) + rho * ai * sums[1] / sums[0];
   du[IVX * ncell + L.q] = d[1];
   if (N == 3) {
@@ -445,10 +449,11 @@ __global__ void __launch_bounds__(PcrShape<N>::kThreads)
   }
 }
 
-// Adjoint of the column solve + redistribution: delta = M^-1 (dux / dt),
-// redistribution adjoint (column sums S, Bd, G), y = M^-T lambda_delta.
-// Writes dl [col][layer][2N] = (delta, y), lam_dx (interior) and the rho
-// part of lam_w; lam_dx enters holding lam_du.
+// Adjoint of the column solve + redistribution, with delta = M^-1 (dux / dt)
+// from the forward recompute (first half of dl): the redistribution adjoint
+// (column sums S, Bd, G) and y = M^-T lambda_delta by PCR.  Writes y to the
+// second half of dl, lam_dx (interior) and the rho part of lam_w; lam_dx
+// enters holding lam_du.
 template <int N>
 __global__ void __launch_bounds__(PcrShape<N>::kThreads)
     vic_adj_solve_kernel(const double* __restrict__ w, const double* __restrict__ dux,
@@ -460,23 +465,17 @@ __global__ void __launch_bounds__(PcrShape<N>::kThreads)
   const int ncell = a.n1 * a.n2 * a.n3, nl = a.nx1;
   const double vol = a.dx3 * (a.dx2 * a.dx1);
   ColLane L = col_lane(a, ncol);
-  double A[NN], B[NN], C[NN], r[N], d[N];
-  if (L.act) {
-    vic_row_blocks<N>(abc, L.col, L.i, nl, false, A, B, C);
-    vic_pick<N>(dux, ncell, L.q, r);
-#pragma unroll
-    for (int e = 0; e < N; ++e) r[e] /= a.dt;
-  }
-  pcr_solve<N>(A, B, C, r, d, sh, blockDim.x, threadIdx.x, L.i, nl, L.act);
+  double* cdl = dl + ((int64_t)L.col * nl + L.i) * 2 * N;
+  double d0 = L.act ? cdl[0] : 0.;
   double rho = L.act ? w[IDN * ncell + L.q] : 0.;
   double lr = L.act ? lam_du[IDN * ncell + L.q] : 0.;
   double ai = rho * vol;
-  double sums[3] = {ai * ai, L.act ? (dux[IDN * ncell + L.q] - d[0]) * 1.0 * vol : 0.,
+  double sums[3] = {ai * ai, L.act ? (dux[IDN * ncell + L.q] - d0) * 1.0 * vol : 0.,
                     lr * rho * ai};
   column_sums<3>(sums, sh, threadIdx.x, a.lanes, a.lshift);
   const double S = sums[0], Bd = sums[1], G = sums[2] / S;
   const double lamS = -G * Bd / S;
-  double ld[N];
+  double A[NN], B[NN], C[NN], ld[N];
   if (L.act) {
     vic_pick<N>(lam_du, ncell, L.q, ld);
     ld[0] = lr - G * vol;
@@ -486,12 +485,8 @@ __global__ void __launch_bounds__(PcrShape<N>::kThreads)
   double y[N];
   pcr_solve<N>(A, B, C, ld, y, sh, blockDim.x, threadIdx.x, L.i, nl, L.act);
   if (!L.act) return;
-  double* cdl = dl + ((int64_t)L.col * nl + L.i) * 2 * N;
 #pragma unroll
-  for (int e = 0; e < N; ++e) {
-    cdl[e] = d[e];
-    cdl[N + e] = y[e];
-  }
+  for (int e = 0; e < N; ++e) cdl[N + e] = y[e];
   lam_dx[IDN * ncell + L.q] = G * vol + y[0] / a.dt;
   lam_dx[IVX * ncell + L.q] = y[1] / a.dt;
   if (N == 3) {
@@ -867,14 +862,18 @@ __global__ void __launch_bounds__(kT)
     int k = dn2.div(dn1.div(q, i), j);
     int c = d == 0 ? i : (d == 1 ? j : k);
     int tv = (!shock && v >= IVX && v <= IPR) ? interp2_type(type) : type;
+    // WENO5 without scaling: weno_face_kernel + weno_gather_kernel
+    if (tv == SNAPX_INTERP_WENO5 && !scale) continue;
     bool linear = tv == SNAPX_INTERP_DC || tv == SNAPX_INTERP_CP3 || tv == SNAPX_INTERP_CP5;
     const double* wv = w + (int64_t)v * ncell + q;
     double cells[13];
+    if (!linear) {
 #pragma unroll
-    for (int o = -6; o <= 6; ++o) {
-      int cc = c + o;
-      cc = cc < 0 ? 0 : (cc >= n ? n - 1 : cc);
-      cells[o + 6] = wv[(cc - c) * s];
+      for (int o = -6; o <= 6; ++o) {
+        int cc = c + o;
+        cc = cc < 0 ? 0 : (cc >= n ? n - 1 : cc);
+        cells[o + 6] = wv[(cc - c) * s];
+      }
     }
     // constant weights of a linear interpolant on its 7-cell window
     double clo[7], chi[7];
@@ -912,6 +911,111 @@ __global__ void __launch_bounds__(kT)
       if (ll != 0.) acc += ll * interp_face<Dual<1>>(tv, 1, scale, win, 1, 2).d[0];
     }
     lam_w[t] += acc;
+  }
+}
+
+// Gradient of the WENO5 lower-face value (weno5_lo, WENO5-JS) with respect
+// to its window f0..f4, in closed form:
+//   v = sum_k w_k p_k,  w_k = a_k / sum a,  a_k = d_k / (b_k + eps)^2,
+//   dv/df = sum_k w_k [dp_k/df - 2 (p_k - v) / (b_k + eps) db_k/df],
+//   b_k = 13/12 L1_k^2 + 1/4 L2_k^2 with L1_k, L2_k linear in f.
+// The upper-face value is the lower-face value of the mirrored window.
+__device__ __forceinline__ void weno5_lo_grad(double f0, double f1, double f2, double f3,
+                                              double f4, double* g) {
+  const double c13 = 13. / 12., eps = 1e-6;
+  const double p[3] = {(-1. / 6.) * f0 + (5. / 6.) * f1 + (1. / 3.) * f2,
+                       (1. / 3.) * f1 + (5. / 6.) * f2 + (-1. / 6.) * f3,
+                       (11. / 6.) * f2 + (-7. / 6.) * f3 + (1. / 3.) * f4};
+  const double L1[3] = {f0 - 2. * f1 + f2, f1 - 2. * f2 + f3, f2 - 2. * f3 + f4};
+  const double L2[3] = {f0 - 4. * f1 + 3. * f2, -f1 + f3, 3. * f2 - 4. * f3 + f4};
+  const double dk[3] = {.3, .6, .1};
+  double a[3], be[3], S = 0.;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    be[k] = c13 * L1[k] * L1[k] + .25 * L2[k] * L2[k] + eps;
+    a[k] = dk[k] / (be[k] * be[k]);
+    S += a[k];
+  }
+  const double iS = 1. / S;
+  const double v = (a[0] * p[0] + a[1] * p[1] + a[2] * p[2]) * iS;
+  // d p_k / d f_m, d L1_k / d f_m, d L2_k / d f_m
+  const double dp[3][5] = {{-1. / 6., 5. / 6., 1. / 3., 0., 0.},
+                           {0., 1. / 3., 5. / 6., -1. / 6., 0.},
+                           {0., 0., 11. / 6., -7. / 6., 1. / 3.}};
+  const double d1[3][5] = {{1., -2., 1., 0., 0.}, {0., 1., -2., 1., 0.}, {0., 0., 1., -2., 1.}};
+  const double d2[3][5] = {{1., -4., 3., 0., 0.}, {0., -1., 0., 1., 0.}, {0., 0., 3., -4., 1.}};
+  double wk[3], hk[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    wk[k] = a[k] * iS;
+    hk[k] = -2. * wk[k] * (p[k] - v) / be[k];
+  }
+#pragma unroll
+  for (int m = 0; m < 5; ++m) {
+    double acc = 0.;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      acc += wk[k] * dp[k][m] + hk[k] * (2. * c13 * L1[k] * d1[k][m] + .5 * L2[k] * d2[k][m]);
+    g[m] = acc;
+  }
+}
+
+// Face f of axis d, variable v (WENO5, no scaling): the contributions of its
+// two interface states to the six cells f-3..f+2 of their windows,
+//   G6[k][q] = ll d(hi of cell f-1)/dw_{f-3+k} + lr d(lo of cell f)/dw_{f-3+k},
+// zero for faces that carry no adjoint.
+__global__ void __launch_bounds__(kT)
+    weno_face_kernel(const double* __restrict__ w, const double* __restrict__ lam_wlr,
+                     double* __restrict__ G6, AdjArgs a, int d, int v) {
+  const int ncell = a.n1 * a.n2 * a.n3;
+  const int s = axis_stride(a, d), n = axis_len(a, d);
+  const DivMod dn1(a.n1), dn2(a.n2);
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < ncell; q += gridDim.x * blockDim.x) {
+    int i, j;
+    int k = dn2.div(dn1.div(q, i), j);
+    int f = d == 0 ? i : (d == 1 ? j : k);
+    double g6[6] = {0., 0., 0., 0., 0., 0.};
+    double ll = lam_wlr[v * ncell + q], lr = lam_wlr[(5 + v) * ncell + q];
+    if (f >= a.ng && f <= n - a.ng && (ll != 0. || lr != 0.)) {
+      const double* wv = w + (int64_t)v * ncell + q;
+      double c6[6];
+#pragma unroll
+      for (int m = 0; m < 6; ++m) c6[m] = wv[(m - 3) * s];  // cells f-3 .. f+2
+      double gl[5], gr[5];
+      // left state: upper face of cell f-1, window f-3..f+1, mirrored
+      weno5_lo_grad(c6[4], c6[3], c6[2], c6[1], c6[0], gl);
+      // right state: lower face of cell f, window f-2..f+2
+      weno5_lo_grad(c6[1], c6[2], c6[3], c6[4], c6[5], gr);
+#pragma unroll
+      for (int m = 0; m < 5; ++m) {
+        g6[m] += ll * gl[4 - m];
+        g6[m + 1] += lr * gr[m];
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < 6; ++m) G6[(int64_t)m * ncell + q] = g6[m];
+  }
+}
+
+// lam_w[v][c] += sum over faces f = c+3-k (k = 0..5) carrying an adjoint of
+// G6[k][f]
+__global__ void __launch_bounds__(kT)
+    weno_gather_kernel(const double* __restrict__ G6, double* __restrict__ lam_w, AdjArgs a,
+                       int d, int v) {
+  const int ncell = a.n1 * a.n2 * a.n3;
+  const int s = axis_stride(a, d), n = axis_len(a, d);
+  const DivMod dn1(a.n1), dn2(a.n2);
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < ncell; q += gridDim.x * blockDim.x) {
+    int i, j;
+    int k = dn2.div(dn1.div(q, i), j);
+    int c = d == 0 ? i : (d == 1 ? j : k);
+    double acc = 0.;
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+      int f = c + 3 - m;
+      if (f >= a.ng && f <= n - a.ng) acc += G6[(int64_t)m * ncell + q + (3 - m) * s];
+    }
+    lam_w[(int64_t)v * ncell + q] += acc;
   }
 }
 
@@ -1037,7 +1141,7 @@ static size_t al(size_t x) { return (x + 255) & ~size_t(255); }
 
 struct AdjSpace {
   double *us[3], *w[3], *dux[3], *lam, *lam_us, *lam_u0, *lam_du, *lam_dx, *lam_w, *lam_wlr,
-      *abc, *pr, *dl, *G;
+      *abc[3], *pr, *dl[3], *G, *G6;
   char* hydro_ws;
   size_t hydro_bytes, bytes;
 };
@@ -1067,10 +1171,13 @@ static AdjSpace adj_layout(const snapx_config* c, char* base) {
   s.lam_dx = take(f);
   s.lam_w = take(f);
   s.lam_wlr = take(2 * f);
-  s.abc = take(sizeof(double) * cells * 3 * N * N);
+  for (int k = 0; k < 3; ++k) {
+    s.abc[k] = take(sizeof(double) * cells * 3 * N * N);
+    s.dl[k] = take(sizeof(double) * cells * 2 * N);
+  }
   s.pr = take(sizeof(double) * cells * (N * N + N));
-  s.dl = take(sizeof(double) * cells * 2 * N);
   s.G = take(sizeof(double) * ncol * (c->nx1 + 1) * 10);
+  s.G6 = take(6 * f / 5);
   s.hydro_bytes = snapx_hydro_workspace_bytes(c, SNAPX_F64);
   s.hydro_ws = reinterpret_cast<char*>(take(s.hydro_bytes));
   s.bytes = off;
@@ -1108,41 +1215,39 @@ static bool adj_pcr(const AdjArgs a, int N) {
 }
 
 template <int N>
-static int launch_vic_fwd(const AdjArgs a, const AdjSpace& s, int k, double* du, int ncol,
+static int launch_vic_fwd(const AdjArgs& a, const AdjSpace& s, int k, double* du, int ncol,
                           hipStream_t st) {
   const int T = PcrShape<N>::kThreads;
   hipLaunchKernelGGL(vic_asm_kernel<N>, blocks_for((int64_t)ncol * a.nx1, kT), dim3(kT), 0, st,
-                     s.w[k], s.abc, a, ncol);
+                     s.w[k], s.abc[k], a, ncol);
   SNAPX_LAUNCH_CHECK();
   int cpw = T >> a.lshift;
   hipLaunchKernelGGL(vic_fwd_solve_kernel<N>, dim3((ncol + cpw - 1) / cpw), dim3(T),
-                     sizeof(double) * T * PcrShape<N>::kPub, st, s.w[k], s.dux[k], s.abc, du, a,
-                     ncol);
+                     sizeof(double) * T * PcrShape<N>::kPub, st, s.w[k], s.dux[k], s.abc[k], du,
+                     s.dl[k], a, ncol);
   SNAPX_LAUNCH_CHECK();
   return 0;
 }
 
 template <int N>
-static int launch_vic_adj(const AdjArgs a, const AdjSpace& s, int k, int ncol, hipStream_t st) {
+static int launch_vic_adj(const AdjArgs& a, const AdjSpace& s, int k, int ncol, hipStream_t st) {
   const int T = PcrShape<N>::kThreads;
   if (adj_pcr(a, N)) {
-    hipLaunchKernelGGL(vic_asm_kernel<N>, blocks_for((int64_t)ncol * a.nx1, kT), dim3(kT), 0,
-                       st, s.w[k], s.abc, a, ncol);
-    SNAPX_LAUNCH_CHECK();
+    // blocks and delta of this stage come from the forward recompute
     int cpw = T >> a.lshift;
     hipLaunchKernelGGL(vic_adj_solve_kernel<N>, dim3((ncol + cpw - 1) / cpw), dim3(T),
                        sizeof(double) * T * PcrShape<N>::kPub, st, s.w[k], s.dux[k], s.lam_du,
-                       s.abc, s.lam_dx, s.lam_w, s.dl, a, ncol);
+                       s.abc[k], s.lam_dx, s.lam_w, s.dl[k], a, ncol);
   } else {
     hipLaunchKernelGGL(vic_adj_column_kernel<N>, blocks_for(ncol, 64), dim3(64), 0, st, s.w[k],
-                       s.dux[k], s.lam_du, s.lam_dx, s.lam_w, s.abc, s.pr, s.dl, a, ncol);
+                       s.dux[k], s.lam_du, s.lam_dx, s.lam_w, s.abc[k], s.pr, s.dl[k], a, ncol);
   }
   SNAPX_LAUNCH_CHECK();
   hipLaunchKernelGGL(vic_adj_face_kernel<N>, blocks_for((int64_t)ncol * (a.nx1 + 1), kT),
-                     dim3(kT), 0, st, s.w[k], s.dl, s.G, a, ncol);
+                     dim3(kT), 0, st, s.w[k], s.dl[k], s.G, a, ncol);
   SNAPX_LAUNCH_CHECK();
   hipLaunchKernelGGL(vic_adj_cell_kernel<N>, blocks_for((int64_t)ncol * (a.nx1 + 2), kT),
-                     dim3(kT), 0, st, s.w[k], s.dl, s.G, s.lam_w, a, ncol);
+                     dim3(kT), 0, st, s.w[k], s.dl[k], s.G, s.lam_w, a, ncol);
   SNAPX_LAUNCH_CHECK();
   return 0;
 }
@@ -1181,8 +1286,8 @@ extern "C" int snapx_adjoint_rk3_step(const snapx_config* cfg, double dt, const 
     if (int e = snapx_hydro_forward(&c0, dt, s.us[k], s.w[k], s.dux[k], nullptr, nullptr,
                                     nullptr, s.hydro_ws, s.hydro_bytes, SNAPX_F64, st))
       return e;
-    if (k == 2) break;
     // du of the stage: the column solve on a copy of the explicit tendency
+    // (stage 2 only needs the blocks and delta it leaves for the reverse sweep)
     double* du = s.lam_du;
     SNAPX_HIP(hipMemcpyAsync(du, s.dux[k], fb, hipMemcpyDeviceToDevice, st));
     if (cfg->implicit_scheme) {
@@ -1190,12 +1295,13 @@ extern "C" int snapx_adjoint_rk3_step(const snapx_config* cfg, double dt, const 
         if (int e = N == 3 ? launch_vic_fwd<3>(a, s, k, du, ncol, st)
                            : launch_vic_fwd<5>(a, s, k, du, ncol, st))
           return e;
-      } else if (int e = snapx_hydro_forward(cfg, dt, s.us[k], s.lam_w, du, nullptr, nullptr,
-                                             nullptr, s.hydro_ws, s.hydro_bytes, SNAPX_F64,
-                                             st)) {
-        return e;
+      } else if (k < 2) {
+        if (int e = snapx_hydro_forward(cfg, dt, s.us[k], s.lam_w, du, nullptr, nullptr,
+                                        nullptr, s.hydro_ws, s.hydro_bytes, SNAPX_F64, st))
+          return e;
       }
     }
+    if (k == 2) break;
     if (int e = snapx_rk_combine(s.us[k + 1], s.us[0], s.us[k], du, W[k][0], W[k][1], W[k][2],
                                  n, SNAPX_F64, st))
       return e;
@@ -1242,6 +1348,17 @@ extern "C" int snapx_adjoint_rk3_step(const snapx_config* cfg, double dt, const 
       SNAPX_LAUNCH_CHECK();
       hipLaunchKernelGGL(recon_adj_kernel, bn, dim3(kT), 0, st, w, s.lam_wlr, s.lam_w, a, d);
       SNAPX_LAUNCH_CHECK();
+      int r = d == 0 ? 0 : 1;
+      if (cfg->recon_type[r] == SNAPX_INTERP_WENO5 && !cfg->recon_scale[r]) {
+        for (int v = 0; v < 5; ++v) {
+          bool weno = cfg->recon_shock[r] || v == IDN;
+          if (!weno) continue;
+          hipLaunchKernelGGL(weno_face_kernel, bc, dim3(kT), 0, st, w, s.lam_wlr, s.G6, a, d, v);
+          SNAPX_LAUNCH_CHECK();
+          hipLaunchKernelGGL(weno_gather_kernel, bc, dim3(kT), 0, st, s.G6, s.lam_w, a, d, v);
+          SNAPX_LAUNCH_CHECK();
+        }
+      }
     }
     if (g.n1 > 1 && cfg->grav1 != 0.) {
       hipLaunchKernelGGL(ghost_adj_kernel, blocks_for(g.n2 * g.n3, kT), dim3(kT), 0, st, w,
