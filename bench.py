@@ -90,6 +90,90 @@ def ensemble_members(capi, n=96, seed=2024):
     return cfgs, ws
 
 
+def c5_sensitivity(capi, dev, nsteps=200, every=20, ndir=2):
+    """Adjoint sensitivity on the 3-D box 128 x 128 x 96 of 200 m (FP64):
+    loss = 0.5 sum (theta_final - theta_target)^2 after nsteps SSP-RK3 steps,
+    gradient with respect to the initial theta perturbation of the lowest 20
+    levels, through the hand-written adjoint with a checkpoint every `every`
+    steps; checked against central differences (step 1e-6 K) along `ndir`
+    seeded directions."""
+    import torch
+    from snapy_linfel_amd import adjoint as A
+    R, cp, g0, p0 = 287.0, 1004.5, 9.81, 1e5
+    cfg = box3d_cfg(capi, n=(96, 128, 128))
+    cfg.x1max, cfg.x2max, cfg.x3max = 200.0 * 96, 200.0 * 128, 200.0 * 128
+    shp = cfg.shape
+    z = (torch.arange(shp[3], dtype=torch.float64, device=dev) - 3 + 0.5) * 200.0
+    T0 = 250.0 - g0 * z / cp
+    pb = p0 * (T0 / 250.0) ** (cp / R)
+    thb = T0 * (p0 / pb) ** (R / cp)
+    gam = cfg.gammad
+
+    def state(dth):
+        th = thb.expand(shp[1:]).clone()
+        th = th + torch.nn.functional.pad(dth, (3, shp[3] - 23, 3, 3, 3, 3))
+        T = th * (pb / p0) ** (R / cp)
+        rho = pb / (R * T)
+        zero = torch.zeros_like(rho)
+        return torch.stack([rho, zero, zero, zero, (pb / (gam - 1.0)).expand_as(rho)])
+
+    def theta(u):
+        rho = u[0]
+        ke = 0.5 * (u[1] ** 2 + u[2] ** 2 + u[3] ** 2) / rho
+        p = (gam - 1.0) * (u[4] - ke)
+        return p / (rho * R) * (p0 / p) ** (R / cp)
+
+    gen = torch.Generator(device="cpu").manual_seed(1234)
+    cshape = (shp[1] - 6, shp[2] - 6, 20)  # interior columns, lowest 20 levels
+    dth0 = ((torch.rand(cshape, generator=gen, dtype=torch.float64) - 0.5)).to(dev)
+    dtt = ((torch.rand(cshape, generator=gen, dtype=torch.float64) - 0.5)).to(dev)
+    dt = 0.4
+    ws_f = capi.rk3_workspace(cfg, capi.F64)
+    w_f = torch.empty(shp, dtype=torch.float64, device=dev)
+
+    def run(dth):
+        u = state(dth)
+        for _ in range(nsteps):
+            capi.rk3_step(cfg, dt, u, w_f, ws_f)
+        return u
+
+    th_target = theta(run(dtt))
+
+    def loss_of(u):
+        return 0.5 * ((theta(u) - th_target)[3:-3, 3:-3, 3:-3] ** 2).sum()
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    L0 = float(loss_of(run(dth0)))
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    x = dth0.clone().requires_grad_(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    y = A.integrate(state(x), cfg, dt, nsteps, checkpoint_every=every)
+    loss = loss_of(y)
+    (grad,) = torch.autograd.grad(loss, x)
+    torch.cuda.synchronize()
+    t_grad = time.perf_counter() - t0
+    rels = []
+    for k in range(ndir):
+        d = torch.randn(cshape, generator=gen, dtype=torch.float64).to(dev)
+
+        def central(eps):
+            return (float(loss_of(run(dth0 + eps * d))) - float(loss_of(run(dth0 - eps * d)))) \
+                / (2 * eps)
+        # the flow crosses branch points of the schemes (upwind switches,
+        # |u|) at some faces within any finite step, so differences converge
+        # at first order over many steps: a small step in FP64
+        fd = central(1e-6)
+        ad = float((grad * d).sum())
+        rels.append(abs(ad - fd) / abs(fd))
+    return {"steps": nsteps, "checkpoint_every": every, "loss": L0,
+            "forward_s": round(t_fwd, 3), "forward_plus_adjoint_s": round(t_grad, 3),
+            "gradient_over_forward": round(t_grad / t_fwd, 3),
+            "fd_directions": ndir, "fd_rel_err_max": max(rels)}
+
+
 def measured_traffic(kernel, workload):
     """HBM bytes per launch of a kernel, as recorded from the FETCH_SIZE and
     WRITE_SIZE counters in profiles/fused_kernels_pmc.md (None if absent)."""
@@ -359,6 +443,7 @@ def main():
             capi.adjoint_rk3_step(ca, dta, ua, lam_a, wsa)
         torch.cuda.synchronize()
         tadj = (time.perf_counter() - t0) / 3
+        extra["c5_sensitivity_box3d_128x128x96_fp64"] = c5_sensitivity(capi, dev)
         extra["adjoint_box3d_128x128x96_fp64"] = {
             "forward_ms_per_step": round(tf * 1e3, 3), "adjoint_ms_per_step": round(tadj * 1e3, 3),
             "adjoint_over_forward": round(tadj / tf, 3),

@@ -48,11 +48,30 @@ def rk3_step(u, cfg, dt, ws=None):
     return RK3Step.apply(u, cfg, dt, ws)
 
 
-def integrate(u, cfg, dt, nsteps, ws=None):
+def integrate(u, cfg, dt, nsteps, ws=None, checkpoint_every=None):
     """``nsteps`` differentiable steps; one adjoint workspace is shared by
-    every backward step (they run one after another on the stream)."""
+    every backward step (they run one after another on the stream).
+
+    With ``checkpoint_every = K`` only every K-th state is kept: the backward
+    of a segment of K steps first replays its forward steps from the
+    segment's first state (torch.utils.checkpoint), so memory holds
+    nsteps / K + K states instead of nsteps."""
     if ws is None and torch.is_grad_enabled() and u.requires_grad:
         ws = capi.adjoint_workspace(cfg, u.device)
-    for _ in range(nsteps):
-        u = RK3Step.apply(u, cfg, dt, ws)
+    if not checkpoint_every or checkpoint_every >= nsteps or not torch.is_grad_enabled():
+        for _ in range(nsteps):
+            u = RK3Step.apply(u, cfg, dt, ws)
+        return u
+    from torch.utils.checkpoint import checkpoint
+
+    def segment(x, n):
+        for _ in range(n):
+            x = RK3Step.apply(x, cfg, dt, ws)
+        return x
+
+    done = 0
+    while done < nsteps:
+        n = min(checkpoint_every, nsteps - done)
+        u = checkpoint(segment, u, n, use_reentrant=False)
+        done += n
     return u

@@ -228,3 +228,20 @@ def test_adjoint_matches_torch_autograd(case):
     err = np.abs(got - g_ref).reshape(5, -1).max(axis=1) / np.maximum(gmax, gmax.max() * 1e-12)
     print(case, "relative error per variable", err)
     assert (err < 1e-9).all(), err
+
+
+def test_segment_checkpointing_gives_the_same_gradient():
+    """integrate(..., checkpoint_every=3) over 7 steps: the same gradient, bit
+    for bit, as keeping every step input."""
+    oc, u_np, dt = _case("straka1")
+    cfg = cfg_of(oc)
+    u0 = to_dev(u_np)
+    g = torch.Generator(device="cpu").manual_seed(4)
+    lam = torch.randn(u0.shape, generator=g, dtype=torch.float64).to(u0.device) / _scales(u0)
+    grads = []
+    for k in (None, 3):
+        x = u0.clone().requires_grad_(True)
+        y = A.integrate(x, cfg, dt, 7, checkpoint_every=k)
+        (gr,) = torch.autograd.grad((y * lam).sum(), x)
+        grads.append(gr)
+    assert torch.equal(grads[0], grads[1])
