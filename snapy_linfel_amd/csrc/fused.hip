@@ -329,6 +329,16 @@ __device__ __forceinline__ void column_prim(const float* __restrict__ u, int nce
   cons_to_prim<double>(U, a.gammad, W);
 }
 
+// FP64 primitives of an FP32 conserved state (normal momentum flipped for a
+// reflected ghost source)
+__device__ __forceinline__ void float_prim(const float* Uf, bool flip, double gammad, double* W) {
+  double U[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) U[v] = (double)Uf[v];
+  if (flip) U[IVX] = -U[IVX];
+  cons_to_prim<double>(U, gammad, W);
+}
+
 // isentropic refill of the ghost next to boundary cell wb (dir -1: below,
 // +1: above) (src/hydro/hydro.cpp:231-269, first ghost).  FP32, the
 // arithmetic of the refill in the RHS kernel, which computes the fluxes of
@@ -574,11 +584,36 @@ __device__ __forceinline__ void column_pcr(float* __restrict__ u, float* __restr
   }
   const int q0 = q - i;
 
+  // ---- every global load of the lane first: one memory round trip ----
+  float Uc[5] = {1.f, 0.f, 0.f, 0.f, 1.f}, Ub[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+  float Um[5] = {1.f, 0.f, 0.f, 0.f, 1.f}, Up[5] = {1.f, 0.f, 0.f, 0.f, 1.f};
+  float Dv[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+  bool flm = false, flp = false;
+  if (act) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      Uc[v] = u[v * ncell + q];
+      Dv[v] = du[v * ncell + q];
+      if (a.stage != 0) Ub[v] = u0[v * ncell + q];
+    }
+    // the x1 ghost sources of the end lanes, through the boundary functions
+    if (a.scheme && i == 0) {
+      int ii = bc_map(-1, nl, a.bc_in, a.bc_out, flm);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) Um[v] = u[v * ncell + q0 + ii];
+    }
+    if (a.scheme && i == nl - 1) {
+      int ii = bc_map(nl, nl, a.bc_in, a.bc_out, flp);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) Up[v] = u[v * ncell + q0 + ii];
+    }
+  }
+
   double dd[5];
   if (a.scheme) {
     // ---- primitives of the cell and of its neighbours ----
-    double wc[5] = {1., 0., 0., 0., 1.}, wm[5], wp[5];
-    if (act) column_prim(u, ncell, q0, i, a, wc);
+    double wc[5], wm[5], wp[5];
+    float_prim(Uc, false, a.gammad, wc);
 #pragma unroll
     for (int v = 0; v < 5; ++v) sh[v * T + t] = wc[v];
     __syncthreads();
@@ -589,14 +624,14 @@ __device__ __forceinline__ void column_pcr(float* __restrict__ u, float* __restr
 #pragma unroll
         for (int v = 0; v < 5; ++v) wm[v] = sh[v * T + t - 1];
       } else {
-        column_prim(u, ncell, q0, -1, a, wm);
+        float_prim(Um, flm, a.gammad, wm);
         refill_ghost(wc, wm, a, -1);
       }
       if (i < nl - 1) {
 #pragma unroll
         for (int v = 0; v < 5; ++v) wp[v] = sh[v * T + t + 1];
       } else {
-        column_prim(u, ncell, q0, nl, a, wp);
+        float_prim(Up, flp, a.gammad, wp);
         refill_ghost(wc, wp, a, +1);
       }
     }
@@ -651,7 +686,7 @@ __device__ __forceinline__ void column_pcr(float* __restrict__ u, float* __restr
         for (int e = 0; e < NN; ++e) C[e] = 0.;
     }
 #pragma unroll
-    for (int v = 0; v < 5; ++v) dd[v] = act ? (double)du[v * ncell + q] : 0.;
+    for (int v = 0; v < 5; ++v) dd[v] = (double)Dv[v];
     r[0] = dd[IDN] / a.dt;
     r[1] = dd[IVX] / a.dt;
     if (N == 3) {
@@ -691,14 +726,14 @@ __device__ __forceinline__ void column_pcr(float* __restrict__ u, float* __restr
     }
   } else {
 #pragma unroll
-    for (int v = 0; v < 5; ++v) dd[v] = act ? (double)du[v * ncell + q] : 0.;
+    for (int v = 0; v < 5; ++v) dd[v] = (double)Dv[v];
   }
   // ---- stage combine ----
   if (act) {
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
-      float uc = u[v * ncell + q];
-      float ub = a.stage == 0 ? uc : u0[v * ncell + q];
+      float uc = Uc[v];
+      float ub = a.stage == 0 ? uc : Ub[v];
       if (a.stage == 0) u0[v * ncell + q] = uc;
       u[v * ncell + q] = (float)((a.w0 * ub + a.w1 * uc) + a.w2 * dd[v]);
     }
