@@ -184,7 +184,10 @@ __device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __r
     __syncthreads();
   }
 
-  // ---- 3. per direction: interface states + fluxes in LDS, differences ----
+  // ---- 3. interface states + Riemann fluxes of every direction at once ----
+  // The faces of all active directions form one index space (direction d
+  // occupies [fo[d], fo[d] + nf[d])), so that a single pass and a single
+  // barrier cover them; the fluxes go to fs [5][total faces].
   float acc[kMaxCellsPerThread][5];
 #pragma unroll
   for (int r = 0; r < kMaxCellsPerThread; ++r)
@@ -192,76 +195,101 @@ __device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __r
     for (int v = 0; v < 5; ++v) acc[r][v] = 0.f;
   const int ncells_t = e1 * e2 * e3;
   const DivMod de1(e1), de2(e2);
+  const bool act2 = a.n2 > 1, act3 = a.n3 > 1;
+  const int nf0 = (e1 + 1) * e2 * e3;
+  const int nf1 = act2 ? e1 * (e2 + 1) * e3 : 0;
+  const int nf2 = act3 ? e1 * e2 * (e3 + 1) : 0;
+  const int fo1 = nf0, fo2 = nf0 + nf1, ftot = fo2 + nf2;
+  const DivMod dq1(e1 + 1), dq2(e2 + 1);
 
-  for (int d = 0; d < 3; ++d) {
-    int nd = d == 0 ? a.nx1 : (d == 1 ? a.nx2 : a.nx3);
-    int ncd = d == 0 ? a.n1 : (d == 1 ? a.n2 : a.n3);
-    if (ncd == 1) continue;
-    // face space: extent along d grows by one
-    int f1 = e1 + (d == 0), f2 = e2 + (d == 1), f3 = e3 + (d == 2);
-    int nf = f1 * f2 * f3;
-    int sd = d == 0 ? 1 : (d == 1 ? H1 : H1 * H2);  // LDS stride along d
-    int r = d == 0 ? 0 : 1;
+  for (int g = threadIdx.x; g < ftot; g += blockDim.x) {
+    const int d = g < fo1 ? 0 : (g < fo2 ? 1 : 2);
+    const int f = g - (d == 0 ? 0 : (d == 1 ? fo1 : fo2));
+    // face space of d: extent along d grows by one
+    const int f1 = e1 + (d == 0), f2 = e2 + (d == 1);
+    const int sd = d == 0 ? 1 : (d == 1 ? H1 : H1 * H2);  // LDS stride along d
+    const int r = d == 0 ? 0 : 1;
     // selects, not a runtime index into the argument struct (which would
     // place it in scratch memory)
-    int type = r ? a.recon_type[1] : a.recon_type[0];
-    int scale = r ? a.recon_scale[1] : a.recon_scale[0];
-    int shock = r ? a.recon_shock[1] : a.recon_shock[0];
-    int type2 = shock ? type : interp2_type(type);
-    int solver = d == 0 ? a.riemann[0] : (d == 1 ? a.riemann[1] : a.riemann[2]);
-    const DivMod df1(f1), df2(f2);
-    for (int f = threadIdx.x; f < nf; f += blockDim.x) {
-      int fx, fy;
-      int fz = df2.div(df1.div(f, fx), fy);
-      // LDS coordinates of the cell above the face along d
-      int x = fx + 3, y = fy + a.h2, z = fz + a.h3;
-      int lin = (z * H2 + y) * H1 + x;
-      int c = d == 0 ? x : (d == 1 ? y : z);
-      int base = lin - c * sd;  // pencil start along d
-      // face frame: slot v holds variable vmap[v] (normal velocity in slot
-      // IVX); register arrays are indexed by constants only, the frame is
-      // entered and left through LDS addresses
-      const int vmap[5] = {IDN, IVX + d, IVX + (d + 1) % 3, IVX + (d + 2) % 3, IPR};
-      float wl[5], wr[5];
-#pragma unroll
-      for (int v = 0; v < 5; ++v) {
-        int tv = (v == IDN) ? type : type2;
-        const float* p = ws + vmap[v] * H + base;
-        wr[v] = interp_face<float>(tv, 0, scale, p, sd, c);
-        wl[v] = interp_face<float>(tv, 1, scale, p, sd, c - 1);
-      }
-      if (d == 0 && a.grav1 != 0.f) {
-        // _revise_x1inner_lr / _revise_x1outer_lr (hydro.cpp:217-229)
-        int gi = o1 + fx;
-        if (gi == 0) {
-          wl[IDN] = wr[IDN];
-          wl[IPR] = wr[IPR];
-        }
-        if (gi == nd) {
-          wr[IDN] = wl[IDN];
-          wr[IPR] = wl[IPR];
-        }
-      }
-      float flx[5];
-      riemann_flux<float>(solver, wl, wr, gammad, 3, flx, (float*)nullptr);
-#pragma unroll
-      for (int v = 0; v < 5; ++v) fs[vmap[v] * nf + f] = flx[v];
+    const int type = r ? a.recon_type[1] : a.recon_type[0];
+    const int scale = r ? a.recon_scale[1] : a.recon_scale[0];
+    const int shock = r ? a.recon_shock[1] : a.recon_shock[0];
+    const int type2 = shock ? type : interp2_type(type);
+    const int solver = d == 0 ? a.riemann[0] : (d == 1 ? a.riemann[1] : a.riemann[2]);
+    int fx, fy, fz;
+    if (d == 0) {
+      fz = de2.div(dq1.div(f, fx), fy);
+    } else if (d == 1) {
+      fz = dq2.div(de1.div(f, fx), fy);
+    } else {
+      fz = de2.div(de1.div(f, fx), fy);
     }
-    __syncthreads();
-    float area = d == 0 ? a.dx3 * a.dx2 : (d == 1 ? a.dx3 * a.dx1 : a.dx2 * a.dx1);
+    // LDS coordinates of the cell above the face along d
+    int x = fx + 3, y = fy + a.h2, z = fz + a.h3;
+    int lin = (z * H2 + y) * H1 + x;
+    int c = d == 0 ? x : (d == 1 ? y : z);
+    int base = lin - c * sd;  // pencil start along d
+    // face frame: slot v holds variable vmap[v] (normal velocity in slot
+    // IVX); register arrays are indexed by constants only, the frame is
+    // entered and left through LDS addresses
+    const int vmap[5] = {IDN, IVX + d, IVX + (d + 1) % 3, IVX + (d + 2) % 3, IPR};
+    float wl[5], wr[5];
 #pragma unroll
-    for (int rr = 0; rr < kMaxCellsPerThread; ++rr) {
-      int cid = threadIdx.x + rr * kRhsThreads;
-      if (cid < ncells_t) {
-        int cx, cy;
-        int cz = de2.div(de1.div(cid, cx), cy);
-        int lo = (cz * f2 + cy) * f1 + cx;
-        int hi = lo + (d == 0 ? 1 : (d == 1 ? f1 : f1 * f2));
-#pragma unroll
-        for (int v = 0; v < 5; ++v) acc[rr][v] += area * (fs[v * nf + hi] - fs[v * nf + lo]);
+    for (int v = 0; v < 5; ++v) {
+      int tv = (v == IDN) ? type : type2;
+      const float* p = ws + vmap[v] * H + base;
+      wr[v] = interp_face<float>(tv, 0, scale, p, sd, c);
+      wl[v] = interp_face<float>(tv, 1, scale, p, sd, c - 1);
+    }
+    if (d == 0 && a.grav1 != 0.f) {
+      // _revise_x1inner_lr / _revise_x1outer_lr (hydro.cpp:217-229)
+      int gi = o1 + fx;
+      if (gi == 0) {
+        wl[IDN] = wr[IDN];
+        wl[IPR] = wr[IPR];
+      }
+      if (gi == a.nx1) {
+        wr[IDN] = wl[IDN];
+        wr[IPR] = wl[IPR];
       }
     }
-    __syncthreads();
+    float flx[5];
+    riemann_flux<float>(solver, wl, wr, gammad, 3, flx, (float*)nullptr);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) fs[vmap[v] * ftot + g] = flx[v];
+    (void)f1;
+    (void)f2;
+  }
+  __syncthreads();
+  // flux differences, directions in order (x1, x2, x3)
+#pragma unroll
+  for (int rr = 0; rr < kMaxCellsPerThread; ++rr) {
+    int cid = threadIdx.x + rr * kRhsThreads;
+    if (cid < ncells_t) {
+      int cx, cy;
+      int cz = de2.div(de1.div(cid, cx), cy);
+      {
+        int lo = (cz * e2 + cy) * (e1 + 1) + cx;
+        float area = a.dx3 * a.dx2;
+#pragma unroll
+        for (int v = 0; v < 5; ++v)
+          acc[rr][v] += area * (fs[v * ftot + lo + 1] - fs[v * ftot + lo]);
+      }
+      if (act2) {
+        int lo = fo1 + (cz * (e2 + 1) + cy) * e1 + cx;
+        float area = a.dx3 * a.dx1;
+#pragma unroll
+        for (int v = 0; v < 5; ++v)
+          acc[rr][v] += area * (fs[v * ftot + lo + e1] - fs[v * ftot + lo]);
+      }
+      if (act3) {
+        int lo = fo2 + (cz * e2 + cy) * e1 + cx;
+        float area = a.dx2 * a.dx1;
+#pragma unroll
+        for (int v = 0; v < 5; ++v)
+          acc[rr][v] += area * (fs[v * ftot + lo + e1 * e2] - fs[v * ftot + lo]);
+      }
+    }
   }
 
   // ---- 4. du = -dt D + gravity ----
@@ -868,10 +896,9 @@ static Tile choose_tile(const Grid& g) {
 static size_t rhs_smem_bytes(const FusedArgs& a) {
   size_t H = (size_t)(a.tx1 + 6) * (a.tx2 + 2 * a.h2) * (a.tx3 + 2 * a.h3);
   size_t f1 = (size_t)(a.tx1 + 1) * a.tx2 * a.tx3;
-  size_t f2 = (size_t)a.tx1 * (a.tx2 + 1) * a.tx3;
-  size_t f3 = (size_t)a.tx1 * a.tx2 * (a.tx3 + 1);
-  size_t fm = std::max(f1, std::max(f2, f3));
-  return sizeof(float) * 5 * (H + fm);
+  size_t f2 = a.h2 ? (size_t)a.tx1 * (a.tx2 + 1) * a.tx3 : 0;
+  size_t f3 = a.h3 ? (size_t)a.tx1 * a.tx2 * (a.tx3 + 1) : 0;
+  return sizeof(float) * 5 * (H + f1 + f2 + f3);
 }
 
 static int check_fused(const snapx_config* c) {
