@@ -54,6 +54,7 @@ struct FusedArgs {
   int recon_type[2], recon_scale[2], recon_shock[2];
   int riemann[3];
   int bc[6];
+  int merged;  // faces of all directions in one pass (small tiles)
   float gammad, dx1, dx2, dx3, dt, grav1, grav2, grav3;
 };
 
@@ -202,93 +203,168 @@ __device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __r
   const int fo1 = nf0, fo2 = nf0 + nf1, ftot = fo2 + nf2;
   const DivMod dq1(e1 + 1), dq2(e2 + 1);
 
-  for (int g = threadIdx.x; g < ftot; g += blockDim.x) {
-    const int d = g < fo1 ? 0 : (g < fo2 ? 1 : 2);
-    const int f = g - (d == 0 ? 0 : (d == 1 ? fo1 : fo2));
-    // face space of d: extent along d grows by one
-    const int f1 = e1 + (d == 0), f2 = e2 + (d == 1);
-    const int sd = d == 0 ? 1 : (d == 1 ? H1 : H1 * H2);  // LDS stride along d
-    const int r = d == 0 ? 0 : 1;
-    // selects, not a runtime index into the argument struct (which would
-    // place it in scratch memory)
-    const int type = r ? a.recon_type[1] : a.recon_type[0];
-    const int scale = r ? a.recon_scale[1] : a.recon_scale[0];
-    const int shock = r ? a.recon_shock[1] : a.recon_shock[0];
-    const int type2 = shock ? type : interp2_type(type);
-    const int solver = d == 0 ? a.riemann[0] : (d == 1 ? a.riemann[1] : a.riemann[2]);
-    int fx, fy, fz;
-    if (d == 0) {
-      fz = de2.div(dq1.div(f, fx), fy);
-    } else if (d == 1) {
-      fz = dq2.div(de1.div(f, fx), fy);
-    } else {
-      fz = de2.div(de1.div(f, fx), fy);
+  if (a.merged) {
+    // few faces per thread: every direction in one pass, one barrier
+    for (int g = threadIdx.x; g < ftot; g += blockDim.x) {
+      const int d = g < fo1 ? 0 : (g < fo2 ? 1 : 2);
+      const int f = g - (d == 0 ? 0 : (d == 1 ? fo1 : fo2));
+      // face space of d: extent along d grows by one
+      const int f1 = e1 + (d == 0), f2 = e2 + (d == 1);
+      const int sd = d == 0 ? 1 : (d == 1 ? H1 : H1 * H2);  // LDS stride along d
+      const int r = d == 0 ? 0 : 1;
+      // selects, not a runtime index into the argument struct (which would
+      // place it in scratch memory)
+      const int type = r ? a.recon_type[1] : a.recon_type[0];
+      const int scale = r ? a.recon_scale[1] : a.recon_scale[0];
+      const int shock = r ? a.recon_shock[1] : a.recon_shock[0];
+      const int type2 = shock ? type : interp2_type(type);
+      const int solver = d == 0 ? a.riemann[0] : (d == 1 ? a.riemann[1] : a.riemann[2]);
+      int fx, fy, fz;
+      if (d == 0) {
+        fz = de2.div(dq1.div(f, fx), fy);
+      } else if (d == 1) {
+        fz = dq2.div(de1.div(f, fx), fy);
+      } else {
+        fz = de2.div(de1.div(f, fx), fy);
+      }
+      // LDS coordinates of the cell above the face along d
+      int x = fx + 3, y = fy + a.h2, z = fz + a.h3;
+      int lin = (z * H2 + y) * H1 + x;
+      int c = d == 0 ? x : (d == 1 ? y : z);
+      int base = lin - c * sd;  // pencil start along d
+      // face frame: slot v holds variable vmap[v] (normal velocity in slot
+      // IVX); register arrays are indexed by constants only, the frame is
+      // entered and left through LDS addresses
+      const int vmap[5] = {IDN, IVX + d, IVX + (d + 1) % 3, IVX + (d + 2) % 3, IPR};
+      float wl[5], wr[5];
+  #pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        int tv = (v == IDN) ? type : type2;
+        const float* p = ws + vmap[v] * H + base;
+        wr[v] = interp_face<float>(tv, 0, scale, p, sd, c);
+        wl[v] = interp_face<float>(tv, 1, scale, p, sd, c - 1);
+      }
+      if (d == 0 && a.grav1 != 0.f) {
+        // _revise_x1inner_lr / _revise_x1outer_lr (hydro.cpp:217-229)
+        int gi = o1 + fx;
+        if (gi == 0) {
+          wl[IDN] = wr[IDN];
+          wl[IPR] = wr[IPR];
+        }
+        if (gi == a.nx1) {
+          wr[IDN] = wl[IDN];
+          wr[IPR] = wl[IPR];
+        }
+      }
+      float flx[5];
+      riemann_flux<float>(solver, wl, wr, gammad, 3, flx, (float*)nullptr);
+  #pragma unroll
+      for (int v = 0; v < 5; ++v) fs[vmap[v] * ftot + g] = flx[v];
+      (void)f1;
+      (void)f2;
     }
-    // LDS coordinates of the cell above the face along d
-    int x = fx + 3, y = fy + a.h2, z = fz + a.h3;
-    int lin = (z * H2 + y) * H1 + x;
-    int c = d == 0 ? x : (d == 1 ? y : z);
-    int base = lin - c * sd;  // pencil start along d
-    // face frame: slot v holds variable vmap[v] (normal velocity in slot
-    // IVX); register arrays are indexed by constants only, the frame is
-    // entered and left through LDS addresses
-    const int vmap[5] = {IDN, IVX + d, IVX + (d + 1) % 3, IVX + (d + 2) % 3, IPR};
-    float wl[5], wr[5];
-#pragma unroll
-    for (int v = 0; v < 5; ++v) {
-      int tv = (v == IDN) ? type : type2;
-      const float* p = ws + vmap[v] * H + base;
-      wr[v] = interp_face<float>(tv, 0, scale, p, sd, c);
-      wl[v] = interp_face<float>(tv, 1, scale, p, sd, c - 1);
+    __syncthreads();
+    // flux differences, directions in order (x1, x2, x3)
+  #pragma unroll
+    for (int rr = 0; rr < kMaxCellsPerThread; ++rr) {
+      int cid = threadIdx.x + rr * kRhsThreads;
+      if (cid < ncells_t) {
+        int cx, cy;
+        int cz = de2.div(de1.div(cid, cx), cy);
+        {
+          int lo = (cz * e2 + cy) * (e1 + 1) + cx;
+          float area = a.dx3 * a.dx2;
+  #pragma unroll
+          for (int v = 0; v < 5; ++v)
+            acc[rr][v] += area * (fs[v * ftot + lo + 1] - fs[v * ftot + lo]);
+        }
+        if (act2) {
+          int lo = fo1 + (cz * (e2 + 1) + cy) * e1 + cx;
+          float area = a.dx3 * a.dx1;
+  #pragma unroll
+          for (int v = 0; v < 5; ++v)
+            acc[rr][v] += area * (fs[v * ftot + lo + e1] - fs[v * ftot + lo]);
+        }
+        if (act3) {
+          int lo = fo2 + (cz * e2 + cy) * e1 + cx;
+          float area = a.dx2 * a.dx1;
+  #pragma unroll
+          for (int v = 0; v < 5; ++v)
+            acc[rr][v] += area * (fs[v * ftot + lo + e1 * e2] - fs[v * ftot + lo]);
+        }
+      }
     }
-    if (d == 0 && a.grav1 != 0.f) {
-      // _revise_x1inner_lr / _revise_x1outer_lr (hydro.cpp:217-229)
-      int gi = o1 + fx;
-      if (gi == 0) {
-        wl[IDN] = wr[IDN];
-        wl[IPR] = wr[IPR];
+  } else {
+    // many faces per thread: one direction at a time, fs holds one direction
+    for (int d = 0; d < 3; ++d) {
+      int nd = d == 0 ? a.nx1 : (d == 1 ? a.nx2 : a.nx3);
+      int ncd = d == 0 ? a.n1 : (d == 1 ? a.n2 : a.n3);
+      if (ncd == 1) continue;
+      // face space: extent along d grows by one
+      int f1 = e1 + (d == 0), f2 = e2 + (d == 1), f3 = e3 + (d == 2);
+      int nf = f1 * f2 * f3;
+      int sd = d == 0 ? 1 : (d == 1 ? H1 : H1 * H2);  // LDS stride along d
+      int r = d == 0 ? 0 : 1;
+      // selects, not a runtime index into the argument struct (which would
+      // place it in scratch memory)
+      int type = r ? a.recon_type[1] : a.recon_type[0];
+      int scale = r ? a.recon_scale[1] : a.recon_scale[0];
+      int shock = r ? a.recon_shock[1] : a.recon_shock[0];
+      int type2 = shock ? type : interp2_type(type);
+      int solver = d == 0 ? a.riemann[0] : (d == 1 ? a.riemann[1] : a.riemann[2]);
+      const DivMod df1(f1), df2(f2);
+      for (int f = threadIdx.x; f < nf; f += blockDim.x) {
+        int fx, fy;
+        int fz = df2.div(df1.div(f, fx), fy);
+        // LDS coordinates of the cell above the face along d
+        int x = fx + 3, y = fy + a.h2, z = fz + a.h3;
+        int lin = (z * H2 + y) * H1 + x;
+        int c = d == 0 ? x : (d == 1 ? y : z);
+        int base = lin - c * sd;  // pencil start along d
+        // face frame: slot v holds variable vmap[v] (normal velocity in slot
+        // IVX); register arrays are indexed by constants only, the frame is
+        // entered and left through LDS addresses
+        const int vmap[5] = {IDN, IVX + d, IVX + (d + 1) % 3, IVX + (d + 2) % 3, IPR};
+        float wl[5], wr[5];
+  #pragma unroll
+        for (int v = 0; v < 5; ++v) {
+          int tv = (v == IDN) ? type : type2;
+          const float* p = ws + vmap[v] * H + base;
+          wr[v] = interp_face<float>(tv, 0, scale, p, sd, c);
+          wl[v] = interp_face<float>(tv, 1, scale, p, sd, c - 1);
+        }
+        if (d == 0 && a.grav1 != 0.f) {
+          // _revise_x1inner_lr / _revise_x1outer_lr (hydro.cpp:217-229)
+          int gi = o1 + fx;
+          if (gi == 0) {
+            wl[IDN] = wr[IDN];
+            wl[IPR] = wr[IPR];
+          }
+          if (gi == nd) {
+            wr[IDN] = wl[IDN];
+            wr[IPR] = wl[IPR];
+          }
+        }
+        float flx[5];
+        riemann_flux<float>(solver, wl, wr, gammad, 3, flx, (float*)nullptr);
+  #pragma unroll
+        for (int v = 0; v < 5; ++v) fs[vmap[v] * nf + f] = flx[v];
       }
-      if (gi == a.nx1) {
-        wr[IDN] = wl[IDN];
-        wr[IPR] = wl[IPR];
+      __syncthreads();
+      float area = d == 0 ? a.dx3 * a.dx2 : (d == 1 ? a.dx3 * a.dx1 : a.dx2 * a.dx1);
+  #pragma unroll
+      for (int rr = 0; rr < kMaxCellsPerThread; ++rr) {
+        int cid = threadIdx.x + rr * kRhsThreads;
+        if (cid < ncells_t) {
+          int cx, cy;
+          int cz = de2.div(de1.div(cid, cx), cy);
+          int lo = (cz * f2 + cy) * f1 + cx;
+          int hi = lo + (d == 0 ? 1 : (d == 1 ? f1 : f1 * f2));
+  #pragma unroll
+          for (int v = 0; v < 5; ++v) acc[rr][v] += area * (fs[v * nf + hi] - fs[v * nf + lo]);
+        }
       }
-    }
-    float flx[5];
-    riemann_flux<float>(solver, wl, wr, gammad, 3, flx, (float*)nullptr);
-#pragma unroll
-    for (int v = 0; v < 5; ++v) fs[vmap[v] * ftot + g] = flx[v];
-    (void)f1;
-    (void)f2;
-  }
-  __syncthreads();
-  // flux differences, directions in order (x1, x2, x3)
-#pragma unroll
-  for (int rr = 0; rr < kMaxCellsPerThread; ++rr) {
-    int cid = threadIdx.x + rr * kRhsThreads;
-    if (cid < ncells_t) {
-      int cx, cy;
-      int cz = de2.div(de1.div(cid, cx), cy);
-      {
-        int lo = (cz * e2 + cy) * (e1 + 1) + cx;
-        float area = a.dx3 * a.dx2;
-#pragma unroll
-        for (int v = 0; v < 5; ++v)
-          acc[rr][v] += area * (fs[v * ftot + lo + 1] - fs[v * ftot + lo]);
-      }
-      if (act2) {
-        int lo = fo1 + (cz * (e2 + 1) + cy) * e1 + cx;
-        float area = a.dx3 * a.dx1;
-#pragma unroll
-        for (int v = 0; v < 5; ++v)
-          acc[rr][v] += area * (fs[v * ftot + lo + e1] - fs[v * ftot + lo]);
-      }
-      if (act3) {
-        int lo = fo2 + (cz * e2 + cy) * e1 + cx;
-        float area = a.dx2 * a.dx1;
-#pragma unroll
-        for (int v = 0; v < 5; ++v)
-          acc[rr][v] += area * (fs[v * ftot + lo + e1 * e2] - fs[v * ftot + lo]);
-      }
+      __syncthreads();
     }
   }
 
@@ -893,12 +969,16 @@ static Tile choose_tile(const Grid& g) {
   return t;
 }
 
-static size_t rhs_smem_bytes(const FusedArgs& a) {
-  size_t H = (size_t)(a.tx1 + 6) * (a.tx2 + 2 * a.h2) * (a.tx3 + 2 * a.h3);
+static size_t rhs_face_count(const FusedArgs& a, bool all) {
   size_t f1 = (size_t)(a.tx1 + 1) * a.tx2 * a.tx3;
   size_t f2 = a.h2 ? (size_t)a.tx1 * (a.tx2 + 1) * a.tx3 : 0;
   size_t f3 = a.h3 ? (size_t)a.tx1 * a.tx2 * (a.tx3 + 1) : 0;
-  return sizeof(float) * 5 * (H + f1 + f2 + f3);
+  return all ? f1 + f2 + f3 : std::max(f1, std::max(f2, f3));
+}
+
+static size_t rhs_smem_bytes(const FusedArgs& a) {
+  size_t H = (size_t)(a.tx1 + 6) * (a.tx2 + 2 * a.h2) * (a.tx3 + 2 * a.h3);
+  return sizeof(float) * 5 * (H + rhs_face_count(a, a.merged));
 }
 
 static int check_fused(const snapx_config* c) {
@@ -950,6 +1030,8 @@ static int make_fused_args(const snapx_config* cfg, Tile t, double dt, FusedArgs
   a.dx1 = (float)g.dx1, a.dx2 = (float)g.dx2, a.dx3 = (float)g.dx3;
   a.dt = (float)dt;
   a.grav1 = (float)cfg->grav1, a.grav2 = (float)cfg->grav2, a.grav3 = (float)cfg->grav3;
+  // one pass over all faces when they number at most two per thread
+  a.merged = rhs_face_count(a, true) <= 2 * (size_t)kRhsThreads;
   SNAPX_CHECK_ARG(a.tx1 >= 1 && a.tx2 >= 1 && a.tx3 >= 1, "fused path: empty tile");
   SNAPX_CHECK_ARG((int64_t)a.tx1 * a.tx2 * a.tx3 <= kRhsThreads * kMaxCellsPerThread,
                   "fused path: tile too large");
