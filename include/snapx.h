@@ -198,6 +198,21 @@ int snapx_rk3_step(const snapx_config* cfg, double dt, void* u, void* w,
 int snapx_max_time_step(const snapx_config* cfg, const void* w, double cfl,
                         double* dt_out, int dtype, void* stream);
 
+/* Interior positivity check of MeshBlockImpl::check_redo
+   (src/mesh/meshblock.cpp:974-1028) on the conserved state u: writes to the
+   device int *flag SNAPX_REDO_DENSITY when min(rho) <= 0 and
+   SNAPX_REDO_PRESSURE when min(E) <= 0 and the cons2prim pressure (with the
+   configured limiter) has min <= 0, both as the reference's NaN-propagating
+   minima decide them, plus SNAPX_STATE_NONFINITE when any interior conserved
+   value is NaN or infinite (information only; the reference does not redo on
+   it).  No host synchronisation and no memset: the flag is cleared by a
+   kernel, so the call can be captured in a HIP graph.  u is not modified. */
+#define SNAPX_REDO_DENSITY 1
+#define SNAPX_REDO_PRESSURE 2
+#define SNAPX_STATE_NONFINITE 4
+int snapx_check_state(const snapx_config* cfg, const void* u, int* flag, int dtype,
+                      void* stream);
+
 /* ---------------------------------------------------------------------- */
 /* Fused FP32 path (no reference counterpart; same results as the plain    */
 /* path to FP32 rounding)                                                  */

@@ -69,6 +69,28 @@ def prim2cons(c, w):
     return torch.cat([rho[None], m, e[None]], 0)
 
 
+def interior(c):
+    """Index of the interior cells of [5, n3, n2, n1] (ghosts along axes > 1)."""
+    ng = c.nghost
+    return (slice(None),) + tuple(slice(ng, -ng) if n > 1 else slice(None)
+                                  for n in _shape(c))
+
+
+def check_redo(c, u):
+    """Verdict bits (1 density, 2 pressure, 4 non-finite) of
+    MeshBlockImpl::check_redo (src/mesh/meshblock.cpp:974-1028), limiter off:
+    NaN-propagating tensor minima over the interior compared with zero; the
+    pressure test runs only when the conserved-energy screen trips."""
+    I = interior(c)
+    ui = u[I]
+    rho = bool(ui[IDN].min() <= 0.0)
+    pres = False
+    if bool(ui[IPR].min() <= 0.0):
+        pres = bool(cons2prim(c, u)[I][IPR].min() <= 0.0)
+    nonfinite = not bool(torch.isfinite(ui).all())
+    return int(rho) | (int(pres) << 1) | (int(nonfinite) << 2)
+
+
 # ------------------------------------------------------------- boundaries
 
 def _axis_bc(x, ax, ng, bin_, bout, normal_var, flip):

@@ -120,6 +120,7 @@ SIGNATURES = {
     "snapx_rk3_workspace_bytes": (_Z, [_CFG, _I]),
     "snapx_rk3_step": (_I, [_CFG, _D, _P, _P, _P, _Z, _I, _P]),
     "snapx_max_time_step": (_I, [_CFG, _P, _D, _P, _I, _P]),
+    "snapx_check_state": (_I, [_CFG, _P, _P, _I, _P]),
     "snapx_fused_workspace_bytes": (_Z, [_CFG]),
     "snapx_fused_rk3_step": (_I, [_CFG, _D, _P, _P, _Z, _P]),
     "snapx_fused_rk3_steps": (_I, [_CFG, _D, _I, _P, _P, _Z, _P, _P]),
@@ -281,6 +282,19 @@ def rk3_step(cfg, dt, u, w, ws=None):
 def max_time_step(cfg, w, cfl, out):
     check(lib().snapx_max_time_step(C.byref(cfg), ptr(w), cfl, ptr(out), dtype_code(w),
                                     stream_ptr()), "max_time_step")
+
+
+REDO_DENSITY, REDO_PRESSURE, STATE_NONFINITE = 1, 2, 4
+
+
+def check_state(cfg, u, flag):
+    """Writes the positivity verdict of ``u`` (REDO_* / STATE_NONFINITE bits)
+    into the one-element int32 device tensor ``flag``; no synchronisation."""
+    torch = _torch()
+    if flag.dtype != torch.int32 or flag.numel() < 1 or flag.device != u.device:
+        raise ValueError("check_state: flag must be an int32 tensor on u's device")
+    check(lib().snapx_check_state(C.byref(cfg), ptr(u), ptr(flag), dtype_code(u), stream_ptr()),
+          "check_state")
 
 
 def adjoint_workspace(cfg, device="cuda"):

@@ -28,35 +28,42 @@ constexpr double kRgas = 8.31446261815324;  // CODATA; kintera::constants::Rgas
 // src/eos/ideal_gas_impl.h:15-35); cos(theta) = 0 makes the contravariant
 // raise an identity.
 template <typename T>
+__device__ __forceinline__ void cons2prim_cell(T (&U)[5], T (&W)[5], T gammad, int limiter,
+                                               T dfloor, T pfloor, T cv_tfloor) {
+  if (limiter) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) U[v] = isnan(U[v]) ? T(0.) : U[v];
+    U[IDN] = U[IDN] < dfloor ? dfloor : U[IDN];
+    T ke = T(0.5) * ((U[IVX] * U[IVX] + U[IVY] * U[IVY]) + U[IVZ] * U[IVZ]) / U[IDN];
+    T lo = ke + U[IDN] * cv_tfloor;
+    U[IPR] = U[IPR] < lo ? lo : U[IPR];
+  }
+  W[IDN] = U[IDN];
+  W[IVX] = U[IVX] / W[IDN];
+  W[IVY] = U[IVY] / W[IDN];
+  W[IVZ] = U[IVZ] / W[IDN];
+  T ke = T(0.5) * ((W[IVX] * U[IVX] + W[IVY] * U[IVY]) + W[IVZ] * U[IVZ]);
+  T ie = U[IPR] - ke;
+  W[IPR] = (gammad - T(1.)) * ie;
+  if (limiter) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) W[v] = isnan(W[v]) ? T(0.) : W[v];
+    W[IDN] = W[IDN] < dfloor ? dfloor : W[IDN];
+    W[IPR] = W[IPR] < pfloor ? pfloor : W[IPR];
+  }
+}
+
+template <typename T>
 __global__ void cons2prim_kernel(T* __restrict__ u, T* __restrict__ w, int ncell, T gammad,
                                  int limiter, T dfloor, T pfloor, T cv_tfloor) {
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < ncell; q += gridDim.x * blockDim.x) {
-    T U[5];
+    T U[5], W[5];
 #pragma unroll
     for (int v = 0; v < 5; ++v) U[v] = u[v * ncell + q];
+    cons2prim_cell(U, W, gammad, limiter, dfloor, pfloor, cv_tfloor);
     if (limiter) {
-#pragma unroll
-      for (int v = 0; v < 5; ++v) U[v] = isnan(U[v]) ? T(0.) : U[v];
-      U[IDN] = U[IDN] < dfloor ? dfloor : U[IDN];
-      T ke = T(0.5) * ((U[IVX] * U[IVX] + U[IVY] * U[IVY]) + U[IVZ] * U[IVZ]) / U[IDN];
-      T lo = ke + U[IDN] * cv_tfloor;
-      U[IPR] = U[IPR] < lo ? lo : U[IPR];
 #pragma unroll
       for (int v = 0; v < 5; ++v) u[v * ncell + q] = U[v];
-    }
-    T W[5];
-    W[IDN] = U[IDN];
-    W[IVX] = U[IVX] / W[IDN];
-    W[IVY] = U[IVY] / W[IDN];
-    W[IVZ] = U[IVZ] / W[IDN];
-    T ke = T(0.5) * ((W[IVX] * U[IVX] + W[IVY] * U[IVY]) + W[IVZ] * U[IVZ]);
-    T ie = U[IPR] - ke;
-    W[IPR] = (gammad - T(1.)) * ie;
-    if (limiter) {
-#pragma unroll
-      for (int v = 0; v < 5; ++v) W[v] = isnan(W[v]) ? T(0.) : W[v];
-      W[IDN] = W[IDN] < dfloor ? dfloor : W[IDN];
-      W[IPR] = W[IPR] < pfloor ? pfloor : W[IPR];
     }
 #pragma unroll
     for (int v = 0; v < 5; ++v) w[v * ncell + q] = W[v];
@@ -391,6 +398,54 @@ __global__ void cfl_finalize_kernel(double* dt, double cfl) {
   *dt = 1.0 * cfl * v;
 }
 
+// Positivity screen of MeshBlockImpl::check_redo (src/mesh/meshblock.cpp:974-1028)
+// over the interior cells.  The reference takes tensor minima, which propagate
+// NaN, and compares them with zero, so the accumulator records "some value <= 0"
+// and "some value is NaN" separately for density, total energy and the
+// cons2prim pressure; state_finalize_kernel turns them into the verdict.
+enum : int { kRhoLe0 = 1, kRhoNan = 2, kELe0 = 4, kENan = 8, kPLe0 = 16, kPNan = 32,
+             kNonFinite = 64 };
+
+__global__ void fill_i32_kernel(int* p, int v) { *p = v; }
+
+template <typename T>
+__global__ void state_screen_kernel(const T* __restrict__ u, int* __restrict__ acc, int n3,
+                                    int n2, int n1, int ng, T gammad, int limiter, T dfloor,
+                                    T pfloor, T cv_tfloor) {
+  int ncell = n1 * n2 * n3;
+  int bits = 0;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < ncell; q += gridDim.x * blockDim.x) {
+    int i = q % n1;
+    int j = (q / n1) % n2;
+    int k = q / (n1 * n2);
+    bool in1 = n1 == 1 || (i >= ng && i < n1 - ng);
+    bool in2 = n2 == 1 || (j >= ng && j < n2 - ng);
+    bool in3 = n3 == 1 || (k >= ng && k < n3 - ng);
+    if (!(in1 && in2 && in3)) continue;
+    T U[5], W[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) U[v] = u[v * ncell + q];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) bits |= isfinite(U[v]) ? 0 : kNonFinite;
+    bits |= isnan(U[IDN]) ? kRhoNan : (U[IDN] <= T(0.) ? kRhoLe0 : 0);
+    bits |= isnan(U[IPR]) ? kENan : (U[IPR] <= T(0.) ? kELe0 : 0);
+    cons2prim_cell(U, W, gammad, limiter, dfloor, pfloor, cv_tfloor);
+    bits |= isnan(W[IPR]) ? kPNan : (W[IPR] <= T(0.) ? kPLe0 : 0);
+  }
+  for (int off = warpSize / 2; off > 0; off >>= 1) bits |= __shfl_down(bits, off);
+  if ((threadIdx.x & (warpSize - 1)) == 0 && bits) atomicOr(acc, bits);
+}
+
+// min(rho) <= 0 is false when some rho is NaN; the pressure redo needs both the
+// energy screen and the pressure test to trip (meshblock.cpp:994-998).
+__global__ void state_finalize_kernel(int* flag) {
+  int b = *flag;
+  int rho = (b & kRhoLe0) && !(b & kRhoNan);
+  int pres = (b & kELe0) && !(b & kENan) && (b & kPLe0) && !(b & kPNan);
+  *flag = (rho ? SNAPX_REDO_DENSITY : 0) | (pres ? SNAPX_REDO_PRESSURE : 0) |
+          ((b & kNonFinite) ? SNAPX_STATE_NONFINITE : 0);
+}
+
 // ------------------------------------------------------------ launches ----
 
 template <typename T>
@@ -642,6 +697,29 @@ extern "C" int snapx_max_time_step(const snapx_config* cfg, const void* w, doubl
                        (float)cfg->gammad, (float)g.dx1, (float)g.dx2, (float)g.dx3, use1, use2,
                        use3);
   hipLaunchKernelGGL(cfl_finalize_kernel, dim3(1), dim3(1), 0, st, dt_out, cfl);
+  SNAPX_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int snapx_check_state(const snapx_config* cfg, const void* u, int* flag, int dtype,
+                                 void* stream) {
+  if (int e = validate_config(cfg)) return e;
+  SNAPX_CHECK_ARG(u && flag, "snapx_check_state: null pointer");
+  Grid g = make_grid(cfg);
+  hipStream_t st = as_stream(stream);
+  double cvt = kRgas / cfg->weight / (cfg->gammad - 1.) * cfg->temperature_floor;
+  hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(1), 0, st, flag, 0);
+  int nb = std::min<int64_t>((g.ncell + kThreads - 1) / kThreads, 4096);
+  if (dtype == SNAPX_F64)
+    hipLaunchKernelGGL(state_screen_kernel<double>, dim3(nb), dim3(kThreads), 0, st,
+                       reinterpret_cast<const double*>(u), flag, g.n3, g.n2, g.n1, g.ng,
+                       cfg->gammad, cfg->limiter, cfg->density_floor, cfg->pressure_floor, cvt);
+  else
+    hipLaunchKernelGGL(state_screen_kernel<float>, dim3(nb), dim3(kThreads), 0, st,
+                       reinterpret_cast<const float*>(u), flag, g.n3, g.n2, g.n1, g.ng,
+                       (float)cfg->gammad, cfg->limiter, (float)cfg->density_floor,
+                       (float)cfg->pressure_floor, (float)cvt);
+  hipLaunchKernelGGL(state_finalize_kernel, dim3(1), dim3(1), 0, st, flag);
   SNAPX_LAUNCH_CHECK();
   return 0;
 }

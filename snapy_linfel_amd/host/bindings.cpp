@@ -96,6 +96,7 @@ PYBIND11_MODULE(_snapy, m) {
       OPT(MeshBlockOptionsImpl, coord) OPT(MeshBlockOptionsImpl, hydro)
       OPT(MeshBlockOptionsImpl, integrator) OPT(MeshBlockOptionsImpl, cfl)
       OPT(MeshBlockOptionsImpl, nlim) OPT(MeshBlockOptionsImpl, tlim)
+      OPT(MeshBlockOptionsImpl, max_redo)
       OPT(MeshBlockOptionsImpl, bfuncs);
 
   py::class_<IdealGasImpl, IdealGas>(m, "IdealGas")
@@ -179,6 +180,13 @@ PYBIND11_MODULE(_snapy, m) {
              b.forward(v, dt, stage);
              for (auto& kv : v) vars[py::str(kv.first)] = kv.second;
            })
+      .def("check_redo", [](MeshBlockImpl& b, py::dict vars) {
+             Variables v;
+             for (auto item : vars) v[item.first.cast<std::string>()] = item.second.cast<torch::Tensor>();
+             return b.check_redo(v);
+           })
+      .def_readwrite("current_redo", &MeshBlockImpl::current_redo)
+      .def_readwrite("cycle", &MeshBlockImpl::cycle)
       .def("step", [](MeshBlockImpl& b, py::dict vars, double dt, bool fused) {
              Variables v;
              for (auto item : vars) v[item.first.cast<std::string>()] = item.second.cast<torch::Tensor>();

@@ -240,6 +240,7 @@ MeshBlockOptions MeshBlockOptionsImpl::from_yaml(std::string const& filename, bo
     op->cfl() = in["cfl"].as<double>(0.9);
     op->nlim() = in["nlim"].as<int>(-1);
     op->tlim() = in["tlim"].as<double>(1.e9);
+    op->max_redo() = in["max-redo"].as<int>(5);
   }
   TORCH_CHECK(op->integrator() == "rk3",
               "integration.type '", op->integrator(), "' is outside this build (rk3 only)");
@@ -553,7 +554,29 @@ void MeshBlockImpl::initialize(Variables& vars) {
 
 double MeshBlockImpl::max_time_step(Variables const& vars) {
   auto w = vars.at("hydro_w");
-  return options->cfl() * phydro->max_time_step(w);
+  return std::pow(2., -current_redo) * options->cfl() * phydro->max_time_step(w);
+}
+
+int MeshBlockImpl::check_redo(Variables& vars) {
+  auto u = vars.at("hydro_u");
+  require_device(u, "hydro_u");
+  if (!flag_.defined() || flag_.device() != u.device())
+    flag_ = torch::empty({1}, u.options().dtype(torch::kInt32));
+  check(snapx_check_state(&cfg_, u.data_ptr(), flag_.data_ptr<int>(), dtype_code(u),
+                          stream_of(u)),
+        "check_state");
+  int verdict = flag_.item<int>();
+  if (verdict & (SNAPX_REDO_DENSITY | SNAPX_REDO_PRESSURE)) {
+    current_redo += 1;
+    if (current_redo > options->max_redo()) return -1;
+    TORCH_CHECK(u0_.defined() && u0_.sizes() == u.sizes() && u0_.dtype() == u.dtype(),
+                "check_redo: no step input to restore (run a step first)");
+    u.copy_(u0_);
+    cycle -= 1;
+    return 1;
+  }
+  current_redo = 0;
+  return 0;
 }
 
 void MeshBlockImpl::forward(Variables& vars, double dt, int stage) {
@@ -591,6 +614,11 @@ void MeshBlockImpl::step(Variables& vars, double dt, bool fused) {
   TORCH_CHECK(u.scalar_type() == torch::kFloat32, "the fused path needs float32 hydro_u");
   size_t n = snapx_fused_workspace_bytes(&cfg_);
   TORCH_CHECK(n > 0, "fused path unavailable for this configuration: ", snapx_last_error());
+  // step input kept for check_redo, as stage 0 of forward() keeps it
+  if (u0_.defined() && u0_.sizes() == u.sizes() && u0_.dtype() == u.dtype())
+    u0_.copy_(u);
+  else
+    u0_ = u.clone();
   if (!fused_ws_.defined() || (size_t)fused_ws_.numel() < n)
     fused_ws_ = torch::empty({(int64_t)n}, u.options().dtype(torch::kUInt8));
   check(snapx_fused_rk3_step(&cfg_, dt, u.data_ptr<float>(), fused_ws_.data_ptr(),

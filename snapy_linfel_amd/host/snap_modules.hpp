@@ -197,6 +197,9 @@ struct MeshBlockOptionsImpl {
   SNAP_ARG(double, cfl) = 0.9;
   SNAP_ARG(int, nlim) = -1;
   SNAP_ARG(double, tlim) = 1.e9;
+  // integration.max-redo: redo attempts with dt halved before check_redo
+  // gives up (the count of harp::IntegratorOptions::max_redo)
+  SNAP_ARG(int, max_redo) = 5;
   // x1-inner, x1-outer, x2-inner, x2-outer, x3-inner, x3-outer ("" = none)
   SNAP_ARG(std::vector<std::string>, bfuncs);
 };
@@ -334,10 +337,16 @@ class MeshBlockImpl {
   // full SSP-RK3 step, plain path (dtype of hydro_u) or the FP32 fused path
   void step(Variables& vars, double dt, bool fused = false);
   void apply_bc(torch::Tensor var, int var_type);
+  // MeshBlockImpl::check_redo (meshblock.cpp:974-1028): 0 = good (redo count
+  // reset), 1 = negative density or pressure, hydro_u restored to the step's
+  // input and the next max_time_step halved once more, -1 = max_redo exceeded
+  int check_redo(Variables& vars);
+  int current_redo = 0;
+  int cycle = 0;
 
  private:
   snapx_config cfg_;
-  torch::Tensor u0_, u1_, fused_ws_;
+  torch::Tensor u0_, u1_, fused_ws_, flag_;
 };
 using MeshBlock = std::shared_ptr<MeshBlockImpl>;
 

@@ -85,3 +85,42 @@ def test_reconstruct_and_riemann_modules():
     fr, fpr = O.riemann("hllc", ref[0], ref[1], 3)
     assert np.allclose(to_host(flx), fr, rtol=1e-12, atol=1e-13)
     assert np.allclose(to_host(fp), fpr, rtol=1e-12, atol=1e-13)
+
+
+def test_meshblock_check_redo_restores_and_halves_dt():
+    """MeshBlockImpl::check_redo (src/mesh/meshblock.cpp:974-1028): a bad
+    state after a step is replaced by the step's input, each redo halves the
+    next CFL step, a good step resets the count, and more than max_redo
+    redos in a row give -1."""
+    opts = host.MeshBlockOptions.from_yaml(os.path.join(G, "straka.yaml"))
+    assert opts.max_redo == 5
+    opts.max_redo = 2
+    mb = host.MeshBlock(opts)
+    oc = _straka_oracle()
+    vars_ = {"hydro_w": torch.from_numpy(O.straka_init(oc)).to(DEV)}
+    mb.initialize(vars_)
+
+    def dt_ratio():
+        r, mb.current_redo = mb.current_redo, 0
+        base = mb.max_time_step(vars_)
+        mb.current_redo = r
+        return mb.max_time_step(vars_) / base
+
+    for fused in (False, True):
+        if fused:
+            vars_ = {k: v.float() for k, v in vars_.items()}
+        mb.step(vars_, 0.1, fused=fused)
+        assert mb.check_redo(vars_) == 0 and mb.current_redo == 0
+        u_in = vars_["hydro_u"].clone()
+        for r in (1, 2):
+            mb.step(vars_, 0.1, fused=fused)
+            vars_["hydro_u"][0, 0, 100, 20] = -1.0
+            assert mb.check_redo(vars_) == 1 and mb.current_redo == r
+            assert torch.equal(vars_["hydro_u"], u_in)
+            assert dt_ratio() == pytest.approx(0.5 ** r, rel=1e-14)
+        mb.step(vars_, 0.1, fused=fused)
+        vars_["hydro_u"][0, 0, 100, 20] = -1.0
+        assert mb.check_redo(vars_) == -1
+        vars_["hydro_u"].copy_(u_in)
+        mb.step(vars_, 0.1, fused=fused)
+        assert mb.check_redo(vars_) == 0 and mb.current_redo == 0

@@ -287,3 +287,56 @@ def test_invalid_arguments_raise():
     u = torch.zeros(cfg.shape, dtype=torch.float64, device=DEV)
     with pytest.raises(capi.SnapxError, match="implicit scheme"):
         capi.cons2prim(cfg, u, u.clone())
+
+
+def _redo_cases(u):
+    """(name, state) pairs covering every branch of the positivity check."""
+    nan, inf = float("nan"), float("inf")
+    edits = {
+        "clean": [],
+        "ghost_negative_rho": [((0, 0, 5, 1), -1.0)],
+        "zero_rho": [((0, 0, 5, 6), 0.0)],
+        "nan_hides_negative_rho": [((0, 0, 5, 6), -2.0), ((0, 0, 7, 7), nan)],
+        "negative_energy": [((4, 0, 4, 4), -1.0)],
+        "ke_above_energy_screen_passes": [((1, 0, 4, 4), 1e6)],
+        "screen_and_pressure": [((1, 0, 4, 4), 1e6), ((4, 0, 6, 6), 0.0)],
+        "infinite_momentum": [((2, 0, 3, 3), inf)],
+        "last_interior_cell": [((0, 0, -4, -4), -1e-30)],
+    }
+    for name, ed in edits.items():
+        x = u.clone()
+        for idx, v in ed:
+            x[idx] = v
+        yield name, x
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_check_state_matches_reference_rules(dtype):
+    import torch_oracle as T
+
+    oc, _, u = _straka_small(0, nx2=16, nx1=12)
+    cfg = cfg_of(oc)
+    flag = torch.full((1,), -7, dtype=torch.int32, device=DEV)
+    for name, x in _redo_cases(torch.from_numpy(u)):
+        expect = T.check_redo(oc, x)
+        capi.check_state(cfg, x.to(DEV, dtype).contiguous(), flag)
+        assert int(flag.item()) == expect, (name, int(flag.item()), expect)
+
+
+def test_check_state_with_limiter():
+    """With the limiter on, cons2prim floors the pressure (> 0), so only the
+    density test can ask for a redo; the input is left untouched."""
+    oc, _, u = _straka_small(0, nx2=16, nx1=12)
+    cfg = cfg_of(oc)
+    cfg.limiter = 1
+    cfg.pressure_floor = 1e-3
+    flag = torch.zeros(1, dtype=torch.int32, device=DEV)
+    x = torch.from_numpy(u).to(DEV)
+    x[4, 0, 6, 6] = -5.0
+    before = x.clone()
+    capi.check_state(cfg, x, flag)
+    assert int(flag.item()) == 0
+    assert torch.equal(x, before)
+    x[0, 0, 6, 7] = -1.0
+    capi.check_state(cfg, x, flag)
+    assert int(flag.item()) == capi.REDO_DENSITY

@@ -99,3 +99,26 @@ def test_torch_autograd_matches_central_differences():
 
     fd = (4 * central(1e-6) - central(2e-6)) / 3
     assert abs(ad - fd) / abs(fd) < 1e-6, (ad, fd)
+
+
+def test_check_redo_known_answers():
+    """Hand-set states with verdicts fixed by the reference's rules: interior
+    only, NaN hides a negative density (min is NaN), the pressure test needs
+    the energy screen, and E <= 0 with rho > 0 always means p <= 0."""
+    oc, u, _ = _straka_oracle(0, nx2=8, nx1=8)
+    u = torch.from_numpy(u)
+    assert T.check_redo(oc, u) == 0
+    x = u.clone(); x[0, 0, 5, 1] = -1.0          # ghost cell along x1
+    assert T.check_redo(oc, x) == 0
+    x = u.clone(); x[0, 0, 5, 6] = 0.0
+    assert T.check_redo(oc, x) == 1
+    x[0, 0, 7, 7] = float("nan")
+    assert T.check_redo(oc, x) == 4
+    x = u.clone(); x[4, 0, 4, 4] = -1.0
+    assert T.check_redo(oc, x) == 2
+    x = u.clone(); x[1, 0, 4, 4] = 1e6             # ke > E, E > 0: screen passes
+    assert T.check_redo(oc, x) == 0
+    x[4, 0, 6, 6] = 0.0                             # screen trips; p < 0 at (4, 4)
+    assert T.check_redo(oc, x) == 2
+    x = u.clone(); x[2, 0, 3, 3] = float("inf")
+    assert T.check_redo(oc, x) == 4
