@@ -206,21 +206,35 @@ def errors(got, ref, interior):
     return out
 
 
-def cpu_baseline(steps_cap=40, seconds=12.0):
-    import numpy as np
+def cpu_baseline(steps_cap=400, seconds=10.0):
+    """The FP64 C oracle (OpenMP over cells and columns) on the host cores this
+    process may use (at most 16), and on one thread for scale."""
     import oracle as O
-    O.set_threads(1)
     oc = O.straka_config(nx2=512, nx1=64, implicit_scheme=1)
-    w, u = O.initialize(oc, O.straka_init(oc))
-    t0 = time.perf_counter()
-    n = 0
-    while n < steps_cap and time.perf_counter() - t0 < seconds:
-        u, w = O.rk3_step(oc, 0.1, u)
-        n += 1
-    t = time.perf_counter() - t0
-    return {"value": 512 * 64 * n / t, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-            "sample": f"FP64 C oracle, Straka 512x64, {n} RK3 steps from the initial state, "
-                      f"1 thread ({t:.1f} s)"}
+    w, u0 = O.initialize(oc, O.straka_init(oc))
+
+    def run(threads, secs, cap):
+        O.set_threads(threads)
+        u = u0
+        t0 = time.perf_counter()
+        n = 0
+        while n < cap and time.perf_counter() - t0 < secs:
+            u, _ = O.rk3_step(oc, 0.1, u)
+            n += 1
+        return n, time.perf_counter() - t0
+
+    try:
+        cores = min(len(os.sched_getaffinity(0)), 16)
+    except AttributeError:
+        cores = min(os.cpu_count() or 1, 16)
+    n1, t1 = run(1, seconds * 0.5, steps_cap)
+    nn, tn = run(cores, seconds, steps_cap)
+    O.set_threads(1)
+    return {"value": round(512 * 64 * nn / tn, 1), "unit": "cell-updates/s", "cores": cores,
+            "kind": "port", "value_1thread": round(512 * 64 * n1 / t1, 1),
+            "sample": f"FP64 C oracle, Straka 512x64, RK3 steps from the initial state: "
+                      f"{nn} steps on {cores} threads ({tn:.1f} s), "
+                      f"{n1} steps on 1 thread ({t1:.1f} s)"}
 
 
 def main():
@@ -231,6 +245,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-3d", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-c5", action="store_true", help="skip the 200-step sensitivity run")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -324,6 +339,21 @@ def main():
             "traffic": measured_traffic(name, "straka512x64"),
             "kernel": name, "avg_launch_ms": round(avg_ms, 5),
             "bytes_per_launch": bytes_}
+
+    # attainable HBM rate: a 2 GiB device-to-device copy (read + write bytes)
+    src = torch.empty(1 << 29, dtype=torch.float32, device=dev).fill_(1.0)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(10):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    copy_gbs = 2 * src.numel() * 4 * 10 / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del src, dst
+    roof["copy_measured_GBs"] = round(copy_gbs, 1)
+    roof["frac_of_copy"] = round(achieved / copy_gbs, 5)
 
     extra = {"fused_rhs_avg_ms": round(rhs_avg, 5), "fused_column_avg_ms": round(col_avg, 5),
              "timed_launches": int(tim.rhs_launches + tim.column_launches + tim.bc_launches)}
@@ -443,7 +473,8 @@ def main():
             capi.adjoint_rk3_step(ca, dta, ua, lam_a, wsa)
         torch.cuda.synchronize()
         tadj = (time.perf_counter() - t0) / 3
-        extra["c5_sensitivity_box3d_128x128x96_fp64"] = c5_sensitivity(capi, dev)
+        if not args.no_c5:
+            extra["c5_sensitivity_box3d_128x128x96_fp64"] = c5_sensitivity(capi, dev)
         extra["adjoint_box3d_128x128x96_fp64"] = {
             "forward_ms_per_step": round(tf * 1e3, 3), "adjoint_ms_per_step": round(tadj * 1e3, 3),
             "adjoint_over_forward": round(tadj / tf, 3),

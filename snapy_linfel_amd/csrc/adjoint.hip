@@ -29,6 +29,7 @@
 // dual.hpp last: its overloads must not hide the double ones at the template
 // definitions above
 #include "dual.hpp"
+#include "riemann_adj.hpp"
 
 namespace snapx {
 
@@ -787,10 +788,14 @@ __global__ void __launch_bounds__(kT)
         wl[v] = lam_wlr[v * ncell + q];
         wr[v] = lam_wlr[(5 + v) * ncell + q];
       }
-      // J^T lambda_F, kFluxDirs input directions per pass: the solvers on
-      // wider duals do not fit the register file
+      // J^T lambda_F: Roe and LMARS by their hand-written reverse sweeps;
+      // HLLC by forward-mode duals, kFluxDirs input directions per pass (the
+      // solver on wider duals does not fit the register file)
+      constexpr bool kReverse = SOLVER != SNAPX_RIEMANN_HLLC;
+      if (SOLVER == SNAPX_RIEMANN_ROE) roe_flux_adj(wl, wr, a.gammad, lf, lwl, lwr);
+      if (SOLVER == SNAPX_RIEMANN_LMARS) lmars_flux_adj(wl, wr, a.gammad, lf, lwl, lwr);
 #pragma unroll 1
-      for (int p0 = 0; p0 < 10; p0 += kFluxDirs) {
+      for (int p0 = 0; p0 < (kReverse ? 0 : 10); p0 += kFluxDirs) {
         Dual<kFluxDirs> sl[5], sr[5], flx[5];
 #pragma unroll
         for (int v = 0; v < 5; ++v) {
