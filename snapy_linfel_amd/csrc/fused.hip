@@ -46,6 +46,35 @@ constexpr int kRhsThreads = 256;
 constexpr int kMaxCellsPerThread = 4;  // tile cells <= 1024
 constexpr int kColThreads = 64;
 
+// Isentropic hydrostatic refill of the x1 ghosts (src/hydro/hydro.cpp:231-269)
+// in FP32.  The reference steps p^a, a = (gamma - 1) / gamma, outward one
+// layer at a time, p_n^a = p_{n-1}^a +- a g dz / K^(1/gamma) with
+// K = p_b / rho_b^gamma, and sets rho_n = (p_n / K)^(1/gamma).  In exact
+// arithmetic that is h_n = p_n^a = (p_b +- n a g dz rho_b) / P with
+// P = p_b^(1/gamma), p_n = h_n^(1/a) = h_n H_n and rho_n = rho_b H_n / P with
+// H_n = h_n^(1/(gamma - 1)): one powf per column end and one per layer
+// instead of two per end and three per layer, and the layers no longer wait
+// on each other.  (The FP64 plain path keeps the reference's order.)
+struct IsentropicRefill {
+  float rho_b, p_b, inv_P, step, inv_gm;
+  // gdz = -grav1 * dx1 (positive for gravity pointing to -x1); dir -1: the
+  // ghosts below the first interior cell, +1: above the last one
+  __device__ __forceinline__ IsentropicRefill(float rho, float p, float gamma, float gdz,
+                                              int dir)
+      : rho_b(rho), p_b(p) {
+    float gm = gamma - 1.f;
+    inv_P = 1.f / powf(p, 1.f / gamma);
+    step = (dir < 0 ? 1.f : -1.f) * (gm / gamma) * gdz * rho;
+    inv_gm = 1.f / gm;
+  }
+  __device__ __forceinline__ void layer(int n, float& rho, float& p) const {
+    float h = (p_b + (float)n * step) * inv_P;
+    float H = powf(h, inv_gm);
+    p = h * H;
+    rho = rho_b * H * inv_P;
+  }
+};
+
 struct FusedArgs {
   int n1, n2, n3, ng;
   int nx1, nx2, nx3;
@@ -168,19 +197,12 @@ __device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __r
       int y = a.h2 + cc % e2, z = a.h3 + cc / e2;
       float* rho = ws + IDN * H + (z * H2 + y) * H1;
       float* pre = ws + IPR * H + (z * H2 + y) * H1;
-      float gamma = gammad, gm = gamma - 1.f, ae = gm / gamma;
-      float inv_g = 1.f / gamma, inv_a = 1.f / ae;
-      float grav = -a.grav1;
       int is = upper ? 3 + e1 - 1 : 3;
-      float K = pre[is] / powf(rho[is], gamma);
-      float Kg = powf(K, inv_g);
-      for (int n = 0; n < 3; ++n) {
-        int src = upper ? is + n : is - n;
-        int dst = upper ? is + n + 1 : is - n - 1;
-        float h = upper ? powf(pre[src], ae) - ae * grav * a.dx1 / Kg
-                        : powf(pre[src], ae) + ae * grav * a.dx1 / Kg;
-        pre[dst] = powf(h, inv_a);
-        rho[dst] = powf(pre[dst] / K, inv_g);
+      const IsentropicRefill R(rho[is], pre[is], gammad, -a.grav1 * a.dx1, upper ? 1 : -1);
+#pragma unroll
+      for (int n = 1; n <= 3; ++n) {
+        int dst = upper ? is + n : is - n;
+        R.layer(n, rho[dst], pre[dst]);
       }
     }
     __syncthreads();
@@ -445,21 +467,17 @@ __device__ __forceinline__ void float_prim(const float* Uf, bool flip, double ga
 }
 
 // isentropic refill of the ghost next to boundary cell wb (dir -1: below,
-// +1: above) (src/hydro/hydro.cpp:231-269, first ghost).  FP32, the
-// arithmetic of the refill in the RHS kernel, which computes the fluxes of
-// the same stage from these ghost values.
+// +1: above; src/hydro/hydro.cpp:231-269, first ghost), with the FP32
+// arithmetic of the refill in the RHS kernel, which computes the fluxes of the
+// same stage from these ghost values.
 __device__ __forceinline__ void refill_ghost(const double* wb, double* wg, const ColArgs& a,
                                              int dir) {
-  float gamma = (float)a.gammad, gm = gamma - 1.f, ae = gm / gamma;
-  float inv_g = 1.f / gamma, inv_a = 1.f / ae;
-  float grav = -(float)a.grav1, dz = (float)a.dx1;
-  float rho = (float)wb[IDN], pre = (float)wb[IPR];
-  float K = pre / powf(rho, gamma);
-  float Kg = powf(K, inv_g);
-  float h = dir < 0 ? powf(pre, ae) + ae * grav * dz / Kg : powf(pre, ae) - ae * grav * dz / Kg;
-  float pg = powf(h, inv_a);
-  wg[IPR] = (double)pg;
-  wg[IDN] = (double)powf(pg / K, inv_g);
+  const IsentropicRefill R((float)wb[IDN], (float)wb[IPR], (float)a.gammad,
+                           -(float)a.grav1 * (float)a.dx1, dir);
+  float rho, pre;
+  R.layer(1, rho, pre);
+  wg[IPR] = (double)pre;
+  wg[IDN] = (double)rho;
 }
 
 template <int N>
