@@ -168,10 +168,14 @@ def c5_sensitivity(capi, dev, nsteps=200, every=20, ndir=2):
         fd = central(1e-6)
         ad = float((grad * d).sum())
         rels.append(abs(ad - fd) / abs(fd))
+        if ndir > 4 and (k + 1) % 8 == 0:
+            print(f"c5 finite differences: {k + 1}/{ndir} directions, max rel err "
+                  f"{max(rels):.3e}", file=sys.stderr, flush=True)
     return {"steps": nsteps, "checkpoint_every": every, "loss": L0,
             "forward_s": round(t_fwd, 3), "forward_plus_adjoint_s": round(t_grad, 3),
             "gradient_over_forward": round(t_grad / t_fwd, 3),
-            "fd_directions": ndir, "fd_rel_err_max": max(rels)}
+            "fd_directions": ndir, "fd_rel_err_max": max(rels),
+            "fd_rel_err_median": sorted(rels)[len(rels) // 2]}
 
 
 def measured_traffic(kernel, workload):
@@ -246,6 +250,8 @@ def main():
     ap.add_argument("--no-3d", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-c5", action="store_true", help="skip the 200-step sensitivity run")
+    ap.add_argument("--c5-directions", type=int, default=2,
+                    help="random directions of the finite-difference check of the gradient")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -474,7 +480,8 @@ def main():
         torch.cuda.synchronize()
         tadj = (time.perf_counter() - t0) / 3
         if not args.no_c5:
-            extra["c5_sensitivity_box3d_128x128x96_fp64"] = c5_sensitivity(capi, dev)
+            extra["c5_sensitivity_box3d_128x128x96_fp64"] = c5_sensitivity(
+                capi, dev, ndir=args.c5_directions)
         extra["adjoint_box3d_128x128x96_fp64"] = {
             "forward_ms_per_step": round(tf * 1e3, 3), "adjoint_ms_per_step": round(tadj * 1e3, 3),
             "adjoint_over_forward": round(tadj / tf, 3),
@@ -500,6 +507,10 @@ def main():
         "data": "synthetic (Straka initial state of examples/straka.cpp)",
         "config": {"workload": "straka_2d_512x64_weno5_lmars_vic3_dt0.1",
                    "cells": [512, 64], "dt_s": dt, "steps": K},
+        # the reference's own counter (src/mesh/meshblock.cpp:938-956): cells
+        # including ghosts x steps x stages / seconds
+        "reference_convention_value": round(
+            cfg.shape[1] * cfg.shape[2] * cfg.shape[3] * K * 3 / t32, 1),
         "fp64_plain": {"value": round(cus64, 1), "ms_per_step": round(t64 / K * 1e3, 5)},
         "fp32_fused_over_fp64_plain": round(cus32 / cus64, 3),
         "error_fp32_vs_fp64_after_steps": err,

@@ -733,13 +733,15 @@ __device__ __forceinline__ void column_pcr(float* __restrict__ u, float* __restr
   }
 
   double dd[5];
+  // every column inside one wavefront: no workgroup barriers
+  const bool wave = a.lanes == 64;
   if (a.scheme) {
     // ---- primitives of the cell and of its neighbours ----
     double wc[5], wm[5], wp[5];
     float_prim(Uc, false, a.gammad, wc);
 #pragma unroll
     for (int v = 0; v < 5; ++v) sh[v * T + t] = wc[v];
-    __syncthreads();
+    column_barrier(wave);
 #pragma unroll
     for (int v = 0; v < 5; ++v) wm[v] = wp[v] = wc[v];
     if (act) {
@@ -758,7 +760,7 @@ __device__ __forceinline__ void column_pcr(float* __restrict__ u, float* __restr
         refill_ghost(wc, wp, a, +1);
       }
     }
-    __syncthreads();
+    column_barrier(wave);
     // ---- interface |A|: lane i owns the face below its cell ----
     const double gm1 = a.gammad - 1.;
     const double gm1r = 0.5 * (a.gammad + a.gammad) - 1.;
@@ -766,14 +768,14 @@ __device__ __forceinline__ void column_pcr(float* __restrict__ u, float* __restr
     abs_jacobian_fast<double, N>(gm1r, wm, wc, Am);
 #pragma unroll
     for (int e = 0; e < NN; ++e) sh[e * T + t] = Am[e];
-    __syncthreads();
+    column_barrier(wave);
     if (act && i < nl - 1) {
 #pragma unroll
       for (int e = 0; e < NN; ++e) Ap[e] = sh[e * T + t + 1];
     } else {
       abs_jacobian_fast<double, N>(gm1r, wc, wp, Ap);
     }
-    __syncthreads();
+    column_barrier(wave);
     // ---- block row i (vic_assemble_cell; uniform Cartesian areas) ----
     double A[NN], B[NN], C[NN], r[N];
     {
@@ -830,7 +832,7 @@ __device__ __forceinline__ void column_pcr(float* __restrict__ u, float* __restr
     }
     // ---- parallel cyclic reduction, then delta = A^-1 r ----
     double d[N];
-    pcr_solve<N>(A, B, C, r, d, sh, T, t, i, nl, act);
+    pcr_solve<N>(A, B, C, r, d, sh, T, t, i, nl, act, wave);
     // ---- MS-VIC column sums ----
     double ai = wc[IDN] * a.vol;
     double sums[2] = {act ? ai * ai : 0., act ? (dd[IDN] - d[0]) * 1.0 * a.vol : 0.};
@@ -1098,6 +1100,15 @@ static int pcr_groups(const ColArgs& c, int N) {
   int cpw = pcr_threads(N) / c.lanes;
   return (c.ncol + cpw - 1) / cpw;
 }
+// Workgroup size of the single-block column kernel: whole columns, at most
+// pcr_threads(N) lanes, and no more columns per workgroup than leaves at least
+// 512 workgroups (two per CU), so that a block of few columns spreads over
+// every CU and each CU's LDS serves fewer wavefronts.
+static int pcr_block(const ColArgs& c, int N) {
+  int cpw = pcr_threads(N) / c.lanes;
+  while (cpw > 1 && (c.ncol + cpw - 1) / cpw < 512) cpw >>= 1;
+  return cpw * c.lanes;
+}
 
 struct EventPair {
   hipEvent_t a, b;
@@ -1137,13 +1148,16 @@ static int fused_step(const snapx_config* cfg, double dt, float* u, void* worksp
     c.w0 = W[s][0], c.w1 = W[s][1], c.w2 = W[s][2];
     hipEvent_t ca = ev ? ev[2 * s + 1].a : nullptr, cb = ev ? ev[2 * s + 1].b : nullptr;
     if (pcr_ok(c, N)) {
-      dim3 grid(pcr_groups(c, N)), block(pcr_threads(N));
+      const int threads = pcr_block(c, N), cpw = threads / c.lanes;
+      dim3 grid((c.ncol + cpw - 1) / cpw), block(threads);
+      const uint32_t smem = (uint32_t)(sizeof(double) * threads *
+                                       (N == 3 ? PcrShape<3>::kPub : PcrShape<5>::kPub));
       if (N == 3)
-        hipExtLaunchKernelGGL(fused_column_pcr_kernel<3>, grid, block, (uint32_t)pcr_smem(3), st,
-                              ca, cb, 0u, u, u0, (const float*)du, c);
+        hipExtLaunchKernelGGL(fused_column_pcr_kernel<3>, grid, block, smem, st, ca, cb, 0u, u,
+                              u0, (const float*)du, c);
       else
-        hipExtLaunchKernelGGL(fused_column_pcr_kernel<5>, grid, block, (uint32_t)pcr_smem(5), st,
-                              ca, cb, 0u, u, u0, (const float*)du, c);
+        hipExtLaunchKernelGGL(fused_column_pcr_kernel<5>, grid, block, smem, st, ca, cb, 0u, u,
+                              u0, (const float*)du, c);
     } else if (N == 3) {
       hipExtLaunchKernelGGL(fused_column_kernel<3>, dim3(cblk), dim3(kColThreads), 0u, st, ca, cb,
                             0u, u, u0, (const float*)du, scr, c);

@@ -120,6 +120,20 @@ __device__ __forceinline__ void pcr_publish(const double* A, const double* B, co
   }
 }
 
+// Barrier among the lanes of one column.  When every column lies within one
+// wavefront (64 lanes) a workgroup barrier is not needed: a wavefront executes
+// its LDS instructions in order, so the compiler only must not move LDS
+// accesses across this point.
+__device__ __forceinline__ void column_barrier(bool wave) {
+  if (wave) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
 __device__ __forceinline__ double wave_sum(double x) {
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) x += __shfl_xor(x, m);
@@ -128,11 +142,12 @@ __device__ __forceinline__ double wave_sum(double x) {
 
 // Solves the system held by the workgroup (all threads call it; act is false
 // for lanes without a row, whose published data is never read).  On return
-// x holds the solution of row i; A, B, C, r are destroyed.
+// x holds the solution of row i; A, B, C, r are destroyed.  wave: every
+// column lies within one wavefront (see column_barrier).
 template <int N>
 __device__ __forceinline__ void pcr_solve(double* A, double* B, double* C, double* r, double* x,
                                           double* __restrict__ sh, int T, int t, int i, int nl,
-                                          bool act) {
+                                          bool act, bool wave = false) {
   constexpr int NN = N * N;
   if (!act) {
 #pragma unroll
@@ -145,7 +160,7 @@ __device__ __forceinline__ void pcr_solve(double* A, double* B, double* C, doubl
   }
   for (int sft = 1; sft < nl; sft <<= 1) {
     pcr_publish<N>(A, B, C, r, sh, T, t);
-    __syncthreads();
+    column_barrier(wave);
     const bool hm = act && i - sft >= 0;
     const bool hp = act && i + sft < nl;
     double nB[NN], nC[NN];
@@ -196,7 +211,7 @@ __device__ __forceinline__ void pcr_solve(double* A, double* B, double* C, doubl
       B[e] = nB[e];
       C[e] = nC[e];
     }
-    __syncthreads();
+    column_barrier(wave);
   }
   if (N == 3) {
     double inv[9];
