@@ -178,11 +178,18 @@ def c5_sensitivity(capi, dev, nsteps=200, every=20, ndir=2):
             "fd_rel_err_median": sorted(rels)[len(rels) // 2]}
 
 
-def measured_traffic(kernel, workload):
-    """HBM bytes per launch of a kernel, as recorded from the FETCH_SIZE and
-    WRITE_SIZE counters in profiles/fused_kernels_pmc.md (None if absent)."""
+# VALU issue ceiling of MI355X: 256 CUs x 4 SIMDs, a wave64 VALU instruction
+# every 4 cycles per SIMD (16 lanes per cycle; FP64 and non-packed FP32 FMA
+# alike: 78.6 TFLOP/s = 256 x 4 x 16 x 2 flop x 2.4 GHz)
+VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4
+
+
+def measured_traffic(kernel, workload, what="traffic"):
+    """HBM bytes per launch of a kernel (what = "traffic"), as recorded from
+    the FETCH_SIZE and WRITE_SIZE counters in profiles/fused_kernels_pmc.md,
+    or its VALU wave-instructions per launch (what = "valu"); None if absent."""
     path = os.path.join(ROOT, "profiles", "fused_kernels_pmc.md")
-    key = f"traffic {kernel} {workload}:"
+    key = f"{what} {kernel} {workload}:"
     try:
         with open(path) as f:
             for line in f:
@@ -345,6 +352,10 @@ def main():
             "traffic": measured_traffic(name, "straka512x64"),
             "kernel": name, "avg_launch_ms": round(avg_ms, 5),
             "bytes_per_launch": bytes_}
+    valu = measured_traffic(name, "straka512x64", "valu")
+    if valu:
+        roof["valu_wave_instructions_per_launch"] = valu
+        roof["valu_issue_frac"] = round(valu / (avg_ms * 1e-3) / VALU_ISSUE_PEAK, 4)
 
     # attainable HBM rate: a 2 GiB device-to-device copy (read + write bytes)
     src = torch.empty(1 << 29, dtype=torch.float32, device=dev).fill_(1.0)
@@ -404,6 +415,12 @@ def main():
             "rhs_hbm_frac": round(2 * NV * 4 * c3.n_interior / (r3 * 1e-3) / 1e9 / HBM_PEAK_GBS,
                                   4),
         }
+        for key, kname, ms in (("rhs", "fused_rhs_kernel", r3),
+                               ("column", "fused_column_pcr_kernel<3>", c3ms)):
+            valu = measured_traffic(kname, "box256x256x128", "valu")
+            if valu:
+                extra["box3d_256x256x128_fp32"][f"{key}_valu_issue_frac"] = round(
+                    valu / (ms * 1e-3) / VALU_ISSUE_PEAK, 4)
 
     # ---- batched ensemble (96 members of uneven size), autotuned tiles,
     #      HIP-graph replays ----

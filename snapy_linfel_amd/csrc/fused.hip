@@ -167,13 +167,17 @@ __device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __r
   int e3 = a.n3 > 1 ? min(a.tx3, a.nx3 - o3) : 1;
   const float gammad = a.gammad;
 
-  // ---- 1. halo'd tile: conserved -> primitive into LDS ----
+  // ---- 1. halo'd tile: conserved -> primitive into LDS; with gravity the
+  //         physical x1 ghosts get the isentropic refill from the boundary
+  //         cell of their column (hydro.cpp:231-269) in the same pass ----
   const DivMod dH1(H1), dH2(H2);
+  const bool refill = a.grav1 != 0.f;
   for (int l = threadIdx.x; l < H; l += blockDim.x) {
     int x, y;
     int z = dH2.div(dH1.div(l, x), y);
+    const int gi = o1 + x - 3, gj = o2 + y - a.h2, gk = o3 + z - a.h3;
     bool fl[3];
-    int q = mapped_cell(a, o1 + x - 3, o2 + y - a.h2, o3 + z - a.h3, fl);
+    int q = mapped_cell(a, gi, gj, gk, fl);
     float U[5], W[5];
 #pragma unroll
     for (int v = 0; v < 5; ++v) U[v] = u[v * ncell + q];
@@ -181,34 +185,23 @@ __device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __r
     if (fl[1]) U[IVY] = -U[IVY];
     if (fl[2]) U[IVZ] = -U[IVZ];
     cons_to_prim(U, gammad, W);
+    if (refill && (gi < 0 || gi >= a.nx1)) {
+      const bool upper = gi >= a.nx1;
+      bool flb[3];
+      int qb = mapped_cell(a, upper ? a.nx1 - 1 : 0, gj, gk, flb);
+      float Ub[5], Wb[5];
+#pragma unroll
+      for (int v = 0; v < 5; ++v) Ub[v] = u[v * ncell + qb];
+      cons_to_prim(Ub, gammad, Wb);  // rho and p do not depend on the flips
+      const IsentropicRefill R(Wb[IDN], Wb[IPR], gammad, -a.grav1 * a.dx1, upper ? 1 : -1);
+      R.layer(upper ? gi - a.nx1 + 1 : -gi, W[IDN], W[IPR]);
+    }
 #pragma unroll
     for (int v = 0; v < 5; ++v) ws[v * H + l] = W[v];
   }
   __syncthreads();
 
-  // ---- 2. isentropic refill of the physical x1 ghosts (hydro.cpp:231-269) ----
-  if (a.grav1 != 0.f) {
-    bool bot = o1 == 0, top = o1 + e1 == a.nx1;
-    int ncolt = e2 * e3;
-    for (int c = threadIdx.x; c < 2 * ncolt; c += blockDim.x) {
-      bool upper = c >= ncolt;
-      if (upper ? !top : !bot) continue;
-      int cc = upper ? c - ncolt : c;
-      int y = a.h2 + cc % e2, z = a.h3 + cc / e2;
-      float* rho = ws + IDN * H + (z * H2 + y) * H1;
-      float* pre = ws + IPR * H + (z * H2 + y) * H1;
-      int is = upper ? 3 + e1 - 1 : 3;
-      const IsentropicRefill R(rho[is], pre[is], gammad, -a.grav1 * a.dx1, upper ? 1 : -1);
-#pragma unroll
-      for (int n = 1; n <= 3; ++n) {
-        int dst = upper ? is + n : is - n;
-        R.layer(n, rho[dst], pre[dst]);
-      }
-    }
-    __syncthreads();
-  }
-
-  // ---- 3. interface states + Riemann fluxes of every direction at once ----
+  // ---- 2. interface states + Riemann fluxes of every direction at once ----
   // The faces of all active directions form one index space (direction d
   // occupies [fo[d], fo[d] + nf[d])), so that a single pass and a single
   // barrier cover them; the fluxes go to fs [5][total faces].
@@ -391,7 +384,7 @@ __device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __r
     }
   }
 
-  // ---- 4. du = -dt D + gravity ----
+  // ---- 3. du = -dt D + gravity ----
   float vol = a.dx3 * (a.dx2 * a.dx1);
   float dt = a.dt;
 #pragma unroll
