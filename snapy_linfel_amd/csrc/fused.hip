@@ -135,10 +135,15 @@ __device__ __forceinline__ int mapped_cell(const FusedArgs& a, int i, int j, int
   return (qk * a.n2 + qj) * a.n1 + qi;
 }
 
+// reciprocals of the fused kernels: the FP32 hardware reciprocal (this unit
+// is built without correctly rounded FP32 division) and rcp_fast in FP64
+__device__ __forceinline__ float recip(float x) { return 1.f / x; }
+__device__ __forceinline__ double recip(double x) { return rcp_fast(x); }
+
 template <typename T>
 __device__ __forceinline__ void cons_to_prim(const T* U, T gammad, T* W) {
   W[IDN] = U[IDN];
-  T inv = T(1) / U[IDN];
+  T inv = recip(U[IDN]);
   W[IVX] = U[IVX] * inv;
   W[IVY] = U[IVY] * inv;
   W[IVZ] = U[IVZ] * inv;
@@ -756,6 +761,7 @@ __device__ __forceinline__ void column_pcr(float* __restrict__ u, float* __restr
     column_barrier(wave);
     // ---- interface |A|: lane i owns the face below its cell ----
     const double gm1 = a.gammad - 1.;
+    const double idt = rcp_fast(a.dt);
     const double gm1r = 0.5 * (a.gammad + a.gammad) - 1.;
     double Am[NN], Ap[NN];
     abs_jacobian_fast<double, N>(gm1r, wm, wc, Am);
@@ -775,7 +781,7 @@ __device__ __forceinline__ void column_pcr(float* __restrict__ u, float* __restr
       double Jp[NN], Jn[NN];
       flux_jacobian<double, N>(gm1, wm, Jp);
       flux_jacobian<double, N>(gm1, wp, Jn);
-      const double hv = 0.5 / a.vol;
+      const double hv = 0.5 * rcp_fast(a.vol);
 #pragma unroll
       for (int rr = 0; rr < N; ++rr)
 #pragma unroll
@@ -783,7 +789,7 @@ __device__ __forceinline__ void column_pcr(float* __restrict__ u, float* __restr
           int o = rr * N + c;
           double phi = ((rr == 1 && c == 0) || (vic_row(N, rr) == IPR && c == 1)) ? a.grav1 : 0.;
           A[o] = (Am[o] * a.area + Ap[o] * a.area) * hv - phi;
-          if (rr == c) A[o] += 1. / a.dt;
+          if (rr == c) A[o] += idt;
           B[o] = -(Am[o] + Jp[o]) * a.area * hv;
           C[o] = -(Ap[o] - Jn[o]) * a.area * hv;
         }
@@ -805,14 +811,14 @@ __device__ __forceinline__ void column_pcr(float* __restrict__ u, float* __restr
     }
 #pragma unroll
     for (int v = 0; v < 5; ++v) dd[v] = (double)Dv[v];
-    r[0] = dd[IDN] / a.dt;
-    r[1] = dd[IVX] / a.dt;
+    r[0] = dd[IDN] * idt;
+    r[1] = dd[IVX] * idt;
     if (N == 3) {
-      r[2] = dd[IPR] / a.dt;
+      r[2] = dd[IPR] * idt;
     } else {
-      r[2] = dd[IVY] / a.dt;
-      r[3] = dd[IVZ] / a.dt;
-      r[4] = dd[IPR] / a.dt;
+      r[2] = dd[IVY] * idt;
+      r[3] = dd[IVZ] * idt;
+      r[4] = dd[IPR] * idt;
     }
     if (!act) {
 #pragma unroll
@@ -825,7 +831,7 @@ __device__ __forceinline__ void column_pcr(float* __restrict__ u, float* __restr
     }
     // ---- parallel cyclic reduction, then delta = A^-1 r ----
     double d[N];
-    pcr_solve<N>(A, B, C, r, d, sh, T, t, i, nl, act, wave);
+    pcr_solve<N, true>(A, B, C, r, d, sh, T, t, i, nl, act, wave);
     // ---- MS-VIC column sums ----
     double ai = wc[IDN] * a.vol;
     double sums[2] = {act ? ai * ai : 0., act ? (dd[IDN] - d[0]) * 1.0 * a.vol : 0.};
@@ -833,7 +839,7 @@ __device__ __forceinline__ void column_pcr(float* __restrict__ u, float* __restr
     const double s_a2 = sums[0], s_b = sums[1];
     double rho = wc[IDN];
     double fix = d[0] - dd[IDN];
-    dd[IDN] = (dd[IDN] + fix) + rho * ai * s_b / s_a2;
+    dd[IDN] = (dd[IDN] + fix) + rho * ai * s_b * rcp_fast(s_a2);
     dd[IVX] = d[1];
     if (N == 3) {
       dd[IPR] = d[2];

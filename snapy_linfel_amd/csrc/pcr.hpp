@@ -77,13 +77,13 @@ __device__ __forceinline__ void solve_rows(double* A, double* R) {
 
 // normalised row of lane t: Pb = A^-1 B, Pc = A^-1 C, pr = A^-1 r, written
 // to LDS (structure of arrays, stride T)
-template <int N>
+template <int N, bool Fast>
 __device__ __forceinline__ void pcr_publish(const double* A, const double* B, const double* C,
                                             const double* r, double* sh, int T, int t) {
   const int NN = N * N;
   if (N == 3) {
     double inv[9], x[9];
-    inverse3<double>(A, inv);
+    inverse3<double, Fast>(A, inv);
     matmul<double, 3>(inv, B, x);
 #pragma unroll
     for (int e = 0; e < 9; ++e) sh[e * T + t] = x[e];
@@ -143,8 +143,9 @@ __device__ __forceinline__ double wave_sum(double x) {
 // Solves the system held by the workgroup (all threads call it; act is false
 // for lanes without a row, whose published data is never read).  On return
 // x holds the solution of row i; A, B, C, r are destroyed.  wave: every
-// column lies within one wavefront (see column_barrier).
-template <int N>
+// column lies within one wavefront (see column_barrier).  Fast: 3x3 inverses
+// through rcp_fast (N = 3 only; the N = 5 pivoted solve divides).
+template <int N, bool Fast = false>
 __device__ __forceinline__ void pcr_solve(double* A, double* B, double* C, double* r, double* x,
                                           double* __restrict__ sh, int T, int t, int i, int nl,
                                           bool act, bool wave = false) {
@@ -159,7 +160,7 @@ __device__ __forceinline__ void pcr_solve(double* A, double* B, double* C, doubl
     for (int e = 0; e < N; ++e) r[e] = 0.;
   }
   for (int sft = 1; sft < nl; sft <<= 1) {
-    pcr_publish<N>(A, B, C, r, sh, T, t);
+    pcr_publish<N, Fast>(A, B, C, r, sh, T, t);
     column_barrier(wave);
     const bool hm = act && i - sft >= 0;
     const bool hp = act && i + sft < nl;
@@ -215,7 +216,7 @@ __device__ __forceinline__ void pcr_solve(double* A, double* B, double* C, doubl
   }
   if (N == 3) {
     double inv[9];
-    inverse3<double>(A, inv);
+    inverse3<double, Fast>(A, inv);
     matvec<double, 3>(inv, r, x);
   } else {
     solve_rows<N, 1>(A, r);

@@ -100,25 +100,37 @@ __device__ __forceinline__ void abs_jacobian(T gm1, const T* wl, const T* wr, T*
   }
 }
 
-// abs_jacobian with the shared divisors inverted once (6 divisions instead
-// of 18); the same matrix up to rounding.  Used by the fused path, whose
-// state is FP32; the FP64 plain path keeps the reference's arithmetic.
+// 1/x within an ulp or two: the hardware reciprocal estimate refined by two
+// Newton steps, in place of the correctly rounded division sequence.  For
+// the fused path only (normal, finite x); the FP64 plain path keeps IEEE
+// division.
+__device__ __forceinline__ double rcp_fast(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
+// abs_jacobian with the shared divisors inverted once, by rcp_fast; the same
+// matrix up to rounding.  Used by the fused path, whose state is FP32; the
+// FP64 plain path keeps the reference's arithmetic.
 template <typename T, int N>
 __device__ __forceinline__ void abs_jacobian_fast(T gm1, const T* wl, const T* wr, T* A) {
   T sqrtdl = sqrt(wl[IDN]);
   T sqrtdr = sqrt(wr[IDN]);
-  T isdlpdr = T(1.0) / (sqrtdl + sqrtdr);
+  T isdlpdr = rcp_fast(sqrtdl + sqrtdr);
   T r = sqrtdl * sqrtdr;
   T u = (sqrtdl * wl[IVX] + sqrtdr * wr[IVX]) * isdlpdr;
   T v = (sqrtdl * wl[IVY] + sqrtdr * wr[IVY]) * isdlpdr;
   T w = (sqrtdl * wl[IVZ] + sqrtdr * wr[IVZ]) * isdlpdr;
-  T igm1 = T(1.0) / gm1;
+  T igm1 = rcp_fast(gm1);
   T el = wl[IPR] * igm1 + T(0.5) * wl[IDN] * ((sq(wl[IVX]) + sq(wl[IVY])) + sq(wl[IVZ]));
   T er = wr[IPR] * igm1 + T(0.5) * wr[IDN] * ((sq(wr[IVX]) + sq(wr[IVY])) + sq(wr[IVZ]));
   // (e + p) / sqrt(rho_l) = (e + p) sqrt(rho_r) / r
-  T ir = T(1.0) / r;
+  T ir = rcp_fast(r);
   T hbar = ((el + wl[IPR]) * sqrtdr + (er + wr[IPR]) * sqrtdl) * isdlpdr * ir;
-  T p = (hbar - T(0.5) * ((sq(u) + sq(v)) + sq(w))) * gm1 / (gm1 + T(1.)) * r;
+  T p = (hbar - T(0.5) * ((sq(u) + sq(v)) + sq(w))) * gm1 * rcp_fast(gm1 + T(1.)) * r;
   T cs = sqrt(p * (gm1 + T(1.)) * ir);
   T ke = T(0.5) * ((u * u + v * v) + w * w);
   T hp = (gm1 + T(1.)) * igm1 * p * ir;
@@ -129,9 +141,9 @@ __device__ __forceinline__ void abs_jacobian_fast(T gm1, const T* wl, const T* w
              v, v, v, T(1.), T(0.),
              w, w, w, T(0.), T(1.),
              h - u * cs, ke, h + u * cs, v, w};
-  T ihp = T(1.) / hp;
+  T ihp = rcp_fast(hp);
   T i2hp = T(0.5) * ihp;
-  T i2cshp = i2hp / cs;
+  T i2cshp = i2hp * rcp_fast(cs);
   T Ri[25] = {(cs * ke + u * hp) * i2cshp, (-hp - cs * u) * i2cshp, -v * i2hp, -w * i2hp, i2hp,
               (hp - ke) * ihp, u * ihp, v * ihp, w * ihp, -ihp,
               (cs * ke - u * hp) * i2cshp, (hp - cs * u) * i2cshp, -v * i2hp, -w * i2hp, i2hp,
@@ -151,7 +163,7 @@ __device__ __forceinline__ void abs_jacobian_fast(T gm1, const T* wl, const T* w
   }
 }
 
-template <typename T>
+template <typename T, bool Fast = false>
 __device__ __forceinline__ void inverse3(const T* m, T* inv) {
 #define M_(r, c) m[(r) * 3 + (c)]
 #define COF_(i, j)                                                         \
@@ -159,7 +171,7 @@ __device__ __forceinline__ void inverse3(const T* m, T* inv) {
    M_(((i) + 1) % 3, ((j) + 2) % 3) * M_(((i) + 2) % 3, ((j) + 1) % 3))
   T c00 = COF_(0, 0), c10 = COF_(1, 0), c20 = COF_(2, 0);
   T det = (c00 * M_(0, 0) + c10 * M_(1, 0)) + c20 * M_(2, 0);
-  T invdet = T(1.0) / det;
+  T invdet = Fast ? rcp_fast(det) : T(1.0) / det;
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
