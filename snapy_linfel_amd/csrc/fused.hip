@@ -155,8 +155,13 @@ __device__ __forceinline__ void cons_to_prim(const T* U, T gammad, T* W) {
 // tile (it synchronises the workgroup internally).  The arguments come by
 // value: a reference to the kernel argument makes the compiler keep a copy
 // of the whole struct in scratch memory.
+// kMerged: 1 / 0 when the launch knows that every tile takes the merged
+// face pass / the per-direction passes (the other path is not compiled in),
+// -1 to follow a.merged at run time (batches of mixed members).
+template <int kMerged>
 __device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __restrict__ du,
                                          const FusedArgs a, int b, float* smem) {
+  const bool merged = kMerged < 0 ? (a.merged != 0) : kMerged == 1;
   const int H1 = a.tx1 + 6, H2 = a.tx2 + 2 * a.h2, H3 = a.tx3 + 2 * a.h3;
   const int H = H1 * H2 * H3;
   float* ws = smem;          // primitives [5][H3][H2][H1]
@@ -172,11 +177,15 @@ __device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __r
   int e3 = a.n3 > 1 ? min(a.tx3, a.nx3 - o3) : 1;
   const float gammad = a.gammad;
 
-  // ---- 1. halo'd tile: conserved -> primitive into LDS; with gravity the
-  //         physical x1 ghosts get the isentropic refill from the boundary
-  //         cell of their column (hydro.cpp:231-269) in the same pass ----
+  // ---- 1. halo'd tile: conserved -> primitive into LDS, and the isentropic
+  //         refill of the physical x1 ghosts (hydro.cpp:231-269): for small
+  //         tiles (merged face pass) inside the load pass, by the ghost cell's
+  //         own thread from the boundary cell of its column; for large tiles
+  //         in a pass of its own, one thread per column end (fewer registers
+  //         in the load loop, which each thread runs many times) ----
   const DivMod dH1(H1), dH2(H2);
   const bool refill = a.grav1 != 0.f;
+  const float gdz = -a.grav1 * a.dx1;
   for (int l = threadIdx.x; l < H; l += blockDim.x) {
     int x, y;
     int z = dH2.div(dH1.div(l, x), y);
@@ -190,7 +199,9 @@ __device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __r
     if (fl[1]) U[IVY] = -U[IVY];
     if (fl[2]) U[IVZ] = -U[IVZ];
     cons_to_prim(U, gammad, W);
-    if (refill && (gi < 0 || gi >= a.nx1)) {
+    // (only the tile's own columns: the x1 ghosts of halo columns are never read)
+    if (merged && refill && (gi < 0 || gi >= a.nx1) && (unsigned)(y - a.h2) < (unsigned)e2 &&
+        (unsigned)(z - a.h3) < (unsigned)e3) {
       const bool upper = gi >= a.nx1;
       bool flb[3];
       int qb = mapped_cell(a, upper ? a.nx1 - 1 : 0, gj, gk, flb);
@@ -198,13 +209,30 @@ __device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __r
 #pragma unroll
       for (int v = 0; v < 5; ++v) Ub[v] = u[v * ncell + qb];
       cons_to_prim(Ub, gammad, Wb);  // rho and p do not depend on the flips
-      const IsentropicRefill R(Wb[IDN], Wb[IPR], gammad, -a.grav1 * a.dx1, upper ? 1 : -1);
+      const IsentropicRefill R(Wb[IDN], Wb[IPR], gammad, gdz, upper ? 1 : -1);
       R.layer(upper ? gi - a.nx1 + 1 : -gi, W[IDN], W[IPR]);
     }
 #pragma unroll
     for (int v = 0; v < 5; ++v) ws[v * H + l] = W[v];
   }
   __syncthreads();
+  if (!merged && refill) {
+    const bool bot = o1 == 0, top = o1 + e1 == a.nx1;
+    const int ncolt = e2 * e3;
+    for (int c = threadIdx.x; c < 2 * ncolt; c += blockDim.x) {
+      const bool upper = c >= ncolt;
+      if (upper ? !top : !bot) continue;
+      const int cc = upper ? c - ncolt : c;
+      const int y = a.h2 + cc % e2, z = a.h3 + cc / e2;
+      float* rho = ws + IDN * H + (z * H2 + y) * H1;
+      float* pre = ws + IPR * H + (z * H2 + y) * H1;
+      const int is = upper ? 3 + e1 - 1 : 3;
+      const IsentropicRefill R(rho[is], pre[is], gammad, gdz, upper ? 1 : -1);
+#pragma unroll
+      for (int n = 1; n <= 3; ++n) R.layer(n, rho[upper ? is + n : is - n], pre[upper ? is + n : is - n]);
+    }
+    __syncthreads();
+  }
 
   // ---- 2. interface states + Riemann fluxes of every direction at once ----
   // The faces of all active directions form one index space (direction d
@@ -224,7 +252,7 @@ __device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __r
   const int fo1 = nf0, fo2 = nf0 + nf1, ftot = fo2 + nf2;
   const DivMod dq1(e1 + 1), dq2(e2 + 1);
 
-  if (a.merged) {
+  if (merged) {
     // few faces per thread: every direction in one pass, one barrier
     for (int g = threadIdx.x; g < ftot; g += blockDim.x) {
       const int d = g < fo1 ? 0 : (g < fo2 ? 1 : 2);
@@ -424,10 +452,11 @@ __device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __r
   }
 }
 
+template <int kMerged>
 __global__ void __launch_bounds__(kRhsThreads)
     fused_rhs_kernel(const float* __restrict__ u, float* __restrict__ du, FusedArgs a) {
   extern __shared__ float smem[];
-  rhs_tile(u, du, a, blockIdx.x, smem);
+  rhs_tile<kMerged>(u, du, a, blockIdx.x, smem);
 }
 
 struct ColArgs {
@@ -915,7 +944,7 @@ __global__ void __launch_bounds__(kRhsThreads)
   for (int item = blockIdx.x; item < total; item += gridDim.x) {
     int m = find_member(mem, nm, item, kListTiles);
     FusedArgs a = mem[m].a;
-    rhs_tile(u + mem[m].off, du + mem[m].off, a, item - mem[m].tile0, smem);
+    rhs_tile<-1>(u + mem[m].off, du + mem[m].off, a, item - mem[m].tile0, smem);
     __syncthreads();
   }
 }
@@ -1140,8 +1169,12 @@ static int fused_step(const snapx_config* cfg, double dt, float* u, void* worksp
   // the kernel dispatch itself (hipExtLaunchKernel), not extra stream markers
   for (int s = 0; s < 3; ++s) {
     hipEvent_t ra = ev ? ev[2 * s].a : nullptr, rb = ev ? ev[2 * s].b : nullptr;
-    hipExtLaunchKernelGGL(fused_rhs_kernel, dim3(nblk), dim3(kRhsThreads), (uint32_t)smem, st, ra,
-                          rb, 0u, (const float*)u, du, a);
+    if (a.merged)
+      hipExtLaunchKernelGGL(fused_rhs_kernel<1>, dim3(nblk), dim3(kRhsThreads), (uint32_t)smem, st,
+                            ra, rb, 0u, (const float*)u, du, a);
+    else
+      hipExtLaunchKernelGGL(fused_rhs_kernel<0>, dim3(nblk), dim3(kRhsThreads), (uint32_t)smem, st,
+                            ra, rb, 0u, (const float*)u, du, a);
     SNAPX_LAUNCH_CHECK();
     c.stage = s;
     c.w0 = W[s][0], c.w1 = W[s][1], c.w2 = W[s][2];
