@@ -108,3 +108,25 @@ def test_fused_graph_replays_match_eager():
         capi.fused_rk3_steps(cfg, 0.25, 3, ue)
         torch.cuda.synchronize()
         assert torch.equal(ug, ue), rep
+
+
+def test_batch_mixed_column_heights_exact():
+    """Members whose columns take one wavefront, two and four (64-, 128- and
+    256-lane column groups in one launch) reproduce the single-block path
+    exactly."""
+    mem = []
+    for nx2, nx1 in ((12, 20), (8, 90), (6, 150), (10, 64)):
+        oc = O.straka_config(nx2=nx2, nx1=nx1, implicit_scheme=1, riemann="hllc")
+        oc.x2max = 50.0 * nx2
+        oc.x1max = 50.0 * nx1
+        w = O.straka_init(oc, xc=25.0 * nx2, xr=300.0, zc=25.0 * nx1, zr=200.0, dT=0.5)
+        w, u = O.initialize(oc, w)
+        mem.append((oc, u))
+    cfgs = [cfg_of(oc) for oc, _ in mem]
+    b = capi.Batch(cfgs, 0.25)
+    u = torch.cat([to_dev(u0, torch.float32).reshape(-1) for _, u0 in mem])
+    b.steps(u, 2)
+    for m, (oc, u0) in enumerate(mem):
+        single = to_dev(u0, torch.float32)
+        capi.fused_rk3_steps(cfgs[m], 0.25, 2, single)
+        assert torch.equal(b.member(u, m), single), m

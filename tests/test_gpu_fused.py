@@ -81,3 +81,19 @@ def test_fused_straka_multistep_error_and_mass():
     m0 = u[0][I[1:]].sum()
     m1 = got[0][I[1:]].astype(np.float64).sum()
     assert abs(m1 - m0) / m0 < 1e-6
+
+
+@pytest.mark.parametrize("scheme", [1, 9])
+@pytest.mark.parametrize("nx1", [64, 100, 200, 300])
+def test_fused_column_heights(nx1, scheme):
+    """Columns of one wavefront, of several (cross-wavefront PCR barriers and
+    column sums), and taller than the column workgroup (the lane-per-column
+    block Thomas kernel), with a ragged x2 tile (6 columns, tiles of 4)."""
+    oc, w, u = _straka_small(scheme, nx2=6, nx1=nx1, wind=10.0)  # dx1 = 6.4 km / nx1
+    dt = 0.2
+    u_ref, _ = O.rk3_step(oc, dt, u)
+    cfg = cfg_of(oc)
+    uf = to_dev(u, torch.float32)
+    capi.fused_rk3_step(cfg, dt, uf)
+    e = var_errors(host(uf), u_ref, O.interior(oc))
+    assert (e < 1e-5).all(), (nx1, scheme, e)
