@@ -415,7 +415,7 @@ def main():
             "rhs_hbm_frac": round(2 * NV * 4 * c3.n_interior / (r3 * 1e-3) / 1e9 / HBM_PEAK_GBS,
                                   4),
         }
-        for key, kname, ms in (("rhs", "fused_rhs_kernel", r3),
+        for key, kname, ms in (("rhs", "fused_rhs_march_kernel", r3),
                                ("column", "fused_column_pcr_kernel<3>", c3ms)):
             valu = measured_traffic(kname, "box256x256x128", "valu")
             if valu:
