@@ -275,8 +275,11 @@ typedef struct snapx_batch_info {
   int64_t scratch_doubles;
   size_t rhs_smem;
   double dt;
-  int col_groups; /* work items of the cell-parallel column kernel (0: the
-                     lane-per-column kernel serves the batch) */
+  int col_groups; /* work items of the cell-parallel column kernel, one row
+                     per lane (col_groups + col_groups2 = 0: the lane-per-column
+                     kernel serves the batch) */
+  int col_groups2; /* work items of the cell-parallel column kernel with two
+                      rows per lane (3x3 columns of more than 64 cells) */
 } snapx_batch_info;
 
 /* Members are independent blocks, each with its own snapx_config (same

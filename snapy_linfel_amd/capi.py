@@ -143,7 +143,7 @@ class BatchInfo(C.Structure):
                 ("tile_x3", C.c_int), ("total_tiles", C.c_int), ("total_cols", C.c_int),
                 ("vic_size", C.c_int), ("total_elems", C.c_int64),
                 ("scratch_doubles", C.c_int64), ("rhs_smem", C.c_size_t), ("dt", C.c_double),
-                ("col_groups", C.c_int)]
+                ("col_groups", C.c_int), ("col_groups2", C.c_int)]
 
 
 class Timing(C.Structure):

@@ -668,6 +668,7 @@ struct ColArgs {
   int scheme;         // 0 none, 1 partial, 9 full
   int stage;
   int lanes, lshift;  // cell-parallel kernel: lanes per column (power of two), log2
+  int rows2;          // two rows per lane (N = 3 columns of more than 64 cells)
   double gammad, dx1, area, vol, dt, grav1;
   double w0, w1, w2;  // stage weights
 };
@@ -917,6 +918,313 @@ __global__ void __launch_bounds__(kColThreads)
 
 // Every thread of the workgroup calls this with the same ColArgs; lane t
 // serves layer i = t mod L of column col (col < a.ncol checked here).
+// Block row i of a column's system (vic_assemble_cell, uniform Cartesian
+// areas): wm / wp the primitives below / above, Am / Ap the interface |A|
+// below / above, Dv the explicit tendency of the cell; reflecting column ends
+// fold the ghost coupling into the diagonal; inactive rows are identities.
+template <int N>
+__device__ __forceinline__ void block_row(const ColArgs& a, int i, int nl, bool act,
+                                          const double* wm, const double* wp, const double* Am,
+                                          const double* Ap, const float* Dv, double gm1,
+                                          double idt, double* A, double* B, double* C,
+                                          double* r) {
+  constexpr int NN = N * N;
+  double Jp[NN], Jn[NN];
+  flux_jacobian<double, N>(gm1, wm, Jp);
+  flux_jacobian<double, N>(gm1, wp, Jn);
+  const double hv = 0.5 * rcp_fast(a.vol);
+#pragma unroll
+  for (int rr = 0; rr < N; ++rr)
+#pragma unroll
+    for (int c = 0; c < N; ++c) {
+      int o = rr * N + c;
+      double phi = ((rr == 1 && c == 0) || (vic_row(N, rr) == IPR && c == 1)) ? a.grav1 : 0.;
+      A[o] = (Am[o] * a.area + Ap[o] * a.area) * hv - phi;
+      if (rr == c) A[o] += idt;
+      B[o] = -(Am[o] + Jp[o]) * a.area * hv;
+      C[o] = -(Ap[o] - Jn[o]) * a.area * hv;
+    }
+#pragma unroll
+  for (int rr = 0; rr < N; ++rr)
+#pragma unroll
+    for (int c = 0; c < N; ++c) {
+      double bnd = c == 1 ? -1. : 1.;
+      if (i == 0) A[rr * N + c] += B[rr * N + c] * bnd;
+      if (i == nl - 1) A[rr * N + c] += C[rr * N + c] * bnd;
+    }
+  if (i == 0)
+#pragma unroll
+    for (int e = 0; e < NN; ++e) B[e] = 0.;
+  if (i == nl - 1)
+#pragma unroll
+    for (int e = 0; e < NN; ++e) C[e] = 0.;
+  r[0] = (double)Dv[IDN] * idt;
+  r[1] = (double)Dv[IVX] * idt;
+  if (N == 3) {
+    r[2] = (double)Dv[IPR] * idt;
+  } else {
+    r[2] = (double)Dv[IVY] * idt;
+    r[3] = (double)Dv[IVZ] * idt;
+    r[4] = (double)Dv[IPR] * idt;
+  }
+  if (!act) {
+#pragma unroll
+    for (int e = 0; e < NN; ++e) {
+      A[e] = (e % (N + 1)) == 0 ? 1. : 0.;
+      B[e] = C[e] = 0.;
+    }
+#pragma unroll
+    for (int e = 0; e < N; ++e) r[e] = 0.;
+  }
+}
+
+// MS-VIC redistribution of one cell (vic_redistribute_impl.h:66-106) from its
+// solution d and the column sums, then the SSP-RK3 stage combine and store
+__device__ __forceinline__ void finish_cell(float* __restrict__ u, float* __restrict__ u0,
+                                            const ColArgs& a, int ncell, int q, int N,
+                                            const double* d, double rho, double ai, double s_a2,
+                                            double s_b, const float* Dv, const float* Uc,
+                                            const float* Ub) {
+  double dd[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) dd[v] = (double)Dv[v];
+  if (a.scheme) {
+    double fix = d[0] - dd[IDN];
+    dd[IDN] = (dd[IDN] + fix) + rho * ai * s_b * rcp_fast(s_a2);
+    dd[IVX] = d[1];
+    if (N == 3) {
+      dd[IPR] = d[2];
+    } else {
+      dd[IVY] = d[2];
+      dd[IVZ] = d[3];
+      dd[IPR] = d[4];
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    float uc = Uc[v];
+    float ub = a.stage == 0 ? uc : Ub[v];
+    if (a.stage == 0) u0[v * ncell + q] = uc;
+    u[v * ncell + q] = (float)((a.w0 * ub + a.w1 * uc) + a.w2 * dd[v]);
+  }
+}
+
+// K2 for 3x3 columns of more than 64 cells: two rows per lane (rows 2m and
+// 2m + 1 in lane m of the column).  One cyclic-reduction level inside each
+// lane removes the even rows: the odd row eliminates the even row below it
+// from its own registers and the even row above it from the next lane
+// (through LDS); the odd rows, one per lane, are then solved by parallel
+// cyclic reduction and the even rows follow by back substitution.  Against
+// one row per lane this halves the PCR work and the lanes (a column of up to
+// 128 cells is one wavefront, without workgroup barriers).
+template <int N>
+__device__ __forceinline__ void column_pcr2(float* __restrict__ u, float* __restrict__ u0,
+                                            const float* __restrict__ du, const ColArgs& a,
+                                            int col, double* __restrict__ sh) {
+  constexpr int NN = N * N;
+  const int T = blockDim.x, t = threadIdx.x;
+  const int ncell = a.n1 * a.n2 * a.n3;
+  const int nl = a.nl;
+  const int half = a.lanes;  // lanes per column
+  const int m = t & (half - 1);
+  const int i0 = 2 * m, i1 = 2 * m + 1;
+  const bool colok = col < a.ncol;
+  const bool act0 = colok && i0 < nl, act1 = colok && i1 < nl;
+  int qb = 0;  // full index of the column's interior cell 0
+  if (colok) {
+    int j = a.n2 > 1 ? a.ng + col % a.ncol2 : 0;
+    int k = a.n3 > 1 ? a.ng + col / a.ncol2 : 0;
+    qb = (k * a.n2 + j) * a.n1 + a.ng;
+  }
+  const int q0 = qb + i0, q1 = qb + i1;
+
+  // ---- every global load of the lane first ----
+  float Uc0[5] = {1.f, 0.f, 0.f, 0.f, 1.f}, Uc1[5] = {1.f, 0.f, 0.f, 0.f, 1.f};
+  float Ub0[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, Ub1[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+  float Dv0[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, Dv1[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+  float Um[5] = {1.f, 0.f, 0.f, 0.f, 1.f}, Up[5] = {1.f, 0.f, 0.f, 0.f, 1.f};
+  bool flm = false, flp = false;
+  const bool top0 = act0 && i0 == nl - 1, top1 = act1 && i1 == nl - 1;
+  if (act0) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      Uc0[v] = u[v * ncell + q0];
+      Dv0[v] = du[v * ncell + q0];
+      if (a.stage != 0) Ub0[v] = u0[v * ncell + q0];
+    }
+    if (a.scheme && i0 == 0) {
+      int ii = bc_map(-1, nl, a.bc_in, a.bc_out, flm);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) Um[v] = u[v * ncell + qb + ii];
+    }
+  }
+  if (act1) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      Uc1[v] = u[v * ncell + q1];
+      Dv1[v] = du[v * ncell + q1];
+      if (a.stage != 0) Ub1[v] = u0[v * ncell + q1];
+    }
+  }
+  if (a.scheme && (top0 || top1)) {
+    int ii = bc_map(nl, nl, a.bc_in, a.bc_out, flp);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) Up[v] = u[v * ncell + qb + ii];
+  }
+
+  double d0[N], d1[N];
+#pragma unroll
+  for (int e = 0; e < N; ++e) d0[e] = d1[e] = 0.;
+  double w0[5], w1[5];
+  float_prim(Uc0, false, a.gammad, w0);
+  float_prim(Uc1, false, a.gammad, w1);
+  double s_a2 = 1., s_b = 0.;
+  const bool wave = half == 64;
+  if (a.scheme) {
+    double Pb0[NN], Pc0[NN], pr0[N];  // the even row, normalised
+    // ---- primitives of the neighbours ----
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      sh[v * T + t] = w0[v];
+      sh[(5 + v) * T + t] = w1[v];
+    }
+    column_barrier(wave);
+    double wm0[5], wp0[5], wp1[5];  // below row i0, above row i0, above row i1
+#pragma unroll
+    for (int v = 0; v < 5; ++v) wm0[v] = wp0[v] = wp1[v] = w1[v];
+    if (act0) {
+      if (m > 0) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) wm0[v] = sh[(5 + v) * T + t - 1];
+      } else {
+        float_prim(Um, flm, a.gammad, wm0);
+        refill_ghost(w0, wm0, a, -1);
+      }
+      if (top0) {
+        float_prim(Up, flp, a.gammad, wp0);
+        refill_ghost(w0, wp0, a, +1);
+      }
+    }
+    if (act1) {
+      if (!top1) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) wp1[v] = sh[v * T + t + 1];
+      } else {
+        float_prim(Up, flp, a.gammad, wp1);
+        refill_ghost(w1, wp1, a, +1);
+      }
+    }
+    column_barrier(wave);
+    // ---- interface |A| of the faces below rows i0, i1 and above i1 ----
+    const double gm1 = a.gammad - 1.;
+    const double idt = rcp_fast(a.dt);
+    const double gm1r = 0.5 * (a.gammad + a.gammad) - 1.;
+    double Af1[NN], Af2[NN];
+    {
+      double Af0[NN];
+      abs_jacobian_fast<double, N>(gm1r, wm0, w0, Af0);
+#pragma unroll
+      for (int e = 0; e < NN; ++e) sh[e * T + t] = Af0[e];
+      column_barrier(wave);
+      abs_jacobian_fast<double, N>(gm1r, w0, wp0, Af1);
+      if (act1 && !top1) {
+#pragma unroll
+        for (int e = 0; e < NN; ++e) Af2[e] = sh[e * T + t + 1];
+      } else {
+        abs_jacobian_fast<double, N>(gm1r, w1, wp1, Af2);
+      }
+      column_barrier(wave);
+      // ---- even row: block row, normalised to registers and LDS ----
+      double A0[NN], B0[NN], C0[NN], r0[N];
+      block_row<N>(a, i0, nl, act0, wm0, wp0, Af0, Af1, Dv0, gm1, idt, A0, B0, C0, r0);
+      double inv[NN];
+      inverse3<double, true>(A0, inv);
+      matmul<double, N>(inv, B0, Pb0);
+      matmul<double, N>(inv, C0, Pc0);
+      matvec<double, N>(inv, r0, pr0);
+    }
+#pragma unroll
+    for (int e = 0; e < NN; ++e) {
+      sh[e * T + t] = Pb0[e];
+      sh[(NN + e) * T + t] = Pc0[e];
+    }
+#pragma unroll
+    for (int e = 0; e < N; ++e) sh[(2 * NN + e) * T + t] = pr0[e];
+    // ---- odd row ----
+    double A1[NN], B1[NN], C1[NN], r1[N];
+    block_row<N>(a, i1, nl, act1, w0, wp1, Af1, Af2, Dv1, gm1, idt, A1, B1, C1, r1);
+    column_barrier(wave);
+    {
+      double nB[NN], nC[NN], y[NN];
+      // the even row below (own registers) couples row i1 to row i1 - 2
+      matmul<double, N>(B1, Pb0, y);
+#pragma unroll
+      for (int e = 0; e < NN; ++e) nB[e] = -y[e];
+      matmul<double, N>(B1, Pc0, y);
+#pragma unroll
+      for (int e = 0; e < NN; ++e) A1[e] -= y[e];
+      matvec<double, N>(B1, pr0, y);
+#pragma unroll
+      for (int e = 0; e < N; ++e) r1[e] -= y[e];
+      // the even row above (next lane) couples it to row i1 + 2
+#pragma unroll
+      for (int e = 0; e < NN; ++e) nC[e] = 0.;
+      if (m + 1 < half) {
+        double Pb[NN], Pc[NN], pr[N];
+#pragma unroll
+        for (int e = 0; e < NN; ++e) {
+          Pb[e] = sh[e * T + t + 1];
+          Pc[e] = sh[(NN + e) * T + t + 1];
+        }
+#pragma unroll
+        for (int e = 0; e < N; ++e) pr[e] = sh[(2 * NN + e) * T + t + 1];
+        matmul<double, N>(C1, Pc, y);
+#pragma unroll
+        for (int e = 0; e < NN; ++e) nC[e] = -y[e];
+        matmul<double, N>(C1, Pb, y);
+#pragma unroll
+        for (int e = 0; e < NN; ++e) A1[e] -= y[e];
+        matvec<double, N>(C1, pr, y);
+#pragma unroll
+        for (int e = 0; e < N; ++e) r1[e] -= y[e];
+      }
+#pragma unroll
+      for (int e = 0; e < NN; ++e) {
+        B1[e] = nB[e];
+        C1[e] = nC[e];
+      }
+    }
+    column_barrier(wave);
+    // ---- the odd rows by parallel cyclic reduction (lane m = row 2m + 1) ----
+    pcr_solve<N, true>(A1, B1, C1, r1, d1, sh, T, t, m, nl / 2, act1, wave);
+    // ---- even rows by back substitution ----
+#pragma unroll
+    for (int e = 0; e < N; ++e) sh[e * T + t] = d1[e];
+    column_barrier(wave);
+    double xb[N];
+#pragma unroll
+    for (int e = 0; e < N; ++e) xb[e] = m > 0 ? sh[e * T + t - 1] : 0.;
+#pragma unroll
+    for (int rr = 0; rr < N; ++rr) {
+      double v = pr0[rr];
+#pragma unroll
+      for (int c = 0; c < N; ++c) v -= Pb0[rr * N + c] * xb[c] + Pc0[rr * N + c] * d1[c];
+      d0[rr] = v;
+    }
+    // ---- MS-VIC column sums ----
+    const double ai0 = w0[IDN] * a.vol, ai1 = w1[IDN] * a.vol;
+    double sums[2] = {(act0 ? ai0 * ai0 : 0.) + (act1 ? ai1 * ai1 : 0.),
+                      (act0 ? ((double)Dv0[IDN] - d0[0]) * 1.0 * a.vol : 0.) +
+                          (act1 ? ((double)Dv1[IDN] - d1[0]) * 1.0 * a.vol : 0.)};
+    column_sums<2>(sums, sh, t, a.lanes, a.lshift);
+    s_a2 = sums[0], s_b = sums[1];
+  }
+  if (act0)
+    finish_cell(u, u0, a, ncell, q0, N, d0, w0[IDN], w0[IDN] * a.vol, s_a2, s_b, Dv0, Uc0, Ub0);
+  if (act1)
+    finish_cell(u, u0, a, ncell, q1, N, d1, w1[IDN], w1[IDN] * a.vol, s_a2, s_b, Dv1, Uc1, Ub1);
+}
+
 template <int N>
 __device__ __forceinline__ void column_pcr(float* __restrict__ u, float* __restrict__ u0,
                                            const float* __restrict__ du, const ColArgs& a,
@@ -1104,6 +1412,14 @@ __global__ void __launch_bounds__(PcrShape<N>::kThreads)
   column_pcr<N>(u, u0, du, a, col, sh);
 }
 
+__global__ void __launch_bounds__(PcrShape<3>::kThreads, 2)
+    fused_column_pcr2_kernel(float* __restrict__ u, float* __restrict__ u0,
+                             const float* __restrict__ du, ColArgs a) {
+  extern __shared__ double sh[];
+  int col = blockIdx.x * (blockDim.x >> a.lshift) + (threadIdx.x >> a.lshift);
+  column_pcr2<3>(u, u0, du, a, col, sh);
+}
+
 // ------------------------------------------------------- batched members ----
 
 // One member of a batch: its block geometry / options, where its fields start
@@ -1119,17 +1435,22 @@ struct BatchMember {
                     // (multiple of the wavefront size)
   int grp0;         // first K2 work item of the cell-parallel kernel (one
                     // workgroup of whole columns)
+  int grp2;         // first work item of the two-rows-per-lane column kernel
 };
 
-enum { kListTiles = 0, kListCols = 1, kListGroups = 2 };
+enum { kListTiles = 0, kListCols = 1, kListGroups = 2, kListGroups2 = 3 };
 
 __device__ __forceinline__ int find_member(const BatchMember* m, int nm, int item, int list) {
   int lo = 0, hi = nm - 1;
   while (lo < hi) {
     int mid = (lo + hi + 1) >> 1;
-    int start = list == kListTiles ? m[mid].tile0
+    // members outside a list start where the next member of the list
+    // starts; among equal starts the last member is taken, which is the one
+    // that owns the items
+    int start = list == kListTiles  ? m[mid].tile0
                 : list == kListCols ? m[mid].col0
-                                    : m[mid].grp0;
+                : list == kListGroups ? m[mid].grp0
+                                      : m[mid].grp2;
     if (start <= item)
       lo = mid;
     else
@@ -1190,6 +1511,26 @@ __global__ void __launch_bounds__(PcrShape<N>::kThreads)
     int col = (item - mem[m].grp0) * (blockDim.x >> c.lshift) + (threadIdx.x >> c.lshift);
     int64_t o = mem[m].off;
     column_pcr<N>(u + o, u0 + o, du + o, c, col, sh);
+    __syncthreads();  // LDS is reused by the next item
+  }
+}
+
+// persistent two-rows-per-lane K2 (3x3) over its (member, column group) list
+__global__ void __launch_bounds__(PcrShape<3>::kThreads, 2)
+    batch_column_pcr2_kernel(float* __restrict__ u, float* __restrict__ u0,
+                             const float* __restrict__ du, const BatchMember* __restrict__ mem,
+                             int nm, int total, int stage) {
+  extern __shared__ double sh[];
+  static const double W[3][3] = {
+      {0., 1., 1.}, {3. / 4., 1. / 4., 1. / 4.}, {1. / 3., 2. / 3., 2. / 3.}};
+  for (int item = blockIdx.x; item < total; item += gridDim.x) {
+    int m = find_member(mem, nm, item, kListGroups2);
+    ColArgs c = mem[m].c;
+    c.stage = stage;
+    c.w0 = W[stage][0], c.w1 = W[stage][1], c.w2 = W[stage][2];
+    int col = (item - mem[m].grp2) * (blockDim.x >> c.lshift) + (threadIdx.x >> c.lshift);
+    int64_t o = mem[m].off;
+    column_pcr2<3>(u + o, u0 + o, du + o, c, col, sh);
     __syncthreads();  // LDS is reused by the next item
   }
 }
@@ -1342,6 +1683,9 @@ static int make_col_args(const snapx_config* cfg, double dt, ColArgs* out) {
   c.w0 = 0., c.w1 = 1., c.w2 = 1.;
   c.lanes = 64, c.lshift = 6;
   while (c.lanes < c.nl) c.lanes <<= 1, c.lshift += 1;
+  // 3x3 columns of more than 64 cells: two rows per lane (column_pcr2)
+  c.rows2 = (!((cfg->implicit_scheme >> 3) & 1) && c.lanes >= 128) ? 1 : 0;
+  if (c.rows2) c.lanes >>= 1, c.lshift -= 1;
   SNAPX_CHECK_ARG(!c.scheme || c.grav1 != 0.,
                   "[ImplicitHydro] forcing does not have const-gravity");
   *out = c;
@@ -1425,7 +1769,10 @@ static int fused_step(const snapx_config* cfg, double dt, float* u, void* worksp
       dim3 grid((c.ncol + cpw - 1) / cpw), block(threads);
       const uint32_t smem = (uint32_t)(sizeof(double) * threads *
                                        (N == 3 ? PcrShape<3>::kPub : PcrShape<5>::kPub));
-      if (N == 3)
+      if (N == 3 && c.rows2)
+        hipExtLaunchKernelGGL(fused_column_pcr2_kernel, grid, block, smem, st, ca, cb, 0u, u, u0,
+                              (const float*)du, c);
+      else if (N == 3)
         hipExtLaunchKernelGGL(fused_column_pcr_kernel<3>, grid, block, smem, st, ca, cb, 0u, u,
                               u0, (const float*)du, c);
       else
@@ -1540,7 +1887,7 @@ static int batch_layout(const snapx_config* members, int nm, double dt, int tx1,
   Tile t = {tx1 > 0 ? tx1 : def.tx1, tx2 > 0 ? tx2 : def.tx2, tx3 > 0 ? tx3 : def.tx3};
   table->resize(nm);
   int64_t off = 0, soff = 0;
-  int tiles = 0, cols = 0, groups = 0;
+  int tiles = 0, cols = 0, groups = 0, groups2 = 0;
   bool pcr = true;
   size_t smem = 0;
   for (int m = 0; m < nm; ++m) {
@@ -1553,6 +1900,7 @@ static int batch_layout(const snapx_config* members, int nm, double dt, int tx1,
     b.tile0 = tiles;
     b.col0 = cols;
     b.grp0 = groups;
+    b.grp2 = groups2;
     Grid g = make_grid(&members[m]);
     off += 5 * (int64_t)g.ncell;
     soff += (int64_t)scratch_bytes(&members[m]) / sizeof(double);
@@ -1560,13 +1908,14 @@ static int batch_layout(const snapx_config* members, int nm, double dt, int tx1,
     cols += (b.c.ncol + kColThreads - 1) / kColThreads * kColThreads;
     int N = ((scheme >> 3) & 1) ? 5 : 3;
     pcr &= pcr_ok(b.c, N);
-    if (pcr_ok(b.c, N)) groups += pcr_groups(b.c, N);
+    if (pcr_ok(b.c, N)) (b.c.rows2 ? groups2 : groups) += pcr_groups(b.c, N);
   }
   info->nmembers = nm;
   info->tile_x1 = t.tx1, info->tile_x2 = t.tx2, info->tile_x3 = t.tx3;
   info->total_tiles = tiles;
   info->total_cols = cols;
   info->col_groups = pcr ? groups : 0;
+  info->col_groups2 = pcr ? groups2 : 0;
   info->total_elems = off;
   info->scratch_doubles = soff;
   info->rhs_smem = smem;
@@ -1633,15 +1982,21 @@ extern "C" int snapx_batch_rk3_steps(const snapx_config* members, const snapx_ba
         SNAPX_HIP(hipEventRecord(ev[0].b, st));
         SNAPX_HIP(hipEventRecord(ev[1].a, st));
       }
-      if (info->col_groups > 0) {
+      if (info->col_groups > 0 || info->col_groups2 > 0) {
         int N = info->vic_size;
-        dim3 grid(std::min(info->col_groups, 4096)), block(pcr_threads(N));
-        if (N == 3)
-          hipLaunchKernelGGL(batch_column_pcr_kernel<3>, grid, block, pcr_smem(3), st, u, u0, du,
-                             table, nm, info->col_groups, s);
-        else
-          hipLaunchKernelGGL(batch_column_pcr_kernel<5>, grid, block, pcr_smem(5), st, u, u0, du,
-                             table, nm, info->col_groups, s);
+        if (info->col_groups > 0) {
+          dim3 grid(std::min(info->col_groups, 4096)), block(pcr_threads(N));
+          if (N == 3)
+            hipLaunchKernelGGL(batch_column_pcr_kernel<3>, grid, block, pcr_smem(3), st, u, u0,
+                               du, table, nm, info->col_groups, s);
+          else
+            hipLaunchKernelGGL(batch_column_pcr_kernel<5>, grid, block, pcr_smem(5), st, u, u0,
+                               du, table, nm, info->col_groups, s);
+        }
+        if (info->col_groups2 > 0)
+          hipLaunchKernelGGL(batch_column_pcr2_kernel, dim3(std::min(info->col_groups2, 4096)),
+                             dim3(pcr_threads(3)), pcr_smem(3), st, u, u0, du, table, nm,
+                             info->col_groups2, s);
       } else if (info->vic_size == 3) {
         hipLaunchKernelGGL(batch_column_kernel<3>, dim3(col_grid), dim3(kColThreads), 0, st, u,
                            u0, du, scr, table, nm, info->total_cols, s);
