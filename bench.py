@@ -417,7 +417,7 @@ def main():
                                   4),
         }
         for key, kname, ms in (("rhs", "fused_rhs_march_kernel", r3),
-                               ("column", "fused_column_pcr_kernel<3>", c3ms)):
+                               ("column", "fused_column_pcr2_kernel", c3ms)):
             valu = measured_traffic(kname, "box256x256x128", "valu")
             if valu:
                 extra["box3d_256x256x128_fp32"][f"{key}_valu_issue_frac"] = round(
