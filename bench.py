@@ -16,7 +16,7 @@ Reported:
                launch / its average device time (HIP events on the stream)
   cpu_baseline the FP64 CPU oracle on a bounded sample of the workload
   step_hbm_frac  160 B per cell-step (FP32) x CU/s / 8 TB/s
-Run ``python bench.py`` (N = 1); more than one GPU is not part of this build.
+Run ``python bench.py`` (N = 1); the package runs on one GPU.
 """
 import argparse
 import ctypes
@@ -261,7 +261,7 @@ def main():
                     help="random directions of the finite-difference check of the gradient")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
+        print("bench.py measures one GPU only: the package has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
 

@@ -36,7 +36,7 @@ int interp_code(std::string const& type) {
   if (type == "cp5") return SNAPX_INTERP_CP5;
   if (type == "weno3") return SNAPX_INTERP_WENO3;
   if (type == "weno5") return SNAPX_INTERP_WENO5;
-  TORCH_CHECK(type != "ppm", "Interp: ppm is not implemented in this build");
+  TORCH_CHECK(type != "ppm", "Interp: ppm is not supported (dc, plm, cp3, cp5, weno3, weno5)");
   TORCH_CHECK(false, "Interp: unknown type " + type);
   return -1;
 }
@@ -179,7 +179,7 @@ CoordinateOptions CoordinateOptionsImpl::from_yaml(yaml::Node const& config) {
   if (!node) return op;
   op->type(node["type"].as<std::string>("cartesian"));
   TORCH_CHECK(op->type() == "cartesian",
-              "geometry.type '", op->type(), "' is outside this build (Cartesian only)");
+              "geometry.type '", op->type(), "' is not supported (Cartesian only)");
   if (node["bounds"]) {
     auto b = node["bounds"];
     op->x1min() = b["x1min"].as<double>(0.0);
@@ -243,7 +243,7 @@ MeshBlockOptions MeshBlockOptionsImpl::from_yaml(std::string const& filename, bo
     op->max_redo() = in["max-redo"].as<int>(5);
   }
   TORCH_CHECK(op->integrator() == "rk3",
-              "integration.type '", op->integrator(), "' is outside this build (rk3 only)");
+              "integration.type '", op->integrator(), "' is not supported (rk3 only)");
   auto& bf = op->bfuncs();
   bf.assign(6, "");
   auto ext = config["boundary-condition"]["external"];
@@ -269,7 +269,7 @@ snapx_config make_config(CoordinateOptions const& coord, HydroOptions const& hyd
   c.x3min = coord->x3min(), c.x3max = coord->x3max();
   auto eos = hydro->eos() ? hydro->eos() : EquationOfStateOptionsImpl::create();
   TORCH_CHECK(eos->type() == "ideal-gas", "EquationOfState: type '", eos->type(),
-              "' is outside this build (ideal-gas only)");
+              "' is not supported (ideal-gas only)");
   c.gammad = eos->gammad();
   c.weight = eos->weight();
   c.limiter = eos->limiter();
@@ -508,7 +508,7 @@ double HydroImpl::max_time_step(torch::Tensor w) const {
 torch::Tensor HydroImpl::forward(double dt, torch::Tensor u, Variables const& other) {
   require_device(u, "hydro_u");
   TORCH_CHECK(other.count("hydro_w"), "Hydro::forward: other[\"hydro_w\"] is required");
-  TORCH_CHECK(!other.count("solid"), "internal solid boundaries are outside this build");
+  TORCH_CHECK(!other.count("solid"), "internal solid boundaries are not supported");
   auto w = other.at("hydro_w");
   require_device(w, "hydro_w");
   TORCH_CHECK(u.sizes() == w.sizes() && u.scalar_type() == w.scalar_type(),
