@@ -1,7 +1,7 @@
 """GPU checks of the fused FP32 path (snapx_fused_rk3_step) against the FP64
 oracle.
 
-Bar (stated in the measurement plan): after one RK3 step from identical
+Bar: after one RK3 step from identical
 state, per conserved variable max |fp32 - fp64| / scale <= 1e-5 (scale as in
 test_gpu_plain); after N steps the per-field L2 / Linf relative errors are
 reported and bounded; interior mass is conserved to FP32 roundoff.

@@ -2,7 +2,7 @@
 
 FP64 bar: per conserved variable, max |gpu - oracle| / scale <= 1e-12, where
 scale is max |oracle value| of that variable (momentum: max(rho) * 300 m/s
-when the field is near zero, the parity protocol of the measurement plan).
+when the field is near zero).
 Module-level checks (EOS, reconstruction, Riemann) use 1e-13 relative.
 """
 import numpy as np
