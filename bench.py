@@ -182,6 +182,25 @@ def c5_sensitivity(capi, dev, nsteps=200, every=20, ndir=2):
 # every 4 cycles per SIMD (16 lanes per cycle; FP64 and non-packed FP32 FMA
 # alike: 78.6 TFLOP/s = 256 x 4 x 16 x 2 flop x 2.4 GHz)
 VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4
+# VALU floating-point peak without packed FP32 (disabled in these kernels):
+# FP64 and FP32 FMA alike, 78.6 TFLOP/s
+VALU_FLOP_PEAK = 78.6e12
+
+
+def measured_flops(kernel, workload):
+    """(FP32, FP64) VALU operations per launch from the SQ_INSTS_VALU_* counters
+    in profiles/fused_kernels_pmc.md, or None."""
+    path = os.path.join(ROOT, "profiles", "fused_kernels_pmc.md")
+    key = f"flops {kernel} {workload}:"
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith(key):
+                    w = line[len(key):].split()
+                    return float(w[0]), float(w[2])
+    except OSError:
+        pass
+    return None
 
 
 def measured_traffic(kernel, workload, what="traffic"):
@@ -356,6 +375,11 @@ def main():
     if valu:
         roof["valu_wave_instructions_per_launch"] = valu
         roof["valu_issue_frac"] = round(valu / (avg_ms * 1e-3) / VALU_ISSUE_PEAK, 4)
+    fl = measured_flops(name, "straka512x64")
+    if fl:
+        roof["valu_flops_per_launch"] = {"f32": fl[0], "f64": fl[1]}
+        roof["valu_tflops"] = round((fl[0] + fl[1]) / (avg_ms * 1e-3) / 1e12, 3)
+        roof["valu_flop_frac"] = round((fl[0] + fl[1]) / (avg_ms * 1e-3) / VALU_FLOP_PEAK, 4)
 
     # attainable HBM rate: a 2 GiB device-to-device copy (read + write bytes)
     src = torch.empty(1 << 29, dtype=torch.float32, device=dev).fill_(1.0)
@@ -422,6 +446,12 @@ def main():
             if valu:
                 extra["box3d_256x256x128_fp32"][f"{key}_valu_issue_frac"] = round(
                     valu / (ms * 1e-3) / VALU_ISSUE_PEAK, 4)
+            fl = measured_flops(kname, "box256x256x128")
+            if fl:
+                extra["box3d_256x256x128_fp32"][f"{key}_valu_tflops"] = round(
+                    (fl[0] + fl[1]) / (ms * 1e-3) / 1e12, 3)
+                extra["box3d_256x256x128_fp32"][f"{key}_valu_flop_frac"] = round(
+                    (fl[0] + fl[1]) / (ms * 1e-3) / VALU_FLOP_PEAK, 4)
 
     # ---- batched ensemble (96 members of uneven size), autotuned tiles,
     #      HIP-graph replays ----
