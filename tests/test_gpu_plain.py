@@ -340,3 +340,35 @@ def test_check_state_with_limiter():
     x[0, 0, 6, 7] = -1.0
     capi.check_state(cfg, x, flag)
     assert int(flag.item()) == capi.REDO_DENSITY
+
+
+def test_check_state_in_a_captured_graph():
+    """The positivity check clears its flag with a kernel (no memset), so a
+    captured graph replays it correctly: three replays on changed states, with
+    eager work between them, each against the eager verdict."""
+    oc, _, u = _straka_small(0, nx2=16, nx1=12)
+    cfg = cfg_of(oc)
+    x = torch.from_numpy(u).to(DEV)
+    flag = torch.zeros(1, dtype=torch.int32, device=DEV)
+    eager = torch.zeros(1, dtype=torch.int32, device=DEV)
+    capi.check_state(cfg, x, flag)  # warm up outside the capture
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        with torch.cuda.graph(g, stream=s):
+            capi.check_state(cfg, x, flag)
+    torch.cuda.synchronize()
+    good = x.clone()
+    for edit, expect in (((0, 0, 5, 6), capi.REDO_DENSITY), (None, 0),
+                         ((4, 0, 4, 4), capi.REDO_PRESSURE)):
+        x.copy_(good)
+        if edit is not None:
+            x[edit] = -1.0
+        y = x * 2.0  # eager work between replays
+        del y
+        g.replay()
+        torch.cuda.synchronize()
+        capi.check_state(cfg, x, eager)
+        assert int(flag.item()) == int(eager.item()) == expect, (edit, int(flag.item()))
