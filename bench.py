@@ -266,7 +266,15 @@ def cpu_baseline(steps_cap=400, seconds=10.0):
     n1, t1 = run(1, seconds * 0.5, steps_cap)
     nn, tn = run(cores, seconds, steps_cap)
     O.set_threads(1)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")),
+                         "")
+    except OSError:
+        pass
     return {"value": round(512 * 64 * nn / tn, 1), "unit": "cell-updates/s", "cores": cores,
+            "cpu_model": model, "cpus_online": os.cpu_count(),
             "kind": "port", "value_1thread": round(512 * 64 * n1 / t1, 1),
             "sample": f"FP64 C oracle, Straka 512x64, RK3 steps from the initial state: "
                       f"{nn} steps on {cores} threads ({tn:.1f} s), "
