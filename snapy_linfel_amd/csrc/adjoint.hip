@@ -919,6 +919,51 @@ __global__ void __launch_bounds__(kT)
   }
 }
 
+// recon_adj_kernel for directions whose interpolants are all linear (DC, CP3,
+// CP5) or unscaled WENO5 (served by weno_face_kernel / weno_gather_kernel):
+// lambda_w[c] += sum over the faces f = c - m of the interpolant's constant
+// weights, which come from evaluating it on unit windows (folded by the
+// compiler) instead of on dual numbers per cell.
+__global__ void __launch_bounds__(kT)
+    recon_adj_linear_kernel(const double* __restrict__ lam_wlr, double* __restrict__ lam_w,
+                            AdjArgs a, int d) {
+  const int ncell = a.n1 * a.n2 * a.n3;
+  const int s = axis_stride(a, d), n = axis_len(a, d);
+  const int r = d == 0 ? 0 : 1;
+  const int type = r ? a.recon_type[1] : a.recon_type[0];
+  const int shock = r ? a.recon_shock[1] : a.recon_shock[0];
+  const int type2 = shock ? type : interp2_type(type);
+  const DivMod dn1(a.n1), dn2(a.n2), dnc(ncell);
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < 5 * ncell;
+       t += gridDim.x * blockDim.x) {
+    int q;
+    const int v = dnc.div(t, q);
+    const int tv = v == IDN ? type : type2;
+    if (tv == SNAPX_INTERP_WENO5) continue;
+    int i, j;
+    const int k = dn2.div(dn1.div(q, i), j);
+    const int c = d == 0 ? i : (d == 1 ? j : k);
+    double clo[7], chi[7];
+#pragma unroll
+    for (int m = 0; m < 7; ++m) {
+      double e[7] = {0., 0., 0., 0., 0., 0., 0.};
+      e[m] = 1.;
+      clo[m] = interp_face<double>(tv, 0, 0, e, 1, 3);
+      chi[m] = interp_face<double>(tv, 1, 0, e, 1, 2);
+    }
+    double acc = 0.;
+#pragma unroll
+    for (int m = -3; m <= 3; ++m) {
+      const int f = c - m;  // cell c sits at window position m + 3 of face f
+      if (f < a.ng || f > n - a.ng) continue;
+      const double lr = lam_wlr[(5 + v) * ncell + q - m * s];
+      const double ll = lam_wlr[v * ncell + q - m * s];
+      acc += lr * clo[m + 3] + ll * chi[m + 3];
+    }
+    lam_w[t] += acc;
+  }
+}
+
 // Gradient of the WENO5 lower-face value (weno5_lo, WENO5-JS) with respect
 // to its window f0..f4, in closed form:
 //   v = sum_k w_k p_k,  w_k = a_k / sum a,  a_k = d_k / (b_k + eps)^2,
@@ -1351,9 +1396,22 @@ extern "C" int snapx_adjoint_rk3_step(const snapx_config* cfg, double dt, const 
         hipLaunchKernelGGL(flux_adj_kernel<SNAPX_RIEMANN_ROE>, bc, dim3(kT), 0, st, s.lam_dx,
                            s.lam_wlr, a, d);
       SNAPX_LAUNCH_CHECK();
-      hipLaunchKernelGGL(recon_adj_kernel, bn, dim3(kT), 0, st, w, s.lam_wlr, s.lam_w, a, d);
-      SNAPX_LAUNCH_CHECK();
       int r = d == 0 ? 0 : 1;
+      {
+        // every interpolant of this direction linear or unscaled WENO5?
+        const int ty = cfg->recon_type[r];
+        const int ty2 = cfg->recon_shock[r] ? ty : interp2_type(ty);
+        auto lean = [&](int tv) {
+          return tv == SNAPX_INTERP_DC || tv == SNAPX_INTERP_CP3 || tv == SNAPX_INTERP_CP5 ||
+                 (tv == SNAPX_INTERP_WENO5 && !cfg->recon_scale[r]);
+        };
+        if (lean(ty) && lean(ty2))
+          hipLaunchKernelGGL(recon_adj_linear_kernel, bn, dim3(kT), 0, st, s.lam_wlr, s.lam_w, a,
+                             d);
+        else
+          hipLaunchKernelGGL(recon_adj_kernel, bn, dim3(kT), 0, st, w, s.lam_wlr, s.lam_w, a, d);
+        SNAPX_LAUNCH_CHECK();
+      }
       if (cfg->recon_type[r] == SNAPX_INTERP_WENO5 && !cfg->recon_scale[r]) {
         for (int v = 0; v < 5; ++v) {
           bool weno = cfg->recon_shock[r] || v == IDN;
