@@ -429,7 +429,7 @@ __global__ void __launch_bounds__(PcrShape<N>::kThreads)
 #pragma unroll
     for (int e = 0; e < N; ++e) r[e] /= a.dt;
   }
-  pcr_solve<N>(A, B, C, r, d, sh, blockDim.x, threadIdx.x, L.i, nl, L.act);
+  pcr_solve<N>(A, B, C, r, d, sh, blockDim.x, threadIdx.x, L.i, nl, L.act, a.lanes == 64);
   double rho = L.act ? w[IDN * ncell + L.q] : 0.;
   double xr = L.act ? dux[IDN * ncell + L.q] : 0.;
   double ai = rho * vol;
@@ -484,7 +484,7 @@ __global__ void __launch_bounds__(PcrShape<N>::kThreads)
     vic_row_blocks<N>(abc, L.col, L.i, nl, true, A, B, C);
   }
   double y[N];
-  pcr_solve<N>(A, B, C, ld, y, sh, blockDim.x, threadIdx.x, L.i, nl, L.act);
+  pcr_solve<N>(A, B, C, ld, y, sh, blockDim.x, threadIdx.x, L.i, nl, L.act, a.lanes == 64);
   if (!L.act) return;
 #pragma unroll
   for (int e = 0; e < N; ++e) cdl[N + e] = y[e];
