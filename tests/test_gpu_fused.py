@@ -97,3 +97,23 @@ def test_fused_column_heights(nx1, scheme):
     capi.fused_rk3_step(cfg, dt, uf)
     e = var_errors(host(uf), u_ref, O.interior(oc))
     assert (e < 1e-5).all(), (nx1, scheme, e)
+
+
+def test_fused_hydrostatic_rest_state_drift():
+    """The Straka background at rest (no bubble): the FP32 fused path stays at
+    rest to the FP32 storage rounding of the total state (measured 1.2e-4 m/s
+    after 100 steps; bar 1e-3), the FP64 plain path to roundoff (bar 1e-9)."""
+    oc = O.straka_config(nx2=64, nx1=64, implicit_scheme=1)
+    oc.x2max = 25.6e3 * 64 / 512
+    w0, u0 = O.initialize(oc, O.straka_init(oc, dT=0.0))
+    cfg = cfg_of(oc)
+    I = O.interior(oc)[1:]
+    u32 = to_dev(u0, torch.float32)
+    capi.fused_rk3_steps(cfg, 0.1, 100, u32)
+    u64 = to_dev(u0)
+    w64 = torch.empty_like(u64)
+    for _ in range(100):
+        capi.rk3_step(cfg, 0.1, u64, w64)
+    for u, bar in ((host(u32), 1e-3), (host(u64), 1e-9)):
+        vmax = max(np.abs(u[v][I] / u[0][I]).max() for v in (1, 2, 3))
+        assert vmax < bar, (vmax, bar)
