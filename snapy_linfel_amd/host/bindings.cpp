@@ -2,6 +2,7 @@
 // module (python/csrc/snapy.cpp:32-61, python/csrc/py*.cpp).
 #include <torch/extension.h>
 
+#include "restart.hpp"
 #include "snap_modules.hpp"
 
 namespace py = pybind11;
@@ -89,6 +90,26 @@ PYBIND11_MODULE(_snapy, m) {
       OPT(HydroOptionsImpl, grav) OPT(HydroOptionsImpl, eos) OPT(HydroOptionsImpl, recon1)
       OPT(HydroOptionsImpl, recon23) OPT(HydroOptionsImpl, riemann) OPT(HydroOptionsImpl, icorr);
 
+  py::class_<OutputOptionsImpl, OutputOptions>(m, "OutputOptions")
+      .def(py::init(&OutputOptionsImpl::create))
+      OPT(OutputOptionsImpl, fid) OPT(OutputOptionsImpl, dt) OPT(OutputOptionsImpl, file_type)
+      OPT(OutputOptionsImpl, variables) OPT(OutputOptionsImpl, combine);
+
+  py::class_<OutputType, std::shared_ptr<OutputType>>(m, "OutputType")
+      .def_readonly("options", &OutputType::options)
+      .def_readwrite("file_number", &OutputType::file_number)
+      .def_readwrite("next_time", &OutputType::next_time);
+
+  // restart files (kintera::save_tensors / load_tensors format,
+  // src/input/read_restart_file.cpp, src/output/combine_restart.cpp)
+  m.def("save_tensors", [](std::map<std::string, torch::Tensor> const& vars,
+                           std::string const& filename) { save_tensors(vars, filename); },
+        py::arg("tensor_map"), py::arg("filename"));
+  m.def("load_tensors", &load_tensors, py::arg("filename"));
+  m.def("load_restart", &load_restart, py::arg("path"), py::arg("block_rank") = 0);
+  m.def("make_restart_bundle", &make_restart_bundle, py::arg("bundle_name"),
+        py::arg("file_list"));
+
   py::class_<MeshBlockOptionsImpl, MeshBlockOptions>(m, "MeshBlockOptions")
       .def(py::init(&MeshBlockOptionsImpl::create))
       .def_static("from_yaml", &MeshBlockOptionsImpl::from_yaml, py::arg("filename"),
@@ -97,7 +118,8 @@ PYBIND11_MODULE(_snapy, m) {
       OPT(MeshBlockOptionsImpl, integrator) OPT(MeshBlockOptionsImpl, cfl)
       OPT(MeshBlockOptionsImpl, nlim) OPT(MeshBlockOptionsImpl, tlim)
       OPT(MeshBlockOptionsImpl, max_redo)
-      OPT(MeshBlockOptionsImpl, bfuncs);
+      OPT(MeshBlockOptionsImpl, bfuncs) OPT(MeshBlockOptionsImpl, basename)
+      OPT(MeshBlockOptionsImpl, output_dir) OPT(MeshBlockOptionsImpl, outputs);
 
   py::class_<IdealGasImpl, IdealGas>(m, "IdealGas")
       .def(py::init<EquationOfStateOptions, CoordinateOptions>(), py::arg("options"),
@@ -163,12 +185,33 @@ PYBIND11_MODULE(_snapy, m) {
       .def(py::init<MeshBlockOptions>())
       .def_readonly("options", &MeshBlockImpl::options)
       .def_readonly("phydro", &MeshBlockImpl::phydro)
+      // (vars, start time), as python/csrc/pymesh.cpp:189-203; the dict is
+      // also filled in place
       .def("initialize", [](MeshBlockImpl& b, py::dict vars) {
              Variables v;
              for (auto item : vars) v[item.first.cast<std::string>()] = item.second.cast<torch::Tensor>();
-             b.initialize(v);
+             double t = b.initialize(v);
              for (auto& kv : v) vars[py::str(kv.first)] = kv.second;
-           })
+             return py::make_tuple(vars, t);
+           }, py::arg("vars"))
+      .def("initialize_from_restart", [](MeshBlockImpl& b, std::string const& restart_file) {
+             Variables v;
+             double t = b.initialize(v, restart_file.c_str());
+             py::dict vars;
+             for (auto& kv : v) vars[py::str(kv.first)] = kv.second;
+             return py::make_tuple(vars, t);
+           }, py::arg("restart_file"))
+      .def("make_outputs", [](MeshBlockImpl& b, py::dict vars, double time, bool final_write) {
+             Variables v;
+             for (auto item : vars) v[item.first.cast<std::string>()] = item.second.cast<torch::Tensor>();
+             b.make_outputs(v, time, final_write);
+           }, py::arg("vars"), py::arg("time"), py::arg("final_write") = false)
+      .def("finalize", [](MeshBlockImpl& b, py::dict vars, double time) {
+             Variables v;
+             for (auto item : vars) v[item.first.cast<std::string>()] = item.second.cast<torch::Tensor>();
+             b.finalize(v, time);
+           }, py::arg("vars"), py::arg("time"))
+      .def("get_outputs", [](MeshBlockImpl& b) { return b.output_types; })
       .def("max_time_step", [](MeshBlockImpl& b, py::dict vars) {
              Variables v;
              for (auto item : vars) v[item.first.cast<std::string>()] = item.second.cast<torch::Tensor>();

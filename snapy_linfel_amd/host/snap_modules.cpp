@@ -3,7 +3,10 @@
 #include <c10/hip/HIPStream.h>
 
 #include <cmath>
+#include <filesystem>
 #include <iostream>
+
+#include "restart.hpp"
 
 namespace snap {
 
@@ -229,9 +232,30 @@ HydroOptions HydroOptionsImpl::from_yaml(std::string const& filename, bool verbo
   return op;
 }
 
+OutputOptions OutputOptionsImpl::from_yaml(yaml::Node const& node, int fid) {
+  auto op = create();
+  op->fid() = fid;
+  op->dt() = node["dt"].as<double>(0.);
+  TORCH_CHECK(node["type"].defined(), "OutputOptions::from_yaml: output file type must be specified");
+  op->file_type() = node["type"].as<std::string>();
+  auto vars = node["variables"];
+  if (vars.defined())
+    for (size_t i = 0; i < vars.size(); ++i) op->variables().push_back(vars[i].as<std::string>());
+  op->combine() = node["combine"].as<bool>(true);
+  return op;
+}
+
 MeshBlockOptions MeshBlockOptionsImpl::from_yaml(std::string const& filename, bool verbose) {
   auto config = yaml::LoadFile(filename);
   auto op = create();
+  // the YAML file's name without directory and extension
+  // (src/mesh/meshblock_options.cpp:19-25)
+  std::string stem = std::filesystem::path(filename).filename().string();
+  op->basename() = stem.substr(0, stem.find_last_of('.'));
+  auto outs = config["outputs"];
+  if (outs.defined())
+    for (size_t i = 0; i < outs.size(); ++i)
+      op->outputs().push_back(OutputOptionsImpl::from_yaml(outs[i], (int)i));
   op->coord() = CoordinateOptionsImpl::from_yaml(config);
   op->hydro() = HydroOptionsImpl::from_yaml(filename, verbose);
   if (config["integration"]) {
@@ -531,6 +555,19 @@ torch::Tensor HydroImpl::forward(double dt, torch::Tensor u, Variables const& ot
 MeshBlockImpl::MeshBlockImpl(MeshBlockOptions options_) : options(std::move(options_)) {
   phydro = std::make_shared<HydroImpl>(options->hydro(), options->coord(), options->bfuncs());
   cfg_ = phydro->config();
+  // output types as in src/mesh/meshblock.cpp:128-175
+  for (auto const& out_op : options->outputs()) {
+    auto const& type = out_op->file_type();
+    if (type == "netcdf" || type == "pnetcdf") {
+      TORCH_WARN("MeshBlock: output type '", type, "' keeps its schedule but is not written ",
+                 "(this library writes restart files only)");
+    } else {
+      TORCH_CHECK(type == "restart", "Output type '", type, "' is not implemented.");
+    }
+    auto ot = std::make_shared<OutputType>();
+    ot->options = out_op;
+    output_types.push_back(ot);
+  }
 }
 
 void MeshBlockImpl::apply_bc(torch::Tensor var, int var_type) {
@@ -540,7 +577,8 @@ void MeshBlockImpl::apply_bc(torch::Tensor var, int var_type) {
         "apply_bc");
 }
 
-void MeshBlockImpl::initialize(Variables& vars) {
+double MeshBlockImpl::initialize(Variables& vars, char const* restart_file) {
+  if (restart_file != nullptr) return init_from_restart(vars, restart_file);
   TORCH_CHECK(vars.count("hydro_w"), "initialize: vars[\"hydro_w\"] is required");
   auto w = vars.at("hydro_w");
   apply_bc(w, kPrimitive);
@@ -550,6 +588,78 @@ void MeshBlockImpl::initialize(Variables& vars) {
         "prim2cons");
   apply_bc(u, kConserved);
   vars["hydro_u"] = u;
+  return 0.;
+}
+
+double MeshBlockImpl::init_from_restart(Variables& vars, std::string const& fname) {
+  std::filesystem::path path(fname);
+  if (!path.is_absolute() && !std::filesystem::exists(path))
+    path = std::filesystem::path(options->output_dir()) / fname;
+  auto data = load_restart(path.string(), /*block_rank=*/0);
+  for (const char* key : {"hydro_u", "last_time", "last_cycle", "file_number", "next_time"})
+    TORCH_CHECK(data.count(key), "Restart file is missing required variable: ", key);
+  // resume at the saved cycle, so that nlim-limited runs take no extra step
+  cycle = (int)data.at("last_cycle").item<int64_t>();
+  const double time = data.at("last_time").item<double>();
+  auto const& numbers = data.at("file_number");
+  auto const& times = data.at("next_time");
+  const int saved = std::min<int>((int)output_types.size(), (int)numbers.size(0));
+  for (int n = 0; n < saved; ++n) {
+    output_types[n]->file_number = (int)numbers[n].item<int64_t>();
+    output_types[n]->next_time = times[n].item<double>();
+  }
+  // outputs added after the restart file was written start now
+  for (size_t n = saved; n < output_types.size(); ++n) output_types[n]->next_time = time;
+  for (auto& [name, t] : data) {
+    if (name == "last_time" || name == "last_cycle" || name == "file_number" || name == "next_time")
+      continue;
+    vars[name] = t.to(torch::kCUDA);
+  }
+  return time;
+}
+
+void MeshBlockImpl::write_restart(OutputType const& out, Variables const& vars, double time,
+                                  bool final_write) {
+  Variables out_vars;
+  const bool skip_scalar_r = vars.count("scalar_s") && vars.count("hydro_u");
+  for (auto const& [name, t] : vars) {
+    if (skip_scalar_r && name == "scalar_r") continue;
+    if (t.defined()) out_vars[name] = t.to(torch::kCPU);
+  }
+  out_vars["last_time"] = torch::tensor({time}, torch::kFloat64);
+  out_vars["last_cycle"] = torch::tensor({(int64_t)cycle}, torch::kInt64);
+  std::vector<int64_t> numbers;
+  std::vector<double> times;
+  for (auto const& ot : output_types) {
+    numbers.push_back(ot->file_number);
+    times.push_back(ot->next_time);
+  }
+  out_vars["file_number"] = torch::tensor(numbers, torch::kInt64);
+  out_vars["next_time"] = torch::tensor(times, torch::kFloat64);
+
+  auto const& dir = options->output_dir();
+  std::error_code ec;
+  std::filesystem::create_directories(dir, ec);
+  TORCH_CHECK(!ec, "Failed to create output directory '", dir, "': ", ec.message());
+  save_tensors(out_vars, restart_part_name(dir, options->basename(), /*rank=*/0,
+                                           out.file_number, final_write));
+  combine_restart_parts(dir, options->basename(), out.file_number, final_write);
+}
+
+void MeshBlockImpl::make_outputs(Variables const& vars, double current_time, bool final_write) {
+  for (auto& ot : output_types) {
+    const bool due = final_write || current_time >= ot->next_time;
+    if (!due) continue;
+    if (ot->options->file_type() == "restart") write_restart(*ot, vars, current_time, final_write);
+    if (!final_write) {
+      ot->next_time += ot->options->dt();
+      ot->file_number += 1;
+    }
+  }
+}
+
+void MeshBlockImpl::finalize(Variables const& vars, double time) {
+  make_outputs(vars, time, /*final_write=*/true);
 }
 
 double MeshBlockImpl::max_time_step(Variables const& vars) {

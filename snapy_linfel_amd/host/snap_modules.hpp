@@ -183,8 +183,32 @@ struct HydroOptionsImpl {
 };
 using HydroOptions = std::shared_ptr<HydroOptionsImpl>;
 
+// One entry of the "outputs" list (src/output/output_type.cpp:32-83).  This
+// library writes the "restart" type; "netcdf" and "pnetcdf" entries keep their
+// schedule (file numbers and next times, which restart files record) but
+// write nothing.
+struct OutputOptionsImpl {
+  static std::shared_ptr<OutputOptionsImpl> create() {
+    return std::make_shared<OutputOptionsImpl>();
+  }
+  static std::shared_ptr<OutputOptionsImpl> from_yaml(yaml::Node const& node, int fid);
+  SNAP_ARG(int, fid) = 0;
+  SNAP_ARG(double, dt) = 0.;
+  SNAP_ARG(std::string, file_type) = "";
+  SNAP_ARG(std::vector<std::string>, variables);
+  SNAP_ARG(bool, combine) = true;
+};
+using OutputOptions = std::shared_ptr<OutputOptionsImpl>;
+
+// the schedule of one output (src/output/output_type.hpp:100-112)
+struct OutputType {
+  OutputOptions options;
+  int file_number = 0;
+  double next_time = 0.;
+};
+
 // MeshBlock options of the hot path (src/mesh/meshblock_options.cpp:12-195):
-// geometry, hydro, integration and external boundary functions.
+// geometry, hydro, integration, outputs and external boundary functions.
 struct MeshBlockOptionsImpl {
   static std::shared_ptr<MeshBlockOptionsImpl> create() {
     return std::make_shared<MeshBlockOptionsImpl>();
@@ -202,6 +226,11 @@ struct MeshBlockOptionsImpl {
   SNAP_ARG(int, max_redo) = 5;
   // x1-inner, x1-outer, x2-inner, x2-outer, x3-inner, x3-outer ("" = none)
   SNAP_ARG(std::vector<std::string>, bfuncs);
+  // file names of outputs: <output_dir>/<basename>.… (basename = the YAML
+  // file's name without its extension)
+  SNAP_ARG(std::string, basename) = "";
+  SNAP_ARG(std::string, output_dir) = ".";
+  SNAP_ARG(std::vector<OutputOptions>, outputs);
 };
 using MeshBlockOptions = std::shared_ptr<MeshBlockOptionsImpl>;
 
@@ -329,8 +358,17 @@ class MeshBlockImpl {
   MeshBlockOptions options;
   Hydro phydro;
   int nstages() const { return 3; }
-  // hydro_w (primitive) -> BCs, hydro_u = W->U, BCs (meshblock.cpp:412-510)
-  void initialize(Variables& vars);
+  // hydro_w (primitive) -> BCs, hydro_u = W->U, BCs (meshblock.cpp:412-510);
+  // returns the start time 0.  With restart_file, the variables, cycle and
+  // output schedule come from that file instead (meshblock.cpp:1030-1094) and
+  // the file's time is returned.
+  double initialize(Variables& vars, char const* restart_file = nullptr);
+  // meshblock.cpp:813-830: every output whose next time has come (all of them
+  // when final_write) is written; its next time advances by its dt
+  void make_outputs(Variables const& vars, double current_time, bool final_write = false);
+  // the final outputs (meshblock.cpp:909-911)
+  void finalize(Variables const& vars, double time);
+  std::vector<std::shared_ptr<OutputType>> output_types;
   double max_time_step(Variables const& vars);
   // one RK stage on vars["hydro_u"] (meshblock.cpp:577-786)
   void forward(Variables& vars, double dt, int stage);
@@ -345,6 +383,10 @@ class MeshBlockImpl {
   int cycle = 0;
 
  private:
+  double init_from_restart(Variables& vars, std::string const& fname);
+  // RestartOutput::write_output_file (src/output/restart.cpp:25-86)
+  void write_restart(OutputType const& out, Variables const& vars, double time,
+                     bool final_write);
   snapx_config cfg_;
   torch::Tensor u0_, u1_, fused_ws_, flag_;
 };

@@ -124,3 +124,72 @@ def test_meshblock_check_redo_restores_and_halves_dt():
         vars_["hydro_u"].copy_(u_in)
         mb.step(vars_, 0.1, fused=fused)
         assert mb.check_redo(vars_) == 0 and mb.current_redo == 0
+
+
+def _run_to(mb, vars_, time, nlim, dt_fixed=None):
+    """The time loop of examples/straka.cpp:138-162 (restart outputs written by
+    make_outputs); returns the final time and (cycle, time, dt) per cycle."""
+    log = []
+    cycle = mb.cycle
+    while cycle < nlim:
+        cycle += 1
+        mb.cycle = cycle
+        dt = dt_fixed if dt_fixed is not None else mb.max_time_step(vars_)
+        log.append((cycle, time, dt))
+        mb.step(vars_, dt, fused=dt_fixed is not None)
+        redo = mb.check_redo(vars_)
+        if redo > 0:
+            cycle = mb.cycle
+            continue
+        assert redo == 0
+        time += dt
+        mb.make_outputs(vars_, time)
+    mb.finalize(vars_, time)
+    return time, log
+
+
+@pytest.mark.parametrize("fused", [False, True], ids=["fp64_cfl_dt", "fp32_fused_fixed_dt"])
+def test_restart_resumes_the_uninterrupted_run(tmp_path, fused):
+    """tests/run_restart_cycle_limit.py: a run restarted from a restart file
+    written mid-way resumes at the cycle and time of the uninterrupted run,
+    takes the same steps (cycle, time and dt equal) and ends at the same
+    cycle, with a bit-identical state."""
+    def block(sub):
+        op = host.MeshBlockOptions.from_yaml(os.path.join(G, "straka.yaml"))
+        out = host.OutputOptions()
+        out.file_type, out.dt = "restart", 0.35
+        op.outputs = [out]
+        op.output_dir = str(tmp_path / sub)
+        return host.MeshBlock(op)
+
+    oc = _straka_oracle()
+    w0 = torch.from_numpy(O.straka_init(oc)).to(DEV)
+    dt_fixed = 0.1 if fused else None
+    nlim = 9
+
+    base = block("base")
+    vars_ = {"hydro_w": w0.clone()}
+    _, t0 = base.initialize(vars_)
+    if fused:
+        vars_ = {k: v.float() for k, v in vars_.items()}
+    base.make_outputs(vars_, t0)
+    t_end, base_log = _run_to(base, vars_, t0, nlim, dt_fixed)
+    assert base.cycle == nlim
+    files = sorted(os.listdir(tmp_path / "base"))
+    assert files[0] == "straka.00000.restart" and files[-1] == "straka.final.restart"
+    assert "straka.00002.restart" in files
+
+    again = block("restart")
+    rvars, t_r = again.initialize_from_restart(str(tmp_path / "base" / "straka.00002.restart"))
+    assert rvars["hydro_u"].device.type == "cuda"
+    assert rvars["hydro_u"].dtype == (torch.float32 if fused else torch.float64)
+    start = again.cycle
+    again.make_outputs(rvars, t_r)
+    t_end2, log2 = _run_to(again, rvars, t_r, nlim, dt_fixed)
+    matched = [e for e in base_log if e[1] == t_r]
+    assert len(matched) == 1 and matched[0][0] == start + 1
+    assert log2 == base_log[start:]
+    assert t_end2 == t_end and again.cycle == base.cycle == nlim
+    assert torch.equal(rvars["hydro_u"], vars_["hydro_u"])
+    final = host.load_restart(str(tmp_path / "base" / "straka.final.restart"))
+    assert torch.equal(final["hydro_u"], vars_["hydro_u"].cpu())
