@@ -22,8 +22,13 @@
  *    workspace pointer and its size; *_workspace_bytes() returns the size.
  *  - Entry points are reentrant: there is no global mutable state apart from
  *    the thread-local error string.
- *  - Geometry is uniform Cartesian with face widths (x?max - x?min)/nx?
- *    (src/coord/cartesian.cpp:8-47).
+ *  - Geometry (snapx_config.coord) is Cartesian with uniform face widths
+ *    (x?max - x?min)/nx? (src/coord/cartesian.cpp:8-47), or spherical-polar
+ *    (r, theta, phi) on the plain FP64/FP32 path: faces uniform in each
+ *    coordinate as torch::linspace places them (src/coord/coordinate.cpp:
+ *    201-221), with the centres, face areas, volumes and source terms of
+ *    src/coord/spherical_polar.cpp.  The fused path, batches and the adjoint
+ *    are Cartesian only.
  */
 #ifndef SNAPX_H
 #define SNAPX_H
@@ -66,6 +71,10 @@ extern "C" {
 #define SNAPX_BC_OUTFLOW 1
 #define SNAPX_BC_PERIODIC 2
 
+/* coordinate systems (src/coord/coordinate.cpp:437-452) */
+#define SNAPX_COORD_CARTESIAN 0
+#define SNAPX_COORD_SPHERICAL_POLAR 1
+
 /* variable kinds for boundary functions (src/snap.h:62-71) */
 #define SNAPX_PRIMITIVE 0
 #define SNAPX_CONSERVED 1
@@ -92,6 +101,8 @@ typedef struct snapx_config {
   int disable_flux[3];
   /* x1-inner, x1-outer, x2-inner, x2-outer, x3-inner, x3-outer */
   int bc[6];
+  /* geometry.type: SNAPX_COORD_*; spherical-polar needs x2 in [0, pi] */
+  int coord;
 } snapx_config;
 
 const char* snapx_last_error(void);
@@ -140,6 +151,16 @@ int snapx_riemann(int solver, const void* wl, const void* wr, void* flx,
    axis.  f1/f2/f3 may be NULL for inactive or disabled axes. */
 int snapx_divergence(const snapx_config* cfg, const void* f1, const void* f2,
                      const void* f3, void* div, int dtype, void* stream);
+
+/* CoordinateImpl::forward (src/coord/coordinate.cpp:428-434) and
+   SphericalPolarImpl::forward (src/coord/spherical_polar.cpp:190-279): the
+   flux divergence plus, in spherical-polar geometry, the geometric source
+   terms of the momenta, using the primitives w and the x1 face pressure fp1
+   ([nc3][nc2][nc1], may be NULL: then 2 p / r stands in).  div is written
+   on every cell, ghosts included, as the reference's module returns it. */
+int snapx_coord_forward(const snapx_config* cfg, const void* w, const void* f1,
+                        const void* f2, const void* f3, const void* fp1,
+                        void* div, int dtype, void* stream);
 
 /* HydroImpl::_revise_x1inner_ghost / _revise_x1outer_ghost
    (src/hydro/hydro.cpp:231-269): isentropic hydrostatic refill of the x1

@@ -23,6 +23,7 @@ ILT, IRT = 0, 1
 INTERP = {"dc": 0, "plm": 1, "cp3": 2, "cp5": 3, "weno3": 4, "weno5": 5}
 RIEMANN = {"roe": 0, "lmars": 1, "hllc": 2}
 BC = {None: -1, "none": -1, "reflecting": 0, "outflow": 1, "periodic": 2}
+COORD = {"cartesian": 0, "spherical-polar": 1}
 PRIMITIVE, CONSERVED, SCALAR = 0, 1, 2
 
 
@@ -44,6 +45,7 @@ class OcConfig(C.Structure):
         ("implicit_scheme", C.c_int),
         ("disable_flux", C.c_int * 3),
         ("bc", C.c_int * 6),
+        ("coord", C.c_int),
     ]
 
 
@@ -78,6 +80,8 @@ def lib():
         L.oc_reconstruct.argtypes = [P, P, i, i, i, i, i, i, i, i, i, i, d, d]
         L.oc_riemann.argtypes = [i, P, P, P, P, i, i, i, i, i, d]
         L.oc_divergence.argtypes = [P, P, P, P, i, i, i, i, i, P, P, P]
+        L.oc_coord_forward.argtypes = [cfgp, P, P, P, P, P, P]
+        L.oc_geometry.argtypes = [cfgp, i, P]
         L.oc_revise_x1_ghost.argtypes = [P, i, i, i, i, i, d, d, P]
         L.oc_revise_x1_lr.argtypes = [P, i, i, i, i, i]
         L.oc_implicit_correction.argtypes = [cfgp, d, P, P]
@@ -149,6 +153,7 @@ def make_config(**kw) -> OcConfig:
     active = [c.nx1 > 1, c.nx1 > 1, c.nx2 > 1, c.nx2 > 1, c.nx3 > 1, c.nx3 > 1]
     for d in range(6):
         c.bc[d] = BC[bcs[d]] if active[d] else -1
+    c.coord = COORD[kw.get("coord", "cartesian")]
     return c
 
 
@@ -251,6 +256,36 @@ def divergence(c, f1, f2, f3):
     lib().oc_divergence(_p(f1), _p(f2), _p(f3), _p(div), 5, n3, n2, n1, c.nghost,
                         _p(d1), _p(d2), _p(d3))
     return div
+
+
+def coord_forward(c, w, f1, f2, f3, fp1=None):
+    """CoordinateImpl::forward: divergence plus, on spherical-polar blocks,
+    the geometric momentum sources (every cell, ghosts included)."""
+    for f in (w, f1, f2, f3):
+        if f is not None:
+            _check_field(c, f, 5)
+    n3, n2, n1 = shape(c)
+    if fp1 is not None:
+        fp1 = np.ascontiguousarray(fp1, dtype=np.float64)
+        assert fp1.shape == (n3, n2, n1)
+    div = np.zeros((5, n3, n2, n1))
+    lib().oc_coord_forward(C.byref(c), _p(w), _p(f1), _p(f2), _p(f3), _p(fp1), _p(div))
+    return div
+
+
+_GEOM = {"x1v": 0, "x2v": 1, "dx1v": 2, "vol": 3, "area1": 4, "area2": 5, "area3": 6,
+         "width1": 7, "width2": 8, "width3": 9}
+
+
+def geometry(c, what):
+    """Coordinate arrays of the block: x1v, x2v, dx1v (1-D), vol, area1/2/3
+    (face arrays, one extra face along their axis) and the CFL widths."""
+    n3, n2, n1 = shape(c)
+    shp = {"x1v": (n1,), "x2v": (n2,), "dx1v": (n1,), "area1": (n3, n2, n1 + 1),
+           "area2": (n3, n2 + 1, n1), "area3": (n3 + 1, n2, n1)}.get(what, (n3, n2, n1))
+    out = np.zeros(shp)
+    lib().oc_geometry(C.byref(c), _GEOM[what], _p(out))
+    return out
 
 
 def hydro_forward(c, dt, u, w=None):

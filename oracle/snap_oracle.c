@@ -864,6 +864,249 @@ void oc_divergence(const double* f1, const double* f2, const double* f3,
 }
 
 /* ------------------------------------------------------------------------ */
+/* coordinate geometry (src/coord/coordinate.cpp:201-221, 300-331;            */
+/* src/coord/spherical_polar.cpp:15-230)                                      */
+/* ------------------------------------------------------------------------ */
+
+/* torch::linspace(lo, hi, n) in double on the CPU: the first half of the
+   points counts up from lo, the rest down from hi, each with one rounding */
+static void linspace(double lo, double hi, int n, double* x) {
+  double step = (hi - lo) / (n - 1);
+  for (int i = 0; i < n; ++i)
+    x[i] = i < n / 2 ? fma(step, (double)i, lo) : fma(-step, (double)(n - 1 - i), hi);
+}
+
+/* CoordinateImpl constructor: faces of an axis including its ghost cells */
+static void axis_faces(double xmin, double xmax, int nx, int ng, double* xf) {
+  double dx = (xmax - xmin) / nx;
+  double lo = nx > 1 ? xmin - ng * dx : xmin, hi = nx > 1 ? xmax + ng * dx : xmax;
+  linspace(lo, hi, (nx > 1 ? nx + 2 * ng : 1) + 1, xf);
+}
+
+typedef struct {
+  int coord, n1, n2, n3;
+  double *x1f, *x2f, *x3f;    /* faces (n + 1) */
+  double *dx1f, *dx2f, *dx3f; /* widths (n) */
+  double *x1v, *x2v, *dx1v;   /* centres and x1 centre distances (n) */
+  /* full arrays as the reference's broadcasts produce them */
+  double *area1; /* [n3][n2][n1 + 1] */
+  double *area2; /* [n3][n2 + 1][n1] */
+  double *area3; /* [n3 + 1][n2][n1] */
+  double *vol;   /* [n3][n2][n1] */
+  double *src1_i, *src2_i, *src1_j, *src2_j; /* coord_src*_i (n1), *_j (n2) */
+} oc_geom;
+
+static void geom_free(oc_geom* g) {
+  double* p[] = {g->x1f, g->x2f, g->x3f, g->dx1f, g->dx2f, g->dx3f, g->x1v, g->x2v, g->dx1v,
+                 g->area1, g->area2, g->area3, g->vol, g->src1_i, g->src2_i, g->src1_j,
+                 g->src2_j};
+  for (size_t i = 0; i < sizeof(p) / sizeof(p[0]); ++i) free(p[i]);
+}
+
+static void geom_init(const oc_config* c, oc_geom* g) {
+  int n1 = oc_nc1(c), n2 = oc_nc2(c), n3 = oc_nc3(c);
+  g->coord = c->coord, g->n1 = n1, g->n2 = n2, g->n3 = n3;
+  g->x1f = malloc(sizeof(double) * (n1 + 1));
+  g->x2f = malloc(sizeof(double) * (n2 + 1));
+  g->x3f = malloc(sizeof(double) * (n3 + 1));
+  g->dx1f = malloc(sizeof(double) * n1);
+  g->dx2f = malloc(sizeof(double) * n2);
+  g->dx3f = malloc(sizeof(double) * n3);
+  g->x1v = malloc(sizeof(double) * n1);
+  g->x2v = malloc(sizeof(double) * n2);
+  g->dx1v = malloc(sizeof(double) * n1);
+  g->area1 = malloc(sizeof(double) * n3 * n2 * (n1 + 1));
+  g->area2 = malloc(sizeof(double) * n3 * (n2 + 1) * n1);
+  g->area3 = malloc(sizeof(double) * (n3 + 1) * n2 * n1);
+  g->vol = malloc(sizeof(double) * n3 * n2 * n1);
+  g->src1_i = malloc(sizeof(double) * n1);
+  g->src2_i = malloc(sizeof(double) * n1);
+  g->src1_j = malloc(sizeof(double) * n2);
+  g->src2_j = malloc(sizeof(double) * n2);
+  axis_faces(c->x1min, c->x1max, c->nx1, c->nghost, g->x1f);
+  axis_faces(c->x2min, c->x2max, c->nx2, c->nghost, g->x2f);
+  axis_faces(c->x3min, c->x3max, c->nx3, c->nghost, g->x3f);
+  if (c->coord != OC_COORD_SPHERICAL_POLAR) {
+    /* Cartesian (src/coord/cartesian.cpp:8-47): uniform widths, midpoints */
+    oc_cartesian_widths(c, g->dx1f, g->dx2f, g->dx3f);
+    for (int i = 0; i < n1; ++i) g->x1v[i] = 0.5 * (g->x1f[i] + g->x1f[i + 1]);
+    for (int j = 0; j < n2; ++j) g->x2v[j] = 0.5 * (g->x2f[j] + g->x2f[j + 1]);
+    for (int i = 0; i < n1; ++i) g->dx1v[i] = g->dx1f[i];
+    for (int k = 0; k < n3; ++k)
+      for (int j = 0; j < n2; ++j) {
+        for (int i = 0; i <= n1; ++i)
+          g->area1[((long)k * n2 + j) * (n1 + 1) + i] = g->dx3f[k] * g->dx2f[j];
+        for (int i = 0; i < n1; ++i)
+          g->vol[((long)k * n2 + j) * n1 + i] = g->dx3f[k] * (g->dx2f[j] * g->dx1f[i]);
+      }
+    for (int k = 0; k < n3; ++k)
+      for (int j = 0; j <= n2; ++j)
+        for (int i = 0; i < n1; ++i)
+          g->area2[((long)k * (n2 + 1) + j) * n1 + i] = g->dx3f[k] * g->dx1f[i];
+    for (int k = 0; k <= n3; ++k)
+      for (int j = 0; j < n2; ++j)
+        for (int i = 0; i < n1; ++i)
+          g->area3[((long)k * n2 + j) * n1 + i] = g->dx2f[j] * g->dx1f[i];
+    for (int i = 0; i < n1; ++i) g->src1_i[i] = g->src2_i[i] = 0.;
+    for (int j = 0; j < n2; ++j) g->src1_j[j] = g->src2_j[j] = 0.;
+    return;
+  }
+  /* SphericalPolarImpl::reset (src/coord/spherical_polar.cpp:42-120) */
+  for (int i = 0; i < n1; ++i) g->dx1f[i] = g->x1f[i + 1] - g->x1f[i];
+  for (int j = 0; j < n2; ++j) g->dx2f[j] = g->x2f[j + 1] - g->x2f[j];
+  for (int k = 0; k < n3; ++k) g->dx3f[k] = g->x3f[k + 1] - g->x3f[k];
+  for (int i = 0; i < n1; ++i) { /* radial_centers (:17-21) */
+    double rm = g->x1f[i], rp = g->x1f[i + 1];
+    g->x1v[i] = 0.75 * (pow(rp, 4) - pow(rm, 4)) / (pow(rp, 3) - pow(rm, 3));
+  }
+  for (int j = 0; j < n2; ++j) { /* polar_centers (:23-28), midpoint if nc2 == 1 */
+    double tm = g->x2f[j], tp = g->x2f[j + 1];
+    g->x2v[j] = n2 == 1 ? 0.5 * (tm + tp)
+                        : ((sin(tp) - tp * cos(tp)) - (sin(tm) - tm * cos(tm))) /
+                              (cos(tm) - cos(tp));
+  }
+  g->dx1v[0] = g->dx1f[0]; /* center_distances (:30-38) */
+  for (int i = 1; i < n1; ++i) g->dx1v[i] = g->x1v[i] - g->x1v[i - 1];
+  for (int i = 0; i < n1; ++i) {
+    double rm = g->x1f[i], rp = g->x1f[i + 1];
+    double radial_volume = (pow(rp, 3) - pow(rm, 3)) / 3.0;
+    double radial_area23 = 0.5 * (rp * rp - rm * rm);
+    g->src1_i[i] = radial_area23 / radial_volume;
+    g->src2_i[i] = g->dx1f[i] / ((rm + rp) * radial_volume);
+  }
+  for (int j = 0; j < n2; ++j) {
+    double sm = fabs(sin(g->x2f[j])), sp = fabs(sin(g->x2f[j + 1]));
+    double polar_volume = fabs(cos(g->x2f[j]) - cos(g->x2f[j + 1]));
+    g->src1_j[j] = (sp - sm) / polar_volume;
+    g->src2_j[j] = (sp - sm) / ((sm + sp) * polar_volume);
+  }
+  for (int k = 0; k < n3; ++k)
+    for (int j = 0; j < n2; ++j) {
+      double polar = fabs(cos(g->x2f[j]) - cos(g->x2f[j + 1]));
+      for (int i = 0; i <= n1; ++i) /* face_area1 (:176-181) */
+        g->area1[((long)k * n2 + j) * (n1 + 1) + i] =
+            g->x1f[i] * g->x1f[i] * polar * g->dx3f[k];
+      for (int i = 0; i < n1; ++i) /* cell_volume (:198-210) */
+        g->vol[((long)k * n2 + j) * n1 + i] =
+            (pow(g->x1f[i + 1], 3) - pow(g->x1f[i], 3)) / 3.0 * polar * g->dx3f[k];
+    }
+  for (int k = 0; k < n3; ++k)
+    for (int j = 0; j <= n2; ++j)
+      for (int i = 0; i < n1; ++i) /* face_area2 (:183-189) */
+        g->area2[((long)k * (n2 + 1) + j) * n1 + i] =
+            0.5 * (g->x1f[i + 1] * g->x1f[i + 1] - g->x1f[i] * g->x1f[i]) *
+            fabs(sin(g->x2f[j])) * g->dx3f[k];
+  for (int k = 0; k <= n3; ++k)
+    for (int j = 0; j < n2; ++j)
+      for (int i = 0; i < n1; ++i) /* face_area3 (:191-196) */
+        g->area3[((long)k * n2 + j) * n1 + i] =
+            0.5 * (g->x1f[i + 1] * g->x1f[i + 1] - g->x1f[i] * g->x1f[i]) * g->dx2f[j];
+}
+
+#define A1(k, j, i) g.area1[((long)(k) * n2 + (j)) * (n1 + 1) + (i)]
+#define A2(k, j, i) g.area2[((long)(k) * (n2 + 1) + (j)) * n1 + (i)]
+#define A3(k, j, i) g.area3[((long)(k) * n2 + (j)) * n1 + (i)]
+
+/* CoordinateImpl::forward (src/coord/coordinate.cpp:352-434) and
+   SphericalPolarImpl::forward (src/coord/spherical_polar.cpp:212-279).
+   div gets every cell, as the reference's tensors do; fp1 may be NULL. */
+void oc_coord_forward(const oc_config* c, const double* w, const double* f1,
+                      const double* f2, const double* f3, const double* fp1,
+                      double* div) {
+  oc_geom g;
+  geom_init(c, &g);
+  int n1 = g.n1, n2 = g.n2, n3 = g.n3, ng = c->nghost;
+  long ncell = (long)n3 * n2 * n1;
+  int si = n1 > 1 ? ng : 0, ei = n1 > 1 ? n1 - ng : 1;
+  int sj = n2 > 1 ? ng : 0, ej = n2 > 1 ? n2 - ng : 1;
+  int sk = n3 > 1 ? ng : 0, ek = n3 > 1 ? n3 - ng : 1;
+  /* divergence (:352-396): each axis on its own interior range */
+  for (int v = 0; v < 5; ++v)
+    for (int k = 0; k < n3; ++k)
+      for (int j = 0; j < n2; ++j)
+        for (int i = 0; i < n1; ++i) {
+          long q = ((long)k * n2 + j) * n1 + i, o = v * ncell + q;
+          double d = 0.;
+          if (f1 && i >= si && i < ei) d += A1(k, j, i + 1) * f1[o + 1] - A1(k, j, i) * f1[o];
+          if (f2 && j >= sj && j < ej) d += A2(k, j + 1, i) * f2[o + n1] - A2(k, j, i) * f2[o];
+          if (f3 && k >= sk && k < ek)
+            d += A3(k + 1, j, i) * f3[o + (long)n1 * n2] - A3(k, j, i) * f3[o];
+          div[o] = d / g.vol[q];
+        }
+  if (c->coord == OC_COORD_SPHERICAL_POLAR) {
+    const double *rho = w + OC_IDN * ncell, *v2 = w + OC_IVY * ncell, *v3 = w + OC_IVZ * ncell;
+    const double* pre = w + OC_IPR * ncell;
+    for (int k = 0; k < n3; ++k)
+      for (int j = 0; j < n2; ++j)
+        for (int i = 0; i < n1; ++i) {
+          long q = ((long)k * n2 + j) * n1 + i;
+          /* radial momentum: centrifugal terms and the pressure source */
+          double src1 = g.src1_i[i] * rho[q] * (v2[q] * v2[q] + v3[q] * v3[q]);
+          if (fp1) {
+            if (i >= si && i < ei)
+              src1 += (A1(k, j, i + 1) * fp1[q + 1] - A1(k, j, i) * fp1[q]) / g.vol[q] -
+                      (fp1[q + 1] - fp1[q]) / g.dx1f[i];
+          } else {
+            src1 += 2.0 * g.src1_i[i] * pre[q];
+          }
+          div[OC_IVX * ncell + q] -= src1;
+          /* polar momentum */
+          double m_pp = rho[q] * (v3[q] * v3[q]) + pre[q];
+          div[OC_IVY * ncell + q] -= g.src1_i[i] * g.src1_j[j] * m_pp;
+          if (i >= si && i < ei && f1) {
+            div[OC_IVY * ncell + q] +=
+                g.src2_i[i] * (A1(k, j, i) * f1[OC_IVY * ncell + q] +
+                               A1(k, j, i + 1) * f1[OC_IVY * ncell + q + 1]);
+            div[OC_IVZ * ncell + q] +=
+                g.src2_i[i] * (A1(k, j, i) * f1[OC_IVZ * ncell + q] +
+                               A1(k, j, i + 1) * f1[OC_IVZ * ncell + q + 1]);
+          }
+          /* azimuthal momentum */
+          if (c->nx2 > 1) {
+            if (j >= sj && j < ej && i >= si && i < ei && f2)
+              div[OC_IVZ * ncell + q] +=
+                  g.src1_i[i] * g.src2_j[j] *
+                  (A2(k, j, i) * f2[OC_IVZ * ncell + q] +
+                   A2(k, j + 1, i) * f2[OC_IVZ * ncell + q + n1]);
+          } else {
+            double m_ph = rho[q] * v3[q] * v2[q];
+            div[OC_IVZ * ncell + q] += g.src1_i[i] * g.src1_j[j] * m_ph; /* coord_src3_j */
+          }
+        }
+  }
+  geom_free(&g);
+}
+
+/* geometry arrays for checks: what = 0 x1v, 1 x2v, 2 dx1v, 3 cell volume,
+   4/5/6 face areas of x1/x2/x3, 7/8/9 the CFL widths of x1/x2/x3 (cells) */
+void oc_geometry(const oc_config* c, int what, double* out) {
+  oc_geom g;
+  geom_init(c, &g);
+  int n1 = g.n1, n2 = g.n2, n3 = g.n3;
+  long n = (long)n3 * n2 * n1;
+  switch (what) {
+    case 0: memcpy(out, g.x1v, sizeof(double) * n1); break;
+    case 1: memcpy(out, g.x2v, sizeof(double) * n2); break;
+    case 2: memcpy(out, g.dx1v, sizeof(double) * n1); break;
+    case 3: memcpy(out, g.vol, sizeof(double) * n); break;
+    case 4: memcpy(out, g.area1, sizeof(double) * n3 * n2 * (n1 + 1)); break;
+    case 5: memcpy(out, g.area2, sizeof(double) * n3 * (n2 + 1) * n1); break;
+    case 6: memcpy(out, g.area3, sizeof(double) * (n3 + 1) * n2 * n1); break;
+    default:
+      for (int k = 0; k < n3; ++k)
+        for (int j = 0; j < n2; ++j)
+          for (int i = 0; i < n1; ++i) {
+            long q = ((long)k * n2 + j) * n1 + i;
+            int sph = c->coord == OC_COORD_SPHERICAL_POLAR;
+            if (what == 7) out[q] = g.dx1f[i];
+            else if (what == 8) out[q] = sph ? g.x1v[i] * g.dx2f[j] : g.dx2f[j];
+            else out[q] = sph ? g.x1v[i] * fabs(sin(g.x2v[j])) * g.dx3f[k] : g.dx3f[k];
+          }
+  }
+  geom_free(&g);
+}
+
+/* ------------------------------------------------------------------------ */
 /* gravity boundary revisions (src/hydro/hydro.cpp:217-269)                  */
 /* ------------------------------------------------------------------------ */
 
@@ -1325,6 +1568,15 @@ void oc_vic_redistribute_cell(int N, double* du, const double* w,
 static void column_geometry(const oc_config* c, int k, int j, double* area,
                             double* vol, double* dx1f) {
   int n1 = oc_nc1(c), n2 = oc_nc2(c), n3 = oc_nc3(c);
+  if (c->coord == OC_COORD_SPHERICAL_POLAR) {
+    oc_geom G;
+    geom_init(c, &G);
+    for (int i = 0; i <= c->nx1; ++i) area[i] = G.area1[((long)k * n2 + j) * (n1 + 1) + c->nghost + i];
+    for (int i = 0; i < c->nx1; ++i) vol[i] = G.vol[((long)k * n2 + j) * n1 + c->nghost + i];
+    memcpy(dx1f, G.dx1f, sizeof(double) * n1);
+    geom_free(&G);
+    return;
+  }
   double* dx2f = malloc(sizeof(double) * n2);
   double* dx3f = malloc(sizeof(double) * n3);
   oc_cartesian_widths(c, dx1f, dx2f, dx3f);
@@ -1356,7 +1608,8 @@ void oc_vic_column_matrices(const oc_config* c, double dt, const double* w,
 
 /* HydroImpl::_apply_implicit_correction + ImplicitHydroImpl::forward
    (src/hydro/hydro.cpp:186-215, src/implicit/implicit_hydro.cpp:99-189) for a
-   Cartesian block (the local-frame projections are exact identities). */
+   Cartesian or spherical-polar block (cosine_cell_kj is zero on both, so the
+   local-frame projections are exact identities). */
 void oc_implicit_correction(const oc_config* c, double dt, double* du,
                             const double* w) {
   if (c->implicit_scheme == 0) return;
@@ -1370,6 +1623,9 @@ void oc_implicit_correction(const oc_config* c, double dt, double* du,
   double* dx2f = malloc(sizeof(double) * n2);
   double* dx3f = malloc(sizeof(double) * n3);
   oc_cartesian_widths(c, dx1f, dx2f, dx3f);
+  int sph = c->coord == OC_COORD_SPHERICAL_POLAR;
+  oc_geom G;
+  geom_init(c, &G);
   long ncol = (long)(ku - kl) * (ju - jl);
 #pragma omp parallel num_threads(g_threads)
   {
@@ -1383,8 +1639,14 @@ void oc_implicit_correction(const oc_config* c, double dt, double* du,
     for (long col = 0; col < ncol; ++col) {
       int k = kl + col / (ju - jl);
       int j = jl + col % (ju - jl);
-      for (int i = 0; i <= nl; ++i) area[i] = dx3f[k] * dx2f[j];
-      for (int i = 0; i < nl; ++i) vol[i] = dx3f[k] * (dx2f[j] * dx1f[c->nghost + i]);
+      if (sph) {
+        /* face_area1 / cell_volume of the column (implicit_hydro.cpp:147-150) */
+        for (int i = 0; i <= nl; ++i) area[i] = G.area1[((long)k * n2 + j) * (n1 + 1) + c->nghost + i];
+        for (int i = 0; i < nl; ++i) vol[i] = G.vol[((long)k * n2 + j) * n1 + c->nghost + i];
+      } else {
+        for (int i = 0; i <= nl; ++i) area[i] = dx3f[k] * dx2f[j];
+        for (int i = 0; i < nl; ++i) vol[i] = dx3f[k] * (dx2f[j] * dx1f[c->nghost + i]);
+      }
       long off = ((long)k * n2 + j) * n1 + c->nghost;
       const double* wc = w + off;
       double* duc = du + off;
@@ -1404,6 +1666,7 @@ void oc_implicit_correction(const oc_config* c, double dt, double* du,
     free(area);
     free(vol);
   }
+  geom_free(&G);
   free(dx1f);
   free(dx2f);
   free(dx3f);
@@ -1424,6 +1687,11 @@ void oc_hydro_forward(const oc_config* c, double dt, double* u, double* w,
   double* dx2f = malloc(sizeof(double) * n2);
   double* dx3f = malloc(sizeof(double) * n3);
   oc_cartesian_widths(c, dx1f, dx2f, dx3f);
+  int sph = c->coord == OC_COORD_SPHERICAL_POLAR;
+  oc_geom G;
+  geom_init(c, &G);
+  /* x1 face pressure (src/hydro/hydro.cpp:88-95: zeros until the x1 solve) */
+  double* fp1 = n1 > 1 ? calloc(ncell, sizeof(double)) : NULL;
   double* wlr = malloc(sizeof(double) * 2 * nvar * ncell);
   double* F1 = n1 > 1 ? calloc(nvar * ncell, sizeof(double)) : NULL;
   double* F2 = n2 > 1 ? calloc(nvar * ncell, sizeof(double)) : NULL;
@@ -1437,7 +1705,7 @@ void oc_hydro_forward(const oc_config* c, double dt, double* u, double* w,
   /* (2) x1 */
   if (n1 > 1) {
     int grav1 = c->grav1 != 0.;
-    if (grav1) oc_revise_x1_ghost(w, nvar, n3, n2, n1, ng, c->grav1, c->gammad, dx1f);
+    if (grav1) oc_revise_x1_ghost(w, nvar, n3, n2, n1, ng, c->grav1, c->gammad, G.dx1v);
     oc_reconstruct(w, wlr, nvar, n3, n2, n1, 3, c->recon_type[0], c->recon_scale[0],
                    c->recon_shock[0], 1, c->limiter, c->density_floor,
                    c->pressure_floor);
@@ -1448,10 +1716,10 @@ void oc_hydro_forward(const oc_config* c, double dt, double* u, double* w,
       const double* pr = wlr + nvar * ncell + OC_IPR * ncell;
       for (long p = 0; p < (long)n3 * n2; ++p)
         for (int i = ng; i < n1 - ng; ++i)
-          rho_grav[p * n1 + i] = (pl[p * n1 + i + 1] - pr[p * n1 + i]) / dx1f[i];
+          rho_grav[p * n1 + i] = (pl[p * n1 + i + 1] - pr[p * n1 + i]) / G.dx1f[i];
     }
     if (!c->disable_flux[0])
-      oc_riemann(c->riemann[0], wlr, wlr + nvar * ncell, F1, NULL, nvar, n3, n2, n1, 3,
+      oc_riemann(c->riemann[0], wlr, wlr + nvar * ncell, F1, fp1, nvar, n3, n2, n1, 3,
                  c->gammad);
   }
   /* (3, 4) x2, x3 */
@@ -1472,8 +1740,11 @@ void oc_hydro_forward(const oc_config* c, double dt, double* u, double* w,
                  c->gammad);
   }
 
-  /* (5) divergence */
-  oc_divergence(F1, F2, F3, div, nvar, n3, n2, n1, ng, dx1f, dx2f, dx3f);
+  /* (5) divergence, and the geometric sources of spherical-polar blocks */
+  if (sph)
+    oc_coord_forward(c, w, F1, F2, F3, fp1, div);
+  else
+    oc_divergence(F1, F2, F3, div, nvar, n3, n2, n1, ng, dx1f, dx2f, dx3f);
 
   /* (6) du = -dt D on the interior, then forcings on every cell */
   memset(du, 0, sizeof(double) * nvar * ncell);
@@ -1518,6 +1789,8 @@ void oc_hydro_forward(const oc_config* c, double dt, double* u, double* w,
   if (f1out && F1) memcpy(f1out, F1, sizeof(double) * nvar * ncell);
   if (f2out && F2) memcpy(f2out, F2, sizeof(double) * nvar * ncell);
   if (f3out && F3) memcpy(f3out, F3, sizeof(double) * nvar * ncell);
+  geom_free(&G);
+  free(fp1);
   free(dx1f);
   free(dx2f);
   free(dx3f);
@@ -1593,6 +1866,14 @@ double oc_max_time_step(const oc_config* c, const double* w, double cfl) {
   double* dx2f = malloc(sizeof(double) * n2);
   double* dx3f = malloc(sizeof(double) * n3);
   oc_cartesian_widths(c, dx1f, dx2f, dx3f);
+  /* center widths (src/coord/coordinate.cpp:276-286,
+     src/coord/spherical_polar.cpp:146-155) */
+  double* W1 = malloc(sizeof(double) * ncell);
+  double* W2 = malloc(sizeof(double) * ncell);
+  double* W3 = malloc(sizeof(double) * ncell);
+  oc_geometry(c, 7, W1);
+  oc_geometry(c, 8, W2);
+  oc_geometry(c, 9, W3);
   int sch = c->implicit_scheme;
   int use1 = n1 > 1 && (!(sch & 1) || (n3 == 1 && n2 == 1));
   int use2 = n2 > 1 && !((sch >> 1) & 1);
@@ -1607,10 +1888,13 @@ double oc_max_time_step(const oc_config* c, const double* w, double cfl) {
         long q = ((long)k * n2 + j) * n1 + i;
         double cs = sqrt(c->gammad * 1.0 * w[OC_IPR * ncell + q] / w[OC_IDN * ncell + q]);
         double t;
-        if (use1 && (t = dx1f[i] / (fabs(w[OC_IVX * ncell + q]) + cs)) < d1) d1 = t;
-        if (use2 && (t = dx2f[j] / (fabs(w[OC_IVY * ncell + q]) + cs)) < d2) d2 = t;
-        if (use3 && (t = dx3f[k] / (fabs(w[OC_IVZ * ncell + q]) + cs)) < d3) d3 = t;
+        if (use1 && (t = W1[q] / (fabs(w[OC_IVX * ncell + q]) + cs)) < d1) d1 = t;
+        if (use2 && (t = W2[q] / (fabs(w[OC_IVY * ncell + q]) + cs)) < d2) d2 = t;
+        if (use3 && (t = W3[q] / (fabs(w[OC_IVZ * ncell + q]) + cs)) < d3) d3 = t;
       }
+  free(W1);
+  free(W2);
+  free(W3);
   free(dx1f);
   free(dx2f);
   free(dx3f);

@@ -66,7 +66,12 @@ typedef struct {
   int disable_flux[3];
   /* external BCs: x1-inner, x1-outer, x2-inner, x2-outer, x3-inner, x3-outer */
   int bc[6];
+  /* geometry.type: OC_COORD_* */
+  int coord;
 } oc_config;
+
+#define OC_COORD_CARTESIAN 0
+#define OC_COORD_SPHERICAL_POLAR 1
 
 /* ---- grid helpers ---- */
 int oc_nc1(const oc_config* c);
@@ -105,6 +110,16 @@ void oc_riemann(int solver, double* wl, double* wr, double* flx, double* fp,
 void oc_divergence(const double* f1, const double* f2, const double* f3,
                    double* div, int nvar, int n3, int n2, int n1, int nghost,
                    const double* dx1f, const double* dx2f, const double* dx3f);
+
+/* ---- coordinate forward: divergence + spherical-polar sources
+   (src/coord/coordinate.cpp:428-434, src/coord/spherical_polar.cpp:212-279);
+   fp1 may be NULL; div covers every cell ---- */
+void oc_coord_forward(const oc_config* c, const double* w, const double* f1,
+                      const double* f2, const double* f3, const double* fp1,
+                      double* div);
+/* geometry arrays: 0 x1v, 1 x2v, 2 dx1v, 3 volume, 4/5/6 face areas,
+   7/8/9 CFL widths */
+void oc_geometry(const oc_config* c, int what, double* out);
 
 /* ---- gravity boundary revisions (src/hydro/hydro.cpp:217-269) ---- */
 void oc_revise_x1_ghost(double* w, int nvar, int n3, int n2, int n1,

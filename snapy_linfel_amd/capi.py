@@ -18,6 +18,7 @@ F32, F64 = 0, 1
 INTERP = {"dc": 0, "plm": 1, "cp3": 2, "cp5": 3, "weno3": 4, "weno5": 5}
 RIEMANN = {"roe": 0, "lmars": 1, "hllc": 2}
 BC = {None: -1, "none": -1, "reflecting": 0, "outflow": 1, "periodic": 2}
+COORD = {"cartesian": 0, "spherical-polar": 1}
 PRIMITIVE, CONSERVED, SCALAR = 0, 1, 2
 
 
@@ -44,6 +45,7 @@ class Config(C.Structure):
         ("implicit_scheme", C.c_int),
         ("disable_flux", C.c_int * 3),
         ("bc", C.c_int * 6),
+        ("coord", C.c_int),
     ]
 
     @property
@@ -92,6 +94,7 @@ def make_config(**kw) -> Config:
     active = [c.nx1 > 1, c.nx1 > 1, c.nx2 > 1, c.nx2 > 1, c.nx3 > 1, c.nx3 > 1]
     for d in range(6):
         c.bc[d] = BC[bcs[d]] if active[d] else -1
+    c.coord = COORD[kw.get("coord", "cartesian")]
     return c
 
 
@@ -110,6 +113,7 @@ SIGNATURES = {
                                _I, _P]),
     "snapx_riemann": (_I, [_I, _P, _P, _P, _P, _I, _I, _I, _I, _I, _D, _I, _P]),
     "snapx_divergence": (_I, [_CFG, _P, _P, _P, _P, _I, _P]),
+    "snapx_coord_forward": (_I, [_CFG, _P, _P, _P, _P, _P, _P, _I, _P]),
     "snapx_revise_x1_ghost": (_I, [_CFG, _P, _I, _P]),
     "snapx_apply_bc": (_I, [_CFG, _P, _I, _I, _I, _P]),
     "snapx_rk_combine": (_I, [_P, _P, _P, _P, _D, _D, _D, _L, _I, _P]),
@@ -237,6 +241,12 @@ def riemann(solver, wl, wr, flx, fp, dim, gammad=1.4):
 def divergence(cfg, f1, f2, f3, div):
     check(lib().snapx_divergence(C.byref(cfg), ptr(f1), ptr(f2), ptr(f3), ptr(div),
                                  dtype_code(div), stream_ptr()), "divergence")
+
+
+def coord_forward(cfg, w, f1, f2, f3, fp1, div):
+    """CoordinateImpl::forward: divergence plus the spherical-polar sources."""
+    check(lib().snapx_coord_forward(C.byref(cfg), ptr(w), ptr(f1), ptr(f2), ptr(f3), ptr(fp1),
+                                    ptr(div), dtype_code(div), stream_ptr()), "coord_forward")
 
 
 def revise_x1_ghost(cfg, w):

@@ -41,6 +41,16 @@ int validate_config(const snapx_config* c) {
   for (int b = 0; b < 6; ++b)
     SNAPX_CHECK_ARG(c->bc[b] >= SNAPX_BC_NONE && c->bc[b] <= SNAPX_BC_PERIODIC,
                     "Boundary function is not defined");
+  SNAPX_CHECK_ARG(c->coord == SNAPX_COORD_CARTESIAN || c->coord == SNAPX_COORD_SPHERICAL_POLAR,
+                  "Unknown coordinate type");
+  if (c->coord == SNAPX_COORD_SPHERICAL_POLAR) {
+    // src/coord/spherical_polar.cpp:46-48
+    SNAPX_CHECK_ARG(c->nx2 <= 1 || (c->x2min >= 0.0 && c->x2max <= M_PI),
+                    "Spherical-polar coordinates require x2 in [0, pi]");
+    // the reference's spherical source reads the x1 fluxes, which exist only
+    // on blocks with an active x1 axis
+    SNAPX_CHECK_ARG(c->nx1 > 1, "spherical-polar blocks need an active x1 (radial) axis");
+  }
   Grid g = make_grid(c);
   SNAPX_CHECK_ARG((int64_t)2 * 5 * g.n1 * g.n2 * g.n3 < ((int64_t)1 << 31),
                   "block too large for 32-bit indexing");

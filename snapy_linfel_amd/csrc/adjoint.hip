@@ -1255,6 +1255,7 @@ using namespace snapx;
 
 extern "C" size_t snapx_adjoint_workspace_bytes(const snapx_config* cfg) {
   if (validate_config(cfg)) return 0;
+  if (cfg->coord != SNAPX_COORD_CARTESIAN) return 0;
   return adj_layout(cfg, nullptr).bytes;
 }
 
@@ -1311,6 +1312,7 @@ extern "C" int snapx_adjoint_rk3_step(const snapx_config* cfg, double dt, const 
   SNAPX_CHECK_ARG(u_in && lambda && workspace, "snapx_adjoint_rk3_step: null pointer");
   SNAPX_CHECK_ARG(!cfg->limiter, "adjoint: the EOS limiter is not differentiable (turn it off)");
   SNAPX_CHECK_ARG(cfg->non_hydrostatic == 1.0, "adjoint: non-hydrostatic < 1 is not supported");
+  SNAPX_CHECK_ARG(cfg->coord == SNAPX_COORD_CARTESIAN, "adjoint: Cartesian geometry only");
   for (int d = 0; d < 3; ++d)
     SNAPX_CHECK_ARG(!cfg->disable_flux[d], "adjoint: disable-flux is not supported");
   AdjSpace s = adj_layout(cfg, reinterpret_cast<char*>(workspace));

@@ -1577,6 +1577,8 @@ static size_t rhs_smem_bytes(const FusedArgs& a) {
 
 static int check_fused(const snapx_config* c) {
   if (int e = validate_config(c)) return e;
+  SNAPX_CHECK_ARG(c->coord == SNAPX_COORD_CARTESIAN,
+                  "fused path: Cartesian geometry only (use the plain path)");
   SNAPX_CHECK_ARG(c->nghost == 3, "fused path: nghost must be 3");
   SNAPX_CHECK_ARG(c->nx1 > 1, "fused path: x1 must be active");
   SNAPX_CHECK_ARG(!c->limiter, "fused path: the EOS limiter is not supported (use the plain path)");

@@ -13,6 +13,7 @@
 #include <cstring>
 
 #include "common.hpp"
+#include "geom_impl.hpp"
 #include "interp_impl.hpp"
 #include "riemann_impl.hpp"
 
@@ -253,6 +254,117 @@ __global__ void du_kernel(const T* __restrict__ f1, const T* __restrict__ f2,
   }
 }
 
+struct CoordArgs {
+  Geometry g;
+  double dx1, dx2, dx3;  // Cartesian widths
+  double dt, grav1, grav2, grav3, nh;
+  int du_mode;  // 0: div on every cell; 1: du = -dt div on the interior + forcings
+};
+
+// CoordinateImpl::forward (src/coord/coordinate.cpp:352-434) for both
+// geometries and SphericalPolarImpl::forward (src/coord/spherical_polar.cpp:
+// 190-279): each axis adds its flux difference on its own interior range
+// (all cells of the other axes), then the momenta get the geometric sources.
+// fp1 == nullptr selects the 2 p / r form of the radial pressure source.
+// With du_mode the interior assembly, gravity and hydrostatic correction of
+// du_kernel follow.
+template <typename T>
+__global__ void coord_forward_kernel(const T* __restrict__ w, const T* __restrict__ f1,
+                                     const T* __restrict__ f2, const T* __restrict__ f3,
+                                     const T* __restrict__ fp1, const T* __restrict__ rho_grav,
+                                     T* __restrict__ out, CoordArgs a) {
+  const Geometry& g = a.g;
+  const int n1 = g.n1, n2 = g.n2, n3 = g.n3, ng = g.ng;
+  const int ncell = n1 * n2 * n3;
+  const bool sph = g.coord == SNAPX_COORD_SPHERICAL_POLAR;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < ncell; q += gridDim.x * blockDim.x) {
+    int i = q % n1;
+    int j = (q / n1) % n2;
+    int k = q / (n1 * n2);
+    bool r1 = n1 == 1 || (i >= ng && i < n1 - ng);
+    bool r2 = n2 == 1 || (j >= ng && j < n2 - ng);
+    bool r3 = n3 == 1 || (k >= ng && k < n3 - ng);
+    double vol, A1m, A1p, A2m, A2p, A3;
+    if (sph) {
+      vol = sph_volume(g, k, j, i);
+      A1m = sph_area1(g, k, j, i), A1p = sph_area1(g, k, j, i + 1);
+      A2m = sph_area2(g, k, j, i), A2p = sph_area2(g, k, j + 1, i);
+      A3 = sph_area3(g, j, i);
+    } else {
+      vol = a.dx3 * (a.dx2 * a.dx1);
+      A1m = A1p = a.dx3 * a.dx2;
+      A2m = A2p = a.dx3 * a.dx1;
+      A3 = a.dx2 * a.dx1;
+    }
+    T d[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      int o = v * ncell + q;
+      T acc = T(0.);
+      if (f1 && r1) acc += T(A1p) * f1[o + 1] - T(A1m) * f1[o];
+      if (f2 && r2) acc += T(A2p) * f2[o + n1] - T(A2m) * f2[o];
+      if (f3 && r3) acc += T(A3) * f3[o + n1 * n2] - T(A3) * f3[o];
+      d[v] = acc / T(vol);
+    }
+    if (sph) {
+      T rho = w[IDN * ncell + q], v2 = w[IVY * ncell + q], v3 = w[IVZ * ncell + q];
+      T pre = w[IPR * ncell + q];
+      T s1i = T(sph_src1_i(g, i)), s1j = T(sph_src1_j(g, j));
+      T src1 = s1i * rho * (v2 * v2 + v3 * v3);
+      if (fp1) {
+        if (r1)
+          src1 += (T(A1p) * fp1[q + 1] - T(A1m) * fp1[q]) / T(vol) -
+                  (fp1[q + 1] - fp1[q]) / T(axis_width(g.ax[0], i));
+      } else {
+        src1 += T(2.) * s1i * pre;
+      }
+      d[IVX] -= src1;
+      d[IVY] -= s1i * s1j * (rho * (v3 * v3) + pre);
+      if (r1 && f1) {
+        T s2i = T(sph_src2_i(g, i));
+        d[IVY] += s2i * (T(A1m) * f1[IVY * ncell + q] + T(A1p) * f1[IVY * ncell + q + 1]);
+        d[IVZ] += s2i * (T(A1m) * f1[IVZ * ncell + q] + T(A1p) * f1[IVZ * ncell + q + 1]);
+      }
+      if (n2 > 1) {
+        if (r1 && r2 && f2)
+          d[IVZ] += s1i * T(sph_src2_j(g, j)) *
+                    (T(A2m) * f2[IVZ * ncell + q] + T(A2p) * f2[IVZ * ncell + q + n1]);
+      } else {
+        d[IVZ] += s1i * s1j * (rho * v3 * v2);
+      }
+    }
+    if (!a.du_mode) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) out[v * ncell + q] = d[v];
+      continue;
+    }
+    T dt = T(a.dt);
+    bool interior = r1 && r2 && r3;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) d[v] = interior ? -dt * d[v] : T(0.);
+    T rho = (a.grav1 != 0. || a.grav2 != 0. || a.grav3 != 0.) ? w[IDN * ncell + q] : T(0.);
+    if (a.grav1 != 0.) {
+      d[IVX] += dt * rho * T(a.grav1) * T(a.nh);
+      d[IPR] += dt * rho * w[IVX * ncell + q] * T(a.grav1) * T(a.nh);
+    }
+    if (a.grav2 != 0.) {
+      d[IVY] += dt * rho * T(a.grav2);
+      d[IPR] += dt * rho * w[IVY * ncell + q] * T(a.grav2);
+    }
+    if (a.grav3 != 0.) {
+      d[IVZ] += dt * rho * T(a.grav3);
+      d[IPR] += dt * rho * w[IVZ * ncell + q] * T(a.grav3);
+    }
+    if (rho_grav) {
+      T rg = rho_grav[q];
+      d[IVX] += dt * rg * (T(1.) - T(a.nh));
+      d[IPR] += dt * w[IVX * ncell + q] * rg * (T(1.) - T(a.nh));
+    }
+#pragma unroll
+    for (int v = 0; v < 5; ++v) out[v * ncell + q] = d[v];
+  }
+}
+
 template <typename T>
 __global__ void zero_kernel(T* __restrict__ x, int64_t n) {
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n;
@@ -275,9 +387,15 @@ __global__ void rho_grav_kernel(const T* __restrict__ wlr, T* __restrict__ rg, i
 
 // HydroImpl::_revise_x1inner_ghost / _revise_x1outer_ghost
 // (src/hydro/hydro.cpp:231-269); one thread per column.
+// dz of each ghost layer: dx1v[is - n - 1] inward and dx1v[ie + n] outward
+// (the centre distances of the coordinate; the face width on Cartesian blocks)
+struct GhostDz {
+  double in[8], out[8];
+};
+
 template <typename T>
 __global__ void revise_ghost_kernel(T* __restrict__ w, int n1, int ncol, int ncell, int ng,
-                                    T grav1, T gammad, T dz) {
+                                    T grav1, T gammad, GhostDz dz) {
   for (int col = blockIdx.x * blockDim.x + threadIdx.x; col < ncol;
        col += gridDim.x * blockDim.x) {
     T* rho = w + IDN * ncell + col * n1;
@@ -290,13 +408,13 @@ __global__ void revise_ghost_kernel(T* __restrict__ w, int n1, int ncol, int nce
     int is = ng, ie = n1 - ng - 1;
     T K = pre[is] / pow(rho[is], gamma);
     for (int n = 0; n < ng; ++n) {
-      T h = pow(pre[is - n], a) + a * grav * dz / pow(K, inv_g);
+      T h = pow(pre[is - n], a) + a * grav * T(dz.in[n]) / pow(K, inv_g);
       pre[is - n - 1] = pow(h, inv_a);
       rho[is - n - 1] = pow(pre[is - n - 1] / K, inv_g);
     }
     K = pre[ie] / pow(rho[ie], gamma);
     for (int n = 0; n < ng; ++n) {
-      T h = pow(pre[ie + n], a) - a * grav * dz / pow(K, inv_g);
+      T h = pow(pre[ie + n], a) - a * grav * T(dz.out[n]) / pow(K, inv_g);
       pre[ie + n + 1] = pow(h, inv_a);
       rho[ie + n + 1] = pow(pre[ie + n + 1] / K, inv_g);
     }
@@ -367,10 +485,13 @@ __global__ void rk_combine_kernel(T* __restrict__ u, const T* __restrict__ u0,
 
 __global__ void fill_u64_kernel(unsigned long long* p, unsigned long long v) { *p = v; }
 
+// HydroImpl::max_time_step (src/hydro/hydro.cpp:118-180) with the
+// coordinate's center widths: Cartesian dx, spherical-polar dr, r dtheta and
+// r sin(theta) dphi (src/coord/spherical_polar.cpp:146-155)
 template <typename T>
 __global__ void cfl_kernel(const T* __restrict__ w, unsigned long long* __restrict__ out, int n3,
                            int n2, int n1, int ng, T gammad, T dx1, T dx2, T dx3, int use1,
-                           int use2, int use3) {
+                           int use2, int use3, Geometry geo) {
   int ncell = n1 * n2 * n3;
   double best = 1.e9;
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < ncell; q += gridDim.x * blockDim.x) {
@@ -382,6 +503,11 @@ __global__ void cfl_kernel(const T* __restrict__ w, unsigned long long* __restri
     bool in3 = n3 == 1 || (k >= ng && k < n3 - ng);
     if (!(in1 && in2 && in3)) continue;
     T cs = sqrt(gammad * T(1.0) * w[IPR * ncell + q] / w[IDN * ncell + q]);
+    if (geo.coord == SNAPX_COORD_SPHERICAL_POLAR) {
+      dx1 = T(axis_width(geo.ax[0], i));
+      if (use2) dx2 = T(sph_width2(geo, j, i));
+      if (use3) dx3 = T(sph_width3(geo, k, j, i));
+    }
     if (use1) best = fmin(best, (double)(dx1 / (fabs(w[IVX * ncell + q]) + cs)));
     if (use2) best = fmin(best, (double)(dx2 / (fabs(w[IVY * ncell + q]) + cs)));
     if (use3) best = fmin(best, (double)(dx3 / (fabs(w[IVZ * ncell + q]) + cs)));
@@ -603,6 +729,9 @@ template <typename T>
 static int launch_du(const snapx_config* c, double dt, const void* f1, const void* f2,
                      const void* f3, const void* w, const void* rg, void* du, hipStream_t st) {
   Grid g = make_grid(c);
+  if (g.n1 == 1) f1 = nullptr;  // no faces on an inactive axis
+  if (g.n2 == 1) f2 = nullptr;
+  if (g.n3 == 1) f3 = nullptr;
   hipLaunchKernelGGL(du_kernel<T>, blocks_for(g.ncell, kThreads), dim3(kThreads), 0, st,
                      reinterpret_cast<const T*>(f1), reinterpret_cast<const T*>(f2),
                      reinterpret_cast<const T*>(f3), reinterpret_cast<const T*>(w),
@@ -611,10 +740,68 @@ static int launch_du(const snapx_config* c, double dt, const void* f1, const voi
   return 0;
 }
 
+static GhostDz ghost_dz(const snapx_config* c) {
+  Grid g = make_grid(c);
+  Geometry geo = make_geometry(c);
+  GhostDz dz;
+  int is = g.ng, ie = g.n1 - g.ng - 1;
+  for (int n = 0; n < g.ng; ++n) {
+    bool sph = c->coord == SNAPX_COORD_SPHERICAL_POLAR;
+    dz.in[n] = sph ? sph_dx1v(geo, is - n - 1) : g.dx1;
+    dz.out[n] = sph ? sph_dx1v(geo, ie + n) : g.dx1;
+  }
+  return dz;
+}
+
+static CoordArgs coord_args(const snapx_config* c, double dt, int du_mode) {
+  Grid g = make_grid(c);
+  CoordArgs a;
+  a.g = make_geometry(c);
+  a.dx1 = g.dx1, a.dx2 = g.dx2, a.dx3 = g.dx3;
+  a.dt = dt;
+  a.grav1 = c->grav1, a.grav2 = c->grav2, a.grav3 = c->grav3, a.nh = c->non_hydrostatic;
+  a.du_mode = du_mode;
+  return a;
+}
+
+template <typename T>
+static int launch_coord(const snapx_config* c, double dt, int du_mode, const void* w,
+                        const void* f1, const void* f2, const void* f3, const void* fp1,
+                        const void* rg, void* out, hipStream_t st) {
+  Grid g = make_grid(c);
+  // an inactive axis has no faces: its flux (and the x1 face pressure) is
+  // never read, so a one-cell axis cannot index past the field
+  if (g.n1 == 1) f1 = fp1 = nullptr;
+  if (g.n2 == 1) f2 = nullptr;
+  if (g.n3 == 1) f3 = nullptr;
+  hipLaunchKernelGGL(coord_forward_kernel<T>, blocks_for(g.ncell, kThreads), dim3(kThreads), 0,
+                     st, reinterpret_cast<const T*>(w), reinterpret_cast<const T*>(f1),
+                     reinterpret_cast<const T*>(f2), reinterpret_cast<const T*>(f3),
+                     reinterpret_cast<const T*>(fp1), reinterpret_cast<const T*>(rg), as<T>(out),
+                     coord_args(c, dt, du_mode));
+  SNAPX_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int snapx_coord_forward(const snapx_config* cfg, const void* w, const void* f1,
+                                   const void* f2, const void* f3, const void* fp1, void* div,
+                                   int dtype, void* stream) {
+  if (int e = validate_config(cfg)) return e;
+  SNAPX_CHECK_ARG(div, "snapx_coord_forward: null output");
+  SNAPX_CHECK_ARG(w || cfg->coord == SNAPX_COORD_CARTESIAN,
+                  "snapx_coord_forward: spherical-polar sources need the primitives w");
+  hipStream_t st = as_stream(stream);
+  return dtype == SNAPX_F64
+             ? launch_coord<double>(cfg, 0., 0, w, f1, f2, f3, fp1, nullptr, div, st)
+             : launch_coord<float>(cfg, 0., 0, w, f1, f2, f3, fp1, nullptr, div, st);
+}
+
 extern "C" int snapx_divergence(const snapx_config* cfg, const void* f1, const void* f2,
                                 const void* f3, void* div, int dtype, void* stream) {
   if (int e = validate_config(cfg)) return e;
   SNAPX_CHECK_ARG(div, "snapx_divergence: null output");
+  SNAPX_CHECK_ARG(cfg->coord == SNAPX_COORD_CARTESIAN,
+                  "snapx_divergence: Cartesian only (snapx_coord_forward covers spherical-polar)");
   // D = -(du / dt) with dt = -1 and no forcing: du_kernel with dt = -1 gives
   // +D on the interior; the reference leaves non-interior cells at 0 / vol = 0
   // except on the ghost rows of the other axes, which no caller reads.
@@ -635,11 +822,11 @@ extern "C" int snapx_revise_x1_ghost(const snapx_config* cfg, void* w, int dtype
   if (dtype == SNAPX_F64)
     hipLaunchKernelGGL(revise_ghost_kernel<double>, blocks_for(ncol, kThreads), dim3(kThreads), 0,
                        st, as<double>(w), g.n1, ncol, g.ncell, g.ng, cfg->grav1, cfg->gammad,
-                       g.dx1);
+                       ghost_dz(cfg));
   else
     hipLaunchKernelGGL(revise_ghost_kernel<float>, blocks_for(ncol, kThreads), dim3(kThreads), 0,
                        st, as<float>(w), g.n1, ncol, g.ncell, g.ng, (float)cfg->grav1,
-                       (float)cfg->gammad, (float)g.dx1);
+                       (float)cfg->gammad, ghost_dz(cfg));
   SNAPX_LAUNCH_CHECK();
   return 0;
 }
@@ -690,12 +877,12 @@ extern "C" int snapx_max_time_step(const snapx_config* cfg, const void* w, doubl
   if (dtype == SNAPX_F64)
     hipLaunchKernelGGL(cfl_kernel<double>, dim3(nb), dim3(kThreads), 0, st,
                        reinterpret_cast<const double*>(w), acc, g.n3, g.n2, g.n1, g.ng,
-                       cfg->gammad, g.dx1, g.dx2, g.dx3, use1, use2, use3);
+                       cfg->gammad, g.dx1, g.dx2, g.dx3, use1, use2, use3, make_geometry(cfg));
   else
     hipLaunchKernelGGL(cfl_kernel<float>, dim3(nb), dim3(kThreads), 0, st,
                        reinterpret_cast<const float*>(w), acc, g.n3, g.n2, g.n1, g.ng,
                        (float)cfg->gammad, (float)g.dx1, (float)g.dx2, (float)g.dx3, use1, use2,
-                       use3);
+                       use3, make_geometry(cfg));
   hipLaunchKernelGGL(cfl_finalize_kernel, dim3(1), dim3(1), 0, st, dt_out, cfl);
   SNAPX_LAUNCH_CHECK();
   return 0;
@@ -731,7 +918,7 @@ namespace {
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 struct HydroSpace {
-  void *wlr, *f[3], *rg, *vic;
+  void *wlr, *f[3], *rg, *fp1, *vic;
   size_t bytes;
 };
 
@@ -748,6 +935,8 @@ HydroSpace hydro_layout(const snapx_config* c, int dtype, char* base) {
     off += field;
   }
   h.rg = base + off;
+  off += align256(es * (size_t)g.ncell);
+  h.fp1 = base + off;  // x1 face pressure (spherical-polar sources)
   off += align256(es * (size_t)g.ncell);
   h.vic = base + off;
   off += align256(implicit_workspace_bytes(c));
@@ -774,6 +963,7 @@ static int hydro_forward_impl(const snapx_config* c, double dt, void* u, void* w
   int dims[3] = {3, 2, 1};
   const void* Fuse[3] = {nullptr, nullptr, nullptr};
   const void* rg = nullptr;
+  const bool sph = c->coord == SNAPX_COORD_SPHERICAL_POLAR;
 
   // (1) primitives
   if (int e = launch_cons2prim<T>(c, u, w, st)) return e;
@@ -784,7 +974,7 @@ static int hydro_forward_impl(const snapx_config* c, double dt, void* u, void* w
     if (grav1) {
       hipLaunchKernelGGL(revise_ghost_kernel<T>, blocks_for(g.n2 * g.n3, kThreads),
                          dim3(kThreads), 0, st, as<T>(w), g.n1, g.n2 * g.n3, g.ncell, g.ng,
-                         T(c->grav1), T(c->gammad), T(g.dx1));
+                         T(c->grav1), T(c->gammad), ghost_dz(c));
       SNAPX_LAUNCH_CHECK();
     }
     int r = a == 0 ? 0 : 1;
@@ -799,19 +989,34 @@ static int hydro_forward_impl(const snapx_config* c, double dt, void* u, void* w
       SNAPX_LAUNCH_CHECK();
       rg = h.rg;
     }
+    // the x1 face pressure feeds the spherical-polar radial source; the
+    // reference keeps it zero when the x1 flux is disabled
+    void* fp = (a == 0 && sph) ? h.fp1 : nullptr;
     if (!c->disable_flux[a]) {
       T* wl = as<T>(h.wlr);
-      if (int e = launch_riemann<T>(c->riemann[a], wl, wl + 5 * g.ncell, F[a], nullptr, g.ncell,
+      if (int e = launch_riemann<T>(c->riemann[a], wl, wl + 5 * g.ncell, F[a], fp, g.ncell,
                                     dims[a], c->gammad, st))
         return e;
       Fuse[a] = F[a];
-    } else if (a == 0 ? f1 : (a == 1 ? f2 : f3)) {
-      hipLaunchKernelGGL(zero_kernel<T>, blocks_for(5 * g.ncell, kThreads), dim3(kThreads), 0,
-                         st, as<T>(F[a]), (int64_t)5 * g.ncell);
-      SNAPX_LAUNCH_CHECK();
+    } else {
+      if (a == 0 ? f1 : (a == 1 ? f2 : f3)) {
+        hipLaunchKernelGGL(zero_kernel<T>, blocks_for(5 * g.ncell, kThreads), dim3(kThreads), 0,
+                           st, as<T>(F[a]), (int64_t)5 * g.ncell);
+        SNAPX_LAUNCH_CHECK();
+      }
+      if (fp) {
+        hipLaunchKernelGGL(zero_kernel<T>, blocks_for(g.ncell, kThreads), dim3(kThreads), 0, st,
+                           as<T>(fp), (int64_t)g.ncell);
+        SNAPX_LAUNCH_CHECK();
+      }
     }
   }
-  if (int e = launch_du<T>(c, dt, Fuse[0], Fuse[1], Fuse[2], w, rg, du, st)) return e;
+  if (sph) {
+    if (int e = launch_coord<T>(c, dt, 1, w, Fuse[0], Fuse[1], Fuse[2], h.fp1, rg, du, st))
+      return e;
+  } else if (int e = launch_du<T>(c, dt, Fuse[0], Fuse[1], Fuse[2], w, rg, du, st)) {
+    return e;
+  }
   if (c->implicit_scheme)
     return implicit_correction(c, dt, du, w, h.vic, implicit_workspace_bytes(c), dtype, st);
   return 0;

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include "common.hpp"
+#include "geom_impl.hpp"
 #include "vic_impl.hpp"
 
 namespace snapx {
@@ -22,13 +23,29 @@ constexpr int kVicThreads = 128;
 struct VicArgs {
   int n1, n2, n3, ng;
   int nl, ncol, ncol2;  // layers, columns, interior columns along x2
-  double gammad, area, vol, dt, grav;
+  double gammad, area, vol, dt, grav;  // Cartesian face area and volume
+  Geometry geo;
 };
 
 __device__ __forceinline__ int col_cell(const VicArgs& a, int col) {
   int j = a.n2 > 1 ? a.ng + col % a.ncol2 : 0;
   int k = a.n3 > 1 ? a.ng + col / a.ncol2 : 0;
   return (k * a.n2 + j) * a.n1 + a.ng;  // first interior cell of the column
+}
+
+// face_area1 at the lower and upper x1 faces and cell_volume of layer i of
+// a column (src/implicit/implicit_hydro.cpp:147-150): constants on Cartesian
+// blocks, r-dependent on spherical-polar ones
+struct LayerGeom {
+  double area_i, area_ip1, vol;
+};
+
+__device__ __forceinline__ LayerGeom layer_geom(const VicArgs& a, int col, int i) {
+  if (a.geo.coord != SNAPX_COORD_SPHERICAL_POLAR) return {a.area, a.area, a.vol};
+  int j = a.n2 > 1 ? a.ng + col % a.ncol2 : 0;
+  int k = a.n3 > 1 ? a.ng + col / a.ncol2 : 0;
+  int c = a.ng + i;
+  return {sph_area1(a.geo, k, j, c), sph_area1(a.geo, k, j, c + 1), sph_volume(a.geo, k, j, c)};
 }
 
 template <typename T>
@@ -55,8 +72,9 @@ __global__ void __launch_bounds__(kVicThreads)
     load_prim(w, ncell, q + 1, wp);
     double* base = abc + ((int64_t)col * a.nl + i) * 3 * NN;
     double A[NN], B[NN], C[NN];
-    vic_assemble_cell<double, N>(wm, w0, wp, a.gammad, a.area, a.area, a.vol, a.dt, a.grav,
-                                 i == 0, i == a.nl - 1, A, B, C);
+    LayerGeom lg = layer_geom(a, col, i);
+    vic_assemble_cell<double, N>(wm, w0, wp, a.gammad, lg.area_i, lg.area_ip1, lg.vol, a.dt,
+                                 a.grav, i == 0, i == a.nl - 1, A, B, C);
 #pragma unroll
     for (int e = 0; e < NN; ++e) {
       base[e] = A[e];
@@ -165,9 +183,10 @@ __global__ void __launch_bounds__(kVicThreads)
     for (int i = 0; i < a.nl; ++i) {
       int q = q0 + i;
       double explicit_total = (double)du[IDN * ncell + q];
-      double a_i = (double)w[IDN * ncell + q] * a.vol;
+      double vol = layer_geom(a, col, i).vol;
+      double a_i = (double)w[IDN * ncell + q] * vol;
       sum_a2 += a_i * a_i;
-      b_dry += (explicit_total - cd[i * N]) * 1.0 * a.vol;
+      b_dry += (explicit_total - cd[i * N]) * 1.0 * vol;
     }
     colsum[2 * col] = sum_a2;
     colsum[2 * col + 1] = b_dry;
@@ -190,7 +209,7 @@ __global__ void __launch_bounds__(kVicThreads)
     const double* d = delta + ((int64_t)col * a.nl + i) * N;
     double sum_a2 = colsum[2 * col];
     double rho = (double)w[IDN * ncell + q];
-    double a_i = rho * a.vol;
+    double a_i = rho * layer_geom(a, col, i).vol;
     double du0 = (double)du[IDN * ncell + q];
     double diffusion_fix = d[0] - du0;
     double dry_structural = rho * a_i * colsum[2 * col + 1] / sum_a2;
@@ -217,6 +236,7 @@ static VicArgs vic_args(const snapx_config* c, double dt) {
   a.gammad = c->gammad;
   a.area = g.dx3 * g.dx2;
   a.vol = g.dx3 * (g.dx2 * g.dx1);
+  a.geo = make_geometry(c);
   a.dt = dt;
   a.grav = c->grav1;
   return a;

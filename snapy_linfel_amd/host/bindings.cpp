@@ -158,8 +158,18 @@ PYBIND11_MODULE(_snapy, m) {
       .def_readonly("x1v", &CoordinateImpl::x1v)
       .def_readonly("x2v", &CoordinateImpl::x2v)
       .def_readonly("x3v", &CoordinateImpl::x3v)
+      .def_readonly("x1f", &CoordinateImpl::x1f)
+      .def_readonly("x2f", &CoordinateImpl::x2f)
+      .def_readonly("x3f", &CoordinateImpl::x3f)
+      .def_readonly("dx1f", &CoordinateImpl::dx1f)
+      .def_readonly("dx2f", &CoordinateImpl::dx2f)
+      .def_readonly("dx3f", &CoordinateImpl::dx3f)
       .def("divergence", &CoordinateImpl::divergence)
-      .def("forward", &CoordinateImpl::forward);
+      .def("forward", &CoordinateImpl::forward, py::arg("prim"), py::arg("flux1"),
+           py::arg("flux2"), py::arg("flux3"), py::arg("face_pressure1") = torch::Tensor());
+  // one module serves both geometries (options.type selects)
+  m.attr("SphericalPolar") = m.attr("Cartesian");
+  m.attr("Coordinate") = m.attr("Cartesian");
 
   py::class_<ImplicitHydroImpl, ImplicitHydro>(m, "ImplicitHydro")
       .def_readonly("options", &ImplicitHydroImpl::options)

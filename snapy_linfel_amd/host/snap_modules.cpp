@@ -181,16 +181,22 @@ CoordinateOptions CoordinateOptionsImpl::from_yaml(yaml::Node const& config) {
   auto node = config["geometry"];
   if (!node) return op;
   op->type(node["type"].as<std::string>("cartesian"));
-  TORCH_CHECK(op->type() == "cartesian",
-              "geometry.type '", op->type(), "' is not supported (Cartesian only)");
+  TORCH_CHECK(op->type() == "cartesian" || op->type() == "spherical-polar", "geometry.type '",
+              op->type(), "' is not supported (cartesian and spherical-polar only)");
   if (node["bounds"]) {
+    // src/coord/coordinate.cpp:72-101: x2/x3 bounds may be given in units of pi
     auto b = node["bounds"];
+    auto bound = [&](const char* key, double fallback) {
+      std::string pi_key = std::string(key) + "_pi";
+      if (b[pi_key].defined()) return b[pi_key].as<double>() * M_PI;
+      return b[key].as<double>(fallback);
+    };
     op->x1min() = b["x1min"].as<double>(0.0);
-    op->x2min() = b["x2min"].as<double>(0.0);
-    op->x3min() = b["x3min"].as<double>(0.0);
+    op->x2min() = bound("x2min", 0.0);
+    op->x3min() = bound("x3min", 0.0);
     op->x1max() = b["x1max"].as<double>(1.0);
-    op->x2max() = b["x2max"].as<double>(1.0);
-    op->x3max() = b["x3max"].as<double>(1.0);
+    op->x2max() = bound("x2max", 1.0);
+    op->x3max() = bound("x3max", 1.0);
   }
   if (!node["cells"]) return op;
   op->nx1() = node["cells"]["nx1"].as<int>(1);
@@ -291,6 +297,7 @@ snapx_config make_config(CoordinateOptions const& coord, HydroOptions const& hyd
   c.x1min = coord->x1min(), c.x1max = coord->x1max();
   c.x2min = coord->x2min(), c.x2max = coord->x2max();
   c.x3min = coord->x3min(), c.x3max = coord->x3max();
+  c.coord = coord->type() == "spherical-polar" ? SNAPX_COORD_SPHERICAL_POLAR : SNAPX_COORD_CARTESIAN;
   auto eos = hydro->eos() ? hydro->eos() : EquationOfStateOptionsImpl::create();
   TORCH_CHECK(eos->type() == "ideal-gas", "EquationOfState: type '", eos->type(),
               "' is not supported (ideal-gas only)");
@@ -431,17 +438,29 @@ RiemannSolver create_riemann_solver(RiemannSolverOptions const& opts, double gam
 
 CoordinateImpl::CoordinateImpl(CoordinateOptions options_) : options(std::move(options_)) {
   auto const& op = options;
-  auto axis = [&](int nx, double lo, double hi, int nc, torch::Tensor& xv, torch::Tensor& dxf) {
+  const bool sph = op->type() == "spherical-polar";
+  auto axis = [&](int nx, double lo, double hi, int nc, torch::Tensor& xf, torch::Tensor& xv,
+                  torch::Tensor& dxf) {
     double dx = (hi - lo) / nx;
     double xmin = nx > 1 ? lo - op->nghost() * dx : lo;
     double xmax = nx > 1 ? hi + op->nghost() * dx : hi;
-    auto xf = torch::linspace(xmin, xmax, nc + 1, torch::kFloat64);
+    xf = torch::linspace(xmin, xmax, nc + 1, torch::kFloat64);
     xv = 0.5 * (xf.slice(0, 0, nc) + xf.slice(0, 1, nc + 1));
-    dxf = torch::ones(nc, torch::kFloat64) * dx;
+    dxf = sph ? xf.slice(0, 1, nc + 1) - xf.slice(0, 0, nc) : torch::ones(nc, torch::kFloat64) * dx;
   };
-  axis(op->nx1(), op->x1min(), op->x1max(), op->nc1(), x1v, dx1f);
-  axis(op->nx2(), op->x2min(), op->x2max(), op->nc2(), x2v, dx2f);
-  axis(op->nx3(), op->x3min(), op->x3max(), op->nc3(), x3v, dx3f);
+  axis(op->nx1(), op->x1min(), op->x1max(), op->nc1(), x1f, x1v, dx1f);
+  axis(op->nx2(), op->x2min(), op->x2max(), op->nc2(), x2f, x2v, dx2f);
+  axis(op->nx3(), op->x3min(), op->x3max(), op->nc3(), x3f, x3v, dx3f);
+  if (sph) {
+    // volume-weighted centres (src/coord/spherical_polar.cpp:17-28, 60-67)
+    int nc1 = op->nc1(), nc2 = op->nc2();
+    auto rm = x1f.slice(0, 0, nc1), rp = x1f.slice(0, 1, nc1 + 1);
+    x1v = 0.75 * (rp.pow(4) - rm.pow(4)) / (rp.pow(3) - rm.pow(3));
+    if (nc2 > 1) {
+      auto tm = x2f.slice(0, 0, nc2), tp = x2f.slice(0, 1, nc2 + 1);
+      x2v = ((tp.sin() - tp * tp.cos()) - (tm.sin() - tm * tm.cos())) / (tm.cos() - tp.cos());
+    }
+  }
 }
 
 torch::Tensor CoordinateImpl::divergence(torch::Tensor f1, torch::Tensor f2,
@@ -461,9 +480,22 @@ torch::Tensor CoordinateImpl::divergence(torch::Tensor f1, torch::Tensor f2,
   return div;
 }
 
-torch::Tensor CoordinateImpl::forward(torch::Tensor, torch::Tensor f1, torch::Tensor f2,
-                                      torch::Tensor f3, torch::Tensor) {
-  return divergence(f1, f2, f3);
+torch::Tensor CoordinateImpl::forward(torch::Tensor prim, torch::Tensor f1, torch::Tensor f2,
+                                      torch::Tensor f3, torch::Tensor face_pressure1) {
+  torch::Tensor ref = f1.defined() ? f1 : (f2.defined() ? f2 : f3);
+  TORCH_CHECK(ref.defined(), "At least one flux tensor must be defined");
+  require_device(ref, "flux");
+  auto h = HydroOptionsImpl::create();
+  h->eos() = EquationOfStateOptionsImpl::create();
+  snapx_config c = make_config(options, h, {});
+  if (c.coord == SNAPX_COORD_CARTESIAN) return divergence(f1, f2, f3);
+  require_device(prim, "prim");
+  auto div = torch::empty_like(ref);
+  auto ptr = [](torch::Tensor const& t) { return t.defined() ? t.data_ptr() : nullptr; };
+  check(snapx_coord_forward(&c, prim.data_ptr(), ptr(f1), ptr(f2), ptr(f3), ptr(face_pressure1),
+                            div.data_ptr(), dtype_code(ref), stream_of(ref)),
+        "coord_forward");
+  return div;
 }
 
 ImplicitHydroImpl::ImplicitHydroImpl(ImplicitOptions options_, snapx_config cfg)

@@ -301,7 +301,9 @@ class CoordinateImpl {
   int ju() const { return options->nx2() > 1 ? options->nghost() + options->nx2() - 1 : 0; }
   int kl() const { return options->nx3() > 1 ? options->nghost() : 0; }
   int ku() const { return options->nx3() > 1 ? options->nghost() + options->nx3() - 1 : 0; }
-  torch::Tensor x1v, x2v, x3v, dx1f, dx2f, dx3f;  // FP64, host
+  // FP64, host: faces, centres (volume-weighted on spherical-polar blocks)
+  // and face widths
+  torch::Tensor x1f, x2f, x3f, x1v, x2v, x3v, dx1f, dx2f, dx3f;
   torch::Tensor divergence(torch::Tensor flux1, torch::Tensor flux2, torch::Tensor flux3) const;
   torch::Tensor forward(torch::Tensor prim, torch::Tensor flux1, torch::Tensor flux2,
                         torch::Tensor flux3, torch::Tensor face_pressure1 = torch::Tensor());

@@ -89,8 +89,8 @@ def test_unsupported_eos_and_geometry_are_errors():
               "geometry:\n  cells: {nx1: 8, nghost: 3}\n")
     with pytest.raises(RuntimeError, match="ideal-gas only"):
         host.MeshBlock(host.MeshBlockOptions.from_yaml(f))
-    g = _yaml("geometry:\n  type: spherical-polar\n")
-    with pytest.raises(RuntimeError, match="Cartesian only"):
+    g = _yaml("geometry:\n  type: gnomonic-equiangle\n")
+    with pytest.raises(RuntimeError, match="cartesian and spherical-polar only"):
         host.CoordinateOptions.from_yaml(g)
 
 
