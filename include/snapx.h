@@ -27,8 +27,9 @@
  *    (r, theta, phi) on the plain FP64/FP32 path: faces uniform in each
  *    coordinate as torch::linspace places them (src/coord/coordinate.cpp:
  *    201-221), with the centres, face areas, volumes and source terms of
- *    src/coord/spherical_polar.cpp.  The fused path, batches and the adjoint
- *    are Cartesian only.
+ *    src/coord/spherical_polar.cpp.  The fused path takes spherical-polar
+ *    blocks with columns of at most 256 (3x3 blocks) / 128 (5x5) cells;
+ *    batches and the adjoint are Cartesian only.
  */
 #ifndef SNAPX_H
 #define SNAPX_H

@@ -32,6 +32,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "geom_impl.hpp"
 #include "interp_impl.hpp"
 #include "riemann_impl.hpp"
 #include "vic_impl.hpp"
@@ -73,6 +74,15 @@ struct IsentropicRefill {
     p = h * H;
     rho = rho_b * H * inv_P;
   }
+  // layers of uneven thickness (spherical-polar x1 centres): built with
+  // gdz = -grav1 (per unit length), z is the summed distance of the layer
+  // from the boundary cell, sum_m dx1v of the layers in between
+  __device__ __forceinline__ void layer_at(float z, float& rho, float& p) const {
+    float h = (p_b + z * step) * inv_P;
+    float H = powf(h, inv_gm);
+    p = h * H;
+    rho = rho_b * H * inv_P;
+  }
 };
 
 struct FusedArgs {
@@ -86,6 +96,10 @@ struct FusedArgs {
   int bc[6];
   int merged;  // faces of all directions in one pass (small tiles)
   float gammad, dx1, dx2, dx3, dt, grav1, grav2, grav3;
+  // spherical-polar blocks: summed ghost-layer distances below / above
+  // (dx1v[is - 1], + dx1v[is - 2], ...; dx1v[ie], + dx1v[ie + 1], ...)
+  float zin[3], zout[3];
+  Geometry geo;
 };
 
 // Maps an interior-relative index x (may lie in the ghost range) of an axis
@@ -151,6 +165,83 @@ __device__ __forceinline__ void cons_to_prim(const T* U, T gammad, T* W) {
   W[IPR] = (gammad - T(1.)) * (U[IPR] - ke);
 }
 
+// Flux differences, geometric sources and du of the tile's cells on a
+// spherical-polar block (merged face pass): the per-face areas and cell
+// volumes of the coordinate, then SphericalPolarImpl::forward
+// (src/coord/spherical_polar.cpp:212-279) with the x1 face pressure, as the
+// plain path's coord_forward_kernel does per cell.  fs holds the fluxes of all
+// directions ([5][faces]) and, after them, the x1 face pressures.
+__device__ __forceinline__ void sph_cell_du(const FusedArgs& a, const float* ws,
+                                            const float* fs, int ftot, int fo1, int fo2, int o1,
+                                            int o2, int o3, int e1, int e2, int e3,
+                                            float* __restrict__ du) {
+  const Geometry& G = a.geo;
+  const int H1 = a.tx1 + 6, H2 = a.tx2 + 2 * a.h2, H = H1 * H2 * (a.tx3 + 2 * a.h3);
+  const int ncell = a.n1 * a.n2 * a.n3;
+  const bool act2 = a.n2 > 1, act3 = a.n3 > 1;
+  const float* fp = fs + 5 * ftot;
+  const DivMod de1(e1), de2(e2);
+  for (int cid = threadIdx.x; cid < e1 * e2 * e3; cid += blockDim.x) {
+    int cx, cy;
+    int cz = de2.div(de1.div(cid, cx), cy);
+    const int i = o1 + cx + a.ng;
+    const int j = act2 ? o2 + cy + a.ng : 0;
+    const int k = act3 ? o3 + cz + a.ng : 0;
+    const float vol = (float)sph_volume(G, k, j, i);
+    const float A1m = (float)sph_area1(G, k, j, i), A1p = (float)sph_area1(G, k, j, i + 1);
+    const int lo1 = (cz * e2 + cy) * (e1 + 1) + cx;
+    const int lo2 = fo1 + (cz * (e2 + 1) + cy) * e1 + cx;
+    const int lo3 = fo2 + (cz * e2 + cy) * e1 + cx;
+    float A2m = 0.f, A2p = 0.f, A3 = 0.f;
+    if (act2) A2m = (float)sph_area2(G, k, j, i), A2p = (float)sph_area2(G, k, j + 1, i);
+    if (act3) A3 = (float)sph_area3(G, j, i);
+    float div[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      float acc = A1p * fs[v * ftot + lo1 + 1] - A1m * fs[v * ftot + lo1];
+      if (act2) acc += A2p * fs[v * ftot + lo2 + e1] - A2m * fs[v * ftot + lo2];
+      if (act3) acc += A3 * fs[v * ftot + lo3 + e1 * e2] - A3 * fs[v * ftot + lo3];
+      div[v] = acc / vol;
+    }
+    const int lin = ((cz + a.h3) * H2 + cy + a.h2) * H1 + cx + 3;
+    const float rho = ws[IDN * H + lin], v1 = ws[IVX * H + lin], v2 = ws[IVY * H + lin],
+                v3 = ws[IVZ * H + lin], pre = ws[IPR * H + lin];
+    const float s1i = (float)sph_src1_i(G, i), s1j = (float)sph_src1_j(G, j);
+    const float s2i = (float)sph_src2_i(G, i);
+    const float fpm = fp[lo1], fpp = fp[lo1 + 1];
+    float src1 = s1i * rho * (v2 * v2 + v3 * v3);
+    src1 += (A1p * fpp - A1m * fpm) / vol - (fpp - fpm) / (float)axis_width(G.ax[0], i);
+    div[IVX] -= src1;
+    div[IVY] -= s1i * s1j * (rho * (v3 * v3) + pre);
+    div[IVY] += s2i * (A1m * fs[IVY * ftot + lo1] + A1p * fs[IVY * ftot + lo1 + 1]);
+    div[IVZ] += s2i * (A1m * fs[IVZ * ftot + lo1] + A1p * fs[IVZ * ftot + lo1 + 1]);
+    if (act2)
+      div[IVZ] += s1i * (float)sph_src2_j(G, j) *
+                  (A2m * fs[IVZ * ftot + lo2] + A2p * fs[IVZ * ftot + lo2 + e1]);
+    else
+      div[IVZ] += s1i * s1j * (rho * v3 * v2);
+    const float dt = a.dt;
+    float d[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) d[v] = -dt * div[v];
+    if (a.grav1 != 0.f) {
+      d[IVX] += dt * rho * a.grav1;
+      d[IPR] += dt * rho * v1 * a.grav1;
+    }
+    if (a.grav2 != 0.f) {
+      d[IVY] += dt * rho * a.grav2;
+      d[IPR] += dt * rho * v2 * a.grav2;
+    }
+    if (a.grav3 != 0.f) {
+      d[IVZ] += dt * rho * a.grav3;
+      d[IPR] += dt * rho * v3 * a.grav3;
+    }
+    const int q = (k * a.n2 + j) * a.n1 + i;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) du[v * ncell + q] = d[v];
+  }
+}
+
 // K1 body for one tile; every thread of the workgroup calls it with the same
 // tile (it synchronises the workgroup internally).  The arguments come by
 // value: a reference to the kernel argument makes the compiler keep a copy
@@ -158,7 +249,7 @@ __device__ __forceinline__ void cons_to_prim(const T* U, T gammad, T* W) {
 // kMerged: 1 / 0 when the launch knows that every tile takes the merged
 // face pass / the per-direction passes (the other path is not compiled in),
 // -1 to follow a.merged at run time (batches of mixed members).
-template <int kMerged>
+template <int kMerged, bool kSph = false>
 __device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __restrict__ du,
                                          const FusedArgs a, int b, float* smem) {
   const bool merged = kMerged < 0 ? (a.merged != 0) : kMerged == 1;
@@ -209,8 +300,13 @@ __device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __r
 #pragma unroll
       for (int v = 0; v < 5; ++v) Ub[v] = u[v * ncell + qb];
       cons_to_prim(Ub, gammad, Wb);  // rho and p do not depend on the flips
-      const IsentropicRefill R(Wb[IDN], Wb[IPR], gammad, gdz, upper ? 1 : -1);
-      R.layer(upper ? gi - a.nx1 + 1 : -gi, W[IDN], W[IPR]);
+      if (kSph) {
+        const IsentropicRefill R(Wb[IDN], Wb[IPR], gammad, -a.grav1, upper ? 1 : -1);
+        R.layer_at(upper ? a.zout[gi - a.nx1] : a.zin[-gi - 1], W[IDN], W[IPR]);
+      } else {
+        const IsentropicRefill R(Wb[IDN], Wb[IPR], gammad, gdz, upper ? 1 : -1);
+        R.layer(upper ? gi - a.nx1 + 1 : -gi, W[IDN], W[IPR]);
+      }
     }
 #pragma unroll
     for (int v = 0; v < 5; ++v) ws[v * H + l] = W[v];
@@ -306,13 +402,24 @@ __device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __r
         }
       }
       float flx[5];
-      riemann_flux<float>(solver, wl, wr, gammad, 3, flx, (float*)nullptr);
+      if (kSph) {
+        // the x1 face pressure feeds the radial momentum source
+        float pf;
+        riemann_flux<float>(solver, wl, wr, gammad, 3, flx, &pf);
+        if (d == 0) fs[5 * ftot + g] = pf;
+      } else {
+        riemann_flux<float>(solver, wl, wr, gammad, 3, flx, (float*)nullptr);
+      }
   #pragma unroll
       for (int v = 0; v < 5; ++v) fs[vmap[v] * ftot + g] = flx[v];
       (void)f1;
       (void)f2;
     }
     __syncthreads();
+    if (kSph) {
+      sph_cell_du(a, ws, fs, ftot, fo1, fo2, o1, o2, o3, e1, e2, e3, du);
+      return;
+    }
     // flux differences, directions in order (x1, x2, x3)
   #pragma unroll
     for (int rr = 0; rr < kMaxCellsPerThread; ++rr) {
@@ -457,6 +564,13 @@ __global__ void __launch_bounds__(kRhsThreads)
     fused_rhs_kernel(const float* __restrict__ u, float* __restrict__ du, FusedArgs a) {
   extern __shared__ float smem[];
   rhs_tile<kMerged>(u, du, a, blockIdx.x, smem);
+}
+
+// K1 of spherical-polar blocks (merged face pass, geometric sources)
+__global__ void __launch_bounds__(kRhsThreads)
+    fused_rhs_sph_kernel(const float* __restrict__ u, float* __restrict__ du, FusedArgs a) {
+  extern __shared__ float smem[];
+  rhs_tile<1, true>(u, du, a, blockIdx.x, smem);
 }
 
 // ------------------------------------------------ K1 for 3-D blocks (2.5-D) ----
@@ -673,6 +787,10 @@ struct ColArgs {
   int rows2;          // two rows per lane (N = 3 columns of more than 64 cells)
   double gammad, dx1, area, vol, dt, grav1;
   double w0, w1, w2;  // stage weights
+  // spherical-polar blocks: the first ghost layer's distance below / above
+  // (dx1v[is - 1], dx1v[ie]) and the coordinate
+  double zin1, zout1;
+  Geometry geo;
 };
 
 // primitive state (FP64) of interior-relative layer i of a column whose
@@ -702,12 +820,19 @@ __device__ __forceinline__ void float_prim(const float* Uf, bool flip, double ga
 // +1: above; src/hydro/hydro.cpp:231-269, first ghost), with the FP32
 // arithmetic of the refill in the RHS kernel, which computes the fluxes of the
 // same stage from these ghost values.
+template <bool kSph = false>
 __device__ __forceinline__ void refill_ghost(const double* wb, double* wg, const ColArgs& a,
                                              int dir) {
-  const IsentropicRefill R((float)wb[IDN], (float)wb[IPR], (float)a.gammad,
-                           -(float)a.grav1 * (float)a.dx1, dir);
   float rho, pre;
-  R.layer(1, rho, pre);
+  if (kSph) {
+    const IsentropicRefill R((float)wb[IDN], (float)wb[IPR], (float)a.gammad, -(float)a.grav1,
+                             dir);
+    R.layer_at((float)(dir < 0 ? a.zin1 : a.zout1), rho, pre);
+  } else {
+    const IsentropicRefill R((float)wb[IDN], (float)wb[IPR], (float)a.gammad,
+                             -(float)a.grav1 * (float)a.dx1, dir);
+    R.layer(1, rho, pre);
+  }
   wg[IPR] = (double)pre;
   wg[IDN] = (double)rho;
 }
@@ -1227,7 +1352,9 @@ __device__ __forceinline__ void column_pcr2(float* __restrict__ u, float* __rest
     finish_cell(u, u0, a, ncell, q1, N, d1, w1[IDN], w1[IDN] * a.vol, s_a2, s_b, Dv1, Uc1, Ub1);
 }
 
-template <int N>
+// kSph: spherical-polar blocks, per-layer face areas and volumes
+// (vic_assemble_cell with area_i != area_i+1) and the uneven ghost refill
+template <int N, bool kSph = false>
 __device__ __forceinline__ void column_pcr(float* __restrict__ u, float* __restrict__ u0,
                                            const float* __restrict__ du, const ColArgs& a,
                                            int col, double* __restrict__ sh) {
@@ -1289,14 +1416,14 @@ __device__ __forceinline__ void column_pcr(float* __restrict__ u, float* __restr
         for (int v = 0; v < 5; ++v) wm[v] = sh[v * T + t - 1];
       } else {
         float_prim(Um, flm, a.gammad, wm);
-        refill_ghost(wc, wm, a, -1);
+        refill_ghost<kSph>(wc, wm, a, -1);
       }
       if (i < nl - 1) {
 #pragma unroll
         for (int v = 0; v < 5; ++v) wp[v] = sh[v * T + t + 1];
       } else {
         float_prim(Up, flp, a.gammad, wp);
-        refill_ghost(wc, wp, a, +1);
+        refill_ghost<kSph>(wc, wp, a, +1);
       }
     }
     column_barrier(wave);
@@ -1318,21 +1445,34 @@ __device__ __forceinline__ void column_pcr(float* __restrict__ u, float* __restr
     column_barrier(wave);
     // ---- block row i (vic_assemble_cell; uniform Cartesian areas) ----
     double A[NN], B[NN], C[NN], r[N];
+    // face areas below / above the layer and its volume
+    double ar_i = a.area, ar_p = a.area, vol_i = a.vol;
+    if (kSph) {
+      const int jj = a.n2 > 1 ? a.ng + col % a.ncol2 : 0;
+      const int kk = a.n3 > 1 ? a.ng + col / a.ncol2 : 0;
+      ar_i = sph_area1(a.geo, kk, jj, a.ng + i);
+      ar_p = sph_area1(a.geo, kk, jj, a.ng + i + 1);
+      vol_i = sph_volume(a.geo, kk, jj, a.ng + i);
+    }
     {
-      double Jp[NN], Jn[NN];
+      double Jp[NN], Jn[NN], Jc[NN];
       flux_jacobian<double, N>(gm1, wm, Jp);
       flux_jacobian<double, N>(gm1, wp, Jn);
-      const double hv = 0.5 * rcp_fast(a.vol);
+      if (kSph) flux_jacobian<double, N>(gm1, wc, Jc);
+      const double hv = 0.5 * rcp_fast(vol_i);
 #pragma unroll
       for (int rr = 0; rr < N; ++rr)
 #pragma unroll
         for (int c = 0; c < N; ++c) {
           int o = rr * N + c;
           double phi = ((rr == 1 && c == 0) || (vic_row(N, rr) == IPR && c == 1)) ? a.grav1 : 0.;
-          A[o] = (Am[o] * a.area + Ap[o] * a.area) * hv - phi;
+          if (kSph)
+            A[o] = (Am[o] * ar_i + Ap[o] * ar_p + (ar_p - ar_i) * Jc[o]) * hv - phi;
+          else
+            A[o] = (Am[o] * ar_i + Ap[o] * ar_p) * hv - phi;
           if (rr == c) A[o] += idt;
-          B[o] = -(Am[o] + Jp[o]) * a.area * hv;
-          C[o] = -(Ap[o] - Jn[o]) * a.area * hv;
+          B[o] = -(Am[o] + Jp[o]) * ar_i * hv;
+          C[o] = -(Ap[o] - Jn[o]) * ar_p * hv;
         }
       // reflecting column ends fold the ghost coupling into the diagonal
 #pragma unroll
@@ -1374,8 +1514,8 @@ __device__ __forceinline__ void column_pcr(float* __restrict__ u, float* __restr
     double d[N];
     pcr_solve<N, true>(A, B, C, r, d, sh, T, t, i, nl, act, wave);
     // ---- MS-VIC column sums ----
-    double ai = wc[IDN] * a.vol;
-    double sums[2] = {act ? ai * ai : 0., act ? (dd[IDN] - d[0]) * 1.0 * a.vol : 0.};
+    double ai = wc[IDN] * vol_i;
+    double sums[2] = {act ? ai * ai : 0., act ? (dd[IDN] - d[0]) * 1.0 * vol_i : 0.};
     column_sums<2>(sums, sh, t, a.lanes, a.lshift);
     const double s_a2 = sums[0], s_b = sums[1];
     double rho = wc[IDN];
@@ -1412,6 +1552,16 @@ __global__ void __launch_bounds__(PcrShape<N>::kThreads)
   extern __shared__ double sh[];
   int col = blockIdx.x * (blockDim.x >> a.lshift) + (threadIdx.x >> a.lshift);
   column_pcr<N>(u, u0, du, a, col, sh);
+}
+
+// K2 of spherical-polar blocks
+template <int N>
+__global__ void __launch_bounds__(PcrShape<N>::kThreads)
+    fused_column_pcr_sph_kernel(float* __restrict__ u, float* __restrict__ u0,
+                                const float* __restrict__ du, ColArgs a) {
+  extern __shared__ double sh[];
+  int col = blockIdx.x * (blockDim.x >> a.lshift) + (threadIdx.x >> a.lshift);
+  column_pcr<N, true>(u, u0, du, a, col, sh);
 }
 
 __global__ void __launch_bounds__(PcrShape<3>::kThreads, 2)
@@ -1572,13 +1722,19 @@ static size_t rhs_face_count(const FusedArgs& a, bool all) {
 
 static size_t rhs_smem_bytes(const FusedArgs& a) {
   size_t H = (size_t)(a.tx1 + 6) * (a.tx2 + 2 * a.h2) * (a.tx3 + 2 * a.h3);
-  return sizeof(float) * 5 * (H + rhs_face_count(a, a.merged));
+  // spherical-polar tiles keep the x1 face pressures after the fluxes
+  size_t fp = a.geo.coord == SNAPX_COORD_SPHERICAL_POLAR ? (size_t)(a.tx1 + 1) * a.tx2 * a.tx3 : 0;
+  return sizeof(float) * (5 * (H + rhs_face_count(a, a.merged)) + fp);
 }
 
 static int check_fused(const snapx_config* c) {
   if (int e = validate_config(c)) return e;
-  SNAPX_CHECK_ARG(c->coord == SNAPX_COORD_CARTESIAN,
-                  "fused path: Cartesian geometry only (use the plain path)");
+  if (c->coord == SNAPX_COORD_SPHERICAL_POLAR) {
+    // spherical K2 is the cell-parallel column kernel only
+    const int N = ((c->implicit_scheme >> 3) & 1) ? 5 : 3;
+    SNAPX_CHECK_ARG(c->nx1 <= (N == 3 ? PcrShape<3>::kThreads : PcrShape<5>::kThreads),
+                    "fused path: spherical-polar columns of at most 256 (3x3) / 128 (5x5) cells");
+  }
   SNAPX_CHECK_ARG(c->nghost == 3, "fused path: nghost must be 3");
   SNAPX_CHECK_ARG(c->nx1 > 1, "fused path: x1 must be active");
   SNAPX_CHECK_ARG(!c->limiter, "fused path: the EOS limiter is not supported (use the plain path)");
@@ -1626,8 +1782,18 @@ static FusedArgs fill_fused_args(const snapx_config* cfg, Tile t, double dt) {
   a.dx1 = (float)g.dx1, a.dx2 = (float)g.dx2, a.dx3 = (float)g.dx3;
   a.dt = (float)dt;
   a.grav1 = (float)cfg->grav1, a.grav2 = (float)cfg->grav2, a.grav3 = (float)cfg->grav3;
-  // one pass over all faces when they number at most two per thread
-  a.merged = rhs_face_count(a, true) <= 2 * (size_t)kRhsThreads;
+  a.geo = make_geometry(cfg);
+  const bool sph = cfg->coord == SNAPX_COORD_SPHERICAL_POLAR;
+  // ghost-layer distances of the refill: dx1v of the layers crossed
+  double zi = 0., zo = 0.;
+  for (int n = 0; n < 3; ++n) {
+    zi += sph ? sph_dx1v(a.geo, g.ng - n - 1) : g.dx1;
+    zo += sph ? sph_dx1v(a.geo, g.n1 - g.ng - 1 + n) : g.dx1;
+    a.zin[n] = (float)zi, a.zout[n] = (float)zo;
+  }
+  // one pass over all faces when they number at most two per thread;
+  // spherical-polar tiles always take it (their sources read every face)
+  a.merged = sph || rhs_face_count(a, true) <= 2 * (size_t)kRhsThreads;
   return a;
 }
 
@@ -1646,7 +1812,9 @@ static int make_fused_args(const snapx_config* cfg, Tile t, double dt, FusedArgs
 // up to kMarchSlab slices along x3.
 constexpr int kMarchSlab = 16;  // measured best of 4 .. 128 on a 256 x 256 x 128 block
 
-static bool march_ok(const snapx_config* cfg) { return cfg->nx3 > 1 && cfg->nx2 > 1; }
+static bool march_ok(const snapx_config* cfg) {
+  return cfg->nx3 > 1 && cfg->nx2 > 1 && cfg->coord == SNAPX_COORD_CARTESIAN;
+}
 
 static size_t march_smem_bytes(const FusedArgs& a) {
   size_t HS = (size_t)(a.tx1 + 6) * (a.tx2 + 6);
@@ -1687,8 +1855,13 @@ static int make_col_args(const snapx_config* cfg, double dt, ColArgs* out) {
   c.w0 = 0., c.w1 = 1., c.w2 = 1.;
   c.lanes = 64, c.lshift = 6;
   while (c.lanes < c.nl) c.lanes <<= 1, c.lshift += 1;
-  // 3x3 columns of more than 64 cells: two rows per lane (column_pcr2)
-  c.rows2 = (!((cfg->implicit_scheme >> 3) & 1) && c.lanes >= 128) ? 1 : 0;
+  c.geo = make_geometry(cfg);
+  const bool sph = cfg->coord == SNAPX_COORD_SPHERICAL_POLAR;
+  c.zin1 = sph ? sph_dx1v(c.geo, g.ng - 1) : g.dx1;
+  c.zout1 = sph ? sph_dx1v(c.geo, g.n1 - g.ng - 1) : g.dx1;
+  // 3x3 columns of more than 64 cells: two rows per lane (column_pcr2;
+  // Cartesian blocks)
+  c.rows2 = (!sph && !((cfg->implicit_scheme >> 3) & 1) && c.lanes >= 128) ? 1 : 0;
   if (c.rows2) c.lanes >>= 1, c.lshift -= 1;
   SNAPX_CHECK_ARG(!c.scheme || c.grav1 != 0.,
                   "[ImplicitHydro] forcing does not have const-gravity");
@@ -1736,11 +1909,15 @@ static int fused_step(const snapx_config* cfg, double dt, float* u, void* worksp
   double* scr = reinterpret_cast<double*>(base + 2 * field);
 
   const bool march = march_ok(cfg);
+  const bool sph = cfg->coord == SNAPX_COORD_SPHERICAL_POLAR;
   FusedArgs a;
   if (march) {
     if (int e = make_march_args(cfg, dt, &a)) return e;
   } else {
-    if (int e = make_fused_args(cfg, choose_tile(g), dt, &a)) return e;
+    Tile t = choose_tile(g);
+    // spherical 3-D tiles: thinner in x3, since every face stays in LDS
+    if (sph && g.n3 > 1) t = {16, 4, 2};
+    if (int e = make_fused_args(cfg, t, dt, &a)) return e;
   }
   size_t smem = march ? march_smem_bytes(a) : rhs_smem_bytes(a);
   int nblk = a.nt1 * a.nt2 * a.nt3;
@@ -1755,7 +1932,10 @@ static int fused_step(const snapx_config* cfg, double dt, float* u, void* worksp
   // the kernel dispatch itself (hipExtLaunchKernel), not extra stream markers
   for (int s = 0; s < 3; ++s) {
     hipEvent_t ra = ev ? ev[2 * s].a : nullptr, rb = ev ? ev[2 * s].b : nullptr;
-    if (march)
+    if (sph)
+      hipExtLaunchKernelGGL(fused_rhs_sph_kernel, dim3(nblk), dim3(kRhsThreads), (uint32_t)smem,
+                            st, ra, rb, 0u, (const float*)u, du, a);
+    else if (march)
       hipExtLaunchKernelGGL(fused_rhs_march_kernel, dim3(nblk), dim3(kRhsThreads),
                             (uint32_t)smem, st, ra, rb, 0u, (const float*)u, du, a);
     else if (a.merged)
@@ -1773,7 +1953,13 @@ static int fused_step(const snapx_config* cfg, double dt, float* u, void* worksp
       dim3 grid((c.ncol + cpw - 1) / cpw), block(threads);
       const uint32_t smem = (uint32_t)(sizeof(double) * threads *
                                        (N == 3 ? PcrShape<3>::kPub : PcrShape<5>::kPub));
-      if (N == 3 && c.rows2)
+      if (sph && N == 3)
+        hipExtLaunchKernelGGL(fused_column_pcr_sph_kernel<3>, grid, block, smem, st, ca, cb, 0u, u,
+                              u0, (const float*)du, c);
+      else if (sph)
+        hipExtLaunchKernelGGL(fused_column_pcr_sph_kernel<5>, grid, block, smem, st, ca, cb, 0u, u,
+                              u0, (const float*)du, c);
+      else if (N == 3 && c.rows2)
         hipExtLaunchKernelGGL(fused_column_pcr2_kernel, grid, block, smem, st, ca, cb, 0u, u, u0,
                               (const float*)du, c);
       else if (N == 3)
@@ -1882,6 +2068,8 @@ static int batch_layout(const snapx_config* members, int nm, double dt, int tx1,
   bool three_d = false, two_d = false;
   for (int m = 0; m < nm; ++m) {
     if (int e = check_fused(&members[m])) return e;
+    SNAPX_CHECK_ARG(members[m].coord == SNAPX_COORD_CARTESIAN,
+                    "batch: Cartesian members only (spherical-polar blocks step one at a time)");
     SNAPX_CHECK_ARG(members[m].implicit_scheme == scheme,
                     "batch: every member needs the same implicit scheme");
     three_d |= members[m].nx3 > 1;

@@ -52,9 +52,10 @@ def test_workspace_queries_without_device():
 
 
 def test_spherical_polar_limits_without_device():
-    """Spherical-polar blocks run on the plain path; the fused path, batches
-    and the adjoint refuse them, as do blocks without a radial axis or with
-    theta outside [0, pi] (src/coord/spherical_polar.cpp:46-48)."""
+    """Spherical-polar blocks run on the plain and fused paths (fused columns
+    up to one column workgroup); batches and the adjoint refuse them, as do
+    blocks without a radial axis or with theta outside [0, pi]
+    (src/coord/spherical_polar.cpp:46-48)."""
     L = capi.lib()
     L.snapx_last_error.restype = ctypes.c_char_p
     kw = dict(nx1=16, nx2=8, nx3=8, nghost=3, x1min=1.0, x1max=2.0, x2min=0.7, x2max=2.4,
@@ -62,8 +63,10 @@ def test_spherical_polar_limits_without_device():
     c = capi.make_config(**kw)
     assert L.snapx_hydro_workspace_bytes(ctypes.byref(c), capi.F64) > 0
     assert L.snapx_rk3_workspace_bytes(ctypes.byref(c), capi.F64) > 0
-    assert L.snapx_fused_workspace_bytes(ctypes.byref(c)) == 0
-    assert b"Cartesian geometry only" in L.snapx_last_error()
+    assert L.snapx_fused_workspace_bytes(ctypes.byref(c)) > 0
+    tall = capi.make_config(**dict(kw, nx1=300))
+    assert L.snapx_fused_workspace_bytes(ctypes.byref(tall)) == 0
+    assert b"spherical-polar columns of at most" in L.snapx_last_error()
     assert L.snapx_adjoint_workspace_bytes(ctypes.byref(c)) == 0
     for bad in (dict(kw, x2max=3.3), dict(kw, nx1=1)):
         assert L.snapx_hydro_workspace_bytes(ctypes.byref(capi.make_config(**bad)), capi.F64) == 0
