@@ -44,6 +44,71 @@ def straka_cfg(capi, nx2=512, nx1=64):
                                 "periodic", "periodic"))
 
 
+def straka_prim(cfg, R=287.0, cp=1004.0, p0=1e5, Ts=300.0, dT=-15.0, xc=0.0, xr=4.0e3,
+                zc=3.0e3, zr=2.0e3, grav=9.81):
+    """Primitive initial state of examples/straka.cpp:53-106 on the block of
+    cfg, ghost cells included: a neutral atmosphere (T = Ts - g z / cp,
+    p = p0 (T / Ts)^(cp / R)) with a -15 K cosine bubble at (xc, zc)."""
+    import numpy as np
+    n = cfg.shape
+    g = cfg.nghost
+    dx1 = (cfg.x1max - cfg.x1min) / cfg.nx1
+    dx2 = (cfg.x2max - cfg.x2min) / cfg.nx2
+    z = cfg.x1min + (np.arange(n[3]) - g + 0.5) * dx1
+    x = cfg.x2min + (np.arange(n[2]) - g + 0.5) * dx2
+    X, Z = np.meshgrid(x, z, indexing="ij")
+    L = np.sqrt(((X - xc) / xr) ** 2 + ((Z - zc) / zr) ** 2)
+    T = Ts - grav * Z / cp
+    w = np.zeros(n)
+    w[4, 0] = p0 * np.power(T / Ts, cp / R)
+    T = T + np.where(L <= 1, dT * (np.cos(L * np.pi) + 1.0) / 2.0, 0.0)
+    w[0, 0] = w[4, 0] / (R * T)
+    return w
+
+
+def conserved_on_device(capi, cfg, w, dev):
+    """W -> U and the boundary functions, on the GPU (the library's own
+    initialisation path); returns the FP64 device state."""
+    import torch
+    wd = torch.from_numpy(w).to(dev, torch.float64)
+    ud = torch.empty_like(wd)
+    capi.prim2cons(cfg, wd, ud)
+    capi.apply_bc(cfg, ud)
+    return ud
+
+
+def sector_cfg(capi, nx1=64, nx2=512):
+    """Spherical-polar shell sector r in [1, 2], theta in [pi/4, 3pi/4] (one
+    azimuthal cell), nondimensional: gamma 1.4, gravity -0.5, WENO5 + LMARS,
+    implicit 3x3 radial solve, reflecting walls."""
+    import math
+    return capi.make_config(nx1=nx1, nx2=nx2, nghost=3, x1min=1.0, x1max=2.0,
+                            x2min=0.25 * math.pi, x2max=0.75 * math.pi, coord="spherical-polar",
+                            gammad=1.4, recon_vertical=("weno5", False, False), riemann="lmars",
+                            grav1=-0.5, implicit_scheme=1,
+                            bc=("reflecting", "reflecting", "reflecting", "reflecting",
+                                "periodic", "periodic"))
+
+
+def sector_prim(cfg):
+    """Isothermal atmosphere (R T = 1, p = exp(-0.5 (r - 1))) with a warm
+    Gaussian bubble (density 5 % low) at r = 1.3, theta = pi/2."""
+    import numpy as np
+    n = cfg.shape
+    g = cfg.nghost
+    dr = (cfg.x1max - cfg.x1min) / cfg.nx1
+    dth = (cfg.x2max - cfg.x2min) / cfg.nx2
+    r = cfg.x1min + (np.arange(n[3]) - g + 0.5) * dr
+    th = cfg.x2min + (np.arange(n[2]) - g + 0.5) * dth
+    TH, RR = np.meshgrid(th, r, indexing="ij")
+    p = np.exp(-0.5 * (RR - 1.0))
+    bub = np.exp(-(((RR - 1.3) / 0.1) ** 2 + ((TH - 0.5 * np.pi) / 0.1) ** 2))
+    w = np.zeros(n)
+    w[4, 0] = p
+    w[0, 0] = p * (1.0 - 0.05 * bub)
+    return w
+
+
 def box3d_cfg(capi, n=(128, 256, 256)):
     return capi.make_config(nx1=n[0], nx2=n[1], nx3=n[2], nghost=3, x1min=0.0,
                             x1max=100.0 * n[0], x2min=0.0, x2max=100.0 * n[1], x3min=0.0,
@@ -225,7 +290,9 @@ def measured_traffic(kernel, workload, what="traffic"):
     return None
 
 
-def errors(got, ref, interior):
+def errors(got, ref, interior, vel_scale=300.0):
+    """Per conserved variable: Linf of the difference over max |ref| (momenta:
+    at least max(rho) * vel_scale), and the relative L2 norm."""
     import numpy as np
     out = {}
     names = ["rho", "mom1", "mom2", "mom3", "E"]
@@ -234,7 +301,7 @@ def errors(got, ref, interior):
         g, r = got[v][interior], ref[v][interior]
         if np.abs(r).max() == 0:
             continue
-        scale = max(np.abs(r).max(), rho_max * 300.0 if v in (1, 2, 3) else 0.0)
+        scale = max(np.abs(r).max(), rho_max * vel_scale if v in (1, 2, 3) else 0.0)
         out[name] = {
             "linf_rel": float(np.abs(g - r).max() / scale),
             "l2_rel": float(np.linalg.norm(g - r) / max(np.linalg.norm(r), 1e-300)),
@@ -289,6 +356,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-3d", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-sph", action="store_true", help="skip the spherical-polar sector")
     ap.add_argument("--no-c5", action="store_true", help="skip the 200-step sensitivity run")
     ap.add_argument("--c5-directions", type=int, default=2,
                     help="random directions of the finite-difference check of the gradient")
@@ -301,13 +369,11 @@ def main():
     import numpy as np
     import torch
     from snapy_linfel_amd import capi
-    import oracle as O
 
     dev = torch.device("cuda:0")
     cfg = straka_cfg(capi)
-    oc = O.straka_config(nx2=512, nx1=64, implicit_scheme=1)
-    w0, u0 = O.initialize(oc, O.straka_init(oc))
-    I = O.interior(oc)[1:]
+    u0 = conserved_on_device(capi, cfg, straka_prim(cfg), dev).cpu().numpy()
+    I = (slice(None), slice(3, -3), slice(3, -3))
     dt = 0.1
     K, W = args.steps, args.warmup
 
@@ -466,6 +532,46 @@ def main():
                     (fl[0] + fl[1]) / (ms * 1e-3) / 1e12, 3)
                 extra["box3d_256x256x128_fp32"][f"{key}_valu_flop_frac"] = round(
                     (fl[0] + fl[1]) / (ms * 1e-3) / VALU_FLOP_PEAK, 4)
+
+    # ---- spherical-polar shell sector (64 radial x 512 polar cells):
+    #      FP32 fused against FP64 plain on the same 100 steps ----
+    if not args.no_sph:
+        cs = sector_cfg(capi)
+        us64 = conserved_on_device(capi, cs, sector_prim(cs), dev)
+        us32 = us64.float()
+        s0 = us64.clone()
+        dts, ks = 0.002, 100  # x2 acoustic CFL ~0.8 at r = 1
+        wss = capi.fused_workspace(cs)
+        capi.fused_rk3_steps(cs, dts, 2, us32.clone(), wss)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        capi.fused_rk3_steps(cs, dts, ks, us32, wss)
+        torch.cuda.synchronize()
+        ts32 = time.perf_counter() - t0
+        tims = capi.Timing()
+        capi.fused_rk3_steps(cs, dts, 3, us32.clone(), wss, timing=tims)
+        ws64s = capi.rk3_workspace(cs, capi.F64)
+        wt = torch.empty_like(us64)
+        capi.rk3_step(cs, dts, us64.clone(), wt, ws64s)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(ks):
+            capi.rk3_step(cs, dts, us64, wt, ws64s)
+        torch.cuda.synchronize()
+        ts64 = time.perf_counter() - t0
+        Is = (slice(None), slice(3, -3), slice(3, -3))
+        es = errors(us32.double().cpu().numpy(), us64.cpu().numpy(), Is, vel_scale=1.0)
+        extra["spherical_sector_64x512_fp32"] = {
+            "cell_updates_per_s": round(cs.n_interior * ks / ts32, 1), "steps": ks, "dt": dts,
+            "ms_per_step": round(ts32 / ks * 1e3, 4),
+            "fp64_plain_cell_updates_per_s": round(cs.n_interior * ks / ts64, 1),
+            "fused_over_fp64_plain": round(ts64 / ts32, 3),
+            "fused_rhs_avg_ms": round(tims.rhs_ms / max(tims.rhs_launches, 1), 5),
+            "fused_column_avg_ms": round(tims.column_ms / max(tims.column_launches, 1), 5),
+            "error_fp32_vs_fp64_after_steps": es,
+            "finite": bool(torch.isfinite(us32).all()),
+        }
+        del wss, ws64s, us32, us64, s0, wt
 
     # ---- batched ensemble (96 members of uneven size), autotuned tiles,
     #      HIP-graph replays ----
