@@ -187,14 +187,13 @@ __device__ __forceinline__ void sph_cell_du(const FusedArgs& a, const float* ws,
     const int i = o1 + cx + a.ng;
     const int j = act2 ? o2 + cy + a.ng : 0;
     const int k = act3 ? o3 + cz + a.ng : 0;
-    const float vol = (float)sph_volume(G, k, j, i);
-    const float A1m = (float)sph_area1(G, k, j, i), A1p = (float)sph_area1(G, k, j, i + 1);
+    const SphCell gc = sph_cell(G, k, j, i);
+    const float vol = (float)gc.vol;
+    const float A1m = (float)gc.a1m, A1p = (float)gc.a1p;
     const int lo1 = (cz * e2 + cy) * (e1 + 1) + cx;
     const int lo2 = fo1 + (cz * (e2 + 1) + cy) * e1 + cx;
     const int lo3 = fo2 + (cz * e2 + cy) * e1 + cx;
-    float A2m = 0.f, A2p = 0.f, A3 = 0.f;
-    if (act2) A2m = (float)sph_area2(G, k, j, i), A2p = (float)sph_area2(G, k, j + 1, i);
-    if (act3) A3 = (float)sph_area3(G, j, i);
+    const float A2m = (float)gc.a2m, A2p = (float)gc.a2p, A3 = (float)gc.a3;
     float div[5];
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
@@ -206,17 +205,17 @@ __device__ __forceinline__ void sph_cell_du(const FusedArgs& a, const float* ws,
     const int lin = ((cz + a.h3) * H2 + cy + a.h2) * H1 + cx + 3;
     const float rho = ws[IDN * H + lin], v1 = ws[IVX * H + lin], v2 = ws[IVY * H + lin],
                 v3 = ws[IVZ * H + lin], pre = ws[IPR * H + lin];
-    const float s1i = (float)sph_src1_i(G, i), s1j = (float)sph_src1_j(G, j);
-    const float s2i = (float)sph_src2_i(G, i);
+    const float s1i = (float)gc.src1_i, s1j = (float)gc.src1_j;
+    const float s2i = (float)gc.src2_i;
     const float fpm = fp[lo1], fpp = fp[lo1 + 1];
     float src1 = s1i * rho * (v2 * v2 + v3 * v3);
-    src1 += (A1p * fpp - A1m * fpm) / vol - (fpp - fpm) / (float)axis_width(G.ax[0], i);
+    src1 += (A1p * fpp - A1m * fpm) / vol - (fpp - fpm) / (float)gc.dx1;
     div[IVX] -= src1;
     div[IVY] -= s1i * s1j * (rho * (v3 * v3) + pre);
     div[IVY] += s2i * (A1m * fs[IVY * ftot + lo1] + A1p * fs[IVY * ftot + lo1 + 1]);
     div[IVZ] += s2i * (A1m * fs[IVZ * ftot + lo1] + A1p * fs[IVZ * ftot + lo1 + 1]);
     if (act2)
-      div[IVZ] += s1i * (float)sph_src2_j(G, j) *
+      div[IVZ] += s1i * (float)gc.src2_j *
                   (A2m * fs[IVZ * ftot + lo2] + A2p * fs[IVZ * ftot + lo2 + e1]);
     else
       div[IVZ] += s1i * s1j * (rho * v3 * v2);
@@ -1450,9 +1449,8 @@ __device__ __forceinline__ void column_pcr(float* __restrict__ u, float* __restr
     if (kSph) {
       const int jj = a.n2 > 1 ? a.ng + col % a.ncol2 : 0;
       const int kk = a.n3 > 1 ? a.ng + col / a.ncol2 : 0;
-      ar_i = sph_area1(a.geo, kk, jj, a.ng + i);
-      ar_p = sph_area1(a.geo, kk, jj, a.ng + i + 1);
-      vol_i = sph_volume(a.geo, kk, jj, a.ng + i);
+      const SphCell gc = sph_cell(a.geo, kk, jj, a.ng + i);
+      ar_i = gc.a1m, ar_p = gc.a1p, vol_i = gc.vol;
     }
     {
       double Jp[NN], Jn[NN], Jc[NN];

@@ -1,4 +1,4 @@
-// Coordinate geometry of a block for the plain path, on host and device.
+// Coordinate geometry of a block for the plain and fused paths, on host and device.
 //
 // Faces are torch::linspace(lo, hi, nc + 1) over the block including its
 // ghost cells (CoordinateImpl constructor, src/coord/coordinate.cpp:201-221).
@@ -138,6 +138,39 @@ __host__ __device__ inline double sph_src1_j(const Geometry& g, int j) {
 __host__ __device__ inline double sph_src2_j(const Geometry& g, int j) {
   double sm = fabs(sin(axis_face(g.ax[1], j))), sp = fabs(sin(axis_face(g.ax[1], j + 1)));
   return (sp - sm) / ((sm + sp) * sph_polar(g, j));
+}
+
+// Everything one cell needs from the coordinate, from its six face
+// coordinates with one sine / cosine pair per polar face (the functions above
+// evaluate each factor on its own): the volume, the areas of its faces, and
+// the source coefficients of its row and layer.
+struct SphCell {
+  double vol, a1m, a1p, a2m, a2p, a3, src1_i, src2_i, src1_j, src2_j, dx1;
+};
+
+__host__ __device__ inline SphCell sph_cell(const Geometry& g, int k, int j, int i) {
+  const double rm = axis_face(g.ax[0], i), rp = axis_face(g.ax[0], i + 1);
+  const double tm = axis_face(g.ax[1], j), tp = axis_face(g.ax[1], j + 1);
+  const double dphi = axis_width(g.ax[2], k);
+  double sm, cm, sp, cp;
+  sincos(tm, &sm, &cm);
+  sincos(tp, &sp, &cp);
+  sm = fabs(sm), sp = fabs(sp);
+  const double polar = fabs(cm - cp);
+  const double vr = (rp * rp * rp - rm * rm * rm) / 3.0, ra = 0.5 * (rp * rp - rm * rm);
+  SphCell c;
+  c.vol = vr * polar * dphi;
+  c.a1m = rm * rm * polar * dphi;
+  c.a1p = rp * rp * polar * dphi;
+  c.a2m = ra * sm * dphi;
+  c.a2p = ra * sp * dphi;
+  c.a3 = ra * (tp - tm);
+  c.src1_i = ra / vr;
+  c.src2_i = (rp - rm) / ((rm + rp) * vr);
+  c.src1_j = (sp - sm) / polar;
+  c.src2_j = (sp - sm) / ((sm + sp) * polar);
+  c.dx1 = rp - rm;
+  return c;
 }
 
 // center_width2 / center_width3 (:146-155), the CFL lengths of x2 and x3
