@@ -285,11 +285,13 @@ __global__ void coord_forward_kernel(const T* __restrict__ w, const T* __restric
     bool r2 = n2 == 1 || (j >= ng && j < n2 - ng);
     bool r3 = n3 == 1 || (k >= ng && k < n3 - ng);
     double vol, A1m, A1p, A2m, A2p, A3;
+    SphCell gc{};
     if (sph) {
-      vol = sph_volume(g, k, j, i);
-      A1m = sph_area1(g, k, j, i), A1p = sph_area1(g, k, j, i + 1);
-      A2m = sph_area2(g, k, j, i), A2p = sph_area2(g, k, j + 1, i);
-      A3 = sph_area3(g, j, i);
+      gc = sph_cell(g, k, j, i);
+      vol = gc.vol;
+      A1m = gc.a1m, A1p = gc.a1p;
+      A2m = gc.a2m, A2p = gc.a2p;
+      A3 = gc.a3;
     } else {
       vol = a.dx3 * (a.dx2 * a.dx1);
       A1m = A1p = a.dx3 * a.dx2;
@@ -309,25 +311,25 @@ __global__ void coord_forward_kernel(const T* __restrict__ w, const T* __restric
     if (sph) {
       T rho = w[IDN * ncell + q], v2 = w[IVY * ncell + q], v3 = w[IVZ * ncell + q];
       T pre = w[IPR * ncell + q];
-      T s1i = T(sph_src1_i(g, i)), s1j = T(sph_src1_j(g, j));
+      T s1i = T(gc.src1_i), s1j = T(gc.src1_j);
       T src1 = s1i * rho * (v2 * v2 + v3 * v3);
       if (fp1) {
         if (r1)
           src1 += (T(A1p) * fp1[q + 1] - T(A1m) * fp1[q]) / T(vol) -
-                  (fp1[q + 1] - fp1[q]) / T(axis_width(g.ax[0], i));
+                  (fp1[q + 1] - fp1[q]) / T(gc.dx1);
       } else {
         src1 += T(2.) * s1i * pre;
       }
       d[IVX] -= src1;
       d[IVY] -= s1i * s1j * (rho * (v3 * v3) + pre);
       if (r1 && f1) {
-        T s2i = T(sph_src2_i(g, i));
+        T s2i = T(gc.src2_i);
         d[IVY] += s2i * (T(A1m) * f1[IVY * ncell + q] + T(A1p) * f1[IVY * ncell + q + 1]);
         d[IVZ] += s2i * (T(A1m) * f1[IVZ * ncell + q] + T(A1p) * f1[IVZ * ncell + q + 1]);
       }
       if (n2 > 1) {
         if (r1 && r2 && f2)
-          d[IVZ] += s1i * T(sph_src2_j(g, j)) *
+          d[IVZ] += s1i * T(gc.src2_j) *
                     (T(A2m) * f2[IVZ * ncell + q] + T(A2p) * f2[IVZ * ncell + q + n1]);
       } else {
         d[IVZ] += s1i * s1j * (rho * v3 * v2);

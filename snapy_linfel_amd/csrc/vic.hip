@@ -44,8 +44,8 @@ __device__ __forceinline__ LayerGeom layer_geom(const VicArgs& a, int col, int i
   if (a.geo.coord != SNAPX_COORD_SPHERICAL_POLAR) return {a.area, a.area, a.vol};
   int j = a.n2 > 1 ? a.ng + col % a.ncol2 : 0;
   int k = a.n3 > 1 ? a.ng + col / a.ncol2 : 0;
-  int c = a.ng + i;
-  return {sph_area1(a.geo, k, j, c), sph_area1(a.geo, k, j, c + 1), sph_volume(a.geo, k, j, c)};
+  const SphCell gc = sph_cell(a.geo, k, j, a.ng + i);
+  return {gc.a1m, gc.a1p, gc.vol};
 }
 
 template <typename T>
