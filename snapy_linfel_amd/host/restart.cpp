@@ -158,9 +158,12 @@ int make_restart_bundle(std::string const& bundle_name, std::vector<std::string>
 
 Variables load_restart(std::string const& path, int block_rank) {
   {
+    // a bundle starts with the magic line; read only that many bytes (a
+    // payload is a zip archive and may have no newline for megabytes)
     std::ifstream in(path, std::ios::binary);
-    std::string first;
-    if (in && std::getline(in, first) && first == kBundleMagic)
+    const std::string magic = std::string(kBundleMagic) + "\n";
+    std::string head(magic.size(), '\0');
+    if (in && in.read(&head[0], (std::streamsize)head.size()) && head == magic)
       return load_from_bundle(in, path, block_rank);
   }
   return load_tensors(path);
