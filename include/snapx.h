@@ -71,6 +71,11 @@ extern "C" {
 #define SNAPX_BC_REFLECTING 0
 #define SNAPX_BC_OUTFLOW 1
 #define SNAPX_BC_PERIODIC 2
+/* custom_inner / custom_outer: no-ops, the caller owns those ghost cells
+   (src/bc/bc_func.cpp:4-5); plain path only (the reference's stage combine
+   and body forces still advance those ghosts, which the fused kernels do not
+   reproduce) */
+#define SNAPX_BC_CUSTOM 3
 
 /* coordinate systems (src/coord/coordinate.cpp:437-452) */
 #define SNAPX_COORD_CARTESIAN 0

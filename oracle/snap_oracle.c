@@ -1812,7 +1812,7 @@ static void bc_axis(double* var, int nvar, int n3, int n2, int n1, int ng,
   long s;
   int n;
   axis_info(n3, n2, n1, dim, &s, &n);
-  if (n == 1 || type == OC_BC_NONE) return;
+  if (n == 1 || type == OC_BC_NONE || type == OC_BC_CUSTOM) return;
   long ncell = (long)n3 * n2 * n1;
   long np = pencil_count(n3, n2, n1, dim);
   int vnorm = 4 - dim;

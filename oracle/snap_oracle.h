@@ -45,7 +45,8 @@ enum {
   OC_BC_NONE = -1,
   OC_BC_REFLECTING = 0,
   OC_BC_OUTFLOW = 1,
-  OC_BC_PERIODIC = 2
+  OC_BC_PERIODIC = 2,
+  OC_BC_CUSTOM = 3 /* custom_inner / custom_outer (src/bc/bc_func.cpp:4-5): no-ops */
 };
 
 /* variable types for boundary functions (src/snap.h:62-71) */

@@ -39,7 +39,7 @@ int validate_config(const snapx_config* c) {
   SNAPX_CHECK_ARG(c->non_hydrostatic >= 0. && c->non_hydrostatic <= 1.,
                   "non-hydrostatic must be in [0, 1]");
   for (int b = 0; b < 6; ++b)
-    SNAPX_CHECK_ARG(c->bc[b] >= SNAPX_BC_NONE && c->bc[b] <= SNAPX_BC_PERIODIC,
+    SNAPX_CHECK_ARG(c->bc[b] >= SNAPX_BC_NONE && c->bc[b] <= SNAPX_BC_CUSTOM,
                     "Boundary function is not defined");
   SNAPX_CHECK_ARG(c->coord == SNAPX_COORD_CARTESIAN || c->coord == SNAPX_COORD_SPHERICAL_POLAR,
                   "Unknown coordinate type");

@@ -1313,6 +1313,8 @@ extern "C" int snapx_adjoint_rk3_step(const snapx_config* cfg, double dt, const 
   SNAPX_CHECK_ARG(!cfg->limiter, "adjoint: the EOS limiter is not differentiable (turn it off)");
   SNAPX_CHECK_ARG(cfg->non_hydrostatic == 1.0, "adjoint: non-hydrostatic < 1 is not supported");
   SNAPX_CHECK_ARG(cfg->coord == SNAPX_COORD_CARTESIAN, "adjoint: Cartesian geometry only");
+  for (int b = 0; b < 6; ++b)
+    SNAPX_CHECK_ARG(cfg->bc[b] != SNAPX_BC_CUSTOM, "adjoint: custom boundaries are not supported");
   for (int d = 0; d < 3; ++d)
     SNAPX_CHECK_ARG(!cfg->disable_flux[d], "adjoint: disable-flux is not supported");
   AdjSpace s = adj_layout(cfg, reinterpret_cast<char*>(workspace));

@@ -1734,6 +1734,11 @@ static int check_fused(const snapx_config* c) {
                     "fused path: spherical-polar columns of at most 256 (3x3) / 128 (5x5) cells");
   }
   SNAPX_CHECK_ARG(c->nghost == 3, "fused path: nghost must be 3");
+  // the reference advances custom ghost cells with the stage combine and the
+  // body forces; the fused kernels never update ghost cells within a step
+  for (int b = 0; b < 6; ++b)
+    SNAPX_CHECK_ARG(c->bc[b] != SNAPX_BC_CUSTOM,
+                    "fused path: custom boundaries are not supported (use the plain path)");
   SNAPX_CHECK_ARG(c->nx1 > 1, "fused path: x1 must be active");
   SNAPX_CHECK_ARG(!c->limiter, "fused path: the EOS limiter is not supported (use the plain path)");
   SNAPX_CHECK_ARG(c->non_hydrostatic == 1.0,

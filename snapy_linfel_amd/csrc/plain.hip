@@ -454,7 +454,7 @@ __global__ void bc_kernel(T* __restrict__ var, int nvar, int n3, int n2, int n1,
     bool outer = g2 >= ng;
     int g = outer ? g2 - ng : g2;
     int type = outer ? type_out : type_in;
-    if (type < 0) continue;
+    if (type < 0 || type == SNAPX_BC_CUSTOM) continue;  // custom: the caller's ghosts
     int dst, src;
     if (!outer) {
       dst = g;

@@ -57,6 +57,7 @@ int bc_code(std::string const& name) {
   if (name == "reflecting") return SNAPX_BC_REFLECTING;
   if (name == "outflow") return SNAPX_BC_OUTFLOW;
   if (name == "periodic") return SNAPX_BC_PERIODIC;
+  if (name == "custom") return SNAPX_BC_CUSTOM;
   TORCH_CHECK(false, "Boundary function '", name, "' is not defined.");
   return -1;
 }

@@ -372,3 +372,26 @@ def test_check_state_in_a_captured_graph():
         torch.cuda.synchronize()
         capi.check_state(cfg, x, eager)
         assert int(flag.item()) == int(eager.item()) == expect, (edit, int(flag.item()))
+
+
+def test_custom_boundaries_plain_path_parity():
+    """custom x2 sides (src/bc/bc_func.cpp:4-5): the ghost cells are the
+    caller's, and the stage combine and gravity still advance them as in the
+    reference; three FP64 steps match the oracle to 1e-12.  The fused path
+    refuses such blocks."""
+    oc = O.straka_config(nx2=48, nx1=24, implicit_scheme=1)
+    oc.x2max = 25.6e3 * 48 / 512 * 2
+    oc.bc[2] = oc.bc[3] = O.BC["custom"]
+    wp = O.straka_init(oc, xc=2400.0)
+    w, u = O.initialize(oc, wp)  # x2 ghosts: the initial profile itself
+    u_ref = u
+    cfg = cfg_of(oc)
+    ud = to_dev(u)
+    wd = torch.empty_like(ud)
+    for _ in range(3):
+        u_ref, _ = O.rk3_step(oc, 0.5, u_ref)
+        capi.rk3_step(cfg, 0.5, ud, wd)
+    e = var_errors(host(ud), u_ref)  # ghosts included
+    assert (e < 1e-12).all(), e
+    with pytest.raises(RuntimeError, match="custom boundaries"):
+        capi.fused_rk3_step(cfg, 0.5, to_dev(u, torch.float32))

@@ -171,3 +171,17 @@ def test_max_time_step_closed_form():
     cs = np.sqrt(1.4 * w[4] / w[0])
     exp = 0.9 * (1 / 64 / (np.abs(w[1]) + cs))[0, 0, 3:-3].min()
     assert abs(O.max_time_step(c, w, 0.9) - exp) < 1e-15
+
+
+def test_custom_boundaries_leave_the_callers_ghosts():
+    """custom_inner / custom_outer (src/bc/bc_func.cpp:4-5) are no-ops: the
+    x2 ghosts keep the caller's values while the x1 walls reflect."""
+    c = O.make_config(nx1=8, nx2=6, nghost=3,
+                      bc=("reflecting", "reflecting", "custom", "custom", "periodic", "periodic"))
+    rng = np.random.default_rng(3)
+    var = rng.standard_normal((5,) + O.shape(c))
+    out = O.apply_bc(c, var, O.CONSERVED)
+    g = 3
+    assert np.array_equal(out[:, :, :g, g:-g], var[:, :, :g, g:-g])
+    assert np.array_equal(out[:, :, -g:, g:-g], var[:, :, -g:, g:-g])
+    assert np.array_equal(out[1, :, g:-g, g - 1], -var[1, :, g:-g, g])
