@@ -476,6 +476,40 @@ def main():
     roof["copy_measured_GBs"] = round(copy_gbs, 1)
     roof["frac_of_copy"] = round(achieved / copy_gbs, 5)
 
+    # the floor a launch of this size meets on this GPU: a 64-element kernel,
+    # and a device copy moving the dominant kernel's algorithmic bytes (half
+    # read, half written); 100 launches captured in a HIP graph and replayed,
+    # GPU time per launch
+    def per_launch_us(fn, n):
+        fn()
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            with torch.cuda.graph(g, stream=side):
+                for _ in range(100):
+                    fn()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(n // 100):
+            g.replay()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / (n // 100 * 100) * 1e3
+    tiny = torch.zeros(64, device=dev)
+    csrc = torch.ones(bytes_ // 8, dtype=torch.float32, device=dev)
+    cdst = torch.empty_like(csrc)
+    t_tiny = per_launch_us(lambda: tiny.add_(1.0), 2000)
+    t_copy = per_launch_us(lambda: cdst.copy_(csrc), 2000)
+    roof["floor"] = {"tiny_kernel_us": round(t_tiny, 3), "copy_same_bytes_us": round(t_copy, 3),
+                     "copy_same_bytes_GBs": round(bytes_ / (t_copy * 1e-6) / 1e9, 1),
+                     "kernel_over_copy_same_bytes": round(avg_ms * 1e3 / t_copy, 2)}
+    del tiny, csrc, cdst
+
     extra = {"fused_rhs_avg_ms": round(rhs_avg, 5), "fused_column_avg_ms": round(col_avg, 5),
              "timed_launches": int(tim.rhs_launches + tim.column_launches + tim.bc_launches)}
 
