@@ -155,7 +155,8 @@ def ensemble_members(capi, n=96, seed=2024):
     return cfgs, ws
 
 
-def c5_sensitivity(capi, dev, nsteps=200, every=20, ndir=2, wind=(0.0, 0.0), richardson=False):
+def c5_sensitivity(capi, dev, nsteps=200, every=20, ndir=2, wind=(0.0, 0.0), richardson=False,
+                   roll=0.0, eps=1e-6):
     """Adjoint sensitivity on the 3-D box 128 x 128 x 96 of 200 m (FP64):
     loss = 0.5 sum (theta_final - theta_target)^2 after nsteps SSP-RK3 steps,
     gradient with respect to the initial theta perturbation of the lowest 20
@@ -164,7 +165,10 @@ def c5_sensitivity(capi, dev, nsteps=200, every=20, ndir=2, wind=(0.0, 0.0), ric
     seeded directions (Richardson-extrapolated from steps 1e-6 and 2e-6 K with
     `richardson`).  `wind` adds a uniform horizontal wind (x2, x3) in m/s: at
     rest every x2 / x3 face starts exactly on the |u| switch of the Roe solver,
-    where central differences straddle a kink."""
+    where central differences straddle a kink.  `roll` adds a convective roll
+    in the x1-x2 plane of that vertical amplitude (m/s), v1 = roll sin(pi z /
+    H) cos(2 pi x / L) with the matching divergence-free v2, so that the x1
+    faces leave the upwind switch of LMARS as well."""
     import torch
     from snapy_linfel_amd import adjoint as A
     R, cp, g0, p0 = 287.0, 1004.5, 9.81, 1e5
@@ -177,15 +181,22 @@ def c5_sensitivity(capi, dev, nsteps=200, every=20, ndir=2, wind=(0.0, 0.0), ric
     thb = T0 * (p0 / pb) ** (R / cp)
     gam = cfg.gammad
 
+    H, Lx = 200.0 * 96, 200.0 * 128
+    xs = (torch.arange(shp[2], dtype=torch.float64, device=dev) - 3 + 0.5) * 200.0
+    X2, Z2 = torch.meshgrid(xs, z, indexing="ij")  # [n2, n1]
+    v1r = roll * torch.sin(torch.pi * Z2 / H) * torch.cos(2 * torch.pi * X2 / Lx)
+    v2r = -roll * (Lx / (2 * H)) * torch.cos(torch.pi * Z2 / H) * torch.sin(2 * torch.pi * X2 / Lx)
+
     def state(dth):
         th = thb.expand(shp[1:]).clone()
         th = th + torch.nn.functional.pad(dth, (3, shp[3] - 23, 3, 3, 3, 3))
         T = th * (pb / p0) ** (R / cp)
         rho = pb / (R * T)
-        zero = torch.zeros_like(rho)
-        m2, m3 = rho * wind[0], rho * wind[1]
-        e = pb / (gam - 1.0) + 0.5 * rho * (wind[0] ** 2 + wind[1] ** 2)
-        return torch.stack([rho, zero, m2, m3, e])
+        v1 = v1r.expand(shp[1:])
+        v2 = wind[0] + v2r.expand(shp[1:])
+        m1, m2, m3 = rho * v1, rho * v2, rho * wind[1]
+        e = pb / (gam - 1.0) + 0.5 * rho * (v1 ** 2 + v2 ** 2 + wind[1] ** 2)
+        return torch.stack([rho, m1, m2, m3, e])
 
     def theta(u):
         rho = u[0]
@@ -235,14 +246,15 @@ def c5_sensitivity(capi, dev, nsteps=200, every=20, ndir=2, wind=(0.0, 0.0), ric
         # the flow crosses branch points of the schemes (upwind switches,
         # |u|) at some faces within any finite step, so differences converge
         # at first order over many steps: a small step in FP64
-        fd = (4 * central(1e-6) - central(2e-6)) / 3 if richardson else central(1e-6)
+        fd = (4 * central(eps) - central(2 * eps)) / 3 if richardson else central(eps)
         ad = float((grad * d).sum())
         rels.append(abs(ad - fd) / abs(fd))
         if ndir > 4 and (k + 1) % 8 == 0:
             print(f"c5 finite differences: {k + 1}/{ndir} directions, max rel err "
                   f"{max(rels):.3e}", file=sys.stderr, flush=True)
-    return {"steps": nsteps, "checkpoint_every": every, "wind_m_s": list(wind),
-            "fd": "richardson 1e-6/2e-6" if richardson else "central 1e-6", "loss": L0,
+    return {"steps": nsteps, "checkpoint_every": every, "wind_m_s": list(wind), "roll_m_s": roll,
+            "fd": (f"richardson {eps:g}/{2 * eps:g}" if richardson else f"central {eps:g}"),
+            "loss": L0,
             "forward_s": round(t_fwd, 3), "forward_plus_adjoint_s": round(t_grad, 3),
             "gradient_over_forward": round(t_grad / t_fwd, 3),
             "fd_directions": ndir, "fd_rel_err_max": max(rels),
@@ -690,9 +702,11 @@ def main():
         if not args.no_c5:
             extra["c5_sensitivity_box3d_128x128x96_fp64"] = c5_sensitivity(
                 capi, dev, ndir=args.c5_directions)
-            # the same gradient check off the Roe |u| kinks of the state at rest
-            extra["c5_fd_check_uniform_wind"] = c5_sensitivity(
-                capi, dev, ndir=8, wind=(2.0, 1.5))
+            # the same gradient check off the kinks of the state at rest: a
+            # uniform horizontal wind (Roe |u| at the x2 / x3 faces) and a
+            # convective roll (LMARS upwind switch at the x1 faces)
+            extra["c5_fd_check_wind_and_roll"] = c5_sensitivity(
+                capi, dev, ndir=8, wind=(2.0, 1.5), roll=1.0)
         extra["adjoint_box3d_128x128x96_fp64"] = {
             "forward_ms_per_step": round(tf * 1e3, 3), "adjoint_ms_per_step": round(tadj * 1e3, 3),
             "adjoint_over_forward": round(tadj / tf, 3),
