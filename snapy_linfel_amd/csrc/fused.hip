@@ -312,7 +312,10 @@ __device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __r
   }
   __syncthreads();
   if (!merged && refill) {
-    const bool bot = o1 == 0, top = o1 + e1 == a.nx1;
+    // the tile's LDS holds physical x1 ghosts below when it starts at the
+    // bottom, and above whenever its 3-cell halo passes the last cell: the
+    // top tile, and also the one below a last tile of fewer than 3 cells
+    const bool bot = o1 == 0, top = o1 + e1 + 3 > a.nx1;
     const int ncolt = e2 * e3;
     for (int c = threadIdx.x; c < 2 * ncolt; c += blockDim.x) {
       const bool upper = c >= ncolt;
@@ -321,10 +324,14 @@ __device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __r
       const int y = a.h2 + cc % e2, z = a.h3 + cc / e2;
       float* rho = ws + IDN * H + (z * H2 + y) * H1;
       float* pre = ws + IPR * H + (z * H2 + y) * H1;
-      const int is = upper ? 3 + e1 - 1 : 3;
+      // LDS index of the boundary cell (the last interior cell above)
+      const int is = upper ? 3 + (a.nx1 - 1 - o1) : 3;
       const IsentropicRefill R(rho[is], pre[is], gammad, gdz, upper ? 1 : -1);
 #pragma unroll
-      for (int n = 1; n <= 3; ++n) R.layer(n, rho[upper ? is + n : is - n], pre[upper ? is + n : is - n]);
+      for (int n = 1; n <= 3; ++n) {
+        const int x = upper ? is + n : is - n;
+        if (x < H1) R.layer(n, rho[x], pre[x]);  // layers past this tile's halo: another tile's
+      }
     }
     __syncthreads();
   }
@@ -1911,15 +1918,21 @@ static int fused_step(const snapx_config* cfg, double dt, float* u, void* worksp
   float* u0 = reinterpret_cast<float*>(base + field);
   double* scr = reinterpret_cast<double*>(base + 2 * field);
 
-  const bool march = march_ok(cfg);
+  bool march = march_ok(cfg);
   const bool sph = cfg->coord == SNAPX_COORD_SPHERICAL_POLAR;
   FusedArgs a;
   if (march) {
     if (int e = make_march_args(cfg, dt, &a)) return e;
-  } else {
+    // the marching kernel needs enough (tile, slab) workgroups to fill the
+    // GPU; below 512 the merged 16x4x2 tiles win (K1 per launch: 64x128x128
+    // 191 against 289 us, 128x128x128 356 against 342, 64x256x256 705
+    // against 432)
+    if (a.nt1 * a.nt2 * a.nt3 < 512) march = false;
+  }
+  if (!march) {
     Tile t = choose_tile(g);
-    // spherical 3-D tiles: thinner in x3, since every face stays in LDS
-    if (sph && g.n3 > 1) t = {16, 4, 2};
+    // 3-D tiles: 16x4x2, every face of the tile in one LDS pass
+    if (g.n3 > 1) t = {16, 4, 2};
     if (int e = make_fused_args(cfg, t, dt, &a)) return e;
   }
   size_t smem = march ? march_smem_bytes(a) : rhs_smem_bytes(a);

@@ -132,15 +132,17 @@ def test_batch_mixed_column_heights_exact():
         assert torch.equal(b.member(u, m), single), m
 
 
-def test_fused_3d_marching_rhs_matches_tiled_rhs():
-    """3-D blocks: the single-block path's x3-marching RHS kernel and the
-    batch path's tiled RHS kernel perform the same operations in the same
-    order; the compiler contracts a few multiply-adds differently in the two
-    kernels, so they agree to FP32 rounding (not bit for bit), and both meet
-    the oracle's FP32 bar.  Ragged tiles in every direction, reflecting and
-    outflow x3 ends."""
-    oc = O.make_config(nx1=37, nx2=21, nx3=45, nghost=3, x1min=0.0, x1max=3700.0,
-                       x2min=0.0, x2max=2100.0, x3min=0.0, x3max=4500.0, gammad=1.4,
+@pytest.mark.parametrize("n", [(37, 21, 45), (33, 65, 460)], ids=["tiled", "marching"])
+def test_fused_3d_marching_rhs_matches_tiled_rhs(n):
+    """3-D blocks: the single-block path's RHS kernel (merged 16x4x2 tiles for
+    the small block, the x3-marching kernel once it has 512 workgroups or
+    more) and the batch path's tiled RHS kernel perform the same operations in
+    the same order; the compiler contracts a few multiply-adds differently in
+    the kernels, so they agree to FP32 rounding (not bit for bit), and both
+    meet the oracle's FP32 bar.  Ragged tiles in every direction, reflecting
+    and outflow x3 ends."""
+    oc = O.make_config(nx1=n[0], nx2=n[1], nx3=n[2], nghost=3, x1min=0.0, x1max=100.0 * n[0],
+                       x2min=0.0, x2max=100.0 * n[1], x3min=0.0, x3max=100.0 * n[2], gammad=1.4,
                        weight=8.31446261815324 / 287.0, recon_vertical=("weno5", False, False),
                        riemann=("lmars", "roe", "hllc"), grav1=-9.81, implicit_scheme=1,
                        bc=("reflecting", "reflecting", "periodic", "periodic", "reflecting",

@@ -117,3 +117,20 @@ def test_fused_hydrostatic_rest_state_drift():
     for u, bar in ((host(u32), 1e-3), (host(u64), 1e-9)):
         vmax = max(np.abs(u[v][I] / u[0][I]).max() for v in (1, 2, 3))
         assert vmax < bar, (vmax, bar)
+
+
+@pytest.mark.parametrize("nx1", [65, 66])
+def test_fused_ragged_last_x1_tile_refill(nx1):
+    """Regression: 2-D blocks on 32x8 tiles (one face pass per direction),
+    with a last x1 tile of one or two cells.  The tile below it has the
+    physical x1 ghosts in its 3-cell halo, and those must get the isentropic
+    refill too.  A single step against the FP64 oracle, at the fused bar."""
+    oc = O.straka_config(nx2=2048, nx1=nx1, implicit_scheme=1)
+    oc.x1max = 100.0 * nx1
+    oc.x2max = 50.0 * 2048
+    w, u = O.initialize(oc, O.straka_init(oc, xc=51200.0))
+    u_ref, _ = O.rk3_step(oc, 0.2, u)
+    uf = to_dev(u, torch.float32)
+    capi.fused_rk3_step(cfg_of(oc), 0.2, uf)
+    e = var_errors(host(uf), u_ref, O.interior(oc))
+    assert (e < 1e-5).all(), e
