@@ -1,0 +1,87 @@
+"""GPU: seeded random block shapes, boundary functions, solvers and implicit
+schemes, one RK3 step of the FP64 plain path (1e-12) and of the FP32 fused
+path (1e-5) against the oracle.
+
+The shapes reach the edges of the tilings: ragged tiles along every axis, last
+tiles of one or two cells, single-tile axes, x1 columns from 3 to 70 cells,
+and 1-, 2- and 3-D blocks."""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from snapy_linfel_amd import capi
+from test_gpu_plain import cfg_of, host, to_dev, var_errors
+
+pytestmark = pytest.mark.gpu
+
+SOLVERS = ("lmars", "hllc", "roe")
+BCS = ("reflecting", "outflow", "periodic")
+
+
+def _case(seed):
+    rng = np.random.default_rng(1000 + seed)
+    nx1 = int(rng.integers(3, 71))
+    dim = int(rng.choice([1, 2, 2, 3, 3, 3]))
+    nx2 = int(rng.integers(3, 41)) if dim >= 2 else 1
+    nx3 = int(rng.integers(3, 13)) if dim == 3 else 1
+    scheme = int(rng.choice([0, 1, 9]))
+    bc = []
+    for ax in range(3):
+        kind = str(rng.choice(BCS))
+        if ax == 0 and kind == "periodic":
+            kind = "reflecting"  # a hydrostatic column has walls
+        bc += [kind, kind]
+    dz = 100.0
+    oc = O.make_config(nx1=nx1, nx2=nx2, nx3=nx3, nghost=3, x1min=0.0, x1max=dz * nx1,
+                       x2min=0.0, x2max=dz * nx2, x3min=0.0, x3max=dz * nx3, gammad=1.4,
+                       weight=8.31446261815324 / 287.0,
+                       recon_vertical=("weno5", False, False),
+                       recon_horizontal=("weno5", False, bool(rng.integers(0, 2))),
+                       riemann=tuple(str(rng.choice(SOLVERS)) for _ in range(3)),
+                       grav1=-9.81, implicit_scheme=scheme, bc=tuple(bc))
+    n3, n2, n1 = O.shape(oc)
+    z = (np.arange(n1) - 3 + 0.5) * dz
+    T = 250.0 - 9.81 * z / 1004.5
+    p = 1e5 * (T / 250.0) ** (1004.5 / 287.0)
+    w = np.zeros((5, n3, n2, n1))
+    w[4] = p
+    w[0] = p / (287.0 * (T + rng.uniform(-0.5, 0.5, size=(n3, n2, n1))))
+    w[1:4] = rng.uniform(-3, 3, size=(3, n3, n2, n1))
+    w, u = O.initialize(oc, w)
+    dt = 0.5 * O.max_time_step(oc, w, 1.0)
+    return oc, u, dt
+
+
+@pytest.mark.parametrize("seed", range(16))
+def test_random_block_one_step(seed):
+    oc, u, dt = _case(seed)
+    desc = (oc.nx1, oc.nx2, oc.nx3, oc.implicit_scheme, list(oc.bc), list(oc.riemann))
+    u_ref, _ = O.rk3_step(oc, dt, u)
+    cfg = cfg_of(oc)
+    I = O.interior(oc)
+    ud = to_dev(u)
+    capi.rk3_step(cfg, dt, ud, torch.empty_like(ud))
+    e64 = var_errors(host(ud), u_ref, I)
+    assert (e64 < 1e-12).all(), (desc, e64)
+    uf = to_dev(u, torch.float32)
+    capi.fused_rk3_step(cfg, dt, uf)
+    e32 = var_errors(host(uf), u_ref, I)
+    assert (e32 < 1e-5).all(), (desc, e32)
+
+
+@pytest.mark.parametrize("scheme", [0, 1, 9])
+def test_random_blocks_as_batches(scheme):
+    """The same random blocks, grouped by implicit scheme, as members of one
+    batch (persistent kernels, default tiles): each member after one step
+    against the oracle at the fused bar."""
+    cases = [c for c in (_case(s) for s in range(16)) if c[0].implicit_scheme == scheme]
+    assert cases
+    dt = min(c[2] for c in cases)
+    b = capi.Batch([cfg_of(oc) for oc, _, _ in cases], dt)
+    u = torch.cat([to_dev(u0, torch.float32).reshape(-1) for _, u0, _ in cases])
+    b.steps(u, 1)
+    for m, (oc, u0, _) in enumerate(cases):
+        u_ref, _ = O.rk3_step(oc, dt, u0)
+        e = var_errors(host(b.member(u, m)), u_ref, O.interior(oc))
+        assert (e < 1e-5).all(), (m, oc.nx1, oc.nx2, oc.nx3, e)
