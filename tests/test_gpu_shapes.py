@@ -85,3 +85,49 @@ def test_random_blocks_as_batches(scheme):
         u_ref, _ = O.rk3_step(oc, dt, u0)
         e = var_errors(host(b.member(u, m)), u_ref, O.interior(oc))
         assert (e < 1e-5).all(), (m, oc.nx1, oc.nx2, oc.nx3, e)
+
+
+def _sph_case(seed):
+    rng = np.random.default_rng(2000 + seed)
+    nx1 = int(rng.integers(3, 71))
+    dim = int(rng.choice([2, 3]))
+    nx2 = int(rng.integers(3, 33))
+    nx3 = int(rng.integers(3, 11)) if dim == 3 else 1
+    scheme = int(rng.choice([0, 1, 9]))
+    t0 = float(rng.uniform(0.1, 0.4)) * np.pi
+    t1 = float(rng.uniform(0.6, 0.9)) * np.pi
+    bc = ["reflecting", str(rng.choice(["reflecting", "outflow"]))]
+    bc += [str(rng.choice(["reflecting", "outflow"]))] * 2
+    bc += [str(rng.choice(BCS))] * 2
+    oc = O.make_config(nx1=nx1, nx2=nx2, nx3=nx3, nghost=3, x1min=1.0, x1max=2.0, x2min=t0,
+                       x2max=t1, x3min=0.0, x3max=float(rng.uniform(0.3, 1.5)),
+                       coord="spherical-polar", gammad=1.4,
+                       recon_vertical=("weno5", False, False),
+                       riemann=tuple(str(rng.choice(SOLVERS)) for _ in range(3)),
+                       grav1=-0.5, implicit_scheme=scheme, bc=tuple(bc))
+    n3, n2, n1 = O.shape(oc)
+    r = O.geometry(oc, "x1v")
+    p = np.exp(-0.5 * (r - 1.0))
+    w = np.zeros((5, n3, n2, n1))
+    w[4] = p * (1.0 + 0.02 * rng.uniform(-1, 1, (n3, n2, n1)))
+    w[0] = p * (1.0 + 0.05 * rng.uniform(-1, 1, (n3, n2, n1)))
+    w[1:4] = 0.05 * rng.uniform(-1, 1, (3, n3, n2, n1))
+    w, u = O.initialize(oc, w)
+    return oc, u, 0.5 * O.max_time_step(oc, w, 1.0)
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_random_spherical_block_one_step(seed):
+    oc, u, dt = _sph_case(seed)
+    desc = (oc.nx1, oc.nx2, oc.nx3, oc.implicit_scheme, list(oc.bc), list(oc.riemann))
+    u_ref, _ = O.rk3_step(oc, dt, u)
+    cfg = cfg_of(oc)
+    I = O.interior(oc)
+    ud = to_dev(u)
+    capi.rk3_step(cfg, dt, ud, torch.empty_like(ud))
+    e64 = var_errors(host(ud), u_ref, I, vel_scale=1.0)
+    assert (e64 < 1e-12).all(), (desc, e64)
+    uf = to_dev(u, torch.float32)
+    capi.fused_rk3_step(cfg, dt, uf)
+    e32 = var_errors(host(uf), u_ref, I, vel_scale=1.0)
+    assert (e32 < 1e-5).all(), (desc, e32)
