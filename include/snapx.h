@@ -29,7 +29,8 @@
  *    201-221), with the centres, face areas, volumes and source terms of
  *    src/coord/spherical_polar.cpp.  The fused path takes spherical-polar
  *    blocks with columns of at most 256 (3x3 blocks) / 128 (5x5) cells;
- *    batches and the adjoint are Cartesian only.
+ *    so do batches whose members all share one coordinate system.  The
+ *    adjoint is Cartesian only.
  */
 #ifndef SNAPX_H
 #define SNAPX_H

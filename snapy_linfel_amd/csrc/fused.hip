@@ -1616,7 +1616,9 @@ __device__ __forceinline__ int find_member(const BatchMember* m, int nm, int ite
   return lo;
 }
 
-// persistent K1 over the (member, tile) work list
+// persistent K1 over the (member, tile) work list; kSph: every member is a
+// spherical-polar block (merged face pass with geometric sources)
+template <bool kSph>
 __global__ void __launch_bounds__(kRhsThreads)
     batch_rhs_kernel(const float* __restrict__ u, float* __restrict__ du,
                      const BatchMember* __restrict__ mem, int nm, int total) {
@@ -1624,7 +1626,7 @@ __global__ void __launch_bounds__(kRhsThreads)
   for (int item = blockIdx.x; item < total; item += gridDim.x) {
     int m = find_member(mem, nm, item, kListTiles);
     FusedArgs a = mem[m].a;
-    rhs_tile<-1>(u + mem[m].off, du + mem[m].off, a, item - mem[m].tile0, smem);
+    rhs_tile<kSph ? 1 : -1, kSph>(u + mem[m].off, du + mem[m].off, a, item - mem[m].tile0, smem);
     __syncthreads();
   }
 }
@@ -1652,7 +1654,7 @@ __global__ void __launch_bounds__(kColThreads)
 }
 
 // persistent cell-parallel K2 over the (member, column group) work list
-template <int N>
+template <int N, bool kSph>
 __global__ void __launch_bounds__(PcrShape<N>::kThreads)
     batch_column_pcr_kernel(float* __restrict__ u, float* __restrict__ u0,
                             const float* __restrict__ du, const BatchMember* __restrict__ mem,
@@ -1667,7 +1669,7 @@ __global__ void __launch_bounds__(PcrShape<N>::kThreads)
     c.w0 = W[stage][0], c.w1 = W[stage][1], c.w2 = W[stage][2];
     int col = (item - mem[m].grp0) * (blockDim.x >> c.lshift) + (threadIdx.x >> c.lshift);
     int64_t o = mem[m].off;
-    column_pcr<N>(u + o, u0 + o, du + o, c, col, sh);
+    column_pcr<N, kSph>(u + o, u0 + o, du + o, c, col, sh);
     __syncthreads();  // LDS is reused by the next item
   }
 }
@@ -2081,11 +2083,12 @@ static int batch_layout(const snapx_config* members, int nm, double dt, int tx1,
                         std::vector<BatchMember>* table, snapx_batch_info* info) {
   SNAPX_CHECK_ARG(members && nm >= 1, "batch: no members");
   int scheme = members[0].implicit_scheme;
+  const bool sph = members[0].coord == SNAPX_COORD_SPHERICAL_POLAR;
   bool three_d = false, two_d = false;
   for (int m = 0; m < nm; ++m) {
     if (int e = check_fused(&members[m])) return e;
-    SNAPX_CHECK_ARG(members[m].coord == SNAPX_COORD_CARTESIAN,
-                    "batch: Cartesian members only (spherical-polar blocks step one at a time)");
+    SNAPX_CHECK_ARG(members[m].coord == members[0].coord,
+                    "batch: every member needs the same coordinate system");
     SNAPX_CHECK_ARG(members[m].implicit_scheme == scheme,
                     "batch: every member needs the same implicit scheme");
     three_d |= members[m].nx3 > 1;
@@ -2115,6 +2118,9 @@ static int batch_layout(const snapx_config* members, int nm, double dt, int tx1,
     tiles += b.a.nt1 * b.a.nt2 * b.a.nt3;
     cols += (b.c.ncol + kColThreads - 1) / kColThreads * kColThreads;
     int N = ((scheme >> 3) & 1) ? 5 : 3;
+    // spherical-polar columns have the cell-parallel kernel only (one row per lane)
+    SNAPX_CHECK_ARG(!sph || (pcr_ok(b.c, N) && !b.c.rows2),
+                    "batch: spherical-polar member column too tall for the column kernel");
     pcr &= pcr_ok(b.c, N);
     if (pcr_ok(b.c, N)) (b.c.rows2 ? groups2 : groups) += pcr_groups(b.c, N);
   }
@@ -2165,6 +2171,7 @@ extern "C" int snapx_batch_rk3_steps(const snapx_config* members, const snapx_ba
                   "snapx_batch_rk3_steps: workspace too small");
   hipStream_t st = as_stream(stream);
   int nm = info->nmembers;
+  const bool sph = members[0].coord == SNAPX_COORD_SPHERICAL_POLAR;
   char* base = reinterpret_cast<char*>(workspace);
   const BatchMember* table = reinterpret_cast<const BatchMember*>(base);
   size_t field = align256(sizeof(float) * (size_t)info->total_elems);
@@ -2183,8 +2190,12 @@ extern "C" int snapx_batch_rk3_steps(const snapx_config* members, const snapx_ba
   for (int n = 0; n < nsteps; ++n) {
     for (int s = 0; s < 3; ++s) {
       if (timing) SNAPX_HIP(hipEventRecord(ev[0].a, st));
-      hipLaunchKernelGGL(batch_rhs_kernel, dim3(rhs_grid), dim3(kRhsThreads), info->rhs_smem, st,
-                         u, du, table, nm, info->total_tiles);
+      if (sph)
+        hipLaunchKernelGGL(batch_rhs_kernel<true>, dim3(rhs_grid), dim3(kRhsThreads),
+                           info->rhs_smem, st, u, du, table, nm, info->total_tiles);
+      else
+        hipLaunchKernelGGL(batch_rhs_kernel<false>, dim3(rhs_grid), dim3(kRhsThreads),
+                           info->rhs_smem, st, u, du, table, nm, info->total_tiles);
       SNAPX_LAUNCH_CHECK();
       if (timing) {
         SNAPX_HIP(hipEventRecord(ev[0].b, st));
@@ -2194,12 +2205,18 @@ extern "C" int snapx_batch_rk3_steps(const snapx_config* members, const snapx_ba
         int N = info->vic_size;
         if (info->col_groups > 0) {
           dim3 grid(std::min(info->col_groups, 4096)), block(pcr_threads(N));
-          if (N == 3)
-            hipLaunchKernelGGL(batch_column_pcr_kernel<3>, grid, block, pcr_smem(3), st, u, u0,
-                               du, table, nm, info->col_groups, s);
+          if (N == 3 && sph)
+            hipLaunchKernelGGL((batch_column_pcr_kernel<3, true>), grid, block, pcr_smem(3), st,
+                               u, u0, du, table, nm, info->col_groups, s);
+          else if (N == 3)
+            hipLaunchKernelGGL((batch_column_pcr_kernel<3, false>), grid, block, pcr_smem(3), st,
+                               u, u0, du, table, nm, info->col_groups, s);
+          else if (sph)
+            hipLaunchKernelGGL((batch_column_pcr_kernel<5, true>), grid, block, pcr_smem(5), st,
+                               u, u0, du, table, nm, info->col_groups, s);
           else
-            hipLaunchKernelGGL(batch_column_pcr_kernel<5>, grid, block, pcr_smem(5), st, u, u0,
-                               du, table, nm, info->col_groups, s);
+            hipLaunchKernelGGL((batch_column_pcr_kernel<5, false>), grid, block, pcr_smem(5), st,
+                               u, u0, du, table, nm, info->col_groups, s);
         }
         if (info->col_groups2 > 0)
           hipLaunchKernelGGL(batch_column_pcr2_kernel, dim3(std::min(info->col_groups2, 4096)),
@@ -2383,12 +2400,14 @@ extern "C" int snapx_batch_autotune(const snapx_config* members, int nmembers, d
     }
     int grid = std::min(inf.total_tiles, 512);
     // K1 reads u and writes du only: the state is not changed
-    hipLaunchKernelGGL(batch_rhs_kernel, dim3(grid), dim3(kRhsThreads), inf.rhs_smem, st, u, du,
-                       table, nmembers, inf.total_tiles);
+    auto rhs = members[0].coord == SNAPX_COORD_SPHERICAL_POLAR ? batch_rhs_kernel<true>
+                                                               : batch_rhs_kernel<false>;
+    hipLaunchKernelGGL(rhs, dim3(grid), dim3(kRhsThreads), inf.rhs_smem, st, u, du, table,
+                       nmembers, inf.total_tiles);
     (void)hipEventRecord(e0, st);
     for (int k = 0; k < trials; ++k)
-      hipLaunchKernelGGL(batch_rhs_kernel, dim3(grid), dim3(kRhsThreads), inf.rhs_smem, st, u,
-                         du, table, nmembers, inf.total_tiles);
+      hipLaunchKernelGGL(rhs, dim3(grid), dim3(kRhsThreads), inf.rhs_smem, st, u, du, table,
+                         nmembers, inf.total_tiles);
     (void)hipEventRecord(e1, st);
     if (hipEventSynchronize(e1) != hipSuccess || hipGetLastError() != hipSuccess) {
       set_error("snapx_batch_autotune: trial launch failed");

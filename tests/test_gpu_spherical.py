@@ -184,3 +184,31 @@ def test_fused_steps_vs_fp64_oracle(case):
     m0 = (vol[I[1:]] * u[0][I[1:]]).sum()
     assert abs((vol[I[1:]] * got[0][I[1:]]).sum() - m0) / m0 < 1e-6
     assert np.array_equal(O.apply_bc(oc, got, O.CONSERVED), got)
+
+
+@pytest.mark.parametrize("scheme", [1, 9])
+def test_spherical_members_as_batch(scheme):
+    """Spherical-polar blocks of different shapes and seeds as members of one
+    batch (persistent K1 with geometric sources, cell-parallel spherical K2):
+    after the autotuner and two steps each member matches the FP64 oracle at
+    the fused bar; a batch that mixes coordinate systems is rejected."""
+    shapes = [(14, 10, 8), (32, 12, 1), (20, 6, 4)] if scheme == 1 else [(14, 10, 8), (24, 8, 1)]
+    cases = [_sector("hllc" if m % 2 else "lmars", scheme, n=n, seed=40 + m)
+             for m, n in enumerate(shapes)]
+    dt = 0.01
+    b = capi.Batch([cfg_of(oc) for oc, _, _ in cases], dt)
+    u = torch.cat([to_dev(u0, torch.float32).reshape(-1) for _, _, u0 in cases])
+    before = u.clone()
+    b.autotune(u, dt, trials=2)
+    assert torch.equal(u, before)
+    b.steps(u, 2)
+    for m, (oc, _, u0) in enumerate(cases):
+        u_ref = u0
+        for _ in range(2):
+            u_ref, _ = O.rk3_step(oc, dt, u_ref)
+        e = var_errors(host(b.member(u, m)), u_ref, O.interior(oc), vel_scale=1.0)
+        assert (e < 1e-5).all(), (m, shapes[m], e)
+    cart = O.make_config(nx1=16, nx2=8, nx3=1, nghost=3, x1min=0.0, x1max=1.0, x2min=0.0,
+                         x2max=1.0, implicit_scheme=scheme)
+    with pytest.raises(capi.SnapxError, match="same coordinate"):
+        capi.Batch([cfg_of(cases[0][0]), cfg_of(cart)], dt)
