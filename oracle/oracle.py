@@ -89,6 +89,10 @@ def lib():
         L.oc_vic_redistribute_cell.argtypes = [i, P, P, P, P, P, P, i, i, i, i, i]
         L.oc_inverse3.argtypes = [P, P]
         L.oc_vic_column_matrices.argtypes = [cfgp, d, P, i, i, P, P, P]
+        L.oc_flux_jacobian5.argtypes = [d, P, P]
+        L.oc_roe_state.argtypes = [d, P, P, P, P]
+        L.oc_roe_eigensystem.argtypes = [d, P, P, P, P, P, P]
+        L.oc_vic_assemble.argtypes = [i, P, C.c_long, i, d, P, P, d, d, P, P, P]
         L.oc_hydro_forward.argtypes = [cfgp, d, P, P, P, P, P, P]
         L.oc_apply_bc.argtypes = [cfgp, P, i, i]
         L.oc_max_time_step.restype = d
@@ -319,6 +323,70 @@ def vic_column_matrices(c, dt, w, k, j):
     w = np.ascontiguousarray(w, dtype=np.float64)
     lib().oc_vic_column_matrices(C.byref(c), dt, _p(w), k, j, _p(a), _p(b), _p(cc))
     return a, b, cc
+
+
+def flux_jacobian5(w, gammad=1.4):
+    """dF1/dU (5x5) at the primitive state w."""
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    J = np.empty((5, 5))
+    lib().oc_flux_jacobian5(gammad - 1.0, _p(w), _p(J))
+    return J
+
+
+def roe_state(wl, wr, gammad=1.4):
+    """Roe-averaged primitive state and sound speed of one interface."""
+    wl = np.ascontiguousarray(wl, dtype=np.float64)
+    wr = np.ascontiguousarray(wr, dtype=np.float64)
+    prim = np.empty(5)
+    cs = C.c_double()
+    lib().oc_roe_state(gammad, _p(wl), _p(wr), _p(prim), C.byref(cs))
+    return prim, cs.value
+
+
+def roe_eigensystem(wl, wr, gammad=1.4):
+    """(R, R^-1, signed eigenvalues, |A|) of the interface between wl and wr."""
+    wl = np.ascontiguousarray(wl, dtype=np.float64)
+    wr = np.ascontiguousarray(wr, dtype=np.float64)
+    R, Ri, A = np.empty((5, 5)), np.empty((5, 5)), np.empty((5, 5))
+    lam = np.empty(5)
+    lib().oc_roe_eigensystem(gammad, _p(wl), _p(wr), _p(R), _p(Ri), _p(lam), _p(A))
+    return R, Ri, lam, A
+
+
+def vic_assemble(N, wcol, area, vol, dt, grav, gammad=1.4):
+    """Blocks a, b, c ([nl, N, N]) of one column; wcol is [5, nl + 2] with one
+    ghost cell at each end, area has nl + 1 faces and vol nl cells."""
+    wcol = np.ascontiguousarray(wcol, dtype=np.float64)
+    nl = wcol.shape[1] - 2
+    area = np.ascontiguousarray(area, dtype=np.float64)
+    vol = np.ascontiguousarray(vol, dtype=np.float64)
+    assert area.shape == (nl + 1,) and vol.shape == (nl,)
+    a, b, cc = (np.zeros((nl, N, N)) for _ in range(3))
+    first = C.cast(wcol.ctypes.data + 8, C.POINTER(C.c_double))
+    lib().oc_vic_assemble(N, first, nl + 2, nl, gammad, _p(area), _p(vol), dt, grav,
+                          _p(a), _p(b), _p(cc))
+    return a, b, cc
+
+
+PLANTS = {"c_jac_i": 1, "no_bottom_fold": 2, "phi_place": 3, "b_abs_plus": 4,
+          "rinv_sign": 5, "fluxjac": 6, "roe_velocity": 7, "roe_upwind_swap": 8,
+          "roe_llf_speed": 9}
+
+
+class plant:
+    """Context manager that switches on one planted mistake in the oracle
+    (tests only: shows that a check fails on a plausible error)."""
+
+    def __init__(self, name):
+        self.code = PLANTS[name]
+
+    def __enter__(self):
+        C.c_int.in_dll(lib(), "oc_plant").value = self.code
+        return self
+
+    def __exit__(self, *exc):
+        C.c_int.in_dll(lib(), "oc_plant").value = 0
+        return False
 
 
 def inverse3(m):

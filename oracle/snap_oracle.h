@@ -145,6 +145,22 @@ void oc_vic_redistribute_cell(int N, double* du, const double* w,
                               int dir, int ny, int stride1, int stride2);
 /* dense helpers exposed for brute-force checks */
 void oc_inverse3(const double* m, double* inv);
+/* flux Jacobian dF1/dU at primitive state w (flux_decomposition_impl.h:107-128) */
+void oc_flux_jacobian5(double gm1, const double* w, double* J);
+/* Roe-averaged primitive state and sound speed of an interface
+   (flux_decomposition_impl.h:16-70) */
+void oc_roe_state(double gammad, const double* wl, const double* wr, double* prim,
+                  double* cs);
+/* R, R^-1, signed eigenvalues and |A| = R|Lambda|R^-1 at the Roe state
+   (flux_decomposition_impl.h:72-105, vic_assemble_partial_impl.h:70-71) */
+void oc_roe_eigensystem(double gammad, const double* wl, const double* wr, double* R,
+                        double* Ri, double* lam, double* absA);
+/* block coefficients of one column: w points at the first interior cell,
+   w[v*stride1 + i] valid for i = -1..nl; area has nl+1 faces, vol nl cells
+   (vic_assemble_partial_impl.h:28-111, vic_assemble_full_impl.h:15-106) */
+void oc_vic_assemble(int N, const double* w, long stride1, int nl, double gammad,
+                     const double* area, const double* vol, double dt, double grav,
+                     double* a, double* b, double* c);
 void oc_vic_column_matrices(const oc_config* c, double dt, const double* w,
                             int k, int j, double* a, double* b, double* cc);
 
@@ -161,6 +177,22 @@ void oc_rk3_step(const oc_config* c, double dt, double* u, double* w);
 /* initialisation from primitives: BCs on w, W->U, BCs on u
    (src/mesh/meshblock.cpp:412-510) */
 void oc_initialize(const oc_config* c, double* w, double* u);
+
+/* planted mistakes, for the tests that show each independent check fails
+   on a plausible error (0 = none) */
+extern int oc_plant;
+enum {
+  OC_PLANT_NONE = 0,
+  OC_PLANT_C_JAC_I = 1,        /* c_i uses J_i in place of J_{i+1} */
+  OC_PLANT_NO_BOTTOM_FOLD = 2, /* reflecting fold skipped at the column bottom */
+  OC_PLANT_PHI_PLACE = 3,      /* energy row of Phi on the density column */
+  OC_PLANT_B_ABS_PLUS = 4,     /* b_i uses |A| of the upper face */
+  OC_PLANT_RINV_SIGN = 5,      /* sign of one left-eigenvector entry */
+  OC_PLANT_FLUXJAC = 6,        /* v1*v1 -> v1*v2 in one flux-Jacobian entry */
+  OC_PLANT_ROE_VELOCITY = 7,   /* arithmetic in place of sqrt(rho)-weighted Roe velocity */
+  OC_PLANT_ROE_UPWIND_SWAP = 8, /* Roe supersonic switches take the downwind flux */
+  OC_PLANT_ROE_LLF_SPEED = 9   /* LLF speed without the factor 1/2 */
+};
 
 void oc_set_threads(int n);
 int oc_get_threads(void);

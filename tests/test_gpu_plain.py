@@ -115,6 +115,42 @@ def test_riemann_parity(solver, dim):
     assert (np.abs(host(fp) - fpr) / np.abs(fpr).max()).max() < 1e-13
 
 
+def _branch_faces(dim):
+    """Faces of four regimes interleaved along x1: supersonic to +n, to -n,
+    strong rarefaction (LLF fallback of Roe) and subsonic close states."""
+    from test_oracle_vic_blocks import faces
+    parts = [faces(kind, dim, n=48, seed=5) for kind in ("right", "left", "llf", "sub")]
+    wl = np.concatenate([p[0] for p in parts], axis=-1)
+    wr = np.concatenate([p[1] for p in parts], axis=-1)
+    return wl, wr
+
+
+@pytest.mark.parametrize("solver", ["roe", "lmars", "hllc"])
+@pytest.mark.parametrize("dim", [3, 2, 1])
+def test_riemann_parity_every_branch(solver, dim):
+    """Supersonic (lambda_- > 0, lambda_+ < 0), LLF-fallback and subsonic
+    faces (src/riemann/roe_impl.h:178-211) against the oracle, whose branches
+    are pinned by test_oracle_vic_blocks.py: FP64 to 1e-13, FP32 to 1e-5
+    (per variable, scaled by its largest flux)."""
+    from test_oracle_vic_blocks import expected_flux
+    wl, wr = _branch_faces(dim)
+    ref, fpr = O.riemann(solver, wl, wr, dim)
+    if solver == "roe":
+        # the oracle itself takes the pinned branch on each regime
+        for q, kind in enumerate(("right", "left", "llf")):
+            s = slice(48 * q, 48 * (q + 1))
+            e = expected_flux(kind, wl[..., s], wr[..., s], dim)
+            assert np.abs(ref[..., s] - e).max() < 1e-12 * np.abs(e).max()
+    scale = np.abs(ref).max(axis=(1, 2, 3), keepdims=True)
+    for dt, tol in ((torch.float64, 1e-13), (torch.float32, 1e-5)):
+        flx = torch.empty(wl.shape, dtype=dt, device=DEV)
+        fp = torch.empty(wl.shape[1:], dtype=dt, device=DEV)
+        capi.riemann(solver, to_dev(wl, dt), to_dev(wr, dt), flx, fp, dim)
+        got = host(flx)
+        assert (np.abs(got - ref) / scale).max() < tol, (dt, (np.abs(got - ref) / scale).max())
+        assert (np.abs(host(fp) - fpr) / np.abs(fpr).max()).max() < tol
+
+
 def test_riemann_reference_face_pressure_seeds():
     """tests/test_riemann.cpp:412-570 inputs (torch seeds 77/78/79) through the
     GPU solvers against the oracle (itself pinned to the closed forms)."""
