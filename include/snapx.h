@@ -21,7 +21,9 @@
  *  - The caller owns all memory.  Entry points that need scratch take a
  *    workspace pointer and its size; *_workspace_bytes() returns the size.
  *  - Entry points are reentrant: there is no global mutable state apart from
- *    the thread-local error string.
+ *    the thread-local error string, and the static per-GPU workspace of
+ *    snapx_fused_rk3_steps_static (the one entry point that allocates
+ *    nothing; its calls on one GPU must not overlap).
  *  - Geometry (snapx_config.coord) is Cartesian with uniform face widths
  *    (x?max - x?min)/nx? (src/coord/cartesian.cpp:8-47), or spherical-polar
  *    (r, theta, phi) on the plain FP64/FP32 path: faces uniform in each
@@ -271,6 +273,19 @@ typedef struct snapx_timing {
 int snapx_fused_rk3_steps(const snapx_config* cfg, double dt, int nsteps,
                           float* u, void* workspace, size_t workspace_bytes,
                           snapx_timing* timing, void* stream);
+
+/* No reference counterpart.  For callers that cannot allocate device memory
+   (code under HIP graph capture, tight serving loops, harnesses that forbid
+   hipMalloc): snapx_fused_rk3_steps (timing NULL) with its workspace in a
+   static device array of SNAPX_STATIC_SCRATCH_BYTES per GPU, which each GPU
+   the library runs on holds until the process ends.  Calls no hipMalloc,
+   hipFree or allocating library routine.  Blocks whose workspace
+   (snapx_fused_workspace_bytes) exceeds the array are rejected with
+   SNAPX_ESPACE.  Calls on one GPU share the array: calls that overlap on one
+   GPU (two streams, two threads) race and must be serialised by the caller. */
+#define SNAPX_STATIC_SCRATCH_BYTES (64u << 20)
+int snapx_fused_rk3_steps_static(const snapx_config* cfg, double dt, int nsteps,
+                                 float* u, void* stream);
 
 /* ---------------------------------------------------------------------- */
 /* Adjoint (no reference counterpart)                                      */

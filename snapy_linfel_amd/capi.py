@@ -128,6 +128,7 @@ SIGNATURES = {
     "snapx_fused_workspace_bytes": (_Z, [_CFG]),
     "snapx_fused_rk3_step": (_I, [_CFG, _D, _P, _P, _Z, _P]),
     "snapx_fused_rk3_steps": (_I, [_CFG, _D, _I, _P, _P, _Z, _P, _P]),
+    "snapx_fused_rk3_steps_static": (_I, [_CFG, _D, _I, _P, _P]),
     "snapx_adjoint_workspace_bytes": (_Z, [_CFG]),
     "snapx_adjoint_rk3_step": (_I, [_CFG, _D, _P, _P, _P, _Z, _P]),
     "snapx_batch_workspace_bytes": (_Z, [_CFG, _I]),
@@ -327,6 +328,17 @@ def fused_rk3_steps(cfg, dt, nsteps, u, ws=None, timing=None):
     tp = C.byref(timing) if timing is not None else None
     check(lib().snapx_fused_rk3_steps(C.byref(cfg), dt, nsteps, ptr(u), ptr(ws), ws.numel(), tp,
                                       stream_ptr()), "fused_rk3_steps")
+
+
+STATIC_SCRATCH_BYTES = 64 << 20
+ESPACE = -3
+
+
+def fused_rk3_steps_static(cfg, dt, nsteps, u):
+    """fused_rk3_steps with the library's static per-GPU workspace: no
+    allocation at all (calls that overlap on one GPU race)."""
+    check(lib().snapx_fused_rk3_steps_static(C.byref(cfg), dt, nsteps, ptr(u), stream_ptr()),
+          "fused_rk3_steps_static")
 
 
 def fused_rk3_step(cfg, dt, u, ws=None):

@@ -29,6 +29,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "common.hpp"
@@ -2071,6 +2073,62 @@ extern "C" int snapx_fused_rk3_steps(const snapx_config* cfg, double dt, int nst
     hipEventDestroy(e.b);
   }
   return status;
+}
+
+// ------------------------------------- fused steps without allocation ----
+
+namespace snapx {
+
+// Workspace of snapx_fused_rk3_steps_static: one array per GPU, held from
+// the first load of this library's code object on that GPU until the process
+// ends.  Every call on a GPU shares it (the last call's contents stay); no
+// part of it has to start at zero, because every kernel of a fused step
+// writes each word it later reads within the same step (K1 writes the
+// tendency of every interior cell, stage 0 of K2 writes the stage-0 state,
+// the lane-per-column solve writes its column scratch in the forward sweep).
+static __device__ __align__(256) char scratch[SNAPX_STATIC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mutex;
+static void* g_scratch_ptr[kMaxDevices];
+
+// Device address of the current GPU's copy of `scratch` (the array's name in
+// host code is only a stand-in for the symbol).  Looked up once per device.
+static int static_scratch(void** out) {
+  int dev = 0;
+  SNAPX_HIP(hipGetDevice(&dev));
+  SNAPX_CHECK_ARG(dev >= 0 && dev < kMaxDevices, "snapx_fused_rk3_steps_static: device index");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    SNAPX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    SNAPX_CHECK_ARG(p != nullptr, "snapx_fused_rk3_steps_static: no device address for scratch");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+}  // namespace snapx
+
+extern "C" int snapx_fused_rk3_steps_static(const snapx_config* cfg, double dt, int nsteps,
+                                            float* u, void* stream) {
+  if (int e = check_fused(cfg)) return e;
+  SNAPX_CHECK_ARG(u && nsteps >= 0, "snapx_fused_rk3_steps_static: bad arguments");
+  const size_t need = snapx_fused_workspace_bytes(cfg);
+  if (need > (size_t)SNAPX_STATIC_SCRATCH_BYTES) {
+    set_error("snapx_fused_rk3_steps_static: the block needs " + std::to_string(need) +
+              " workspace bytes, more than the static " +
+              std::to_string((size_t)SNAPX_STATIC_SCRATCH_BYTES) +
+              " (use snapx_fused_rk3_steps with a caller workspace)");
+    return SNAPX_ESPACE;
+  }
+  void* ws = nullptr;
+  if (int e = static_scratch(&ws)) return e;
+  hipStream_t st = as_stream(stream);
+  for (int n = 0; n < nsteps; ++n)
+    if (int e = fused_step(cfg, dt, u, ws, st, nullptr, n == nsteps - 1)) return e;
+  return 0;
 }
 
 // ---------------------------------------------------- batched C ABI ----
