@@ -1,21 +1,29 @@
 """Benchmark: cell-updates/s per SSP-RK3 step on one MI355X.
 
-Workload (the configuration the metric is quoted on): 2-D Straka density
-current, 512 x 64 cells (x2 horizontal, periodic, 25.6 km; x1 vertical,
-reflecting, 6.4 km), dry ideal gas cp = 1004, R = 287, theta0 = 300 K,
--15 K cosine bubble, gravity 9.81, WENO5 (shock: false) + LMARS, vertically
-implicit 3x3 solve, SSP-RK3 with dt = 0.1 s.  Synthetic initial state (the
-problem set-up of examples/straka.cpp), inputs resident in HBM.
+Headline workload (the largest single-GPU configuration of the metric):
+3-D dry convective box, 256 x 256 x 128 cells of 100 m (x2, x3 horizontal and
+periodic, 25.6 km; x1 vertical and reflecting, 12.8 km), ideal gas gamma 1.4,
+R = 287, gravity 9.81, hydrostatic background of a dry adiabat from 250 K at
+the ground capped by an isothermal layer at 200 K (the profile of
+examples/linfel2026/uranus_dry_cart.cpp:86-101), theta += U(-0.5, 0.5) K in
+the lowest 20 levels (torch CPU generator, seed 1234) at fixed pressure,
+WENO5 (shock: false) with Roe in x2 / x3 and LMARS in x1, vertically
+implicit 3x3 column solve (65 536 columns), SSP-RK3, dt = 0.2 s, 100 steps.
+Synthetic initial state, inputs resident in HBM.
 
 Reported:
-  value        FP32 fused path, interior cells x steps / wall seconds
+  value        FP32 fused path (FP64 column solve): interior cells x steps /
+               wall seconds
   fp64_plain   the FP64 plain (module-by-module) GPU path on the same steps
-  error        per-field L2 / Linf relative error of FP32 fused vs FP64 plain
-               after the timed steps, and the FP32 mass drift
+  error        per-variable L2 / Linf relative error of FP32 fused vs FP64
+               plain after the timed steps (each field against its own norm,
+               and momenta also against rho_max x 300 m/s), mass drift
+  oracle_check both GPU paths against the FP64 CPU oracle after 2 steps
   roofline     dominant kernel of the fused step, algorithmic bytes per
                launch / its average device time (HIP events on the stream)
-  cpu_baseline the FP64 CPU oracle on a bounded sample of the workload
-  step_hbm_frac  160 B per cell-step (FP32) x CU/s / 8 TB/s
+  cpu_baseline the FP64 CPU oracle timed on a 2-step window of the workload
+  extra        Straka 512 x 64, spherical-polar sector, 96-member ensemble,
+               adjoint / forward and the adjoint sensitivity run
 Run ``python bench.py`` (N = 1); the package runs on one GPU.
 """
 import argparse
@@ -119,6 +127,41 @@ def box3d_cfg(capi, n=(128, 256, 256)):
                                 "periodic"))
 
 
+def capped_background(z, T0=250.0, Tmin=200.0, R=287.0, gamma=1.4, grav=9.81, P0=1e5):
+    """Hydrostatic (T, p) at heights z (torch tensor): a dry adiabat from T0 at
+    z = 0, capped by an isothermal layer at Tmin above
+    z_iso = cp (T0 - Tmin) / g (examples/linfel2026/uranus_dry_cart.cpp:86-101)."""
+    import torch
+    cp = gamma / (gamma - 1.0) * R
+    t_ad = T0 - grav * z / cp
+    T = torch.clamp_min(t_ad, Tmin)
+    z_iso = cp * (T0 - Tmin) / grav
+    p_ad = P0 * (torch.clamp_min(t_ad, Tmin) / T0) ** (cp / R)
+    p_iso = P0 * (Tmin / T0) ** (cp / R) * torch.exp(-grav * (z - z_iso) / (R * Tmin))
+    return T, torch.where(t_ad > Tmin, p_ad, p_iso)
+
+
+def box3d_prim(cfg, dz=100.0, levels=20, seed=1234, R=287.0, P0=1e5):
+    """Primitive initial state of the 3-D box (ghosts included): the capped
+    background with theta += U(-0.5, 0.5) K, i.i.d. per cell, in the lowest
+    `levels` interior levels, applied at fixed pressure."""
+    import torch
+    shp = cfg.shape
+    g = cfg.nghost
+    cp = cfg.gammad / (cfg.gammad - 1.0) * R
+    z = (torch.arange(shp[3], dtype=torch.float64) - g + 0.5) * dz
+    T, p = capped_background(z, gamma=cfg.gammad)
+    gen = torch.Generator().manual_seed(seed)
+    dth = torch.rand(shp[1:], generator=gen, dtype=torch.float64) - 0.5
+    keep = torch.zeros(shp[3], dtype=torch.float64)
+    keep[g:g + levels] = 1.0
+    dT = dth * keep * (p / P0) ** (R / cp)
+    w = torch.zeros(shp, dtype=torch.float64)
+    w[4] = p
+    w[0] = p / (R * (T + dT))
+    return w
+
+
 def ensemble_members(capi, n=96, seed=2024):
     """Batched warm-bubble ensemble: grid sizes drawn by seed from
     {128,192,256,384,512} x {64,96,128}, 50 m cells, dry ideal gas with
@@ -157,7 +200,8 @@ def ensemble_members(capi, n=96, seed=2024):
 
 def c5_sensitivity(capi, dev, nsteps=200, every=20, ndir=2, wind=(0.0, 0.0), richardson=False,
                    roll=0.0, eps=1e-6):
-    """Adjoint sensitivity on the 3-D box 128 x 128 x 96 of 200 m (FP64):
+    """Adjoint sensitivity on the 3-D box 128 x 128 x 96 of 200 m with the
+    capped background of the 3-D workload (FP64):
     loss = 0.5 sum (theta_final - theta_target)^2 after nsteps SSP-RK3 steps,
     gradient with respect to the initial theta perturbation of the lowest 20
     levels, through the hand-written adjoint with a checkpoint every `every`
@@ -171,13 +215,12 @@ def c5_sensitivity(capi, dev, nsteps=200, every=20, ndir=2, wind=(0.0, 0.0), ric
     faces leave the upwind switch of LMARS as well."""
     import torch
     from snapy_linfel_amd import adjoint as A
-    R, cp, g0, p0 = 287.0, 1004.5, 9.81, 1e5
+    R, cp, p0 = 287.0, 1004.5, 1e5
     cfg = box3d_cfg(capi, n=(96, 128, 128))
     cfg.x1max, cfg.x2max, cfg.x3max = 200.0 * 96, 200.0 * 128, 200.0 * 128
     shp = cfg.shape
     z = (torch.arange(shp[3], dtype=torch.float64, device=dev) - 3 + 0.5) * 200.0
-    T0 = 250.0 - g0 * z / cp
-    pb = p0 * (T0 / 250.0) ** (cp / R)
+    T0, pb = capped_background(z)
     thb = T0 * (p0 / pb) ** (R / cp)
     gam = cfg.gammad
 
@@ -303,8 +346,9 @@ def measured_traffic(kernel, workload, what="traffic"):
 
 
 def errors(got, ref, interior, vel_scale=300.0):
-    """Per conserved variable: Linf of the difference over max |ref| (momenta:
-    at least max(rho) * vel_scale), and the relative L2 norm."""
+    """Per conserved variable, against the field's own norm: Linf of the
+    difference over max |ref| and the relative L2 norm; for the momenta also
+    the Linf over max(max |ref|, max(rho) * vel_scale)."""
     import numpy as np
     out = {}
     names = ["rho", "mom1", "mom2", "mom3", "E"]
@@ -313,62 +357,195 @@ def errors(got, ref, interior, vel_scale=300.0):
         g, r = got[v][interior], ref[v][interior]
         if np.abs(r).max() == 0:
             continue
-        scale = max(np.abs(r).max(), rho_max * vel_scale if v in (1, 2, 3) else 0.0)
-        out[name] = {
-            "linf_rel": float(np.abs(g - r).max() / scale),
-            "l2_rel": float(np.linalg.norm(g - r) / max(np.linalg.norm(r), 1e-300)),
-        }
+        d = np.abs(g - r).max()
+        e = {"linf_rel": float(d / np.abs(r).max()),
+             "l2_rel": float(np.linalg.norm(g - r) / max(np.linalg.norm(r), 1e-300))}
+        if v in (1, 2, 3):
+            e["linf_rel_vs_rho_max_x_%g_m_s" % vel_scale] = float(
+                d / max(np.abs(r).max(), rho_max * vel_scale))
+        out[name] = e
     return out
 
 
-def cpu_baseline(steps_cap=400, seconds=10.0):
-    """The FP64 C oracle (OpenMP over cells and columns) on the host cores this
-    process may use (at most 16), and on one thread for scale."""
-    import oracle as O
-    oc = O.straka_config(nx2=512, nx1=64, implicit_scheme=1)
-    w, u0 = O.initialize(oc, O.straka_init(oc))
+def max_rel(got, ref, interior):
+    """Per conserved variable, max |got - ref| / max |ref| over the interior."""
+    import numpy as np
+    names = ["rho", "mom1", "mom2", "mom3", "E"]
+    return {n: float(np.abs(got[v][interior] - ref[v][interior]).max()
+                     / max(np.abs(ref[v][interior]).max(), 1e-300))
+            for v, n in enumerate(names)}
 
-    def run(threads, secs, cap):
-        O.set_threads(threads)
-        u = u0
-        t0 = time.perf_counter()
-        n = 0
-        while n < cap and time.perf_counter() - t0 < secs:
-            u, _ = O.rk3_step(oc, 0.1, u)
-            n += 1
-        return n, time.perf_counter() - t0
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def host_cores():
     try:
-        cores = min(len(os.sched_getaffinity(0)), 16)
+        return min(len(os.sched_getaffinity(0)), 16)
     except AttributeError:
-        cores = min(os.cpu_count() or 1, 16)
-    n1, t1 = run(1, seconds * 0.5, steps_cap)
-    nn, tn = run(cores, seconds, steps_cap)
-    O.set_threads(1)
-    model = ""
+        return min(os.cpu_count() or 1, 16)
+
+
+def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
-            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")),
-                         "")
+            return next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")),
+                        "")
     except OSError:
-        pass
-    return {"value": round(512 * 64 * nn / tn, 1), "unit": "cell-updates/s", "cores": cores,
-            "cpu_model": model, "cpus_online": os.cpu_count(),
-            "kind": "port", "value_1thread": round(512 * 64 * n1 / t1, 1),
-            "sample": f"FP64 C oracle, Straka 512x64, RK3 steps from the initial state: "
-                      f"{nn} steps on {cores} threads ({tn:.1f} s), "
-                      f"{n1} steps on 1 thread ({t1:.1f} s)"}
+        return ""
+
+
+def cpu_baseline(cfg, u0, dt, steps=2, sub=64):
+    """The FP64 C oracle (OpenMP over cells and columns) on a `steps`-step
+    window of the workload, on the host cores this process may use (at most
+    16); for scale, one step on one thread of a sub x sub column patch of the
+    same block (periodic horizontally, so the same per-cell work).  Returns
+    (record, state after the window)."""
+    import oracle as O
+    oc = O.OcConfig.from_buffer_copy(cfg)
+    cores = host_cores()
+    O.set_threads(cores)
+    u = u0.copy()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        u, _ = O.rk3_step(oc, dt, u)
+    tn = time.perf_counter() - t0
+    ps = O.OcConfig.from_buffer_copy(cfg)
+    ps.nx2, ps.nx3 = sub, sub
+    ps.x2max = cfg.x2min + (cfg.x2max - cfg.x2min) * sub / cfg.nx2
+    ps.x3max = cfg.x3min + (cfg.x3max - cfg.x3min) * sub / cfg.nx3
+    patch = O.apply_bc(ps, u0[:, :sub + 6, :sub + 6, :].copy(), O.CONSERVED)
+    O.set_threads(1)
+    t0 = time.perf_counter()
+    O.rk3_step(ps, dt, patch)
+    t1 = time.perf_counter() - t0
+    n = cfg.n_interior
+    rec = {"value": round(n * steps / tn, 1), "unit": "cell-updates/s", "cores": cores,
+           "cpu_model": cpu_model(), "cpus_online": os.cpu_count(), "kind": "port",
+           "value_1thread": round(sub * sub * cfg.nx1 / t1, 1),
+           "sample": f"FP64 C oracle (restatement of the reference CPU path): the headline "
+                     f"workload's first {steps} RK3 steps on {cores} threads ({tn:.1f} s); "
+                     f"value_1thread: one step of a {sub}x{sub}x{cfg.nx1} column patch of it on "
+                     f"1 thread ({t1:.1f} s)"}
+    return rec, u
+
+
+def kernel_roofline(name, avg_ms, bytes_, workload):
+    """Roofline record of one kernel: algorithmic bytes per launch over its
+    average device time, with the counter traffic and VALU rates recorded in
+    profiles/fused_kernels_pmc.md for this kernel and workload."""
+    achieved = bytes_ / (avg_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "traffic": measured_traffic(name, workload),
+            "kernel": name, "avg_launch_ms": round(avg_ms, 5), "bytes_per_launch": bytes_}
+    valu = measured_traffic(name, workload, "valu")
+    if valu:
+        roof["valu_wave_instructions_per_launch"] = valu
+        roof["valu_issue_frac"] = round(valu / (avg_ms * 1e-3) / VALU_ISSUE_PEAK, 4)
+    fl = measured_flops(name, workload)
+    if fl:
+        roof["valu_flops_per_launch"] = {"f32": fl[0], "f64": fl[1]}
+        roof["valu_tflops"] = round((fl[0] + fl[1]) / (avg_ms * 1e-3) / 1e12, 3)
+        roof["valu_flop_frac"] = round((fl[0] + fl[1]) / (avg_ms * 1e-3) / VALU_FLOP_PEAK, 4)
+    return roof
+
+
+def copy_rate_gbs(dev):
+    """Attainable HBM rate: a 2 GiB device-to-device copy (read + write)."""
+    import torch
+    src = torch.empty(1 << 29, dtype=torch.float32, device=dev).fill_(1.0)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(10):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    return 2 * src.numel() * 4 * 10 / (e0.elapsed_time(e1) * 1e-3) / 1e9
+
+
+def fused_vs_plain(capi, cfg, u0, dt, K, W, dev, timing_steps=3):
+    """Times K fused FP32 steps and K plain FP64 steps from the same state
+    (after W warm-up steps on copies); per-kernel device times of the fused
+    step from HIP events over `timing_steps` more steps."""
+    import torch
+    u32 = torch.from_numpy(u0).to(dev, torch.float32)
+    ws32 = capi.fused_workspace(cfg)
+    capi.fused_rk3_steps(cfg, dt, W, u32.clone(), ws32)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    capi.fused_rk3_steps(cfg, dt, K, u32, ws32)
+    torch.cuda.synchronize()
+    t32 = time.perf_counter() - t0
+    tim = capi.Timing()
+    capi.fused_rk3_steps(cfg, dt, timing_steps, u32.clone(), ws32, timing=tim)
+    torch.cuda.synchronize()
+    del ws32
+    u64 = torch.from_numpy(u0).to(dev, torch.float64)
+    w64 = torch.empty_like(u64)
+    ws64 = capi.rk3_workspace(cfg, capi.F64)
+    uw = u64.clone()
+    for _ in range(max(W, 1)):
+        capi.rk3_step(cfg, dt, uw, w64, ws64)
+    del uw
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(K):
+        capi.rk3_step(cfg, dt, u64, w64, ws64)
+    torch.cuda.synchronize()
+    t64 = time.perf_counter() - t0
+    del ws64, w64
+    return {"u32": u32, "u64": u64, "t32": t32, "t64": t64,
+            "rhs_ms": tim.rhs_ms / max(tim.rhs_launches, 1),
+            "col_ms": tim.column_ms / max(tim.column_launches, 1),
+            "launches": int(tim.rhs_launches + tim.column_launches + tim.bc_launches)}
+
+
+def adjoint_ratio(capi, cfg, u0, dt, dev, n=3):
+    """FP64 adjoint step (it recomputes its three stages) over the FP64 plain
+    forward step, on the same state."""
+    import torch
+    u = torch.from_numpy(u0).to(dev, torch.float64)
+    w = torch.empty_like(u)
+    ws = capi.rk3_workspace(cfg, capi.F64)
+    capi.rk3_step(cfg, dt, u.clone(), w, ws)
+    torch.cuda.synchronize()
+    uf = u.clone()
+    t0 = time.perf_counter()
+    for _ in range(n):
+        capi.rk3_step(cfg, dt, uf, w, ws)
+    torch.cuda.synchronize()
+    tf = (time.perf_counter() - t0) / n
+    del ws, uf, w
+    wsa = capi.adjoint_workspace(cfg)
+    lam = torch.randn(u.shape, dtype=torch.float64,
+                      generator=torch.Generator().manual_seed(5)).to(dev)
+    capi.adjoint_rk3_step(cfg, dt, u, lam.clone(), wsa)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(n):
+        capi.adjoint_rk3_step(cfg, dt, u, lam, wsa)
+    torch.cuda.synchronize()
+    ta = (time.perf_counter() - t0) / n
+    finite = bool(torch.isfinite(lam).all())
+    del wsa, lam, u
+    return {"forward_ms_per_step": round(tf * 1e3, 3), "adjoint_ms_per_step": round(ta * 1e3, 3),
+            "adjoint_over_forward": round(ta / tf, 3), "finite": finite}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-3d", action="store_true")
+    ap.add_argument("--steps", type=int, default=100, help="timed steps of the 3-D workload")
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and oracle check")
+    ap.add_argument("--no-straka", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-sph", action="store_true", help="skip the spherical-polar sector")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip every adjoint timing")
     ap.add_argument("--no-c5", action="store_true", help="skip the 200-step sensitivity run")
     ap.add_argument("--c5-directions", type=int, default=2,
                     help="random directions of the finite-difference check of the gradient")
@@ -383,338 +560,170 @@ def main():
     from snapy_linfel_amd import capi
 
     dev = torch.device("cuda:0")
-    cfg = straka_cfg(capi)
-    u0 = conserved_on_device(capi, cfg, straka_prim(cfg), dev).cpu().numpy()
-    I = (slice(None), slice(3, -3), slice(3, -3))
-    dt = 0.1
     K, W = args.steps, args.warmup
+    extra = {}
 
-    # ---- FP32 fused: warm-up, then K timed steps (launches only) ----
-    u32 = torch.from_numpy(u0).to(dev, torch.float32)
-    ws32 = capi.fused_workspace(cfg)
-    capi.fused_rk3_steps(cfg, dt, W, u32.clone(), ws32)
-    torch.cuda.synchronize()
-    u32.copy_(torch.from_numpy(u0).to(dev, torch.float32))
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    capi.fused_rk3_steps(cfg, dt, K, u32, ws32)
-    torch.cuda.synchronize()
-    t32 = time.perf_counter() - t0
-    cus32 = cfg.n_interior * K / t32
-
-    # ---- per-kernel device times (HIP events around every launch) ----
-    tim = capi.Timing()
-    ut = torch.from_numpy(u0).to(dev, torch.float32)
-    nk = min(K, 200)
-    capi.fused_rk3_steps(cfg, dt, nk, ut, ws32, timing=tim)
-    torch.cuda.synchronize()
-    rhs_avg = tim.rhs_ms / max(tim.rhs_launches, 1)
-    col_avg = tim.column_ms / max(tim.column_launches, 1)
-
-    # ---- FP64 plain path on the same steps ----
-    u64 = torch.from_numpy(u0).to(dev, torch.float64)
-    w64 = torch.empty_like(u64)
-    ws64 = capi.rk3_workspace(cfg, capi.F64)
-    for _ in range(W):
-        capi.rk3_step(cfg, dt, u64.clone(), w64, ws64)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(K):
-        capi.rk3_step(cfg, dt, u64, w64, ws64)
-    torch.cuda.synchronize()
-    t64 = time.perf_counter() - t0
-    cus64 = cfg.n_interior * K / t64
-
-    # ---- adjoint of the FP64 step (stage recomputation included) ----
-    ws_adj = capi.adjoint_workspace(cfg)
-    lam = torch.randn(u64.shape, dtype=torch.float64,
-                      generator=torch.Generator().manual_seed(5)).to(dev)
-    na = max(1, min(K, 50))
-    capi.adjoint_rk3_step(cfg, dt, u64, lam.clone(), ws_adj)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(na):
-        capi.adjoint_rk3_step(cfg, dt, u64, lam, ws_adj)
-    torch.cuda.synchronize()
-    t_adj = (time.perf_counter() - t0) / na
-    adj_ratio = t_adj / (t64 / K)
-    del ws_adj, lam
-
-    got = u32.double().cpu().numpy()
-    ref = u64.cpu().numpy()
+    # ---------------- headline: 3-D box 256 x 256 x 128 ----------------
+    cfg = box3d_cfg(capi)
+    dt = 0.2  # horizontal acoustic CFL 0.7 on 100 m cells (0.5 s blows up, oracle too)
+    u0 = conserved_on_device(capi, cfg, box3d_prim(cfg).numpy(), dev).cpu().numpy()
+    I = (slice(3, -3), slice(3, -3), slice(3, -3))  # interior of one field [n3, n2, n1]
+    log(f"3-D box 256x256x128: {K} fused FP32 and {K} plain FP64 steps")
+    r = fused_vs_plain(capi, cfg, u0, dt, K, W, dev)
+    cus32 = cfg.n_interior * K / r["t32"]
+    cus64 = cfg.n_interior * K / r["t64"]
+    got, ref = r["u32"].double().cpu().numpy(), r["u64"].cpu().numpy()
     err = errors(got, ref, I)
     m0 = u0[0][I].sum()
     mass_drift = float(abs(got[0][I].sum() - m0) / m0)
     mass_drift64 = float(abs(ref[0][I].sum() - m0) / m0)
+    finite = bool(np.isfinite(got).all())
+    del r["u32"], r["u64"], got, ref
 
-    # ---- roofline of the dominant kernel ----
-    # K1 (fused RHS) algorithmic bytes: read the stage state + write the
-    # tendency, 2 * NV * 4 B per interior cell; K2 (column solve + combine):
-    # read tendency, stage state, u0 and write the state, 4 * NV * 4 B.
+    # roofline of the dominant kernel: K1 (fused RHS, 3-D marching tiles)
+    # reads the stage state and writes the tendency, 2 NV 4 B per interior
+    # cell; K2 (column solve + stage combine, two rows per lane) reads the
+    # tendency, the stage state and u0 and writes the state, 4 NV 4 B
     n = cfg.n_interior
-    k1_bytes, k2_bytes = 2 * NV * 4 * n, 4 * NV * 4 * n
-    if rhs_avg >= col_avg:
-        name, avg_ms, bytes_ = "fused_rhs_kernel", rhs_avg, k1_bytes
-    else:
-        name, avg_ms, bytes_ = "fused_column_pcr_kernel<3>", col_avg, k2_bytes
-    achieved = bytes_ / (avg_ms * 1e-3) / 1e9
-    roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": measured_traffic(name, "straka512x64"),
-            "kernel": name, "avg_launch_ms": round(avg_ms, 5),
-            "bytes_per_launch": bytes_}
-    valu = measured_traffic(name, "straka512x64", "valu")
-    if valu:
-        roof["valu_wave_instructions_per_launch"] = valu
-        roof["valu_issue_frac"] = round(valu / (avg_ms * 1e-3) / VALU_ISSUE_PEAK, 4)
-    fl = measured_flops(name, "straka512x64")
-    if fl:
-        roof["valu_flops_per_launch"] = {"f32": fl[0], "f64": fl[1]}
-        roof["valu_tflops"] = round((fl[0] + fl[1]) / (avg_ms * 1e-3) / 1e12, 3)
-        roof["valu_flop_frac"] = round((fl[0] + fl[1]) / (avg_ms * 1e-3) / VALU_FLOP_PEAK, 4)
-
-    # attainable HBM rate: a 2 GiB device-to-device copy (read + write bytes)
-    src = torch.empty(1 << 29, dtype=torch.float32, device=dev).fill_(1.0)
-    dst = torch.empty_like(src)
-    dst.copy_(src)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(10):
-        dst.copy_(src)
-    e1.record()
-    torch.cuda.synchronize()
-    copy_gbs = 2 * src.numel() * 4 * 10 / (e0.elapsed_time(e1) * 1e-3) / 1e9
-    del src, dst
+    k1 = ("fused_rhs_march_kernel", r["rhs_ms"], 2 * NV * 4 * n)
+    k2 = ("fused_column_pcr2_kernel", r["col_ms"], 4 * NV * 4 * n)
+    dom = k1 if k1[1] >= k2[1] else k2
+    roof = kernel_roofline(*dom, "box256x256x128")
+    other = k2 if dom is k1 else k1
+    roof["other_kernel"] = {"kernel": other[0], "avg_launch_ms": round(other[1], 5),
+                            "hbm_frac": round(other[2] / (other[1] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                              5)}
+    ovr = kernel_roofline(*other, "box256x256x128")
+    for key in ("traffic", "valu_issue_frac", "valu_flop_frac"):
+        if key in ovr:
+            roof["other_kernel"][key] = ovr[key]
+    copy_gbs = copy_rate_gbs(dev)
     roof["copy_measured_GBs"] = round(copy_gbs, 1)
-    roof["frac_of_copy"] = round(achieved / copy_gbs, 5)
+    roof["frac_of_copy"] = round(roof["achieved"] / copy_gbs, 5)
 
-    # the floor a launch of this size meets on this GPU: a 64-element kernel,
-    # and a device copy moving the dominant kernel's algorithmic bytes (half
-    # read, half written); 100 launches captured in a HIP graph and replayed,
-    # GPU time per launch
-    def per_launch_us(fn, n):
-        fn()
+    # ---------------- oracle check and CPU baseline ----------------
+    cpu, oracle_check = None, None
+    if not args.no_cpu:
+        log("CPU baseline: FP64 oracle, 2 steps of the 3-D box")
+        cpu, u_or = cpu_baseline(cfg, u0, dt, steps=2)
+        a = torch.from_numpy(u0).to(dev, torch.float32)
+        capi.fused_rk3_steps(cfg, dt, 2, a)
+        b = torch.from_numpy(u0).to(dev, torch.float64)
+        wb = torch.empty_like(b)
+        wsb = capi.rk3_workspace(cfg, capi.F64)
+        for _ in range(2):
+            capi.rk3_step(cfg, dt, b, wb, wsb)
         torch.cuda.synchronize()
-        g = torch.cuda.CUDAGraph()
-        side = torch.cuda.Stream()
-        side.wait_stream(torch.cuda.current_stream())
-        with torch.cuda.stream(side):
-            with torch.cuda.graph(g, stream=side):
-                for _ in range(100):
-                    fn()
-        torch.cuda.synchronize()
-        g.replay()
-        torch.cuda.synchronize()
-        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        a.record()
-        for _ in range(n // 100):
-            g.replay()
-        b.record()
-        torch.cuda.synchronize()
-        return a.elapsed_time(b) / (n // 100 * 100) * 1e3
-    tiny = torch.zeros(64, device=dev)
-    csrc = torch.ones(bytes_ // 8, dtype=torch.float32, device=dev)
-    cdst = torch.empty_like(csrc)
-    t_tiny = per_launch_us(lambda: tiny.add_(1.0), 2000)
-    t_copy = per_launch_us(lambda: cdst.copy_(csrc), 2000)
-    roof["floor"] = {"tiny_kernel_us": round(t_tiny, 3), "copy_same_bytes_us": round(t_copy, 3),
-                     "copy_same_bytes_GBs": round(bytes_ / (t_copy * 1e-6) / 1e9, 1),
-                     "kernel_over_copy_same_bytes": round(avg_ms * 1e3 / t_copy, 2)}
-    del tiny, csrc, cdst
+        oracle_check = {"steps": 2,
+                        "fp64_plain_max_rel": max_rel(b.cpu().numpy(), u_or, I),
+                        "fp32_fused_max_rel": max_rel(a.double().cpu().numpy(), u_or, I)}
+        del a, b, wb, wsb, u_or
 
-    extra = {"fused_rhs_avg_ms": round(rhs_avg, 5), "fused_column_avg_ms": round(col_avg, 5),
-             "timed_launches": int(tim.rhs_launches + tim.column_launches + tim.bc_launches)}
+    # ---------------- adjoint / forward on the headline block ----------------
+    adj = None
+    if not args.no_adjoint:
+        log("adjoint / forward, FP64, 3-D box 256x256x128")
+        adj = adjoint_ratio(capi, cfg, u0, dt, dev)
+    del u0
 
-    # ---- 3-D box (256 x 256 x 128, configuration-3 shape) secondary line ----
-    if not args.no_3d:
-        c3 = box3d_cfg(capi)
-        shp = c3.shape
-        z = (torch.arange(shp[3], dtype=torch.float64) - 3 + 0.5) * 100.0
-        T = 250.0 - 9.81 * z / 1004.5
-        p = 1e5 * (T / 250.0) ** (1004.5 / 287.0)
-        w3 = torch.zeros(shp, dtype=torch.float64)
-        w3[4] = p
-        g = torch.Generator().manual_seed(1234)
-        th = (torch.rand(shp[1:], generator=g, dtype=torch.float64) - 0.5)
-        th[..., 23:] = 0.0
-        w3[0] = p / (287.0 * (T + th))
-        wd = w3.to(dev)
-        ud = torch.empty_like(wd)
-        capi.prim2cons(c3, wd, ud)
-        capi.apply_bc(c3, ud)
-        u3 = ud.float()
-        ws3 = capi.fused_workspace(c3)
-        dt3 = 0.2  # horizontal acoustic CFL 0.7 on 100 m cells
-        capi.fused_rk3_steps(c3, dt3, 2, u3.clone(), ws3)
-        torch.cuda.synchronize()
-        k3 = 100  # the configuration's 100 steps
-        t0 = time.perf_counter()
-        capi.fused_rk3_steps(c3, dt3, k3, u3, ws3)
-        torch.cuda.synchronize()
-        t3 = time.perf_counter() - t0
-        tim3 = capi.Timing()
-        capi.fused_rk3_steps(c3, dt3, 3, u3, ws3, timing=tim3)
-        cus3 = c3.n_interior * k3 / t3
-        r3 = tim3.rhs_ms / max(tim3.rhs_launches, 1)
-        c3ms = tim3.column_ms / max(tim3.column_launches, 1)
-        extra["box3d_256x256x128_fp32"] = {
-            "cell_updates_per_s": round(cus3, 1), "steps": k3, "dt_s": dt3,
-            "finite": bool(torch.isfinite(u3).all()),
-            "ms_per_step": round(t3 / k3 * 1e3, 4),
-            "step_hbm_frac": round(cus3 * 160 / (HBM_PEAK_GBS * 1e9), 4),
-            "fused_rhs_avg_ms": round(r3, 4), "fused_column_avg_ms": round(c3ms, 4),
-            "rhs_hbm_frac": round(2 * NV * 4 * c3.n_interior / (r3 * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                                  4),
+    # ---------------- Straka 512 x 64 ----------------
+    if not args.no_straka:
+        log("Straka 512x64: 1000 fused FP32 and 1000 plain FP64 steps")
+        cs = straka_cfg(capi)
+        us0 = conserved_on_device(capi, cs, straka_prim(cs), dev).cpu().numpy()
+        Is = (slice(None), slice(3, -3), slice(3, -3))
+        rs = fused_vs_plain(capi, cs, us0, 0.1, 1000, 10, dev, timing_steps=100)
+        g32, g64 = rs["u32"].double().cpu().numpy(), rs["u64"].cpu().numpy()
+        ms0 = us0[0][Is].sum()
+        n2 = cs.n_interior
+        ks1 = ("fused_rhs_kernel", rs["rhs_ms"], 2 * NV * 4 * n2)
+        ks2 = ("fused_column_pcr_kernel<3>", rs["col_ms"], 4 * NV * 4 * n2)
+        doms = ks1 if ks1[1] >= ks2[1] else ks2
+        extra["straka_2d_512x64_fp32"] = {
+            "cell_updates_per_s": round(n2 * 1000 / rs["t32"], 1), "steps": 1000, "dt_s": 0.1,
+            "ms_per_step": round(rs["t32"] / 1000 * 1e3, 5),
+            "fp64_plain_cell_updates_per_s": round(n2 * 1000 / rs["t64"], 1),
+            "fused_over_fp64_plain": round(rs["t64"] / rs["t32"], 3),
+            "fused_rhs_avg_ms": round(rs["rhs_ms"], 5),
+            "fused_column_avg_ms": round(rs["col_ms"], 5),
+            "roofline": kernel_roofline(*doms, "straka512x64"),
+            "error_fp32_vs_fp64_after_steps": errors(g32, g64, Is),
+            "mass_drift_fp32": float(abs(g32[0][Is].sum() - ms0) / ms0),
+            "mass_drift_fp64": float(abs(g64[0][Is].sum() - ms0) / ms0),
+            "reference_convention_value": round(
+                cs.shape[1] * cs.shape[2] * cs.shape[3] * 1000 * 3 / rs["t32"], 1),
         }
-        for key, kname, ms in (("rhs", "fused_rhs_march_kernel", r3),
-                               ("column", "fused_column_pcr2_kernel", c3ms)):
-            valu = measured_traffic(kname, "box256x256x128", "valu")
-            if valu:
-                extra["box3d_256x256x128_fp32"][f"{key}_valu_issue_frac"] = round(
-                    valu / (ms * 1e-3) / VALU_ISSUE_PEAK, 4)
-            fl = measured_flops(kname, "box256x256x128")
-            if fl:
-                extra["box3d_256x256x128_fp32"][f"{key}_valu_tflops"] = round(
-                    (fl[0] + fl[1]) / (ms * 1e-3) / 1e12, 3)
-                extra["box3d_256x256x128_fp32"][f"{key}_valu_flop_frac"] = round(
-                    (fl[0] + fl[1]) / (ms * 1e-3) / VALU_FLOP_PEAK, 4)
+        if not args.no_adjoint:
+            extra["straka_2d_512x64_fp32"]["adjoint_fp64"] = adjoint_ratio(capi, cs, us0, 0.1,
+                                                                          dev, n=20)
+        del rs, g32, g64
 
     # ---- spherical-polar shell sector (64 radial x 512 polar cells):
     #      FP32 fused against FP64 plain on the same 100 steps ----
     if not args.no_sph:
+        log("spherical-polar sector 64x512: 100 fused FP32 and 100 plain FP64 steps")
         cs = sector_cfg(capi)
-        us64 = conserved_on_device(capi, cs, sector_prim(cs), dev)
-        us32 = us64.float()
-        s0 = us64.clone()
-        dts, ks = 0.002, 100  # x2 acoustic CFL ~0.8 at r = 1
-        wss = capi.fused_workspace(cs)
-        capi.fused_rk3_steps(cs, dts, 2, us32.clone(), wss)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        capi.fused_rk3_steps(cs, dts, ks, us32, wss)
-        torch.cuda.synchronize()
-        ts32 = time.perf_counter() - t0
-        tims = capi.Timing()
-        capi.fused_rk3_steps(cs, dts, 3, us32.clone(), wss, timing=tims)
-        ws64s = capi.rk3_workspace(cs, capi.F64)
-        wt = torch.empty_like(us64)
-        capi.rk3_step(cs, dts, us64.clone(), wt, ws64s)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(ks):
-            capi.rk3_step(cs, dts, us64, wt, ws64s)
-        torch.cuda.synchronize()
-        ts64 = time.perf_counter() - t0
+        us0 = conserved_on_device(capi, cs, sector_prim(cs), dev).cpu().numpy()
+        rs = fused_vs_plain(capi, cs, us0, 0.002, 100, 2, dev)
         Is = (slice(None), slice(3, -3), slice(3, -3))
-        es = errors(us32.double().cpu().numpy(), us64.cpu().numpy(), Is, vel_scale=1.0)
         extra["spherical_sector_64x512_fp32"] = {
-            "cell_updates_per_s": round(cs.n_interior * ks / ts32, 1), "steps": ks, "dt": dts,
-            "ms_per_step": round(ts32 / ks * 1e3, 4),
-            "fp64_plain_cell_updates_per_s": round(cs.n_interior * ks / ts64, 1),
-            "fused_over_fp64_plain": round(ts64 / ts32, 3),
-            "fused_rhs_avg_ms": round(tims.rhs_ms / max(tims.rhs_launches, 1), 5),
-            "fused_column_avg_ms": round(tims.column_ms / max(tims.column_launches, 1), 5),
-            "error_fp32_vs_fp64_after_steps": es,
-            "finite": bool(torch.isfinite(us32).all()),
+            "cell_updates_per_s": round(cs.n_interior * 100 / rs["t32"], 1), "steps": 100,
+            "dt": 0.002, "ms_per_step": round(rs["t32"] / 100 * 1e3, 4),
+            "fp64_plain_cell_updates_per_s": round(cs.n_interior * 100 / rs["t64"], 1),
+            "fused_over_fp64_plain": round(rs["t64"] / rs["t32"], 3),
+            "fused_rhs_avg_ms": round(rs["rhs_ms"], 5),
+            "fused_column_avg_ms": round(rs["col_ms"], 5),
+            "error_fp32_vs_fp64_after_steps": errors(rs["u32"].double().cpu().numpy(),
+                                                     rs["u64"].cpu().numpy(), Is, vel_scale=1.0),
+            "finite": bool(torch.isfinite(rs["u32"]).all()),
         }
-        del wss, ws64s, us32, us64, s0, wt
+        del rs
 
     # ---- batched ensemble (96 members of uneven size), autotuned tiles,
-    #      HIP-graph replays ----
+    #      one HIP-graph launch of the configuration's 500 steps ----
     if not args.no_ensemble:
+        log("ensemble: 96 members, autotune, 500 steps in one graph launch")
         cfgs_e, ws_e = ensemble_members(capi)
         parts = []
         for c, w in zip(cfgs_e, ws_e):
-            wd = torch.from_numpy(w).to(dev)
-            ud = torch.empty_like(wd)
-            capi.prim2cons(c, wd, ud)
-            capi.apply_bc(c, ud)
-            parts.append(ud.float().reshape(-1))
+            parts.append(conserved_on_device(capi, c, w, dev).float().reshape(-1))
         ue = torch.cat(parts)
         del parts
         dte = 0.1
         b = capi.Batch(cfgs_e, dte)
         tile, rhs_ms = b.autotune(ue, dte, trials=3)
-        # the configuration's 500 steps as one graph launch (a warm-up launch
-        # first, on a copy of the state)
-        steps_per_graph, ngraph = 500, 1
+        steps_e = 500
         u_keep = ue.clone()
-        g = b.graph(ue, steps_per_graph)
+        g = b.graph(ue, steps_e)
         g.launch()
         torch.cuda.synchronize()
         ue.copy_(u_keep)
         del u_keep
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for _ in range(ngraph):
-            g.launch()
+        g.launch()
         torch.cuda.synchronize()
         te = time.perf_counter() - t0
         ncells_e = sum(c.n_interior for c in cfgs_e)
         extra["ensemble_96_fp32_graph"] = {
-            "members": len(cfgs_e), "cells": int(ncells_e), "steps": steps_per_graph * ngraph,
-            "cell_updates_per_s": round(ncells_e * steps_per_graph * ngraph / te, 1),
-            "ms_per_step": round(te / (steps_per_graph * ngraph) * 1e3, 4),
+            "members": len(cfgs_e), "cells": int(ncells_e), "steps": steps_e,
+            "cell_updates_per_s": round(ncells_e * steps_e / te, 1),
+            "ms_per_step": round(te / steps_e * 1e3, 4),
             "autotuned_tile": list(tile), "rhs_ms_at_tile": round(rhs_ms, 4),
             "finite": bool(torch.isfinite(ue).all()),
         }
         del g, b, ue
 
-    # ---- adjoint at scale: 3-D box 128 x 128 x 96 of 200 m, FP64 ----
-    if not args.no_3d:
-        ca = box3d_cfg(capi, n=(96, 128, 128))
-        ca.x1max, ca.x2max, ca.x3max = 200.0 * 96, 200.0 * 128, 200.0 * 128
-        shp = ca.shape
-        z = (torch.arange(shp[3], dtype=torch.float64) - 3 + 0.5) * 200.0
-        T = 250.0 - 9.81 * z / 1004.5
-        p = 1e5 * (T / 250.0) ** (1004.5 / 287.0)
-        wa = torch.zeros(shp, dtype=torch.float64)
-        wa[4] = p
-        g2 = torch.Generator().manual_seed(99)
-        th = (torch.rand(shp[1:], generator=g2, dtype=torch.float64) - 0.5)
-        th[..., 23:] = 0.0
-        wa[0] = p / (287.0 * (T + th))
-        wad = wa.to(dev)
-        ua = torch.empty_like(wad)
-        capi.prim2cons(ca, wad, ua)
-        capi.apply_bc(ca, ua)
-        dta = 0.4
-        wsf = capi.rk3_workspace(ca, capi.F64)
-        wwa = torch.empty_like(ua)
-        capi.rk3_step(ca, dta, ua.clone(), wwa, wsf)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        uf = ua.clone()
-        for _ in range(3):
-            capi.rk3_step(ca, dta, uf, wwa, wsf)
-        torch.cuda.synchronize()
-        tf = (time.perf_counter() - t0) / 3
-        del wsf
-        wsa = capi.adjoint_workspace(ca)
-        lam_a = torch.randn(ua.shape, dtype=torch.float64, generator=g2).to(dev)
-        capi.adjoint_rk3_step(ca, dta, ua, lam_a.clone(), wsa)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(3):
-            capi.adjoint_rk3_step(ca, dta, ua, lam_a, wsa)
-        torch.cuda.synchronize()
-        tadj = (time.perf_counter() - t0) / 3
-        if not args.no_c5:
-            extra["c5_sensitivity_box3d_128x128x96_fp64"] = c5_sensitivity(
-                capi, dev, ndir=args.c5_directions)
-            # the same gradient check off the kinks of the state at rest: a
-            # uniform horizontal wind (Roe |u| at the x2 / x3 faces) and a
-            # convective roll (LMARS upwind switch at the x1 faces)
-            extra["c5_fd_check_wind_and_roll"] = c5_sensitivity(
-                capi, dev, ndir=8, wind=(2.0, 1.5), roll=1.0)
-        extra["adjoint_box3d_128x128x96_fp64"] = {
-            "forward_ms_per_step": round(tf * 1e3, 3), "adjoint_ms_per_step": round(tadj * 1e3, 3),
-            "adjoint_over_forward": round(tadj / tf, 3),
-            "finite": bool(torch.isfinite(lam_a).all()),
-        }
-        del wsa, lam_a
-
-    cpu = None if args.no_cpu else cpu_baseline()
+    # ---- adjoint sensitivity: 3-D box 128 x 128 x 96 of 200 m, FP64 ----
+    if not args.no_adjoint and not args.no_c5:
+        log("adjoint sensitivity, 128x128x96, 200 steps (FP64)")
+        extra["c5_sensitivity_box3d_128x128x96_fp64"] = c5_sensitivity(
+            capi, dev, ndir=args.c5_directions)
+        # the same gradient check off the kinks of the state at rest: a
+        # uniform horizontal wind (Roe |u| at the x2 / x3 faces) and a
+        # convective roll (LMARS upwind switch at the x1 faces)
+        extra["c5_fd_check_wind_and_roll"] = c5_sensitivity(
+            capi, dev, ndir=8, wind=(2.0, 1.5), roll=1.0)
 
     line = {
         "metric": "cell-updates/s per RK3 step, FP32 fused vs FP64 plain at stated err; "
@@ -724,26 +733,31 @@ def main():
         "n_gpus": 1,
         "steps": K,
         "warmup": W,
-        "ms_per_step": round(t32 / K * 1e3, 5),
+        "ms_per_step": round(r["t32"] / K * 1e3, 5),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 (FP64 implicit column solve)",
-        "data": "synthetic (Straka initial state of examples/straka.cpp)",
-        "config": {"workload": "straka_2d_512x64_weno5_lmars_vic3_dt0.1",
-                   "cells": [512, 64], "dt_s": dt, "steps": K},
+        "data": "synthetic (capped dry-adiabat background with seeded theta noise)",
+        "config": {"workload": "box3d_256x256x128_100m_weno5_lmars_x1_roe_x2x3_vic3_dt0.2",
+                   "cells": [256, 256, 128], "dt_s": dt, "steps": K},
         # the reference's own counter (src/mesh/meshblock.cpp:938-956): cells
         # including ghosts x steps x stages / seconds
         "reference_convention_value": round(
-            cfg.shape[1] * cfg.shape[2] * cfg.shape[3] * K * 3 / t32, 1),
-        "fp64_plain": {"value": round(cus64, 1), "ms_per_step": round(t64 / K * 1e3, 5)},
+            cfg.shape[1] * cfg.shape[2] * cfg.shape[3] * K * 3 / r["t32"], 1),
+        "fp64_plain": {"value": round(cus64, 1), "ms_per_step": round(r["t64"] / K * 1e3, 5)},
         "fp32_fused_over_fp64_plain": round(cus32 / cus64, 3),
         "error_fp32_vs_fp64_after_steps": err,
         "mass_drift_fp32": mass_drift,
         "mass_drift_fp64": mass_drift64,
-        "adjoint_over_forward": round(adj_ratio, 3),
-        "adjoint_note": "FP64 adjoint step (recomputes its 3 stages) / FP64 plain forward step",
-        "adjoint_ms_per_step": round(t_adj * 1e3, 4),
+        "finite": finite,
+        "oracle_check": oracle_check,
+        "adjoint_over_forward": adj["adjoint_over_forward"] if adj else None,
+        "adjoint_note": "FP64 adjoint step (recomputes its 3 stages) / FP64 plain forward step "
+                        "on the headline block",
+        "adjoint": adj,
+        "fused_rhs_avg_ms": round(r["rhs_ms"], 5),
+        "fused_column_avg_ms": round(r["col_ms"], 5),
         "step_hbm_frac": round(cus32 * 160 / (HBM_PEAK_GBS * 1e9), 6),
         "roofline": roof,
         "cpu_baseline": cpu,
