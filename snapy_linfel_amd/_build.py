@@ -34,8 +34,11 @@ HIP_UNITS = {
     "plain.hip": ["-ffp-contract=off"],
     "vic.hip": ["-ffp-contract=off"],
     # FP32 fused path: hardware reciprocal / square root (1-2 ulp) instead of
-    # the correctly rounded division and sqrt sequences
-    "fused.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt"],
+    # the correctly rounded division and sqrt sequences, and FP32 denormals
+    # flushed to zero, so that a division is one reciprocal and one multiply
+    # (without the exponent scaling that keeps denormal results); FP64 keeps
+    # its denormals
+    "fused.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt", "-fgpu-flush-denormals-to-zero"],
     "adjoint.hip": ["-ffp-contract=off"],
 }
 

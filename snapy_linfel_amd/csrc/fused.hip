@@ -595,6 +595,39 @@ __global__ void __launch_bounds__(kRhsThreads)
 // (the compiler contracts a few multiply-adds differently).
 // (four wavefronts per SIMD: 128 registers, a few of the slice window spilled;
 // 0.81 against 0.85 ms per launch at three wavefronts and no spills)
+//
+// Face options fixed at compile time (-1: read from the arguments at run
+// time): the density interpolant and the interpolant of velocities and
+// pressure for x1 (kTy0, kTy20) and for x2 / x3 (kTy1, kTy21), and the
+// Riemann solver per direction.  A fixed instance has the scale option off.
+// The instances of the configurations in use compile one interpolant pair and
+// one solver per direction, without the run-time dispatch.
+template <int kTy0, int kTy20, int kTy1, int kTy21, int kR1, int kR2, int kR3>
+struct MarchFaces {
+  static constexpr int ty0 = kTy0, ty20 = kTy20, ty1 = kTy1, ty21 = kTy21;
+  static constexpr int r1 = kR1, r2 = kR2, r3 = kR3;
+};
+using MarchGeneric = MarchFaces<-1, -1, -1, -1, -1, -1, -1>;
+
+// Interface states of one face in the face frame of direction kD (normal
+// velocity in slot IVX) from the pencil of each variable, p_v = ws + vmap[v]
+// * HS + base with stride sd; c is the pencil index of the cell above.
+template <int kD, int kTy, int kTy2>
+__device__ __forceinline__ void face_states(const float* ws, int HS, int base, int sd, int c,
+                                            int type, int type2, int scale, float* wl,
+                                            float* wr) {
+  constexpr int vmap[5] = {IDN, IVX + kD, IVX + (kD + 1) % 3, IVX + (kD + 2) % 3, IPR};
+  const int t = kTy >= 0 ? kTy : type, t2 = kTy2 >= 0 ? kTy2 : type2;
+  const int sc = kTy >= 0 ? 0 : scale;
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    const float* p = ws + vmap[v] * HS + base;
+    wr[v] = interp_face<float>(v == IDN ? t : t2, 0, sc, p, sd, c);
+    wl[v] = interp_face<float>(v == IDN ? t : t2, 1, sc, p, sd, c - 1);
+  }
+}
+
+template <class P>
 __global__ void __launch_bounds__(kRhsThreads, 4)
     fused_rhs_march_kernel(const float* __restrict__ u, float* __restrict__ du, FusedArgs a) {
   extern __shared__ float smem[];
@@ -619,6 +652,9 @@ __global__ void __launch_bounds__(kRhsThreads, 4)
   const int type1 = a.recon_type[1], scale1 = a.recon_scale[1];
   const int ty20 = a.recon_shock[0] ? type0 : interp2_type(type0);
   const int ty21 = a.recon_shock[1] ? type1 : interp2_type(type1);
+  const int sol1 = P::r1 >= 0 ? P::r1 : a.riemann[0];
+  const int sol2 = P::r2 >= 0 ? P::r2 : a.riemann[1];
+  const int sol3 = P::r3 >= 0 ? P::r3 : a.riemann[2];
   const int nf0 = (e1 + 1) * e2, ftot = nf0 + e1 * (e2 + 1);
   float* fs = smem + 10 * HS;  // x1 then x2 face fluxes of one slice [5][ftot]
   const float area1 = a.dx3 * a.dx2, area2 = a.dx3 * a.dx1, area3 = a.dx2 * a.dx1;
@@ -682,15 +718,17 @@ __global__ void __launch_bounds__(kRhsThreads, 4)
     if (own && f >= o3 && f <= o3 + ez) {
       // face frame of x3: slots (IDN, IVZ, IVX, IVY, IPR)
       const int vmap[5] = {IDN, IVZ, IVX, IVY, IPR};
+      const int t = P::ty1 >= 0 ? P::ty1 : type1, t2 = P::ty21 >= 0 ? P::ty21 : ty21;
+      const int sc = P::ty1 >= 0 ? 0 : scale1;
       float wl[5], wr[5];
 #pragma unroll
       for (int v = 0; v < 5; ++v) {
-        const int tv = (v == IDN) ? type1 : ty21;
-        wr[v] = interp_face<float>(tv, 0, scale1, &win[vmap[v]][0], 1, 3);
-        wl[v] = interp_face<float>(tv, 1, scale1, &win[vmap[v]][0], 1, 2);
+        const int tv = (v == IDN) ? t : t2;
+        wr[v] = interp_face<float>(tv, 0, sc, &win[vmap[v]][0], 1, 3);
+        wl[v] = interp_face<float>(tv, 1, sc, &win[vmap[v]][0], 1, 2);
       }
       float flx[5];
-      riemann_flux<float>(a.riemann[2], wl, wr, gammad, 3, flx, (float*)nullptr);
+      riemann_flux<float>(sol3, wl, wr, gammad, 3, flx, (float*)nullptr);
 #pragma unroll
       for (int v = 0; v < 5; ++v) f3hi[vmap[v]] = flx[v];
     }
@@ -699,41 +737,44 @@ __global__ void __launch_bounds__(kRhsThreads, 4)
     float pn[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
     if (kk >= o3 && kk < o3 + ez) {
       for (int g = threadIdx.x; g < ftot; g += blockDim.x) {
-        const int d = g < nf0 ? 0 : 1;
-        int fx;
-        const int fy = d == 0 ? dq1.div(g, fx) : de1.div(g - nf0, fx);
-        const int x = fx + 3, y = fy + 3;
-        const int c = d == 0 ? x : y;
-        const int sd = d == 0 ? 1 : H1;
-        const int base = y * H1 + x - c * sd;  // pencil start along d
-        const int type = d == 0 ? type0 : type1, scale = d == 0 ? scale0 : scale1;
-        const int type2 = d == 0 ? ty20 : ty21;
-        const int vmap[5] = {IDN, IVX + d, IVX + (d + 1) % 3, IVX + (d + 2) % 3, IPR};
-        float wl[5], wr[5];
-#pragma unroll
-        for (int v = 0; v < 5; ++v) {
-          const int tv = (v == IDN) ? type : type2;
-          const float* p = ws + vmap[v] * HS + base;
-          wr[v] = interp_face<float>(tv, 0, scale, p, sd, c);
-          wl[v] = interp_face<float>(tv, 1, scale, p, sd, c - 1);
-        }
-        if (d == 0 && refill) {
-          // _revise_x1inner_lr / _revise_x1outer_lr (hydro.cpp:217-229)
-          const int gi = o1 + fx;
-          if (gi == 0) {
-            wl[IDN] = wr[IDN];
-            wl[IPR] = wr[IPR];
+        float wl[5], wr[5], flx[5];
+        if (g < nf0) {
+          // x1 face below cell (fx, fy) of the slice
+          int fx;
+          const int fy = dq1.div(g, fx);
+          face_states<0, P::ty0, P::ty20>(ws, HS, (fy + 3) * H1, 1, fx + 3, type0, ty20, scale0,
+                                         wl, wr);
+          if (refill) {
+            // _revise_x1inner_lr / _revise_x1outer_lr (hydro.cpp:217-229)
+            const int gi = o1 + fx;
+            if (gi == 0) {
+              wl[IDN] = wr[IDN];
+              wl[IPR] = wr[IPR];
+            }
+            if (gi == a.nx1) {
+              wr[IDN] = wl[IDN];
+              wr[IPR] = wl[IPR];
+            }
           }
-          if (gi == a.nx1) {
-            wr[IDN] = wl[IDN];
-            wr[IPR] = wl[IPR];
-          }
+          riemann_flux<float>(sol1, wl, wr, gammad, 3, flx, (float*)nullptr);
+          fs[IDN * ftot + g] = flx[0];
+          fs[IVX * ftot + g] = flx[1];
+          fs[IVY * ftot + g] = flx[2];
+          fs[IVZ * ftot + g] = flx[3];
+          fs[IPR * ftot + g] = flx[4];
+        } else {
+          // x2 face below cell (fx, fy) of the slice
+          int fx;
+          const int fy = de1.div(g - nf0, fx);
+          face_states<1, P::ty1, P::ty21>(ws, HS, fx + 3, H1, fy + 3, type1, ty21, scale1, wl,
+                                         wr);
+          riemann_flux<float>(sol2, wl, wr, gammad, 3, flx, (float*)nullptr);
+          fs[IDN * ftot + g] = flx[0];
+          fs[IVY * ftot + g] = flx[1];
+          fs[IVZ * ftot + g] = flx[2];
+          fs[IVX * ftot + g] = flx[3];
+          fs[IPR * ftot + g] = flx[4];
         }
-        float flx[5];
-        riemann_flux<float>(d == 0 ? a.riemann[0] : a.riemann[1], wl, wr, gammad, 3, flx,
-                            (float*)nullptr);
-#pragma unroll
-        for (int v = 0; v < 5; ++v) fs[vmap[v] * ftot + g] = flx[v];
       }
       __syncthreads();
       if (own) {
@@ -1850,6 +1891,31 @@ static int make_march_args(const snapx_config* cfg, double dt, FusedArgs* out) {
   return 0;
 }
 
+using MarchKernel = void (*)(const float*, float*, FusedArgs);
+
+// Compiled instances of the marching K1 (WENO5 density with CP5 velocities
+// and pressure, shock off, scale off, on both axis groups): LMARS in x1 with
+// Roe in x2 / x3, and one solver in every direction.  Other options run the
+// generic instance.
+static MarchKernel march_kernel_for(const snapx_config* c) {
+  constexpr int W5 = SNAPX_INTERP_WENO5, C5 = SNAPX_INTERP_CP5;
+  constexpr int RO = SNAPX_RIEMANN_ROE, LM = SNAPX_RIEMANN_LMARS, HL = SNAPX_RIEMANN_HLLC;
+  const bool weno5_cp5 = c->recon_type[0] == W5 && c->recon_type[1] == W5 && !c->recon_shock[0] &&
+                         !c->recon_shock[1] && !c->recon_scale[0] && !c->recon_scale[1];
+  if (weno5_cp5) {
+    const int r1 = c->riemann[0], r2 = c->riemann[1], r3 = c->riemann[2];
+    if (r1 == LM && r2 == RO && r3 == RO)
+      return fused_rhs_march_kernel<MarchFaces<W5, C5, W5, C5, LM, RO, RO>>;
+    if (r1 == LM && r2 == LM && r3 == LM)
+      return fused_rhs_march_kernel<MarchFaces<W5, C5, W5, C5, LM, LM, LM>>;
+    if (r1 == RO && r2 == RO && r3 == RO)
+      return fused_rhs_march_kernel<MarchFaces<W5, C5, W5, C5, RO, RO, RO>>;
+    if (r1 == HL && r2 == HL && r3 == HL)
+      return fused_rhs_march_kernel<MarchFaces<W5, C5, W5, C5, HL, HL, HL>>;
+  }
+  return fused_rhs_march_kernel<MarchGeneric>;
+}
+
 static int make_col_args(const snapx_config* cfg, double dt, ColArgs* out) {
   Grid g = make_grid(cfg);
   ColArgs c;
@@ -1956,7 +2022,7 @@ static int fused_step(const snapx_config* cfg, double dt, float* u, void* worksp
       hipExtLaunchKernelGGL(fused_rhs_sph_kernel, dim3(nblk), dim3(kRhsThreads), (uint32_t)smem,
                             st, ra, rb, 0u, (const float*)u, du, a);
     else if (march)
-      hipExtLaunchKernelGGL(fused_rhs_march_kernel, dim3(nblk), dim3(kRhsThreads),
+      hipExtLaunchKernelGGL(march_kernel_for(cfg), dim3(nblk), dim3(kRhsThreads),
                             (uint32_t)smem, st, ra, rb, 0u, (const float*)u, du, a);
     else if (a.merged)
       hipExtLaunchKernelGGL(fused_rhs_kernel<1>, dim3(nblk), dim3(kRhsThreads), (uint32_t)smem, st,
