@@ -274,6 +274,44 @@ int snapx_fused_rk3_steps(const snapx_config* cfg, double dt, int nsteps,
                           float* u, void* workspace, size_t workspace_bytes,
                           snapx_timing* timing, void* stream);
 
+/* ---------------------------------------------------------------------- */
+/* FP32 perturbation path about an FP64 hydrostatic background (no         */
+/* reference counterpart: the reference steps the total state in one       */
+/* precision)                                                              */
+/* ---------------------------------------------------------------------- */
+
+/* The state is carried in FP32 as u' = u - ubar(x1) about a background at
+   rest that varies with x1 only, ubar = (rho_bar, 0, 0, 0, p_bar /
+   (gamma - 1)), given in FP64.  The fused kernels then difference pressure,
+   energy and density perturbations where the total state would cancel
+   (the hydrostatic pressure gradient against gravity, the jumps of the Roe
+   dissipation), with the background's own flux divergence and gravity as an
+   FP64 residual per level; in exact arithmetic the step is the fused step of
+   the total state.  Requires a Cartesian block, vertical reconstruction
+   weno5 with shock: false, scale off, grav2 = grav3 = 0, and columns of at
+   most 512 (3x3 blocks) / 128 (5x5) cells.
+
+   snapx_fused_background fills the device tables bg (bytes from
+   snapx_fused_background_bytes) from the host arrays rho_bar[nc1],
+   p_bar[nc1] (x1 cell levels, ghosts included; synchronises the stream).
+   snapx_fused_split writes u' = u - ubar (u float or double per dtype, every
+   cell); snapx_fused_join writes u = ubar + u' and applies the external
+   boundaries to u.  snapx_fused_rk3_steps_bg advances u' nsteps fused steps
+   (workspace as snapx_fused_workspace_bytes; the stored ghosts of u' are not
+   refreshed: join restores them on the total state). */
+size_t snapx_fused_background_bytes(const snapx_config* cfg);
+int snapx_fused_background(const snapx_config* cfg, const double* rho_bar,
+                           const double* p_bar, void* bg, size_t bg_bytes,
+                           void* stream);
+int snapx_fused_split(const snapx_config* cfg, const void* bg, const void* u,
+                      int dtype, float* u_pert, void* stream);
+int snapx_fused_join(const snapx_config* cfg, const void* bg,
+                     const float* u_pert, void* u, int dtype, void* stream);
+int snapx_fused_rk3_steps_bg(const snapx_config* cfg, const void* bg,
+                             double dt, int nsteps, float* u_pert,
+                             void* workspace, size_t workspace_bytes,
+                             snapx_timing* timing, void* stream);
+
 /* No reference counterpart.  For callers that cannot allocate device memory
    (code under HIP graph capture, tight serving loops, harnesses that forbid
    hipMalloc): snapx_fused_rk3_steps (timing NULL) with its workspace in a

@@ -129,6 +129,11 @@ SIGNATURES = {
     "snapx_fused_rk3_step": (_I, [_CFG, _D, _P, _P, _Z, _P]),
     "snapx_fused_rk3_steps": (_I, [_CFG, _D, _I, _P, _P, _Z, _P, _P]),
     "snapx_fused_rk3_steps_static": (_I, [_CFG, _D, _I, _P, _P]),
+    "snapx_fused_background_bytes": (_Z, [_CFG]),
+    "snapx_fused_background": (_I, [_CFG, _P, _P, _P, _Z, _P]),
+    "snapx_fused_split": (_I, [_CFG, _P, _P, _I, _P, _P]),
+    "snapx_fused_join": (_I, [_CFG, _P, _P, _P, _I, _P]),
+    "snapx_fused_rk3_steps_bg": (_I, [_CFG, _P, _D, _I, _P, _P, _Z, _P, _P]),
     "snapx_adjoint_workspace_bytes": (_Z, [_CFG]),
     "snapx_adjoint_rk3_step": (_I, [_CFG, _D, _P, _P, _P, _Z, _P]),
     "snapx_batch_workspace_bytes": (_Z, [_CFG, _I]),
@@ -328,6 +333,72 @@ def fused_rk3_steps(cfg, dt, nsteps, u, ws=None, timing=None):
     tp = C.byref(timing) if timing is not None else None
     check(lib().snapx_fused_rk3_steps(C.byref(cfg), dt, nsteps, ptr(u), ptr(ws), ws.numel(), tp,
                                       stream_ptr()), "fused_rk3_steps")
+
+
+class Background:
+    """FP64 hydrostatic background of the FP32 perturbation path: rho_bar and
+    p_bar per x1 cell level (ghosts included, length nc1), horizontally
+    uniform and at rest.  ``split`` / ``join`` convert a total state to the
+    FP32 perturbation and back; ``steps`` advances the perturbation."""
+
+    def __init__(self, cfg, rho_bar, p_bar, device="cuda"):
+        import numpy as np
+        n1 = cfg.shape[3]
+        rb = np.ascontiguousarray(rho_bar, dtype=np.float64)
+        pb = np.ascontiguousarray(p_bar, dtype=np.float64)
+        if rb.shape != (n1,) or pb.shape != (n1,):
+            raise SnapxError(f"background: rho_bar and p_bar need {n1} levels")
+        nb = lib().snapx_fused_background_bytes(C.byref(cfg))
+        if nb == 0:
+            raise SnapxError("background: " + lib().snapx_last_error().decode())
+        self.cfg = cfg
+        self.buf = workspace(nb, device)
+        check(lib().snapx_fused_background(C.byref(cfg), rb.ctypes.data_as(C.c_void_p),
+                                           pb.ctypes.data_as(C.c_void_p), ptr(self.buf), nb,
+                                           stream_ptr()), "fused_background")
+
+    def split(self, u, out=None):
+        """FP32 perturbation u - ubar of the total state u (float or double)."""
+        torch = _torch()
+        _check_state(self.cfg, u, (torch.float32, torch.float64))
+        out = torch.empty(u.shape, dtype=torch.float32, device=u.device) if out is None else out
+        check(lib().snapx_fused_split(C.byref(self.cfg), ptr(self.buf), ptr(u), dtype_code(u),
+                                      ptr(out), stream_ptr()), "fused_split")
+        return out
+
+    def join(self, up, out):
+        """out = ubar + up with the external boundaries applied (out float or
+        double)."""
+        torch = _torch()
+        _check_state(self.cfg, up, (torch.float32,))
+        _check_state(self.cfg, out, (torch.float32, torch.float64))
+        check(lib().snapx_fused_join(C.byref(self.cfg), ptr(self.buf), ptr(up), ptr(out),
+                                     dtype_code(out), stream_ptr()), "fused_join")
+        return out
+
+    def steps(self, dt, nsteps, up, ws=None, timing=None):
+        """nsteps fused steps of the FP32 perturbation up."""
+        torch = _torch()
+        _check_state(self.cfg, up, (torch.float32,))
+        ws = fused_workspace(self.cfg, up.device) if ws is None else ws
+        tp = C.byref(timing) if timing is not None else None
+        check(lib().snapx_fused_rk3_steps_bg(C.byref(self.cfg), ptr(self.buf), dt, nsteps,
+                                             ptr(up), ptr(ws), ws.numel(), tp, stream_ptr()),
+              "fused_rk3_steps_bg")
+
+
+def _check_state(cfg, u, dtypes):
+    """Rejects a tensor the kernels would misread: wrong dtype, element count
+    or device."""
+    n = 1
+    for d in cfg.shape:
+        n *= d
+    if u.dtype not in dtypes:
+        raise SnapxError(f"state dtype {u.dtype}, expected one of {dtypes}")
+    if u.numel() != n:
+        raise SnapxError(f"state has {u.numel()} elements, the block needs {n}")
+    if not u.is_cuda or not u.is_contiguous():
+        raise SnapxError("state must be a contiguous device tensor")
 
 
 STATIC_SCRATCH_BYTES = 64 << 20

@@ -39,6 +39,8 @@
 #include "riemann_impl.hpp"
 #include "vic_impl.hpp"
 #include "pcr.hpp"
+#include "bg_impl.hpp"
+#include "riemann_pert.hpp"
 
 namespace snapx {
 
@@ -102,6 +104,11 @@ struct FusedArgs {
   // (dx1v[is - 1], + dx1v[is - 2], ...; dx1v[ie], + dx1v[ie + 1], ...)
   float zin[3], zout[3];
   Geometry geo;
+  // hydrostatic background of the perturbation path (null: total state);
+  // tables of bg_impl.hpp
+  const float* bgt;
+  const double* bgd;
+  double bg_gdz;  // -grav1 dx1 in FP64 for the FP64 ghost refill
 };
 
 // Maps an interior-relative index x (may lie in the ghost range) of an axis
@@ -165,6 +172,106 @@ __device__ __forceinline__ void cons_to_prim(const T* U, T gammad, T* W) {
   W[IVZ] = U[IVZ] * inv;
   T ke = T(0.5) * ((W[IVX] * U[IVX] + W[IVY] * U[IVY]) + W[IVZ] * U[IVZ]);
   W[IPR] = (gammad - T(1.)) * (U[IPR] - ke);
+}
+
+
+// ---- perturbation path (hydrostatic background, bg_impl.hpp) ----
+
+// Primitive perturbation (rho', v1, v2, v3, p') of a loaded cell U' (flips
+// applied) at the LDS position of interior-relative x1 index gi, and the
+// total density.  U' is relative to its source level; a ghost position
+// (mirror or edge source) adds the background difference of the source and
+// ghost levels.
+__device__ __forceinline__ void bg_load_prim(const FusedArgs& a, const BgLayout& B,
+                                             const float* U, int gi, float* W, float& rtot) {
+  const int lg = gi + a.ng;
+  const float* t = a.bgt;
+  W[IDN] = U[IDN] + t[B.gdr + lg];
+  rtot = t[B.rho_c + lg] + W[IDN];
+  const float inv = 1.f / rtot;
+  W[IVX] = U[IVX] * inv;
+  W[IVY] = U[IVY] * inv;
+  W[IVZ] = U[IVZ] * inv;
+  const float ke = 0.5f * ((W[IVX] * U[IVX] + W[IVY] * U[IVY]) + W[IVZ] * U[IVZ]);
+  W[IPR] = (a.gammad - 1.f) * (U[IPR] - ke) + t[B.gdp + lg];
+}
+
+// isentropic refill of the physical x1 ghost at interior-relative index gi
+// in FP64 from the total state of the boundary cell qb; writes the density
+// and pressure perturbations against the ghost level and the total density
+__device__ __forceinline__ void bg_refill_prim(const FusedArgs& a, const BgLayout& B,
+                                               const float* __restrict__ u, int ncell, int qb,
+                                               int gi, float* W, float& rtot) {
+  const bool upper = gi >= a.nx1;
+  const int lb = upper ? a.ng + a.nx1 - 1 : a.ng, lg = gi + a.ng;
+  const double* d = a.bgd;
+  const double g = (double)a.gammad, gm1 = g - 1.;
+  const double rb = d[lb] + (double)u[IDN * ncell + qb];
+  const double m1 = u[IVX * ncell + qb], m2 = u[IVY * ncell + qb], m3 = u[IVZ * ncell + qb];
+  const double eb = d[B.L + lb] / gm1 + (double)u[IPR * ncell + qb];
+  const double pb = gm1 * (eb - 0.5 * ((m1 * m1 + m2 * m2) + m3 * m3) / rb);
+  double rho, pre;
+  refill_layer64(rb, pb, g, a.bg_gdz, upper ? 1 : -1, upper ? gi - a.nx1 + 1 : -gi, rho, pre);
+  W[IDN] = (float)(rho - d[lg]);
+  W[IPR] = (float)(pre - d[B.L + lg]);
+  rtot = (float)rho;
+}
+
+// Flux of an x2 / x3 face (perturbation form) from the perturbation states
+// wl, wr in the face frame: the reference is the background of level lv.
+__device__ __forceinline__ void bg_horizontal_flux(const float* t, const BgLayout& B, int lv,
+                                                   int solver, float* wl, float* wr, float gammad,
+                                                   float* flx) {
+  const float rb = t[B.rho_c + lv], pb = t[B.p_c + lv];
+  const float rl = wl[IDN], rr = wr[IDN], pl = wl[IPR], pr = wr[IPR];
+  wl[IDN] += rb;
+  wr[IDN] += rb;
+  wl[IPR] += pb;
+  wr[IPR] += pb;
+  riemann_flux_pert<float>(solver, wl, wr, rl, rr, pl, pr, pb, gammad, flx);
+}
+
+// Flux of the x1 face below LDS cell c of a pencil (start ws + base, stride
+// 1) in perturbation form: WENO5 density from the total-density plane for
+// the weights plus the background offsets, CP5 pressure perturbation plus
+// its offset, CP5 velocities; face reference (rho_f, p_f) of face level fl.
+// rev_lo / rev_hi: physical bottom / top face with gravity (the reference's
+// LR revision copies density and pressure across the face).
+__device__ __forceinline__ void bg_x1_face_flux(const float* ws, int HS, int base, int c, int fl,
+                                                const BgLayout& B, const float* t, bool rev_lo,
+                                                bool rev_hi, int solver, float gammad,
+                                                float* flx) {
+  const float* rp = ws + IDN * HS + base;
+  const float* rt = ws + 5 * HS + base;
+  const float* pp = ws + IPR * HS + base;
+  float tw[6], fw[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    tw[k] = rt[c - 3 + k];
+    fw[k] = rp[c - 3 + k];
+  }
+  const int F = B.F;
+  const float dl[3] = {t[B.drl + fl], t[B.drl + F + fl], t[B.drl + 2 * F + fl]};
+  const float dr[3] = {t[B.drr + fl], t[B.drr + F + fl], t[B.drr + 2 * F + fl]};
+  float rl = weno5_off<true>(tw, fw, dl);
+  float rr = weno5_off<false>(tw + 1, fw + 1, dr);
+  float pl = cp5_hi(pp[c - 3], pp[c - 2], pp[c - 1], pp[c], pp[c + 1]) + t[B.epl + fl];
+  float pr = cp5_lo(pp[c - 2], pp[c - 1], pp[c], pp[c + 1], pp[c + 2]) + t[B.epr + fl];
+  if (rev_lo) rl = rr, pl = pr;
+  if (rev_hi) rr = rl, pr = pl;
+  const float rf = t[B.rho_f + fl], pf = t[B.p_f + fl];
+  float wl[5], wr[5];
+  wl[IDN] = rf + rl;
+  wr[IDN] = rf + rr;
+  wl[IPR] = pf + pl;
+  wr[IPR] = pf + pr;
+#pragma unroll
+  for (int v = IVX; v <= IVZ; ++v) {
+    const float* q = ws + v * HS + base;
+    wl[v] = cp5_hi(q[c - 3], q[c - 2], q[c - 1], q[c], q[c + 1]);
+    wr[v] = cp5_lo(q[c - 2], q[c - 1], q[c], q[c + 1], q[c + 2]);
+  }
+  riemann_flux_pert<float>(solver, wl, wr, rl, rr, pl, pr, pf, gammad, flx);
 }
 
 // Flux differences, geometric sources and du of the tile's cells on a
@@ -250,14 +357,20 @@ __device__ __forceinline__ void sph_cell_du(const FusedArgs& a, const float* ws,
 // kMerged: 1 / 0 when the launch knows that every tile takes the merged
 // face pass / the per-direction passes (the other path is not compiled in),
 // -1 to follow a.merged at run time (batches of mixed members).
-template <int kMerged, bool kSph = false>
+// kBg (merged face pass, Cartesian): perturbation path about the
+// hydrostatic background of a.bgt / a.bgd, as in fused_rhs_march_kernel.
+template <int kMerged, bool kSph = false, bool kBg = false>
 __device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __restrict__ du,
                                          const FusedArgs a, int b, float* smem) {
+  static_assert(!kBg || (kMerged == 1 && !kSph), "background path: merged Cartesian tiles");
   const bool merged = kMerged < 0 ? (a.merged != 0) : kMerged == 1;
   const int H1 = a.tx1 + 6, H2 = a.tx2 + 2 * a.h2, H3 = a.tx3 + 2 * a.h3;
   const int H = H1 * H2 * H3;
-  float* ws = smem;          // primitives [5][H3][H2][H1]
-  float* fs = smem + 5 * H;  // face fluxes of one direction [5][faces]
+  constexpr int NP = kBg ? 6 : 5;  // LDS planes: primitives (+ total density)
+  float* ws = smem;           // primitives [5][H3][H2][H1] (+ total density)
+  float* fs = smem + NP * H;  // face fluxes of one direction [5][faces]
+  const BgLayout BL(a.n1);
+  const float* bgt = a.bgt;
   const int ncell = a.n1 * a.n2 * a.n3;
 
   int t1 = b % a.nt1;
@@ -290,6 +403,20 @@ __device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __r
     if (fl[0]) U[IVX] = -U[IVX];
     if (fl[1]) U[IVY] = -U[IVY];
     if (fl[2]) U[IVZ] = -U[IVZ];
+    if (kBg) {
+      float rtot;
+      bg_load_prim(a, BL, U, gi, W, rtot);
+      if (refill && (gi < 0 || gi >= a.nx1) && (unsigned)(y - a.h2) < (unsigned)e2 &&
+          (unsigned)(z - a.h3) < (unsigned)e3) {
+        bool flb[3];
+        const int qb = mapped_cell(a, gi >= a.nx1 ? a.nx1 - 1 : 0, gj, gk, flb);
+        bg_refill_prim(a, BL, u, ncell, qb, gi, W, rtot);
+      }
+      ws[5 * H + l] = rtot;
+#pragma unroll
+      for (int v = 0; v < 5; ++v) ws[v * H + l] = W[v];
+      continue;
+    }
     cons_to_prim(U, gammad, W);
     // (only the tile's own columns: the x1 ghosts of halo columns are never read)
     if (merged && refill && (gi < 0 || gi >= a.nx1) && (unsigned)(y - a.h2) < (unsigned)e2 &&
@@ -389,6 +516,29 @@ __device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __r
       // IVX); register arrays are indexed by constants only, the frame is
       // entered and left through LDS addresses
       const int vmap[5] = {IDN, IVX + d, IVX + (d + 1) % 3, IVX + (d + 2) % 3, IPR};
+      if (kBg) {
+        float flx[5];
+        const int lv = o1 + fx + a.ng;  // x1 level of the face / of the x2, x3 face's cells
+        if (d == 0) {
+          const int gi = o1 + fx;
+          const bool rev = a.grav1 != 0.f;
+          bg_x1_face_flux(ws, H, base, c, lv, BL, bgt, rev && gi == 0, rev && gi == a.nx1,
+                          solver, gammad, flx);
+        } else {
+          float wl[5], wr[5];
+#pragma unroll
+          for (int v = 0; v < 5; ++v) {
+            int tv = (v == IDN) ? type : type2;
+            const float* p = ws + vmap[v] * H + base;
+            wr[v] = interp_face<float>(tv, 0, scale, p, sd, c);
+            wl[v] = interp_face<float>(tv, 1, scale, p, sd, c - 1);
+          }
+          bg_horizontal_flux(bgt, BL, lv, solver, wl, wr, gammad, flx);
+        }
+#pragma unroll
+        for (int v = 0; v < 5; ++v) fs[vmap[v] * ftot + g] = flx[v];
+        continue;
+      }
       float wl[5], wr[5];
   #pragma unroll
       for (int v = 0; v < 5; ++v) {
@@ -546,7 +696,13 @@ __device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __r
 #pragma unroll
     for (int v = 0; v < 5; ++v) d[v] = -dt * (acc[rr][v] / vol);
     float rho = ws[IDN * H + lin];
-    if (a.grav1 != 0.f) {
+    if (kBg) {
+      // the background's flux divergence and gravity are the residual
+      const int lv = o1 + cx + a.ng;
+      d[IVX] += dt * (rho * a.grav1 + bgt[BL.res + lv]);
+      rho = ws[5 * H + lin];
+      d[IPR] += dt * rho * ws[IVX * H + lin] * a.grav1;
+    } else if (a.grav1 != 0.f) {
       d[IVX] += dt * rho * a.grav1;
       d[IPR] += dt * rho * ws[IVX * H + lin] * a.grav1;
     }
@@ -572,6 +728,13 @@ __global__ void __launch_bounds__(kRhsThreads)
     fused_rhs_kernel(const float* __restrict__ u, float* __restrict__ du, FusedArgs a) {
   extern __shared__ float smem[];
   rhs_tile<kMerged>(u, du, a, blockIdx.x, smem);
+}
+
+// K1 of the perturbation path (merged face pass, Cartesian)
+__global__ void __launch_bounds__(kRhsThreads)
+    fused_rhs_bg_kernel(const float* __restrict__ u, float* __restrict__ du, FusedArgs a) {
+  extern __shared__ float smem[];
+  rhs_tile<1, false, true>(u, du, a, blockIdx.x, smem);
 }
 
 // K1 of spherical-polar blocks (merged face pass, geometric sources)
@@ -627,10 +790,15 @@ __device__ __forceinline__ void face_states(const float* ws, int HS, int base, i
   }
 }
 
-template <class P>
+// kBg: the state is the perturbation about the hydrostatic background of
+// a.bgt / a.bgd (bg_impl.hpp); the LDS planes hold (rho', v1, v2, v3, p') and
+// the total density, face states are references plus perturbations and the
+// solvers run in perturbation form (riemann_pert.hpp).
+template <class P, bool kBg = false>
 __global__ void __launch_bounds__(kRhsThreads, 4)
     fused_rhs_march_kernel(const float* __restrict__ u, float* __restrict__ du, FusedArgs a) {
   extern __shared__ float smem[];
+  constexpr int NP = kBg ? 6 : 5;  // LDS planes per slice
   const int H1 = a.tx1 + 6, H2 = a.tx2 + 6, HS = H1 * H2;
   const int ncell = a.n1 * a.n2 * a.n3;
   const int b = blockIdx.x;
@@ -656,7 +824,9 @@ __global__ void __launch_bounds__(kRhsThreads, 4)
   const int sol2 = P::r2 >= 0 ? P::r2 : a.riemann[1];
   const int sol3 = P::r3 >= 0 ? P::r3 : a.riemann[2];
   const int nf0 = (e1 + 1) * e2, ftot = nf0 + e1 * (e2 + 1);
-  float* fs = smem + 10 * HS;  // x1 then x2 face fluxes of one slice [5][ftot]
+  float* fs = smem + 2 * NP * HS;  // x1 then x2 face fluxes of one slice [5][ftot]
+  const BgLayout BL(a.n1);
+  const float* bgt = a.bgt;
   const float area1 = a.dx3 * a.dx2, area2 = a.dx3 * a.dx1, area3 = a.dx2 * a.dx1;
   const float vol = a.dx3 * (a.dx2 * a.dx1);
   const float dt = a.dt;
@@ -674,7 +844,7 @@ __global__ void __launch_bounds__(kRhsThreads, 4)
   }
 
   for (int kk = o3 - 3; kk <= o3 + ez + 2; ++kk) {
-    float* ws = smem + ((kk - o3 + 3) & 1) * 5 * HS;
+    float* ws = smem + ((kk - o3 + 3) & 1) * NP * HS;
     // ---- halo'd x1-x2 plane of slice kk: conserved -> primitive; x1 ghost
     //      refill of the tile's own columns as in rhs_tile ----
     for (int l = threadIdx.x; l < HS; l += blockDim.x) {
@@ -689,17 +859,29 @@ __global__ void __launch_bounds__(kRhsThreads, 4)
       if (fl[0]) U[IVX] = -U[IVX];
       if (fl[1]) U[IVY] = -U[IVY];
       if (fl[2]) U[IVZ] = -U[IVZ];
-      cons_to_prim(U, gammad, W);
-      if (refill && (gi < 0 || gi >= a.nx1) && (unsigned)(y - 3) < (unsigned)e2) {
-        const bool upper = gi >= a.nx1;
-        bool flb[3];
-        const int qb = mapped_cell(a, upper ? a.nx1 - 1 : 0, gj, kk, flb);
-        float Ub[5], Wb[5];
+      if (kBg) {
+        float rtot;
+        bg_load_prim(a, BL, U, gi, W, rtot);
+        if (refill && (gi < 0 || gi >= a.nx1) && (unsigned)(y - 3) < (unsigned)e2) {
+          bool flb[3];
+          const bool upper = gi >= a.nx1;
+          const int qb = mapped_cell(a, upper ? a.nx1 - 1 : 0, gj, kk, flb);
+          bg_refill_prim(a, BL, u, ncell, qb, gi, W, rtot);
+        }
+        ws[5 * HS + l] = rtot;
+      } else {
+        cons_to_prim(U, gammad, W);
+        if (refill && (gi < 0 || gi >= a.nx1) && (unsigned)(y - 3) < (unsigned)e2) {
+          const bool upper = gi >= a.nx1;
+          bool flb[3];
+          const int qb = mapped_cell(a, upper ? a.nx1 - 1 : 0, gj, kk, flb);
+          float Ub[5], Wb[5];
 #pragma unroll
-        for (int v = 0; v < 5; ++v) Ub[v] = u[v * ncell + qb];
-        cons_to_prim(Ub, gammad, Wb);  // rho and p do not depend on the flips
-        const IsentropicRefill R(Wb[IDN], Wb[IPR], gammad, gdz, upper ? 1 : -1);
-        R.layer(upper ? gi - a.nx1 + 1 : -gi, W[IDN], W[IPR]);
+          for (int v = 0; v < 5; ++v) Ub[v] = u[v * ncell + qb];
+          cons_to_prim(Ub, gammad, Wb);  // rho and p do not depend on the flips
+          const IsentropicRefill R(Wb[IDN], Wb[IPR], gammad, gdz, upper ? 1 : -1);
+          R.layer(upper ? gi - a.nx1 + 1 : -gi, W[IDN], W[IPR]);
+        }
       }
 #pragma unroll
       for (int v = 0; v < 5; ++v) ws[v * HS + l] = W[v];
@@ -728,7 +910,13 @@ __global__ void __launch_bounds__(kRhsThreads, 4)
         wl[v] = interp_face<float>(tv, 1, sc, &win[vmap[v]][0], 1, 2);
       }
       float flx[5];
-      riemann_flux<float>(sol3, wl, wr, gammad, 3, flx, (float*)nullptr);
+      if (kBg) {
+        // reference: the background of the own column's level
+        const int lv = o1 + cx + a.ng;
+        bg_horizontal_flux(bgt, BL, lv, sol3, wl, wr, gammad, flx);
+      } else {
+        riemann_flux<float>(sol3, wl, wr, gammad, 3, flx, (float*)nullptr);
+      }
 #pragma unroll
       for (int v = 0; v < 5; ++v) f3hi[vmap[v]] = flx[v];
     }
@@ -742,21 +930,27 @@ __global__ void __launch_bounds__(kRhsThreads, 4)
           // x1 face below cell (fx, fy) of the slice
           int fx;
           const int fy = dq1.div(g, fx);
-          face_states<0, P::ty0, P::ty20>(ws, HS, (fy + 3) * H1, 1, fx + 3, type0, ty20, scale0,
-                                         wl, wr);
-          if (refill) {
-            // _revise_x1inner_lr / _revise_x1outer_lr (hydro.cpp:217-229)
+          if (kBg) {
             const int gi = o1 + fx;
-            if (gi == 0) {
-              wl[IDN] = wr[IDN];
-              wl[IPR] = wr[IPR];
+            bg_x1_face_flux(ws, HS, (fy + 3) * H1, fx + 3, gi + a.ng, BL, bgt, refill && gi == 0,
+                            refill && gi == a.nx1, sol1, gammad, flx);
+          } else {
+            face_states<0, P::ty0, P::ty20>(ws, HS, (fy + 3) * H1, 1, fx + 3, type0, ty20,
+                                           scale0, wl, wr);
+            if (refill) {
+              // _revise_x1inner_lr / _revise_x1outer_lr (hydro.cpp:217-229)
+              const int gi = o1 + fx;
+              if (gi == 0) {
+                wl[IDN] = wr[IDN];
+                wl[IPR] = wr[IPR];
+              }
+              if (gi == a.nx1) {
+                wr[IDN] = wl[IDN];
+                wr[IPR] = wl[IPR];
+              }
             }
-            if (gi == a.nx1) {
-              wr[IDN] = wl[IDN];
-              wr[IPR] = wl[IPR];
-            }
+            riemann_flux<float>(sol1, wl, wr, gammad, 3, flx, (float*)nullptr);
           }
-          riemann_flux<float>(sol1, wl, wr, gammad, 3, flx, (float*)nullptr);
           fs[IDN * ftot + g] = flx[0];
           fs[IVX * ftot + g] = flx[1];
           fs[IVY * ftot + g] = flx[2];
@@ -768,7 +962,10 @@ __global__ void __launch_bounds__(kRhsThreads, 4)
           const int fy = de1.div(g - nf0, fx);
           face_states<1, P::ty1, P::ty21>(ws, HS, fx + 3, H1, fy + 3, type1, ty21, scale1, wl,
                                          wr);
-          riemann_flux<float>(sol2, wl, wr, gammad, 3, flx, (float*)nullptr);
+          if (kBg)
+            bg_horizontal_flux(bgt, BL, o1 + fx + a.ng, sol2, wl, wr, gammad, flx);
+          else
+            riemann_flux<float>(sol2, wl, wr, gammad, 3, flx, (float*)nullptr);
           fs[IDN * ftot + g] = flx[0];
           fs[IVY * ftot + g] = flx[1];
           fs[IVZ * ftot + g] = flx[2];
@@ -804,8 +1001,14 @@ __global__ void __launch_bounds__(kRhsThreads, 4)
         acc += area3 * (f3hi[v] - f3lo[v]);
         d5[v] = -dt * (acc / vol);
       }
-      const float rho = win[IDN][2];
-      if (a.grav1 != 0.f) {
+      float rho = win[IDN][2];
+      if (kBg) {
+        // the background's flux divergence and gravity are the residual
+        const int lv = o1 + cx + a.ng;
+        d5[IVX] += dt * (rho * a.grav1 + bgt[BL.res + lv]);
+        rho += bgt[BL.rho_c + lv];
+        d5[IPR] += dt * rho * win[IVX][2] * a.grav1;
+      } else if (a.grav1 != 0.f) {
         d5[IVX] += dt * rho * a.grav1;
         d5[IPR] += dt * rho * win[IVX][2] * a.grav1;
       }
@@ -840,6 +1043,9 @@ struct ColArgs {
   // (dx1v[is - 1], dx1v[ie]) and the coordinate
   double zin1, zout1;
   Geometry geo;
+  // hydrostatic background of the perturbation path (FP64 rho_b, p_b per x1
+  // level, bg_impl.hpp), null for the total state
+  const double* bgd;
 };
 
 // primitive state (FP64) of interior-relative layer i of a column whose
@@ -865,13 +1071,34 @@ __device__ __forceinline__ void float_prim(const float* Uf, bool flip, double ga
   cons_to_prim<double>(U, gammad, W);
 }
 
+// FP64 primitives of an FP32 cell of x1 level lvl: on the perturbation
+// path the background of that level is added first
+template <bool kBg>
+__device__ __forceinline__ void float_prim_at(const float* Uf, bool flip, const ColArgs& a,
+                                              int lvl, double* W) {
+  double U[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) U[v] = (double)Uf[v];
+  if (kBg) {
+    U[IDN] += a.bgd[lvl];
+    U[IPR] += a.bgd[a.n1 + lvl] / (a.gammad - 1.);
+  }
+  if (flip) U[IVX] = -U[IVX];
+  cons_to_prim<double>(U, a.gammad, W);
+}
+
 // isentropic refill of the ghost next to boundary cell wb (dir -1: below,
 // +1: above; src/hydro/hydro.cpp:231-269, first ghost), with the FP32
 // arithmetic of the refill in the RHS kernel, which computes the fluxes of the
-// same stage from these ghost values.
-template <bool kSph = false>
+// same stage from these ghost values (FP64 on the perturbation path, as
+// there).
+template <bool kSph = false, bool kBg = false>
 __device__ __forceinline__ void refill_ghost(const double* wb, double* wg, const ColArgs& a,
                                              int dir) {
+  if (kBg) {
+    refill_layer64(wb[IDN], wb[IPR], a.gammad, -a.grav1 * a.dx1, dir, 1, wg[IDN], wg[IPR]);
+    return;
+  }
   float rho, pre;
   if (kSph) {
     const IsentropicRefill R((float)wb[IDN], (float)wb[IPR], (float)a.gammad, -(float)a.grav1,
@@ -1193,7 +1420,7 @@ __device__ __forceinline__ void finish_cell(float* __restrict__ u, float* __rest
 // cyclic reduction and the even rows follow by back substitution.  Against
 // one row per lane this halves the PCR work and the lanes (a column of up to
 // 128 cells is one wavefront, without workgroup barriers).
-template <int N>
+template <int N, bool kBg = false>
 __device__ __forceinline__ void column_pcr2(float* __restrict__ u, float* __restrict__ u0,
                                             const float* __restrict__ du, const ColArgs& a,
                                             int col, double* __restrict__ sh) {
@@ -1220,6 +1447,7 @@ __device__ __forceinline__ void column_pcr2(float* __restrict__ u, float* __rest
   float Dv0[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, Dv1[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
   float Um[5] = {1.f, 0.f, 0.f, 0.f, 1.f}, Up[5] = {1.f, 0.f, 0.f, 0.f, 1.f};
   bool flm = false, flp = false;
+  int lm = a.ng, lp = a.ng;  // x1 levels of the ghost sources
   const bool top0 = act0 && i0 == nl - 1, top1 = act1 && i1 == nl - 1;
   if (act0) {
 #pragma unroll
@@ -1230,6 +1458,7 @@ __device__ __forceinline__ void column_pcr2(float* __restrict__ u, float* __rest
     }
     if (a.scheme && i0 == 0) {
       int ii = bc_map(-1, nl, a.bc_in, a.bc_out, flm);
+      lm = a.ng + ii;
 #pragma unroll
       for (int v = 0; v < 5; ++v) Um[v] = u[v * ncell + qb + ii];
     }
@@ -1244,6 +1473,7 @@ __device__ __forceinline__ void column_pcr2(float* __restrict__ u, float* __rest
   }
   if (a.scheme && (top0 || top1)) {
     int ii = bc_map(nl, nl, a.bc_in, a.bc_out, flp);
+    lp = a.ng + ii;
 #pragma unroll
     for (int v = 0; v < 5; ++v) Up[v] = u[v * ncell + qb + ii];
   }
@@ -1252,8 +1482,8 @@ __device__ __forceinline__ void column_pcr2(float* __restrict__ u, float* __rest
 #pragma unroll
   for (int e = 0; e < N; ++e) d0[e] = d1[e] = 0.;
   double w0[5], w1[5];
-  float_prim(Uc0, false, a.gammad, w0);
-  float_prim(Uc1, false, a.gammad, w1);
+  float_prim_at<kBg>(Uc0, false, a, a.ng + (act0 ? i0 : 0), w0);
+  float_prim_at<kBg>(Uc1, false, a, a.ng + (act1 ? i1 : 0), w1);
   double s_a2 = 1., s_b = 0.;
   const bool wave = half == 64;
   if (a.scheme) {
@@ -1273,12 +1503,12 @@ __device__ __forceinline__ void column_pcr2(float* __restrict__ u, float* __rest
 #pragma unroll
         for (int v = 0; v < 5; ++v) wm0[v] = sh[(5 + v) * T + t - 1];
       } else {
-        float_prim(Um, flm, a.gammad, wm0);
-        refill_ghost(w0, wm0, a, -1);
+        float_prim_at<kBg>(Um, flm, a, lm, wm0);
+        refill_ghost<false, kBg>(w0, wm0, a, -1);
       }
       if (top0) {
-        float_prim(Up, flp, a.gammad, wp0);
-        refill_ghost(w0, wp0, a, +1);
+        float_prim_at<kBg>(Up, flp, a, lp, wp0);
+        refill_ghost<false, kBg>(w0, wp0, a, +1);
       }
     }
     if (act1) {
@@ -1286,8 +1516,8 @@ __device__ __forceinline__ void column_pcr2(float* __restrict__ u, float* __rest
 #pragma unroll
         for (int v = 0; v < 5; ++v) wp1[v] = sh[v * T + t + 1];
       } else {
-        float_prim(Up, flp, a.gammad, wp1);
-        refill_ghost(w1, wp1, a, +1);
+        float_prim_at<kBg>(Up, flp, a, lp, wp1);
+        refill_ghost<false, kBg>(w1, wp1, a, +1);
       }
     }
     column_barrier(wave);
@@ -1403,7 +1633,7 @@ __device__ __forceinline__ void column_pcr2(float* __restrict__ u, float* __rest
 
 // kSph: spherical-polar blocks, per-layer face areas and volumes
 // (vic_assemble_cell with area_i != area_i+1) and the uneven ghost refill
-template <int N, bool kSph = false>
+template <int N, bool kSph = false, bool kBg = false>
 __device__ __forceinline__ void column_pcr(float* __restrict__ u, float* __restrict__ u0,
                                            const float* __restrict__ du, const ColArgs& a,
                                            int col, double* __restrict__ sh) {
@@ -1427,6 +1657,7 @@ __device__ __forceinline__ void column_pcr(float* __restrict__ u, float* __restr
   float Um[5] = {1.f, 0.f, 0.f, 0.f, 1.f}, Up[5] = {1.f, 0.f, 0.f, 0.f, 1.f};
   float Dv[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
   bool flm = false, flp = false;
+  int lm = a.ng, lp = a.ng;  // x1 levels of the ghost sources
   if (act) {
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
@@ -1437,11 +1668,13 @@ __device__ __forceinline__ void column_pcr(float* __restrict__ u, float* __restr
     // the x1 ghost sources of the end lanes, through the boundary functions
     if (a.scheme && i == 0) {
       int ii = bc_map(-1, nl, a.bc_in, a.bc_out, flm);
+      lm = a.ng + ii;
 #pragma unroll
       for (int v = 0; v < 5; ++v) Um[v] = u[v * ncell + q0 + ii];
     }
     if (a.scheme && i == nl - 1) {
       int ii = bc_map(nl, nl, a.bc_in, a.bc_out, flp);
+      lp = a.ng + ii;
 #pragma unroll
       for (int v = 0; v < 5; ++v) Up[v] = u[v * ncell + q0 + ii];
     }
@@ -1453,7 +1686,7 @@ __device__ __forceinline__ void column_pcr(float* __restrict__ u, float* __restr
   if (a.scheme) {
     // ---- primitives of the cell and of its neighbours ----
     double wc[5], wm[5], wp[5];
-    float_prim(Uc, false, a.gammad, wc);
+    float_prim_at<kBg>(Uc, false, a, a.ng + (act ? i : 0), wc);
 #pragma unroll
     for (int v = 0; v < 5; ++v) sh[v * T + t] = wc[v];
     column_barrier(wave);
@@ -1464,15 +1697,15 @@ __device__ __forceinline__ void column_pcr(float* __restrict__ u, float* __restr
 #pragma unroll
         for (int v = 0; v < 5; ++v) wm[v] = sh[v * T + t - 1];
       } else {
-        float_prim(Um, flm, a.gammad, wm);
-        refill_ghost<kSph>(wc, wm, a, -1);
+        float_prim_at<kBg>(Um, flm, a, lm, wm);
+        refill_ghost<kSph, kBg>(wc, wm, a, -1);
       }
       if (i < nl - 1) {
 #pragma unroll
         for (int v = 0; v < 5; ++v) wp[v] = sh[v * T + t + 1];
       } else {
-        float_prim(Up, flp, a.gammad, wp);
-        refill_ghost<kSph>(wc, wp, a, +1);
+        float_prim_at<kBg>(Up, flp, a, lp, wp);
+        refill_ghost<kSph, kBg>(wc, wp, a, +1);
       }
     }
     column_barrier(wave);
@@ -1593,13 +1826,14 @@ __device__ __forceinline__ void column_pcr(float* __restrict__ u, float* __restr
   }
 }
 
-template <int N>
+// kBg: perturbation state about the hydrostatic background a.bgd
+template <int N, bool kBg = false>
 __global__ void __launch_bounds__(PcrShape<N>::kThreads)
     fused_column_pcr_kernel(float* __restrict__ u, float* __restrict__ u0,
                             const float* __restrict__ du, ColArgs a) {
   extern __shared__ double sh[];
   int col = blockIdx.x * (blockDim.x >> a.lshift) + (threadIdx.x >> a.lshift);
-  column_pcr<N>(u, u0, du, a, col, sh);
+  column_pcr<N, false, kBg>(u, u0, du, a, col, sh);
 }
 
 // K2 of spherical-polar blocks
@@ -1612,12 +1846,13 @@ __global__ void __launch_bounds__(PcrShape<N>::kThreads)
   column_pcr<N, true>(u, u0, du, a, col, sh);
 }
 
+template <bool kBg = false>
 __global__ void __launch_bounds__(PcrShape<3>::kThreads, 2)
     fused_column_pcr2_kernel(float* __restrict__ u, float* __restrict__ u0,
                              const float* __restrict__ du, ColArgs a) {
   extern __shared__ double sh[];
   int col = blockIdx.x * (blockDim.x >> a.lshift) + (threadIdx.x >> a.lshift);
-  column_pcr2<3>(u, u0, du, a, col, sh);
+  column_pcr2<3, kBg>(u, u0, du, a, col, sh);
 }
 
 // ------------------------------------------------------- batched members ----
@@ -1774,7 +2009,9 @@ static size_t rhs_smem_bytes(const FusedArgs& a) {
   size_t H = (size_t)(a.tx1 + 6) * (a.tx2 + 2 * a.h2) * (a.tx3 + 2 * a.h3);
   // spherical-polar tiles keep the x1 face pressures after the fluxes
   size_t fp = a.geo.coord == SNAPX_COORD_SPHERICAL_POLAR ? (size_t)(a.tx1 + 1) * a.tx2 * a.tx3 : 0;
-  return sizeof(float) * (5 * (H + rhs_face_count(a, a.merged)) + fp);
+  // the perturbation path keeps the total density as a sixth plane
+  size_t bgp = a.bgt ? H : 0;
+  return sizeof(float) * (5 * (H + rhs_face_count(a, a.merged)) + fp + bgp);
 }
 
 static int check_fused(const snapx_config* c) {
@@ -1838,6 +2075,9 @@ static FusedArgs fill_fused_args(const snapx_config* cfg, Tile t, double dt) {
   a.dt = (float)dt;
   a.grav1 = (float)cfg->grav1, a.grav2 = (float)cfg->grav2, a.grav3 = (float)cfg->grav3;
   a.geo = make_geometry(cfg);
+  a.bgt = nullptr;
+  a.bgd = nullptr;
+  a.bg_gdz = -cfg->grav1 * g.dx1;
   const bool sph = cfg->coord == SNAPX_COORD_SPHERICAL_POLAR;
   // ghost-layer distances of the refill: dx1v of the layers crossed
   double zi = 0., zo = 0.;
@@ -1874,7 +2114,7 @@ static bool march_ok(const snapx_config* cfg) {
 static size_t march_smem_bytes(const FusedArgs& a) {
   size_t HS = (size_t)(a.tx1 + 6) * (a.tx2 + 6);
   size_t ftot = (size_t)(a.tx1 + 1) * a.tx2 + (size_t)a.tx1 * (a.tx2 + 1);
-  return sizeof(float) * (10 * HS + 5 * ftot);
+  return sizeof(float) * ((a.bgt ? 12 : 10) * HS + 5 * ftot);
 }
 
 static int make_march_args(const snapx_config* cfg, double dt, FusedArgs* out) {
@@ -1897,11 +2137,17 @@ using MarchKernel = void (*)(const float*, float*, FusedArgs);
 // and pressure, shock off, scale off, on both axis groups): LMARS in x1 with
 // Roe in x2 / x3, and one solver in every direction.  Other options run the
 // generic instance.
-static MarchKernel march_kernel_for(const snapx_config* c) {
+static MarchKernel march_kernel_for(const snapx_config* c, bool bg) {
   constexpr int W5 = SNAPX_INTERP_WENO5, C5 = SNAPX_INTERP_CP5;
   constexpr int RO = SNAPX_RIEMANN_ROE, LM = SNAPX_RIEMANN_LMARS, HL = SNAPX_RIEMANN_HLLC;
   const bool weno5_cp5 = c->recon_type[0] == W5 && c->recon_type[1] == W5 && !c->recon_shock[0] &&
                          !c->recon_shock[1] && !c->recon_scale[0] && !c->recon_scale[1];
+  if (bg) {
+    // perturbation path: the 3-D workload's solvers, else the generic instance
+    if (weno5_cp5 && c->riemann[0] == LM && c->riemann[1] == RO && c->riemann[2] == RO)
+      return fused_rhs_march_kernel<MarchFaces<W5, C5, W5, C5, LM, RO, RO>, true>;
+    return fused_rhs_march_kernel<MarchGeneric, true>;
+  }
   if (weno5_cp5) {
     const int r1 = c->riemann[0], r2 = c->riemann[1], r3 = c->riemann[2];
     if (r1 == LM && r2 == RO && r3 == RO)
@@ -1936,6 +2182,7 @@ static int make_col_args(const snapx_config* cfg, double dt, ColArgs* out) {
   c.lanes = 64, c.lshift = 6;
   while (c.lanes < c.nl) c.lanes <<= 1, c.lshift += 1;
   c.geo = make_geometry(cfg);
+  c.bgd = nullptr;
   const bool sph = cfg->coord == SNAPX_COORD_SPHERICAL_POLAR;
   c.zin1 = sph ? sph_dx1v(c.geo, g.ng - 1) : g.dx1;
   c.zout1 = sph ? sph_dx1v(c.geo, g.n1 - g.ng - 1) : g.dx1;
@@ -1979,8 +2226,10 @@ struct EventPair {
 
 // One fused RK3 step; with ev != nullptr the 7 launches are bracketed by the
 // event pairs ev[0..6] (K1, K2 per stage, then the boundary kernel).
+// bg: device tables of the hydrostatic background (bg_impl.hpp); u is then
+// the perturbation state and the stored ghosts are not refreshed.
 static int fused_step(const snapx_config* cfg, double dt, float* u, void* workspace,
-                      hipStream_t st, EventPair* ev, bool bc = true) {
+                      hipStream_t st, EventPair* ev, bool bc = true, const void* bg = nullptr) {
   Grid g = make_grid(cfg);
   size_t field = align256(sizeof(float) * 5 * (size_t)g.ncell);
   char* base = reinterpret_cast<char*>(workspace);
@@ -2003,12 +2252,21 @@ static int fused_step(const snapx_config* cfg, double dt, float* u, void* worksp
     Tile t = choose_tile(g);
     // 3-D tiles: 16x4x2, every face of the tile in one LDS pass
     if (g.n3 > 1) t = {16, 4, 2};
+    // the perturbation path runs merged tiles only
+    if (bg && g.n3 == 1 && g.n2 > 1) t = {16, 4, 1};
     if (int e = make_fused_args(cfg, t, dt, &a)) return e;
   }
+  if (bg) {
+    a.bgd = reinterpret_cast<const double*>(bg);
+    a.bgt = reinterpret_cast<const float*>(a.bgd + 2 * g.n1);
+    SNAPX_CHECK_ARG(march || a.merged, "perturbation path: tile without a merged face pass");
+  }
   size_t smem = march ? march_smem_bytes(a) : rhs_smem_bytes(a);
+  SNAPX_CHECK_ARG(smem <= 64 * 1024, "fused path: tile needs more than 64 KB LDS");
   int nblk = a.nt1 * a.nt2 * a.nt3;
   ColArgs c;
   if (int e = make_col_args(cfg, dt, &c)) return e;
+  c.bgd = a.bgd;
   int N = ((cfg->implicit_scheme >> 3) & 1) ? 5 : 3;
   int cblk = (c.ncol + kColThreads - 1) / kColThreads;
 
@@ -2022,8 +2280,11 @@ static int fused_step(const snapx_config* cfg, double dt, float* u, void* worksp
       hipExtLaunchKernelGGL(fused_rhs_sph_kernel, dim3(nblk), dim3(kRhsThreads), (uint32_t)smem,
                             st, ra, rb, 0u, (const float*)u, du, a);
     else if (march)
-      hipExtLaunchKernelGGL(march_kernel_for(cfg), dim3(nblk), dim3(kRhsThreads),
+      hipExtLaunchKernelGGL(march_kernel_for(cfg, bg != nullptr), dim3(nblk), dim3(kRhsThreads),
                             (uint32_t)smem, st, ra, rb, 0u, (const float*)u, du, a);
+    else if (bg)
+      hipExtLaunchKernelGGL(fused_rhs_bg_kernel, dim3(nblk), dim3(kRhsThreads), (uint32_t)smem,
+                            st, ra, rb, 0u, (const float*)u, du, a);
     else if (a.merged)
       hipExtLaunchKernelGGL(fused_rhs_kernel<1>, dim3(nblk), dim3(kRhsThreads), (uint32_t)smem, st,
                             ra, rb, 0u, (const float*)u, du, a);
@@ -2045,9 +2306,18 @@ static int fused_step(const snapx_config* cfg, double dt, float* u, void* worksp
       else if (sph)
         hipExtLaunchKernelGGL(fused_column_pcr_sph_kernel<5>, grid, block, smem, st, ca, cb, 0u, u,
                               u0, (const float*)du, c);
+      else if (bg && N == 3 && c.rows2)
+        hipExtLaunchKernelGGL(fused_column_pcr2_kernel<true>, grid, block, smem, st, ca, cb, 0u, u,
+                              u0, (const float*)du, c);
+      else if (bg && N == 3)
+        hipExtLaunchKernelGGL((fused_column_pcr_kernel<3, true>), grid, block, smem, st, ca, cb, 0u,
+                              u, u0, (const float*)du, c);
+      else if (bg)
+        hipExtLaunchKernelGGL((fused_column_pcr_kernel<5, true>), grid, block, smem, st, ca, cb, 0u,
+                              u, u0, (const float*)du, c);
       else if (N == 3 && c.rows2)
-        hipExtLaunchKernelGGL(fused_column_pcr2_kernel, grid, block, smem, st, ca, cb, 0u, u, u0,
-                              (const float*)du, c);
+        hipExtLaunchKernelGGL(fused_column_pcr2_kernel<false>, grid, block, smem, st, ca, cb, 0u, u,
+                              u0, (const float*)du, c);
       else if (N == 3)
         hipExtLaunchKernelGGL(fused_column_pcr_kernel<3>, grid, block, smem, st, ca, cb, 0u, u,
                               u0, (const float*)du, c);
@@ -2137,6 +2407,171 @@ extern "C" int snapx_fused_rk3_steps(const snapx_config* cfg, double dt, int nst
   for (auto& e : ev) {
     hipEventDestroy(e.a);
     hipEventDestroy(e.b);
+  }
+  return status;
+}
+
+// ------------------------------ perturbation path (hydrostatic background) ----
+
+namespace snapx {
+
+static int check_bg(const snapx_config* c) {
+  if (int e = check_fused(c)) return e;
+  SNAPX_CHECK_ARG(c->coord == SNAPX_COORD_CARTESIAN, "perturbation path: Cartesian blocks only");
+  SNAPX_CHECK_ARG(c->recon_type[0] == SNAPX_INTERP_WENO5 && !c->recon_shock[0],
+                  "perturbation path: vertical reconstruction must be weno5 with shock: false");
+  SNAPX_CHECK_ARG(!c->recon_scale[0] && !c->recon_scale[1],
+                  "perturbation path: reconstruction scale must be off");
+  SNAPX_CHECK_ARG(c->grav2 == 0.0 && c->grav3 == 0.0,
+                  "perturbation path: the background varies with x1 only (grav2 = grav3 = 0)");
+  ColArgs ca;
+  if (int e = make_col_args(c, 1.0, &ca)) return e;
+  const int N = ((c->implicit_scheme >> 3) & 1) ? 5 : 3;
+  SNAPX_CHECK_ARG(pcr_ok(ca, N),
+                  "perturbation path: columns of at most 512 (3x3) / 128 (5x5) cells");
+  return 0;
+}
+
+template <typename T>
+__global__ void bg_split_kernel(const T* __restrict__ u, const double* __restrict__ bgd,
+                                float* __restrict__ up, int n1, int ncell, double igm1) {
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < ncell; q += gridDim.x * blockDim.x) {
+    const int l = q % n1;
+    up[IDN * ncell + q] = (float)((double)u[IDN * ncell + q] - bgd[l]);
+    up[IVX * ncell + q] = (float)u[IVX * ncell + q];
+    up[IVY * ncell + q] = (float)u[IVY * ncell + q];
+    up[IVZ * ncell + q] = (float)u[IVZ * ncell + q];
+    up[IPR * ncell + q] = (float)((double)u[IPR * ncell + q] - bgd[n1 + l] * igm1);
+  }
+}
+
+template <typename T>
+__global__ void bg_join_kernel(const float* __restrict__ up, const double* __restrict__ bgd,
+                               T* __restrict__ u, int n1, int ncell, double igm1) {
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < ncell; q += gridDim.x * blockDim.x) {
+    const int l = q % n1;
+    u[IDN * ncell + q] = (T)(bgd[l] + (double)up[IDN * ncell + q]);
+    u[IVX * ncell + q] = (T)up[IVX * ncell + q];
+    u[IVY * ncell + q] = (T)up[IVY * ncell + q];
+    u[IVZ * ncell + q] = (T)up[IVZ * ncell + q];
+    u[IPR * ncell + q] = (T)(bgd[n1 + l] * igm1 + (double)up[IPR * ncell + q]);
+  }
+}
+
+}  // namespace snapx
+
+extern "C" size_t snapx_fused_background_bytes(const snapx_config* cfg) {
+  if (check_bg(cfg)) return 0;
+  return bg_bytes(make_grid(cfg).n1);
+}
+
+extern "C" int snapx_fused_background(const snapx_config* cfg, const double* rho_bar,
+                                      const double* p_bar, void* bg, size_t bg_bytes_,
+                                      void* stream) {
+  if (int e = check_bg(cfg)) return e;
+  SNAPX_CHECK_ARG(rho_bar && p_bar && bg, "snapx_fused_background: null pointer");
+  const int n1 = make_grid(cfg).n1;
+  SNAPX_CHECK_ARG(bg_bytes_ >= bg_bytes(n1), "snapx_fused_background: buffer too small");
+  for (int l = 0; l < n1; ++l)
+    SNAPX_CHECK_ARG(rho_bar[l] > 0. && p_bar[l] > 0. && std::isfinite(rho_bar[l]) &&
+                        std::isfinite(p_bar[l]),
+                    "snapx_fused_background: background density and pressure must be positive");
+  std::vector<double> d64;
+  std::vector<float> f32;
+  bg_build_tables(cfg, rho_bar, p_bar, d64, f32);
+  hipStream_t st = as_stream(stream);
+  char* dst = reinterpret_cast<char*>(bg);
+  SNAPX_HIP(hipMemcpyAsync(dst, d64.data(), sizeof(double) * d64.size(), hipMemcpyHostToDevice, st));
+  SNAPX_HIP(hipMemcpyAsync(dst + sizeof(double) * d64.size(), f32.data(),
+                           sizeof(float) * f32.size(), hipMemcpyHostToDevice, st));
+  // the host tables go out of scope here
+  SNAPX_HIP(hipStreamSynchronize(st));
+  return 0;
+}
+
+extern "C" int snapx_fused_split(const snapx_config* cfg, const void* bg, const void* u, int dtype,
+                                 float* u_pert, void* stream) {
+  if (int e = check_bg(cfg)) return e;
+  SNAPX_CHECK_ARG(bg && u && u_pert, "snapx_fused_split: null pointer");
+  SNAPX_CHECK_ARG(dtype == SNAPX_F32 || dtype == SNAPX_F64, "snapx_fused_split: dtype");
+  Grid g = make_grid(cfg);
+  const double igm1 = 1. / (cfg->gammad - 1.);
+  const double* bgd = reinterpret_cast<const double*>(bg);
+  hipStream_t st = as_stream(stream);
+  if (dtype == SNAPX_F64)
+    hipLaunchKernelGGL(bg_split_kernel<double>, blocks_for(g.ncell, 256), dim3(256), 0, st,
+                       reinterpret_cast<const double*>(u), bgd, u_pert, g.n1, g.ncell, igm1);
+  else
+    hipLaunchKernelGGL(bg_split_kernel<float>, blocks_for(g.ncell, 256), dim3(256), 0, st,
+                       reinterpret_cast<const float*>(u), bgd, u_pert, g.n1, g.ncell, igm1);
+  SNAPX_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int snapx_fused_join(const snapx_config* cfg, const void* bg, const float* u_pert,
+                                void* u, int dtype, void* stream) {
+  if (int e = check_bg(cfg)) return e;
+  SNAPX_CHECK_ARG(bg && u && u_pert, "snapx_fused_join: null pointer");
+  SNAPX_CHECK_ARG(dtype == SNAPX_F32 || dtype == SNAPX_F64, "snapx_fused_join: dtype");
+  Grid g = make_grid(cfg);
+  const double igm1 = 1. / (cfg->gammad - 1.);
+  const double* bgd = reinterpret_cast<const double*>(bg);
+  hipStream_t st = as_stream(stream);
+  if (dtype == SNAPX_F64)
+    hipLaunchKernelGGL(bg_join_kernel<double>, blocks_for(g.ncell, 256), dim3(256), 0, st, u_pert,
+                       bgd, reinterpret_cast<double*>(u), g.n1, g.ncell, igm1);
+  else
+    hipLaunchKernelGGL(bg_join_kernel<float>, blocks_for(g.ncell, 256), dim3(256), 0, st, u_pert,
+                       bgd, reinterpret_cast<float*>(u), g.n1, g.ncell, igm1);
+  SNAPX_LAUNCH_CHECK();
+  // the stored ghosts of the perturbation are not maintained: the total
+  // state gets the reference's boundary pass
+  return snapx_apply_bc(cfg, u, 5, SNAPX_CONSERVED, dtype, stream);
+}
+
+extern "C" int snapx_fused_rk3_steps_bg(const snapx_config* cfg, const void* bg, double dt,
+                                        int nsteps, float* u_pert, void* workspace,
+                                        size_t workspace_bytes, snapx_timing* timing,
+                                        void* stream) {
+  if (int e = check_bg(cfg)) return e;
+  SNAPX_CHECK_ARG(bg && u_pert && workspace && nsteps >= 0, "snapx_fused_rk3_steps_bg: bad arguments");
+  SNAPX_CHECK_ARG(workspace_bytes >= snapx_fused_workspace_bytes(cfg),
+                  "snapx_fused_rk3_steps_bg: workspace too small");
+  hipStream_t st = as_stream(stream);
+  if (!timing) {
+    for (int n = 0; n < nsteps; ++n)
+      if (int e = fused_step(cfg, dt, u_pert, workspace, st, nullptr, false, bg)) return e;
+    return 0;
+  }
+  EventPair ev[7];
+  for (auto& e : ev) {
+    SNAPX_HIP(hipEventCreate(&e.a));
+    SNAPX_HIP(hipEventCreate(&e.b));
+  }
+  int status = 0;
+  for (int n = 0; n < nsteps && !status; ++n) {
+    status = fused_step(cfg, dt, u_pert, workspace, st, ev, false, bg);
+    if (status) break;
+    if (hipEventSynchronize(ev[5].b) != hipSuccess) {
+      set_error("snapx_fused_rk3_steps_bg: event synchronisation failed");
+      status = SNAPX_EHIP;
+      break;
+    }
+    for (int k = 0; k < 6; ++k) {
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, ev[k].a, ev[k].b);
+      if (k % 2 == 0) {
+        timing->rhs_ms += ms;
+        timing->rhs_launches += 1;
+      } else {
+        timing->column_ms += ms;
+        timing->column_launches += 1;
+      }
+    }
+  }
+  for (auto& e : ev) {
+    (void)hipEventDestroy(e.a);
+    (void)hipEventDestroy(e.b);
   }
   return status;
 }
