@@ -12,8 +12,10 @@ implicit 3x3 column solve (65 536 columns), SSP-RK3, dt = 0.2 s, 100 steps.
 Synthetic initial state, inputs resident in HBM.
 
 Reported:
-  value        FP32 fused path (FP64 column solve): interior cells x steps /
-               wall seconds
+  value        FP32 fused path carrying the state as a perturbation about the
+               FP64 hydrostatic background (FP64 column solve): interior
+               cells x steps / wall seconds
+  fp32_total_state  the same fused step on the FP32 total state
   fp64_plain   the FP64 plain (module-by-module) GPU path on the same steps
   error        per-variable L2 / Linf relative error of FP32 fused vs FP64
                plain after the timed steps (each field against its own norm,
@@ -467,11 +469,33 @@ def copy_rate_gbs(dev):
     return 2 * src.numel() * 4 * 10 / (e0.elapsed_time(e1) * 1e-3) / 1e9
 
 
-def fused_vs_plain(capi, cfg, u0, dt, K, W, dev, timing_steps=3):
+def fused_vs_plain(capi, cfg, u0, dt, K, W, dev, timing_steps=3, bgcols=None):
     """Times K fused FP32 steps and K plain FP64 steps from the same state
     (after W warm-up steps on copies); per-kernel device times of the fused
-    step from HIP events over `timing_steps` more steps."""
+    step from HIP events over `timing_steps` more steps.  With bgcols =
+    (rho_bar, p_bar) per x1 level, also the FP32 perturbation path about that
+    FP64 background (split from the FP64 state, joined back to FP64)."""
     import torch
+    out = {}
+    if bgcols is not None:
+        bg = capi.Background(cfg, *bgcols)
+        wsb = capi.fused_workspace(cfg)
+        up = bg.split(torch.from_numpy(u0).to(dev, torch.float64))
+        bg.steps(dt, W, up.clone(), wsb)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        bg.steps(dt, K, up, wsb)
+        torch.cuda.synchronize()
+        out["tbg"] = time.perf_counter() - t0
+        tim = capi.Timing()
+        bg.steps(dt, timing_steps, up.clone(), wsb, timing=tim)
+        torch.cuda.synchronize()
+        out["bg_rhs_ms"] = tim.rhs_ms / max(tim.rhs_launches, 1)
+        out["bg_col_ms"] = tim.column_ms / max(tim.column_launches, 1)
+        ubg = torch.empty(up.shape, dtype=torch.float64, device=dev)
+        bg.join(up, ubg)
+        out["ubg"] = ubg
+        del wsb, up, bg
     u32 = torch.from_numpy(u0).to(dev, torch.float32)
     ws32 = capi.fused_workspace(cfg)
     capi.fused_rk3_steps(cfg, dt, W, u32.clone(), ws32)
@@ -498,10 +522,11 @@ def fused_vs_plain(capi, cfg, u0, dt, K, W, dev, timing_steps=3):
     torch.cuda.synchronize()
     t64 = time.perf_counter() - t0
     del ws64, w64
-    return {"u32": u32, "u64": u64, "t32": t32, "t64": t64,
-            "rhs_ms": tim.rhs_ms / max(tim.rhs_launches, 1),
-            "col_ms": tim.column_ms / max(tim.column_launches, 1),
-            "launches": int(tim.rhs_launches + tim.column_launches + tim.bc_launches)}
+    out.update({"u32": u32, "u64": u64, "t32": t32, "t64": t64,
+                "rhs_ms": tim.rhs_ms / max(tim.rhs_launches, 1),
+                "col_ms": tim.column_ms / max(tim.column_launches, 1),
+                "launches": int(tim.rhs_launches + tim.column_launches + tim.bc_launches)})
+    return out
 
 
 def adjoint_ratio(capi, cfg, u0, dt, dev, n=3):
@@ -567,26 +592,35 @@ def main():
     cfg = box3d_cfg(capi)
     dt = 0.2  # horizontal acoustic CFL 0.7 on 100 m cells (0.5 s blows up, oracle too)
     u0 = conserved_on_device(capi, cfg, box3d_prim(cfg).numpy(), dev).cpu().numpy()
+    z = (torch.arange(cfg.shape[3], dtype=torch.float64) - 3 + 0.5) * 100.0
+    Tb, pb = capped_background(z, gamma=cfg.gammad)
+    bgcols = ((pb / (287.0 * Tb)).numpy(), pb.numpy())
     I = (slice(3, -3), slice(3, -3), slice(3, -3))  # interior of one field [n3, n2, n1]
-    log(f"3-D box 256x256x128: {K} fused FP32 and {K} plain FP64 steps")
-    r = fused_vs_plain(capi, cfg, u0, dt, K, W, dev)
-    cus32 = cfg.n_interior * K / r["t32"]
+    log(f"3-D box 256x256x128: {K} fused FP32 steps (perturbation path and total state) and "
+        f"{K} plain FP64 steps")
+    r = fused_vs_plain(capi, cfg, u0, dt, K, W, dev, bgcols=bgcols)
+    # headline: the FP32 fused step about the FP64 hydrostatic background
+    cus32 = cfg.n_interior * K / r["tbg"]
+    cus32_total = cfg.n_interior * K / r["t32"]
     cus64 = cfg.n_interior * K / r["t64"]
-    got, ref = r["u32"].double().cpu().numpy(), r["u64"].cpu().numpy()
+    got, ref = r["ubg"].cpu().numpy(), r["u64"].cpu().numpy()
+    got_tot = r["u32"].double().cpu().numpy()
     err = errors(got, ref, I)
+    err_tot = errors(got_tot, ref, I)
     m0 = u0[0][I].sum()
     mass_drift = float(abs(got[0][I].sum() - m0) / m0)
+    mass_drift_tot = float(abs(got_tot[0][I].sum() - m0) / m0)
     mass_drift64 = float(abs(ref[0][I].sum() - m0) / m0)
     finite = bool(np.isfinite(got).all())
-    del r["u32"], r["u64"], got, ref
+    del r["u32"], r["u64"], r["ubg"], got, ref, got_tot
 
     # roofline of the dominant kernel: K1 (fused RHS, 3-D marching tiles)
     # reads the stage state and writes the tendency, 2 NV 4 B per interior
     # cell; K2 (column solve + stage combine, two rows per lane) reads the
     # tendency, the stage state and u0 and writes the state, 4 NV 4 B
     n = cfg.n_interior
-    k1 = ("fused_rhs_march_kernel", r["rhs_ms"], 2 * NV * 4 * n)
-    k2 = ("fused_column_pcr2_kernel", r["col_ms"], 4 * NV * 4 * n)
+    k1 = ("fused_rhs_march_kernel<bg>", r["bg_rhs_ms"], 2 * NV * 4 * n)
+    k2 = ("fused_column_pcr2_kernel<bg>", r["bg_col_ms"], 4 * NV * 4 * n)
     dom = k1 if k1[1] >= k2[1] else k2
     roof = kernel_roofline(*dom, "box256x256x128")
     other = k2 if dom is k1 else k1
@@ -608,6 +642,11 @@ def main():
         cpu, u_or = cpu_baseline(cfg, u0, dt, steps=2)
         a = torch.from_numpy(u0).to(dev, torch.float32)
         capi.fused_rk3_steps(cfg, dt, 2, a)
+        bgo = capi.Background(cfg, *bgcols)
+        up = bgo.split(torch.from_numpy(u0).to(dev, torch.float64))
+        bgo.steps(dt, 2, up)
+        ab = torch.empty(up.shape, dtype=torch.float64, device=dev)
+        bgo.join(up, ab)
         b = torch.from_numpy(u0).to(dev, torch.float64)
         wb = torch.empty_like(b)
         wsb = capi.rk3_workspace(cfg, capi.F64)
@@ -616,8 +655,10 @@ def main():
         torch.cuda.synchronize()
         oracle_check = {"steps": 2,
                         "fp64_plain_max_rel": max_rel(b.cpu().numpy(), u_or, I),
-                        "fp32_fused_max_rel": max_rel(a.double().cpu().numpy(), u_or, I)}
-        del a, b, wb, wsb, u_or
+                        "fp32_fused_background_max_rel": max_rel(ab.cpu().numpy(), u_or, I),
+                        "fp32_fused_total_state_max_rel": max_rel(a.double().cpu().numpy(), u_or,
+                                                                  I)}
+        del a, b, wb, wsb, u_or, ab, up, bgo
 
     # ---------------- adjoint / forward on the headline block ----------------
     adj = None
@@ -632,8 +673,11 @@ def main():
         cs = straka_cfg(capi)
         us0 = conserved_on_device(capi, cs, straka_prim(cs), dev).cpu().numpy()
         Is = (slice(None), slice(3, -3), slice(3, -3))
-        rs = fused_vs_plain(capi, cs, us0, 0.1, 1000, 10, dev, timing_steps=100)
+        wbg = straka_prim(cs, dT=0.0)
+        rs = fused_vs_plain(capi, cs, us0, 0.1, 1000, 10, dev, timing_steps=100,
+                            bgcols=(wbg[0, 0, 0].copy(), wbg[4, 0, 0].copy()))
         g32, g64 = rs["u32"].double().cpu().numpy(), rs["u64"].cpu().numpy()
+        gbg = rs["ubg"].cpu().numpy()
         ms0 = us0[0][Is].sum()
         n2 = cs.n_interior
         ks1 = ("fused_rhs_kernel", rs["rhs_ms"], 2 * NV * 4 * n2)
@@ -652,11 +696,19 @@ def main():
             "mass_drift_fp64": float(abs(g64[0][Is].sum() - ms0) / ms0),
             "reference_convention_value": round(
                 cs.shape[1] * cs.shape[2] * cs.shape[3] * 1000 * 3 / rs["t32"], 1),
+            "background_path": {
+                "cell_updates_per_s": round(n2 * 1000 / rs["tbg"], 1),
+                "ms_per_step": round(rs["tbg"] / 1000 * 1e3, 5),
+                "fused_over_fp64_plain": round(rs["t64"] / rs["tbg"], 3),
+                "fused_rhs_avg_ms": round(rs["bg_rhs_ms"], 5),
+                "fused_column_avg_ms": round(rs["bg_col_ms"], 5),
+                "error_fp32_vs_fp64_after_steps": errors(gbg, g64, Is),
+                "mass_drift_fp32": float(abs(gbg[0][Is].sum() - ms0) / ms0)},
         }
         if not args.no_adjoint:
             extra["straka_2d_512x64_fp32"]["adjoint_fp64"] = adjoint_ratio(capi, cs, us0, 0.1,
                                                                           dev, n=20)
-        del rs, g32, g64
+        del rs, g32, g64, gbg
 
     # ---- spherical-polar shell sector (64 radial x 512 polar cells):
     #      FP32 fused against FP64 plain on the same 100 steps ----
@@ -733,31 +785,39 @@ def main():
         "n_gpus": 1,
         "steps": K,
         "warmup": W,
-        "ms_per_step": round(r["t32"] / K * 1e3, 5),
+        "ms_per_step": round(r["tbg"] / K * 1e3, 5),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 (FP64 implicit column solve)",
+        "dtype": "f32 (perturbation about an FP64 hydrostatic background; FP64 implicit "
+                 "column solve)",
         "data": "synthetic (capped dry-adiabat background with seeded theta noise)",
         "config": {"workload": "box3d_256x256x128_100m_weno5_lmars_x1_roe_x2x3_vic3_dt0.2",
                    "cells": [256, 256, 128], "dt_s": dt, "steps": K},
         # the reference's own counter (src/mesh/meshblock.cpp:938-956): cells
         # including ghosts x steps x stages / seconds
         "reference_convention_value": round(
-            cfg.shape[1] * cfg.shape[2] * cfg.shape[3] * K * 3 / r["t32"], 1),
+            cfg.shape[1] * cfg.shape[2] * cfg.shape[3] * K * 3 / r["tbg"], 1),
         "fp64_plain": {"value": round(cus64, 1), "ms_per_step": round(r["t64"] / K * 1e3, 5)},
         "fp32_fused_over_fp64_plain": round(cus32 / cus64, 3),
         "error_fp32_vs_fp64_after_steps": err,
         "mass_drift_fp32": mass_drift,
         "mass_drift_fp64": mass_drift64,
+        # the same fused step on the FP32 total state (no background)
+        "fp32_total_state": {"value": round(cus32_total, 1),
+                             "ms_per_step": round(r["t32"] / K * 1e3, 5),
+                             "fused_rhs_avg_ms": round(r["rhs_ms"], 5),
+                             "fused_column_avg_ms": round(r["col_ms"], 5),
+                             "error_fp32_vs_fp64_after_steps": err_tot,
+                             "mass_drift_fp32": mass_drift_tot},
         "finite": finite,
         "oracle_check": oracle_check,
         "adjoint_over_forward": adj["adjoint_over_forward"] if adj else None,
         "adjoint_note": "FP64 adjoint step (recomputes its 3 stages) / FP64 plain forward step "
                         "on the headline block",
         "adjoint": adj,
-        "fused_rhs_avg_ms": round(r["rhs_ms"], 5),
-        "fused_column_avg_ms": round(r["col_ms"], 5),
+        "fused_rhs_avg_ms": round(r["bg_rhs_ms"], 5),
+        "fused_column_avg_ms": round(r["bg_col_ms"], 5),
         "step_hbm_frac": round(cus32 * 160 / (HBM_PEAK_GBS * 1e9), 6),
         "roofline": roof,
         "cpu_baseline": cpu,

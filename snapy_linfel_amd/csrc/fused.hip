@@ -177,52 +177,95 @@ __device__ __forceinline__ void cons_to_prim(const T* U, T gammad, T* W) {
 
 // ---- perturbation path (hydrostatic background, bg_impl.hpp) ----
 
+// FP32 background tables as seen by one kernel: per x1 level (index = level -
+// lbase) and per x1 face (index = face - fbase; the three WENO offset rows
+// fstride apart).  Either the global tables (lbase = fbase = 0) or the slice
+// a tile copied to LDS.
+struct BgView {
+  const float *rho_c, *p_c, *gdr, *gdp, *res;
+  const float *rho_f, *p_f, *drl, *drr, *epl, *epr;
+  int lbase, fbase, fstride;
+};
+
+__device__ __forceinline__ BgView bg_global_view(const float* t, const BgLayout& B) {
+  BgView V;
+  V.rho_c = t + B.rho_c, V.p_c = t + B.p_c, V.gdr = t + B.gdr, V.gdp = t + B.gdp;
+  V.res = t + B.res;
+  V.rho_f = t + B.rho_f, V.p_f = t + B.p_f, V.drl = t + B.drl, V.drr = t + B.drr;
+  V.epl = t + B.epl, V.epr = t + B.epr;
+  V.lbase = 0, V.fbase = 0, V.fstride = B.F;
+  return V;
+}
+
+// Copies the tables of levels lbase .. lbase + nl - 1 and faces fbase ..
+// fbase + nf - 1 into LDS at s (5 nl + 10 nf floats; the caller
+// synchronises before any read) and returns the view of the copy.
+__device__ __forceinline__ BgView bg_stage_lds(const float* t, const BgLayout& B, int lbase,
+                                               int nl, int fbase, int nf, float* s) {
+  const int lsrc[5] = {B.rho_c, B.p_c, B.gdr, B.gdp, B.res};
+  for (int k = threadIdx.x; k < 5 * nl; k += blockDim.x) {
+    const int r = k / nl, l = k - r * nl;
+    s[k] = t[lsrc[r] + lbase + l];
+  }
+  const int fsrc[10] = {B.rho_f, B.p_f, B.drl, B.drl + B.F, B.drl + 2 * B.F,
+                        B.drr, B.drr + B.F, B.drr + 2 * B.F, B.epl, B.epr};
+  float* qf = s + 5 * nl;
+  for (int k = threadIdx.x; k < 10 * nf; k += blockDim.x) {
+    const int r = k / nf, f = k - r * nf;
+    qf[k] = t[fsrc[r] + fbase + f];
+  }
+  BgView V;
+  V.rho_c = s, V.p_c = s + nl, V.gdr = s + 2 * nl, V.gdp = s + 3 * nl, V.res = s + 4 * nl;
+  V.rho_f = qf, V.p_f = qf + nf, V.drl = qf + 2 * nf, V.drr = qf + 5 * nf;
+  V.epl = qf + 8 * nf, V.epr = qf + 9 * nf;
+  V.lbase = lbase, V.fbase = fbase, V.fstride = nf;
+  return V;
+}
+
 // Primitive perturbation (rho', v1, v2, v3, p') of a loaded cell U' (flips
-// applied) at the LDS position of interior-relative x1 index gi, and the
-// total density.  U' is relative to its source level; a ghost position
-// (mirror or edge source) adds the background difference of the source and
-// ghost levels.
-__device__ __forceinline__ void bg_load_prim(const FusedArgs& a, const BgLayout& B,
-                                             const float* U, int gi, float* W, float& rtot) {
-  const int lg = gi + a.ng;
-  const float* t = a.bgt;
-  W[IDN] = U[IDN] + t[B.gdr + lg];
-  rtot = t[B.rho_c + lg] + W[IDN];
+// applied) at the LDS position of x1 level lg, and the total density.  U' is
+// relative to its source level; a ghost position (mirror or edge source)
+// adds the background difference of the source and ghost levels.
+__device__ __forceinline__ void bg_load_prim(const FusedArgs& a, const BgView& V,
+                                             const float* U, int lg, float* W, float& rtot) {
+  const int l = lg - V.lbase;
+  W[IDN] = U[IDN] + V.gdr[l];
+  rtot = V.rho_c[l] + W[IDN];
   const float inv = 1.f / rtot;
   W[IVX] = U[IVX] * inv;
   W[IVY] = U[IVY] * inv;
   W[IVZ] = U[IVZ] * inv;
   const float ke = 0.5f * ((W[IVX] * U[IVX] + W[IVY] * U[IVY]) + W[IVZ] * U[IVZ]);
-  W[IPR] = (a.gammad - 1.f) * (U[IPR] - ke) + t[B.gdp + lg];
+  W[IPR] = (a.gammad - 1.f) * (U[IPR] - ke) + V.gdp[l];
 }
 
 // isentropic refill of the physical x1 ghost at interior-relative index gi
 // in FP64 from the total state of the boundary cell qb; writes the density
 // and pressure perturbations against the ghost level and the total density
-__device__ __forceinline__ void bg_refill_prim(const FusedArgs& a, const BgLayout& B,
-                                               const float* __restrict__ u, int ncell, int qb,
-                                               int gi, float* W, float& rtot) {
+__device__ __forceinline__ void bg_refill_prim(const FusedArgs& a, const float* __restrict__ u,
+                                               int ncell, int qb, int gi, float* W,
+                                               float& rtot) {
   const bool upper = gi >= a.nx1;
   const int lb = upper ? a.ng + a.nx1 - 1 : a.ng, lg = gi + a.ng;
   const double* d = a.bgd;
+  const int L = a.n1;
   const double g = (double)a.gammad, gm1 = g - 1.;
   const double rb = d[lb] + (double)u[IDN * ncell + qb];
   const double m1 = u[IVX * ncell + qb], m2 = u[IVY * ncell + qb], m3 = u[IVZ * ncell + qb];
-  const double eb = d[B.L + lb] / gm1 + (double)u[IPR * ncell + qb];
+  const double eb = d[L + lb] / gm1 + (double)u[IPR * ncell + qb];
   const double pb = gm1 * (eb - 0.5 * ((m1 * m1 + m2 * m2) + m3 * m3) / rb);
   double rho, pre;
   refill_layer64(rb, pb, g, a.bg_gdz, upper ? 1 : -1, upper ? gi - a.nx1 + 1 : -gi, rho, pre);
   W[IDN] = (float)(rho - d[lg]);
-  W[IPR] = (float)(pre - d[B.L + lg]);
+  W[IPR] = (float)(pre - d[L + lg]);
   rtot = (float)rho;
 }
 
 // Flux of an x2 / x3 face (perturbation form) from the perturbation states
 // wl, wr in the face frame: the reference is the background of level lv.
-__device__ __forceinline__ void bg_horizontal_flux(const float* t, const BgLayout& B, int lv,
-                                                   int solver, float* wl, float* wr, float gammad,
-                                                   float* flx) {
-  const float rb = t[B.rho_c + lv], pb = t[B.p_c + lv];
+__device__ __forceinline__ void bg_horizontal_flux(const BgView& V, int lv, int solver, float* wl,
+                                                   float* wr, float gammad, float* flx) {
+  const float rb = V.rho_c[lv - V.lbase], pb = V.p_c[lv - V.lbase];
   const float rl = wl[IDN], rr = wr[IDN], pl = wl[IPR], pr = wr[IPR];
   wl[IDN] += rb;
   wr[IDN] += rb;
@@ -238,9 +281,8 @@ __device__ __forceinline__ void bg_horizontal_flux(const float* t, const BgLayou
 // rev_lo / rev_hi: physical bottom / top face with gravity (the reference's
 // LR revision copies density and pressure across the face).
 __device__ __forceinline__ void bg_x1_face_flux(const float* ws, int HS, int base, int c, int fl,
-                                                const BgLayout& B, const float* t, bool rev_lo,
-                                                bool rev_hi, int solver, float gammad,
-                                                float* flx) {
+                                                const BgView& V, bool rev_lo, bool rev_hi,
+                                                int solver, float gammad, float* flx) {
   const float* rp = ws + IDN * HS + base;
   const float* rt = ws + 5 * HS + base;
   const float* pp = ws + IPR * HS + base;
@@ -250,16 +292,16 @@ __device__ __forceinline__ void bg_x1_face_flux(const float* ws, int HS, int bas
     tw[k] = rt[c - 3 + k];
     fw[k] = rp[c - 3 + k];
   }
-  const int F = B.F;
-  const float dl[3] = {t[B.drl + fl], t[B.drl + F + fl], t[B.drl + 2 * F + fl]};
-  const float dr[3] = {t[B.drr + fl], t[B.drr + F + fl], t[B.drr + 2 * F + fl]};
+  const int f = fl - V.fbase, F = V.fstride;
+  const float dl[3] = {V.drl[f], V.drl[F + f], V.drl[2 * F + f]};
+  const float dr[3] = {V.drr[f], V.drr[F + f], V.drr[2 * F + f]};
   float rl = weno5_off<true>(tw, fw, dl);
   float rr = weno5_off<false>(tw + 1, fw + 1, dr);
-  float pl = cp5_hi(pp[c - 3], pp[c - 2], pp[c - 1], pp[c], pp[c + 1]) + t[B.epl + fl];
-  float pr = cp5_lo(pp[c - 2], pp[c - 1], pp[c], pp[c + 1], pp[c + 2]) + t[B.epr + fl];
+  float pl = cp5_hi(pp[c - 3], pp[c - 2], pp[c - 1], pp[c], pp[c + 1]) + V.epl[f];
+  float pr = cp5_lo(pp[c - 2], pp[c - 1], pp[c], pp[c + 1], pp[c + 2]) + V.epr[f];
   if (rev_lo) rl = rr, pl = pr;
   if (rev_hi) rr = rl, pr = pl;
-  const float rf = t[B.rho_f + fl], pf = t[B.p_f + fl];
+  const float rf = V.rho_f[f], pf = V.p_f[f];
   float wl[5], wr[5];
   wl[IDN] = rf + rl;
   wr[IDN] = rf + rr;
@@ -369,8 +411,8 @@ __device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __r
   constexpr int NP = kBg ? 6 : 5;  // LDS planes: primitives (+ total density)
   float* ws = smem;           // primitives [5][H3][H2][H1] (+ total density)
   float* fs = smem + NP * H;  // face fluxes of one direction [5][faces]
-  const BgLayout BL(a.n1);
-  const float* bgt = a.bgt;
+  BgView BV;
+  if (kBg) BV = bg_global_view(a.bgt, BgLayout(a.n1));
   const int ncell = a.n1 * a.n2 * a.n3;
 
   int t1 = b % a.nt1;
@@ -405,12 +447,12 @@ __device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __r
     if (fl[2]) U[IVZ] = -U[IVZ];
     if (kBg) {
       float rtot;
-      bg_load_prim(a, BL, U, gi, W, rtot);
+      bg_load_prim(a, BV, U, gi + a.ng, W, rtot);
       if (refill && (gi < 0 || gi >= a.nx1) && (unsigned)(y - a.h2) < (unsigned)e2 &&
           (unsigned)(z - a.h3) < (unsigned)e3) {
         bool flb[3];
         const int qb = mapped_cell(a, gi >= a.nx1 ? a.nx1 - 1 : 0, gj, gk, flb);
-        bg_refill_prim(a, BL, u, ncell, qb, gi, W, rtot);
+        bg_refill_prim(a, u, ncell, qb, gi, W, rtot);
       }
       ws[5 * H + l] = rtot;
 #pragma unroll
@@ -522,8 +564,8 @@ __device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __r
         if (d == 0) {
           const int gi = o1 + fx;
           const bool rev = a.grav1 != 0.f;
-          bg_x1_face_flux(ws, H, base, c, lv, BL, bgt, rev && gi == 0, rev && gi == a.nx1,
-                          solver, gammad, flx);
+          bg_x1_face_flux(ws, H, base, c, lv, BV, rev && gi == 0, rev && gi == a.nx1, solver,
+                          gammad, flx);
         } else {
           float wl[5], wr[5];
 #pragma unroll
@@ -533,7 +575,7 @@ __device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __r
             wr[v] = interp_face<float>(tv, 0, scale, p, sd, c);
             wl[v] = interp_face<float>(tv, 1, scale, p, sd, c - 1);
           }
-          bg_horizontal_flux(bgt, BL, lv, solver, wl, wr, gammad, flx);
+          bg_horizontal_flux(BV, lv, solver, wl, wr, gammad, flx);
         }
 #pragma unroll
         for (int v = 0; v < 5; ++v) fs[vmap[v] * ftot + g] = flx[v];
@@ -699,7 +741,7 @@ __device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __r
     if (kBg) {
       // the background's flux divergence and gravity are the residual
       const int lv = o1 + cx + a.ng;
-      d[IVX] += dt * (rho * a.grav1 + bgt[BL.res + lv]);
+      d[IVX] += dt * (rho * a.grav1 + BV.res[lv]);
       rho = ws[5 * H + lin];
       d[IPR] += dt * rho * ws[IVX * H + lin] * a.grav1;
     } else if (a.grav1 != 0.f) {
@@ -825,8 +867,14 @@ __global__ void __launch_bounds__(kRhsThreads, 4)
   const int sol3 = P::r3 >= 0 ? P::r3 : a.riemann[2];
   const int nf0 = (e1 + 1) * e2, ftot = nf0 + e1 * (e2 + 1);
   float* fs = smem + 2 * NP * HS;  // x1 then x2 face fluxes of one slice [5][ftot]
-  const BgLayout BL(a.n1);
-  const float* bgt = a.bgt;
+  // perturbation path: the tables of the tile's H1 levels and tx1 + 1 x1
+  // faces, copied to LDS after the face fluxes
+  BgView BV;
+  if (kBg) {
+    BV = bg_stage_lds(a.bgt, BgLayout(a.n1), o1, H1, o1 + a.ng, a.tx1 + 1,
+                      fs + 5 * ((a.tx1 + 1) * a.tx2 + a.tx1 * (a.tx2 + 1)));
+    __syncthreads();
+  }
   const float area1 = a.dx3 * a.dx2, area2 = a.dx3 * a.dx1, area3 = a.dx2 * a.dx1;
   const float vol = a.dx3 * (a.dx2 * a.dx1);
   const float dt = a.dt;
@@ -861,12 +909,12 @@ __global__ void __launch_bounds__(kRhsThreads, 4)
       if (fl[2]) U[IVZ] = -U[IVZ];
       if (kBg) {
         float rtot;
-        bg_load_prim(a, BL, U, gi, W, rtot);
+        bg_load_prim(a, BV, U, gi + a.ng, W, rtot);
         if (refill && (gi < 0 || gi >= a.nx1) && (unsigned)(y - 3) < (unsigned)e2) {
           bool flb[3];
           const bool upper = gi >= a.nx1;
           const int qb = mapped_cell(a, upper ? a.nx1 - 1 : 0, gj, kk, flb);
-          bg_refill_prim(a, BL, u, ncell, qb, gi, W, rtot);
+          bg_refill_prim(a, u, ncell, qb, gi, W, rtot);
         }
         ws[5 * HS + l] = rtot;
       } else {
@@ -913,7 +961,7 @@ __global__ void __launch_bounds__(kRhsThreads, 4)
       if (kBg) {
         // reference: the background of the own column's level
         const int lv = o1 + cx + a.ng;
-        bg_horizontal_flux(bgt, BL, lv, sol3, wl, wr, gammad, flx);
+        bg_horizontal_flux(BV, lv, sol3, wl, wr, gammad, flx);
       } else {
         riemann_flux<float>(sol3, wl, wr, gammad, 3, flx, (float*)nullptr);
       }
@@ -932,7 +980,7 @@ __global__ void __launch_bounds__(kRhsThreads, 4)
           const int fy = dq1.div(g, fx);
           if (kBg) {
             const int gi = o1 + fx;
-            bg_x1_face_flux(ws, HS, (fy + 3) * H1, fx + 3, gi + a.ng, BL, bgt, refill && gi == 0,
+            bg_x1_face_flux(ws, HS, (fy + 3) * H1, fx + 3, gi + a.ng, BV, refill && gi == 0,
                             refill && gi == a.nx1, sol1, gammad, flx);
           } else {
             face_states<0, P::ty0, P::ty20>(ws, HS, (fy + 3) * H1, 1, fx + 3, type0, ty20,
@@ -963,7 +1011,7 @@ __global__ void __launch_bounds__(kRhsThreads, 4)
           face_states<1, P::ty1, P::ty21>(ws, HS, fx + 3, H1, fy + 3, type1, ty21, scale1, wl,
                                          wr);
           if (kBg)
-            bg_horizontal_flux(bgt, BL, o1 + fx + a.ng, sol2, wl, wr, gammad, flx);
+            bg_horizontal_flux(BV, o1 + fx + a.ng, sol2, wl, wr, gammad, flx);
           else
             riemann_flux<float>(sol2, wl, wr, gammad, 3, flx, (float*)nullptr);
           fs[IDN * ftot + g] = flx[0];
@@ -1005,8 +1053,8 @@ __global__ void __launch_bounds__(kRhsThreads, 4)
       if (kBg) {
         // the background's flux divergence and gravity are the residual
         const int lv = o1 + cx + a.ng;
-        d5[IVX] += dt * (rho * a.grav1 + bgt[BL.res + lv]);
-        rho += bgt[BL.rho_c + lv];
+        d5[IVX] += dt * (rho * a.grav1 + BV.res[lv - BV.lbase]);
+        rho += BV.rho_c[lv - BV.lbase];
         d5[IPR] += dt * rho * win[IVX][2] * a.grav1;
       } else if (a.grav1 != 0.f) {
         d5[IVX] += dt * rho * a.grav1;
@@ -2114,7 +2162,10 @@ static bool march_ok(const snapx_config* cfg) {
 static size_t march_smem_bytes(const FusedArgs& a) {
   size_t HS = (size_t)(a.tx1 + 6) * (a.tx2 + 6);
   size_t ftot = (size_t)(a.tx1 + 1) * a.tx2 + (size_t)a.tx1 * (a.tx2 + 1);
-  return sizeof(float) * ((a.bgt ? 12 : 10) * HS + 5 * ftot);
+  // perturbation path: a sixth plane per slice and the tile's background
+  // tables (5 per level, 10 per x1 face)
+  const size_t bgt = a.bgt ? 2 * HS + 5 * (size_t)(a.tx1 + 6) + 10 * (size_t)(a.tx1 + 1) : 0;
+  return sizeof(float) * (10 * HS + 5 * ftot + bgt);
 }
 
 static int make_march_args(const snapx_config* cfg, double dt, FusedArgs* out) {
