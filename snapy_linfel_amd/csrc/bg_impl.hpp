@@ -28,17 +28,27 @@
 
 namespace snapx {
 
+constexpr int kBgLevelRows = 10;
+constexpr int kBgFaceRows = 10;
+
 struct BgLayout {
   int L, F;
   // FP32 offsets (floats) of the per-level and per-face arrays
   int rho_c, p_c, gdr, gdp, res;  // [L]
+  // ghost refill of the physical x1 ghosts (rows at the ghost levels):
+  // cn = n (+-1) a g dz, qk = qbar^(gamma / (gamma - 1)),
+  // qk2 = qbar^(1 / (gamma - 1)) with qbar = 1 + cn rho_b / p_b of the
+  // background's boundary cell, and the refilled background minus the
+  // background at the ghost level, dpg (pressure) and drg (density)
+  int cn, qk, qk2, dpg, drg;      // [L]
   int rho_f, p_f;                 // [F]
   int drl, drr;                   // WENO5 density offsets, [3][F] each
   int epl, epr;                   // CP5 pressure offsets, [F] each
   int nfloat;
   __host__ __device__ explicit BgLayout(int n1) : L(n1), F(n1 + 1) {
     rho_c = 0, p_c = L, gdr = 2 * L, gdp = 3 * L, res = 4 * L;
-    rho_f = 5 * L, p_f = rho_f + F;
+    cn = 5 * L, qk = 6 * L, qk2 = 7 * L, dpg = 8 * L, drg = 9 * L;
+    rho_f = kBgLevelRows * L, p_f = rho_f + F;
     drl = p_f + F, drr = drl + 3 * F;
     epl = drr + 3 * F, epr = epl + F;
     nfloat = epr + F;
@@ -91,17 +101,26 @@ __device__ __forceinline__ float weno5_off(const float* t, const float* f, const
   return ((a0 * (p[0] + d[0]) + a1 * (p[1] + d[1])) + a2 * (p[2] + d[2])) / ((a0 + a1) + a2);
 }
 
-// Isentropic refill (closed form of IsentropicRefill) in FP64: ghost layer n
-// of the boundary state (rho_b, p_b); gdz = -grav1 dx1, dir -1 below, +1 above
-__device__ __forceinline__ void refill_layer64(double rho_b, double p_b, double gamma, double gdz,
-                                               int dir, int n, double& rho, double& p) {
-  const double gm = gamma - 1.;
-  const double inv_P = 1. / pow(p_b, 1. / gamma);
-  const double step = (dir < 0 ? 1. : -1.) * (gm / gamma) * gdz * rho_b;
-  const double h = (p_b + (double)n * step) * inv_P;
-  const double H = pow(h, 1. / gm);
-  p = h * H;
-  rho = rho_b * H * inv_P;
+// Isentropic refill of ghost layer n (closed form of IsentropicRefill):
+// with q = 1 + cn rho_b / p_b, p_g = p_b q^k and rho_g = rho_b q^k2,
+// k = gamma / (gamma - 1), k2 = 1 / (gamma - 1).  In perturbation form, for
+// a boundary cell (rho_bar + rho', p_bar + p'): q - qbar = cn (rho' p_bar -
+// rho_bar p') / (p p_bar) has no cancellation, q^k - qbar^k = qbar^k
+// expm1(k log1p((q - qbar) / qbar)), and
+//   p_g - p_bar_g = p' q^k + p_bar (q^k - qbar^k) + (p_bar qbar^k - p_bar_g)
+// (the last term from the tables, dpg), likewise the density.  FP32
+// throughout, accurate to the perturbations' own rounding.
+__device__ __forceinline__ void bg_refill_pert(float rbb, float pbb, float rp, float pp, float cn,
+                                               float qk, float qk2, float dpg, float drg,
+                                               float gammad, float& rg, float& pg) {
+  const float p = pbb + pp;
+  const float dq = cn * (rp * pbb - rbb * pp) / (p * pbb);
+  const float qbar = 1.f + cn * rbb / pbb;
+  const float L = log1pf(dq / qbar);
+  const float gm1 = gammad - 1.f;
+  const float ek = expm1f(gammad / gm1 * L), ek2 = expm1f(L / gm1);
+  pg = pp * (qk * (1.f + ek)) + pbb * (qk * ek) + dpg;
+  rg = rp * (qk2 * (1.f + ek2)) + rbb * (qk2 * ek2) + drg;
 }
 
 // host: x1 source level of ghost level l through the x1 boundary functions
@@ -138,6 +157,23 @@ inline void bg_build_tables(const snapx_config* c, const double* rb, const doubl
     const int s = bg_source_level(l, ng, nx1, c->bc[0], c->bc[1]);
     f32[B.gdr + l] = (float)(rb[s] - rb[l]);
     f32[B.gdp + l] = (float)(pb[s] - pb[l]);
+    // physical x1 ghosts: refill of the background's boundary cell, layer
+    // n = 1, 2, 3 away from it (without gravity a copy of the boundary cell,
+    // as the column solve takes it)
+    const bool lower = l < ng, upper = l >= ng + nx1;
+    if (lower || upper) {
+      const int lb = lower ? ng : ng + nx1 - 1;
+      const int n = lower ? ng - l : l - (ng + nx1 - 1);
+      const double gam = c->gammad, gm = gam - 1.;
+      const double cn = n * (lower ? 1. : -1.) * (gm / gam) * (-c->grav1 * dx1);
+      const double q = 1. + cn * rb[lb] / pb[lb];
+      const double qk = std::pow(q, gam / gm), qk2 = std::pow(q, 1. / gm);
+      f32[B.cn + l] = (float)cn;
+      f32[B.qk + l] = (float)qk;
+      f32[B.qk2 + l] = (float)qk2;
+      f32[B.dpg + l] = (float)(pb[lb] * qk - pb[l]);
+      f32[B.drg + l] = (float)(rb[lb] * qk2 - rb[l]);
+    }
   }
   for (int f = ng; f <= ng + nx1; ++f) {
     // windows of the cells below (f - 1, upper face) and above (f, lower face)

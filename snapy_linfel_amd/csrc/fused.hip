@@ -108,7 +108,6 @@ struct FusedArgs {
   // tables of bg_impl.hpp
   const float* bgt;
   const double* bgd;
-  double bg_gdz;  // -grav1 dx1 in FP64 for the FP64 ghost refill
 };
 
 // Maps an interior-relative index x (may lie in the ghost range) of an axis
@@ -177,49 +176,42 @@ __device__ __forceinline__ void cons_to_prim(const T* U, T gammad, T* W) {
 
 // ---- perturbation path (hydrostatic background, bg_impl.hpp) ----
 
-// FP32 background tables as seen by one kernel: per x1 level (index = level -
-// lbase) and per x1 face (index = face - fbase; the three WENO offset rows
-// fstride apart).  Either the global tables (lbase = fbase = 0) or the slice
-// a tile copied to LDS.
+// FP32 background tables as seen by one kernel, in the layout of BgLayout:
+// five rows over nl levels (rho_c, p_c, gdr, gdp, res; index = level -
+// lbase), then ten rows over nf faces (rho_f, p_f, three WENO offsets below,
+// three above, the two CP5 offsets; index = face - fbase).  Either the
+// global tables (lbase = fbase = 0) or the slice a tile copied to LDS.
 struct BgView {
-  const float *rho_c, *p_c, *gdr, *gdp, *res;
-  const float *rho_f, *p_f, *drl, *drr, *epl, *epr;
-  int lbase, fbase, fstride;
+  const float* t;
+  int nl, nf, lbase, fbase;
+  __device__ __forceinline__ float lv(int row, int level) const {
+    return t[row * nl + level - lbase];
+  }
+  __device__ __forceinline__ float fc(int row, int face) const {
+    return t[kBgLevelRows * nl + row * nf + face - fbase];
+  }
 };
+enum { BG_RHO_C = 0, BG_P_C, BG_GDR, BG_GDP, BG_RES, BG_CN, BG_QK, BG_QK2, BG_DPG, BG_DRG };
+enum { BG_RHO_F = 0, BG_P_F, BG_DRL, BG_DRR = 5, BG_EPL = 8, BG_EPR };
 
 __device__ __forceinline__ BgView bg_global_view(const float* t, const BgLayout& B) {
-  BgView V;
-  V.rho_c = t + B.rho_c, V.p_c = t + B.p_c, V.gdr = t + B.gdr, V.gdp = t + B.gdp;
-  V.res = t + B.res;
-  V.rho_f = t + B.rho_f, V.p_f = t + B.p_f, V.drl = t + B.drl, V.drr = t + B.drr;
-  V.epl = t + B.epl, V.epr = t + B.epr;
-  V.lbase = 0, V.fbase = 0, V.fstride = B.F;
-  return V;
+  return BgView{t, B.L, B.F, 0, 0};
 }
 
 // Copies the tables of levels lbase .. lbase + nl - 1 and faces fbase ..
-// fbase + nf - 1 into LDS at s (5 nl + 10 nf floats; the caller
-// synchronises before any read) and returns the view of the copy.
+// fbase + nf - 1 into LDS at s (kBgLevelRows nl + kBgFaceRows nf floats;
+// the caller synchronises before any read) and returns the view of the copy.
 __device__ __forceinline__ BgView bg_stage_lds(const float* t, const BgLayout& B, int lbase,
                                                int nl, int fbase, int nf, float* s) {
-  const int lsrc[5] = {B.rho_c, B.p_c, B.gdr, B.gdp, B.res};
-  for (int k = threadIdx.x; k < 5 * nl; k += blockDim.x) {
+  for (int k = threadIdx.x; k < kBgLevelRows * nl; k += blockDim.x) {
     const int r = k / nl, l = k - r * nl;
-    s[k] = t[lsrc[r] + lbase + l];
+    s[k] = t[r * B.L + lbase + l];
   }
-  const int fsrc[10] = {B.rho_f, B.p_f, B.drl, B.drl + B.F, B.drl + 2 * B.F,
-                        B.drr, B.drr + B.F, B.drr + 2 * B.F, B.epl, B.epr};
-  float* qf = s + 5 * nl;
-  for (int k = threadIdx.x; k < 10 * nf; k += blockDim.x) {
+  for (int k = threadIdx.x; k < kBgFaceRows * nf; k += blockDim.x) {
     const int r = k / nf, f = k - r * nf;
-    qf[k] = t[fsrc[r] + fbase + f];
+    s[kBgLevelRows * nl + k] = t[kBgLevelRows * B.L + r * B.F + fbase + f];
   }
-  BgView V;
-  V.rho_c = s, V.p_c = s + nl, V.gdr = s + 2 * nl, V.gdp = s + 3 * nl, V.res = s + 4 * nl;
-  V.rho_f = qf, V.p_f = qf + nf, V.drl = qf + 2 * nf, V.drr = qf + 5 * nf;
-  V.epl = qf + 8 * nf, V.epr = qf + 9 * nf;
-  V.lbase = lbase, V.fbase = fbase, V.fstride = nf;
-  return V;
+  return BgView{s, nl, nf, lbase, fbase};
 }
 
 // Primitive perturbation (rho', v1, v2, v3, p') of a loaded cell U' (flips
@@ -228,44 +220,56 @@ __device__ __forceinline__ BgView bg_stage_lds(const float* t, const BgLayout& B
 // adds the background difference of the source and ghost levels.
 __device__ __forceinline__ void bg_load_prim(const FusedArgs& a, const BgView& V,
                                              const float* U, int lg, float* W, float& rtot) {
-  const int l = lg - V.lbase;
-  W[IDN] = U[IDN] + V.gdr[l];
-  rtot = V.rho_c[l] + W[IDN];
+  W[IDN] = U[IDN] + V.lv(BG_GDR, lg);
+  rtot = V.lv(BG_RHO_C, lg) + W[IDN];
   const float inv = 1.f / rtot;
   W[IVX] = U[IVX] * inv;
   W[IVY] = U[IVY] * inv;
   W[IVZ] = U[IVZ] * inv;
   const float ke = 0.5f * ((W[IVX] * U[IVX] + W[IVY] * U[IVY]) + W[IVZ] * U[IVZ]);
-  W[IPR] = (a.gammad - 1.f) * (U[IPR] - ke) + V.gdp[l];
+  W[IPR] = (a.gammad - 1.f) * (U[IPR] - ke) + V.lv(BG_GDP, lg);
 }
 
-// isentropic refill of the physical x1 ghost at interior-relative index gi
-// in FP64 from the total state of the boundary cell qb; writes the density
-// and pressure perturbations against the ghost level and the total density
-__device__ __forceinline__ void bg_refill_prim(const FusedArgs& a, const float* __restrict__ u,
-                                               int ncell, int qb, int gi, float* W,
-                                               float& rtot) {
+// Density and pressure perturbations of the boundary cell U' (conserved
+// perturbation of an interior cell; the velocities need the total density)
+__device__ __forceinline__ void bg_cell_rho_p(const BgView& V, const float* U, int lb,
+                                              float gammad, float& rp, float& pp) {
+  rp = U[IDN];
+  const float rt = V.lv(BG_RHO_C, lb) + rp;
+  const float ke = 0.5f * ((U[IVX] * U[IVX] + U[IVY] * U[IVY]) + U[IVZ] * U[IVZ]) / rt;
+  pp = (gammad - 1.f) * (U[IPR] - ke);
+}
+
+// isentropic refill of the physical x1 ghost at x1 level lg from the
+// boundary cell U' of level lb (bg_refill_pert): density and pressure
+// perturbations against the ghost level
+__device__ __forceinline__ void bg_refill_ghost(const BgView& V, const float* U, int lb, int lg,
+                                                float gammad, float& rg, float& pg) {
+  float rp, pp;
+  bg_cell_rho_p(V, U, lb, gammad, rp, pp);
+  bg_refill_pert(V.lv(BG_RHO_C, lb), V.lv(BG_P_C, lb), rp, pp, V.lv(BG_CN, lg), V.lv(BG_QK, lg),
+                 V.lv(BG_QK2, lg), V.lv(BG_DPG, lg), V.lv(BG_DRG, lg), gammad, rg, pg);
+}
+
+// the refill of the tile's own x1 ghost at interior-relative index gi from
+// the boundary cell qb, into W and the total density
+__device__ __forceinline__ void bg_refill_prim(const FusedArgs& a, const BgView& V,
+                                               const float* __restrict__ u, int ncell, int qb,
+                                               int gi, float* W, float& rtot) {
   const bool upper = gi >= a.nx1;
   const int lb = upper ? a.ng + a.nx1 - 1 : a.ng, lg = gi + a.ng;
-  const double* d = a.bgd;
-  const int L = a.n1;
-  const double g = (double)a.gammad, gm1 = g - 1.;
-  const double rb = d[lb] + (double)u[IDN * ncell + qb];
-  const double m1 = u[IVX * ncell + qb], m2 = u[IVY * ncell + qb], m3 = u[IVZ * ncell + qb];
-  const double eb = d[L + lb] / gm1 + (double)u[IPR * ncell + qb];
-  const double pb = gm1 * (eb - 0.5 * ((m1 * m1 + m2 * m2) + m3 * m3) / rb);
-  double rho, pre;
-  refill_layer64(rb, pb, g, a.bg_gdz, upper ? 1 : -1, upper ? gi - a.nx1 + 1 : -gi, rho, pre);
-  W[IDN] = (float)(rho - d[lg]);
-  W[IPR] = (float)(pre - d[L + lg]);
-  rtot = (float)rho;
+  float U[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) U[v] = u[v * ncell + qb];
+  bg_refill_ghost(V, U, lb, lg, a.gammad, W[IDN], W[IPR]);
+  rtot = V.lv(BG_RHO_C, lg) + W[IDN];
 }
 
 // Flux of an x2 / x3 face (perturbation form) from the perturbation states
 // wl, wr in the face frame: the reference is the background of level lv.
 __device__ __forceinline__ void bg_horizontal_flux(const BgView& V, int lv, int solver, float* wl,
                                                    float* wr, float gammad, float* flx) {
-  const float rb = V.rho_c[lv - V.lbase], pb = V.p_c[lv - V.lbase];
+  const float rb = V.lv(BG_RHO_C, lv), pb = V.lv(BG_P_C, lv);
   const float rl = wl[IDN], rr = wr[IDN], pl = wl[IPR], pr = wr[IPR];
   wl[IDN] += rb;
   wr[IDN] += rb;
@@ -292,16 +296,15 @@ __device__ __forceinline__ void bg_x1_face_flux(const float* ws, int HS, int bas
     tw[k] = rt[c - 3 + k];
     fw[k] = rp[c - 3 + k];
   }
-  const int f = fl - V.fbase, F = V.fstride;
-  const float dl[3] = {V.drl[f], V.drl[F + f], V.drl[2 * F + f]};
-  const float dr[3] = {V.drr[f], V.drr[F + f], V.drr[2 * F + f]};
+  const float dl[3] = {V.fc(BG_DRL, fl), V.fc(BG_DRL + 1, fl), V.fc(BG_DRL + 2, fl)};
+  const float dr[3] = {V.fc(BG_DRR, fl), V.fc(BG_DRR + 1, fl), V.fc(BG_DRR + 2, fl)};
   float rl = weno5_off<true>(tw, fw, dl);
   float rr = weno5_off<false>(tw + 1, fw + 1, dr);
-  float pl = cp5_hi(pp[c - 3], pp[c - 2], pp[c - 1], pp[c], pp[c + 1]) + V.epl[f];
-  float pr = cp5_lo(pp[c - 2], pp[c - 1], pp[c], pp[c + 1], pp[c + 2]) + V.epr[f];
+  float pl = cp5_hi(pp[c - 3], pp[c - 2], pp[c - 1], pp[c], pp[c + 1]) + V.fc(BG_EPL, fl);
+  float pr = cp5_lo(pp[c - 2], pp[c - 1], pp[c], pp[c + 1], pp[c + 2]) + V.fc(BG_EPR, fl);
   if (rev_lo) rl = rr, pl = pr;
   if (rev_hi) rr = rl, pr = pl;
-  const float rf = V.rho_f[f], pf = V.p_f[f];
+  const float rf = V.fc(BG_RHO_F, fl), pf = V.fc(BG_P_F, fl);
   float wl[5], wr[5];
   wl[IDN] = rf + rl;
   wr[IDN] = rf + rr;
@@ -452,7 +455,7 @@ __device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __r
           (unsigned)(z - a.h3) < (unsigned)e3) {
         bool flb[3];
         const int qb = mapped_cell(a, gi >= a.nx1 ? a.nx1 - 1 : 0, gj, gk, flb);
-        bg_refill_prim(a, u, ncell, qb, gi, W, rtot);
+        bg_refill_prim(a, BV, u, ncell, qb, gi, W, rtot);
       }
       ws[5 * H + l] = rtot;
 #pragma unroll
@@ -741,7 +744,7 @@ __device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __r
     if (kBg) {
       // the background's flux divergence and gravity are the residual
       const int lv = o1 + cx + a.ng;
-      d[IVX] += dt * (rho * a.grav1 + BV.res[lv]);
+      d[IVX] += dt * (rho * a.grav1 + BV.lv(BG_RES, lv));
       rho = ws[5 * H + lin];
       d[IPR] += dt * rho * ws[IVX * H + lin] * a.grav1;
     } else if (a.grav1 != 0.f) {
@@ -914,7 +917,7 @@ __global__ void __launch_bounds__(kRhsThreads, 4)
           bool flb[3];
           const bool upper = gi >= a.nx1;
           const int qb = mapped_cell(a, upper ? a.nx1 - 1 : 0, gj, kk, flb);
-          bg_refill_prim(a, u, ncell, qb, gi, W, rtot);
+          bg_refill_prim(a, BV, u, ncell, qb, gi, W, rtot);
         }
         ws[5 * HS + l] = rtot;
       } else {
@@ -1053,8 +1056,8 @@ __global__ void __launch_bounds__(kRhsThreads, 4)
       if (kBg) {
         // the background's flux divergence and gravity are the residual
         const int lv = o1 + cx + a.ng;
-        d5[IVX] += dt * (rho * a.grav1 + BV.res[lv - BV.lbase]);
-        rho += BV.rho_c[lv - BV.lbase];
+        d5[IVX] += dt * (rho * a.grav1 + BV.lv(BG_RES, lv));
+        rho += BV.lv(BG_RHO_C, lv);
         d5[IPR] += dt * rho * win[IVX][2] * a.grav1;
       } else if (a.grav1 != 0.f) {
         d5[IVX] += dt * rho * a.grav1;
@@ -1092,8 +1095,9 @@ struct ColArgs {
   double zin1, zout1;
   Geometry geo;
   // hydrostatic background of the perturbation path (FP64 rho_b, p_b per x1
-  // level, bg_impl.hpp), null for the total state
+  // level and the FP32 tables of bg_impl.hpp), null for the total state
   const double* bgd;
+  const float* bgt;
 };
 
 // primitive state (FP64) of interior-relative layer i of a column whose
@@ -1140,11 +1144,18 @@ __device__ __forceinline__ void float_prim_at(const float* Uf, bool flip, const 
 // arithmetic of the refill in the RHS kernel, which computes the fluxes of the
 // same stage from these ghost values (FP64 on the perturbation path, as
 // there).
+// (on the perturbation path Ub is the boundary cell's FP32 conserved
+// perturbation, refilled as in the RHS kernel)
 template <bool kSph = false, bool kBg = false>
 __device__ __forceinline__ void refill_ghost(const double* wb, double* wg, const ColArgs& a,
-                                             int dir) {
+                                             int dir, const float* Ub = nullptr) {
   if (kBg) {
-    refill_layer64(wb[IDN], wb[IPR], a.gammad, -a.grav1 * a.dx1, dir, 1, wg[IDN], wg[IPR]);
+    const BgView V = bg_global_view(a.bgt, BgLayout(a.n1));
+    const int lb = dir < 0 ? a.ng : a.ng + a.nl - 1, lg = dir < 0 ? a.ng - 1 : a.ng + a.nl;
+    float rg, pg;
+    bg_refill_ghost(V, Ub, lb, lg, (float)a.gammad, rg, pg);
+    wg[IDN] = a.bgd[lg] + (double)rg;
+    wg[IPR] = a.bgd[a.n1 + lg] + (double)pg;
     return;
   }
   float rho, pre;
@@ -1552,11 +1563,11 @@ __device__ __forceinline__ void column_pcr2(float* __restrict__ u, float* __rest
         for (int v = 0; v < 5; ++v) wm0[v] = sh[(5 + v) * T + t - 1];
       } else {
         float_prim_at<kBg>(Um, flm, a, lm, wm0);
-        refill_ghost<false, kBg>(w0, wm0, a, -1);
+        refill_ghost<false, kBg>(w0, wm0, a, -1, Uc0);
       }
       if (top0) {
         float_prim_at<kBg>(Up, flp, a, lp, wp0);
-        refill_ghost<false, kBg>(w0, wp0, a, +1);
+        refill_ghost<false, kBg>(w0, wp0, a, +1, Uc0);
       }
     }
     if (act1) {
@@ -1565,7 +1576,7 @@ __device__ __forceinline__ void column_pcr2(float* __restrict__ u, float* __rest
         for (int v = 0; v < 5; ++v) wp1[v] = sh[v * T + t + 1];
       } else {
         float_prim_at<kBg>(Up, flp, a, lp, wp1);
-        refill_ghost<false, kBg>(w1, wp1, a, +1);
+        refill_ghost<false, kBg>(w1, wp1, a, +1, Uc1);
       }
     }
     column_barrier(wave);
@@ -1746,14 +1757,14 @@ __device__ __forceinline__ void column_pcr(float* __restrict__ u, float* __restr
         for (int v = 0; v < 5; ++v) wm[v] = sh[v * T + t - 1];
       } else {
         float_prim_at<kBg>(Um, flm, a, lm, wm);
-        refill_ghost<kSph, kBg>(wc, wm, a, -1);
+        refill_ghost<kSph, kBg>(wc, wm, a, -1, Uc);
       }
       if (i < nl - 1) {
 #pragma unroll
         for (int v = 0; v < 5; ++v) wp[v] = sh[v * T + t + 1];
       } else {
         float_prim_at<kBg>(Up, flp, a, lp, wp);
-        refill_ghost<kSph, kBg>(wc, wp, a, +1);
+        refill_ghost<kSph, kBg>(wc, wp, a, +1, Uc);
       }
     }
     column_barrier(wave);
@@ -2125,7 +2136,6 @@ static FusedArgs fill_fused_args(const snapx_config* cfg, Tile t, double dt) {
   a.geo = make_geometry(cfg);
   a.bgt = nullptr;
   a.bgd = nullptr;
-  a.bg_gdz = -cfg->grav1 * g.dx1;
   const bool sph = cfg->coord == SNAPX_COORD_SPHERICAL_POLAR;
   // ghost-layer distances of the refill: dx1v of the layers crossed
   double zi = 0., zo = 0.;
@@ -2164,7 +2174,9 @@ static size_t march_smem_bytes(const FusedArgs& a) {
   size_t ftot = (size_t)(a.tx1 + 1) * a.tx2 + (size_t)a.tx1 * (a.tx2 + 1);
   // perturbation path: a sixth plane per slice and the tile's background
   // tables (5 per level, 10 per x1 face)
-  const size_t bgt = a.bgt ? 2 * HS + 5 * (size_t)(a.tx1 + 6) + 10 * (size_t)(a.tx1 + 1) : 0;
+  const size_t bgt = a.bgt ? 2 * HS + kBgLevelRows * (size_t)(a.tx1 + 6) +
+                                 kBgFaceRows * (size_t)(a.tx1 + 1)
+                           : 0;
   return sizeof(float) * (10 * HS + 5 * ftot + bgt);
 }
 
@@ -2234,6 +2246,7 @@ static int make_col_args(const snapx_config* cfg, double dt, ColArgs* out) {
   while (c.lanes < c.nl) c.lanes <<= 1, c.lshift += 1;
   c.geo = make_geometry(cfg);
   c.bgd = nullptr;
+  c.bgt = nullptr;
   const bool sph = cfg->coord == SNAPX_COORD_SPHERICAL_POLAR;
   c.zin1 = sph ? sph_dx1v(c.geo, g.ng - 1) : g.dx1;
   c.zout1 = sph ? sph_dx1v(c.geo, g.n1 - g.ng - 1) : g.dx1;
@@ -2318,6 +2331,7 @@ static int fused_step(const snapx_config* cfg, double dt, float* u, void* worksp
   ColArgs c;
   if (int e = make_col_args(cfg, dt, &c)) return e;
   c.bgd = a.bgd;
+  c.bgt = a.bgt;
   int N = ((cfg->implicit_scheme >> 3) & 1) ? 5 : 3;
   int cblk = (c.ncol + kColThreads - 1) / kColThreads;
 
