@@ -846,7 +846,11 @@ __global__ void __launch_bounds__(kRhsThreads, 4)
   constexpr int NP = kBg ? 6 : 5;  // LDS planes per slice
   const int H1 = a.tx1 + 6, H2 = a.tx2 + 6, HS = H1 * H2;
   const int ncell = a.n1 * a.n2 * a.n3;
-  const int b = blockIdx.x;
+  // workgroups go to the eight XCDs round robin (workgroup i to XCD i mod
+  // 8); each XCD takes a contiguous run of tiles, so that the halos shared
+  // by neighbouring tiles meet in the same L2
+  const int nb = gridDim.x, xq = nb >> 3, xr = nb & 7, xcd = blockIdx.x & 7;
+  const int b = xcd * xq + min(xcd, xr) + (blockIdx.x >> 3);
   const int t1 = b % a.nt1, t2 = (b / a.nt1) % a.nt2, t3 = b / (a.nt1 * a.nt2);
   const int o1 = t1 * a.tx1, o2 = t2 * a.tx2, o3 = t3 * a.tx3;
   const int e1 = min(a.tx1, a.nx1 - o1), e2 = min(a.tx2, a.nx2 - o2), ez = min(a.tx3, a.nx3 - o3);
