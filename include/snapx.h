@@ -222,9 +222,19 @@ int snapx_rk3_step(const snapx_config* cfg, double dt, void* u, void* w,
                    void* workspace, size_t workspace_bytes, int dtype,
                    void* stream);
 
+/* snapx_rk3_step with the time step read from device memory (*dt, a
+   double) by the kernels that use it, so that a CFL step computed on the
+   device (snapx_max_time_step) needs no host round trip.  Same results as
+   snapx_rk3_step with that dt. */
+int snapx_rk3_step_dt(const snapx_config* cfg, const double* dt, void* u,
+                      void* w, void* workspace, size_t workspace_bytes,
+                      int dtype, void* stream);
+
 /* HydroImpl::max_time_step (src/hydro/hydro.cpp:118-180) times cfl
    (src/mesh/meshblock.cpp:511-530): writes the CFL time step to the device
-   double *dt_out (no host synchronisation). */
+   double *dt_out (no host synchronisation).  A NaN candidate (a negative
+   pressure or density in w) gives NaN, as the reference's tensor minimum
+   does. */
 int snapx_max_time_step(const snapx_config* cfg, const void* w, double cfl,
                         double* dt_out, int dtype, void* stream);
 
@@ -257,6 +267,12 @@ size_t snapx_fused_workspace_bytes(const snapx_config* cfg);
 int snapx_fused_rk3_step(const snapx_config* cfg, double dt, float* u,
                          void* workspace, size_t workspace_bytes,
                          void* stream);
+
+/* snapx_fused_rk3_step with the time step read from device memory (*dt, a
+   double, rounded to FP32 where the FP32 kernels use it, as the host dt is). */
+int snapx_fused_rk3_step_dt(const snapx_config* cfg, const double* dt,
+                            float* u, void* workspace, size_t workspace_bytes,
+                            void* stream);
 
 /* Per-kernel device time accumulated by snapx_fused_rk3_steps. */
 typedef struct snapx_timing {
@@ -380,6 +396,13 @@ int snapx_batch_rk3_steps(const snapx_config* members,
                           void* workspace, size_t workspace_bytes,
                           snapx_timing* timing, void* stream);
 
+/* Replaces the planned dt of every member with dt[m] read from device memory
+   (one double per member) by a kernel on the stream: the batched steps (and
+   graphs of them) enqueued after it use the new time steps; info->dt keeps
+   the planned value. */
+int snapx_batch_set_dt(const snapx_batch_info* info, const double* dt,
+                       void* workspace, size_t workspace_bytes, void* stream);
+
 /* Tile autotuner of the batched path (no reference counterpart).  Times
    `trials` launches of the RHS kernel (it reads u, writes only workspace) for
    every candidate tile that fits the members, then plans the batch with the
@@ -388,6 +411,76 @@ int snapx_batch_autotune(const snapx_config* members, int nmembers, double dt,
                          const float* u, void* workspace, size_t workspace_bytes,
                          int trials, void* stream, snapx_batch_info* info,
                          double* best_ms);
+
+/* ---------------------------------------------------------------------- */
+/* CFL time loop with redo on the device (examples/shock.cpp:59-79 around  */
+/* MeshBlockImpl::max_time_step and check_redo,                            */
+/* src/mesh/meshblock.cpp:511-530,974-1028)                                */
+/* ---------------------------------------------------------------------- */
+
+/* Loop state in device memory.  The caller fills the settings, zeroes the
+   rest and copies the record to the device; each attempt of the loop then
+   runs as kernels that read and update it, so any number of attempts is
+   enqueued without a host synchronisation and can be captured in one HIP
+   graph (no memset nodes).  One attempt is one pass of the reference loop:
+     stop when status != 0, nlim >= 0 && cycle >= nlim, or time >= tlim;
+     ++cycle;  dt = 2^-redo * cfl * HydroImpl::max_time_step(w);
+     one SSP-RK3 step with dt;
+     verdict = snapx_check_state(u);
+     if (verdict & redo_mask): ++redo; if redo > max_redo: status = -1 (u
+       keeps the failed step, as the reference returns before restoring);
+       else u = the step's input, --cycle;
+     else: redo = 0, time += dt.
+   Attempts after the loop has stopped change nothing (their step is
+   discarded: u and w are restored). */
+typedef struct snapx_step_control {
+  /* settings */
+  double cfl;        /* Courant number */
+  double tlim;       /* the loop stops once time >= tlim */
+  int64_t nlim;      /* ... or once nlim >= 0 and cycle >= nlim */
+  int32_t max_redo;  /* more redos in a row than this end the loop */
+  int32_t redo_mask; /* verdict bits that ask for a redo; the reference's:
+                        SNAPX_REDO_DENSITY | SNAPX_REDO_PRESSURE */
+  /* loop state, updated on the device */
+  double time;       /* current time */
+  double dt;         /* time step of the latest attempt */
+  double dt_raw;     /* HydroImpl::max_time_step of the latest attempt */
+  int64_t cycle;     /* MeshBlock::cycle */
+  int32_t redo;      /* current_redo */
+  int32_t status;    /* 0 running, 1 stopped on tlim / nlim, -1 more than
+                        max_redo redos in a row */
+  int32_t verdict;   /* snapx_check_state bits of the latest attempt */
+  int32_t active;    /* 1 when the latest attempt was a pass of the loop */
+  int32_t steps;     /* accepted steps */
+  int32_t redos;     /* redos in total */
+  int32_t restore;   /* kernel scratch */
+  int32_t reserved2;
+} snapx_step_control;
+
+/* Where an attempt's CFL step reads the primitives (dt_source): */
+#define SNAPX_DT_FROM_W 0 /* w as the previous attempt left it, the last
+                             stage's primitives (the reference) */
+#define SNAPX_DT_FROM_U 1 /* the primitives of the attempt's input u */
+
+/* nattempts attempts on the plain path (snapx_rk3_step_dt).  With
+   SNAPX_DT_FROM_W, w holds the primitives of the first attempt's CFL step
+   (after initialisation, the initial primitives).  dt_log, when not NULL,
+   receives one double per attempt: its dt, or 0 for an attempt after the
+   loop stopped.  ctrl, u, w, dt_log and the workspace are device memory. */
+size_t snapx_rk3_attempts_workspace_bytes(const snapx_config* cfg, int dtype);
+int snapx_rk3_attempts(const snapx_config* cfg, snapx_step_control* ctrl,
+                       int nattempts, int dt_source, void* u, void* w,
+                       double* dt_log, void* workspace, size_t workspace_bytes,
+                       int dtype, void* stream);
+
+/* The same loop on the fused FP32 path (snapx_fused_rk3_step_dt).  The fused
+   step keeps no primitives, so the CFL step reads those of the attempt's
+   input (SNAPX_DT_FROM_U). */
+size_t snapx_fused_attempts_workspace_bytes(const snapx_config* cfg);
+int snapx_fused_attempts(const snapx_config* cfg, snapx_step_control* ctrl,
+                         int nattempts, float* u, double* dt_log,
+                         void* workspace, size_t workspace_bytes,
+                         void* stream);
 
 /* ---------------------------------------------------------------------- */
 /* HIP graphs (no reference counterpart)                                   */

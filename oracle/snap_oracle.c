@@ -1953,10 +1953,14 @@ double oc_max_time_step(const oc_config* c, const double* w, double cfl) {
       for (int i = il; i < iu; ++i) {
         long q = ((long)k * n2 + j) * n1 + i;
         double cs = sqrt(c->gammad * 1.0 * w[OC_IPR * ncell + q] / w[OC_IDN * ncell + q]);
-        double t;
-        if (use1 && (t = W1[q] / (fabs(w[OC_IVX * ncell + q]) + cs)) < d1) d1 = t;
-        if (use2 && (t = W2[q] / (fabs(w[OC_IVY * ncell + q]) + cs)) < d2) d2 = t;
-        if (use3 && (t = W3[q] / (fabs(w[OC_IVZ * ncell + q]) + cs)) < d3) d3 = t;
+        /* torch's min() propagates NaN: once a candidate is NaN, the axis
+           minimum stays NaN */
+        double t1 = W1[q] / (fabs(w[OC_IVX * ncell + q]) + cs);
+        double t2 = W2[q] / (fabs(w[OC_IVY * ncell + q]) + cs);
+        double t3 = W3[q] / (fabs(w[OC_IVZ * ncell + q]) + cs);
+        if (use1 && (isnan(t1) || t1 < d1)) d1 = t1;
+        if (use2 && (isnan(t2) || t2 < d2)) d2 = t2;
+        if (use3 && (isnan(t3) || t3 < d3)) d3 = t3;
       }
   free(W1);
   free(W2);
@@ -1964,6 +1968,7 @@ double oc_max_time_step(const oc_config* c, const double* w, double cfl) {
   free(dx1f);
   free(dx2f);
   free(dx3f);
+  if (isnan(d1) || isnan(d2) || isnan(d3)) return 1.0 * cfl * NAN;
   double dt = d1 < d2 ? d1 : d2;
   dt = dt < d3 ? dt : d3;
   return 1.0 * cfl * dt;

@@ -40,6 +40,7 @@ HIP_UNITS = {
     # its denormals
     "fused.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt", "-fgpu-flush-denormals-to-zero"],
     "adjoint.hip": ["-ffp-contract=off"],
+    "control.hip": ["-ffp-contract=off"],
 }
 
 HIP_FLAGS = [

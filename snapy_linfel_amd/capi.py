@@ -10,6 +10,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libsnapx.so")
@@ -123,10 +124,12 @@ SIGNATURES = {
     "snapx_hydro_forward": (_I, [_CFG, _D, _P, _P, _P, _P, _P, _P, _P, _Z, _I, _P]),
     "snapx_rk3_workspace_bytes": (_Z, [_CFG, _I]),
     "snapx_rk3_step": (_I, [_CFG, _D, _P, _P, _P, _Z, _I, _P]),
+    "snapx_rk3_step_dt": (_I, [_CFG, _P, _P, _P, _P, _Z, _I, _P]),
     "snapx_max_time_step": (_I, [_CFG, _P, _D, _P, _I, _P]),
     "snapx_check_state": (_I, [_CFG, _P, _P, _I, _P]),
     "snapx_fused_workspace_bytes": (_Z, [_CFG]),
     "snapx_fused_rk3_step": (_I, [_CFG, _D, _P, _P, _Z, _P]),
+    "snapx_fused_rk3_step_dt": (_I, [_CFG, _P, _P, _P, _Z, _P]),
     "snapx_fused_rk3_steps": (_I, [_CFG, _D, _I, _P, _P, _Z, _P, _P]),
     "snapx_fused_rk3_steps_static": (_I, [_CFG, _D, _I, _P, _P]),
     "snapx_fused_background_bytes": (_Z, [_CFG]),
@@ -140,6 +143,11 @@ SIGNATURES = {
     "snapx_batch_plan": (_I, [_CFG, _I, _D, _I, _I, _I, _P, _Z, _P]),
     "snapx_batch_rk3_steps": (_I, [_CFG, _P, _I, _P, _P, _Z, _P, _P]),
     "snapx_batch_autotune": (_I, [_CFG, _I, _D, _P, _P, _Z, _I, _P, _P, _P]),
+    "snapx_batch_set_dt": (_I, [_P, _P, _P, _Z, _P]),
+    "snapx_rk3_attempts_workspace_bytes": (_Z, [_CFG, _I]),
+    "snapx_rk3_attempts": (_I, [_CFG, _P, _I, _I, _P, _P, _P, _P, _Z, _I, _P]),
+    "snapx_fused_attempts_workspace_bytes": (_Z, [_CFG]),
+    "snapx_fused_attempts": (_I, [_CFG, _P, _I, _P, _P, _P, _Z, _P]),
     "snapx_fused_graph_create": (_I, [_CFG, _D, _I, _P, _P, _Z, _P]),
     "snapx_batch_graph_create": (_I, [_CFG, _P, _I, _P, _P, _Z, _P]),
     "snapx_graph_launch": (_I, [_P, _P]),
@@ -295,6 +303,21 @@ def rk3_step(cfg, dt, u, w, ws=None):
                                dtype_code(u), stream_ptr()), "rk3_step")
 
 
+def rk3_step_dt(cfg, dt, u, w, ws=None):
+    """rk3_step with the time step read from the one-element float64 device
+    tensor ``dt``."""
+    _check_dt(dt, 1, u.device)
+    ws = rk3_workspace(cfg, dtype_code(u), u.device) if ws is None else ws
+    check(lib().snapx_rk3_step_dt(C.byref(cfg), ptr(dt), ptr(u), ptr(w), ptr(ws), ws.numel(),
+                                  dtype_code(u), stream_ptr()), "rk3_step_dt")
+
+
+def _check_dt(dt, n, device):
+    torch = _torch()
+    if dt.dtype != torch.float64 or dt.numel() < n or dt.device != device:
+        raise SnapxError(f"dt must be a float64 device tensor of at least {n} elements")
+
+
 def max_time_step(cfg, w, cfl, out):
     check(lib().snapx_max_time_step(C.byref(cfg), ptr(w), cfl, ptr(out), dtype_code(w),
                                     stream_ptr()), "max_time_step")
@@ -329,6 +352,7 @@ def fused_workspace(cfg, device="cuda"):
 
 
 def fused_rk3_steps(cfg, dt, nsteps, u, ws=None, timing=None):
+    _check_state(cfg, u, (_torch().float32,))
     ws = fused_workspace(cfg, u.device) if ws is None else ws
     tp = C.byref(timing) if timing is not None else None
     check(lib().snapx_fused_rk3_steps(C.byref(cfg), dt, nsteps, ptr(u), ptr(ws), ws.numel(), tp,
@@ -413,9 +437,91 @@ def fused_rk3_steps_static(cfg, dt, nsteps, u):
 
 
 def fused_rk3_step(cfg, dt, u, ws=None):
+    _check_state(cfg, u, (_torch().float32,))
     ws = fused_workspace(cfg, u.device) if ws is None else ws
     check(lib().snapx_fused_rk3_step(C.byref(cfg), dt, ptr(u), ptr(ws), ws.numel(),
                                      stream_ptr()), "fused_rk3_step")
+
+
+def fused_rk3_step_dt(cfg, dt, u, ws=None):
+    """fused_rk3_step with the time step read from the one-element float64
+    device tensor ``dt``."""
+    _check_state(cfg, u, (_torch().float32,))
+    _check_dt(dt, 1, u.device)
+    ws = fused_workspace(cfg, u.device) if ws is None else ws
+    check(lib().snapx_fused_rk3_step_dt(C.byref(cfg), ptr(dt), ptr(u), ptr(ws), ws.numel(),
+                                        stream_ptr()), "fused_rk3_step_dt")
+
+
+DT_FROM_W, DT_FROM_U = 0, 1
+
+
+class StepControl(C.Structure):
+    """snapx_step_control (include/snapx.h)."""
+    _fields_ = [("cfl", C.c_double), ("tlim", C.c_double), ("nlim", C.c_int64),
+                ("max_redo", C.c_int32), ("redo_mask", C.c_int32),
+                ("time", C.c_double), ("dt", C.c_double), ("dt_raw", C.c_double),
+                ("cycle", C.c_int64), ("redo", C.c_int32), ("status", C.c_int32),
+                ("verdict", C.c_int32), ("active", C.c_int32), ("steps", C.c_int32),
+                ("redos", C.c_int32), ("restore", C.c_int32), ("reserved2", C.c_int32)]
+
+
+class StepLoop:
+    """The CFL time loop with redo, its state in device memory
+    (snapx_step_control): ``attempts`` / ``fused_attempts`` enqueue attempts
+    without synchronising, so they can be captured in a graph; ``state``
+    reads the record back (synchronising)."""
+
+    def __init__(self, cfg, cfl, tlim=float("inf"), nlim=-1, max_redo=5,
+                 redo_mask=None, time=0.0, cycle=0, device="cuda"):
+        torch = _torch()
+        self.cfg = cfg
+        self.rec = torch.zeros(C.sizeof(StepControl), dtype=torch.uint8, device=device)
+        self.ws = None
+        c = StepControl()
+        c.cfl, c.tlim, c.nlim, c.max_redo = cfl, tlim, nlim, max_redo
+        c.redo_mask = (REDO_DENSITY | REDO_PRESSURE) if redo_mask is None else redo_mask
+        c.time, c.cycle = time, cycle
+        self.rec.copy_(torch.frombuffer(bytearray(bytes(c)), dtype=torch.uint8))
+
+    def state(self) -> StepControl:
+        return StepControl.from_buffer_copy(bytes(self.rec.cpu().numpy().tobytes()))
+
+    def prepare(self, u, fused=False):
+        """Allocates the attempts' workspace for states like u (before a
+        graph capture, so that the capture allocates nothing)."""
+        if fused:
+            nb = lib().snapx_fused_attempts_workspace_bytes(C.byref(self.cfg))
+            if nb == 0:
+                raise SnapxError("fused_attempts: " + lib().snapx_last_error().decode())
+        else:
+            nb = lib().snapx_rk3_attempts_workspace_bytes(C.byref(self.cfg), dtype_code(u))
+        if self.ws is None or self.ws.numel() < nb:
+            self.ws = workspace(nb, u.device)
+
+    def _log(self, dt_log, n, device):
+        if dt_log is not None:
+            _check_dt(dt_log, n, device)
+        return ptr(dt_log)
+
+    def attempts(self, u, w, n, dt_source=DT_FROM_W, dt_log=None):
+        """n attempts on the plain path (u, w float64 or float32)."""
+        torch = _torch()
+        _check_state(self.cfg, u, (torch.float64, torch.float32))
+        _check_state(self.cfg, w, (u.dtype,))
+        self.prepare(u)
+        check(lib().snapx_rk3_attempts(C.byref(self.cfg), ptr(self.rec), n, dt_source, ptr(u),
+                                       ptr(w), self._log(dt_log, n, u.device), ptr(self.ws),
+                                       self.ws.numel(), dtype_code(u), stream_ptr()),
+              "rk3_attempts")
+
+    def fused_attempts(self, u, n, dt_log=None):
+        """n attempts on the fused FP32 path."""
+        _check_state(self.cfg, u, (_torch().float32,))
+        self.prepare(u, fused=True)
+        check(lib().snapx_fused_attempts(C.byref(self.cfg), ptr(self.rec), n, ptr(u),
+                                         self._log(dt_log, n, u.device), ptr(self.ws),
+                                         self.ws.numel(), stream_ptr()), "fused_attempts")
 
 
 class Batch:
@@ -429,6 +535,7 @@ class Batch:
         if nb == 0:
             raise SnapxError("batch: " + lib().snapx_last_error().decode())
         self.ws = workspace(nb, device)
+        self._graphs = weakref.WeakSet()  # live graphs captured on self.ws
         self.info = BatchInfo()
         check(lib().snapx_batch_plan(self.cfgs, self.n, dt, tile[0], tile[1], tile[2],
                                      ptr(self.ws), self.ws.numel(), C.byref(self.info)),
@@ -451,7 +558,16 @@ class Batch:
             n *= d
         return u[self.offsets[m]:self.offsets[m] + n].view(*c.shape)
 
+    def _check(self, u):
+        torch = _torch()
+        if u.dtype != torch.float32 or u.numel() != self.total:
+            raise SnapxError(f"batch state must be float32 with {self.total} elements "
+                             f"(got {u.dtype}, {u.numel()})")
+        if not u.is_cuda or not u.is_contiguous():
+            raise SnapxError("batch state must be a contiguous device tensor")
+
     def steps(self, u, nsteps, timing=None):
+        self._check(u)
         tp = C.byref(timing) if timing is not None else None
         check(lib().snapx_batch_rk3_steps(self.cfgs, C.byref(self.info), nsteps, ptr(u),
                                           ptr(self.ws), self.ws.numel(), tp, stream_ptr()),
@@ -459,7 +575,12 @@ class Batch:
 
     def autotune(self, u, dt, trials=3):
         """Re-plan with the fastest RHS tile for these members (u unchanged);
-        returns (tile, ms per RHS launch)."""
+        returns (tile, ms per RHS launch).  Refused while a graph of this
+        batch exists: the graph keeps the launch shapes of the old plan."""
+        self._check(u)
+        if len(self._graphs):
+            raise SnapxError("batch autotune: destroy the batch's graphs first (they keep "
+                             "the launch shapes of the current plan)")
         ms = C.c_double(0.0)
         check(lib().snapx_batch_autotune(self.cfgs, self.n, dt, ptr(u), ptr(self.ws),
                                          self.ws.numel(), trials, stream_ptr(),
@@ -468,11 +589,22 @@ class Batch:
 
     def graph(self, u, nsteps):
         """A Graph replaying nsteps batched steps on u."""
+        self._check(u)
         h = C.c_void_p()
         check(lib().snapx_batch_graph_create(self.cfgs, C.byref(self.info), nsteps, ptr(u),
                                              ptr(self.ws), self.ws.numel(), C.byref(h)),
               "batch_graph_create")
-        return Graph(h, (u, self.ws))
+        g = Graph(h, (u, self.ws))
+        self._graphs.add(g)
+        return g
+
+    def set_dt(self, dt):
+        """Per-member time steps from the float64 device tensor dt
+        (nmembers values), written into the member table by a kernel: the
+        steps and graph replays enqueued after it use them."""
+        _check_dt(dt, self.n, self.ws.device)
+        check(lib().snapx_batch_set_dt(C.byref(self.info), ptr(dt), ptr(self.ws),
+                                       self.ws.numel(), stream_ptr()), "batch_set_dt")
 
 
 class Graph:

@@ -108,6 +108,8 @@ struct FusedArgs {
   // tables of bg_impl.hpp
   const float* bgt;
   const double* bgd;
+  // device time step (double); replaces dt when set
+  const double* dtp;
 };
 
 // Maps an interior-relative index x (may lie in the ghost range) of an axis
@@ -772,6 +774,7 @@ template <int kMerged>
 __global__ void __launch_bounds__(kRhsThreads)
     fused_rhs_kernel(const float* __restrict__ u, float* __restrict__ du, FusedArgs a) {
   extern __shared__ float smem[];
+  if (a.dtp) a.dt = (float)*a.dtp;
   rhs_tile<kMerged>(u, du, a, blockIdx.x, smem);
 }
 
@@ -779,6 +782,7 @@ __global__ void __launch_bounds__(kRhsThreads)
 __global__ void __launch_bounds__(kRhsThreads)
     fused_rhs_bg_kernel(const float* __restrict__ u, float* __restrict__ du, FusedArgs a) {
   extern __shared__ float smem[];
+  if (a.dtp) a.dt = (float)*a.dtp;
   rhs_tile<1, false, true>(u, du, a, blockIdx.x, smem);
 }
 
@@ -786,6 +790,7 @@ __global__ void __launch_bounds__(kRhsThreads)
 __global__ void __launch_bounds__(kRhsThreads)
     fused_rhs_sph_kernel(const float* __restrict__ u, float* __restrict__ du, FusedArgs a) {
   extern __shared__ float smem[];
+  if (a.dtp) a.dt = (float)*a.dtp;
   rhs_tile<1, true>(u, du, a, blockIdx.x, smem);
 }
 
@@ -843,6 +848,7 @@ template <class P, bool kBg = false>
 __global__ void __launch_bounds__(kRhsThreads, 4)
     fused_rhs_march_kernel(const float* __restrict__ u, float* __restrict__ du, FusedArgs a) {
   extern __shared__ float smem[];
+  if (a.dtp) a.dt = (float)*a.dtp;
   constexpr int NP = kBg ? 6 : 5;  // LDS planes per slice
   const int H1 = a.tx1 + 6, H2 = a.tx2 + 6, HS = H1 * H2;
   const int ncell = a.n1 * a.n2 * a.n3;
@@ -1102,6 +1108,8 @@ struct ColArgs {
   // level and the FP32 tables of bg_impl.hpp), null for the total state
   const double* bgd;
   const float* bgt;
+  // device time step; replaces dt when set
+  const double* dtp;
 };
 
 // primitive state (FP64) of interior-relative layer i of a column whose
@@ -1368,6 +1376,7 @@ __global__ void __launch_bounds__(kColThreads)
                         const float* __restrict__ du, double* __restrict__ scr, ColArgs a) {
   int col = blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= a.ncol) return;
+  if (a.dtp) a.dt = *a.dtp;
   column_work<N>(u, u0, du, scr, a, col);
 }
 
@@ -1895,6 +1904,7 @@ __global__ void __launch_bounds__(PcrShape<N>::kThreads)
     fused_column_pcr_kernel(float* __restrict__ u, float* __restrict__ u0,
                             const float* __restrict__ du, ColArgs a) {
   extern __shared__ double sh[];
+  if (a.dtp) a.dt = *a.dtp;
   int col = blockIdx.x * (blockDim.x >> a.lshift) + (threadIdx.x >> a.lshift);
   column_pcr<N, false, kBg>(u, u0, du, a, col, sh);
 }
@@ -1905,6 +1915,7 @@ __global__ void __launch_bounds__(PcrShape<N>::kThreads)
     fused_column_pcr_sph_kernel(float* __restrict__ u, float* __restrict__ u0,
                                 const float* __restrict__ du, ColArgs a) {
   extern __shared__ double sh[];
+  if (a.dtp) a.dt = *a.dtp;
   int col = blockIdx.x * (blockDim.x >> a.lshift) + (threadIdx.x >> a.lshift);
   column_pcr<N, true>(u, u0, du, a, col, sh);
 }
@@ -1914,6 +1925,7 @@ __global__ void __launch_bounds__(PcrShape<3>::kThreads, 2)
     fused_column_pcr2_kernel(float* __restrict__ u, float* __restrict__ u0,
                              const float* __restrict__ du, ColArgs a) {
   extern __shared__ double sh[];
+  if (a.dtp) a.dt = *a.dtp;
   int col = blockIdx.x * (blockDim.x >> a.lshift) + (threadIdx.x >> a.lshift);
   column_pcr2<3, kBg>(u, u0, du, a, col, sh);
 }
@@ -2140,6 +2152,7 @@ static FusedArgs fill_fused_args(const snapx_config* cfg, Tile t, double dt) {
   a.geo = make_geometry(cfg);
   a.bgt = nullptr;
   a.bgd = nullptr;
+  a.dtp = nullptr;
   const bool sph = cfg->coord == SNAPX_COORD_SPHERICAL_POLAR;
   // ghost-layer distances of the refill: dx1v of the layers crossed
   double zi = 0., zo = 0.;
@@ -2251,6 +2264,7 @@ static int make_col_args(const snapx_config* cfg, double dt, ColArgs* out) {
   c.geo = make_geometry(cfg);
   c.bgd = nullptr;
   c.bgt = nullptr;
+  c.dtp = nullptr;
   const bool sph = cfg->coord == SNAPX_COORD_SPHERICAL_POLAR;
   c.zin1 = sph ? sph_dx1v(c.geo, g.ng - 1) : g.dx1;
   c.zout1 = sph ? sph_dx1v(c.geo, g.n1 - g.ng - 1) : g.dx1;
@@ -2295,9 +2309,11 @@ struct EventPair {
 // One fused RK3 step; with ev != nullptr the 7 launches are bracketed by the
 // event pairs ev[0..6] (K1, K2 per stage, then the boundary kernel).
 // bg: device tables of the hydrostatic background (bg_impl.hpp); u is then
-// the perturbation state and the stored ghosts are not refreshed.
+// the perturbation state and the stored ghosts are not refreshed.  dtp: the
+// time step in device memory (replaces dt).
 static int fused_step(const snapx_config* cfg, double dt, float* u, void* workspace,
-                      hipStream_t st, EventPair* ev, bool bc = true, const void* bg = nullptr) {
+                      hipStream_t st, EventPair* ev, bool bc = true, const void* bg = nullptr,
+                      const double* dtp = nullptr) {
   Grid g = make_grid(cfg);
   size_t field = align256(sizeof(float) * 5 * (size_t)g.ncell);
   char* base = reinterpret_cast<char*>(workspace);
@@ -2329,6 +2345,7 @@ static int fused_step(const snapx_config* cfg, double dt, float* u, void* worksp
     a.bgt = reinterpret_cast<const float*>(a.bgd + 2 * g.n1);
     SNAPX_CHECK_ARG(march || a.merged, "perturbation path: tile without a merged face pass");
   }
+  a.dtp = dtp;
   size_t smem = march ? march_smem_bytes(a) : rhs_smem_bytes(a);
   SNAPX_CHECK_ARG(smem <= 64 * 1024, "fused path: tile needs more than 64 KB LDS");
   int nblk = a.nt1 * a.nt2 * a.nt3;
@@ -2336,6 +2353,7 @@ static int fused_step(const snapx_config* cfg, double dt, float* u, void* worksp
   if (int e = make_col_args(cfg, dt, &c)) return e;
   c.bgd = a.bgd;
   c.bgt = a.bgt;
+  c.dtp = dtp;
   int N = ((cfg->implicit_scheme >> 3) & 1) ? 5 : 3;
   int cblk = (c.ncol + kColThreads - 1) / kColThreads;
 
@@ -2427,6 +2445,15 @@ extern "C" int snapx_fused_rk3_step(const snapx_config* cfg, double dt, float* u
   SNAPX_CHECK_ARG(workspace_bytes >= snapx_fused_workspace_bytes(cfg),
                   "snapx_fused_rk3_step: workspace too small");
   return fused_step(cfg, dt, u, workspace, as_stream(stream), nullptr);
+}
+
+extern "C" int snapx_fused_rk3_step_dt(const snapx_config* cfg, const double* dt, float* u,
+                                       void* workspace, size_t workspace_bytes, void* stream) {
+  if (int e = check_fused(cfg)) return e;
+  SNAPX_CHECK_ARG(dt && u && workspace, "snapx_fused_rk3_step_dt: null pointer");
+  SNAPX_CHECK_ARG(workspace_bytes >= snapx_fused_workspace_bytes(cfg),
+                  "snapx_fused_rk3_step_dt: workspace too small");
+  return fused_step(cfg, 0., u, workspace, as_stream(stream), nullptr, true, nullptr, dt);
 }
 
 extern "C" int snapx_fused_rk3_steps(const snapx_config* cfg, double dt, int nsteps, float* u,
@@ -2927,6 +2954,32 @@ static int capture_graph(F enqueue, void** graph_out) {
 }
 
 }  // namespace snapx
+
+namespace snapx {
+
+// per-member time steps from device memory into the member table
+__global__ void batch_set_dt_kernel(BatchMember* __restrict__ mem, const double* __restrict__ dt,
+                                    int nm) {
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nm) return;
+  const double d = dt[m];
+  mem[m].a.dt = (float)d;
+  mem[m].c.dt = d;
+}
+
+}  // namespace snapx
+
+extern "C" int snapx_batch_set_dt(const snapx_batch_info* info, const double* dt,
+                                  void* workspace, size_t workspace_bytes, void* stream) {
+  SNAPX_CHECK_ARG(info && dt && workspace, "snapx_batch_set_dt: null pointer");
+  SNAPX_CHECK_ARG(info->nmembers >= 1 && workspace_bytes >= batch_table_bytes(info->nmembers),
+                  "snapx_batch_set_dt: workspace too small");
+  hipLaunchKernelGGL(batch_set_dt_kernel, dim3((info->nmembers + 63) / 64), dim3(64), 0,
+                     as_stream(stream), reinterpret_cast<BatchMember*>(workspace), dt,
+                     info->nmembers);
+  SNAPX_LAUNCH_CHECK();
+  return 0;
+}
 
 extern "C" int snapx_fused_graph_create(const snapx_config* cfg, double dt, int nsteps, float* u,
                                         void* workspace, size_t workspace_bytes,

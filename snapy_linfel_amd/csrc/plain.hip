@@ -186,6 +186,7 @@ struct DuArgs {
   int n1, n2, n3, ng;
   double dx1, dx2, dx3;
   double dt, grav1, grav2, grav3, nh;
+  const double* dtp;  // device time step; replaces dt when set
 };
 
 // CoordinateImpl::divergence (src/coord/coordinate.cpp:352-396), the
@@ -199,7 +200,7 @@ __global__ void du_kernel(const T* __restrict__ f1, const T* __restrict__ f2,
   int n1 = a.n1, n2 = a.n2, n3 = a.n3, ng = a.ng;
   int ncell = n1 * n2 * n3;
   T dx1 = T(a.dx1), dx2 = T(a.dx2), dx3 = T(a.dx3);
-  T dt = T(a.dt);
+  T dt = T(a.dtp ? *a.dtp : a.dt);
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < ncell; q += gridDim.x * blockDim.x) {
     int i = q % n1;
     int j = (q / n1) % n2;
@@ -259,6 +260,7 @@ struct CoordArgs {
   double dx1, dx2, dx3;  // Cartesian widths
   double dt, grav1, grav2, grav3, nh;
   int du_mode;  // 0: div on every cell; 1: du = -dt div on the interior + forcings
+  const double* dtp;  // device time step; replaces dt when set
 };
 
 // CoordinateImpl::forward (src/coord/coordinate.cpp:352-434) for both
@@ -340,7 +342,7 @@ __global__ void coord_forward_kernel(const T* __restrict__ w, const T* __restric
       for (int v = 0; v < 5; ++v) out[v * ncell + q] = d[v];
       continue;
     }
-    T dt = T(a.dt);
+    T dt = T(a.dtp ? *a.dtp : a.dt);
     bool interior = r1 && r2 && r3;
 #pragma unroll
     for (int v = 0; v < 5; ++v) d[v] = interior ? -dt * d[v] : T(0.);
@@ -487,6 +489,22 @@ __global__ void rk_combine_kernel(T* __restrict__ u, const T* __restrict__ u0,
 
 __global__ void fill_u64_kernel(unsigned long long* p, unsigned long long v) { *p = v; }
 
+// Order key of a CFL candidate for an unsigned atomic minimum.  Candidates
+// are width / (|v| + cs) >= 0 (or +inf), whose bit patterns order like their
+// values; shifted up by one they leave key 0 for NaN, so that one NaN
+// candidate wins the minimum, as the reference's tensor min() propagates NaN.
+__host__ __device__ inline unsigned long long cfl_key(double c) {
+  if (c != c) return 0ull;
+  unsigned long long b;
+  memcpy(&b, &c, sizeof(b));
+  return b + 1ull;
+}
+
+__device__ inline double cfl_value(unsigned long long key) {
+  return key == 0ull ? __longlong_as_double(0x7ff8000000000000ll)
+                     : __longlong_as_double((long long)(key - 1ull));
+}
+
 // HydroImpl::max_time_step (src/hydro/hydro.cpp:118-180) with the
 // coordinate's center widths: Cartesian dx, spherical-polar dr, r dtheta and
 // r sin(theta) dphi (src/coord/spherical_polar.cpp:146-155)
@@ -495,7 +513,7 @@ __global__ void cfl_kernel(const T* __restrict__ w, unsigned long long* __restri
                            int n2, int n1, int ng, T gammad, T dx1, T dx2, T dx3, int use1,
                            int use2, int use3, Geometry geo) {
   int ncell = n1 * n2 * n3;
-  double best = 1.e9;
+  unsigned long long best = cfl_key(1.e9);
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < ncell; q += gridDim.x * blockDim.x) {
     int i = q % n1;
     int j = (q / n1) % n2;
@@ -510,19 +528,17 @@ __global__ void cfl_kernel(const T* __restrict__ w, unsigned long long* __restri
       if (use2) dx2 = T(sph_width2(geo, j, i));
       if (use3) dx3 = T(sph_width3(geo, k, j, i));
     }
-    if (use1) best = fmin(best, (double)(dx1 / (fabs(w[IVX * ncell + q]) + cs)));
-    if (use2) best = fmin(best, (double)(dx2 / (fabs(w[IVY * ncell + q]) + cs)));
-    if (use3) best = fmin(best, (double)(dx3 / (fabs(w[IVZ * ncell + q]) + cs)));
+    if (use1) best = min(best, cfl_key((double)(dx1 / (fabs(w[IVX * ncell + q]) + cs))));
+    if (use2) best = min(best, cfl_key((double)(dx2 / (fabs(w[IVY * ncell + q]) + cs))));
+    if (use3) best = min(best, cfl_key((double)(dx3 / (fabs(w[IVZ * ncell + q]) + cs))));
   }
-  // wavefront reduction, then one atomic per wavefront (positive doubles order
-  // like their bit patterns)
-  for (int off = warpSize / 2; off > 0; off >>= 1) best = fmin(best, __shfl_down(best, off));
-  if ((threadIdx.x & (warpSize - 1)) == 0)
-    atomicMin(out, (unsigned long long)__double_as_longlong(best));
+  // wavefront reduction, then one atomic per wavefront
+  for (int off = warpSize / 2; off > 0; off >>= 1) best = min(best, __shfl_down(best, off));
+  if ((threadIdx.x & (warpSize - 1)) == 0) atomicMin(out, best);
 }
 
 __global__ void cfl_finalize_kernel(double* dt, double cfl) {
-  double v = *dt;
+  double v = cfl_value(*reinterpret_cast<const unsigned long long*>(dt));
   *dt = 1.0 * cfl * v;
 }
 
@@ -644,7 +660,8 @@ using namespace snapx;
 // implemented in vic.hip
 namespace snapx {
 int implicit_correction(const snapx_config* cfg, double dt, void* du, const void* w,
-                        void* workspace, size_t bytes, int dtype, hipStream_t st);
+                        void* workspace, size_t bytes, int dtype, hipStream_t st,
+                        const double* dtp = nullptr);
 size_t implicit_workspace_bytes(const snapx_config* cfg);
 }  // namespace snapx
 
@@ -717,19 +734,21 @@ extern "C" int snapx_riemann(int solver, const void* wl, const void* wr, void* f
   return launch_riemann<float>(solver, wl, wr, flx, face_pressure, ncell, dim, gammad, st);
 }
 
-static DuArgs du_args(const snapx_config* c, double dt) {
+static DuArgs du_args(const snapx_config* c, double dt, const double* dtp) {
   Grid g = make_grid(c);
   DuArgs a;
   a.n1 = g.n1, a.n2 = g.n2, a.n3 = g.n3, a.ng = g.ng;
   a.dx1 = g.dx1, a.dx2 = g.dx2, a.dx3 = g.dx3;
   a.dt = dt;
+  a.dtp = dtp;
   a.grav1 = c->grav1, a.grav2 = c->grav2, a.grav3 = c->grav3, a.nh = c->non_hydrostatic;
   return a;
 }
 
 template <typename T>
 static int launch_du(const snapx_config* c, double dt, const void* f1, const void* f2,
-                     const void* f3, const void* w, const void* rg, void* du, hipStream_t st) {
+                     const void* f3, const void* w, const void* rg, void* du, hipStream_t st,
+                     const double* dtp = nullptr) {
   Grid g = make_grid(c);
   if (g.n1 == 1) f1 = nullptr;  // no faces on an inactive axis
   if (g.n2 == 1) f2 = nullptr;
@@ -737,7 +756,7 @@ static int launch_du(const snapx_config* c, double dt, const void* f1, const voi
   hipLaunchKernelGGL(du_kernel<T>, blocks_for(g.ncell, kThreads), dim3(kThreads), 0, st,
                      reinterpret_cast<const T*>(f1), reinterpret_cast<const T*>(f2),
                      reinterpret_cast<const T*>(f3), reinterpret_cast<const T*>(w),
-                     reinterpret_cast<const T*>(rg), as<T>(du), du_args(c, dt));
+                     reinterpret_cast<const T*>(rg), as<T>(du), du_args(c, dt, dtp));
   SNAPX_LAUNCH_CHECK();
   return 0;
 }
@@ -755,12 +774,14 @@ static GhostDz ghost_dz(const snapx_config* c) {
   return dz;
 }
 
-static CoordArgs coord_args(const snapx_config* c, double dt, int du_mode) {
+static CoordArgs coord_args(const snapx_config* c, double dt, int du_mode,
+                            const double* dtp) {
   Grid g = make_grid(c);
   CoordArgs a;
   a.g = make_geometry(c);
   a.dx1 = g.dx1, a.dx2 = g.dx2, a.dx3 = g.dx3;
   a.dt = dt;
+  a.dtp = dtp;
   a.grav1 = c->grav1, a.grav2 = c->grav2, a.grav3 = c->grav3, a.nh = c->non_hydrostatic;
   a.du_mode = du_mode;
   return a;
@@ -769,7 +790,7 @@ static CoordArgs coord_args(const snapx_config* c, double dt, int du_mode) {
 template <typename T>
 static int launch_coord(const snapx_config* c, double dt, int du_mode, const void* w,
                         const void* f1, const void* f2, const void* f3, const void* fp1,
-                        const void* rg, void* out, hipStream_t st) {
+                        const void* rg, void* out, hipStream_t st, const double* dtp = nullptr) {
   Grid g = make_grid(c);
   // an inactive axis has no faces: its flux (and the x1 face pressure) is
   // never read, so a one-cell axis cannot index past the field
@@ -780,7 +801,7 @@ static int launch_coord(const snapx_config* c, double dt, int du_mode, const voi
                      st, reinterpret_cast<const T*>(w), reinterpret_cast<const T*>(f1),
                      reinterpret_cast<const T*>(f2), reinterpret_cast<const T*>(f3),
                      reinterpret_cast<const T*>(fp1), reinterpret_cast<const T*>(rg), as<T>(out),
-                     coord_args(c, dt, du_mode));
+                     coord_args(c, dt, du_mode, dtp));
   SNAPX_LAUNCH_CHECK();
   return 0;
 }
@@ -870,11 +891,8 @@ extern "C" int snapx_max_time_step(const snapx_config* cfg, const void* w, doubl
   int use2 = g.n2 > 1 && !((sch >> 1) & 1);
   int use3 = g.n3 > 1 && !((sch >> 2) & 1);
   hipStream_t st = as_stream(stream);
-  double init = 1.e9;
-  unsigned long long bits;
-  memcpy(&bits, &init, sizeof(bits));
   auto* acc = reinterpret_cast<unsigned long long*>(dt_out);
-  hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(1), 0, st, acc, bits);
+  hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(1), 0, st, acc, cfl_key(1.e9));
   int nb = std::min<int64_t>((g.ncell + kThreads - 1) / kThreads, 4096);
   if (dtype == SNAPX_F64)
     hipLaunchKernelGGL(cfl_kernel<double>, dim3(nb), dim3(kThreads), 0, st,
@@ -956,7 +974,7 @@ extern "C" size_t snapx_hydro_workspace_bytes(const snapx_config* cfg, int dtype
 template <typename T>
 static int hydro_forward_impl(const snapx_config* c, double dt, void* u, void* w, void* du,
                               void* f1, void* f2, void* f3, void* ws, size_t bytes, int dtype,
-                              hipStream_t st) {
+                              hipStream_t st, const double* dtp = nullptr) {
   Grid g = make_grid(c);
   HydroSpace h = hydro_layout(c, dtype, reinterpret_cast<char*>(ws));
   SNAPX_CHECK_ARG(bytes >= h.bytes, "snapx_hydro_forward: workspace too small");
@@ -1014,13 +1032,13 @@ static int hydro_forward_impl(const snapx_config* c, double dt, void* u, void* w
     }
   }
   if (sph) {
-    if (int e = launch_coord<T>(c, dt, 1, w, Fuse[0], Fuse[1], Fuse[2], h.fp1, rg, du, st))
+    if (int e = launch_coord<T>(c, dt, 1, w, Fuse[0], Fuse[1], Fuse[2], h.fp1, rg, du, st, dtp))
       return e;
-  } else if (int e = launch_du<T>(c, dt, Fuse[0], Fuse[1], Fuse[2], w, rg, du, st)) {
+  } else if (int e = launch_du<T>(c, dt, Fuse[0], Fuse[1], Fuse[2], w, rg, du, st, dtp)) {
     return e;
   }
   if (c->implicit_scheme)
-    return implicit_correction(c, dt, du, w, h.vic, implicit_workspace_bytes(c), dtype, st);
+    return implicit_correction(c, dt, du, w, h.vic, implicit_workspace_bytes(c), dtype, st, dtp);
   return 0;
 }
 
@@ -1046,7 +1064,7 @@ extern "C" size_t snapx_rk3_workspace_bytes(const snapx_config* cfg, int dtype) 
 
 template <typename T>
 static int rk3_step_impl(const snapx_config* c, double dt, void* u, void* w, void* ws,
-                         size_t bytes, int dtype, hipStream_t st) {
+                         size_t bytes, int dtype, hipStream_t st, const double* dtp = nullptr) {
   Grid g = make_grid(c);
   size_t field = align256(sizeof(T) * 5 * (size_t)g.ncell);
   char* base = reinterpret_cast<char*>(ws);
@@ -1063,7 +1081,7 @@ static int rk3_step_impl(const snapx_config* c, double dt, void* u, void* w, voi
   SNAPX_HIP(hipMemcpyAsync(u1, u, sizeof(T) * n, hipMemcpyDeviceToDevice, st));
   for (int s = 0; s < 3; ++s) {
     if (int e = hydro_forward_impl<T>(c, dt, u, w, du, nullptr, nullptr, nullptr, hws, hbytes,
-                                      dtype, st))
+                                      dtype, st, dtp))
       return e;
     hipLaunchKernelGGL(rk_combine_kernel<T>, blocks_for(n, kThreads), dim3(kThreads), 0, st,
                        as<T>(u), u0, u1, du, T(W[s][0]), T(W[s][1]), T(W[s][2]), (int64_t)n);
@@ -1090,4 +1108,17 @@ extern "C" int snapx_rk3_step(const snapx_config* cfg, double dt, void* u, void*
   return dtype == SNAPX_F64
              ? rk3_step_impl<double>(cfg, dt, u, w, workspace, workspace_bytes, dtype, st)
              : rk3_step_impl<float>(cfg, dt, u, w, workspace, workspace_bytes, dtype, st);
+}
+
+extern "C" int snapx_rk3_step_dt(const snapx_config* cfg, const double* dt, void* u, void* w,
+                                 void* workspace, size_t workspace_bytes, int dtype,
+                                 void* stream) {
+  if (int e = validate_config(cfg)) return e;
+  SNAPX_CHECK_ARG(dt && u && w && workspace, "snapx_rk3_step_dt: null pointer");
+  SNAPX_CHECK_ARG(workspace_bytes >= snapx_rk3_workspace_bytes(cfg, dtype),
+                  "snapx_rk3_step_dt: workspace too small");
+  hipStream_t st = as_stream(stream);
+  return dtype == SNAPX_F64
+             ? rk3_step_impl<double>(cfg, 0., u, w, workspace, workspace_bytes, dtype, st, dt)
+             : rk3_step_impl<float>(cfg, 0., u, w, workspace, workspace_bytes, dtype, st, dt);
 }

@@ -25,7 +25,10 @@ struct VicArgs {
   int nl, ncol, ncol2;  // layers, columns, interior columns along x2
   double gammad, area, vol, dt, grav;  // Cartesian face area and volume
   Geometry geo;
+  const double* dtp;  // device time step; replaces dt when set
 };
+
+__device__ __forceinline__ double step_dt(const VicArgs& a) { return a.dtp ? *a.dtp : a.dt; }
 
 __device__ __forceinline__ int col_cell(const VicArgs& a, int col) {
   int j = a.n2 > 1 ? a.ng + col % a.ncol2 : 0;
@@ -61,6 +64,7 @@ __global__ void __launch_bounds__(kVicThreads)
   int ncell = a.n1 * a.n2 * a.n3;
   int64_t total = (int64_t)a.ncol * a.nl;
   const int NN = N * N;
+  const double dt = step_dt(a);
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
        t += (int64_t)gridDim.x * blockDim.x) {
     int col = (int)(t / a.nl);
@@ -73,7 +77,7 @@ __global__ void __launch_bounds__(kVicThreads)
     double* base = abc + ((int64_t)col * a.nl + i) * 3 * NN;
     double A[NN], B[NN], C[NN];
     LayerGeom lg = layer_geom(a, col, i);
-    vic_assemble_cell<double, N>(wm, w0, wp, a.gammad, lg.area_i, lg.area_ip1, lg.vol, a.dt,
+    vic_assemble_cell<double, N>(wm, w0, wp, a.gammad, lg.area_i, lg.area_ip1, lg.vol, dt,
                                  a.grav, i == 0, i == a.nl - 1, A, B, C);
 #pragma unroll
     for (int e = 0; e < NN; ++e) {
@@ -92,6 +96,7 @@ __global__ void __launch_bounds__(kVicThreads)
                      double* __restrict__ delta, double* __restrict__ colsum, VicArgs a) {
   int ncell = a.n1 * a.n2 * a.n3;
   const int NN = N * N;
+  const double dt = step_dt(a);
   for (int col = blockIdx.x * blockDim.x + threadIdx.x; col < a.ncol;
        col += gridDim.x * blockDim.x) {
     int q0 = col_cell(a, col);
@@ -102,14 +107,14 @@ __global__ void __launch_bounds__(kVicThreads)
       double rhs[N];
       int q = q0 + i;
       rhs[0] = (double)du[IDN * ncell + q];
-      rhs[0] /= a.dt;
-      rhs[1] = (double)du[IVX * ncell + q] / a.dt;
+      rhs[0] /= dt;
+      rhs[1] = (double)du[IVX * ncell + q] / dt;
       if (N == 3) {
-        rhs[2] = (double)du[IPR * ncell + q] / a.dt;
+        rhs[2] = (double)du[IPR * ncell + q] / dt;
       } else {
-        rhs[2] = (double)du[IVY * ncell + q] / a.dt;
-        rhs[3] = (double)du[IVZ * ncell + q] / a.dt;
-        rhs[4] = (double)du[IPR * ncell + q] / a.dt;
+        rhs[2] = (double)du[IVY * ncell + q] / dt;
+        rhs[3] = (double)du[IVZ * ncell + q] / dt;
+        rhs[4] = (double)du[IPR * ncell + q] / dt;
       }
       double* ai = cabc + (int64_t)i * 3 * NN;
       double A[NN], B[NN], C[NN];
@@ -225,7 +230,7 @@ __global__ void __launch_bounds__(kVicThreads)
   }
 }
 
-static VicArgs vic_args(const snapx_config* c, double dt) {
+static VicArgs vic_args(const snapx_config* c, double dt, const double* dtp = nullptr) {
   Grid g = make_grid(c);
   VicArgs a;
   a.n1 = g.n1, a.n2 = g.n2, a.n3 = g.n3, a.ng = g.ng;
@@ -238,6 +243,7 @@ static VicArgs vic_args(const snapx_config* c, double dt) {
   a.vol = g.dx3 * (g.dx2 * g.dx1);
   a.geo = make_geometry(c);
   a.dt = dt;
+  a.dtp = dtp;
   a.grav = c->grav1;
   return a;
 }
@@ -254,8 +260,8 @@ size_t implicit_workspace_bytes(const snapx_config* c) {
 
 template <typename T, int N>
 static int vic_launch(const snapx_config* c, double dt, void* du, const void* w, void* ws,
-                      hipStream_t st) {
-  VicArgs a = vic_args(c, dt);
+                      hipStream_t st, const double* dtp) {
+  VicArgs a = vic_args(c, dt, dtp);
   size_t n = (size_t)a.ncol * a.nl;
   double* abc = reinterpret_cast<double*>(ws);
   double* delta = abc + n * 3 * N * N;
@@ -275,7 +281,8 @@ static int vic_launch(const snapx_config* c, double dt, void* du, const void* w,
 }
 
 int implicit_correction(const snapx_config* c, double dt, void* du, const void* w,
-                        void* workspace, size_t bytes, int dtype, hipStream_t st) {
+                        void* workspace, size_t bytes, int dtype, hipStream_t st,
+                        const double* dtp) {
   if (!c->implicit_scheme) return 0;
   SNAPX_CHECK_ARG(c->implicit_scheme == 1 || c->implicit_scheme == 9,
                   "implicit correction: unsupported implicit scheme (use 0, 1 or 9)");
@@ -284,10 +291,10 @@ int implicit_correction(const snapx_config* c, double dt, void* du, const void* 
                   "implicit correction: workspace too small");
   int N = vic_size(c);
   if (dtype == SNAPX_F64)
-    return N == 3 ? vic_launch<double, 3>(c, dt, du, w, workspace, st)
-                  : vic_launch<double, 5>(c, dt, du, w, workspace, st);
-  return N == 3 ? vic_launch<float, 3>(c, dt, du, w, workspace, st)
-                : vic_launch<float, 5>(c, dt, du, w, workspace, st);
+    return N == 3 ? vic_launch<double, 3>(c, dt, du, w, workspace, st, dtp)
+                  : vic_launch<double, 5>(c, dt, du, w, workspace, st, dtp);
+  return N == 3 ? vic_launch<float, 3>(c, dt, du, w, workspace, st, dtp)
+                : vic_launch<float, 5>(c, dt, du, w, workspace, st, dtp);
 }
 
 }  // namespace snapx
@@ -306,5 +313,5 @@ extern "C" int snapx_implicit_correction(const snapx_config* cfg, double dt, voi
   SNAPX_CHECK_ARG(du && w, "snapx_implicit_correction: null pointer");
   SNAPX_CHECK_ARG(!cfg->implicit_scheme || workspace, "snapx_implicit_correction: no workspace");
   return implicit_correction(cfg, dt, du, w, workspace, workspace_bytes, dtype,
-                             as_stream(stream));
+                             as_stream(stream), nullptr);
 }
