@@ -72,7 +72,8 @@ def build_libsnapx(jobs: int = 8) -> str:
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(_run, cmds))
     tmp = LIBSNAPX + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs,
+          "-L/opt/rocm/lib", "-lrocprofiler-sdk-roctx"])
     os.replace(tmp, LIBSNAPX)
     return LIBSNAPX
 

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 #include <stdint.h>
 
 #include <string>
@@ -82,6 +83,16 @@ inline dim3 blocks_for(int64_t n, int threads) {
 }
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// rocTX range over the launches enqueued in one scope (host side): a
+// profiler run with marker tracing (rocprofv3 --marker-trace) attributes
+// each kernel to the step, stage and kernel role that launched it.
+struct TraceRange {
+  explicit TraceRange(const char* name) { roctxRangePushA(name); }
+  ~TraceRange() { roctxRangePop(); }
+  TraceRange(const TraceRange&) = delete;
+  TraceRange& operator=(const TraceRange&) = delete;
+};
 
 // Validates a config for the kernels of this library.
 int validate_config(const snapx_config* c);

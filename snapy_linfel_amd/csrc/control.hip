@@ -102,6 +102,7 @@ int run_attempts(const snapx_config* cfg, snapx_step_control* ctrl, int nattempt
   const size_t bytes = (dtype == SNAPX_F64 ? sizeof(double) : sizeof(float)) * (size_t)n;
   void* sv = reinterpret_cast<void*>(st);
   for (int k = 0; k < nattempts; ++k) {
+    TraceRange range("snapx CFL attempt");
     if (int e = launch_small(attempt_begin_kernel, ctrl, st)) return e;
     if (wb) SNAPX_HIP(hipMemcpyAsync(wb, w, bytes, hipMemcpyDeviceToDevice, st));
     if (from_u)

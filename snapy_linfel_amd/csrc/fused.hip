@@ -2359,10 +2359,18 @@ static int fused_step(const snapx_config* cfg, double dt, float* u, void* worksp
 
   static const double W[3][3] = {
       {0., 1., 1.}, {3. / 4., 1. / 4., 1. / 4.}, {1. / 3., 2. / 3., 2. / 3.}};
+  static const char* const kRhsRange[3] = {"snapx fused K1 rhs stage 0",
+                                           "snapx fused K1 rhs stage 1",
+                                           "snapx fused K1 rhs stage 2"};
+  static const char* const kColRange[3] = {"snapx fused K2 column stage 0",
+                                           "snapx fused K2 column stage 1",
+                                           "snapx fused K2 column stage 2"};
+  TraceRange step_range(bg ? "snapx fused step (perturbation)" : "snapx fused step");
   // with a timing record, the events carry the start and end time stamps of
   // the kernel dispatch itself (hipExtLaunchKernel), not extra stream markers
   for (int s = 0; s < 3; ++s) {
     hipEvent_t ra = ev ? ev[2 * s].a : nullptr, rb = ev ? ev[2 * s].b : nullptr;
+    roctxRangePushA(kRhsRange[s]);
     if (sph)
       hipExtLaunchKernelGGL(fused_rhs_sph_kernel, dim3(nblk), dim3(kRhsThreads), (uint32_t)smem,
                             st, ra, rb, 0u, (const float*)u, du, a);
@@ -2378,7 +2386,9 @@ static int fused_step(const snapx_config* cfg, double dt, float* u, void* worksp
     else
       hipExtLaunchKernelGGL(fused_rhs_kernel<0>, dim3(nblk), dim3(kRhsThreads), (uint32_t)smem, st,
                             ra, rb, 0u, (const float*)u, du, a);
+    roctxRangePop();
     SNAPX_LAUNCH_CHECK();
+    TraceRange col_range(kColRange[s]);
     c.stage = s;
     c.w0 = W[s][0], c.w1 = W[s][1], c.w2 = W[s][2];
     hipEvent_t ca = ev ? ev[2 * s + 1].a : nullptr, cb = ev ? ev[2 * s + 1].b : nullptr;
@@ -2421,8 +2431,10 @@ static int fused_step(const snapx_config* cfg, double dt, float* u, void* worksp
     SNAPX_LAUNCH_CHECK();
   }
   if (ev) SNAPX_HIP(hipEventRecord(ev[6].a, st));
-  if (bc)
+  if (bc) {
+    TraceRange bc_range("snapx fused boundary");
     if (int e = launch_bc_f32(cfg, u, st)) return e;
+  }
   if (ev) SNAPX_HIP(hipEventRecord(ev[6].b, st));
   return 0;
 }
@@ -2843,6 +2855,7 @@ extern "C" int snapx_batch_rk3_steps(const snapx_config* members, const snapx_ba
       SNAPX_HIP(hipEventCreate(&e.b));
     }
   for (int n = 0; n < nsteps; ++n) {
+    TraceRange step_range("snapx batch step");
     for (int s = 0; s < 3; ++s) {
       if (timing) SNAPX_HIP(hipEventRecord(ev[0].a, st));
       if (sph)

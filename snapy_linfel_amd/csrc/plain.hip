@@ -1077,9 +1077,13 @@ static int rk3_step_impl(const snapx_config* c, double dt, void* u, void* w, voi
   // Shu-Osher SSP-RK3 stage weights of u <- w0 u0 + w1 u1 + w2 du
   static const double W[3][3] = {
       {0., 1., 1.}, {3. / 4., 1. / 4., 1. / 4.}, {1. / 3., 2. / 3., 2. / 3.}};
+  static const char* const kStageRange[3] = {"snapx plain stage 0", "snapx plain stage 1",
+                                             "snapx plain stage 2"};
+  TraceRange step_range("snapx plain step");
   SNAPX_HIP(hipMemcpyAsync(u0, u, sizeof(T) * n, hipMemcpyDeviceToDevice, st));
   SNAPX_HIP(hipMemcpyAsync(u1, u, sizeof(T) * n, hipMemcpyDeviceToDevice, st));
   for (int s = 0; s < 3; ++s) {
+    TraceRange stage_range(kStageRange[s]);
     if (int e = hydro_forward_impl<T>(c, dt, u, w, du, nullptr, nullptr, nullptr, hws, hbytes,
                                       dtype, st, dtp))
       return e;
