@@ -278,9 +278,12 @@ def c5_sensitivity(capi, dev, nsteps=200, every=20, ndir=2, wind=(0.0, 0.0), ric
     t0 = time.perf_counter()
     y = A.integrate(state(x), cfg, dt, nsteps, checkpoint_every=every)
     loss = loss_of(y)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
     (grad,) = torch.autograd.grad(loss, x)
     torch.cuda.synchronize()
     t_grad = time.perf_counter() - t0
+    t_back = time.perf_counter() - t1
     rels = []
     for k in range(ndir):
         d = torch.randn(cshape, generator=gen, dtype=torch.float64).to(dev)
@@ -302,6 +305,9 @@ def c5_sensitivity(capi, dev, nsteps=200, every=20, ndir=2, wind=(0.0, 0.0), ric
             "loss": L0,
             "forward_s": round(t_fwd, 3), "forward_plus_adjoint_s": round(t_grad, 3),
             "gradient_over_forward": round(t_grad / t_fwd, 3),
+            "backward_s": round(t_back, 3), "backward_over_forward": round(t_back / t_fwd, 3),
+            "replay": "each segment recorded once (tapes of stage inputs, primitives, "
+                      "tendencies, column blocks); the last segment recorded in the forward",
             "fd_directions": ndir, "fd_rel_err_max": max(rels),
             "fd_rel_err_median": sorted(rels)[len(rels) // 2]}
 
@@ -572,6 +578,8 @@ def main():
     ap.add_argument("--no-sph", action="store_true", help="skip the spherical-polar sector")
     ap.add_argument("--no-adjoint", action="store_true", help="skip every adjoint timing")
     ap.add_argument("--no-c5", action="store_true", help="skip the 200-step sensitivity run")
+    ap.add_argument("--c5-fd-directions", type=int, default=64,
+                    help="central-difference directions of the kink-free C5 check")
     ap.add_argument("--c5-directions", type=int, default=2,
                     help="random directions of the finite-difference check of the gradient")
     args = ap.parse_args()
@@ -775,7 +783,7 @@ def main():
         # uniform horizontal wind (Roe |u| at the x2 / x3 faces) and a
         # convective roll (LMARS upwind switch at the x1 faces)
         extra["c5_fd_check_wind_and_roll"] = c5_sensitivity(
-            capi, dev, ndir=8, wind=(2.0, 1.5), roll=1.0)
+            capi, dev, ndir=args.c5_fd_directions, wind=(2.0, 1.5), roll=1.0)
 
     line = {
         "metric": "cell-updates/s per RK3 step, FP32 fused vs FP64 plain at stated err; "

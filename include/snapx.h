@@ -356,6 +356,28 @@ int snapx_adjoint_rk3_step(const snapx_config* cfg, double dt,
                            void* workspace, size_t workspace_bytes,
                            void* stream);
 
+/* snapx_adjoint_rk3_step in two halves, so that a checkpointed trajectory
+   replays each segment once: snapx_adjoint_record recomputes the step from
+   u_in into a tape of snapx_adjoint_tape_bytes (stage inputs, primitives,
+   explicit tendencies, VIC blocks) and, when u_out is not NULL, writes the
+   step's output (the replay's next input; it agrees with snapx_rk3_step to
+   rounding: the recompute solves the columns by cyclic reduction);
+   snapx_adjoint_sweep then maps lambda = dL/du_out to dL/du_in from that
+   tape (and may overwrite the tape).  The work space
+   (snapx_adjoint_work_bytes) is scratch shared by both; a step's workspace
+   for snapx_adjoint_rk3_step is one work space followed by one tape.  Record
+   followed by sweep gives the bits of snapx_adjoint_rk3_step.  No reference
+   counterpart. */
+size_t snapx_adjoint_tape_bytes(const snapx_config* cfg);
+size_t snapx_adjoint_work_bytes(const snapx_config* cfg);
+int snapx_adjoint_record(const snapx_config* cfg, double dt,
+                         const double* u_in, double* u_out, void* tape,
+                         size_t tape_bytes, void* work, size_t work_bytes,
+                         void* stream);
+int snapx_adjoint_sweep(const snapx_config* cfg, double dt, void* tape,
+                        size_t tape_bytes, double* lambda, void* work,
+                        size_t work_bytes, void* stream);
+
 /* ---------------------------------------------------------------------- */
 /* Batched members (ensembles / blocks of uneven size; no reference        */
 /* counterpart: the reference runs one block per thread of                 */

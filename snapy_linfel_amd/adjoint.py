@@ -12,6 +12,15 @@ states, primitives, interface states or fluxes.
 ``torch.autograd.grad(loss(integrate(u0, ...)), u0)`` is the exact gradient of
 the discrete map (limiter off, non-hydrostatic = 1, no disable-flux: the
 library rejects the rest).
+
+With ``checkpoint_every = K`` the trajectory keeps every K-th state.  By
+default (``tape=True``) the backward of a segment replays its K steps once
+through ``snapx_adjoint_record``, which keeps each step's stage inputs,
+primitives, tendencies and column blocks in a tape (K tapes, reused by every
+segment: about 26 fields of 5 x cells doubles per step), and then sweeps the
+tapes in reverse (``snapx_adjoint_sweep``): one forward recompute per step
+instead of two.  The replay follows the recompute's own column solve (cyclic
+reduction), so its states agree with the forward's to rounding.
 """
 from __future__ import annotations
 
@@ -42,20 +51,100 @@ class RK3Step(torch.autograd.Function):
         return lam, None, None, None
 
 
+class _Tapes:
+    """The tapes of one segment and the scratch of the record and sweep
+    calls, allocated on first use and shared by all segments of a
+    trajectory (the segments' backwards run one after another)."""
+
+    def __init__(self, cfg, device):
+        self.cfg = cfg
+        self.nbytes = capi.adjoint_tape_bytes(cfg)
+        if self.nbytes == 0:
+            raise capi.SnapxError("adjoint: " + capi.lib().snapx_last_error().decode())
+        self.work = capi.adjoint_work(cfg, device)
+        self.tapes = []
+        self.states = None
+
+    def get(self, n, like):
+        while len(self.tapes) < n:
+            self.tapes.append(torch.empty(self.nbytes, dtype=torch.uint8, device=like.device))
+        if self.states is None:
+            self.states = (torch.empty_like(like), torch.empty_like(like))
+        return self.tapes[:n], self.states
+
+
+def _record(cfg, dt, u_in, n, st, out=None):
+    """Records n steps from u_in into the store's tapes; out (or None) gets
+    the last step's output."""
+    tapes, (a, b) = st.get(n, u_in)
+    x = u_in
+    for i in range(n):
+        y = (a if i % 2 == 0 else b) if i < n - 1 else out
+        capi.adjoint_record(cfg, dt, x, y, tapes[i], st.work)
+        x = y
+    return tapes
+
+
+class TapedSegment(torch.autograd.Function):
+    """n FP64 steps as one autograd node: the forward keeps only the
+    segment's input (plain-path steps); the backward records the n steps
+    once and sweeps their tapes in reverse.  With ``last`` (the segment
+    whose backward runs first) the forward records the tapes itself, through
+    the recompute's steps, and the backward only sweeps."""
+
+    @staticmethod
+    def forward(ctx, u, cfg, dt, n, store, last=False):
+        if u.dtype != torch.float64 or not u.is_cuda:
+            raise TypeError("TapedSegment: u must be a float64 GPU tensor")
+        u_in = u.detach().contiguous()
+        out = torch.empty_like(u_in)
+        if last:
+            _record(cfg, dt, u_in, n, store, out)
+        else:
+            out.copy_(u_in)
+            w = torch.empty_like(out)
+            ws = capi.rk3_workspace(cfg, capi.F64, out.device)
+            for _ in range(n):
+                capi.rk3_step(cfg, dt, out, w, ws)
+        ctx.save_for_backward(u_in)
+        ctx.cfg, ctx.dt, ctx.n, ctx.store, ctx.recorded = cfg, dt, n, store, last
+        return out
+
+    @staticmethod
+    def backward(ctx, grad):
+        (u_in,) = ctx.saved_tensors
+        cfg, dt, n, st = ctx.cfg, ctx.dt, ctx.n, ctx.store
+        tapes = st.get(n, u_in)[0] if ctx.recorded else _record(cfg, dt, u_in, n, st)
+        lam = grad.detach().contiguous().clone()
+        for i in reversed(range(n)):
+            capi.adjoint_sweep(cfg, dt, tapes[i], lam, st.work)
+        return lam, None, None, None, None, None
+
+
 def rk3_step(u, cfg, dt, ws=None):
     """One differentiable FP64 SSP-RK3 step; ``ws`` is an optional adjoint
     workspace (``capi.adjoint_workspace``) reused by the backward."""
     return RK3Step.apply(u, cfg, dt, ws)
 
 
-def integrate(u, cfg, dt, nsteps, ws=None, checkpoint_every=None):
+def integrate(u, cfg, dt, nsteps, ws=None, checkpoint_every=None, tape=True):
     """``nsteps`` differentiable steps; one adjoint workspace is shared by
     every backward step (they run one after another on the stream).
 
-    With ``checkpoint_every = K`` only every K-th state is kept: the backward
-    of a segment of K steps first replays its forward steps from the
-    segment's first state (torch.utils.checkpoint), so memory holds
-    nsteps / K + K states instead of nsteps."""
+    With ``checkpoint_every = K`` only every K-th state is kept.  With
+    ``tape`` (the default) each segment is one TapedSegment node; without,
+    the backward of a segment replays its forward steps from the segment's
+    first state (torch.utils.checkpoint) and each step's adjoint recomputes
+    its stages again, so memory holds nsteps / K + K states and no tapes."""
+    if (checkpoint_every and checkpoint_every < nsteps and tape and torch.is_grad_enabled()
+            and u.requires_grad):
+        store = _Tapes(cfg, u.device)
+        done = 0
+        while done < nsteps:
+            n = min(checkpoint_every, nsteps - done)
+            u = TapedSegment.apply(u, cfg, dt, n, store, done + n == nsteps)
+            done += n
+        return u
     if ws is None and torch.is_grad_enabled() and u.requires_grad:
         ws = capi.adjoint_workspace(cfg, u.device)
     if not checkpoint_every or checkpoint_every >= nsteps or not torch.is_grad_enabled():

@@ -139,6 +139,10 @@ SIGNATURES = {
     "snapx_fused_rk3_steps_bg": (_I, [_CFG, _P, _D, _I, _P, _P, _Z, _P, _P]),
     "snapx_adjoint_workspace_bytes": (_Z, [_CFG]),
     "snapx_adjoint_rk3_step": (_I, [_CFG, _D, _P, _P, _P, _Z, _P]),
+    "snapx_adjoint_tape_bytes": (_Z, [_CFG]),
+    "snapx_adjoint_work_bytes": (_Z, [_CFG]),
+    "snapx_adjoint_record": (_I, [_CFG, _D, _P, _P, _P, _Z, _P, _Z, _P]),
+    "snapx_adjoint_sweep": (_I, [_CFG, _D, _P, _Z, _P, _P, _Z, _P]),
     "snapx_batch_workspace_bytes": (_Z, [_CFG, _I]),
     "snapx_batch_plan": (_I, [_CFG, _I, _D, _I, _I, _I, _P, _Z, _P]),
     "snapx_batch_rk3_steps": (_I, [_CFG, _P, _I, _P, _P, _Z, _P, _P]),
@@ -345,6 +349,29 @@ def adjoint_rk3_step(cfg, dt, u_in, lam, ws=None):
     ws = adjoint_workspace(cfg, u_in.device) if ws is None else ws
     check(lib().snapx_adjoint_rk3_step(C.byref(cfg), dt, ptr(u_in), ptr(lam), ptr(ws),
                                        ws.numel(), stream_ptr()), "adjoint_rk3_step")
+
+
+def adjoint_tape_bytes(cfg):
+    return lib().snapx_adjoint_tape_bytes(C.byref(cfg))
+
+
+def adjoint_work(cfg, device="cuda"):
+    return workspace(lib().snapx_adjoint_work_bytes(C.byref(cfg)), device)
+
+
+def adjoint_record(cfg, dt, u_in, u_out, tape, work):
+    """Forward recompute of one FP64 step from u_in into ``tape`` (a uint8
+    device tensor of adjoint_tape_bytes); u_out (or None) gets the step's
+    output."""
+    check(lib().snapx_adjoint_record(C.byref(cfg), dt, ptr(u_in), ptr(u_out), ptr(tape),
+                                     tape.numel(), ptr(work), work.numel(), stream_ptr()),
+          "adjoint_record")
+
+
+def adjoint_sweep(cfg, dt, tape, lam, work):
+    """lam <- (d u_out / d u_in)^T lam from a recorded tape."""
+    check(lib().snapx_adjoint_sweep(C.byref(cfg), dt, ptr(tape), tape.numel(), ptr(lam),
+                                    ptr(work), work.numel(), stream_ptr()), "adjoint_sweep")
 
 
 def fused_workspace(cfg, device="cuda"):

@@ -1189,20 +1189,21 @@ static AdjArgs adj_args(const snapx_config* c, double dt) {
 
 static size_t al(size_t x) { return (x + 255) & ~size_t(255); }
 
-struct AdjSpace {
-  double *us[3], *w[3], *dux[3], *lam, *lam_us, *lam_u0, *lam_du, *lam_dx, *lam_w, *lam_wlr,
-      *abc[3], *pr, *dl[3], *G, *G6;
-  char* hydro_ws;
-  size_t hydro_bytes, bytes;
+// What the reverse sweep of one step reads from its forward recompute: per
+// stage the input state, its primitives (revised ghosts), the explicit
+// tendency, and the VIC blocks and (delta, rows) of the column solve.
+struct AdjTape {
+  double *us[3], *w[3], *dux[3], *abc[3], *dl[3];
+  size_t bytes;
 };
 
-static AdjSpace adj_layout(const snapx_config* c, char* base) {
+static AdjTape tape_layout(const snapx_config* c, char* base) {
   Grid g = make_grid(c);
   size_t f = al(sizeof(double) * 5 * (size_t)g.ncell);
   int N = ((c->implicit_scheme >> 3) & 1) ? 5 : 3;
   size_t ncol = (size_t)(g.n2 > 1 ? c->nx2 : 1) * (g.n3 > 1 ? c->nx3 : 1);
   size_t cells = ncol * c->nx1;
-  AdjSpace s;
+  AdjTape t;
   size_t off = 0;
   auto take = [&](size_t b) {
     char* p = base + off;
@@ -1210,10 +1211,45 @@ static AdjSpace adj_layout(const snapx_config* c, char* base) {
     return reinterpret_cast<double*>(p);
   };
   for (int k = 0; k < 3; ++k) {
-    s.us[k] = take(f);
-    s.w[k] = take(f);
-    s.dux[k] = take(f);
+    t.us[k] = take(f);
+    t.w[k] = take(f);
+    t.dux[k] = take(f);
   }
+  for (int k = 0; k < 3; ++k) {
+    t.abc[k] = take(sizeof(double) * cells * 3 * N * N);
+    t.dl[k] = take(sizeof(double) * cells * 2 * N);
+  }
+  t.bytes = off;
+  return t;
+}
+
+struct AdjSpace {
+  double *us[3], *w[3], *dux[3], *lam, *lam_us, *lam_u0, *lam_du, *lam_dx, *lam_w, *lam_wlr,
+      *abc[3], *pr, *dl[3], *G, *G6;
+  char* hydro_ws;
+  size_t hydro_bytes, bytes;
+};
+
+// The work space of the recompute and the sweep (everything but the tape);
+// the tape fields of the result point into `tape`.
+static AdjSpace adj_layout(const snapx_config* c, char* base, char* tape) {
+  Grid g = make_grid(c);
+  size_t f = al(sizeof(double) * 5 * (size_t)g.ncell);
+  int N = ((c->implicit_scheme >> 3) & 1) ? 5 : 3;
+  size_t ncol = (size_t)(g.n2 > 1 ? c->nx2 : 1) * (g.n3 > 1 ? c->nx3 : 1);
+  size_t cells = ncol * c->nx1;
+  AdjSpace s;
+  AdjTape t = tape_layout(c, tape);
+  for (int k = 0; k < 3; ++k) {
+    s.us[k] = t.us[k], s.w[k] = t.w[k], s.dux[k] = t.dux[k];
+    s.abc[k] = t.abc[k], s.dl[k] = t.dl[k];
+  }
+  size_t off = 0;
+  auto take = [&](size_t b) {
+    char* p = base + off;
+    off += al(b);
+    return reinterpret_cast<double*>(p);
+  };
   s.lam = take(f);
   s.lam_us = take(f);
   s.lam_u0 = take(f);
@@ -1221,10 +1257,6 @@ static AdjSpace adj_layout(const snapx_config* c, char* base) {
   s.lam_dx = take(f);
   s.lam_w = take(f);
   s.lam_wlr = take(2 * f);
-  for (int k = 0; k < 3; ++k) {
-    s.abc[k] = take(sizeof(double) * cells * 3 * N * N);
-    s.dl[k] = take(sizeof(double) * cells * 2 * N);
-  }
   s.pr = take(sizeof(double) * cells * (N * N + N));
   s.G = take(sizeof(double) * ncol * (c->nx1 + 1) * 10);
   s.G6 = take(6 * f / 5);
@@ -1253,10 +1285,22 @@ static int bc_adj(const snapx_config* c, const AdjArgs a, double* lam, hipStream
 
 using namespace snapx;
 
-extern "C" size_t snapx_adjoint_workspace_bytes(const snapx_config* cfg) {
+extern "C" size_t snapx_adjoint_tape_bytes(const snapx_config* cfg) {
   if (validate_config(cfg)) return 0;
   if (cfg->coord != SNAPX_COORD_CARTESIAN) return 0;
-  return adj_layout(cfg, nullptr).bytes;
+  return tape_layout(cfg, nullptr).bytes;
+}
+
+extern "C" size_t snapx_adjoint_work_bytes(const snapx_config* cfg) {
+  if (validate_config(cfg)) return 0;
+  if (cfg->coord != SNAPX_COORD_CARTESIAN) return 0;
+  return adj_layout(cfg, nullptr, nullptr).bytes;
+}
+
+// one step's work space followed by its tape
+extern "C" size_t snapx_adjoint_workspace_bytes(const snapx_config* cfg) {
+  const size_t w = snapx_adjoint_work_bytes(cfg);
+  return w ? w + snapx_adjoint_tape_bytes(cfg) : 0;
 }
 
 namespace snapx {
@@ -1305,11 +1349,10 @@ static int launch_vic_adj(const AdjArgs& a, const AdjSpace& s, int k, int ncol, 
 
 }  // namespace snapx
 
-extern "C" int snapx_adjoint_rk3_step(const snapx_config* cfg, double dt, const double* u_in,
-                                      double* lambda, void* workspace, size_t workspace_bytes,
-                                      void* stream) {
+namespace snapx {
+
+static int adjoint_check(const snapx_config* cfg) {
   if (int e = validate_config(cfg)) return e;
-  SNAPX_CHECK_ARG(u_in && lambda && workspace, "snapx_adjoint_rk3_step: null pointer");
   SNAPX_CHECK_ARG(!cfg->limiter, "adjoint: the EOS limiter is not differentiable (turn it off)");
   SNAPX_CHECK_ARG(cfg->non_hydrostatic == 1.0, "adjoint: non-hydrostatic < 1 is not supported");
   SNAPX_CHECK_ARG(cfg->coord == SNAPX_COORD_CARTESIAN, "adjoint: Cartesian geometry only");
@@ -1317,31 +1360,35 @@ extern "C" int snapx_adjoint_rk3_step(const snapx_config* cfg, double dt, const 
     SNAPX_CHECK_ARG(cfg->bc[b] != SNAPX_BC_CUSTOM, "adjoint: custom boundaries are not supported");
   for (int d = 0; d < 3; ++d)
     SNAPX_CHECK_ARG(!cfg->disable_flux[d], "adjoint: disable-flux is not supported");
-  AdjSpace s = adj_layout(cfg, reinterpret_cast<char*>(workspace));
-  SNAPX_CHECK_ARG(workspace_bytes >= s.bytes, "snapx_adjoint_rk3_step: workspace too small");
-  hipStream_t st = as_stream(stream);
+  return 0;
+}
+
+static const double kStageW[3][3] = {
+    {0., 1., 1.}, {3. / 4., 1. / 4., 1. / 4.}, {1. / 3., 2. / 3., 2. / 3.}};
+
+// Forward recompute of one step from u_in into the tape: stage inputs, their
+// primitives (revised ghosts), explicit tendencies, VIC blocks and delta.
+// With u_out, also the step's output (third stage combined, boundaries).
+static int adjoint_record(const snapx_config* cfg, double dt, const double* u_in, double* u_out,
+                          const AdjSpace& s, hipStream_t st) {
   Grid g = make_grid(cfg);
   AdjArgs a = adj_args(cfg, dt);
   int64_t n = 5 * (int64_t)g.ncell;
   size_t fb = sizeof(double) * n;
   snapx_config c0 = *cfg;
   c0.implicit_scheme = 0;
-  static const double W[3][3] = {
-      {0., 1., 1.}, {3. / 4., 1. / 4., 1. / 4.}, {1. / 3., 2. / 3., 2. / 3.}};
-  dim3 bn = blocks_for(n, kT), bc = blocks_for(g.ncell, kT);
+  const auto& W = kStageW;
   const int N = ((cfg->implicit_scheme >> 3) & 1) ? 5 : 3;
   const int ncol = (g.n2 > 1 ? cfg->nx2 : 1) * (g.n3 > 1 ? cfg->nx3 : 1);
-  const int ns[3] = {g.n1, g.n2, g.n3};
-
-  // ---- forward recompute: stage inputs, their primitives (revised ghosts)
-  //      and explicit tendencies ----
+  TraceRange range("snapx adjoint record");
   SNAPX_HIP(hipMemcpyAsync(s.us[0], u_in, fb, hipMemcpyDeviceToDevice, st));
   for (int k = 0; k < 3; ++k) {
     if (int e = snapx_hydro_forward(&c0, dt, s.us[k], s.w[k], s.dux[k], nullptr, nullptr,
                                     nullptr, s.hydro_ws, s.hydro_bytes, SNAPX_F64, st))
       return e;
     // du of the stage: the column solve on a copy of the explicit tendency
-    // (stage 2 only needs the blocks and delta it leaves for the reverse sweep)
+    // (without u_out, the third stage's solve matters only for the blocks and
+    // delta it leaves for the reverse sweep)
     double* du = s.lam_du;
     SNAPX_HIP(hipMemcpyAsync(du, s.dux[k], fb, hipMemcpyDeviceToDevice, st));
     if (cfg->implicit_scheme) {
@@ -1349,20 +1396,36 @@ extern "C" int snapx_adjoint_rk3_step(const snapx_config* cfg, double dt, const 
         if (int e = N == 3 ? launch_vic_fwd<3>(a, s, k, du, ncol, st)
                            : launch_vic_fwd<5>(a, s, k, du, ncol, st))
           return e;
-      } else if (k < 2) {
+      } else if (k < 2 || u_out) {
         if (int e = snapx_hydro_forward(cfg, dt, s.us[k], s.lam_w, du, nullptr, nullptr,
                                         nullptr, s.hydro_ws, s.hydro_bytes, SNAPX_F64, st))
           return e;
       }
     }
-    if (k == 2) break;
-    if (int e = snapx_rk_combine(s.us[k + 1], s.us[0], s.us[k], du, W[k][0], W[k][1], W[k][2],
-                                 n, SNAPX_F64, st))
+    double* next = k < 2 ? s.us[k + 1] : u_out;
+    if (!next) break;
+    if (int e = snapx_rk_combine(next, s.us[0], s.us[k], du, W[k][0], W[k][1], W[k][2], n,
+                                 SNAPX_F64, st))
       return e;
-    if (int e = snapx_apply_bc(cfg, s.us[k + 1], 5, SNAPX_CONSERVED, SNAPX_F64, st)) return e;
+    if (int e = snapx_apply_bc(cfg, next, 5, SNAPX_CONSERVED, SNAPX_F64, st)) return e;
   }
+  return 0;
+}
 
-  // ---- reverse sweep ----
+// Reverse sweep of one step over its tape: lambda (dL/du_out) becomes
+// dL/du_in.
+static int adjoint_sweep(const snapx_config* cfg, double dt, double* lambda, const AdjSpace& s,
+                         hipStream_t st) {
+  Grid g = make_grid(cfg);
+  AdjArgs a = adj_args(cfg, dt);
+  int64_t n = 5 * (int64_t)g.ncell;
+  size_t fb = sizeof(double) * n;
+  const auto& W = kStageW;
+  dim3 bn = blocks_for(n, kT), bc = blocks_for(g.ncell, kT);
+  const int N = ((cfg->implicit_scheme >> 3) & 1) ? 5 : 3;
+  const int ncol = (g.n2 > 1 ? cfg->nx2 : 1) * (g.n3 > 1 ? cfg->nx3 : 1);
+  const int ns[3] = {g.n1, g.n2, g.n3};
+  TraceRange range("snapx adjoint sweep");
   SNAPX_HIP(hipMemcpyAsync(s.lam, lambda, fb, hipMemcpyDeviceToDevice, st));
   hipLaunchKernelGGL(zero_kernel, bn, dim3(kT), 0, st, s.lam_u0, n);
   SNAPX_LAUNCH_CHECK();
@@ -1442,4 +1505,47 @@ extern "C" int snapx_adjoint_rk3_step(const snapx_config* cfg, double dt, const 
   SNAPX_LAUNCH_CHECK();
   SNAPX_HIP(hipMemcpyAsync(lambda, s.lam, fb, hipMemcpyDeviceToDevice, st));
   return 0;
+}
+
+}  // namespace snapx
+
+extern "C" int snapx_adjoint_rk3_step(const snapx_config* cfg, double dt, const double* u_in,
+                                      double* lambda, void* workspace, size_t workspace_bytes,
+                                      void* stream) {
+  if (int e = adjoint_check(cfg)) return e;
+  SNAPX_CHECK_ARG(u_in && lambda && workspace, "snapx_adjoint_rk3_step: null pointer");
+  SNAPX_CHECK_ARG(workspace_bytes >= snapx_adjoint_workspace_bytes(cfg),
+                  "snapx_adjoint_rk3_step: workspace too small");
+  char* base = reinterpret_cast<char*>(workspace);
+  AdjSpace s = adj_layout(cfg, base, base + snapx_adjoint_work_bytes(cfg));
+  hipStream_t st = as_stream(stream);
+  if (int e = adjoint_record(cfg, dt, u_in, nullptr, s, st)) return e;
+  return adjoint_sweep(cfg, dt, lambda, s, st);
+}
+
+extern "C" int snapx_adjoint_record(const snapx_config* cfg, double dt, const double* u_in,
+                                    double* u_out, void* tape, size_t tape_bytes, void* work,
+                                    size_t work_bytes, void* stream) {
+  if (int e = adjoint_check(cfg)) return e;
+  SNAPX_CHECK_ARG(u_in && tape && work, "snapx_adjoint_record: null pointer");
+  SNAPX_CHECK_ARG(u_out != u_in, "snapx_adjoint_record: u_out must not alias u_in");
+  SNAPX_CHECK_ARG(tape_bytes >= snapx_adjoint_tape_bytes(cfg),
+                  "snapx_adjoint_record: tape too small");
+  SNAPX_CHECK_ARG(work_bytes >= snapx_adjoint_work_bytes(cfg),
+                  "snapx_adjoint_record: work space too small");
+  AdjSpace s = adj_layout(cfg, reinterpret_cast<char*>(work), reinterpret_cast<char*>(tape));
+  return adjoint_record(cfg, dt, u_in, u_out, s, as_stream(stream));
+}
+
+extern "C" int snapx_adjoint_sweep(const snapx_config* cfg, double dt, void* tape,
+                                   size_t tape_bytes, double* lambda, void* work,
+                                   size_t work_bytes, void* stream) {
+  if (int e = adjoint_check(cfg)) return e;
+  SNAPX_CHECK_ARG(lambda && tape && work, "snapx_adjoint_sweep: null pointer");
+  SNAPX_CHECK_ARG(tape_bytes >= snapx_adjoint_tape_bytes(cfg),
+                  "snapx_adjoint_sweep: tape too small");
+  SNAPX_CHECK_ARG(work_bytes >= snapx_adjoint_work_bytes(cfg),
+                  "snapx_adjoint_sweep: work space too small");
+  AdjSpace s = adj_layout(cfg, reinterpret_cast<char*>(work), reinterpret_cast<char*>(tape));
+  return adjoint_sweep(cfg, dt, lambda, s, as_stream(stream));
 }
