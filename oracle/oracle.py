@@ -23,7 +23,7 @@ ILT, IRT = 0, 1
 INTERP = {"dc": 0, "plm": 1, "cp3": 2, "cp5": 3, "weno3": 4, "weno5": 5}
 RIEMANN = {"roe": 0, "lmars": 1, "hllc": 2}
 BC = {None: -1, "none": -1, "reflecting": 0, "outflow": 1, "periodic": 2, "custom": 3}
-COORD = {"cartesian": 0, "spherical-polar": 1}
+COORD = {"cartesian": 0, "spherical-polar": 1, "gnomonic-equiangle": 2}
 PRIMITIVE, CONSERVED, SCALAR = 0, 1, 2
 
 
@@ -77,6 +77,8 @@ def lib():
         L.oc_interp_window.argtypes = [i, i, i, P]
         L.oc_cons2prim.argtypes = [cfgp, P, P, C.c_int64]
         L.oc_prim2cons.argtypes = [cfgp, P, P, C.c_int64]
+        L.oc_gno_prim2local.argtypes = [cfgp, P, i]
+        L.oc_gno_flux2global.argtypes = [cfgp, P, i]
         L.oc_reconstruct.argtypes = [P, P, i, i, i, i, i, i, i, i, i, i, d, d]
         L.oc_riemann.argtypes = [i, P, P, P, P, i, i, i, i, i, d]
         L.oc_divergence.argtypes = [P, P, P, P, i, i, i, i, i, P, P, P]
@@ -278,7 +280,9 @@ def coord_forward(c, w, f1, f2, f3, fp1=None):
 
 
 _GEOM = {"x1v": 0, "x2v": 1, "dx1v": 2, "vol": 3, "area1": 4, "area2": 5, "area3": 6,
-         "width1": 7, "width2": 8, "width3": 9}
+         "width1": 7, "width2": 8, "width3": 9, "cos_cell": 10, "sin_cell": 11,
+         "cos_face2": 12, "sin_face2": 13, "cos_face3": 14, "sin_face3": 15,
+         "x_src": 16, "y_src": 17}
 
 
 def geometry(c, what):
@@ -286,10 +290,27 @@ def geometry(c, what):
     (face arrays, one extra face along their axis) and the CFL widths."""
     n3, n2, n1 = shape(c)
     shp = {"x1v": (n1,), "x2v": (n2,), "dx1v": (n1,), "area1": (n3, n2, n1 + 1),
-           "area2": (n3, n2 + 1, n1), "area3": (n3 + 1, n2, n1)}.get(what, (n3, n2, n1))
+           "area2": (n3, n2 + 1, n1), "area3": (n3 + 1, n2, n1),
+           "cos_cell": (n3, n2), "sin_cell": (n3, n2), "cos_face2": (n3, n2 + 1),
+           "sin_face2": (n3, n2 + 1), "cos_face3": (n3 + 1, n2),
+           "sin_face3": (n3 + 1, n2)}.get(what, (n3, n2, n1))
     out = np.zeros(shp)
     lib().oc_geometry(C.byref(c), _GEOM[what], _p(out))
     return out
+
+
+def gno_prim2local(c, w, dim):
+    """Interface states of dim's faces in the gnomonic face frame (copy)."""
+    w = np.ascontiguousarray(w, dtype=np.float64).copy()
+    lib().oc_gno_prim2local(C.byref(c), _p(w), dim)
+    return w
+
+
+def gno_flux2global(c, f, dim):
+    """Face-frame fluxes of dim's faces back to the panel components (copy)."""
+    f = np.ascontiguousarray(f, dtype=np.float64).copy()
+    lib().oc_gno_flux2global(C.byref(c), _p(f), dim)
+    return f
 
 
 def hydro_forward(c, dt, u, w=None):
@@ -370,7 +391,8 @@ def vic_assemble(N, wcol, area, vol, dt, grav, gammad=1.4):
 
 PLANTS = {"c_jac_i": 1, "no_bottom_fold": 2, "phi_place": 3, "b_abs_plus": 4,
           "rinv_sign": 5, "fluxjac": 6, "roe_velocity": 7, "roe_upwind_swap": 8,
-          "roe_llf_speed": 9}
+          "roe_llf_speed": 9, "gno_flux_cos": 10, "gno_xov_sin": 11, "gno_no_raise": 12,
+          "gno_ang3": 13}
 
 
 class plant:

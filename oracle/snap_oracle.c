@@ -797,13 +797,22 @@ static void primitive_limiter(const oc_config* c, double* w, int64_t ncell) {
   }
 }
 
+/* cosine_cell_kj of a gnomonic panel ([n3][n2], caller frees), NULL for the
+   other geometries (whose cosine is zero: src/coord/cartesian.cpp:12-14,
+   src/coord/spherical_polar.cpp:52-57) */
+static double* cell_cosines(const oc_config* c);
+
 /* IdealGasImpl::_cons2prim (src/eos/ideal_gas.cpp:90-109) with the kernel
-   ideal_gas_cons2prim (src/eos/ideal_gas_impl.h:15-35).  The contravariant
-   raise with cos(theta) = 0 (Cartesian, src/coord/cartesian.cpp:12-14) is an
-   exact identity and is omitted. */
+   ideal_gas_cons2prim (src/eos/ideal_gas_impl.h:15-35): velocities are the
+   momenta over density raised to contravariant components with the cell's
+   cos(theta) (coord_vec_raise_impl, src/coord/coord_utils_impl.h:8-16).  With
+   cos(theta) = 0 (Cartesian, spherical-polar) the raise is an exact identity
+   and is skipped. */
 void oc_cons2prim(const oc_config* c, double* u, double* w, int64_t ncell) {
   conserved_limiter(c, u, ncell);
   double gammad = c->gammad;
+  double* cth = cell_cosines(c);
+  int n1 = oc_nc1(c), n2 = oc_nc2(c);
 #pragma omp parallel for num_threads(g_threads) schedule(static)
   for (int64_t q = 0; q < ncell; ++q) {
     const double* U = u + q;
@@ -813,18 +822,29 @@ void oc_cons2prim(const oc_config* c, double* u, double* w, int64_t ncell) {
     W[OC_IVX * s] = U[OC_IVX * s] / W[OC_IDN * s];
     W[OC_IVY * s] = U[OC_IVY * s] / W[OC_IDN * s];
     W[OC_IVZ * s] = U[OC_IVZ * s] / W[OC_IDN * s];
+    if (cth && oc_plant != OC_PLANT_GNO_NO_RAISE) {
+      double ct = cth[q / n1], v = W[OC_IVY * s], x = W[OC_IVZ * s];
+      double sth2 = 1. - ct * ct;
+      W[OC_IVY * s] = v / sth2 - x * ct / sth2;
+      W[OC_IVZ * s] = -v * ct / sth2 + x / sth2;
+    }
+    (void)n2;
     double ke = 0.5 * (W[OC_IVX * s] * U[OC_IVX * s] + W[OC_IVY * s] * U[OC_IVY * s] +
                        W[OC_IVZ * s] * U[OC_IVZ * s]);
     double ie = U[OC_IPR * s] - ke;
     W[OC_IPR * s] = (gammad - 1.) * ie;
   }
   primitive_limiter(c, w, ncell);
+  free(cth);
 }
 
-/* IdealGasImpl::_prim2cons (src/eos/ideal_gas.cpp:67-88) */
+/* IdealGasImpl::_prim2cons (src/eos/ideal_gas.cpp:67-88): momenta lowered to
+   covariant components (coord_vec_lower_impl, coord_utils_impl.h:18-24) */
 void oc_prim2cons(const oc_config* c, double* w, double* u, int64_t ncell) {
   primitive_limiter(c, w, ncell);
   double gammad = c->gammad;
+  double* cth = cell_cosines(c);
+  int n1 = oc_nc1(c);
   for (int64_t q = 0; q < ncell; ++q) {
     const double* W = w + q;
     double* U = u + q;
@@ -833,12 +853,18 @@ void oc_prim2cons(const oc_config* c, double* w, double* u, int64_t ncell) {
     U[OC_IVX * s] = W[OC_IVX * s] * W[OC_IDN * s];
     U[OC_IVY * s] = W[OC_IVY * s] * W[OC_IDN * s];
     U[OC_IVZ * s] = W[OC_IVZ * s] * W[OC_IDN * s];
+    if (cth) {
+      double ct = cth[q / n1], m2 = U[OC_IVY * s], m3 = U[OC_IVZ * s];
+      U[OC_IVY * s] = m2 + m3 * ct;
+      U[OC_IVZ * s] = m3 + m2 * ct;
+    }
     double ke = 0.5 * ((W[OC_IVX * s] * U[OC_IVX * s] + W[OC_IVY * s] * U[OC_IVY * s]) +
                        W[OC_IVZ * s] * U[OC_IVZ * s]);
     U[OC_IPR * s] = W[OC_IPR * s] / (gammad - 1);
     U[OC_IPR * s] += ke;
   }
   conserved_limiter(c, u, ncell);
+  free(cth);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -909,12 +935,116 @@ typedef struct {
   double *area3; /* [n3 + 1][n2][n1] */
   double *vol;   /* [n3][n2][n1] */
   double *src1_i, *src2_i, *src1_j, *src2_j; /* coord_src*_i (n1), *_j (n2) */
+  /* gnomonic-equiangle panels: the angle between the x2 and x3 coordinate
+     lines at cell centres [n3][n2], x2 faces [n3][n2 + 1] and x3 faces
+     [n3 + 1][n2]; the x2 / x3 source coefficients and CFL widths [n3][n2][n1] */
+  double *cos_c, *sin_c, *cos_f2, *sin_f2, *cos_f3, *sin_f3;
+  double *xov, *yov, *w2, *w3;
 } oc_geom;
 
+/* GnomonicEquiangleImpl::reset (src/coord/gnomonic_equiangle.cpp:18-118) and
+   its face areas and volume (:161-184): x1 = r with uniform widths, x2 and x3
+   the equiangular panel coordinates (radians, tangents x = tan x2, y = tan x3).
+   C = sqrt(1 + x^2), D = sqrt(1 + y^2); the coordinate lines meet at an angle
+   with cosine -x y / (C D) and sine sqrt(1 + x^2 + y^2) / (C D); the angular
+   widths of a cell are the angles between its two bounding coordinate lines.
+   Every expression keeps the reference's broadcasting order. */
+static double gno_ang2(double x1, double x2, double y) { /* dx2f_ang (:84-96) */
+  double d1 = sqrt(1. + x1 * x1 + y * y), d2 = sqrt(1. + x2 * x2 + y * y);
+  return acos((1. + x1 * x2 + y * y) / (d1 * d2));
+}
+static double gno_ang3(double x, double y1, double y2) { /* dx3f_ang (:98-110) */
+  double d1 = sqrt(1. + x * x + y1 * y1), d2 = sqrt(1. + x * x + y2 * y2);
+  if (oc_plant == OC_PLANT_GNO_ANG3) return acos((1. + x * x + y1 * y1) / (d1 * d2));
+  return acos((1. + x * x + y1 * y2) / (d1 * d2));
+}
+
+static void gnomonic_init(const oc_config* c, oc_geom* g) {
+  int n1 = g->n1, n2 = g->n2, n3 = g->n3;
+  oc_cartesian_widths(c, g->dx1f, g->dx2f, g->dx3f); /* ones(nc) * dx */
+  double* x3v = malloc(sizeof(double) * n3);
+  for (int i = 0; i < n1; ++i) g->x1v[i] = 0.5 * (g->x1f[i] + g->x1f[i + 1]);
+  for (int j = 0; j < n2; ++j) g->x2v[j] = 0.5 * (g->x2f[j] + g->x2f[j + 1]);
+  for (int k = 0; k < n3; ++k) x3v[k] = 0.5 * (g->x3f[k] + g->x3f[k + 1]);
+  for (int i = 0; i < n1; ++i) g->dx1v[i] = g->dx1f[i];
+  double* a2 = malloc(sizeof(double) * n3 * n2);        /* dx2f_ang_kj */
+  double* a3 = malloc(sizeof(double) * n3 * n2);        /* dx3f_ang_kj */
+  double* a2f3 = malloc(sizeof(double) * (n3 + 1) * n2); /* dx2f_ang_face3_kj */
+  double* a3f2 = malloc(sizeof(double) * n3 * (n2 + 1)); /* dx3f_ang_face2_kj */
+  for (int k = 0; k <= n3; ++k)
+    for (int j = 0; j <= n2; ++j) {
+      if (k < n3 && j < n2) {
+        double x = tan(g->x2v[j]), y = tan(x3v[k]);
+        double Cc = sqrt(1.0 + x * x), Dc = sqrt(1.0 + y * y);
+        g->cos_c[k * n2 + j] = -x * y / (Cc * Dc);
+        g->sin_c[k * n2 + j] = sqrt(1.0 + x * x + y * y) / (Cc * Dc);
+        a2[k * n2 + j] = gno_ang2(tan(g->x2f[j]), tan(g->x2f[j + 1]), y);
+        a3[k * n2 + j] = gno_ang3(x, tan(g->x3f[k]), tan(g->x3f[k + 1]));
+      }
+      if (k < n3) { /* x2 face j of row k */
+        double xf = tan(g->x2f[j]), y = tan(x3v[k]);
+        double Cf = sqrt(1.0 + xf * xf), Dc = sqrt(1.0 + y * y);
+        g->cos_f2[k * (n2 + 1) + j] = -xf * y / (Cf * Dc);
+        g->sin_f2[k * (n2 + 1) + j] = sqrt(1.0 + xf * xf + y * y) / (Cf * Dc);
+        a3f2[k * (n2 + 1) + j] = gno_ang3(xf, tan(g->x3f[k]), tan(g->x3f[k + 1]));
+      }
+      if (j < n2) { /* x3 face k of column j */
+        double x = tan(g->x2v[j]), yf = tan(g->x3f[k]);
+        double Cc = sqrt(1.0 + x * x), Df = sqrt(1.0 + yf * yf);
+        g->cos_f3[k * n2 + j] = -x * yf / (Cc * Df);
+        g->sin_f3[k * n2 + j] = sqrt(1.0 + x * x + yf * yf) / (Cc * Df);
+        a2f3[k * n2 + j] = gno_ang2(tan(g->x2f[j]), tan(g->x2f[j + 1]), yf);
+      }
+    }
+  for (int k = 0; k < n3; ++k)
+    for (int j = 0; j < n2; ++j) {
+      double ang = a2[k * n2 + j] * a3[k * n2 + j] * g->sin_c[k * n2 + j];
+      for (int i = 0; i <= n1; ++i) /* face_area1 (:165-168) */
+        g->area1[((long)k * n2 + j) * (n1 + 1) + i] = g->x1f[i] * g->x1f[i] * ang;
+    }
+  for (int k = 0; k < n3; ++k)
+    for (int j = 0; j <= n2; ++j)
+      for (int i = 0; i < n1; ++i) /* face_area2 (:170-172) */
+        g->area2[((long)k * (n2 + 1) + j) * n1 + i] =
+            g->x1v[i] * g->dx1f[i] * a3f2[k * (n2 + 1) + j];
+  for (int k = 0; k <= n3; ++k)
+    for (int j = 0; j < n2; ++j)
+      for (int i = 0; i < n1; ++i) /* face_area3 (:174-176) */
+        g->area3[((long)k * n2 + j) * n1 + i] = g->x1v[i] * g->dx1f[i] * a2f3[k * n2 + j];
+  for (int k = 0; k < n3; ++k)
+    for (int j = 0; j < n2; ++j)
+      for (int i = 0; i < n1; ++i) {
+        long q = ((long)k * n2 + j) * n1 + i;
+        long f1 = ((long)k * n2 + j) * (n1 + 1) + i;
+        /* cell_volume (:178-184) */
+        double vol = 0.5 * (g->area1[f1] + g->area1[f1 + 1]) * g->dx1f[i];
+        g->vol[q] = vol;
+        /* x_ov_rD_kji, y_ov_rC_kji (:112-118) */
+        long f2 = ((long)k * (n2 + 1) + j) * n1 + i;
+        g->xov[q] = (g->area2[f2 + n1] * g->sin_f2[k * (n2 + 1) + j + 1] -
+                     g->area2[f2] * g->sin_f2[k * (n2 + 1) + j]) /
+                    vol;
+        if (oc_plant == OC_PLANT_GNO_XOV_SIN)
+          g->xov[q] = (g->area2[f2 + n1] - g->area2[f2]) * g->sin_c[k * n2 + j] / vol;
+        g->yov[q] = (g->area3[q + (long)n1 * n2] * g->sin_f3[(k + 1) * n2 + j] -
+                     g->area3[q] * g->sin_f3[k * n2 + j]) /
+                    vol;
+        /* center_width2 / center_width3 (:157-163) */
+        g->w2[q] = g->x1v[i] * a2[k * n2 + j];
+        g->w3[q] = g->x1v[i] * a3[k * n2 + j];
+      }
+  free(x3v);
+  free(a2);
+  free(a3);
+  free(a2f3);
+  free(a3f2);
+}
+
 static void geom_free(oc_geom* g) {
-  double* p[] = {g->x1f, g->x2f, g->x3f, g->dx1f, g->dx2f, g->dx3f, g->x1v, g->x2v, g->dx1v,
-                 g->area1, g->area2, g->area3, g->vol, g->src1_i, g->src2_i, g->src1_j,
-                 g->src2_j};
+  double* p[] = {g->x1f,    g->x2f,    g->x3f,    g->dx1f,   g->dx2f,  g->dx3f,  g->x1v,
+                 g->x2v,    g->dx1v,   g->area1,  g->area2,  g->area3, g->vol,   g->src1_i,
+                 g->src2_i, g->src1_j, g->src2_j, g->cos_c,  g->sin_c, g->cos_f2, g->sin_f2,
+                 g->cos_f3, g->sin_f3, g->xov,    g->yov,    g->w2,    g->w3};
   for (size_t i = 0; i < sizeof(p) / sizeof(p[0]); ++i) free(p[i]);
 }
 
@@ -938,9 +1068,23 @@ static void geom_init(const oc_config* c, oc_geom* g) {
   g->src2_i = malloc(sizeof(double) * n1);
   g->src1_j = malloc(sizeof(double) * n2);
   g->src2_j = malloc(sizeof(double) * n2);
+  g->cos_c = calloc((size_t)n3 * n2, sizeof(double));
+  g->sin_c = calloc((size_t)n3 * n2, sizeof(double));
+  g->cos_f2 = calloc((size_t)n3 * (n2 + 1), sizeof(double));
+  g->sin_f2 = calloc((size_t)n3 * (n2 + 1), sizeof(double));
+  g->cos_f3 = calloc((size_t)(n3 + 1) * n2, sizeof(double));
+  g->sin_f3 = calloc((size_t)(n3 + 1) * n2, sizeof(double));
+  g->xov = calloc((size_t)n3 * n2 * n1, sizeof(double));
+  g->yov = calloc((size_t)n3 * n2 * n1, sizeof(double));
+  g->w2 = calloc((size_t)n3 * n2 * n1, sizeof(double));
+  g->w3 = calloc((size_t)n3 * n2 * n1, sizeof(double));
   axis_faces(c->x1min, c->x1max, c->nx1, c->nghost, g->x1f);
   axis_faces(c->x2min, c->x2max, c->nx2, c->nghost, g->x2f);
   axis_faces(c->x3min, c->x3max, c->nx3, c->nghost, g->x3f);
+  if (c->coord == OC_COORD_GNOMONIC_EQUIANGLE) {
+    gnomonic_init(c, g);
+    return;
+  }
   if (c->coord != OC_COORD_SPHERICAL_POLAR) {
     /* Cartesian (src/coord/cartesian.cpp:8-47): uniform widths, midpoints */
     oc_cartesian_widths(c, g->dx1f, g->dx2f, g->dx3f);
@@ -1018,6 +1162,91 @@ static void geom_init(const oc_config* c, oc_geom* g) {
             0.5 * (g->x1f[i + 1] * g->x1f[i + 1] - g->x1f[i] * g->x1f[i]) * g->dx2f[j];
 }
 
+static double* cell_cosines(const oc_config* c) {
+  if (c->coord != OC_COORD_GNOMONIC_EQUIANGLE) return NULL;
+  oc_geom g;
+  geom_init(c, &g);
+  double* out = malloc(sizeof(double) * g.n3 * g.n2);
+  memcpy(out, g.cos_c, sizeof(double) * g.n3 * g.n2);
+  geom_free(&g);
+  return out;
+}
+
+/* GnomonicEquiangleImpl::prim2local{1,2,3}_ (src/coord/gnomonic_equiangle.cpp:
+   236-279): the interface states of dim's faces (3: x1, with the cell's angle;
+   2: x2 faces; 1: x3 faces) in an orthonormal frame of the face, as the LMARS
+   and HLLC solvers take them (src/riemann/lmars.cpp:54-68,
+   src/riemann/hllc.cpp:56-70) */
+static void gno_prim2local(const oc_geom* g, double* w, int dim) {
+  int n1 = g->n1, n2 = g->n2, n3 = g->n3;
+  long ncell = (long)n3 * n2 * n1;
+  double *v2 = w + OC_IVY * ncell, *v3 = w + OC_IVZ * ncell;
+  for (long q = 0; q < ncell; ++q) {
+    int j = (int)((q / n1) % n2), k = (int)(q / ((long)n1 * n2));
+    if (dim == 3) {
+      double ct = g->cos_c[k * n2 + j], st = g->sin_c[k * n2 + j];
+      v2[q] += v3[q] * ct;
+      v3[q] *= st;
+    } else if (dim == 2) {
+      double ct = g->cos_f2[k * (n2 + 1) + j], st = g->sin_f2[k * (n2 + 1) + j];
+      double T22 = 1.0 / sqrt(1. / (st * st)), a = v2[q], b = v3[q];
+      v3[q] = ct * a + b;
+      v2[q] = T22 * a;
+    } else {
+      double ct = g->cos_f3[k * n2 + j], st = g->sin_f3[k * n2 + j];
+      double T33 = 1.0 / sqrt(1. / (st * st)), a = v2[q], b = v3[q];
+      v2[q] = a + ct * b;
+      v3[q] = T33 * b;
+    }
+  }
+}
+
+/* GnomonicEquiangleImpl::flux2global{1,2,3}_ (:281-360): face fluxes back to
+   the panel's covariant momentum components */
+static void gno_flux2global(const oc_geom* g, double* f, int dim) {
+  int n1 = g->n1, n2 = g->n2, n3 = g->n3;
+  long ncell = (long)n3 * n2 * n1;
+  double *f2 = f + OC_IVY * ncell, *f3 = f + OC_IVZ * ncell;
+  for (long q = 0; q < ncell; ++q) {
+    int j = (int)((q / n1) % n2), k = (int)(q / ((long)n1 * n2));
+    double ct, ty, tz;
+    if (dim == 3) {
+      ct = g->cos_c[k * n2 + j];
+      tz = f3[q] / g->sin_c[k * n2 + j];
+      ty = f2[q] - tz * ct;
+    } else if (dim == 2) {
+      ct = g->cos_f2[k * (n2 + 1) + j];
+      double st = g->sin_f2[k * (n2 + 1) + j];
+      double T22 = sqrt(1. / (st * st)), T32 = -T22 * ct / 1.0;
+      if (oc_plant == OC_PLANT_GNO_FLUX_COS) T32 = -T32;
+      tz = T32 * f2[q] + f3[q];
+      ty = T22 * f2[q];
+    } else {
+      ct = g->cos_f3[k * n2 + j];
+      double st = g->sin_f3[k * n2 + j];
+      double T33 = sqrt(1. / (st * st)), T23 = -ct / 1.0 * T33;
+      ty = f2[q] + T23 * f3[q];
+      tz = T33 * f3[q];
+    }
+    f2[q] = ty + tz * ct;
+    f3[q] = tz + ty * ct;
+  }
+}
+
+/* the face frames on their own, for checks: w (or f) is [5][n3][n2][n1] */
+void oc_gno_prim2local(const oc_config* c, double* w, int dim) {
+  oc_geom g;
+  geom_init(c, &g);
+  gno_prim2local(&g, w, dim);
+  geom_free(&g);
+}
+void oc_gno_flux2global(const oc_config* c, double* f, int dim) {
+  oc_geom g;
+  geom_init(c, &g);
+  gno_flux2global(&g, f, dim);
+  geom_free(&g);
+}
+
 #define A1(k, j, i) g.area1[((long)(k) * n2 + (j)) * (n1 + 1) + (i)]
 #define A2(k, j, i) g.area2[((long)(k) * (n2 + 1) + (j)) * n1 + (i)]
 #define A3(k, j, i) g.area3[((long)(k) * n2 + (j)) * n1 + (i)]
@@ -1089,11 +1318,45 @@ void oc_coord_forward(const oc_config* c, const double* w, const double* f1,
           }
         }
   }
+  if (c->coord == OC_COORD_GNOMONIC_EQUIANGLE) {
+    /* GnomonicEquiangleImpl::forward (src/coord/gnomonic_equiangle.cpp:362-418),
+       dry gas: centrifugal and pressure terms of the radial momentum, and
+       the metric terms of the two panel momenta */
+    const double *rho = w + OC_IDN * ncell, *v1 = w + OC_IVX * ncell;
+    const double *v2 = w + OC_IVY * ncell, *v3 = w + OC_IVZ * ncell, *pr = w + OC_IPR * ncell;
+    for (int k = 0; k < n3; ++k)
+      for (int j = 0; j < n2; ++j)
+        for (int i = 0; i < n1; ++i) {
+          long q = ((long)k * n2 + j) * n1 + i;
+          double cosine = g.cos_c[k * n2 + j];
+          double sine2 = g.sin_c[k * n2 + j] * g.sin_c[k * n2 + j];
+          double radius = g.x1v[i];
+          double v_2 = v2[q] + v3[q] * cosine, v_3 = v3[q] + v2[q] * cosine;
+          double src1 = rho[q] * (v2[q] * v_2 + v3[q] * v_3) / radius;
+          if (fp1) {
+            if (i >= si && i < ei)
+              src1 += (A1(k, j, i + 1) * fp1[q + 1] - A1(k, j, i) * fp1[q]) / g.vol[q] -
+                      (fp1[q + 1] - fp1[q]) / g.dx1f[i];
+          } else {
+            src1 += 2.0 * pr[q] / radius;
+          }
+          div[OC_IVX * ncell + q] -= src1;
+          double src2 = g.xov[q] * (pr[q] + rho[q] * v3[q] * v3[q] * sine2) -
+                        rho[q] * v1[q] * v_2 / radius;
+          div[OC_IVY * ncell + q] -= src2;
+          double src3 = g.yov[q] * (pr[q] + rho[q] * v2[q] * v2[q] * sine2) -
+                        rho[q] * v1[q] * v_3 / radius;
+          div[OC_IVZ * ncell + q] -= src3;
+        }
+  }
   geom_free(&g);
 }
 
 /* geometry arrays for checks: what = 0 x1v, 1 x2v, 2 dx1v, 3 cell volume,
-   4/5/6 face areas of x1/x2/x3, 7/8/9 the CFL widths of x1/x2/x3 (cells) */
+   4/5/6 face areas of x1/x2/x3, 7/8/9 the CFL widths of x1/x2/x3 (cells);
+   gnomonic panels: 10/11 cos/sin of the cell angle [n3][n2], 12/13 of the x2
+   faces [n3][n2 + 1], 14/15 of the x3 faces [n3 + 1][n2], 16/17 the x2 / x3
+   metric source coefficients (cells) */
 void oc_geometry(const oc_config* c, int what, double* out) {
   oc_geom g;
   geom_init(c, &g);
@@ -1107,15 +1370,25 @@ void oc_geometry(const oc_config* c, int what, double* out) {
     case 4: memcpy(out, g.area1, sizeof(double) * n3 * n2 * (n1 + 1)); break;
     case 5: memcpy(out, g.area2, sizeof(double) * n3 * (n2 + 1) * n1); break;
     case 6: memcpy(out, g.area3, sizeof(double) * (n3 + 1) * n2 * n1); break;
+    case 10: memcpy(out, g.cos_c, sizeof(double) * n3 * n2); break;
+    case 11: memcpy(out, g.sin_c, sizeof(double) * n3 * n2); break;
+    case 12: memcpy(out, g.cos_f2, sizeof(double) * n3 * (n2 + 1)); break;
+    case 13: memcpy(out, g.sin_f2, sizeof(double) * n3 * (n2 + 1)); break;
+    case 14: memcpy(out, g.cos_f3, sizeof(double) * (n3 + 1) * n2); break;
+    case 15: memcpy(out, g.sin_f3, sizeof(double) * (n3 + 1) * n2); break;
+    case 16: memcpy(out, g.xov, sizeof(double) * n); break;
+    case 17: memcpy(out, g.yov, sizeof(double) * n); break;
     default:
       for (int k = 0; k < n3; ++k)
         for (int j = 0; j < n2; ++j)
           for (int i = 0; i < n1; ++i) {
             long q = ((long)k * n2 + j) * n1 + i;
             int sph = c->coord == OC_COORD_SPHERICAL_POLAR;
+            int gno = c->coord == OC_COORD_GNOMONIC_EQUIANGLE;
             if (what == 7) out[q] = g.dx1f[i];
-            else if (what == 8) out[q] = sph ? g.x1v[i] * g.dx2f[j] : g.dx2f[j];
-            else out[q] = sph ? g.x1v[i] * fabs(sin(g.x2v[j])) * g.dx3f[k] : g.dx3f[k];
+            else if (what == 8) out[q] = gno ? g.w2[q] : sph ? g.x1v[i] * g.dx2f[j] : g.dx2f[j];
+            else out[q] = gno ? g.w3[q]
+                              : sph ? g.x1v[i] * fabs(sin(g.x2v[j])) * g.dx3f[k] : g.dx3f[k];
           }
   }
   geom_free(&g);
@@ -1634,7 +1907,7 @@ void oc_vic_redistribute_cell(int N, double* du, const double* w,
 static void column_geometry(const oc_config* c, int k, int j, double* area,
                             double* vol, double* dx1f) {
   int n1 = oc_nc1(c), n2 = oc_nc2(c), n3 = oc_nc3(c);
-  if (c->coord == OC_COORD_SPHERICAL_POLAR) {
+  if (c->coord != OC_COORD_CARTESIAN) {
     oc_geom G;
     geom_init(c, &G);
     for (int i = 0; i <= c->nx1; ++i) area[i] = G.area1[((long)k * n2 + j) * (n1 + 1) + c->nghost + i];
@@ -1689,7 +1962,7 @@ void oc_implicit_correction(const oc_config* c, double dt, double* du,
   double* dx2f = malloc(sizeof(double) * n2);
   double* dx3f = malloc(sizeof(double) * n3);
   oc_cartesian_widths(c, dx1f, dx2f, dx3f);
-  int sph = c->coord == OC_COORD_SPHERICAL_POLAR;
+  int sph = c->coord != OC_COORD_CARTESIAN;
   oc_geom G;
   geom_init(c, &G);
   long ncol = (long)(ku - kl) * (ju - jl);
@@ -1742,6 +2015,24 @@ void oc_implicit_correction(const oc_config* c, double dt, double* du,
 /* hydro RHS: HydroImpl::forward (src/hydro/hydro_forward.cpp:14-217)         */
 /* ------------------------------------------------------------------------ */
 
+/* RiemannSolverImpl::forward of one direction; on gnomonic panels the LMARS
+   and HLLC solvers work in the face's orthonormal frame (prim2local before,
+   flux2global after: src/riemann/lmars.cpp:54-99, src/riemann/hllc.cpp:56-103);
+   the Roe solver takes the states as they are (src/riemann/roe.cpp:14-59) */
+static void riemann_in_frame(const oc_config* c, const oc_geom* G, int gno, double* wlr,
+                             double* F, double* fp, int dim, int axis) {
+  int nvar = 5, n1 = G->n1, n2 = G->n2, n3 = G->n3;
+  long ncell = (long)n3 * n2 * n1;
+  int solver = c->riemann[axis];
+  int frame = gno && solver != OC_RIEMANN_ROE;
+  if (frame) {
+    gno_prim2local(G, wlr, dim);
+    gno_prim2local(G, wlr + nvar * ncell, dim);
+  }
+  oc_riemann(solver, wlr, wlr + nvar * ncell, F, fp, nvar, n3, n2, n1, dim, c->gammad);
+  if (frame) gno_flux2global(G, F, dim);
+}
+
 void oc_hydro_forward(const oc_config* c, double dt, double* u, double* w,
                       double* du, double* f1out, double* f2out,
                       double* f3out) {
@@ -1753,7 +2044,8 @@ void oc_hydro_forward(const oc_config* c, double dt, double* u, double* w,
   double* dx2f = malloc(sizeof(double) * n2);
   double* dx3f = malloc(sizeof(double) * n3);
   oc_cartesian_widths(c, dx1f, dx2f, dx3f);
-  int sph = c->coord == OC_COORD_SPHERICAL_POLAR;
+  int sph = c->coord != OC_COORD_CARTESIAN;
+  int gno = c->coord == OC_COORD_GNOMONIC_EQUIANGLE;
   oc_geom G;
   geom_init(c, &G);
   /* x1 face pressure (src/hydro/hydro.cpp:88-95: zeros until the x1 solve) */
@@ -1785,8 +2077,7 @@ void oc_hydro_forward(const oc_config* c, double dt, double* u, double* w,
           rho_grav[p * n1 + i] = (pl[p * n1 + i + 1] - pr[p * n1 + i]) / G.dx1f[i];
     }
     if (!c->disable_flux[0])
-      oc_riemann(c->riemann[0], wlr, wlr + nvar * ncell, F1, fp1, nvar, n3, n2, n1, 3,
-                 c->gammad);
+      riemann_in_frame(c, &G, gno, wlr, F1, fp1, 3, 0);
   }
   /* (3, 4) x2, x3 */
   if (n2 > 1) {
@@ -1794,16 +2085,14 @@ void oc_hydro_forward(const oc_config* c, double dt, double* u, double* w,
                    c->recon_shock[1], 1, c->limiter, c->density_floor,
                    c->pressure_floor);
     if (!c->disable_flux[1])
-      oc_riemann(c->riemann[1], wlr, wlr + nvar * ncell, F2, NULL, nvar, n3, n2, n1, 2,
-                 c->gammad);
+      riemann_in_frame(c, &G, gno, wlr, F2, NULL, 2, 1);
   }
   if (n3 > 1) {
     oc_reconstruct(w, wlr, nvar, n3, n2, n1, 1, c->recon_type[1], c->recon_scale[1],
                    c->recon_shock[1], 1, c->limiter, c->density_floor,
                    c->pressure_floor);
     if (!c->disable_flux[2])
-      oc_riemann(c->riemann[2], wlr, wlr + nvar * ncell, F3, NULL, nvar, n3, n2, n1, 1,
-                 c->gammad);
+      riemann_in_frame(c, &G, gno, wlr, F3, NULL, 1, 2);
   }
 
   /* (5) divergence, and the geometric sources of spherical-polar blocks */

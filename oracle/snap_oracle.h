@@ -73,9 +73,13 @@ typedef struct {
 
 #define OC_COORD_CARTESIAN 0
 #define OC_COORD_SPHERICAL_POLAR 1
+#define OC_COORD_GNOMONIC_EQUIANGLE 2
 
 /* ---- grid helpers ---- */
 int oc_nc1(const oc_config* c);
+/* gnomonic face frames (src/coord/gnomonic_equiangle.cpp:236-360) */
+void oc_gno_prim2local(const oc_config* c, double* w, int dim);
+void oc_gno_flux2global(const oc_config* c, double* f, int dim);
 int oc_nc2(const oc_config* c);
 int oc_nc3(const oc_config* c);
 /* Cartesian face widths dx?f (length nc?) (src/coord/cartesian.cpp:8-47) */
@@ -191,7 +195,11 @@ enum {
   OC_PLANT_FLUXJAC = 6,        /* v1*v1 -> v1*v2 in one flux-Jacobian entry */
   OC_PLANT_ROE_VELOCITY = 7,   /* arithmetic in place of sqrt(rho)-weighted Roe velocity */
   OC_PLANT_ROE_UPWIND_SWAP = 8, /* Roe supersonic switches take the downwind flux */
-  OC_PLANT_ROE_LLF_SPEED = 9   /* LLF speed without the factor 1/2 */
+  OC_PLANT_ROE_LLF_SPEED = 9,  /* LLF speed without the factor 1/2 */
+  OC_PLANT_GNO_FLUX_COS = 10,  /* gnomonic x2-face flux2global with +cos in T32 */
+  OC_PLANT_GNO_XOV_SIN = 11,   /* x2 metric source with the cell sine for the face sine */
+  OC_PLANT_GNO_NO_RAISE = 12,  /* cons2prim without the contravariant raise */
+  OC_PLANT_GNO_ANG3 = 13       /* dx3f_ang with y1 * y1 for y1 * y2 */
 };
 
 void oc_set_threads(int n);
