@@ -83,6 +83,10 @@ extern "C" {
 /* coordinate systems (src/coord/coordinate.cpp:437-452) */
 #define SNAPX_COORD_CARTESIAN 0
 #define SNAPX_COORD_SPHERICAL_POLAR 1
+/* one panel of the cubed sphere with external boundaries: x1 = r, x2 and x3
+   equiangular panel coordinates in radians with nx2 = nx3
+   (src/coord/gnomonic_equiangle.cpp); plain path, limiter off */
+#define SNAPX_COORD_GNOMONIC_EQUIANGLE 2
 
 /* variable kinds for boundary functions (src/snap.h:62-71) */
 #define SNAPX_PRIMITIVE 0
@@ -110,7 +114,8 @@ typedef struct snapx_config {
   int disable_flux[3];
   /* x1-inner, x1-outer, x2-inner, x2-outer, x3-inner, x3-outer */
   int bc[6];
-  /* geometry.type: SNAPX_COORD_*; spherical-polar needs x2 in [0, pi] */
+  /* geometry.type: SNAPX_COORD_*; spherical-polar needs x2 in [0, pi];
+     gnomonic-equiangle needs nx2 = nx3 and |x2|, |x3| < pi / 2 with ghosts */
   int coord;
 } snapx_config;
 

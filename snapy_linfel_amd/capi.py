@@ -19,7 +19,7 @@ F32, F64 = 0, 1
 INTERP = {"dc": 0, "plm": 1, "cp3": 2, "cp5": 3, "weno3": 4, "weno5": 5}
 RIEMANN = {"roe": 0, "lmars": 1, "hllc": 2}
 BC = {None: -1, "none": -1, "reflecting": 0, "outflow": 1, "periodic": 2, "custom": 3}
-COORD = {"cartesian": 0, "spherical-polar": 1}
+COORD = {"cartesian": 0, "spherical-polar": 1, "gnomonic-equiangle": 2}
 PRIMITIVE, CONSERVED, SCALAR = 0, 1, 2
 
 

@@ -41,8 +41,22 @@ int validate_config(const snapx_config* c) {
   for (int b = 0; b < 6; ++b)
     SNAPX_CHECK_ARG(c->bc[b] >= SNAPX_BC_NONE && c->bc[b] <= SNAPX_BC_CUSTOM,
                     "Boundary function is not defined");
-  SNAPX_CHECK_ARG(c->coord == SNAPX_COORD_CARTESIAN || c->coord == SNAPX_COORD_SPHERICAL_POLAR,
+  SNAPX_CHECK_ARG(c->coord == SNAPX_COORD_CARTESIAN || c->coord == SNAPX_COORD_SPHERICAL_POLAR ||
+                      c->coord == SNAPX_COORD_GNOMONIC_EQUIANGLE,
                   "Unknown coordinate type");
+  if (c->coord == SNAPX_COORD_GNOMONIC_EQUIANGLE) {
+    // src/coord/gnomonic_equiangle.cpp:21-23
+    SNAPX_CHECK_ARG(c->nx2 == c->nx3, "GnomonicEquiangleImpl::reset(): nx2 must equal nx3");
+    SNAPX_CHECK_ARG(c->nx1 > 1 && c->nx2 > 1,
+                    "gnomonic-equiangle panels need active x1, x2 and x3 axes");
+    const double hw = 0.5 * M_PI;
+    const double d2 = (c->x2max - c->x2min) / c->nx2, d3 = (c->x3max - c->x3min) / c->nx3;
+    SNAPX_CHECK_ARG(c->x2min - c->nghost * d2 > -hw && c->x2max + c->nghost * d2 < hw &&
+                        c->x3min - c->nghost * d3 > -hw && c->x3max + c->nghost * d3 < hw,
+                    "gnomonic-equiangle panel coordinates (ghosts included) must lie in "
+                    "(-pi/2, pi/2)");
+    SNAPX_CHECK_ARG(!c->limiter, "gnomonic-equiangle panels: the EOS limiter is not supported");
+  }
   if (c->coord == SNAPX_COORD_SPHERICAL_POLAR) {
     // src/coord/spherical_polar.cpp:46-48
     SNAPX_CHECK_ARG(c->nx2 <= 1 || (c->x2min >= 0.0 && c->x2max <= M_PI),

@@ -2091,6 +2091,8 @@ static size_t rhs_smem_bytes(const FusedArgs& a) {
 
 static int check_fused(const snapx_config* c) {
   if (int e = validate_config(c)) return e;
+  SNAPX_CHECK_ARG(c->coord != SNAPX_COORD_GNOMONIC_EQUIANGLE,
+                  "fused path: gnomonic-equiangle panels run on the plain path");
   if (c->coord == SNAPX_COORD_SPHERICAL_POLAR) {
     // spherical K2 is the cell-parallel column kernel only
     const int N = ((c->implicit_scheme >> 3) & 1) ? 5 : 3;

@@ -26,11 +26,14 @@ constexpr double kRgas = 8.31446261815324;  // CODATA; kintera::constants::Rgas
 
 // EquationOfStateImpl::apply_conserved_limiter_ + ideal_gas_cons2prim +
 // apply_primitive_limiter_ (src/eos/equation_of_state.cpp:110-179,
-// src/eos/ideal_gas_impl.h:15-35); cos(theta) = 0 makes the contravariant
-// raise an identity.
+// src/eos/ideal_gas_impl.h:15-35).  The velocities are raised to
+// contravariant components with the cell's cos(theta)
+// (coord_vec_raise_impl, src/coord/coord_utils_impl.h:8-16) on gnomonic
+// panels (gno); elsewhere cos(theta) = 0 makes the raise an identity.
 template <typename T>
 __device__ __forceinline__ void cons2prim_cell(T (&U)[5], T (&W)[5], T gammad, int limiter,
-                                               T dfloor, T pfloor, T cv_tfloor) {
+                                               T dfloor, T pfloor, T cv_tfloor,
+                                               bool gno = false, T cth = T(0.)) {
   if (limiter) {
 #pragma unroll
     for (int v = 0; v < 5; ++v) U[v] = isnan(U[v]) ? T(0.) : U[v];
@@ -43,6 +46,11 @@ __device__ __forceinline__ void cons2prim_cell(T (&U)[5], T (&W)[5], T gammad, i
   W[IVX] = U[IVX] / W[IDN];
   W[IVY] = U[IVY] / W[IDN];
   W[IVZ] = U[IVZ] / W[IDN];
+  if (gno) {
+    const T v = W[IVY], x = W[IVZ], sth2 = T(1.) - cth * cth;
+    W[IVY] = v / sth2 - x * cth / sth2;
+    W[IVZ] = -v * cth / sth2 + x / sth2;
+  }
   T ke = T(0.5) * ((W[IVX] * U[IVX] + W[IVY] * U[IVY]) + W[IVZ] * U[IVZ]);
   T ie = U[IPR] - ke;
   W[IPR] = (gammad - T(1.)) * ie;
@@ -56,12 +64,15 @@ __device__ __forceinline__ void cons2prim_cell(T (&U)[5], T (&W)[5], T gammad, i
 
 template <typename T>
 __global__ void cons2prim_kernel(T* __restrict__ u, T* __restrict__ w, int ncell, T gammad,
-                                 int limiter, T dfloor, T pfloor, T cv_tfloor) {
+                                 int limiter, T dfloor, T pfloor, T cv_tfloor, Geometry geo) {
+  const bool gno = geo.coord == SNAPX_COORD_GNOMONIC_EQUIANGLE;
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < ncell; q += gridDim.x * blockDim.x) {
     T U[5], W[5];
 #pragma unroll
     for (int v = 0; v < 5; ++v) U[v] = u[v * ncell + q];
-    cons2prim_cell(U, W, gammad, limiter, dfloor, pfloor, cv_tfloor);
+    const T cth = gno ? T(gno_cos_cell(geo, q / (geo.n1 * geo.n2), (q / geo.n1) % geo.n2))
+                      : T(0.);
+    cons2prim_cell(U, W, gammad, limiter, dfloor, pfloor, cv_tfloor, gno, cth);
     if (limiter) {
 #pragma unroll
       for (int v = 0; v < 5; ++v) u[v * ncell + q] = U[v];
@@ -74,7 +85,8 @@ __global__ void cons2prim_kernel(T* __restrict__ u, T* __restrict__ w, int ncell
 // IdealGasImpl::_prim2cons (src/eos/ideal_gas.cpp:67-88) with both limiters
 template <typename T>
 __global__ void prim2cons_kernel(T* __restrict__ w, T* __restrict__ u, int ncell, T gammad,
-                                 int limiter, T dfloor, T pfloor, T cv_tfloor) {
+                                 int limiter, T dfloor, T pfloor, T cv_tfloor, Geometry geo) {
+  const bool gno = geo.coord == SNAPX_COORD_GNOMONIC_EQUIANGLE;
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < ncell; q += gridDim.x * blockDim.x) {
     T W[5];
 #pragma unroll
@@ -92,6 +104,12 @@ __global__ void prim2cons_kernel(T* __restrict__ w, T* __restrict__ u, int ncell
     U[IVX] = W[IVX] * W[IDN];
     U[IVY] = W[IVY] * W[IDN];
     U[IVZ] = W[IVZ] * W[IDN];
+    if (gno) {  // coord_vec_lower_impl (src/coord/coord_utils_impl.h:18-24)
+      const T cth = T(gno_cos_cell(geo, q / (geo.n1 * geo.n2), (q / geo.n1) % geo.n2));
+      const T m2 = U[IVY], m3 = U[IVZ];
+      U[IVY] = m2 + m3 * cth;
+      U[IVZ] = m3 + m2 * cth;
+    }
     T ke = T(0.5) * ((W[IVX] * U[IVX] + W[IVY] * U[IVY]) + W[IVZ] * U[IVZ]);
     U[IPR] = W[IPR] / (gammad - T(1));
     U[IPR] += ke;
@@ -162,10 +180,14 @@ __global__ void recon_kernel(const T* __restrict__ w, T* __restrict__ wlr, int n
 
 // ------------------------------------------------------------ Riemann ----
 
+// frame: on gnomonic panels LMARS and HLLC take the states in the face's
+// orthonormal frame and return the flux to the panel's components
+// (src/riemann/lmars.cpp:54-99, src/riemann/hllc.cpp:56-103); Roe does not
+// (src/riemann/roe.cpp:14-59)
 template <typename T>
 __global__ void riemann_kernel(int solver, const T* __restrict__ wl, const T* __restrict__ wr,
                                T* __restrict__ flx, T* __restrict__ fp, int ncell, int dim,
-                               T gammad) {
+                               T gammad, int frame, Geometry geo) {
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < ncell; q += gridDim.x * blockDim.x) {
     T a[5], b[5], f[5], p;
 #pragma unroll
@@ -173,7 +195,15 @@ __global__ void riemann_kernel(int solver, const T* __restrict__ wl, const T* __
       a[v] = wl[v * ncell + q];
       b[v] = wr[v * ncell + q];
     }
+    T fc = T(0.), fs = T(1.);
+    if (frame) {
+      const GnoAngle ang = gno_face_angle(geo, dim, q / (geo.n1 * geo.n2), (q / geo.n1) % geo.n2);
+      fc = T(ang.c), fs = T(ang.s);
+      gno_prim2local(a, dim, fc, fs);
+      gno_prim2local(b, dim, fc, fs);
+    }
     riemann_flux(solver, a, b, gammad, dim, f, &p);
+    if (frame) gno_flux2global(f, dim, fc, fs);
 #pragma unroll
     for (int v = 0; v < 5; ++v) flx[v * ncell + q] = f[v];
     if (fp) fp[q] = p;
@@ -279,6 +309,7 @@ __global__ void coord_forward_kernel(const T* __restrict__ w, const T* __restric
   const int n1 = g.n1, n2 = g.n2, n3 = g.n3, ng = g.ng;
   const int ncell = n1 * n2 * n3;
   const bool sph = g.coord == SNAPX_COORD_SPHERICAL_POLAR;
+  const bool gno = g.coord == SNAPX_COORD_GNOMONIC_EQUIANGLE;
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < ncell; q += gridDim.x * blockDim.x) {
     int i = q % n1;
     int j = (q / n1) % n2;
@@ -286,19 +317,26 @@ __global__ void coord_forward_kernel(const T* __restrict__ w, const T* __restric
     bool r1 = n1 == 1 || (i >= ng && i < n1 - ng);
     bool r2 = n2 == 1 || (j >= ng && j < n2 - ng);
     bool r3 = n3 == 1 || (k >= ng && k < n3 - ng);
-    double vol, A1m, A1p, A2m, A2p, A3;
+    double vol, A1m, A1p, A2m, A2p, A3m, A3p;
     SphCell gc{};
+    GnoCell gn{};
     if (sph) {
       gc = sph_cell(g, k, j, i);
       vol = gc.vol;
       A1m = gc.a1m, A1p = gc.a1p;
       A2m = gc.a2m, A2p = gc.a2p;
-      A3 = gc.a3;
+      A3m = A3p = gc.a3;
+    } else if (gno) {
+      gn = gno_cell(g, k, j, i);
+      vol = gn.vol;
+      A1m = gn.a1m, A1p = gn.a1p;
+      A2m = gn.a2m, A2p = gn.a2p;
+      A3m = gn.a3m, A3p = gn.a3p;
     } else {
       vol = a.dx3 * (a.dx2 * a.dx1);
       A1m = A1p = a.dx3 * a.dx2;
       A2m = A2p = a.dx3 * a.dx1;
-      A3 = a.dx2 * a.dx1;
+      A3m = A3p = a.dx2 * a.dx1;
     }
     T d[5];
 #pragma unroll
@@ -307,8 +345,26 @@ __global__ void coord_forward_kernel(const T* __restrict__ w, const T* __restric
       T acc = T(0.);
       if (f1 && r1) acc += T(A1p) * f1[o + 1] - T(A1m) * f1[o];
       if (f2 && r2) acc += T(A2p) * f2[o + n1] - T(A2m) * f2[o];
-      if (f3 && r3) acc += T(A3) * f3[o + n1 * n2] - T(A3) * f3[o];
+      if (f3 && r3) acc += T(A3p) * f3[o + n1 * n2] - T(A3m) * f3[o];
       d[v] = acc / T(vol);
+    }
+    if (gno) {
+      // GnomonicEquiangleImpl::forward (src/coord/gnomonic_equiangle.cpp:362-418)
+      T rho = w[IDN * ncell + q], v1 = w[IVX * ncell + q];
+      T v2 = w[IVY * ncell + q], v3 = w[IVZ * ncell + q], pre = w[IPR * ncell + q];
+      T cosine = T(gn.cos), sine2 = T(gn.sin) * T(gn.sin), radius = T(gn.r);
+      T v_2 = v2 + v3 * cosine, v_3 = v3 + v2 * cosine;
+      T src1 = rho * (v2 * v_2 + v3 * v_3) / radius;
+      if (fp1) {
+        if (r1)
+          src1 += (T(A1p) * fp1[q + 1] - T(A1m) * fp1[q]) / T(vol) -
+                  (fp1[q + 1] - fp1[q]) / T(gn.dx1);
+      } else {
+        src1 += T(2.0) * pre / radius;
+      }
+      d[IVX] -= src1;
+      d[IVY] -= T(gn.xov) * (pre + rho * v3 * v3 * sine2) - rho * v1 * v_2 / radius;
+      d[IVZ] -= T(gn.yov) * (pre + rho * v2 * v2 * sine2) - rho * v1 * v_3 / radius;
     }
     if (sph) {
       T rho = w[IDN * ncell + q], v2 = w[IVY * ncell + q], v3 = w[IVZ * ncell + q];
@@ -527,6 +583,9 @@ __global__ void cfl_kernel(const T* __restrict__ w, unsigned long long* __restri
       dx1 = T(axis_width(geo.ax[0], i));
       if (use2) dx2 = T(sph_width2(geo, j, i));
       if (use3) dx3 = T(sph_width3(geo, k, j, i));
+    } else if (geo.coord == SNAPX_COORD_GNOMONIC_EQUIANGLE) {
+      if (use2) dx2 = T(gno_width2(geo, k, j, i));
+      if (use3) dx3 = T(gno_width3(geo, k, j, i));
     }
     if (use1) best = min(best, cfl_key((double)(dx1 / (fabs(w[IVX * ncell + q]) + cs))));
     if (use2) best = min(best, cfl_key((double)(dx2 / (fabs(w[IVY * ncell + q]) + cs))));
@@ -555,9 +614,10 @@ __global__ void fill_i32_kernel(int* p, int v) { *p = v; }
 template <typename T>
 __global__ void state_screen_kernel(const T* __restrict__ u, int* __restrict__ acc, int n3,
                                     int n2, int n1, int ng, T gammad, int limiter, T dfloor,
-                                    T pfloor, T cv_tfloor) {
+                                    T pfloor, T cv_tfloor, Geometry geo) {
   int ncell = n1 * n2 * n3;
   int bits = 0;
+  const bool gno = geo.coord == SNAPX_COORD_GNOMONIC_EQUIANGLE;
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < ncell; q += gridDim.x * blockDim.x) {
     int i = q % n1;
     int j = (q / n1) % n2;
@@ -573,7 +633,8 @@ __global__ void state_screen_kernel(const T* __restrict__ u, int* __restrict__ a
     for (int v = 0; v < 5; ++v) bits |= isfinite(U[v]) ? 0 : kNonFinite;
     bits |= isnan(U[IDN]) ? kRhoNan : (U[IDN] <= T(0.) ? kRhoLe0 : 0);
     bits |= isnan(U[IPR]) ? kENan : (U[IPR] <= T(0.) ? kELe0 : 0);
-    cons2prim_cell(U, W, gammad, limiter, dfloor, pfloor, cv_tfloor);
+    const T cth = gno ? T(gno_cos_cell(geo, k, j)) : T(0.);
+    cons2prim_cell(U, W, gammad, limiter, dfloor, pfloor, cv_tfloor, gno, cth);
     bits |= isnan(W[IPR]) ? kPNan : (W[IPR] <= T(0.) ? kPLe0 : 0);
   }
   for (int off = warpSize / 2; off > 0; off >>= 1) bits |= __shfl_down(bits, off);
@@ -603,7 +664,7 @@ int launch_cons2prim(const snapx_config* c, void* u, void* w, hipStream_t st) {
   double cvt = kRgas / c->weight / (c->gammad - 1.) * c->temperature_floor;
   hipLaunchKernelGGL(cons2prim_kernel<T>, blocks_for(g.ncell, kThreads), dim3(kThreads), 0, st,
                      as<T>(u), as<T>(w), g.ncell, T(c->gammad), c->limiter, T(c->density_floor),
-                     T(c->pressure_floor), T(cvt));
+                     T(c->pressure_floor), T(cvt), make_geometry(c));
   SNAPX_LAUNCH_CHECK();
   return 0;
 }
@@ -620,12 +681,16 @@ int launch_recon(const void* w, void* wlr, int nvar, int n3, int n2, int n1, int
   return 0;
 }
 
+// geo: the block's geometry when the face frames of a gnomonic panel apply
 template <typename T>
 int launch_riemann(int solver, const void* wl, const void* wr, void* flx, void* fp, int ncell,
-                   int dim, double gammad, hipStream_t st) {
+                   int dim, double gammad, hipStream_t st, const Geometry* geo = nullptr) {
+  const int frame = geo && geo->coord == SNAPX_COORD_GNOMONIC_EQUIANGLE &&
+                    solver != SNAPX_RIEMANN_ROE;
   hipLaunchKernelGGL(riemann_kernel<T>, blocks_for(ncell, kThreads), dim3(kThreads), 0, st,
                      solver, reinterpret_cast<const T*>(wl), reinterpret_cast<const T*>(wr),
-                     as<T>(flx), as<T>(fp), ncell, dim, T(gammad));
+                     as<T>(flx), as<T>(fp), ncell, dim, T(gammad), frame,
+                     geo ? *geo : Geometry{});
   SNAPX_LAUNCH_CHECK();
   return 0;
 }
@@ -683,12 +748,12 @@ extern "C" int snapx_ideal_gas_prim2cons(const snapx_config* cfg, void* w, void*
   if (dtype == SNAPX_F64)
     hipLaunchKernelGGL(prim2cons_kernel<double>, blocks_for(g.ncell, kThreads), dim3(kThreads),
                        0, st, as<double>(w), as<double>(u), g.ncell, cfg->gammad, cfg->limiter,
-                       cfg->density_floor, cfg->pressure_floor, cvt);
+                       cfg->density_floor, cfg->pressure_floor, cvt, make_geometry(cfg));
   else
     hipLaunchKernelGGL(prim2cons_kernel<float>, blocks_for(g.ncell, kThreads), dim3(kThreads),
                        0, st, as<float>(w), as<float>(u), g.ncell, (float)cfg->gammad,
                        cfg->limiter, (float)cfg->density_floor, (float)cfg->pressure_floor,
-                       (float)cvt);
+                       (float)cvt, make_geometry(cfg));
   SNAPX_LAUNCH_CHECK();
   return 0;
 }
@@ -920,12 +985,13 @@ extern "C" int snapx_check_state(const snapx_config* cfg, const void* u, int* fl
   if (dtype == SNAPX_F64)
     hipLaunchKernelGGL(state_screen_kernel<double>, dim3(nb), dim3(kThreads), 0, st,
                        reinterpret_cast<const double*>(u), flag, g.n3, g.n2, g.n1, g.ng,
-                       cfg->gammad, cfg->limiter, cfg->density_floor, cfg->pressure_floor, cvt);
+                       cfg->gammad, cfg->limiter, cfg->density_floor, cfg->pressure_floor, cvt,
+                       make_geometry(cfg));
   else
     hipLaunchKernelGGL(state_screen_kernel<float>, dim3(nb), dim3(kThreads), 0, st,
                        reinterpret_cast<const float*>(u), flag, g.n3, g.n2, g.n1, g.ng,
                        (float)cfg->gammad, cfg->limiter, (float)cfg->density_floor,
-                       (float)cfg->pressure_floor, (float)cvt);
+                       (float)cfg->pressure_floor, (float)cvt, make_geometry(cfg));
   hipLaunchKernelGGL(state_finalize_kernel, dim3(1), dim3(1), 0, st, flag);
   SNAPX_LAUNCH_CHECK();
   return 0;
@@ -983,7 +1049,10 @@ static int hydro_forward_impl(const snapx_config* c, double dt, void* u, void* w
   int dims[3] = {3, 2, 1};
   const void* Fuse[3] = {nullptr, nullptr, nullptr};
   const void* rg = nullptr;
-  const bool sph = c->coord == SNAPX_COORD_SPHERICAL_POLAR;
+  // curvilinear blocks (spherical-polar, gnomonic panel): geometric sources
+  // with the x1 face pressure
+  const bool sph = c->coord != SNAPX_COORD_CARTESIAN;
+  const Geometry geo = make_geometry(c);
 
   // (1) primitives
   if (int e = launch_cons2prim<T>(c, u, w, st)) return e;
@@ -1009,13 +1078,13 @@ static int hydro_forward_impl(const snapx_config* c, double dt, void* u, void* w
       SNAPX_LAUNCH_CHECK();
       rg = h.rg;
     }
-    // the x1 face pressure feeds the spherical-polar radial source; the
+    // the x1 face pressure feeds the curvilinear radial source; the
     // reference keeps it zero when the x1 flux is disabled
     void* fp = (a == 0 && sph) ? h.fp1 : nullptr;
     if (!c->disable_flux[a]) {
       T* wl = as<T>(h.wlr);
       if (int e = launch_riemann<T>(c->riemann[a], wl, wl + 5 * g.ncell, F[a], fp, g.ncell,
-                                    dims[a], c->gammad, st))
+                                    dims[a], c->gammad, st, &geo))
         return e;
       Fuse[a] = F[a];
     } else {

@@ -44,6 +44,12 @@ struct LayerGeom {
 };
 
 __device__ __forceinline__ LayerGeom layer_geom(const VicArgs& a, int col, int i) {
+  if (a.geo.coord == SNAPX_COORD_GNOMONIC_EQUIANGLE) {
+    int j = a.n2 > 1 ? a.ng + col % a.ncol2 : 0;
+    int k = a.n3 > 1 ? a.ng + col / a.ncol2 : 0;
+    const GnoCell gc = gno_cell(a.geo, k, j, a.ng + i);
+    return {gc.a1m, gc.a1p, gc.vol};
+  }
   if (a.geo.coord != SNAPX_COORD_SPHERICAL_POLAR) return {a.area, a.area, a.vol};
   int j = a.n2 > 1 ? a.ng + col % a.ncol2 : 0;
   int k = a.n3 > 1 ? a.ng + col / a.ncol2 : 0;

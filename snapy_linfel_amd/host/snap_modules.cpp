@@ -182,8 +182,10 @@ CoordinateOptions CoordinateOptionsImpl::from_yaml(yaml::Node const& config) {
   auto node = config["geometry"];
   if (!node) return op;
   op->type(node["type"].as<std::string>("cartesian"));
-  TORCH_CHECK(op->type() == "cartesian" || op->type() == "spherical-polar", "geometry.type '",
-              op->type(), "' is not supported (cartesian and spherical-polar only)");
+  TORCH_CHECK(op->type() == "cartesian" || op->type() == "spherical-polar" ||
+                  op->type() == "gnomonic-equiangle",
+              "geometry.type '", op->type(),
+              "' is not supported (cartesian, spherical-polar and gnomonic-equiangle only)");
   if (node["bounds"]) {
     // src/coord/coordinate.cpp:72-101: x2/x3 bounds may be given in units of pi
     auto b = node["bounds"];
@@ -298,7 +300,9 @@ snapx_config make_config(CoordinateOptions const& coord, HydroOptions const& hyd
   c.x1min = coord->x1min(), c.x1max = coord->x1max();
   c.x2min = coord->x2min(), c.x2max = coord->x2max();
   c.x3min = coord->x3min(), c.x3max = coord->x3max();
-  c.coord = coord->type() == "spherical-polar" ? SNAPX_COORD_SPHERICAL_POLAR : SNAPX_COORD_CARTESIAN;
+  c.coord = coord->type() == "spherical-polar"      ? SNAPX_COORD_SPHERICAL_POLAR
+            : coord->type() == "gnomonic-equiangle" ? SNAPX_COORD_GNOMONIC_EQUIANGLE
+                                                    : SNAPX_COORD_CARTESIAN;
   auto eos = hydro->eos() ? hydro->eos() : EquationOfStateOptionsImpl::create();
   TORCH_CHECK(eos->type() == "ideal-gas", "EquationOfState: type '", eos->type(),
               "' is not supported (ideal-gas only)");

@@ -89,9 +89,11 @@ def test_unsupported_eos_and_geometry_are_errors():
               "geometry:\n  cells: {nx1: 8, nghost: 3}\n")
     with pytest.raises(RuntimeError, match="ideal-gas only"):
         host.MeshBlock(host.MeshBlockOptions.from_yaml(f))
-    g = _yaml("geometry:\n  type: gnomonic-equiangle\n")
-    with pytest.raises(RuntimeError, match="cartesian and spherical-polar only"):
+    g = _yaml("geometry:\n  type: cylindrical\n")
+    with pytest.raises(RuntimeError, match="gnomonic-equiangle only"):
         host.CoordinateOptions.from_yaml(g)
+    assert host.CoordinateOptions.from_yaml(
+        _yaml("geometry:\n  type: gnomonic-equiangle\n")).type == "gnomonic-equiangle"
 
 
 def test_modules_refuse_host_tensors():
