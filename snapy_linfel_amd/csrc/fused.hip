@@ -175,6 +175,46 @@ __device__ __forceinline__ void cons_to_prim(const T* U, T gammad, T* W) {
   W[IPR] = (gammad - T(1.)) * (U[IPR] - ke);
 }
 
+// cons_to_prim on a gnomonic panel: the covariant momenta over the density
+// raised to contravariant velocities with the cell's cos(theta) before the
+// kinetic energy (src/eos/ideal_gas_impl.h:15-35, coord_utils_impl.h:8-16)
+template <typename T>
+__device__ __forceinline__ void cons_to_prim_gno(const T* U, T gammad, T cth, T* W) {
+  W[IDN] = U[IDN];
+  T inv = recip(U[IDN]);
+  W[IVX] = U[IVX] * inv;
+  const T v = U[IVY] * inv, x = U[IVZ] * inv, sth2 = T(1.) - cth * cth;
+  W[IVY] = v / sth2 - x * cth / sth2;
+  W[IVZ] = -v * cth / sth2 + x / sth2;
+  T ke = T(0.5) * ((W[IVX] * U[IVX] + W[IVY] * U[IVY]) + W[IVZ] * U[IVZ]);
+  W[IPR] = (gammad - T(1.)) * (U[IPR] - ke);
+}
+
+// The face frame of a gnomonic panel on an interface state in the RHS
+// kernel's slot order (the normal velocity in slot IVX; d = 0, 1, 2 for x1,
+// x2, x3 faces): gno_prim2local on the panel velocities (v2, v3)
+__device__ __forceinline__ void gno_slots_to_local(float* w, int d, float c, float s) {
+  float g[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+  const int i2 = d == 0 ? IVY : (d == 1 ? IVX : IVZ), i3 = d == 0 ? IVZ : (d == 1 ? IVY : IVX);
+  g[IVY] = w[i2], g[IVZ] = w[i3];
+  gno_prim2local(g, d == 0 ? 3 : (d == 1 ? 2 : 1), c, s);
+  w[i2] = g[IVY], w[i3] = g[IVZ];
+}
+__device__ __forceinline__ void gno_slots_to_global(float* f, int d, float c, float s) {
+  float g[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+  const int i2 = d == 0 ? IVY : (d == 1 ? IVX : IVZ), i3 = d == 0 ? IVZ : (d == 1 ? IVY : IVX);
+  g[IVY] = f[i2], g[IVZ] = f[i3];
+  gno_flux2global(g, d == 0 ? 3 : (d == 1 ? 2 : 1), c, s);
+  f[i2] = g[IVY], f[i3] = g[IVZ];
+}
+
+
+// the step's dt: read from device memory when the caller passed a pointer
+// (the kernel argument itself stays unmodified, so that it is not copied to
+// scratch memory)
+__device__ __forceinline__ float step_dt(const FusedArgs& a) {
+  return a.dtp ? (float)*a.dtp : a.dt;
+}
 
 // ---- perturbation path (hydrostatic background, bg_impl.hpp) ----
 
@@ -327,9 +367,78 @@ __device__ __forceinline__ void bg_x1_face_flux(const float* ws, int HS, int bas
 // (src/coord/spherical_polar.cpp:212-279) with the x1 face pressure, as the
 // plain path's coord_forward_kernel does per cell.  fs holds the fluxes of all
 // directions ([5][faces]) and, after them, the x1 face pressures.
+// du of the cells of a gnomonic panel tile: flux differences with the
+// panel's face areas and volume, the metric sources of
+// GnomonicEquiangleImpl::forward (src/coord/gnomonic_equiangle.cpp:362-418)
+// with the x1 face pressure, and gravity
+__device__ __forceinline__ void gno_cell_du(const FusedArgs& a, const float* ws,
+                                            const float* fs, int ftot, int fo1, int fo2, int o1,
+                                            int o2, int o3, int e1, int e2, int e3, float dt,
+                                            float* __restrict__ du) {
+  const Geometry& G = a.geo;
+  const int H1 = a.tx1 + 6, H2 = a.tx2 + 2 * a.h2, H = H1 * H2 * (a.tx3 + 2 * a.h3);
+  const int ncell = a.n1 * a.n2 * a.n3;
+  const bool act2 = a.n2 > 1, act3 = a.n3 > 1;
+  const float* fp = fs + 5 * ftot;
+  const DivMod de1(e1), de2(e2);
+  for (int cid = threadIdx.x; cid < e1 * e2 * e3; cid += blockDim.x) {
+    int cx, cy;
+    int cz = de2.div(de1.div(cid, cx), cy);
+    const int i = o1 + cx + a.ng;
+    const int j = act2 ? o2 + cy + a.ng : 0;
+    const int k = act3 ? o3 + cz + a.ng : 0;
+    const GnoCell gc = gno_cell(G, k, j, i);
+    const float vol = (float)gc.vol;
+    const float A1m = (float)gc.a1m, A1p = (float)gc.a1p;
+    const float A2m = (float)gc.a2m, A2p = (float)gc.a2p;
+    const float A3m = (float)gc.a3m, A3p = (float)gc.a3p;
+    const int lo1 = (cz * e2 + cy) * (e1 + 1) + cx;
+    const int lo2 = fo1 + (cz * (e2 + 1) + cy) * e1 + cx;
+    const int lo3 = fo2 + (cz * e2 + cy) * e1 + cx;
+    float div[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      float acc = A1p * fs[v * ftot + lo1 + 1] - A1m * fs[v * ftot + lo1];
+      if (act2) acc += A2p * fs[v * ftot + lo2 + e1] - A2m * fs[v * ftot + lo2];
+      if (act3) acc += A3p * fs[v * ftot + lo3 + e1 * e2] - A3m * fs[v * ftot + lo3];
+      div[v] = acc / vol;
+    }
+    const int lin = ((cz + a.h3) * H2 + cy + a.h2) * H1 + cx + 3;
+    const float rho = ws[IDN * H + lin], v1 = ws[IVX * H + lin], v2 = ws[IVY * H + lin],
+                v3 = ws[IVZ * H + lin], pre = ws[IPR * H + lin];
+    const float cosine = (float)gc.cos, sine2 = (float)gc.sin * (float)gc.sin;
+    const float radius = (float)gc.r;
+    const float v_2 = v2 + v3 * cosine, v_3 = v3 + v2 * cosine;
+    const float fpm = fp[lo1], fpp = fp[lo1 + 1];
+    float src1 = rho * (v2 * v_2 + v3 * v_3) / radius;
+    src1 += (A1p * fpp - A1m * fpm) / vol - (fpp - fpm) / (float)gc.dx1;
+    div[IVX] -= src1;
+    div[IVY] -= (float)gc.xov * (pre + rho * v3 * v3 * sine2) - rho * v1 * v_2 / radius;
+    div[IVZ] -= (float)gc.yov * (pre + rho * v2 * v2 * sine2) - rho * v1 * v_3 / radius;
+    float d[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) d[v] = -dt * div[v];
+    if (a.grav1 != 0.f) {
+      d[IVX] += dt * rho * a.grav1;
+      d[IPR] += dt * rho * v1 * a.grav1;
+    }
+    if (a.grav2 != 0.f) {
+      d[IVY] += dt * rho * a.grav2;
+      d[IPR] += dt * rho * v2 * a.grav2;
+    }
+    if (a.grav3 != 0.f) {
+      d[IVZ] += dt * rho * a.grav3;
+      d[IPR] += dt * rho * v3 * a.grav3;
+    }
+    const int q = (k * a.n2 + j) * a.n1 + i;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) du[v * ncell + q] = d[v];
+  }
+}
+
 __device__ __forceinline__ void sph_cell_du(const FusedArgs& a, const float* ws,
                                             const float* fs, int ftot, int fo1, int fo2, int o1,
-                                            int o2, int o3, int e1, int e2, int e3,
+                                            int o2, int o3, int e1, int e2, int e3, float dt,
                                             float* __restrict__ du) {
   const Geometry& G = a.geo;
   const int H1 = a.tx1 + 6, H2 = a.tx2 + 2 * a.h2, H = H1 * H2 * (a.tx3 + 2 * a.h3);
@@ -375,7 +484,6 @@ __device__ __forceinline__ void sph_cell_du(const FusedArgs& a, const float* ws,
                   (A2m * fs[IVZ * ftot + lo2] + A2p * fs[IVZ * ftot + lo2 + e1]);
     else
       div[IVZ] += s1i * s1j * (rho * v3 * v2);
-    const float dt = a.dt;
     float d[5];
 #pragma unroll
     for (int v = 0; v < 5; ++v) d[v] = -dt * div[v];
@@ -406,9 +514,19 @@ __device__ __forceinline__ void sph_cell_du(const FusedArgs& a, const float* ws,
 // -1 to follow a.merged at run time (batches of mixed members).
 // kBg (merged face pass, Cartesian): perturbation path about the
 // hydrostatic background of a.bgt / a.bgd, as in fused_rhs_march_kernel.
-template <int kMerged, bool kSph = false, bool kBg = false>
+// kSph: 0 Cartesian, 1 spherical-polar, 2 gnomonic panel (curvilinear
+// tiles take the merged face pass).
+// The arguments reach rhs_tile by value on Cartesian tiles and by reference
+// on curvilinear ones: with these forms the compiler keeps every instance's
+// copy of FusedArgs out of scratch memory (the per-direction field selects
+// of the Cartesian two-pass tile and the geometry reads of the curvilinear
+// ones defeat it otherwise).
+template <int kSph>
+using RhsArgs = std::conditional_t<kSph != 0, const FusedArgs&, const FusedArgs>;
+
+template <int kMerged, int kSph = 0, bool kBg = false>
 __device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __restrict__ du,
-                                         const FusedArgs a, int b, float* smem) {
+                                         RhsArgs<kSph> a, int b, float* smem, float dt) {
   static_assert(!kBg || (kMerged == 1 && !kSph), "background path: merged Cartesian tiles");
   const bool merged = kMerged < 0 ? (a.merged != 0) : kMerged == 1;
   const int H1 = a.tx1 + 6, H2 = a.tx2 + 2 * a.h2, H3 = a.tx3 + 2 * a.h3;
@@ -464,7 +582,16 @@ __device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __r
       for (int v = 0; v < 5; ++v) ws[v * H + l] = W[v];
       continue;
     }
-    cons_to_prim(U, gammad, W);
+    // cos(theta) of a gnomonic panel cell (the cell's own position, ghosts
+    // included, as the reference's cosine_cell_kj)
+    constexpr bool gno = kSph == 2;
+    const float cth = gno ? (float)gno_cos_cell(a.geo, a.n3 > 1 ? gk + a.ng : 0,
+                                                a.n2 > 1 ? gj + a.ng : 0)
+                          : 0.f;
+    if (gno)
+      cons_to_prim_gno(U, gammad, cth, W);
+    else
+      cons_to_prim(U, gammad, W);
     // (only the tile's own columns: the x1 ghosts of halo columns are never read)
     if (merged && refill && (gi < 0 || gi >= a.nx1) && (unsigned)(y - a.h2) < (unsigned)e2 &&
         (unsigned)(z - a.h3) < (unsigned)e3) {
@@ -474,7 +601,11 @@ __device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __r
       float Ub[5], Wb[5];
 #pragma unroll
       for (int v = 0; v < 5; ++v) Ub[v] = u[v * ncell + qb];
-      cons_to_prim(Ub, gammad, Wb);  // rho and p do not depend on the flips
+      // rho and p do not depend on the flips
+      if (gno)
+        cons_to_prim_gno(Ub, gammad, cth, Wb);
+      else
+        cons_to_prim(Ub, gammad, Wb);
       if (kSph) {
         const IsentropicRefill R(Wb[IDN], Wb[IPR], gammad, -a.grav1, upper ? 1 : -1);
         R.layer_at(upper ? a.zout[gi - a.nx1] : a.zin[-gi - 1], W[IDN], W[IPR]);
@@ -608,9 +739,21 @@ __device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __r
       }
       float flx[5];
       if (kSph) {
+        // gnomonic panels: LMARS and HLLC in the face's orthonormal frame
+        // (src/riemann/lmars.cpp:54-99, src/riemann/hllc.cpp:56-103)
+        const bool frame = kSph == 2 && solver != SNAPX_RIEMANN_ROE;
+        float fc = 0.f, fsn = 1.f;
+        if (frame) {
+          const int kk = act3 ? o3 + fz + a.ng : 0, jj = act2 ? o2 + fy + a.ng : 0;
+          const GnoAngle ang = gno_face_angle(a.geo, d == 0 ? 3 : (d == 1 ? 2 : 1), kk, jj);
+          fc = (float)ang.c, fsn = (float)ang.s;
+          gno_slots_to_local(wl, d, fc, fsn);
+          gno_slots_to_local(wr, d, fc, fsn);
+        }
         // the x1 face pressure feeds the radial momentum source
         float pf;
         riemann_flux<float>(solver, wl, wr, gammad, 3, flx, &pf);
+        if (frame) gno_slots_to_global(flx, d, fc, fsn);
         if (d == 0) fs[5 * ftot + g] = pf;
       } else {
         riemann_flux<float>(solver, wl, wr, gammad, 3, flx, (float*)nullptr);
@@ -622,7 +765,10 @@ __device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __r
     }
     __syncthreads();
     if (kSph) {
-      sph_cell_du(a, ws, fs, ftot, fo1, fo2, o1, o2, o3, e1, e2, e3, du);
+      if (kSph == 2)
+        gno_cell_du(a, ws, fs, ftot, fo1, fo2, o1, o2, o3, e1, e2, e3, dt, du);
+      else
+        sph_cell_du(a, ws, fs, ftot, fo1, fo2, o1, o2, o3, e1, e2, e3, dt, du);
       return;
     }
     // flux differences, directions in order (x1, x2, x3)
@@ -731,7 +877,6 @@ __device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __r
 
   // ---- 3. du = -dt D + gravity ----
   float vol = a.dx3 * (a.dx2 * a.dx1);
-  float dt = a.dt;
 #pragma unroll
   for (int rr = 0; rr < kMaxCellsPerThread; ++rr) {
     int cid = threadIdx.x + rr * kRhsThreads;
@@ -774,24 +919,28 @@ template <int kMerged>
 __global__ void __launch_bounds__(kRhsThreads)
     fused_rhs_kernel(const float* __restrict__ u, float* __restrict__ du, FusedArgs a) {
   extern __shared__ float smem[];
-  if (a.dtp) a.dt = (float)*a.dtp;
-  rhs_tile<kMerged>(u, du, a, blockIdx.x, smem);
+  rhs_tile<kMerged>(u, du, a, blockIdx.x, smem, step_dt(a));
 }
 
 // K1 of the perturbation path (merged face pass, Cartesian)
 __global__ void __launch_bounds__(kRhsThreads)
     fused_rhs_bg_kernel(const float* __restrict__ u, float* __restrict__ du, FusedArgs a) {
   extern __shared__ float smem[];
-  if (a.dtp) a.dt = (float)*a.dtp;
-  rhs_tile<1, false, true>(u, du, a, blockIdx.x, smem);
+  rhs_tile<1, 0, true>(u, du, a, blockIdx.x, smem, step_dt(a));
 }
 
 // K1 of spherical-polar blocks (merged face pass, geometric sources)
 __global__ void __launch_bounds__(kRhsThreads)
     fused_rhs_sph_kernel(const float* __restrict__ u, float* __restrict__ du, FusedArgs a) {
   extern __shared__ float smem[];
-  if (a.dtp) a.dt = (float)*a.dtp;
-  rhs_tile<1, true>(u, du, a, blockIdx.x, smem);
+  rhs_tile<1, 1>(u, du, a, blockIdx.x, smem, step_dt(a));
+}
+
+// K1 of gnomonic panels (merged face pass, face frames, metric sources)
+__global__ void __launch_bounds__(kRhsThreads)
+    fused_rhs_gno_kernel(const float* __restrict__ u, float* __restrict__ du, FusedArgs a) {
+  extern __shared__ float smem[];
+  rhs_tile<1, 2>(u, du, a, blockIdx.x, smem, step_dt(a));
 }
 
 // ------------------------------------------------ K1 for 3-D blocks (2.5-D) ----
@@ -848,7 +997,6 @@ template <class P, bool kBg = false>
 __global__ void __launch_bounds__(kRhsThreads, 4)
     fused_rhs_march_kernel(const float* __restrict__ u, float* __restrict__ du, FusedArgs a) {
   extern __shared__ float smem[];
-  if (a.dtp) a.dt = (float)*a.dtp;
   constexpr int NP = kBg ? 6 : 5;  // LDS planes per slice
   const int H1 = a.tx1 + 6, H2 = a.tx2 + 6, HS = H1 * H2;
   const int ncell = a.n1 * a.n2 * a.n3;
@@ -890,7 +1038,7 @@ __global__ void __launch_bounds__(kRhsThreads, 4)
   }
   const float area1 = a.dx3 * a.dx2, area2 = a.dx3 * a.dx1, area3 = a.dx2 * a.dx1;
   const float vol = a.dx3 * (a.dx2 * a.dx1);
-  const float dt = a.dt;
+  const float dt = step_dt(a);
 
   float win[5][6];   // own column, primitives of slices kk - 5 .. kk
   float part[4][5];  // x1 + x2 flux differences of slices kk .. kk - 3
@@ -1137,9 +1285,11 @@ __device__ __forceinline__ void float_prim(const float* Uf, bool flip, double ga
 
 // FP64 primitives of an FP32 cell of x1 level lvl: on the perturbation
 // path the background of that level is added first
+// (gno: a gnomonic panel's column, whose cells share cos(theta) = cth)
 template <bool kBg>
 __device__ __forceinline__ void float_prim_at(const float* Uf, bool flip, const ColArgs& a,
-                                              int lvl, double* W) {
+                                              int lvl, double* W, bool gno = false,
+                                              double cth = 0.) {
   double U[5];
 #pragma unroll
   for (int v = 0; v < 5; ++v) U[v] = (double)Uf[v];
@@ -1148,7 +1298,10 @@ __device__ __forceinline__ void float_prim_at(const float* Uf, bool flip, const 
     U[IPR] += a.bgd[a.n1 + lvl] / (a.gammad - 1.);
   }
   if (flip) U[IVX] = -U[IVX];
-  cons_to_prim<double>(U, a.gammad, W);
+  if (gno)
+    cons_to_prim_gno<double>(U, a.gammad, cth, W);
+  else
+    cons_to_prim<double>(U, a.gammad, W);
 }
 
 // isentropic refill of the ghost next to boundary cell wb (dir -1: below,
@@ -1595,7 +1748,7 @@ __device__ __forceinline__ void column_pcr2(float* __restrict__ u, float* __rest
     column_barrier(wave);
     // ---- interface |A| of the faces below rows i0, i1 and above i1 ----
     const double gm1 = a.gammad - 1.;
-    const double idt = rcp_fast(a.dt);
+    const double idt = rcp_fast(a.dtp ? *a.dtp : a.dt);
     const double gm1r = 0.5 * (a.gammad + a.gammad) - 1.;
     double Af1[NN], Af2[NN];
     {
@@ -1705,7 +1858,7 @@ __device__ __forceinline__ void column_pcr2(float* __restrict__ u, float* __rest
 
 // kSph: spherical-polar blocks, per-layer face areas and volumes
 // (vic_assemble_cell with area_i != area_i+1) and the uneven ghost refill
-template <int N, bool kSph = false, bool kBg = false>
+template <int N, int kSph = 0, bool kBg = false>
 __device__ __forceinline__ void column_pcr(float* __restrict__ u, float* __restrict__ u0,
                                            const float* __restrict__ du, const ColArgs& a,
                                            int col, double* __restrict__ sh) {
@@ -1755,10 +1908,14 @@ __device__ __forceinline__ void column_pcr(float* __restrict__ u, float* __restr
   double dd[5];
   // every column inside one wavefront: no workgroup barriers
   const bool wave = a.lanes == 64;
+  // a gnomonic panel's column: one cos(theta) for all its cells
+  constexpr bool gno = kSph == 2;
+  const int cj = a.n2 > 1 ? a.ng + col % a.ncol2 : 0, ck = a.n3 > 1 ? a.ng + col / a.ncol2 : 0;
+  const double cth = gno ? gno_cos_cell(a.geo, ck, cj) : 0.;
   if (a.scheme) {
     // ---- primitives of the cell and of its neighbours ----
     double wc[5], wm[5], wp[5];
-    float_prim_at<kBg>(Uc, false, a, a.ng + (act ? i : 0), wc);
+    float_prim_at<kBg>(Uc, false, a, a.ng + (act ? i : 0), wc, gno, cth);
 #pragma unroll
     for (int v = 0; v < 5; ++v) sh[v * T + t] = wc[v];
     column_barrier(wave);
@@ -1769,21 +1926,21 @@ __device__ __forceinline__ void column_pcr(float* __restrict__ u, float* __restr
 #pragma unroll
         for (int v = 0; v < 5; ++v) wm[v] = sh[v * T + t - 1];
       } else {
-        float_prim_at<kBg>(Um, flm, a, lm, wm);
-        refill_ghost<kSph, kBg>(wc, wm, a, -1, Uc);
+        float_prim_at<kBg>(Um, flm, a, lm, wm, gno, cth);
+        refill_ghost<(kSph != 0), kBg>(wc, wm, a, -1, Uc);
       }
       if (i < nl - 1) {
 #pragma unroll
         for (int v = 0; v < 5; ++v) wp[v] = sh[v * T + t + 1];
       } else {
-        float_prim_at<kBg>(Up, flp, a, lp, wp);
-        refill_ghost<kSph, kBg>(wc, wp, a, +1, Uc);
+        float_prim_at<kBg>(Up, flp, a, lp, wp, gno, cth);
+        refill_ghost<(kSph != 0), kBg>(wc, wp, a, +1, Uc);
       }
     }
     column_barrier(wave);
     // ---- interface |A|: lane i owns the face below its cell ----
     const double gm1 = a.gammad - 1.;
-    const double idt = rcp_fast(a.dt);
+    const double idt = rcp_fast(a.dtp ? *a.dtp : a.dt);
     const double gm1r = 0.5 * (a.gammad + a.gammad) - 1.;
     double Am[NN], Ap[NN];
     abs_jacobian_fast<double, N>(gm1r, wm, wc, Am);
@@ -1801,10 +1958,11 @@ __device__ __forceinline__ void column_pcr(float* __restrict__ u, float* __restr
     double A[NN], B[NN], C[NN], r[N];
     // face areas below / above the layer and its volume
     double ar_i = a.area, ar_p = a.area, vol_i = a.vol;
-    if (kSph) {
-      const int jj = a.n2 > 1 ? a.ng + col % a.ncol2 : 0;
-      const int kk = a.n3 > 1 ? a.ng + col / a.ncol2 : 0;
-      const SphCell gc = sph_cell(a.geo, kk, jj, a.ng + i);
+    if (gno) {
+      const GnoCell gc = gno_cell(a.geo, ck, cj, a.ng + i);
+      ar_i = gc.a1m, ar_p = gc.a1p, vol_i = gc.vol;
+    } else if (kSph) {
+      const SphCell gc = sph_cell(a.geo, ck, cj, a.ng + i);
       ar_i = gc.a1m, ar_p = gc.a1p, vol_i = gc.vol;
     }
     {
@@ -1904,7 +2062,6 @@ __global__ void __launch_bounds__(PcrShape<N>::kThreads)
     fused_column_pcr_kernel(float* __restrict__ u, float* __restrict__ u0,
                             const float* __restrict__ du, ColArgs a) {
   extern __shared__ double sh[];
-  if (a.dtp) a.dt = *a.dtp;
   int col = blockIdx.x * (blockDim.x >> a.lshift) + (threadIdx.x >> a.lshift);
   column_pcr<N, false, kBg>(u, u0, du, a, col, sh);
 }
@@ -1915,9 +2072,18 @@ __global__ void __launch_bounds__(PcrShape<N>::kThreads)
     fused_column_pcr_sph_kernel(float* __restrict__ u, float* __restrict__ u0,
                                 const float* __restrict__ du, ColArgs a) {
   extern __shared__ double sh[];
-  if (a.dtp) a.dt = *a.dtp;
   int col = blockIdx.x * (blockDim.x >> a.lshift) + (threadIdx.x >> a.lshift);
-  column_pcr<N, true>(u, u0, du, a, col, sh);
+  column_pcr<N, 1>(u, u0, du, a, col, sh);
+}
+
+// K2 of gnomonic panels
+template <int N>
+__global__ void __launch_bounds__(PcrShape<N>::kThreads)
+    fused_column_pcr_gno_kernel(float* __restrict__ u, float* __restrict__ u0,
+                                const float* __restrict__ du, ColArgs a) {
+  extern __shared__ double sh[];
+  int col = blockIdx.x * (blockDim.x >> a.lshift) + (threadIdx.x >> a.lshift);
+  column_pcr<N, 2>(u, u0, du, a, col, sh);
 }
 
 template <bool kBg = false>
@@ -1925,7 +2091,6 @@ __global__ void __launch_bounds__(PcrShape<3>::kThreads, 2)
     fused_column_pcr2_kernel(float* __restrict__ u, float* __restrict__ u0,
                              const float* __restrict__ du, ColArgs a) {
   extern __shared__ double sh[];
-  if (a.dtp) a.dt = *a.dtp;
   int col = blockIdx.x * (blockDim.x >> a.lshift) + (threadIdx.x >> a.lshift);
   column_pcr2<3, kBg>(u, u0, du, a, col, sh);
 }
@@ -1971,15 +2136,18 @@ __device__ __forceinline__ int find_member(const BatchMember* m, int nm, int ite
 
 // persistent K1 over the (member, tile) work list; kSph: every member is a
 // spherical-polar block (merged face pass with geometric sources)
-template <bool kSph>
+template <int kSph>
 __global__ void __launch_bounds__(kRhsThreads)
     batch_rhs_kernel(const float* __restrict__ u, float* __restrict__ du,
                      const BatchMember* __restrict__ mem, int nm, int total) {
   extern __shared__ float smem[];
   for (int item = blockIdx.x; item < total; item += gridDim.x) {
     int m = find_member(mem, nm, item, kListTiles);
-    FusedArgs a = mem[m].a;
-    rhs_tile<kSph ? 1 : -1, kSph>(u + mem[m].off, du + mem[m].off, a, item - mem[m].tile0, smem);
+    // (curvilinear members read their arguments in place, Cartesian ones
+    // from a copy: either way without a copy in scratch memory)
+    RhsArgs<kSph> a = mem[m].a;
+    rhs_tile<kSph ? 1 : -1, kSph>(u + mem[m].off, du + mem[m].off, a, item - mem[m].tile0, smem,
+                                  a.dt);
     __syncthreads();
   }
 }
@@ -2007,7 +2175,7 @@ __global__ void __launch_bounds__(kColThreads)
 }
 
 // persistent cell-parallel K2 over the (member, column group) work list
-template <int N, bool kSph>
+template <int N, int kSph>
 __global__ void __launch_bounds__(PcrShape<N>::kThreads)
     batch_column_pcr_kernel(float* __restrict__ u, float* __restrict__ u0,
                             const float* __restrict__ du, const BatchMember* __restrict__ mem,
@@ -2082,8 +2250,8 @@ static size_t rhs_face_count(const FusedArgs& a, bool all) {
 
 static size_t rhs_smem_bytes(const FusedArgs& a) {
   size_t H = (size_t)(a.tx1 + 6) * (a.tx2 + 2 * a.h2) * (a.tx3 + 2 * a.h3);
-  // spherical-polar tiles keep the x1 face pressures after the fluxes
-  size_t fp = a.geo.coord == SNAPX_COORD_SPHERICAL_POLAR ? (size_t)(a.tx1 + 1) * a.tx2 * a.tx3 : 0;
+  // curvilinear tiles keep the x1 face pressures after the fluxes
+  size_t fp = a.geo.coord != SNAPX_COORD_CARTESIAN ? (size_t)(a.tx1 + 1) * a.tx2 * a.tx3 : 0;
   // the perturbation path keeps the total density as a sixth plane
   size_t bgp = a.bgt ? H : 0;
   return sizeof(float) * (5 * (H + rhs_face_count(a, a.merged)) + fp + bgp);
@@ -2091,13 +2259,11 @@ static size_t rhs_smem_bytes(const FusedArgs& a) {
 
 static int check_fused(const snapx_config* c) {
   if (int e = validate_config(c)) return e;
-  SNAPX_CHECK_ARG(c->coord != SNAPX_COORD_GNOMONIC_EQUIANGLE,
-                  "fused path: gnomonic-equiangle panels run on the plain path");
-  if (c->coord == SNAPX_COORD_SPHERICAL_POLAR) {
-    // spherical K2 is the cell-parallel column kernel only
+  if (c->coord != SNAPX_COORD_CARTESIAN) {
+    // the curvilinear K2 is the cell-parallel column kernel only
     const int N = ((c->implicit_scheme >> 3) & 1) ? 5 : 3;
     SNAPX_CHECK_ARG(c->nx1 <= (N == 3 ? PcrShape<3>::kThreads : PcrShape<5>::kThreads),
-                    "fused path: spherical-polar columns of at most 256 (3x3) / 128 (5x5) cells");
+                    "fused path: curvilinear columns of at most 256 (3x3) / 128 (5x5) cells");
   }
   SNAPX_CHECK_ARG(c->nghost == 3, "fused path: nghost must be 3");
   // the reference advances custom ghost cells with the stage combine and the
@@ -2164,8 +2330,9 @@ static FusedArgs fill_fused_args(const snapx_config* cfg, Tile t, double dt) {
     a.zin[n] = (float)zi, a.zout[n] = (float)zo;
   }
   // one pass over all faces when they number at most two per thread;
-  // spherical-polar tiles always take it (their sources read every face)
-  a.merged = sph || rhs_face_count(a, true) <= 2 * (size_t)kRhsThreads;
+  // curvilinear tiles always take it (their sources read every face)
+  a.merged = cfg->coord != SNAPX_COORD_CARTESIAN ||
+             rhs_face_count(a, true) <= 2 * (size_t)kRhsThreads;
   return a;
 }
 
@@ -2272,7 +2439,8 @@ static int make_col_args(const snapx_config* cfg, double dt, ColArgs* out) {
   c.zout1 = sph ? sph_dx1v(c.geo, g.n1 - g.ng - 1) : g.dx1;
   // 3x3 columns of more than 64 cells: two rows per lane (column_pcr2;
   // Cartesian blocks)
-  c.rows2 = (!sph && !((cfg->implicit_scheme >> 3) & 1) && c.lanes >= 128) ? 1 : 0;
+  c.rows2 = (cfg->coord == SNAPX_COORD_CARTESIAN && !((cfg->implicit_scheme >> 3) & 1) &&
+             c.lanes >= 128) ? 1 : 0;
   if (c.rows2) c.lanes >>= 1, c.lshift -= 1;
   SNAPX_CHECK_ARG(!c.scheme || c.grav1 != 0.,
                   "[ImplicitHydro] forcing does not have const-gravity");
@@ -2324,7 +2492,8 @@ static int fused_step(const snapx_config* cfg, double dt, float* u, void* worksp
   double* scr = reinterpret_cast<double*>(base + 2 * field);
 
   bool march = march_ok(cfg);
-  const bool sph = cfg->coord == SNAPX_COORD_SPHERICAL_POLAR;
+  const bool sph = cfg->coord != SNAPX_COORD_CARTESIAN;  // curvilinear kernels
+  const bool gno = cfg->coord == SNAPX_COORD_GNOMONIC_EQUIANGLE;
   FusedArgs a;
   if (march) {
     if (int e = make_march_args(cfg, dt, &a)) return e;
@@ -2374,8 +2543,9 @@ static int fused_step(const snapx_config* cfg, double dt, float* u, void* worksp
     hipEvent_t ra = ev ? ev[2 * s].a : nullptr, rb = ev ? ev[2 * s].b : nullptr;
     roctxRangePushA(kRhsRange[s]);
     if (sph)
-      hipExtLaunchKernelGGL(fused_rhs_sph_kernel, dim3(nblk), dim3(kRhsThreads), (uint32_t)smem,
-                            st, ra, rb, 0u, (const float*)u, du, a);
+      hipExtLaunchKernelGGL(gno ? fused_rhs_gno_kernel : fused_rhs_sph_kernel, dim3(nblk),
+                            dim3(kRhsThreads), (uint32_t)smem, st, ra, rb, 0u, (const float*)u,
+                            du, a);
     else if (march)
       hipExtLaunchKernelGGL(march_kernel_for(cfg, bg != nullptr), dim3(nblk), dim3(kRhsThreads),
                             (uint32_t)smem, st, ra, rb, 0u, (const float*)u, du, a);
@@ -2400,11 +2570,11 @@ static int fused_step(const snapx_config* cfg, double dt, float* u, void* worksp
       const uint32_t smem = (uint32_t)(sizeof(double) * threads *
                                        (N == 3 ? PcrShape<3>::kPub : PcrShape<5>::kPub));
       if (sph && N == 3)
-        hipExtLaunchKernelGGL(fused_column_pcr_sph_kernel<3>, grid, block, smem, st, ca, cb, 0u, u,
-                              u0, (const float*)du, c);
+        hipExtLaunchKernelGGL(gno ? fused_column_pcr_gno_kernel<3> : fused_column_pcr_sph_kernel<3>,
+                              grid, block, smem, st, ca, cb, 0u, u, u0, (const float*)du, c);
       else if (sph)
-        hipExtLaunchKernelGGL(fused_column_pcr_sph_kernel<5>, grid, block, smem, st, ca, cb, 0u, u,
-                              u0, (const float*)du, c);
+        hipExtLaunchKernelGGL(gno ? fused_column_pcr_gno_kernel<5> : fused_column_pcr_sph_kernel<5>,
+                              grid, block, smem, st, ca, cb, 0u, u, u0, (const float*)du, c);
       else if (bg && N == 3 && c.rows2)
         hipExtLaunchKernelGGL(fused_column_pcr2_kernel<true>, grid, block, smem, st, ca, cb, 0u, u,
                               u0, (const float*)du, c);
@@ -2752,7 +2922,7 @@ static int batch_layout(const snapx_config* members, int nm, double dt, int tx1,
                         std::vector<BatchMember>* table, snapx_batch_info* info) {
   SNAPX_CHECK_ARG(members && nm >= 1, "batch: no members");
   int scheme = members[0].implicit_scheme;
-  const bool sph = members[0].coord == SNAPX_COORD_SPHERICAL_POLAR;
+  const bool sph = members[0].coord != SNAPX_COORD_CARTESIAN;  // curvilinear
   bool three_d = false, two_d = false;
   for (int m = 0; m < nm; ++m) {
     if (int e = check_fused(&members[m])) return e;
@@ -2840,7 +3010,8 @@ extern "C" int snapx_batch_rk3_steps(const snapx_config* members, const snapx_ba
                   "snapx_batch_rk3_steps: workspace too small");
   hipStream_t st = as_stream(stream);
   int nm = info->nmembers;
-  const bool sph = members[0].coord == SNAPX_COORD_SPHERICAL_POLAR;
+  const bool sph = members[0].coord != SNAPX_COORD_CARTESIAN;  // curvilinear
+  const bool gno = members[0].coord == SNAPX_COORD_GNOMONIC_EQUIANGLE;
   char* base = reinterpret_cast<char*>(workspace);
   const BatchMember* table = reinterpret_cast<const BatchMember*>(base);
   size_t field = align256(sizeof(float) * (size_t)info->total_elems);
@@ -2860,12 +3031,9 @@ extern "C" int snapx_batch_rk3_steps(const snapx_config* members, const snapx_ba
     TraceRange step_range("snapx batch step");
     for (int s = 0; s < 3; ++s) {
       if (timing) SNAPX_HIP(hipEventRecord(ev[0].a, st));
-      if (sph)
-        hipLaunchKernelGGL(batch_rhs_kernel<true>, dim3(rhs_grid), dim3(kRhsThreads),
-                           info->rhs_smem, st, u, du, table, nm, info->total_tiles);
-      else
-        hipLaunchKernelGGL(batch_rhs_kernel<false>, dim3(rhs_grid), dim3(kRhsThreads),
-                           info->rhs_smem, st, u, du, table, nm, info->total_tiles);
+      hipLaunchKernelGGL(gno ? batch_rhs_kernel<2> : (sph ? batch_rhs_kernel<1> : batch_rhs_kernel<0>),
+                         dim3(rhs_grid), dim3(kRhsThreads), info->rhs_smem, st, u, du, table, nm,
+                         info->total_tiles);
       SNAPX_LAUNCH_CHECK();
       if (timing) {
         SNAPX_HIP(hipEventRecord(ev[0].b, st));
@@ -2875,18 +3043,12 @@ extern "C" int snapx_batch_rk3_steps(const snapx_config* members, const snapx_ba
         int N = info->vic_size;
         if (info->col_groups > 0) {
           dim3 grid(std::min(info->col_groups, 4096)), block(pcr_threads(N));
-          if (N == 3 && sph)
-            hipLaunchKernelGGL((batch_column_pcr_kernel<3, true>), grid, block, pcr_smem(3), st,
-                               u, u0, du, table, nm, info->col_groups, s);
-          else if (N == 3)
-            hipLaunchKernelGGL((batch_column_pcr_kernel<3, false>), grid, block, pcr_smem(3), st,
-                               u, u0, du, table, nm, info->col_groups, s);
-          else if (sph)
-            hipLaunchKernelGGL((batch_column_pcr_kernel<5, true>), grid, block, pcr_smem(5), st,
-                               u, u0, du, table, nm, info->col_groups, s);
-          else
-            hipLaunchKernelGGL((batch_column_pcr_kernel<5, false>), grid, block, pcr_smem(5), st,
-                               u, u0, du, table, nm, info->col_groups, s);
+          auto k3 = gno ? batch_column_pcr_kernel<3, 2>
+                        : (sph ? batch_column_pcr_kernel<3, 1> : batch_column_pcr_kernel<3, 0>);
+          auto k5 = gno ? batch_column_pcr_kernel<5, 2>
+                        : (sph ? batch_column_pcr_kernel<5, 1> : batch_column_pcr_kernel<5, 0>);
+          hipLaunchKernelGGL(N == 3 ? k3 : k5, grid, block, pcr_smem(N), st, u, u0, du, table, nm,
+                             info->col_groups, s);
         }
         if (info->col_groups2 > 0)
           hipLaunchKernelGGL(batch_column_pcr2_kernel, dim3(std::min(info->col_groups2, 4096)),
@@ -3096,8 +3258,9 @@ extern "C" int snapx_batch_autotune(const snapx_config* members, int nmembers, d
     }
     int grid = std::min(inf.total_tiles, 512);
     // K1 reads u and writes du only: the state is not changed
-    auto rhs = members[0].coord == SNAPX_COORD_SPHERICAL_POLAR ? batch_rhs_kernel<true>
-                                                               : batch_rhs_kernel<false>;
+    auto rhs = members[0].coord == SNAPX_COORD_GNOMONIC_EQUIANGLE ? batch_rhs_kernel<2>
+               : members[0].coord != SNAPX_COORD_CARTESIAN        ? batch_rhs_kernel<1>
+                                                                   : batch_rhs_kernel<0>;
     hipLaunchKernelGGL(rhs, dim3(grid), dim3(kRhsThreads), inf.rhs_smem, st, u, du, table,
                        nmembers, inf.total_tiles);
     (void)hipEventRecord(e0, st);

@@ -1,10 +1,12 @@
-"""GPU parity of the gnomonic-equiangle panel (plain path) against the oracle.
+"""GPU parity of the gnomonic-equiangle panel against the oracle.
 
-Same bars as the other geometries of the plain path: EOS and the coordinate
+Plain path, same bars as its other geometries: EOS and the coordinate
 forward to 1e-13 of each variable's largest value, the hydro tendency to the
 tolerance of the implicit scheme, RK3 steps to 1e-12 per conserved variable,
 the CFL step to 1e-14; a uniform pressure at rest keeps no tendency with the
-solvers that work in the face frames (LMARS, HLLC)."""
+solvers that work in the face frames (LMARS, HLLC).  Fused FP32 path: 1e-5
+per conserved variable against the FP64 oracle, single blocks and batch
+members."""
 import numpy as np
 import pytest
 import torch
@@ -168,7 +170,99 @@ def test_fp32_plain_path_close_to_fp64():
     assert (e < 1e-5).all(), e
 
 
-def test_fused_path_refuses_panels():
-    oc, w, u = _panel()
-    with pytest.raises(capi.SnapxError, match="gnomonic"):
-        capi.fused_rk3_step(cfg_of(oc), 0.01, to_dev(u, torch.float32))
+@pytest.mark.parametrize("case", [("lmars", 1, (12, 10, 10)), ("hllc", 9, (12, 10, 10)),
+                                  ("roe", 1, (12, 10, 10)), ("lmars", 0, (12, 10, 10)),
+                                  ("hllc", 1, (100, 8, 8))],
+                         ids=["lmars_vic3", "hllc_vic5", "roe_vic3", "lmars_explicit",
+                              "tall_hllc_vic3"])
+def test_fused_steps_vs_fp64_oracle(case):
+    """The FP32 fused path on a whole panel (face frames and metric sources
+    in K1, per-layer areas and the metric EOS in the FP64 column solve)
+    against the FP64 oracle: the fused bar, 1e-5 per conserved variable after
+    three steps; stored ghosts as the reference boundary pass leaves them."""
+    riemann, scheme, n = case
+    oc, w, u = _panel(riemann, scheme, n=n)
+    cfg = cfg_of(oc)
+    uf = to_dev(u, torch.float32)
+    u_ref = u
+    for _ in range(3):
+        u_ref, _ = O.rk3_step(oc, 0.01, u_ref)
+        capi.fused_rk3_step(cfg, 0.01, uf)
+    got = host(uf)
+    e = var_errors(got, u_ref, O.interior(oc), vel_scale=1.0)
+    assert (e < 1e-5).all(), e
+    assert np.array_equal(O.apply_bc(oc, got, O.CONSERVED), got)
+
+
+def test_fused_part_of_a_panel():
+    """Part of a panel (x2, x3 in [-0.3, 0.5], 16 x 20 x 20): the fused bar."""
+    oc, w, u = _panel("hllc", 1, n=(16, 20, 20), lo=-0.3, hi=0.5)
+    uf = to_dev(u, torch.float32)
+    u_ref = u
+    for _ in range(2):
+        u_ref, _ = O.rk3_step(oc, 0.005, u_ref)
+        capi.fused_rk3_step(cfg_of(oc), 0.005, uf)
+    e = var_errors(host(uf), u_ref, O.interior(oc), vel_scale=1.0)
+    assert (e < 1e-5).all(), e
+
+
+@pytest.mark.parametrize("riemann", ["lmars", "hllc"])
+def test_fused_rest_state_keeps_still(riemann):
+    """A uniform pressure at rest stays at rest on the fused path: the face
+    frames and the face-pressure form of the sources balance in FP32 to
+    roundoff (velocities below 1e-5 of the sound speed after three steps)."""
+    oc = O.make_config(nx1=12, nx2=10, nx3=10, nghost=3, x1min=1.0, x1max=1.5, x2min=-PI4,
+                       x2max=PI4, x3min=-PI4, x3max=PI4, coord="gnomonic-equiangle",
+                       riemann=riemann, implicit_scheme=0, bc=("outflow",) * 6)
+    n3, n2, n1 = O.shape(oc)
+    w = np.zeros((5, n3, n2, n1))
+    w[0], w[4] = 1.3, 2.5
+    u = O.prim2cons(oc, w)
+    uf = to_dev(u, torch.float32)
+    for _ in range(3):
+        capi.fused_rk3_step(cfg_of(oc), 1e-3, uf)
+    got = host(uf)[O.interior(oc)]
+    cs = np.sqrt(1.4 * 2.5 / 1.3)
+    assert np.abs(got[1:4] / 1.3).max() < 1e-5 * cs
+    assert np.abs(got[0] - 1.3).max() < 1e-6
+
+
+@pytest.mark.parametrize("scheme", [1, 9])
+def test_panels_as_batch(scheme):
+    """Gnomonic panels of different shapes and seeds as members of one batch
+    (persistent K1 with face frames and metric sources, cell-parallel K2):
+    after the autotuner and two steps each member matches the FP64 oracle at
+    the fused bar; a batch mixing panels and spherical-polar blocks is
+    rejected."""
+    shapes = [(12, 10, 10), (16, 12, 12), (20, 8, 8)] if scheme == 1 else [(12, 10, 10), (10, 8, 8)]
+    cases = [_panel("hllc" if m % 2 else "lmars", scheme, n=n, seed=50 + m)
+             for m, n in enumerate(shapes)]
+    dt = 0.01
+    b = capi.Batch([cfg_of(oc) for oc, _, _ in cases], dt)
+    u = torch.cat([to_dev(u0, torch.float32).reshape(-1) for _, _, u0 in cases])
+    before = u.clone()
+    b.autotune(u, dt, trials=2)
+    assert torch.equal(u, before)
+    b.steps(u, 2)
+    for m, (oc, _, u0) in enumerate(cases):
+        u_ref = u0
+        for _ in range(2):
+            u_ref, _ = O.rk3_step(oc, dt, u_ref)
+        e = var_errors(host(b.member(u, m)), u_ref, O.interior(oc), vel_scale=1.0)
+        assert (e < 1e-5).all(), (m, shapes[m], e)
+    sph = O.make_config(nx1=12, nx2=10, nx3=10, nghost=3, x1min=1.0, x1max=1.5, x2min=0.5,
+                        x2max=2.5, x3min=0.0, x3max=1.0, coord="spherical-polar",
+                        implicit_scheme=scheme, grav1=-0.5)
+    with pytest.raises(capi.SnapxError, match="same coordinate"):
+        capi.Batch([cfg_of(cases[0][0]), cfg_of(sph)], dt)
+
+
+def test_fused_columns_too_tall_are_refused():
+    """Curvilinear columns run on the cell-parallel solve only: a panel with
+    more than 256 cells along x1 (3x3 blocks) is refused with a message."""
+    oc, w, u = _panel(n=(12, 10, 10))
+    cfg = cfg_of(oc)
+    cfg.nx1 = 300
+    uf = torch.zeros((5, 16, 16, 306), dtype=torch.float32, device="cuda")
+    with pytest.raises(capi.SnapxError, match="curvilinear columns"):
+        capi.fused_rk3_step(cfg, 0.01, uf)
