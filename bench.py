@@ -201,7 +201,7 @@ def ensemble_members(capi, n=96, seed=2024):
 
 
 def c5_sensitivity(capi, dev, nsteps=200, every=20, ndir=2, wind=(0.0, 0.0), richardson=False,
-                   roll=0.0, eps=1e-6):
+                   roll=0.0, eps=1e-6, f32=False, return_grad=False):
     """Adjoint sensitivity on the 3-D box 128 x 128 x 96 of 200 m with the
     capped background of the 3-D workload (FP64):
     loss = 0.5 sum (theta_final - theta_target)^2 after nsteps SSP-RK3 steps,
@@ -214,7 +214,12 @@ def c5_sensitivity(capi, dev, nsteps=200, every=20, ndir=2, wind=(0.0, 0.0), ric
     where central differences straddle a kink.  `roll` adds a convective roll
     in the x1-x2 plane of that vertical amplitude (m/s), v1 = roll sin(pi z /
     H) cos(2 pi x / L) with the matching divergence-free v2, so that the x1
-    faces leave the upwind switch of LMARS as well."""
+    faces leave the upwind switch of LMARS as well.
+
+    With `f32` the forward is the fused FP32 path (snapx_fused_rk3_steps) and
+    the gradient the FP32-storage adjoint (FP32 tapes and adjoint fields,
+    FP64 column solves; fused steps between the checkpoints); the target and
+    the finite differences stay FP64."""
     import torch
     from snapy_linfel_amd import adjoint as A
     R, cp, p0 = 287.0, 1004.5, 1e5
@@ -266,17 +271,28 @@ def c5_sensitivity(capi, dev, nsteps=200, every=20, ndir=2, wind=(0.0, 0.0), ric
     th_target = theta(run(dtt))
 
     def loss_of(u):
-        return 0.5 * ((theta(u) - th_target)[3:-3, 3:-3, 3:-3] ** 2).sum()
+        return 0.5 * ((theta(u.double()) - th_target)[3:-3, 3:-3, 3:-3] ** 2).sum()
 
+    if f32:
+        ws32 = capi.fused_workspace(cfg)
+
+        def run_fwd(dth):
+            u = state(dth).float()
+            capi.fused_rk3_steps(cfg, dt, nsteps, u, ws32)
+            return u
+    else:
+        run_fwd = run
+    run_fwd(dth0)  # warm-up (first launches, workspace)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    L0 = float(loss_of(run(dth0)))
+    L0 = float(loss_of(run_fwd(dth0)))
     torch.cuda.synchronize()
     t_fwd = time.perf_counter() - t0
     x = dth0.clone().requires_grad_(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    y = A.integrate(state(x), cfg, dt, nsteps, checkpoint_every=every)
+    u_start = state(x).float() if f32 else state(x)
+    y = A.integrate(u_start, cfg, dt, nsteps, checkpoint_every=every)
     loss = loss_of(y)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
@@ -300,16 +316,20 @@ def c5_sensitivity(capi, dev, nsteps=200, every=20, ndir=2, wind=(0.0, 0.0), ric
         if ndir > 4 and (k + 1) % 8 == 0:
             print(f"c5 finite differences: {k + 1}/{ndir} directions, max rel err "
                   f"{max(rels):.3e}", file=sys.stderr, flush=True)
-    return {"steps": nsteps, "checkpoint_every": every, "wind_m_s": list(wind), "roll_m_s": roll,
-            "fd": (f"richardson {eps:g}/{2 * eps:g}" if richardson else f"central {eps:g}"),
-            "loss": L0,
-            "forward_s": round(t_fwd, 3), "forward_plus_adjoint_s": round(t_grad, 3),
-            "gradient_over_forward": round(t_grad / t_fwd, 3),
-            "backward_s": round(t_back, 3), "backward_over_forward": round(t_back / t_fwd, 3),
-            "replay": "each segment recorded once (tapes of stage inputs, primitives, "
-                      "tendencies, column blocks); the last segment recorded in the forward",
-            "fd_directions": ndir, "fd_rel_err_max": max(rels),
-            "fd_rel_err_median": sorted(rels)[len(rels) // 2]}
+    out = {"steps": nsteps, "checkpoint_every": every, "wind_m_s": list(wind), "roll_m_s": roll,
+           "precision": ("FP32 fused forward; FP32-storage adjoint with FP64 column solves"
+                         if f32 else "FP64 plain forward and adjoint"),
+           "fd": (f"richardson {eps:g}/{2 * eps:g}" if richardson else f"central {eps:g}"),
+           "loss": L0,
+           "forward_s": round(t_fwd, 3), "forward_plus_adjoint_s": round(t_grad, 3),
+           "gradient_over_forward": round(t_grad / t_fwd, 3),
+           "backward_s": round(t_back, 3), "backward_over_forward": round(t_back / t_fwd, 3),
+           "replay": "each segment recorded once (tapes of stage inputs, primitives, "
+                     "tendencies, column blocks); the last segment recorded in the forward",
+           "fd_directions": ndir,
+           "fd_rel_err_max": max(rels) if rels else None,
+           "fd_rel_err_median": sorted(rels)[len(rels) // 2] if rels else None}
+    return (out, grad) if return_grad else out
 
 
 # VALU issue ceiling of MI355X: 256 CUs x 4 SIMDs, a wave64 VALU instruction
@@ -777,8 +797,17 @@ def main():
     # ---- adjoint sensitivity: 3-D box 128 x 128 x 96 of 200 m, FP64 ----
     if not args.no_adjoint and not args.no_c5:
         log("adjoint sensitivity, 128x128x96, 200 steps (FP64)")
-        extra["c5_sensitivity_box3d_128x128x96_fp64"] = c5_sensitivity(
-            capi, dev, ndir=args.c5_directions)
+        c5, g64 = c5_sensitivity(capi, dev, ndir=args.c5_directions, return_grad=True)
+        extra["c5_sensitivity_box3d_128x128x96_fp64"] = c5
+        log("adjoint sensitivity, 128x128x96, 200 steps (FP32 fused forward, FP32 adjoint)")
+        c5f, g32 = c5_sensitivity(capi, dev, ndir=0, f32=True, return_grad=True)
+        dg = (g32 - g64).double()
+        c5f["gradient_vs_fp64_adjoint"] = {
+            "l2_rel": float(dg.norm() / g64.norm()),
+            "linf_rel": float(dg.abs().max() / g64.abs().max())}
+        c5f["backward_over_fp64_backward"] = round(c5f["backward_s"] / c5["backward_s"], 3)
+        extra["c5_sensitivity_box3d_128x128x96_fp32"] = c5f
+        del g64, g32, dg
         # the same gradient check off the kinks of the state at rest: a
         # uniform horizontal wind (Roe |u| at the x2 / x3 faces) and a
         # convective roll (LMARS upwind switch at the x1 faces)

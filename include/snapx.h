@@ -383,6 +383,29 @@ int snapx_adjoint_sweep(const snapx_config* cfg, double dt, void* tape,
                         size_t tape_bytes, double* lambda, void* work,
                         size_t work_bytes, void* stream);
 
+/* The same adjoint with FP32 storage: u_in, u_out and lambda are float
+   [5][nc3][nc2][nc1], and the tape holds FP32 stage inputs, primitives and
+   explicit tendencies.  Arithmetic in registers is FP64; the VIC blocks,
+   the column solves (forward and transposed) and their tape entries stay
+   FP64.  The recompute runs the explicit part on the FP32 plain path
+   (snapx_hydro_forward, SNAPX_F32), so the stage states follow an FP32
+   trajectory with FP64 column solves.  Tape and work space are about half
+   the FP64 sizes.  No reference counterpart. */
+size_t snapx_adjoint_f32_tape_bytes(const snapx_config* cfg);
+size_t snapx_adjoint_f32_work_bytes(const snapx_config* cfg);
+size_t snapx_adjoint_f32_workspace_bytes(const snapx_config* cfg);
+int snapx_adjoint_f32_rk3_step(const snapx_config* cfg, double dt,
+                               const float* u_in, float* lambda,
+                               void* workspace, size_t workspace_bytes,
+                               void* stream);
+int snapx_adjoint_f32_record(const snapx_config* cfg, double dt,
+                             const float* u_in, float* u_out, void* tape,
+                             size_t tape_bytes, void* work, size_t work_bytes,
+                             void* stream);
+int snapx_adjoint_f32_sweep(const snapx_config* cfg, double dt, void* tape,
+                            size_t tape_bytes, float* lambda, void* work,
+                            size_t work_bytes, void* stream);
+
 /* ---------------------------------------------------------------------- */
 /* Batched members (ensembles / blocks of uneven size; no reference        */
 /* counterpart: the reference runs one block per thread of                 */

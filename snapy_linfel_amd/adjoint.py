@@ -21,6 +21,12 @@ segment: about 26 fields of 5 x cells doubles per step), and then sweeps the
 tapes in reverse (``snapx_adjoint_sweep``): one forward recompute per step
 instead of two.  The replay follows the recompute's own column solve (cyclic
 reduction), so its states agree with the forward's to rounding.
+
+float32 states select the FP32-storage adjoint (``snapx_adjoint_f32_*``):
+FP32 tapes and adjoint fields, FP64 arithmetic in registers and FP64 column
+solves.  Their forward steps are plain FP32 steps (``RK3Step``) or, in the
+segments of ``integrate`` whose tapes are recorded later, fused FP32 steps
+(``snapx_fused_rk3_steps``) when the configuration runs on the fused path.
 """
 from __future__ import annotations
 
@@ -29,11 +35,18 @@ import torch
 from . import capi
 
 
+_DTYPES = (torch.float64, torch.float32)
+
+
+def _code(dtype):
+    return capi.F64 if dtype == torch.float64 else capi.F32
+
+
 class RK3Step(torch.autograd.Function):
     @staticmethod
     def forward(ctx, u, cfg, dt, ws):
-        if u.dtype != torch.float64 or not u.is_cuda:
-            raise TypeError("RK3Step: u must be a float64 GPU tensor")
+        if u.dtype not in _DTYPES or not u.is_cuda:
+            raise TypeError("RK3Step: u must be a float64 or float32 GPU tensor")
         u_in = u.detach().contiguous()
         out = u_in.clone()
         w = torch.empty_like(out)
@@ -46,7 +59,8 @@ class RK3Step(torch.autograd.Function):
     def backward(ctx, grad):
         (u_in,) = ctx.saved_tensors
         lam = grad.detach().contiguous().clone()
-        ws = ctx.ws if ctx.ws is not None else capi.adjoint_workspace(ctx.cfg, u_in.device)
+        ws = (ctx.ws if ctx.ws is not None else
+              capi.adjoint_workspace(ctx.cfg, u_in.device, u_in.dtype))
         capi.adjoint_rk3_step(ctx.cfg, ctx.dt, u_in, lam, ws)
         return lam, None, None, None
 
@@ -56,12 +70,12 @@ class _Tapes:
     calls, allocated on first use and shared by all segments of a
     trajectory (the segments' backwards run one after another)."""
 
-    def __init__(self, cfg, device):
+    def __init__(self, cfg, device, dtype=torch.float64):
         self.cfg = cfg
-        self.nbytes = capi.adjoint_tape_bytes(cfg)
+        self.nbytes = capi.adjoint_tape_bytes(cfg, dtype)
         if self.nbytes == 0:
             raise capi.SnapxError("adjoint: " + capi.lib().snapx_last_error().decode())
-        self.work = capi.adjoint_work(cfg, device)
+        self.work = capi.adjoint_work(cfg, device, dtype)
         self.tapes = []
         self.states = None
 
@@ -85,25 +99,33 @@ def _record(cfg, dt, u_in, n, st, out=None):
     return tapes
 
 
+def _fused_ok(cfg):
+    return capi.lib().snapx_fused_workspace_bytes(capi.C.byref(cfg)) > 0
+
+
 class TapedSegment(torch.autograd.Function):
-    """n FP64 steps as one autograd node: the forward keeps only the
-    segment's input (plain-path steps); the backward records the n steps
+    """n steps as one autograd node: the forward keeps only the segment's
+    input (plain-path steps; fused steps for float32 states when the
+    configuration runs on the fused path); the backward records the n steps
     once and sweeps their tapes in reverse.  With ``last`` (the segment
     whose backward runs first) the forward records the tapes itself, through
     the recompute's steps, and the backward only sweeps."""
 
     @staticmethod
     def forward(ctx, u, cfg, dt, n, store, last=False):
-        if u.dtype != torch.float64 or not u.is_cuda:
-            raise TypeError("TapedSegment: u must be a float64 GPU tensor")
+        if u.dtype not in _DTYPES or not u.is_cuda:
+            raise TypeError("TapedSegment: u must be a float64 or float32 GPU tensor")
         u_in = u.detach().contiguous()
         out = torch.empty_like(u_in)
         if last:
             _record(cfg, dt, u_in, n, store, out)
+        elif u_in.dtype == torch.float32 and _fused_ok(cfg):
+            out.copy_(u_in)
+            capi.fused_rk3_steps(cfg, dt, n, out)
         else:
             out.copy_(u_in)
             w = torch.empty_like(out)
-            ws = capi.rk3_workspace(cfg, capi.F64, out.device)
+            ws = capi.rk3_workspace(cfg, _code(out.dtype), out.device)
             for _ in range(n):
                 capi.rk3_step(cfg, dt, out, w, ws)
         ctx.save_for_backward(u_in)
@@ -122,8 +144,9 @@ class TapedSegment(torch.autograd.Function):
 
 
 def rk3_step(u, cfg, dt, ws=None):
-    """One differentiable FP64 SSP-RK3 step; ``ws`` is an optional adjoint
-    workspace (``capi.adjoint_workspace``) reused by the backward."""
+    """One differentiable SSP-RK3 step (FP64, or FP32 storage for float32
+    states); ``ws`` is an optional adjoint workspace
+    (``capi.adjoint_workspace`` of u's dtype) reused by the backward."""
     return RK3Step.apply(u, cfg, dt, ws)
 
 
@@ -138,7 +161,7 @@ def integrate(u, cfg, dt, nsteps, ws=None, checkpoint_every=None, tape=True):
     its stages again, so memory holds nsteps / K + K states and no tapes."""
     if (checkpoint_every and checkpoint_every < nsteps and tape and torch.is_grad_enabled()
             and u.requires_grad):
-        store = _Tapes(cfg, u.device)
+        store = _Tapes(cfg, u.device, u.dtype)
         done = 0
         while done < nsteps:
             n = min(checkpoint_every, nsteps - done)
@@ -146,7 +169,7 @@ def integrate(u, cfg, dt, nsteps, ws=None, checkpoint_every=None, tape=True):
             done += n
         return u
     if ws is None and torch.is_grad_enabled() and u.requires_grad:
-        ws = capi.adjoint_workspace(cfg, u.device)
+        ws = capi.adjoint_workspace(cfg, u.device, u.dtype)
     if not checkpoint_every or checkpoint_every >= nsteps or not torch.is_grad_enabled():
         for _ in range(nsteps):
             u = RK3Step.apply(u, cfg, dt, ws)

@@ -143,6 +143,12 @@ SIGNATURES = {
     "snapx_adjoint_work_bytes": (_Z, [_CFG]),
     "snapx_adjoint_record": (_I, [_CFG, _D, _P, _P, _P, _Z, _P, _Z, _P]),
     "snapx_adjoint_sweep": (_I, [_CFG, _D, _P, _Z, _P, _P, _Z, _P]),
+    "snapx_adjoint_f32_workspace_bytes": (_Z, [_CFG]),
+    "snapx_adjoint_f32_rk3_step": (_I, [_CFG, _D, _P, _P, _P, _Z, _P]),
+    "snapx_adjoint_f32_tape_bytes": (_Z, [_CFG]),
+    "snapx_adjoint_f32_work_bytes": (_Z, [_CFG]),
+    "snapx_adjoint_f32_record": (_I, [_CFG, _D, _P, _P, _P, _Z, _P, _Z, _P]),
+    "snapx_adjoint_f32_sweep": (_I, [_CFG, _D, _P, _Z, _P, _P, _Z, _P]),
     "snapx_batch_workspace_bytes": (_Z, [_CFG, _I]),
     "snapx_batch_plan": (_I, [_CFG, _I, _D, _I, _I, _I, _P, _Z, _P]),
     "snapx_batch_rk3_steps": (_I, [_CFG, _P, _I, _P, _P, _Z, _P, _P]),
@@ -340,38 +346,65 @@ def check_state(cfg, u, flag):
           "check_state")
 
 
-def adjoint_workspace(cfg, device="cuda"):
-    return workspace(lib().snapx_adjoint_workspace_bytes(C.byref(cfg)), device)
+def _adj(name, dtype):
+    """The FP64 (float64) or FP32-storage (float32) adjoint entry point."""
+    torch = _torch()
+    if dtype == torch.float64:
+        return getattr(lib(), "snapx_adjoint_" + name)
+    if dtype == torch.float32:
+        return getattr(lib(), "snapx_adjoint_f32_" + name)
+    raise SnapxError(f"adjoint: state dtype {dtype}, expected float64 or float32")
+
+
+def _adj_pair(cfg, a, b, what):
+    torch = _torch()
+    _check_state(cfg, a, (torch.float64, torch.float32))
+    if b is not None:
+        _check_state(cfg, b, (a.dtype,))
+        if b.device != a.device:
+            raise SnapxError(f"{what}: tensors on different devices")
+
+
+def adjoint_workspace(cfg, device="cuda", dtype=None):
+    dtype = _torch().float64 if dtype is None else dtype
+    return workspace(_adj("workspace_bytes", dtype)(C.byref(cfg)), device)
 
 
 def adjoint_rk3_step(cfg, dt, u_in, lam, ws=None):
-    """lam <- (d u_out / d u_in)^T lam for one FP64 plain-path RK3 step."""
-    ws = adjoint_workspace(cfg, u_in.device) if ws is None else ws
-    check(lib().snapx_adjoint_rk3_step(C.byref(cfg), dt, ptr(u_in), ptr(lam), ptr(ws),
+    """lam <- (d u_out / d u_in)^T lam for one plain-path RK3 step (FP64, or
+    FP32 storage with FP64 column solves when the tensors are float32)."""
+    _adj_pair(cfg, u_in, lam, "adjoint_rk3_step")
+    ws = adjoint_workspace(cfg, u_in.device, u_in.dtype) if ws is None else ws
+    check(_adj("rk3_step", u_in.dtype)(C.byref(cfg), dt, ptr(u_in), ptr(lam), ptr(ws),
                                        ws.numel(), stream_ptr()), "adjoint_rk3_step")
 
 
-def adjoint_tape_bytes(cfg):
-    return lib().snapx_adjoint_tape_bytes(C.byref(cfg))
+def adjoint_tape_bytes(cfg, dtype=None):
+    dtype = _torch().float64 if dtype is None else dtype
+    return _adj("tape_bytes", dtype)(C.byref(cfg))
 
 
-def adjoint_work(cfg, device="cuda"):
-    return workspace(lib().snapx_adjoint_work_bytes(C.byref(cfg)), device)
+def adjoint_work(cfg, device="cuda", dtype=None):
+    dtype = _torch().float64 if dtype is None else dtype
+    return workspace(_adj("work_bytes", dtype)(C.byref(cfg)), device)
 
 
 def adjoint_record(cfg, dt, u_in, u_out, tape, work):
-    """Forward recompute of one FP64 step from u_in into ``tape`` (a uint8
-    device tensor of adjoint_tape_bytes); u_out (or None) gets the step's
-    output."""
-    check(lib().snapx_adjoint_record(C.byref(cfg), dt, ptr(u_in), ptr(u_out), ptr(tape),
+    """Forward recompute of one step from u_in into ``tape`` (a uint8
+    device tensor of adjoint_tape_bytes for u_in's dtype); u_out (or None)
+    gets the step's output."""
+    _adj_pair(cfg, u_in, u_out, "adjoint_record")
+    check(_adj("record", u_in.dtype)(C.byref(cfg), dt, ptr(u_in), ptr(u_out), ptr(tape),
                                      tape.numel(), ptr(work), work.numel(), stream_ptr()),
           "adjoint_record")
 
 
 def adjoint_sweep(cfg, dt, tape, lam, work):
-    """lam <- (d u_out / d u_in)^T lam from a recorded tape."""
-    check(lib().snapx_adjoint_sweep(C.byref(cfg), dt, ptr(tape), tape.numel(), ptr(lam),
-                                    ptr(work), work.numel(), stream_ptr()), "adjoint_sweep")
+    """lam <- (d u_out / d u_in)^T lam from a recorded tape (of lam's
+    dtype)."""
+    _adj_pair(cfg, lam, None, "adjoint_sweep")
+    check(_adj("sweep", lam.dtype)(C.byref(cfg), dt, ptr(tape), tape.numel(), ptr(lam),
+                                   ptr(work), work.numel(), stream_ptr()), "adjoint_sweep")
 
 
 def fused_workspace(cfg, device="cuda"):

@@ -63,7 +63,8 @@ __device__ __forceinline__ bool interior(const AdjArgs a, int i, int j, int k) {
 
 // reverse of one axis of bc_kernel: one thread per (variable, pencil) folds
 // the ghost adjoints into their source cells, then zeroes the ghosts
-__global__ void bc_adj_kernel(double* __restrict__ lam, AdjArgs a, int dim, int tin, int tout,
+template <typename S>
+__global__ void bc_adj_kernel(S* __restrict__ lam, AdjArgs a, int dim, int tin, int tout,
                               int flip) {
   int s, n, np;
   if (dim == 3) {
@@ -79,7 +80,7 @@ __global__ void bc_adj_kernel(double* __restrict__ lam, AdjArgs a, int dim, int 
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < 5 * np; t += gridDim.x * blockDim.x) {
     int p = t % np, v = t / np;
     int base = dim == 3 ? p * a.n1 : (dim == 2 ? (p / a.n1) * a.n1 * a.n2 + p % a.n1 : p);
-    double* x = lam + v * ncell + base;
+    S* x = lam + v * ncell + base;
     for (int side = 0; side < 2; ++side) {
       int type = side ? tout : tin;
       if (type < 0) continue;
@@ -96,9 +97,9 @@ __global__ void bc_adj_kernel(double* __restrict__ lam, AdjArgs a, int dim, int 
                 : type == SNAPX_BC_OUTFLOW  ? n - ng - 1
                                             : ng + g;
         }
-        double sign = (type == SNAPX_BC_REFLECTING && v == vnorm && flip) ? -1. : 1.;
-        x[src * s] += sign * x[dst * s];
-        x[dst * s] = 0.;
+        const bool neg = type == SNAPX_BC_REFLECTING && v == vnorm && flip;
+        x[src * s] += neg ? -x[dst * s] : x[dst * s];
+        x[dst * s] = S(0);
       }
     }
   }
@@ -106,15 +107,16 @@ __global__ void bc_adj_kernel(double* __restrict__ lam, AdjArgs a, int dim, int 
 
 // --------------------------------------------------------- stage combine
 
-__global__ void combine_adj_kernel(const double* __restrict__ lam, double* __restrict__ lam_du,
-                                   double* __restrict__ lam_us, double* __restrict__ lam_u0,
-                                   double wa, double wb, double wc, int64_t n) {
+template <typename S>
+__global__ void combine_adj_kernel(const S* __restrict__ lam, S* __restrict__ lam_du,
+                                   S* __restrict__ lam_us, S* __restrict__ lam_u0, double wa,
+                                   double wb, double wc, int64_t n) {
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n;
        q += (int64_t)gridDim.x * blockDim.x) {
-    double l = lam[q];
-    lam_du[q] = wc * l;
-    lam_us[q] = wb * l;
-    lam_u0[q] += wa * l;
+    const double l = lam[q];
+    lam_du[q] = S(wc * l);
+    lam_us[q] = S(wb * l);
+    lam_u0[q] = S((double)lam_u0[q] + wa * l);
   }
 }
 
@@ -127,10 +129,10 @@ __device__ __forceinline__ int col_q0(const AdjArgs a, int col) {
   return (k * a.n2 + j) * a.n1 + a.ng;
 }
 
-template <typename T>
-__device__ __forceinline__ void load_w(const double* w, int ncell, int q, T* out) {
+template <typename T, typename S>
+__device__ __forceinline__ void load_w(const S* w, int ncell, int q, T* out) {
 #pragma unroll
-  for (int v = 0; v < 5; ++v) out[v] = T(w[v * ncell + q]);
+  for (int v = 0; v < 5; ++v) out[v] = T((double)w[v * ncell + q]);
 }
 
 // One thread per column: forward assembly + block Thomas (delta), MS-VIC
@@ -138,11 +140,11 @@ __device__ __forceinline__ void load_w(const double* w, int ncell, int q, T* out
 //   lam_w   (rho part of the redistribution)  += ...
 //   lam_dx  (adjoint of the explicit tendency) = ...
 //   dl      [col][layer][2N]: delta and lambda_r for the row-derivative kernel
-template <int N>
+template <int N, typename S>
 __global__ void __launch_bounds__(64)
-    vic_adj_column_kernel(const double* __restrict__ w, const double* __restrict__ dux,
-                          const double* __restrict__ lam_du, double* __restrict__ lam_dx,
-                          double* __restrict__ lam_w, double* __restrict__ abc,
+    vic_adj_column_kernel(const S* __restrict__ w, const S* __restrict__ dux,
+                          const S* __restrict__ lam_du, S* __restrict__ lam_dx,
+                          S* __restrict__ lam_w, double* __restrict__ abc,
                           double* __restrict__ pr, double* __restrict__ dl, AdjArgs a, int ncol) {
   const int NN = N * N;
   const int ncell = a.n1 * a.n2 * a.n3;
@@ -169,14 +171,14 @@ __global__ void __launch_bounds__(64)
     for (int i = 0; i < nl; ++i) {
       int q = q0 + i;
       double rhs[N];
-      rhs[0] = dux[IDN * ncell + q] / a.dt;
-      rhs[1] = dux[IVX * ncell + q] / a.dt;
+      rhs[0] = (double)dux[IDN * ncell + q] / a.dt;
+      rhs[1] = (double)dux[IVX * ncell + q] / a.dt;
       if (N == 3) {
-        rhs[2] = dux[IPR * ncell + q] / a.dt;
+        rhs[2] = (double)dux[IPR * ncell + q] / a.dt;
       } else {
-        rhs[2] = dux[IVY * ncell + q] / a.dt;
-        rhs[3] = dux[IVZ * ncell + q] / a.dt;
-        rhs[4] = dux[IPR * ncell + q] / a.dt;
+        rhs[2] = (double)dux[IVY * ncell + q] / a.dt;
+        rhs[3] = (double)dux[IVZ * ncell + q] / a.dt;
+        rhs[4] = (double)dux[IPR * ncell + q] / a.dt;
       }
       double A[NN], B[NN], C[NN];
       for (int e = 0; e < NN; ++e) {
@@ -223,17 +225,17 @@ __global__ void __launch_bounds__(64)
       }
     }
     // ---- redistribution adjoint ----
-    double S = 0., Bd = 0., G = 0.;
+    double Ssum = 0., Bd = 0., G = 0.;
     for (int i = 0; i < nl; ++i) {
       int q = q0 + i;
       double rho = w[IDN * ncell + q];
       double ai = rho * vol;
-      S += ai * ai;
-      Bd += (dux[IDN * ncell + q] - cdl[i * 2 * N]) * 1.0 * vol;
-      G += lam_du[IDN * ncell + q] * rho * ai;
+      Ssum += ai * ai;
+      Bd += ((double)dux[IDN * ncell + q] - cdl[i * 2 * N]) * 1.0 * vol;
+      G += (double)lam_du[IDN * ncell + q] * rho * ai;
     }
-    G /= S;
-    double lamS = -G * Bd / S;
+    G /= Ssum;
+    double lamS = -G * Bd / Ssum;
     // lambda_delta into the second half of dl (temporarily), direct lam_dx
     for (int i = 0; i < nl; ++i) {
       int q = q0 + i;
@@ -253,7 +255,8 @@ __global__ void __launch_bounds__(64)
         ld[4] = ln[IPR];
       }
       lam_dx[IDN * ncell + q] = G * vol;
-      lam_w[IDN * ncell + q] += ln[IDN] * 2. * rho * vol * Bd / S + lamS * 2. * rho * vol * vol;
+      lam_w[IDN * ncell + q] =
+          (double)lam_w[IDN * ncell + q] + ln[IDN] * 2. * rho * vol * Bd / Ssum + lamS * 2. * rho * vol * vol;
       for (int e = 0; e < N; ++e) cdl[i * 2 * N + N + e] = ld[e];
     }
     // ---- transposed block solve  M^T y = lambda_delta ----
@@ -311,7 +314,7 @@ __global__ void __launch_bounds__(64)
     for (int i = 0; i < nl; ++i) {
       int q = q0 + i;
       const double* y = cdl + i * 2 * N + N;
-      lam_dx[IDN * ncell + q] += y[0] / a.dt;
+      lam_dx[IDN * ncell + q] = (double)lam_dx[IDN * ncell + q] + y[0] / a.dt;
       lam_dx[IVX * ncell + q] = y[1] / a.dt;
       if (N == 3) {
         lam_dx[IPR * ncell + q] = y[2] / a.dt;
@@ -327,9 +330,9 @@ __global__ void __launch_bounds__(64)
 // ---------------------------------------- cell-parallel VIC (forward+adjoint)
 
 // block rows a_i, b_i, c_i of every interior cell, abc [col][layer][3][NN]
-template <int N>
+template <int N, typename S>
 __global__ void __launch_bounds__(kT)
-    vic_asm_kernel(const double* __restrict__ w, double* __restrict__ abc, AdjArgs a, int ncol) {
+    vic_asm_kernel(const S* __restrict__ w, double* __restrict__ abc, AdjArgs a, int ncol) {
   const int NN = N * N;
   const int ncell = a.n1 * a.n2 * a.n3;
   const int nl = a.nx1;
@@ -350,8 +353,8 @@ __global__ void __launch_bounds__(kT)
 }
 
 // rhs layout of the column solve: (rho, v1, p) or (rho, v1, v2, v3, p)
-template <int N>
-__device__ __forceinline__ void vic_pick(const double* x, int ncell, int q, double* r) {
+template <int N, typename S>
+__device__ __forceinline__ void vic_pick(const S* x, int ncell, int q, double* r) {
   r[0] = x[IDN * ncell + q];
   r[1] = x[IVX * ncell + q];
   if (N == 3) {
@@ -412,10 +415,10 @@ __device__ __forceinline__ void vic_row_blocks(const double* __restrict__ abc, i
 // Forward column solve + MS-VIC redistribution of one stage (the recompute
 // of the stage states): du (interior) from the explicit tendency dux; delta
 // is kept in the first half of dl [col][layer][2N] for the reverse sweep.
-template <int N>
+template <int N, typename S>
 __global__ void __launch_bounds__(PcrShape<N>::kThreads)
-    vic_fwd_solve_kernel(const double* __restrict__ w, const double* __restrict__ dux,
-                         const double* __restrict__ abc, double* __restrict__ du,
+    vic_fwd_solve_kernel(const S* __restrict__ w, const S* __restrict__ dux,
+                         const double* __restrict__ abc, S* __restrict__ du,
                          double* __restrict__ dl, AdjArgs a, int ncol) {
   extern __shared__ double sh[];
   const int NN = N * N;
@@ -455,11 +458,11 @@ __global__ void __launch_bounds__(PcrShape<N>::kThreads)
 // (column sums S, Bd, G) and y = M^-T lambda_delta by PCR.  Writes y to the
 // second half of dl, lam_dx (interior) and the rho part of lam_w; lam_dx
 // enters holding lam_du.
-template <int N>
+template <int N, typename S>
 __global__ void __launch_bounds__(PcrShape<N>::kThreads)
-    vic_adj_solve_kernel(const double* __restrict__ w, const double* __restrict__ dux,
-                         const double* __restrict__ lam_du, const double* __restrict__ abc,
-                         double* __restrict__ lam_dx, double* __restrict__ lam_w,
+    vic_adj_solve_kernel(const S* __restrict__ w, const S* __restrict__ dux,
+                         const S* __restrict__ lam_du, const double* __restrict__ abc,
+                         S* __restrict__ lam_dx, S* __restrict__ lam_w,
                          double* __restrict__ dl, AdjArgs a, int ncol) {
   extern __shared__ double sh[];
   const int NN = N * N;
@@ -471,16 +474,16 @@ __global__ void __launch_bounds__(PcrShape<N>::kThreads)
   double rho = L.act ? w[IDN * ncell + L.q] : 0.;
   double lr = L.act ? lam_du[IDN * ncell + L.q] : 0.;
   double ai = rho * vol;
-  double sums[3] = {ai * ai, L.act ? (dux[IDN * ncell + L.q] - d0) * 1.0 * vol : 0.,
+  double sums[3] = {ai * ai, L.act ? ((double)dux[IDN * ncell + L.q] - d0) * 1.0 * vol : 0.,
                     lr * rho * ai};
   column_sums<3>(sums, sh, threadIdx.x, a.lanes, a.lshift);
-  const double S = sums[0], Bd = sums[1], G = sums[2] / S;
-  const double lamS = -G * Bd / S;
+  const double Ssum = sums[0], Bd = sums[1], G = sums[2] / Ssum;
+  const double lamS = -G * Bd / Ssum;
   double A[NN], B[NN], C[NN], ld[N];
   if (L.act) {
     vic_pick<N>(lam_du, ncell, L.q, ld);
     ld[0] = lr - G * vol;
-    lam_w[IDN * ncell + L.q] += lr * 2. * rho * vol * Bd / S + lamS * 2. * rho * vol * vol;
+    lam_w[IDN * ncell + L.q] += lr * 2. * rho * vol * Bd / Ssum + lamS * 2. * rho * vol * vol;
     vic_row_blocks<N>(abc, L.col, L.i, nl, true, A, B, C);
   }
   double y[N];
@@ -562,9 +565,9 @@ __device__ __forceinline__ T abs_contract(T gm1, const T* z, const double* P, co
 // contraction: phi_f = (y_{f-1} - y_f)^T |A|_f (delta_{f-1} - delta_f);
 // G[col][f][0..4] = d phi / d w_{f-1}, G[col][f][5..9] = d phi / d w_f.
 // The Roe state is differentiated with Dual<10>, phi(z) with Dual<5>.
-template <int N>
+template <int N, typename S>
 __global__ void __launch_bounds__(kT)
-    vic_adj_face_kernel(const double* __restrict__ w, const double* __restrict__ dl,
+    vic_adj_face_kernel(const S* __restrict__ w, const double* __restrict__ dl,
                         double* __restrict__ G, AdjArgs a, int ncol) {
   const int ncell = a.n1 * a.n2 * a.n3;
   const int nl = a.nx1;
@@ -610,10 +613,10 @@ __global__ void __launch_bounds__(kT)
 //   phi_j + phi_{j+1} + (y_{j-1} - y_{j+1})^T J(w_j) delta_j
 // (the faces below / above and the flux Jacobian of the cell, which enters
 // row j+1 as its lower and row j-1 as its upper coupling).
-template <int N>
+template <int N, typename S>
 __global__ void __launch_bounds__(kT)
-    vic_adj_cell_kernel(const double* __restrict__ w, const double* __restrict__ dl,
-                        const double* __restrict__ G, double* __restrict__ lam_w, AdjArgs a,
+    vic_adj_cell_kernel(const S* __restrict__ w, const double* __restrict__ dl,
+                        const double* __restrict__ G, S* __restrict__ lam_w, AdjArgs a,
                         int ncol) {
   const int NN = N * N;
   const int ncell = a.n1 * a.n2 * a.n3;
@@ -669,8 +672,8 @@ __device__ __forceinline__ void face_frame(int d, int* vmap) {
 // interface states of face q along the axis with stride s (face f along the
 // axis) in the face frame; the same arithmetic as recon_kernel for the faces
 // whose adjoint is nonzero (f in [ng, n - ng]: never a dummy face)
-template <typename T>
-__device__ __forceinline__ void face_states(const double* w, int ncell, int q, int s, int f,
+template <typename T, typename S>
+__device__ __forceinline__ void face_states(const S* w, int ncell, int q, int s, int f,
                                             const AdjArgs a, int d, const int* vmap, T* wl,
                                             T* wr) {
   int r = d == 0 ? 0 : 1;
@@ -688,7 +691,7 @@ __device__ __forceinline__ void face_states(const double* w, int ncell, int q, i
       int c = f + m;
       int n = d == 0 ? a.n1 : (d == 1 ? a.n2 : a.n3);
       c = c < 0 ? 0 : (c >= n ? n - 1 : c);
-      win[m + 3] = T(w[vmap[v] * ncell + q + (c - f) * s]);
+      win[m + 3] = T((double)w[vmap[v] * ncell + q + (c - f) * s]);
     }
     // window centred on cell f (index 3) for the right state and cell f-1
     // (index 2) for the left state
@@ -707,7 +710,8 @@ __device__ __forceinline__ int axis_stride(const AdjArgs a, int d) {
 // Faces of axis d whose flux reaches an interior cell (a face f in
 // [ng, n - ng], never a dummy face), and lambda_F there in the face frame:
 // lambda_F = (-dt A / V)(lambda_dx[left cell] - lambda_dx[right cell]).
-__device__ __forceinline__ bool face_lambda(const AdjArgs a, int d, const double* lam_dx,
+template <typename S>
+__device__ __forceinline__ bool face_lambda(const AdjArgs a, int d, const S* lam_dx,
                                             int ncell, int q, int s, const int* vmap,
                                             int& f, double* lf) {
   const DivMod dn1(a.n1), dn2(a.n2);
@@ -738,9 +742,10 @@ __device__ __forceinline__ bool face_lambda(const AdjArgs a, int d, const double
 
 // Interface states (face frame, x1 LR revision applied) of the faces that
 // carry an adjoint, into buf [2][5][ncell] (slot order; left, then right).
+template <typename S>
 __global__ void __launch_bounds__(kT)
-    face_state_kernel(const double* __restrict__ w, const double* __restrict__ lam_dx,
-                      double* __restrict__ buf, AdjArgs a, int d) {
+    face_state_kernel(const S* __restrict__ w, const S* __restrict__ lam_dx,
+                      S* __restrict__ buf, AdjArgs a, int d) {
   const int ncell = a.n1 * a.n2 * a.n3;
   const int s = axis_stride(a, d), n = axis_len(a, d);
   int vmap[5];
@@ -750,7 +755,7 @@ __global__ void __launch_bounds__(kT)
     double lf[5];
     if (!face_lambda(a, d, lam_dx, ncell, q, s, vmap, f, lf)) continue;
     double wl[5], wr[5];
-    face_states<double>(w, ncell, q, s, f, a, d, vmap, wl, wr);
+    face_states<double, S>(w, ncell, q, s, f, a, d, vmap, wl, wr);
     if (d == 0 && a.grav1 != 0.) {
       if (f == a.ng) wl[IDN] = wr[IDN], wl[IPR] = wr[IPR];
       if (f == n - a.ng) wr[IDN] = wl[IDN], wr[IPR] = wl[IPR];
@@ -767,10 +772,9 @@ __global__ void __launch_bounds__(kT)
 // every face of axis d: lam_wlr [2][5][ncell] (ILT, IRT; variable order)
 // replaces the face states face_state_kernel left there (same index).
 // One instantiation per solver: each holds only its own solver's registers.
-template <int SOLVER>
+template <int SOLVER, typename S>
 __global__ void __launch_bounds__(kT)
-    flux_adj_kernel(const double* __restrict__ lam_dx, double* __restrict__ lam_wlr, AdjArgs a,
-                    int d) {
+    flux_adj_kernel(const S* __restrict__ lam_dx, S* __restrict__ lam_wlr, AdjArgs a, int d) {
   const int ncell = a.n1 * a.n2 * a.n3;
   const int s = axis_stride(a, d), n = axis_len(a, d);
   const int solver = SOLVER;
@@ -847,9 +851,10 @@ __global__ void __launch_bounds__(kT)
 // span are loaded once; linear interpolants (DC, CP3, CP5) contribute their
 // constant weights, the others are differentiated with Dual<1> seeded at
 // cell c.
+template <typename S>
 __global__ void __launch_bounds__(kT)
-    recon_adj_kernel(const double* __restrict__ w, const double* __restrict__ lam_wlr,
-                     double* __restrict__ lam_w, AdjArgs a, int d) {
+    recon_adj_kernel(const S* __restrict__ w, const S* __restrict__ lam_wlr,
+                     S* __restrict__ lam_w, AdjArgs a, int d) {
   const int ncell = a.n1 * a.n2 * a.n3;
   const int s = axis_stride(a, d), n = axis_len(a, d);
   int r = d == 0 ? 0 : 1;
@@ -870,7 +875,7 @@ __global__ void __launch_bounds__(kT)
     // WENO5 without scaling: weno_face_kernel + weno_gather_kernel
     if (tv == SNAPX_INTERP_WENO5 && !scale) continue;
     bool linear = tv == SNAPX_INTERP_DC || tv == SNAPX_INTERP_CP3 || tv == SNAPX_INTERP_CP5;
-    const double* wv = w + (int64_t)v * ncell + q;
+    const S* wv = w + (int64_t)v * ncell + q;
     double cells[13];
     if (!linear) {
 #pragma unroll
@@ -924,9 +929,10 @@ __global__ void __launch_bounds__(kT)
 // lambda_w[c] += sum over the faces f = c - m of the interpolant's constant
 // weights, which come from evaluating it on unit windows (folded by the
 // compiler) instead of on dual numbers per cell.
+template <typename S>
 __global__ void __launch_bounds__(kT)
-    recon_adj_linear_kernel(const double* __restrict__ lam_wlr, double* __restrict__ lam_w,
-                            AdjArgs a, int d) {
+    recon_adj_linear_kernel(const S* __restrict__ lam_wlr, S* __restrict__ lam_w, AdjArgs a,
+                            int d) {
   const int ncell = a.n1 * a.n2 * a.n3;
   const int s = axis_stride(a, d), n = axis_len(a, d);
   const int r = d == 0 ? 0 : 1;
@@ -1014,9 +1020,10 @@ __device__ __forceinline__ void weno5_lo_grad(double f0, double f1, double f2, d
 // two interface states to the six cells f-3..f+2 of their windows,
 //   G6[k][q] = ll d(hi of cell f-1)/dw_{f-3+k} + lr d(lo of cell f)/dw_{f-3+k},
 // zero for faces that carry no adjoint.
+template <typename S>
 __global__ void __launch_bounds__(kT)
-    weno_face_kernel(const double* __restrict__ w, const double* __restrict__ lam_wlr,
-                     double* __restrict__ G6, AdjArgs a, int d, int v) {
+    weno_face_kernel(const S* __restrict__ w, const S* __restrict__ lam_wlr,
+                     S* __restrict__ G6, AdjArgs a, int d, int v) {
   const int ncell = a.n1 * a.n2 * a.n3;
   const int s = axis_stride(a, d), n = axis_len(a, d);
   const DivMod dn1(a.n1), dn2(a.n2);
@@ -1027,7 +1034,7 @@ __global__ void __launch_bounds__(kT)
     double g6[6] = {0., 0., 0., 0., 0., 0.};
     double ll = lam_wlr[v * ncell + q], lr = lam_wlr[(5 + v) * ncell + q];
     if (f >= a.ng && f <= n - a.ng && (ll != 0. || lr != 0.)) {
-      const double* wv = w + (int64_t)v * ncell + q;
+      const S* wv = w + (int64_t)v * ncell + q;
       double c6[6];
 #pragma unroll
       for (int m = 0; m < 6; ++m) c6[m] = wv[(m - 3) * s];  // cells f-3 .. f+2
@@ -1049,9 +1056,10 @@ __global__ void __launch_bounds__(kT)
 
 // lam_w[v][c] += sum over faces f = c+3-k (k = 0..5) carrying an adjoint of
 // G6[k][f]
+template <typename S>
 __global__ void __launch_bounds__(kT)
-    weno_gather_kernel(const double* __restrict__ G6, double* __restrict__ lam_w, AdjArgs a,
-                       int d, int v) {
+    weno_gather_kernel(const S* __restrict__ G6, S* __restrict__ lam_w, AdjArgs a, int d,
+                       int v) {
   const int ncell = a.n1 * a.n2 * a.n3;
   const int s = axis_stride(a, d), n = axis_len(a, d);
   const DivMod dn1(a.n1), dn2(a.n2);
@@ -1070,8 +1078,9 @@ __global__ void __launch_bounds__(kT)
 }
 
 // constant gravity (src/forcing/const_gravity.cpp:26-45) adjoint
-__global__ void grav_adj_kernel(const double* __restrict__ w, const double* __restrict__ lam_dx,
-                                double* __restrict__ lam_w, AdjArgs a) {
+template <typename S>
+__global__ void grav_adj_kernel(const S* __restrict__ w, const S* __restrict__ lam_dx,
+                                S* __restrict__ lam_w, AdjArgs a) {
   const int ncell = a.n1 * a.n2 * a.n3;
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < ncell; q += gridDim.x * blockDim.x) {
     double rho = w[IDN * ncell + q];
@@ -1097,8 +1106,8 @@ __global__ void grav_adj_kernel(const double* __restrict__ w, const double* __re
 // isentropic ghost refill (src/hydro/hydro.cpp:231-269) adjoint, one thread
 // per column; wpre holds the primitives before the refill (only rho, p of
 // the boundary cells are read)
-__global__ void ghost_adj_kernel(const double* __restrict__ w, double* __restrict__ lam_w,
-                                 AdjArgs a) {
+template <typename S>
+__global__ void ghost_adj_kernel(const S* __restrict__ w, S* __restrict__ lam_w, AdjArgs a) {
   const int ncell = a.n1 * a.n2 * a.n3;
   int ncol = a.n2 * a.n3;
   for (int col = blockIdx.x * blockDim.x + threadIdx.x; col < ncol;
@@ -1134,8 +1143,9 @@ __global__ void ghost_adj_kernel(const double* __restrict__ w, double* __restric
 }
 
 // cons2prim (src/eos/ideal_gas_impl.h:15-35) adjoint: lam_u += J^T lam_w
-__global__ void c2p_adj_kernel(const double* __restrict__ u, const double* __restrict__ lam_w,
-                               double* __restrict__ lam_u, AdjArgs a) {
+template <typename S>
+__global__ void c2p_adj_kernel(const S* __restrict__ u, const S* __restrict__ lam_w,
+                               S* __restrict__ lam_u, AdjArgs a) {
   const int ncell = a.n1 * a.n2 * a.n3;
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < ncell; q += gridDim.x * blockDim.x) {
     Dual<5> U[5], W[5];
@@ -1154,13 +1164,15 @@ __global__ void c2p_adj_kernel(const double* __restrict__ u, const double* __res
   }
 }
 
-__global__ void zero_kernel(double* __restrict__ x, int64_t n) {
+template <typename S>
+__global__ void zero_kernel(S* __restrict__ x, int64_t n) {
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n;
        q += (int64_t)gridDim.x * blockDim.x)
-    x[q] = 0.;
+    x[q] = S(0);
 }
 
-__global__ void axpy_kernel(double* __restrict__ y, const double* __restrict__ x, int64_t n) {
+template <typename S>
+__global__ void axpy_kernel(S* __restrict__ y, const S* __restrict__ x, int64_t n) {
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n;
        q += (int64_t)gridDim.x * blockDim.x)
     y[q] += x[q];
@@ -1189,57 +1201,68 @@ static AdjArgs adj_args(const snapx_config* c, double dt) {
 
 static size_t al(size_t x) { return (x + 255) & ~size_t(255); }
 
+template <typename S>
+constexpr int dtype_of() {
+  return sizeof(S) == sizeof(double) ? SNAPX_F64 : SNAPX_F32;
+}
+
 // What the reverse sweep of one step reads from its forward recompute: per
 // stage the input state, its primitives (revised ghosts), the explicit
-// tendency, and the VIC blocks and (delta, rows) of the column solve.
+// tendency (storage type S), and the VIC blocks and (delta, rows) of the
+// column solve (FP64 for either storage type).
+template <typename S>
 struct AdjTape {
-  double *us[3], *w[3], *dux[3], *abc[3], *dl[3];
+  S *us[3], *w[3], *dux[3];
+  double *abc[3], *dl[3];
   size_t bytes;
 };
 
-static AdjTape tape_layout(const snapx_config* c, char* base) {
+template <typename S>
+static AdjTape<S> tape_layout(const snapx_config* c, char* base) {
   Grid g = make_grid(c);
-  size_t f = al(sizeof(double) * 5 * (size_t)g.ncell);
+  size_t f = al(sizeof(S) * 5 * (size_t)g.ncell);
   int N = ((c->implicit_scheme >> 3) & 1) ? 5 : 3;
   size_t ncol = (size_t)(g.n2 > 1 ? c->nx2 : 1) * (g.n3 > 1 ? c->nx3 : 1);
   size_t cells = ncol * c->nx1;
-  AdjTape t;
+  AdjTape<S> t;
   size_t off = 0;
   auto take = [&](size_t b) {
     char* p = base + off;
     off += al(b);
-    return reinterpret_cast<double*>(p);
+    return p;
   };
   for (int k = 0; k < 3; ++k) {
-    t.us[k] = take(f);
-    t.w[k] = take(f);
-    t.dux[k] = take(f);
+    t.us[k] = reinterpret_cast<S*>(take(f));
+    t.w[k] = reinterpret_cast<S*>(take(f));
+    t.dux[k] = reinterpret_cast<S*>(take(f));
   }
   for (int k = 0; k < 3; ++k) {
-    t.abc[k] = take(sizeof(double) * cells * 3 * N * N);
-    t.dl[k] = take(sizeof(double) * cells * 2 * N);
+    t.abc[k] = reinterpret_cast<double*>(take(sizeof(double) * cells * 3 * N * N));
+    t.dl[k] = reinterpret_cast<double*>(take(sizeof(double) * cells * 2 * N));
   }
   t.bytes = off;
   return t;
 }
 
+template <typename S>
 struct AdjSpace {
-  double *us[3], *w[3], *dux[3], *lam, *lam_us, *lam_u0, *lam_du, *lam_dx, *lam_w, *lam_wlr,
-      *abc[3], *pr, *dl[3], *G, *G6;
+  S *us[3], *w[3], *dux[3], *lam, *lam_us, *lam_u0, *lam_du, *lam_dx, *lam_w, *lam_wlr, *G6;
+  double *abc[3], *pr, *dl[3], *G;
   char* hydro_ws;
   size_t hydro_bytes, bytes;
 };
 
 // The work space of the recompute and the sweep (everything but the tape);
 // the tape fields of the result point into `tape`.
-static AdjSpace adj_layout(const snapx_config* c, char* base, char* tape) {
+template <typename S>
+static AdjSpace<S> adj_layout(const snapx_config* c, char* base, char* tape) {
   Grid g = make_grid(c);
-  size_t f = al(sizeof(double) * 5 * (size_t)g.ncell);
+  size_t f = al(sizeof(S) * 5 * (size_t)g.ncell);
   int N = ((c->implicit_scheme >> 3) & 1) ? 5 : 3;
   size_t ncol = (size_t)(g.n2 > 1 ? c->nx2 : 1) * (g.n3 > 1 ? c->nx3 : 1);
   size_t cells = ncol * c->nx1;
-  AdjSpace s;
-  AdjTape t = tape_layout(c, tape);
+  AdjSpace<S> s;
+  AdjTape<S> t = tape_layout<S>(c, tape);
   for (int k = 0; k < 3; ++k) {
     s.us[k] = t.us[k], s.w[k] = t.w[k], s.dux[k] = t.dux[k];
     s.abc[k] = t.abc[k], s.dl[k] = t.dl[k];
@@ -1248,25 +1271,27 @@ static AdjSpace adj_layout(const snapx_config* c, char* base, char* tape) {
   auto take = [&](size_t b) {
     char* p = base + off;
     off += al(b);
-    return reinterpret_cast<double*>(p);
+    return p;
   };
-  s.lam = take(f);
-  s.lam_us = take(f);
-  s.lam_u0 = take(f);
-  s.lam_du = take(f);
-  s.lam_dx = take(f);
-  s.lam_w = take(f);
-  s.lam_wlr = take(2 * f);
-  s.pr = take(sizeof(double) * cells * (N * N + N));
-  s.G = take(sizeof(double) * ncol * (c->nx1 + 1) * 10);
-  s.G6 = take(6 * f / 5);
-  s.hydro_bytes = snapx_hydro_workspace_bytes(c, SNAPX_F64);
-  s.hydro_ws = reinterpret_cast<char*>(take(s.hydro_bytes));
+  auto field = [&](size_t b) { return reinterpret_cast<S*>(take(b)); };
+  s.lam = field(f);
+  s.lam_us = field(f);
+  s.lam_u0 = field(f);
+  s.lam_du = field(f);
+  s.lam_dx = field(f);
+  s.lam_w = field(f);
+  s.lam_wlr = field(2 * f);
+  s.pr = reinterpret_cast<double*>(take(sizeof(double) * cells * (N * N + N)));
+  s.G = reinterpret_cast<double*>(take(sizeof(double) * ncol * (c->nx1 + 1) * 10));
+  s.G6 = field(6 * f / 5);
+  s.hydro_bytes = snapx_hydro_workspace_bytes(c, dtype_of<S>());
+  s.hydro_ws = take(s.hydro_bytes);
   s.bytes = off;
   return s;
 }
 
-static int bc_adj(const snapx_config* c, const AdjArgs a, double* lam, hipStream_t st) {
+template <typename S>
+static int bc_adj(const snapx_config* c, const AdjArgs a, S* lam, hipStream_t st) {
   int dims[3] = {3, 2, 1};
   int ns[3] = {a.n1, a.n2, a.n3};
   for (int ax = 2; ax >= 0; --ax) {  // reverse of the forward x1, x2, x3 order
@@ -1274,82 +1299,56 @@ static int bc_adj(const snapx_config* c, const AdjArgs a, double* lam, hipStream
     int tin = c->bc[2 * ax], tout = c->bc[2 * ax + 1];
     if (tin < 0 && tout < 0) continue;
     int np = a.n1 * a.n2 * a.n3 / ns[ax];
-    hipLaunchKernelGGL(bc_adj_kernel, blocks_for(5 * np, kT), dim3(kT), 0, st, lam, a, dims[ax],
-                       tin, tout, 1);
+    hipLaunchKernelGGL(bc_adj_kernel<S>, blocks_for(5 * np, kT), dim3(kT), 0, st, lam, a,
+                       dims[ax], tin, tout, 1);
     SNAPX_LAUNCH_CHECK();
   }
   return 0;
 }
 
-}  // namespace snapx
-
-using namespace snapx;
-
-extern "C" size_t snapx_adjoint_tape_bytes(const snapx_config* cfg) {
-  if (validate_config(cfg)) return 0;
-  if (cfg->coord != SNAPX_COORD_CARTESIAN) return 0;
-  return tape_layout(cfg, nullptr).bytes;
-}
-
-extern "C" size_t snapx_adjoint_work_bytes(const snapx_config* cfg) {
-  if (validate_config(cfg)) return 0;
-  if (cfg->coord != SNAPX_COORD_CARTESIAN) return 0;
-  return adj_layout(cfg, nullptr, nullptr).bytes;
-}
-
-// one step's work space followed by its tape
-extern "C" size_t snapx_adjoint_workspace_bytes(const snapx_config* cfg) {
-  const size_t w = snapx_adjoint_work_bytes(cfg);
-  return w ? w + snapx_adjoint_tape_bytes(cfg) : 0;
-}
-
-namespace snapx {
-
 static bool adj_pcr(const AdjArgs a, int N) {
   return a.lanes <= (N == 3 ? PcrShape<3>::kThreads : PcrShape<5>::kThreads);
 }
 
-template <int N>
-static int launch_vic_fwd(const AdjArgs& a, const AdjSpace& s, int k, double* du, int ncol,
+template <int N, typename S>
+static int launch_vic_fwd(const AdjArgs& a, const AdjSpace<S>& s, int k, S* du, int ncol,
                           hipStream_t st) {
   const int T = PcrShape<N>::kThreads;
-  hipLaunchKernelGGL(vic_asm_kernel<N>, blocks_for((int64_t)ncol * a.nx1, kT), dim3(kT), 0, st,
-                     s.w[k], s.abc[k], a, ncol);
+  hipLaunchKernelGGL((vic_asm_kernel<N, S>), blocks_for((int64_t)ncol * a.nx1, kT), dim3(kT), 0,
+                     st, s.w[k], s.abc[k], a, ncol);
   SNAPX_LAUNCH_CHECK();
   int cpw = T >> a.lshift;
-  hipLaunchKernelGGL(vic_fwd_solve_kernel<N>, dim3((ncol + cpw - 1) / cpw), dim3(T),
+  hipLaunchKernelGGL((vic_fwd_solve_kernel<N, S>), dim3((ncol + cpw - 1) / cpw), dim3(T),
                      sizeof(double) * T * PcrShape<N>::kPub, st, s.w[k], s.dux[k], s.abc[k], du,
                      s.dl[k], a, ncol);
   SNAPX_LAUNCH_CHECK();
   return 0;
 }
 
-template <int N>
-static int launch_vic_adj(const AdjArgs& a, const AdjSpace& s, int k, int ncol, hipStream_t st) {
+template <int N, typename S>
+static int launch_vic_adj(const AdjArgs& a, const AdjSpace<S>& s, int k, int ncol,
+                          hipStream_t st) {
   const int T = PcrShape<N>::kThreads;
   if (adj_pcr(a, N)) {
     // blocks and delta of this stage come from the forward recompute
     int cpw = T >> a.lshift;
-    hipLaunchKernelGGL(vic_adj_solve_kernel<N>, dim3((ncol + cpw - 1) / cpw), dim3(T),
+    hipLaunchKernelGGL((vic_adj_solve_kernel<N, S>), dim3((ncol + cpw - 1) / cpw), dim3(T),
                        sizeof(double) * T * PcrShape<N>::kPub, st, s.w[k], s.dux[k], s.lam_du,
                        s.abc[k], s.lam_dx, s.lam_w, s.dl[k], a, ncol);
   } else {
-    hipLaunchKernelGGL(vic_adj_column_kernel<N>, blocks_for(ncol, 64), dim3(64), 0, st, s.w[k],
-                       s.dux[k], s.lam_du, s.lam_dx, s.lam_w, s.abc[k], s.pr, s.dl[k], a, ncol);
+    hipLaunchKernelGGL((vic_adj_column_kernel<N, S>), blocks_for(ncol, 64), dim3(64), 0, st,
+                       s.w[k], s.dux[k], s.lam_du, s.lam_dx, s.lam_w, s.abc[k], s.pr, s.dl[k], a,
+                       ncol);
   }
   SNAPX_LAUNCH_CHECK();
-  hipLaunchKernelGGL(vic_adj_face_kernel<N>, blocks_for((int64_t)ncol * (a.nx1 + 1), kT),
+  hipLaunchKernelGGL((vic_adj_face_kernel<N, S>), blocks_for((int64_t)ncol * (a.nx1 + 1), kT),
                      dim3(kT), 0, st, s.w[k], s.dl[k], s.G, a, ncol);
   SNAPX_LAUNCH_CHECK();
-  hipLaunchKernelGGL(vic_adj_cell_kernel<N>, blocks_for((int64_t)ncol * (a.nx1 + 2), kT),
+  hipLaunchKernelGGL((vic_adj_cell_kernel<N, S>), blocks_for((int64_t)ncol * (a.nx1 + 2), kT),
                      dim3(kT), 0, st, s.w[k], s.dl[k], s.G, s.lam_w, a, ncol);
   SNAPX_LAUNCH_CHECK();
   return 0;
 }
-
-}  // namespace snapx
-
-namespace snapx {
 
 static int adjoint_check(const snapx_config* cfg) {
   if (int e = validate_config(cfg)) return e;
@@ -1369,12 +1368,16 @@ static const double kStageW[3][3] = {
 // Forward recompute of one step from u_in into the tape: stage inputs, their
 // primitives (revised ghosts), explicit tendencies, VIC blocks and delta.
 // With u_out, also the step's output (third stage combined, boundaries).
-static int adjoint_record(const snapx_config* cfg, double dt, const double* u_in, double* u_out,
-                          const AdjSpace& s, hipStream_t st) {
+// The explicit part runs on the plain path in the storage type; the column
+// solve is this file's FP64 one for either type.
+template <typename S>
+static int adjoint_record(const snapx_config* cfg, double dt, const S* u_in, S* u_out,
+                          const AdjSpace<S>& s, hipStream_t st) {
   Grid g = make_grid(cfg);
   AdjArgs a = adj_args(cfg, dt);
   int64_t n = 5 * (int64_t)g.ncell;
-  size_t fb = sizeof(double) * n;
+  size_t fb = sizeof(S) * n;
+  const int dtype = dtype_of<S>();
   snapx_config c0 = *cfg;
   c0.implicit_scheme = 0;
   const auto& W = kStageW;
@@ -1384,42 +1387,43 @@ static int adjoint_record(const snapx_config* cfg, double dt, const double* u_in
   SNAPX_HIP(hipMemcpyAsync(s.us[0], u_in, fb, hipMemcpyDeviceToDevice, st));
   for (int k = 0; k < 3; ++k) {
     if (int e = snapx_hydro_forward(&c0, dt, s.us[k], s.w[k], s.dux[k], nullptr, nullptr,
-                                    nullptr, s.hydro_ws, s.hydro_bytes, SNAPX_F64, st))
+                                    nullptr, s.hydro_ws, s.hydro_bytes, dtype, st))
       return e;
     // du of the stage: the column solve on a copy of the explicit tendency
     // (without u_out, the third stage's solve matters only for the blocks and
     // delta it leaves for the reverse sweep)
-    double* du = s.lam_du;
+    S* du = s.lam_du;
     SNAPX_HIP(hipMemcpyAsync(du, s.dux[k], fb, hipMemcpyDeviceToDevice, st));
     if (cfg->implicit_scheme) {
       if (adj_pcr(a, N)) {
-        if (int e = N == 3 ? launch_vic_fwd<3>(a, s, k, du, ncol, st)
-                           : launch_vic_fwd<5>(a, s, k, du, ncol, st))
+        if (int e = N == 3 ? launch_vic_fwd<3, S>(a, s, k, du, ncol, st)
+                           : launch_vic_fwd<5, S>(a, s, k, du, ncol, st))
           return e;
       } else if (k < 2 || u_out) {
         if (int e = snapx_hydro_forward(cfg, dt, s.us[k], s.lam_w, du, nullptr, nullptr,
-                                        nullptr, s.hydro_ws, s.hydro_bytes, SNAPX_F64, st))
+                                        nullptr, s.hydro_ws, s.hydro_bytes, dtype, st))
           return e;
       }
     }
-    double* next = k < 2 ? s.us[k + 1] : u_out;
+    S* next = k < 2 ? s.us[k + 1] : u_out;
     if (!next) break;
     if (int e = snapx_rk_combine(next, s.us[0], s.us[k], du, W[k][0], W[k][1], W[k][2], n,
-                                 SNAPX_F64, st))
+                                 dtype, st))
       return e;
-    if (int e = snapx_apply_bc(cfg, next, 5, SNAPX_CONSERVED, SNAPX_F64, st)) return e;
+    if (int e = snapx_apply_bc(cfg, next, 5, SNAPX_CONSERVED, dtype, st)) return e;
   }
   return 0;
 }
 
 // Reverse sweep of one step over its tape: lambda (dL/du_out) becomes
 // dL/du_in.
-static int adjoint_sweep(const snapx_config* cfg, double dt, double* lambda, const AdjSpace& s,
+template <typename S>
+static int adjoint_sweep(const snapx_config* cfg, double dt, S* lambda, const AdjSpace<S>& s,
                          hipStream_t st) {
   Grid g = make_grid(cfg);
   AdjArgs a = adj_args(cfg, dt);
   int64_t n = 5 * (int64_t)g.ncell;
-  size_t fb = sizeof(double) * n;
+  size_t fb = sizeof(S) * n;
   const auto& W = kStageW;
   dim3 bn = blocks_for(n, kT), bc = blocks_for(g.ncell, kT);
   const int N = ((cfg->implicit_scheme >> 3) & 1) ? 5 : 3;
@@ -1427,41 +1431,42 @@ static int adjoint_sweep(const snapx_config* cfg, double dt, double* lambda, con
   const int ns[3] = {g.n1, g.n2, g.n3};
   TraceRange range("snapx adjoint sweep");
   SNAPX_HIP(hipMemcpyAsync(s.lam, lambda, fb, hipMemcpyDeviceToDevice, st));
-  hipLaunchKernelGGL(zero_kernel, bn, dim3(kT), 0, st, s.lam_u0, n);
+  hipLaunchKernelGGL(zero_kernel<S>, bn, dim3(kT), 0, st, s.lam_u0, n);
   SNAPX_LAUNCH_CHECK();
   for (int stg = 2; stg >= 0; --stg) {
-    if (int e = bc_adj(cfg, a, s.lam, st)) return e;
-    hipLaunchKernelGGL(combine_adj_kernel, bn, dim3(kT), 0, st, s.lam, s.lam_du, s.lam_us,
+    if (int e = bc_adj<S>(cfg, a, s.lam, st)) return e;
+    hipLaunchKernelGGL(combine_adj_kernel<S>, bn, dim3(kT), 0, st, s.lam, s.lam_du, s.lam_us,
                        s.lam_u0, W[stg][0], W[stg][1], W[stg][2], n);
     SNAPX_LAUNCH_CHECK();
-    hipLaunchKernelGGL(zero_kernel, bn, dim3(kT), 0, st, s.lam_w, n);
-    hipLaunchKernelGGL(zero_kernel, bn, dim3(kT), 0, st, s.lam_dx, n);
+    hipLaunchKernelGGL(zero_kernel<S>, bn, dim3(kT), 0, st, s.lam_w, n);
+    hipLaunchKernelGGL(zero_kernel<S>, bn, dim3(kT), 0, st, s.lam_dx, n);
     SNAPX_LAUNCH_CHECK();
     // the column solve rewrites the interior of du only; ghost entries (the
     // gravity forcing) pass through
-    hipLaunchKernelGGL(axpy_kernel, bn, dim3(kT), 0, st, s.lam_dx, s.lam_du, n);
+    hipLaunchKernelGGL(axpy_kernel<S>, bn, dim3(kT), 0, st, s.lam_dx, s.lam_du, n);
     SNAPX_LAUNCH_CHECK();
     if (cfg->implicit_scheme)
-      if (int e = N == 3 ? launch_vic_adj<3>(a, s, stg, ncol, st)
-                         : launch_vic_adj<5>(a, s, stg, ncol, st))
+      if (int e = N == 3 ? launch_vic_adj<3, S>(a, s, stg, ncol, st)
+                         : launch_vic_adj<5, S>(a, s, stg, ncol, st))
         return e;
-    const double* w = s.w[stg];
-    hipLaunchKernelGGL(grav_adj_kernel, bc, dim3(kT), 0, st, w, s.lam_dx, s.lam_w, a);
+    const S* w = s.w[stg];
+    hipLaunchKernelGGL(grav_adj_kernel<S>, bc, dim3(kT), 0, st, w, s.lam_dx, s.lam_w, a);
     SNAPX_LAUNCH_CHECK();
     for (int d = 0; d < 3; ++d) {
       if (ns[d] == 1) continue;
-      hipLaunchKernelGGL(face_state_kernel, bc, dim3(kT), 0, st, w, s.lam_dx, s.lam_wlr, a, d);
+      hipLaunchKernelGGL(face_state_kernel<S>, bc, dim3(kT), 0, st, w, s.lam_dx, s.lam_wlr, a,
+                         d);
       SNAPX_LAUNCH_CHECK();
       int solver = cfg->riemann[d];
       if (solver == SNAPX_RIEMANN_LMARS)
-        hipLaunchKernelGGL(flux_adj_kernel<SNAPX_RIEMANN_LMARS>, bc, dim3(kT), 0, st, s.lam_dx,
-                           s.lam_wlr, a, d);
+        hipLaunchKernelGGL((flux_adj_kernel<SNAPX_RIEMANN_LMARS, S>), bc, dim3(kT), 0, st,
+                           s.lam_dx, s.lam_wlr, a, d);
       else if (solver == SNAPX_RIEMANN_HLLC)
-        hipLaunchKernelGGL(flux_adj_kernel<SNAPX_RIEMANN_HLLC>, bc, dim3(kT), 0, st, s.lam_dx,
-                           s.lam_wlr, a, d);
+        hipLaunchKernelGGL((flux_adj_kernel<SNAPX_RIEMANN_HLLC, S>), bc, dim3(kT), 0, st,
+                           s.lam_dx, s.lam_wlr, a, d);
       else
-        hipLaunchKernelGGL(flux_adj_kernel<SNAPX_RIEMANN_ROE>, bc, dim3(kT), 0, st, s.lam_dx,
-                           s.lam_wlr, a, d);
+        hipLaunchKernelGGL((flux_adj_kernel<SNAPX_RIEMANN_ROE, S>), bc, dim3(kT), 0, st,
+                           s.lam_dx, s.lam_wlr, a, d);
       SNAPX_LAUNCH_CHECK();
       int r = d == 0 ? 0 : 1;
       {
@@ -1473,79 +1478,172 @@ static int adjoint_sweep(const snapx_config* cfg, double dt, double* lambda, con
                  (tv == SNAPX_INTERP_WENO5 && !cfg->recon_scale[r]);
         };
         if (lean(ty) && lean(ty2))
-          hipLaunchKernelGGL(recon_adj_linear_kernel, bn, dim3(kT), 0, st, s.lam_wlr, s.lam_w, a,
-                             d);
+          hipLaunchKernelGGL(recon_adj_linear_kernel<S>, bn, dim3(kT), 0, st, s.lam_wlr,
+                             s.lam_w, a, d);
         else
-          hipLaunchKernelGGL(recon_adj_kernel, bn, dim3(kT), 0, st, w, s.lam_wlr, s.lam_w, a, d);
+          hipLaunchKernelGGL(recon_adj_kernel<S>, bn, dim3(kT), 0, st, w, s.lam_wlr, s.lam_w,
+                             a, d);
         SNAPX_LAUNCH_CHECK();
       }
       if (cfg->recon_type[r] == SNAPX_INTERP_WENO5 && !cfg->recon_scale[r]) {
         for (int v = 0; v < 5; ++v) {
           bool weno = cfg->recon_shock[r] || v == IDN;
           if (!weno) continue;
-          hipLaunchKernelGGL(weno_face_kernel, bc, dim3(kT), 0, st, w, s.lam_wlr, s.G6, a, d, v);
+          hipLaunchKernelGGL(weno_face_kernel<S>, bc, dim3(kT), 0, st, w, s.lam_wlr, s.G6, a,
+                             d, v);
           SNAPX_LAUNCH_CHECK();
-          hipLaunchKernelGGL(weno_gather_kernel, bc, dim3(kT), 0, st, s.G6, s.lam_w, a, d, v);
+          hipLaunchKernelGGL(weno_gather_kernel<S>, bc, dim3(kT), 0, st, s.G6, s.lam_w, a, d,
+                             v);
           SNAPX_LAUNCH_CHECK();
         }
       }
     }
     if (g.n1 > 1 && cfg->grav1 != 0.) {
-      hipLaunchKernelGGL(ghost_adj_kernel, blocks_for(g.n2 * g.n3, kT), dim3(kT), 0, st, w,
+      hipLaunchKernelGGL(ghost_adj_kernel<S>, blocks_for(g.n2 * g.n3, kT), dim3(kT), 0, st, w,
                          s.lam_w, a);
       SNAPX_LAUNCH_CHECK();
     }
-    hipLaunchKernelGGL(c2p_adj_kernel, bc, dim3(kT), 0, st, s.us[stg], s.lam_w, s.lam_us, a);
+    hipLaunchKernelGGL(c2p_adj_kernel<S>, bc, dim3(kT), 0, st, s.us[stg], s.lam_w, s.lam_us, a);
     SNAPX_LAUNCH_CHECK();
     // lam_us is now dL/du_stg; the next iteration folds its ghosts (u_stg,
     // stg > 0, left a boundary pass); u_0 = u_in keeps its ghost adjoints
     SNAPX_HIP(hipMemcpyAsync(s.lam, s.lam_us, fb, hipMemcpyDeviceToDevice, st));
   }
-  hipLaunchKernelGGL(axpy_kernel, bn, dim3(kT), 0, st, s.lam, s.lam_u0, n);
+  hipLaunchKernelGGL(axpy_kernel<S>, bn, dim3(kT), 0, st, s.lam, s.lam_u0, n);
   SNAPX_LAUNCH_CHECK();
   SNAPX_HIP(hipMemcpyAsync(lambda, s.lam, fb, hipMemcpyDeviceToDevice, st));
   return 0;
 }
 
+template <typename S>
+static size_t tape_bytes(const snapx_config* cfg) {
+  if (validate_config(cfg)) return 0;
+  if (cfg->coord != SNAPX_COORD_CARTESIAN) return 0;
+  return tape_layout<S>(cfg, nullptr).bytes;
+}
+
+template <typename S>
+static size_t work_bytes(const snapx_config* cfg) {
+  if (validate_config(cfg)) return 0;
+  if (cfg->coord != SNAPX_COORD_CARTESIAN) return 0;
+  return adj_layout<S>(cfg, nullptr, nullptr).bytes;
+}
+
+// one step's work space followed by its tape
+template <typename S>
+static size_t workspace_bytes(const snapx_config* cfg) {
+  const size_t w = work_bytes<S>(cfg);
+  return w ? w + tape_bytes<S>(cfg) : 0;
+}
+
+template <typename S>
+static int rk3_step(const snapx_config* cfg, double dt, const S* u_in, S* lambda,
+                    void* workspace, size_t workspace_bytes_, void* stream, const char* who) {
+  if (int e = adjoint_check(cfg)) return e;
+  const std::string w(who);
+  SNAPX_CHECK_ARG(u_in && lambda && workspace, w + ": null pointer");
+  SNAPX_CHECK_ARG(workspace_bytes_ >= workspace_bytes<S>(cfg), w + ": workspace too small");
+  char* base = reinterpret_cast<char*>(workspace);
+  AdjSpace<S> s = adj_layout<S>(cfg, base, base + work_bytes<S>(cfg));
+  hipStream_t st = as_stream(stream);
+  if (int e = adjoint_record<S>(cfg, dt, u_in, nullptr, s, st)) return e;
+  return adjoint_sweep<S>(cfg, dt, lambda, s, st);
+}
+
+template <typename S>
+static int record(const snapx_config* cfg, double dt, const S* u_in, S* u_out, void* tape,
+                  size_t tape_bytes_, void* work, size_t work_bytes_, void* stream,
+                  const char* who) {
+  if (int e = adjoint_check(cfg)) return e;
+  const std::string w(who);
+  SNAPX_CHECK_ARG(u_in && tape && work, w + ": null pointer");
+  SNAPX_CHECK_ARG(u_out != u_in, w + ": u_out must not alias u_in");
+  SNAPX_CHECK_ARG(tape_bytes_ >= tape_bytes<S>(cfg), w + ": tape too small");
+  SNAPX_CHECK_ARG(work_bytes_ >= work_bytes<S>(cfg), w + ": work space too small");
+  AdjSpace<S> s =
+      adj_layout<S>(cfg, reinterpret_cast<char*>(work), reinterpret_cast<char*>(tape));
+  return adjoint_record<S>(cfg, dt, u_in, u_out, s, as_stream(stream));
+}
+
+template <typename S>
+static int sweep(const snapx_config* cfg, double dt, void* tape, size_t tape_bytes_, S* lambda,
+                 void* work, size_t work_bytes_, void* stream, const char* who) {
+  if (int e = adjoint_check(cfg)) return e;
+  const std::string w(who);
+  SNAPX_CHECK_ARG(lambda && tape && work, w + ": null pointer");
+  SNAPX_CHECK_ARG(tape_bytes_ >= tape_bytes<S>(cfg), w + ": tape too small");
+  SNAPX_CHECK_ARG(work_bytes_ >= work_bytes<S>(cfg), w + ": work space too small");
+  AdjSpace<S> s =
+      adj_layout<S>(cfg, reinterpret_cast<char*>(work), reinterpret_cast<char*>(tape));
+  return adjoint_sweep<S>(cfg, dt, lambda, s, as_stream(stream));
+}
+
 }  // namespace snapx
+
+using namespace snapx;
+
+extern "C" size_t snapx_adjoint_tape_bytes(const snapx_config* cfg) {
+  return tape_bytes<double>(cfg);
+}
+
+extern "C" size_t snapx_adjoint_work_bytes(const snapx_config* cfg) {
+  return work_bytes<double>(cfg);
+}
+
+extern "C" size_t snapx_adjoint_workspace_bytes(const snapx_config* cfg) {
+  return workspace_bytes<double>(cfg);
+}
 
 extern "C" int snapx_adjoint_rk3_step(const snapx_config* cfg, double dt, const double* u_in,
                                       double* lambda, void* workspace, size_t workspace_bytes,
                                       void* stream) {
-  if (int e = adjoint_check(cfg)) return e;
-  SNAPX_CHECK_ARG(u_in && lambda && workspace, "snapx_adjoint_rk3_step: null pointer");
-  SNAPX_CHECK_ARG(workspace_bytes >= snapx_adjoint_workspace_bytes(cfg),
-                  "snapx_adjoint_rk3_step: workspace too small");
-  char* base = reinterpret_cast<char*>(workspace);
-  AdjSpace s = adj_layout(cfg, base, base + snapx_adjoint_work_bytes(cfg));
-  hipStream_t st = as_stream(stream);
-  if (int e = adjoint_record(cfg, dt, u_in, nullptr, s, st)) return e;
-  return adjoint_sweep(cfg, dt, lambda, s, st);
+  return rk3_step<double>(cfg, dt, u_in, lambda, workspace, workspace_bytes, stream,
+                          "snapx_adjoint_rk3_step");
 }
 
 extern "C" int snapx_adjoint_record(const snapx_config* cfg, double dt, const double* u_in,
                                     double* u_out, void* tape, size_t tape_bytes, void* work,
                                     size_t work_bytes, void* stream) {
-  if (int e = adjoint_check(cfg)) return e;
-  SNAPX_CHECK_ARG(u_in && tape && work, "snapx_adjoint_record: null pointer");
-  SNAPX_CHECK_ARG(u_out != u_in, "snapx_adjoint_record: u_out must not alias u_in");
-  SNAPX_CHECK_ARG(tape_bytes >= snapx_adjoint_tape_bytes(cfg),
-                  "snapx_adjoint_record: tape too small");
-  SNAPX_CHECK_ARG(work_bytes >= snapx_adjoint_work_bytes(cfg),
-                  "snapx_adjoint_record: work space too small");
-  AdjSpace s = adj_layout(cfg, reinterpret_cast<char*>(work), reinterpret_cast<char*>(tape));
-  return adjoint_record(cfg, dt, u_in, u_out, s, as_stream(stream));
+  return record<double>(cfg, dt, u_in, u_out, tape, tape_bytes, work, work_bytes, stream,
+                        "snapx_adjoint_record");
 }
 
 extern "C" int snapx_adjoint_sweep(const snapx_config* cfg, double dt, void* tape,
                                    size_t tape_bytes, double* lambda, void* work,
                                    size_t work_bytes, void* stream) {
-  if (int e = adjoint_check(cfg)) return e;
-  SNAPX_CHECK_ARG(lambda && tape && work, "snapx_adjoint_sweep: null pointer");
-  SNAPX_CHECK_ARG(tape_bytes >= snapx_adjoint_tape_bytes(cfg),
-                  "snapx_adjoint_sweep: tape too small");
-  SNAPX_CHECK_ARG(work_bytes >= snapx_adjoint_work_bytes(cfg),
-                  "snapx_adjoint_sweep: work space too small");
-  AdjSpace s = adj_layout(cfg, reinterpret_cast<char*>(work), reinterpret_cast<char*>(tape));
-  return adjoint_sweep(cfg, dt, lambda, s, as_stream(stream));
+  return sweep<double>(cfg, dt, tape, tape_bytes, lambda, work, work_bytes, stream,
+                       "snapx_adjoint_sweep");
+}
+
+extern "C" size_t snapx_adjoint_f32_tape_bytes(const snapx_config* cfg) {
+  return tape_bytes<float>(cfg);
+}
+
+extern "C" size_t snapx_adjoint_f32_work_bytes(const snapx_config* cfg) {
+  return work_bytes<float>(cfg);
+}
+
+extern "C" size_t snapx_adjoint_f32_workspace_bytes(const snapx_config* cfg) {
+  return workspace_bytes<float>(cfg);
+}
+
+extern "C" int snapx_adjoint_f32_rk3_step(const snapx_config* cfg, double dt, const float* u_in,
+                                          float* lambda, void* workspace,
+                                          size_t workspace_bytes, void* stream) {
+  return rk3_step<float>(cfg, dt, u_in, lambda, workspace, workspace_bytes, stream,
+                         "snapx_adjoint_f32_rk3_step");
+}
+
+extern "C" int snapx_adjoint_f32_record(const snapx_config* cfg, double dt, const float* u_in,
+                                        float* u_out, void* tape, size_t tape_bytes, void* work,
+                                        size_t work_bytes, void* stream) {
+  return record<float>(cfg, dt, u_in, u_out, tape, tape_bytes, work, work_bytes, stream,
+                       "snapx_adjoint_f32_record");
+}
+
+extern "C" int snapx_adjoint_f32_sweep(const snapx_config* cfg, double dt, void* tape,
+                                       size_t tape_bytes, float* lambda, void* work,
+                                       size_t work_bytes, void* stream) {
+  return sweep<float>(cfg, dt, tape, tape_bytes, lambda, work, work_bytes, stream,
+                      "snapx_adjoint_f32_sweep");
 }
