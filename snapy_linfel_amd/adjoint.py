@@ -27,6 +27,10 @@ FP32 tapes and adjoint fields, FP64 arithmetic in registers and FP64 column
 solves.  Their forward steps are plain FP32 steps (``RK3Step``) or, in the
 segments of ``integrate`` whose tapes are recorded later, fused FP32 steps
 (``snapx_fused_rk3_steps``) when the configuration runs on the fused path.
+
+``integrate_batch`` differentiates steps of a batch of members
+(``capi.Batch``): fused FP32 batch steps forward, the FP32-storage adjoint of
+each member backward.
 """
 from __future__ import annotations
 
@@ -141,6 +145,69 @@ class TapedSegment(torch.autograd.Function):
         for i in reversed(range(n)):
             capi.adjoint_sweep(cfg, dt, tapes[i], lam, st.work)
         return lam, None, None, None, None, None
+
+
+class _MemberStores:
+    """One tape store per batch member (created on first use)."""
+
+    def __init__(self, batch, device):
+        self.batch, self.device, self.stores = batch, device, {}
+
+    def get(self, m):
+        if m not in self.stores:
+            self.stores[m] = _Tapes(self.batch.cfgs[m], self.device, torch.float32)
+        return self.stores[m]
+
+
+class BatchSegment(torch.autograd.Function):
+    """n steps of a batch (``capi.Batch``: fused FP32 steps of every member in
+    one launch per kernel and stage) as one autograd node.  The members are
+    independent, so the batch's Jacobian is block-diagonal: the backward
+    replays each member's n steps through the FP32-storage adjoint's record
+    half and sweeps that member's tapes in reverse on its slice of lambda."""
+
+    @staticmethod
+    def forward(ctx, u, batch, dt, n, stores):
+        batch._check(u)
+        u_in = u.detach().contiguous()
+        out = u_in.clone()
+        batch.steps(out, n)
+        ctx.save_for_backward(u_in)
+        ctx.batch, ctx.dt, ctx.n, ctx.stores = batch, dt, n, stores
+        return out
+
+    @staticmethod
+    def backward(ctx, grad):
+        (u_in,) = ctx.saved_tensors
+        b, dt, n = ctx.batch, ctx.dt, ctx.n
+        lam = grad.detach().contiguous().clone()
+        for m in range(b.n):
+            cfg, st = b.cfgs[m], ctx.stores.get(m)
+            tapes = _record(cfg, dt, b.member(u_in, m), n, st)
+            lm = b.member(lam, m)
+            for i in reversed(range(n)):
+                capi.adjoint_sweep(cfg, dt, tapes[i], lm, st.work)
+        return lam, None, None, None, None
+
+
+def integrate_batch(u, batch, dt, nsteps, checkpoint_every=None):
+    """``nsteps`` differentiable steps of a batch: ``u`` is the float32 array
+    of the members' states in member order (``batch.total`` elements), ``dt``
+    the batch's planned step.  Every member must be a block the adjoint
+    takes (Cartesian, limiter off, ...).  With ``checkpoint_every = K`` only
+    every K-th batch state is kept; each member keeps K tapes of its own."""
+    for m in range(batch.n):
+        if capi.adjoint_tape_bytes(batch.cfgs[m], torch.float32) == 0:
+            raise capi.SnapxError(f"integrate_batch: member {m}: " +
+                                  capi.lib().snapx_last_error().decode())
+    every = checkpoint_every if checkpoint_every else nsteps
+    stores = _MemberStores(batch, u.device)
+    done = 0
+    while done < nsteps:
+        n = min(every, nsteps - done)
+        u = BatchSegment.apply(u, batch, dt, n, stores)
+        done += n
+    return u
 
 
 def rk3_step(u, cfg, dt, ws=None):
