@@ -29,10 +29,12 @@
  *    (r, theta, phi) on the plain FP64/FP32 path: faces uniform in each
  *    coordinate as torch::linspace places them (src/coord/coordinate.cpp:
  *    201-221), with the centres, face areas, volumes and source terms of
- *    src/coord/spherical_polar.cpp.  The fused path takes spherical-polar
- *    blocks with columns of at most 256 (3x3 blocks) / 128 (5x5) cells;
- *    so do batches whose members all share one coordinate system.  The
- *    adjoint is Cartesian only.
+ *    src/coord/spherical_polar.cpp, or one gnomonic-equiangle panel
+ *    (src/coord/gnomonic_equiangle.cpp) with external boundaries.  The fused
+ *    path takes curvilinear blocks with columns of at most 256 (3x3 blocks)
+ *    / 128 (5x5) cells; so do batches whose members all share one coordinate
+ *    system.  The adjoint takes Cartesian and spherical-polar blocks
+ *    (spherical columns of at most 256 / 128 cells), not panels.
  */
 #ifndef SNAPX_H
 #define SNAPX_H
@@ -354,7 +356,10 @@ int snapx_fused_rk3_steps_static(const snapx_config* cfg, double dt, int nsteps,
    (snapx_rk3_step): on entry lambda (double [5][nc3][nc2][nc1]) holds
    dL/du_out; on exit it holds dL/du_in for the step input u_in.  The
    forward stages are recomputed from u_in.  Requires limiter off,
-   non-hydrostatic = 1 and no disable-flux. */
+   non-hydrostatic = 1 and no disable-flux; Cartesian or spherical-polar
+   geometry (on spherical-polar blocks the adjoint includes the per-cell
+   areas and volumes, the geometric sources with the x1 face pressure, the
+   per-layer column geometry and the ghost distances). */
 size_t snapx_adjoint_workspace_bytes(const snapx_config* cfg);
 int snapx_adjoint_rk3_step(const snapx_config* cfg, double dt,
                            const double* u_in, double* lambda,

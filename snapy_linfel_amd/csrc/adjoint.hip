@@ -22,6 +22,7 @@
 #include <cstring>
 
 #include "common.hpp"
+#include "geom_impl.hpp"
 #include "interp_impl.hpp"
 #include "riemann_impl.hpp"
 #include "vic_impl.hpp"
@@ -45,7 +46,31 @@ struct AdjArgs {
   int bc[6];
   int scheme;
   int lanes, lshift;  // cell-parallel column kernels: lanes per column, log2
+  // spherical-polar blocks: per-cell areas, volumes and source factors
+  // (sph_cell), per-layer ghost distances (sph_dx1v)
+  int sph;
+  Geometry geo;
 };
+
+// (j, k) full indices of column col
+__device__ __forceinline__ void col_jk(const AdjArgs& a, int col, int& j, int& k) {
+  const int ncol2 = a.n2 > 1 ? a.nx2 : 1;
+  j = a.n2 > 1 ? a.ng + col % ncol2 : 0;
+  k = a.n3 > 1 ? a.ng + col / ncol2 : 0;
+}
+
+// spherical-polar layer i (interior-relative, ghosts included) of column
+// col: x1 face areas below / above and volume (layer_geom of the column
+// solve)
+struct SphLayer {
+  double am, ap, vol;
+};
+__device__ __forceinline__ SphLayer sph_layer(const AdjArgs& a, int col, int i) {
+  int j, k;
+  col_jk(a, col, j, k);
+  const SphCell c = sph_cell(a.geo, k, j, a.ng + i);
+  return {c.a1m, c.a1p, c.vol};
+}
 
 __device__ __forceinline__ void decode(const AdjArgs a, int q, int& i, int& j, int& k) {
   i = q % a.n1;
@@ -330,7 +355,7 @@ __global__ void __launch_bounds__(64)
 // ---------------------------------------- cell-parallel VIC (forward+adjoint)
 
 // block rows a_i, b_i, c_i of every interior cell, abc [col][layer][3][NN]
-template <int N, typename S>
+template <int N, typename S, bool kSph>
 __global__ void __launch_bounds__(kT)
     vic_asm_kernel(const S* __restrict__ w, double* __restrict__ abc, AdjArgs a, int ncol) {
   const int NN = N * N;
@@ -347,6 +372,12 @@ __global__ void __launch_bounds__(kT)
     load_w(w, ncell, q, w0);
     load_w(w, ncell, q + 1, wp);
     double* row = abc + t * 3 * NN;
+    if constexpr (kSph) {
+      const SphLayer lg = sph_layer(a, col, i);
+      vic_assemble_cell<double, N>(wm, w0, wp, a.gammad, lg.am, lg.ap, lg.vol, a.dt, a.grav1,
+                                   i == 0, i == nl - 1, row, row + NN, row + 2 * NN);
+      continue;
+    }
     vic_assemble_cell<double, N>(wm, w0, wp, a.gammad, area, area, vol, a.dt, a.grav1, i == 0,
                                  i == nl - 1, row, row + NN, row + 2 * NN);
   }
@@ -415,7 +446,7 @@ __device__ __forceinline__ void vic_row_blocks(const double* __restrict__ abc, i
 // Forward column solve + MS-VIC redistribution of one stage (the recompute
 // of the stage states): du (interior) from the explicit tendency dux; delta
 // is kept in the first half of dl [col][layer][2N] for the reverse sweep.
-template <int N, typename S>
+template <int N, typename S, bool kSph>
 __global__ void __launch_bounds__(PcrShape<N>::kThreads)
     vic_fwd_solve_kernel(const S* __restrict__ w, const S* __restrict__ dux,
                          const double* __restrict__ abc, S* __restrict__ du,
@@ -423,8 +454,8 @@ __global__ void __launch_bounds__(PcrShape<N>::kThreads)
   extern __shared__ double sh[];
   const int NN = N * N;
   const int ncell = a.n1 * a.n2 * a.n3, nl = a.nx1;
-  const double vol = a.dx3 * (a.dx2 * a.dx1);
   ColLane L = col_lane(a, ncol);
+  const double vol = !kSph ? a.dx3 * (a.dx2 * a.dx1) : (L.act ? sph_layer(a, L.col, L.i).vol : 1.);
   double A[NN], B[NN], C[NN], r[N], d[N];
   if (L.act) {
     vic_row_blocks<N>(abc, L.col, L.i, nl, false, A, B, C);
@@ -458,7 +489,7 @@ __global__ void __launch_bounds__(PcrShape<N>::kThreads)
 // (column sums S, Bd, G) and y = M^-T lambda_delta by PCR.  Writes y to the
 // second half of dl, lam_dx (interior) and the rho part of lam_w; lam_dx
 // enters holding lam_du.
-template <int N, typename S>
+template <int N, typename S, bool kSph>
 __global__ void __launch_bounds__(PcrShape<N>::kThreads)
     vic_adj_solve_kernel(const S* __restrict__ w, const S* __restrict__ dux,
                          const S* __restrict__ lam_du, const double* __restrict__ abc,
@@ -467,8 +498,8 @@ __global__ void __launch_bounds__(PcrShape<N>::kThreads)
   extern __shared__ double sh[];
   const int NN = N * N;
   const int ncell = a.n1 * a.n2 * a.n3, nl = a.nx1;
-  const double vol = a.dx3 * (a.dx2 * a.dx1);
   ColLane L = col_lane(a, ncol);
+  const double vol = !kSph ? a.dx3 * (a.dx2 * a.dx1) : (L.act ? sph_layer(a, L.col, L.i).vol : 1.);
   double* cdl = dl + ((int64_t)L.col * nl + L.i) * 2 * N;
   double d0 = L.act ? cdl[0] : 0.;
   double rho = L.act ? w[IDN * ncell + L.q] : 0.;
@@ -565,7 +596,7 @@ __device__ __forceinline__ T abs_contract(T gm1, const T* z, const double* P, co
 // contraction: phi_f = (y_{f-1} - y_f)^T |A|_f (delta_{f-1} - delta_f);
 // G[col][f][0..4] = d phi / d w_{f-1}, G[col][f][5..9] = d phi / d w_f.
 // The Roe state is differentiated with Dual<10>, phi(z) with Dual<5>.
-template <int N, typename S>
+template <int N, typename S, bool kSph>
 __global__ void __launch_bounds__(kT)
     vic_adj_face_kernel(const S* __restrict__ w, const double* __restrict__ dl,
                         double* __restrict__ G, AdjArgs a, int ncol) {
@@ -581,10 +612,22 @@ __global__ void __launch_bounds__(kT)
     ext_delta_y<N>(cdl, f - 1, nl, dm, ym);
     ext_delta_y<N>(cdl, f, nl, dp, yp);
     double P[5] = {0., 0., 0., 0., 0.}, Q[5] = {0., 0., 0., 0., 0.};
+    if constexpr (kSph) {
+      // rows f - 1 and f carry |A|_f with the face's area and their own
+      // 0.5 / V: P = a_f (y_{f-1} / 2V_{f-1} - y_f / 2V_f)
+      const SphLayer lo = sph_layer(a, col, f - 1), hi = sph_layer(a, col, f);
+      const double cm = hi.am * (0.5 / lo.vol), cp = hi.am * (0.5 / hi.vol);
 #pragma unroll
-    for (int e = 0; e < N; ++e) {
-      P[vic_row(N, e)] = ym[e] - yp[e];
-      Q[vic_row(N, e)] = dm[e] - dp[e];
+      for (int e = 0; e < N; ++e) {
+        P[vic_row(N, e)] = cm * ym[e] - cp * yp[e];
+        Q[vic_row(N, e)] = dm[e] - dp[e];
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < N; ++e) {
+        P[vic_row(N, e)] = ym[e] - yp[e];
+        Q[vic_row(N, e)] = dm[e] - dp[e];
+      }
     }
     int q = col_q0(a, col) + f;  // the cell above the face
     Dual<10> wl[5], wr[5], z[5];
@@ -613,7 +656,7 @@ __global__ void __launch_bounds__(kT)
 //   phi_j + phi_{j+1} + (y_{j-1} - y_{j+1})^T J(w_j) delta_j
 // (the faces below / above and the flux Jacobian of the cell, which enters
 // row j+1 as its lower and row j-1 as its upper coupling).
-template <int N, typename S>
+template <int N, typename S, bool kSph>
 __global__ void __launch_bounds__(kT)
     vic_adj_cell_kernel(const S* __restrict__ w, const double* __restrict__ dl,
                         const double* __restrict__ G, S* __restrict__ lam_w, AdjArgs a,
@@ -645,6 +688,24 @@ __global__ void __launch_bounds__(kT)
     for (int v = 0; v < 5; ++v) wj[v] = Dual<5>::seed(w[v * ncell + q], v);
     flux_jacobian<Dual<5>, N>(Dual<5>(a.gammad - 1.), wj, J);
     Dual<5> psi = Dual<5>(0.);
+    if constexpr (kSph) {
+      // J(w_j) enters row j + 1 (lower coupling, area a_{j+1}), row j - 1
+      // (upper coupling, area a_j) and, with a_{j+1} - a_j, row j itself,
+      // each with its own 0.5 / V; the face terms of G carry their factors
+      const SphLayer lo = sph_layer(a, col, j - 1), me = sph_layer(a, col, j),
+                     hi = sph_layer(a, col, j + 1);
+      const double clo = me.am * (0.5 / lo.vol), chi = me.ap * (0.5 / hi.vol),
+                   cme = (me.ap - me.am) * (0.5 / me.vol);
+#pragma unroll
+      for (int r = 0; r < N; ++r) {
+        const double coef = clo * ylo[r] - chi * yhi[r] + cme * yj[r];
+#pragma unroll
+        for (int c = 0; c < N; ++c) psi += J[r * N + c] * Dual<5>(coef * dj[c]);
+      }
+#pragma unroll
+      for (int v = 0; v < 5; ++v) lam_w[v * ncell + q] -= acc[v] + psi.d[v];
+      continue;
+    }
 #pragma unroll
     for (int r = 0; r < N; ++r)
 #pragma unroll
@@ -710,20 +771,68 @@ __device__ __forceinline__ int axis_stride(const AdjArgs a, int d) {
 // Faces of axis d whose flux reaches an interior cell (a face f in
 // [ng, n - ng], never a dummy face), and lambda_F there in the face frame:
 // lambda_F = (-dt A / V)(lambda_dx[left cell] - lambda_dx[right cell]).
-template <typename S>
+// Spherical-polar blocks (SphericalPolarImpl::forward,
+// src/coord/spherical_polar.cpp:212-279): each cell's own area over its own
+// volume, plus the sources that read the fluxes (x1 faces: src2_i A1 F of the
+// transverse momenta; x2 faces: src1_i src2_j A2 F of v3) and, for x1 faces,
+// lambda of the face pressure in lfp.
+template <typename S, bool kSph>
 __device__ __forceinline__ bool face_lambda(const AdjArgs a, int d, const S* lam_dx,
                                             int ncell, int q, int s, const int* vmap,
-                                            int& f, double* lf) {
+                                            int& f, double* lf, double* lfp = nullptr) {
   const DivMod dn1(a.n1), dn2(a.n2);
   int i, j;
   int k = dn2.div(dn1.div(q, i), j);
   const int n = axis_len(a, d);
   f = d == 0 ? i : (d == 1 ? j : k);
-  double area = d == 0 ? a.dx3 * a.dx2 : (d == 1 ? a.dx3 * a.dx1 : a.dx2 * a.dx1);
-  double coef = -a.dt * area / (a.dx3 * (a.dx2 * a.dx1));
 #pragma unroll
   for (int v = 0; v < 5; ++v) lf[v] = 0.;
+  if (lfp) *lfp = 0.;
   bool any = false;
+  if constexpr (kSph) {
+    // F enters the left cell (its upper face) with +Ap / V and the right
+    // cell (its lower face) with -Am / V; lambda_d = -dt lambda_dx
+    if (f >= 1) {
+      int il = i - (d == 0), jl = j - (d == 1), kl = k - (d == 2);
+      if (interior(a, il, jl, kl)) {
+        const SphCell g = sph_cell(a.geo, kl, jl, il);
+        const double ap = d == 0 ? g.a1p : (d == 1 ? g.a2p : g.a3);
+        const double c = -a.dt * ap / g.vol;
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+          const int var = vmap[v];
+          const double l = lam_dx[var * ncell + q - s];
+          double x = c * l;
+          if (d == 0 && (var == IVY || var == IVZ)) x += -a.dt * g.src2_i * g.a1p * l;
+          if (d == 1 && var == IVZ) x += -a.dt * g.src1_i * g.src2_j * g.a2p * l;
+          lf[v] += x;
+        }
+        if (d == 0 && lfp)
+          *lfp += -a.dt * (double)lam_dx[IVX * ncell + q - s] * (1. / g.dx1 - g.a1p / g.vol);
+        any = true;
+      }
+    }
+    if (interior(a, i, j, k)) {
+      const SphCell g = sph_cell(a.geo, k, j, i);
+      const double am = d == 0 ? g.a1m : (d == 1 ? g.a2m : g.a3);
+      const double c = -a.dt * am / g.vol;
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        const int var = vmap[v];
+        const double l = lam_dx[var * ncell + q];
+        double x = -c * l;
+        if (d == 0 && (var == IVY || var == IVZ)) x += -a.dt * g.src2_i * g.a1m * l;
+        if (d == 1 && var == IVZ) x += -a.dt * g.src1_i * g.src2_j * g.a2m * l;
+        lf[v] += x;
+      }
+      if (d == 0 && lfp)
+        *lfp += -a.dt * (double)lam_dx[IVX * ncell + q] * (g.a1m / g.vol - 1. / g.dx1);
+      any = true;
+    }
+    return any && f >= a.ng && f <= n - a.ng;
+  }
+  double area = d == 0 ? a.dx3 * a.dx2 : (d == 1 ? a.dx3 * a.dx1 : a.dx2 * a.dx1);
+  double coef = -a.dt * area / (a.dx3 * (a.dx2 * a.dx1));
   if (f >= 1) {
     int il = i - (d == 0), jl = j - (d == 1), kl = k - (d == 2);
     if (interior(a, il, jl, kl)) {
@@ -742,7 +851,7 @@ __device__ __forceinline__ bool face_lambda(const AdjArgs a, int d, const S* lam
 
 // Interface states (face frame, x1 LR revision applied) of the faces that
 // carry an adjoint, into buf [2][5][ncell] (slot order; left, then right).
-template <typename S>
+template <typename S, bool kSph>
 __global__ void __launch_bounds__(kT)
     face_state_kernel(const S* __restrict__ w, const S* __restrict__ lam_dx,
                       S* __restrict__ buf, AdjArgs a, int d) {
@@ -753,7 +862,7 @@ __global__ void __launch_bounds__(kT)
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < ncell; q += gridDim.x * blockDim.x) {
     int f;
     double lf[5];
-    if (!face_lambda(a, d, lam_dx, ncell, q, s, vmap, f, lf)) continue;
+    if (!face_lambda<S, kSph>(a, d, lam_dx, ncell, q, s, vmap, f, lf)) continue;
     double wl[5], wr[5];
     face_states<double, S>(w, ncell, q, s, f, a, d, vmap, wl, wr);
     if (d == 0 && a.grav1 != 0.) {
@@ -772,7 +881,7 @@ __global__ void __launch_bounds__(kT)
 // every face of axis d: lam_wlr [2][5][ncell] (ILT, IRT; variable order)
 // replaces the face states face_state_kernel left there (same index).
 // One instantiation per solver: each holds only its own solver's registers.
-template <int SOLVER, typename S>
+template <int SOLVER, typename S, bool kSph>
 __global__ void __launch_bounds__(kT)
     flux_adj_kernel(const S* __restrict__ lam_dx, S* __restrict__ lam_wlr, AdjArgs a, int d) {
   const int ncell = a.n1 * a.n2 * a.n3;
@@ -782,8 +891,9 @@ __global__ void __launch_bounds__(kT)
   face_frame(d, vmap);
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < ncell; q += gridDim.x * blockDim.x) {
     int f;
-    double lf[5];
-    bool act = face_lambda(a, d, lam_dx, ncell, q, s, vmap, f, lf);
+    double lf[5], lfp = 0.;
+    bool act = face_lambda<S, kSph>(a, d, lam_dx, ncell, q, s, vmap, f, lf,
+                                    (kSph && d == 0) ? &lfp : nullptr);
     double lwl[5] = {0., 0., 0., 0., 0.}, lwr[5] = {0., 0., 0., 0., 0.};
     if (act) {
       double wl[5], wr[5];
@@ -823,6 +933,28 @@ __global__ void __launch_bounds__(kT)
           for (int v = 0; v < 5; ++v) {
             if (p0 + j == v) lwl[v] += acc;
             if (p0 + j == 5 + v) lwr[v] += acc;
+          }
+        }
+      }
+      // spherical-polar x1 faces: the adjoint of the face pressure the solver
+      // returns for the radial source, by Dual<1> per input direction through
+      // the same solver code
+      if (kSph && d == 0 && lfp != 0.) {
+#pragma unroll 1
+        for (int p0 = 0; p0 < 10; ++p0) {
+          Dual<1> sl[5], sr[5], flx[5], pf;
+#pragma unroll
+          for (int v = 0; v < 5; ++v) {
+            sl[v] = Dual<1>(wl[v]);
+            sr[v] = Dual<1>(wr[v]);
+            sl[v].d[0] = p0 == v ? 1. : 0.;
+            sr[v].d[0] = p0 == 5 + v ? 1. : 0.;
+          }
+          riemann_flux<Dual<1>>(solver, sl, sr, Dual<1>(a.gammad), 3, flx, &pf);
+#pragma unroll
+          for (int v = 0; v < 5; ++v) {
+            if (p0 == v) lwl[v] += lfp * pf.d[0];
+            if (p0 == 5 + v) lwr[v] += lfp * pf.d[0];
           }
         }
       }
@@ -1103,10 +1235,40 @@ __global__ void grav_adj_kernel(const S* __restrict__ w, const S* __restrict__ l
   }
 }
 
+// the cell sources of spherical-polar blocks that read the cell's own
+// primitives (spherical_polar.cpp:212-279: -src1_i rho (v2^2 + v3^2) on v1,
+// -src1_i src1_j (rho v3^2 + p) on v2 and, without an x2 axis,
+// src1_i src1_j rho v3 v2 on v3), by Dual<5>; lambda_d = -dt lambda_dx of
+// the interior cells
+template <typename S>
+__global__ void sph_src_adj_kernel(const S* __restrict__ w, const S* __restrict__ lam_dx,
+                                   S* __restrict__ lam_w, AdjArgs a) {
+  const int ncell = a.n1 * a.n2 * a.n3;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < ncell; q += gridDim.x * blockDim.x) {
+    int i, j, k;
+    decode(a, q, i, j, k);
+    if (!interior(a, i, j, k)) continue;
+    const SphCell g = sph_cell(a.geo, k, j, i);
+    Dual<5> W[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) W[v] = Dual<5>::seed((double)w[v * ncell + q], v);
+    const Dual<5> s1i(g.src1_i), s1ij(g.src1_i * g.src1_j);
+    const Dual<5> sx = Dual<5>(0.) - s1i * W[IDN] * (W[IVY] * W[IVY] + W[IVZ] * W[IVZ]);
+    const Dual<5> sy = Dual<5>(0.) - s1ij * (W[IDN] * (W[IVZ] * W[IVZ]) + W[IPR]);
+    const Dual<5> sz = a.n2 > 1 ? Dual<5>(0.) : s1ij * (W[IDN] * W[IVZ] * W[IVY]);
+    const double lx = -a.dt * (double)lam_dx[IVX * ncell + q];
+    const double ly = -a.dt * (double)lam_dx[IVY * ncell + q];
+    const double lz = -a.dt * (double)lam_dx[IVZ * ncell + q];
+#pragma unroll
+    for (int m = 0; m < 5; ++m)
+      lam_w[m * ncell + q] += lx * sx.d[m] + ly * sy.d[m] + lz * sz.d[m];
+  }
+}
+
 // isentropic ghost refill (src/hydro/hydro.cpp:231-269) adjoint, one thread
 // per column; wpre holds the primitives before the refill (only rho, p of
 // the boundary cells are read)
-template <typename S>
+template <typename S, bool kSph>
 __global__ void ghost_adj_kernel(const S* __restrict__ w, S* __restrict__ lam_w, AdjArgs a) {
   const int ncell = a.n1 * a.n2 * a.n3;
   int ncol = a.n2 * a.n3;
@@ -1119,11 +1281,13 @@ __global__ void ghost_adj_kernel(const S* __restrict__ w, S* __restrict__ lam_w,
       Dual<2> pre_b = Dual<2>::seed(w[IPR * ncell + base + is], 1);
       Dual<2> gamma = Dual<2>(a.gammad), gm = gamma - Dual<2>(1.), ae = gm / gamma;
       Dual<2> inv_g = Dual<2>(1.) / gamma, inv_a = Dual<2>(1.) / ae;
-      Dual<2> grav = Dual<2>(-a.grav1), dz = Dual<2>(a.dx1);
+      Dual<2> grav = Dual<2>(-a.grav1);
       Dual<2> K = pre_b / pow(rho_b, gamma);
       Dual<2> prev = pre_b;
       double lr = 0., lp = 0.;
       for (int n = 0; n < a.ng; ++n) {
+        // the centre distance of the ghost layer (dx1v of the coordinate)
+        const Dual<2> dz = Dual<2>(!kSph ? a.dx1 : sph_dx1v(a.geo, side ? is + n : is - n - 1));
         Dual<2> h = side ? pow(prev, ae) - ae * grav * dz / pow(K, inv_g)
                          : pow(prev, ae) + ae * grav * dz / pow(K, inv_g);
         Dual<2> pg = pow(h, inv_a);
@@ -1196,6 +1360,8 @@ static AdjArgs adj_args(const snapx_config* c, double dt) {
   a.scheme = c->implicit_scheme;
   a.lanes = 64, a.lshift = 6;
   while (a.lanes < c->nx1) a.lanes <<= 1, a.lshift += 1;
+  a.sph = c->coord == SNAPX_COORD_SPHERICAL_POLAR;
+  a.geo = make_geometry(c);
   return a;
 }
 
@@ -1310,29 +1476,29 @@ static bool adj_pcr(const AdjArgs a, int N) {
   return a.lanes <= (N == 3 ? PcrShape<3>::kThreads : PcrShape<5>::kThreads);
 }
 
-template <int N, typename S>
+template <int N, typename S, bool kSph>
 static int launch_vic_fwd(const AdjArgs& a, const AdjSpace<S>& s, int k, S* du, int ncol,
                           hipStream_t st) {
   const int T = PcrShape<N>::kThreads;
-  hipLaunchKernelGGL((vic_asm_kernel<N, S>), blocks_for((int64_t)ncol * a.nx1, kT), dim3(kT), 0,
+  hipLaunchKernelGGL((vic_asm_kernel<N, S, kSph>), blocks_for((int64_t)ncol * a.nx1, kT), dim3(kT), 0,
                      st, s.w[k], s.abc[k], a, ncol);
   SNAPX_LAUNCH_CHECK();
   int cpw = T >> a.lshift;
-  hipLaunchKernelGGL((vic_fwd_solve_kernel<N, S>), dim3((ncol + cpw - 1) / cpw), dim3(T),
+  hipLaunchKernelGGL((vic_fwd_solve_kernel<N, S, kSph>), dim3((ncol + cpw - 1) / cpw), dim3(T),
                      sizeof(double) * T * PcrShape<N>::kPub, st, s.w[k], s.dux[k], s.abc[k], du,
                      s.dl[k], a, ncol);
   SNAPX_LAUNCH_CHECK();
   return 0;
 }
 
-template <int N, typename S>
+template <int N, typename S, bool kSph>
 static int launch_vic_adj(const AdjArgs& a, const AdjSpace<S>& s, int k, int ncol,
                           hipStream_t st) {
   const int T = PcrShape<N>::kThreads;
   if (adj_pcr(a, N)) {
     // blocks and delta of this stage come from the forward recompute
     int cpw = T >> a.lshift;
-    hipLaunchKernelGGL((vic_adj_solve_kernel<N, S>), dim3((ncol + cpw - 1) / cpw), dim3(T),
+    hipLaunchKernelGGL((vic_adj_solve_kernel<N, S, kSph>), dim3((ncol + cpw - 1) / cpw), dim3(T),
                        sizeof(double) * T * PcrShape<N>::kPub, st, s.w[k], s.dux[k], s.lam_du,
                        s.abc[k], s.lam_dx, s.lam_w, s.dl[k], a, ncol);
   } else {
@@ -1341,10 +1507,10 @@ static int launch_vic_adj(const AdjArgs& a, const AdjSpace<S>& s, int k, int nco
                        ncol);
   }
   SNAPX_LAUNCH_CHECK();
-  hipLaunchKernelGGL((vic_adj_face_kernel<N, S>), blocks_for((int64_t)ncol * (a.nx1 + 1), kT),
+  hipLaunchKernelGGL((vic_adj_face_kernel<N, S, kSph>), blocks_for((int64_t)ncol * (a.nx1 + 1), kT),
                      dim3(kT), 0, st, s.w[k], s.dl[k], s.G, a, ncol);
   SNAPX_LAUNCH_CHECK();
-  hipLaunchKernelGGL((vic_adj_cell_kernel<N, S>), blocks_for((int64_t)ncol * (a.nx1 + 2), kT),
+  hipLaunchKernelGGL((vic_adj_cell_kernel<N, S, kSph>), blocks_for((int64_t)ncol * (a.nx1 + 2), kT),
                      dim3(kT), 0, st, s.w[k], s.dl[k], s.G, s.lam_w, a, ncol);
   SNAPX_LAUNCH_CHECK();
   return 0;
@@ -1354,7 +1520,13 @@ static int adjoint_check(const snapx_config* cfg) {
   if (int e = validate_config(cfg)) return e;
   SNAPX_CHECK_ARG(!cfg->limiter, "adjoint: the EOS limiter is not differentiable (turn it off)");
   SNAPX_CHECK_ARG(cfg->non_hydrostatic == 1.0, "adjoint: non-hydrostatic < 1 is not supported");
-  SNAPX_CHECK_ARG(cfg->coord == SNAPX_COORD_CARTESIAN, "adjoint: Cartesian geometry only");
+  SNAPX_CHECK_ARG(cfg->coord == SNAPX_COORD_CARTESIAN || cfg->coord == SNAPX_COORD_SPHERICAL_POLAR,
+                  "adjoint: Cartesian and spherical-polar geometry only");
+  if (cfg->coord == SNAPX_COORD_SPHERICAL_POLAR && cfg->implicit_scheme) {
+    const int N = ((cfg->implicit_scheme >> 3) & 1) ? 5 : 3;
+    SNAPX_CHECK_ARG(adj_pcr(adj_args(cfg, 1.0), N),
+                    "adjoint: spherical-polar columns of at most 256 (3x3) / 128 (5x5) cells");
+  }
   for (int b = 0; b < 6; ++b)
     SNAPX_CHECK_ARG(cfg->bc[b] != SNAPX_BC_CUSTOM, "adjoint: custom boundaries are not supported");
   for (int d = 0; d < 3; ++d)
@@ -1370,7 +1542,7 @@ static const double kStageW[3][3] = {
 // With u_out, also the step's output (third stage combined, boundaries).
 // The explicit part runs on the plain path in the storage type; the column
 // solve is this file's FP64 one for either type.
-template <typename S>
+template <typename S, bool kSph>
 static int adjoint_record(const snapx_config* cfg, double dt, const S* u_in, S* u_out,
                           const AdjSpace<S>& s, hipStream_t st) {
   Grid g = make_grid(cfg);
@@ -1396,8 +1568,8 @@ static int adjoint_record(const snapx_config* cfg, double dt, const S* u_in, S* 
     SNAPX_HIP(hipMemcpyAsync(du, s.dux[k], fb, hipMemcpyDeviceToDevice, st));
     if (cfg->implicit_scheme) {
       if (adj_pcr(a, N)) {
-        if (int e = N == 3 ? launch_vic_fwd<3, S>(a, s, k, du, ncol, st)
-                           : launch_vic_fwd<5, S>(a, s, k, du, ncol, st))
+        if (int e = N == 3 ? launch_vic_fwd<3, S, kSph>(a, s, k, du, ncol, st)
+                           : launch_vic_fwd<5, S, kSph>(a, s, k, du, ncol, st))
           return e;
       } else if (k < 2 || u_out) {
         if (int e = snapx_hydro_forward(cfg, dt, s.us[k], s.lam_w, du, nullptr, nullptr,
@@ -1417,7 +1589,7 @@ static int adjoint_record(const snapx_config* cfg, double dt, const S* u_in, S* 
 
 // Reverse sweep of one step over its tape: lambda (dL/du_out) becomes
 // dL/du_in.
-template <typename S>
+template <typename S, bool kSph>
 static int adjoint_sweep(const snapx_config* cfg, double dt, S* lambda, const AdjSpace<S>& s,
                          hipStream_t st) {
   Grid g = make_grid(cfg);
@@ -1446,26 +1618,30 @@ static int adjoint_sweep(const snapx_config* cfg, double dt, S* lambda, const Ad
     hipLaunchKernelGGL(axpy_kernel<S>, bn, dim3(kT), 0, st, s.lam_dx, s.lam_du, n);
     SNAPX_LAUNCH_CHECK();
     if (cfg->implicit_scheme)
-      if (int e = N == 3 ? launch_vic_adj<3, S>(a, s, stg, ncol, st)
-                         : launch_vic_adj<5, S>(a, s, stg, ncol, st))
+      if (int e = N == 3 ? launch_vic_adj<3, S, kSph>(a, s, stg, ncol, st)
+                         : launch_vic_adj<5, S, kSph>(a, s, stg, ncol, st))
         return e;
     const S* w = s.w[stg];
     hipLaunchKernelGGL(grav_adj_kernel<S>, bc, dim3(kT), 0, st, w, s.lam_dx, s.lam_w, a);
     SNAPX_LAUNCH_CHECK();
+    if (kSph) {
+      hipLaunchKernelGGL(sph_src_adj_kernel<S>, bc, dim3(kT), 0, st, w, s.lam_dx, s.lam_w, a);
+      SNAPX_LAUNCH_CHECK();
+    }
     for (int d = 0; d < 3; ++d) {
       if (ns[d] == 1) continue;
-      hipLaunchKernelGGL(face_state_kernel<S>, bc, dim3(kT), 0, st, w, s.lam_dx, s.lam_wlr, a,
+      hipLaunchKernelGGL((face_state_kernel<S, kSph>), bc, dim3(kT), 0, st, w, s.lam_dx, s.lam_wlr, a,
                          d);
       SNAPX_LAUNCH_CHECK();
       int solver = cfg->riemann[d];
       if (solver == SNAPX_RIEMANN_LMARS)
-        hipLaunchKernelGGL((flux_adj_kernel<SNAPX_RIEMANN_LMARS, S>), bc, dim3(kT), 0, st,
+        hipLaunchKernelGGL((flux_adj_kernel<SNAPX_RIEMANN_LMARS, S, kSph>), bc, dim3(kT), 0, st,
                            s.lam_dx, s.lam_wlr, a, d);
       else if (solver == SNAPX_RIEMANN_HLLC)
-        hipLaunchKernelGGL((flux_adj_kernel<SNAPX_RIEMANN_HLLC, S>), bc, dim3(kT), 0, st,
+        hipLaunchKernelGGL((flux_adj_kernel<SNAPX_RIEMANN_HLLC, S, kSph>), bc, dim3(kT), 0, st,
                            s.lam_dx, s.lam_wlr, a, d);
       else
-        hipLaunchKernelGGL((flux_adj_kernel<SNAPX_RIEMANN_ROE, S>), bc, dim3(kT), 0, st,
+        hipLaunchKernelGGL((flux_adj_kernel<SNAPX_RIEMANN_ROE, S, kSph>), bc, dim3(kT), 0, st,
                            s.lam_dx, s.lam_wlr, a, d);
       SNAPX_LAUNCH_CHECK();
       int r = d == 0 ? 0 : 1;
@@ -1499,7 +1675,7 @@ static int adjoint_sweep(const snapx_config* cfg, double dt, S* lambda, const Ad
       }
     }
     if (g.n1 > 1 && cfg->grav1 != 0.) {
-      hipLaunchKernelGGL(ghost_adj_kernel<S>, blocks_for(g.n2 * g.n3, kT), dim3(kT), 0, st, w,
+      hipLaunchKernelGGL((ghost_adj_kernel<S, kSph>), blocks_for(g.n2 * g.n3, kT), dim3(kT), 0, st, w,
                          s.lam_w, a);
       SNAPX_LAUNCH_CHECK();
     }
@@ -1517,15 +1693,13 @@ static int adjoint_sweep(const snapx_config* cfg, double dt, S* lambda, const Ad
 
 template <typename S>
 static size_t tape_bytes(const snapx_config* cfg) {
-  if (validate_config(cfg)) return 0;
-  if (cfg->coord != SNAPX_COORD_CARTESIAN) return 0;
+  if (adjoint_check(cfg)) return 0;
   return tape_layout<S>(cfg, nullptr).bytes;
 }
 
 template <typename S>
 static size_t work_bytes(const snapx_config* cfg) {
-  if (validate_config(cfg)) return 0;
-  if (cfg->coord != SNAPX_COORD_CARTESIAN) return 0;
+  if (adjoint_check(cfg)) return 0;
   return adj_layout<S>(cfg, nullptr, nullptr).bytes;
 }
 
@@ -1546,8 +1720,12 @@ static int rk3_step(const snapx_config* cfg, double dt, const S* u_in, S* lambda
   char* base = reinterpret_cast<char*>(workspace);
   AdjSpace<S> s = adj_layout<S>(cfg, base, base + work_bytes<S>(cfg));
   hipStream_t st = as_stream(stream);
-  if (int e = adjoint_record<S>(cfg, dt, u_in, nullptr, s, st)) return e;
-  return adjoint_sweep<S>(cfg, dt, lambda, s, st);
+  if (cfg->coord == SNAPX_COORD_SPHERICAL_POLAR) {
+    if (int e = adjoint_record<S, true>(cfg, dt, u_in, nullptr, s, st)) return e;
+    return adjoint_sweep<S, true>(cfg, dt, lambda, s, st);
+  }
+  if (int e = adjoint_record<S, false>(cfg, dt, u_in, nullptr, s, st)) return e;
+  return adjoint_sweep<S, false>(cfg, dt, lambda, s, st);
 }
 
 template <typename S>
@@ -1562,7 +1740,9 @@ static int record(const snapx_config* cfg, double dt, const S* u_in, S* u_out, v
   SNAPX_CHECK_ARG(work_bytes_ >= work_bytes<S>(cfg), w + ": work space too small");
   AdjSpace<S> s =
       adj_layout<S>(cfg, reinterpret_cast<char*>(work), reinterpret_cast<char*>(tape));
-  return adjoint_record<S>(cfg, dt, u_in, u_out, s, as_stream(stream));
+  return cfg->coord == SNAPX_COORD_SPHERICAL_POLAR
+             ? adjoint_record<S, true>(cfg, dt, u_in, u_out, s, as_stream(stream))
+             : adjoint_record<S, false>(cfg, dt, u_in, u_out, s, as_stream(stream));
 }
 
 template <typename S>
@@ -1575,7 +1755,9 @@ static int sweep(const snapx_config* cfg, double dt, void* tape, size_t tape_byt
   SNAPX_CHECK_ARG(work_bytes_ >= work_bytes<S>(cfg), w + ": work space too small");
   AdjSpace<S> s =
       adj_layout<S>(cfg, reinterpret_cast<char*>(work), reinterpret_cast<char*>(tape));
-  return adjoint_sweep<S>(cfg, dt, lambda, s, as_stream(stream));
+  return cfg->coord == SNAPX_COORD_SPHERICAL_POLAR
+             ? adjoint_sweep<S, true>(cfg, dt, lambda, s, as_stream(stream))
+             : adjoint_sweep<S, false>(cfg, dt, lambda, s, as_stream(stream));
 }
 
 }  // namespace snapx

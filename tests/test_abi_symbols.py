@@ -53,10 +53,10 @@ def test_workspace_queries_without_device():
 
 def test_spherical_polar_limits_without_device():
     """Spherical-polar blocks run on the plain and fused paths (fused columns
-    up to one column workgroup); the adjoint refuses them, as do all paths
-    for blocks without a radial axis or with theta outside [0, pi]
-    (src/coord/spherical_polar.cpp:46-48).  Gnomonic panels: the same fused
-    column limit."""
+    up to one column workgroup) and in the adjoint (columns up to one column
+    workgroup); every path refuses blocks without a radial axis or with
+    theta outside [0, pi] (src/coord/spherical_polar.cpp:46-48).  Gnomonic
+    panels: the same fused column limit; the adjoint refuses them."""
     L = capi.lib()
     L.snapx_last_error.restype = ctypes.c_char_p
     kw = dict(nx1=16, nx2=8, nx3=8, nghost=3, x1min=1.0, x1max=2.0, x2min=0.7, x2max=2.4,
@@ -68,13 +68,18 @@ def test_spherical_polar_limits_without_device():
     tall = capi.make_config(**dict(kw, nx1=300))
     assert L.snapx_fused_workspace_bytes(ctypes.byref(tall)) == 0
     assert b"curvilinear columns of at most" in L.snapx_last_error()
-    assert L.snapx_adjoint_workspace_bytes(ctypes.byref(c)) == 0
+    assert L.snapx_adjoint_workspace_bytes(ctypes.byref(c)) > 0
+    assert L.snapx_adjoint_f32_workspace_bytes(ctypes.byref(c)) > 0
+    assert L.snapx_adjoint_workspace_bytes(ctypes.byref(tall)) == 0
+    assert b"spherical-polar columns of at most" in L.snapx_last_error()
     pi4 = 0.785
     gk = dict(kw, nx2=10, nx3=10, x2min=-pi4, x2max=pi4, x3min=-pi4, x3max=pi4,
               coord="gnomonic-equiangle")
     assert L.snapx_fused_workspace_bytes(ctypes.byref(capi.make_config(**gk))) > 0
     assert L.snapx_fused_workspace_bytes(ctypes.byref(capi.make_config(**dict(gk, nx1=300)))) == 0
     assert b"curvilinear columns of at most" in L.snapx_last_error()
+    assert L.snapx_adjoint_workspace_bytes(ctypes.byref(capi.make_config(**gk))) == 0
+    assert b"Cartesian and spherical-polar" in L.snapx_last_error()
     for bad in (dict(kw, x2max=3.3), dict(kw, nx1=1)):
         assert L.snapx_hydro_workspace_bytes(ctypes.byref(capi.make_config(**bad)), capi.F64) == 0
     c.coord = 7

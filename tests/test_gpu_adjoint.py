@@ -62,6 +62,14 @@ def _case(name):
         oc.grav1 = 0.0
         w, u = O.initialize(oc, w)
         return oc, u, 0.2
+    if name.startswith("sph"):
+        # spherical-polar sector (r in [1, 2], theta in [pi/4, 3pi/4], phi
+        # periodic) of tests/test_gpu_spherical.py: "sph<dims>_<solver><scheme>"
+        from test_gpu_spherical import _sector
+        dims, rest = name[3:].split("_")
+        n = (14, 10, 1) if dims == "2d" else (10, 8, 6)
+        oc, w, u = _sector(rest[:-1], int(rest[-1]), n=n, seed=9)
+        return oc, u, 0.01
     if name == "sod":
         oc = O.sod_config(64)
         w, u = O.initialize(oc, _sod_moving(oc))
@@ -96,9 +104,12 @@ def _bc_consistent(cfg, v):
 
 CASES = ["straka0", "straka1", "straka9", "straka_tall1", "straka_tall9", "box_lmars_roe",
          "box_hllc", "box_nograv", "sod"]
+# spherical-polar blocks: per-cell areas and volumes, the geometric sources
+# (with the x1 face pressure), per-layer column geometry and ghost distances
+SPH_CASES = ["sph3d_hllc1", "sph3d_lmars9", "sph3d_roe0", "sph2d_hllc1", "sph2d_roe9"]
 
 
-@pytest.mark.parametrize("case", CASES)
+@pytest.mark.parametrize("case", CASES + SPH_CASES)
 def test_adjoint_dot_product_vs_central_differences(case):
     oc, u_np, dt = _case(case)
     cfg = cfg_of(oc)
