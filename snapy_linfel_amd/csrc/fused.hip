@@ -1546,6 +1546,29 @@ __global__ void __launch_bounds__(kColThreads)
 
 // Every thread of the workgroup calls this with the same ColArgs; lane t
 // serves layer i = t mod L of column col (col < a.ncol checked here).
+// flux_jacobian with the enthalpy term (gamma / (gamma - 1)) p / rho formed
+// once through rcp_fast: the same matrix to an ulp or two
+template <int N>
+__device__ __forceinline__ void flux_jacobian_fast(double gm1, const double* w, double* J) {
+  const double ht = (gm1 + 1.) * rcp_fast(gm1) * w[IPR] * rcp_fast(w[IDN]);
+  const double v1 = w[IVX], v2 = w[IVY], v3 = w[IVZ];
+  const double s2 = (v1 * v1 + v2 * v2) + v3 * v3;
+#pragma unroll
+  for (int r = 0; r < N; ++r)
+#pragma unroll
+    for (int c = 0; c < N; ++c) {
+      const int rv = vic_row(N, r), cv = vic_row(N, c);
+      double x;
+      if (rv == 4 && cv == 0)
+        x = ((gm1 - 1.) * s2 * 0.5 - ht) * v1;
+      else if (rv == 4 && cv == 1)
+        x = (ht + s2 * 0.5) - gm1 * v1 * v1;
+      else
+        x = flux_jac(rv, cv, gm1, w[IDN], v1, v2, v3, w[IPR]);
+      J[r * N + c] = x;
+    }
+}
+
 // Block row i of a column's system (vic_assemble_cell, uniform Cartesian
 // areas): wm / wp the primitives below / above, Am / Ap the interface |A|
 // below / above, Dv the explicit tendency of the cell; reflecting column ends
@@ -1558,19 +1581,20 @@ __device__ __forceinline__ void block_row(const ColArgs& a, int i, int nl, bool 
                                           double* r) {
   constexpr int NN = N * N;
   double Jp[NN], Jn[NN];
-  flux_jacobian<double, N>(gm1, wm, Jp);
-  flux_jacobian<double, N>(gm1, wp, Jn);
-  const double hv = 0.5 * rcp_fast(a.vol);
+  flux_jacobian_fast<N>(gm1, wm, Jp);
+  flux_jacobian_fast<N>(gm1, wp, Jn);
+  // area * 0.5 / V (the FP64 plain path keeps the reference's order)
+  const double ch = a.area * (0.5 * rcp_fast(a.vol));
 #pragma unroll
   for (int rr = 0; rr < N; ++rr)
 #pragma unroll
     for (int c = 0; c < N; ++c) {
       int o = rr * N + c;
       double phi = ((rr == 1 && c == 0) || (vic_row(N, rr) == IPR && c == 1)) ? a.grav1 : 0.;
-      A[o] = (Am[o] * a.area + Ap[o] * a.area) * hv - phi;
+      A[o] = (Am[o] + Ap[o]) * ch - phi;
       if (rr == c) A[o] += idt;
-      B[o] = -(Am[o] + Jp[o]) * a.area * hv;
-      C[o] = -(Ap[o] - Jn[o]) * a.area * hv;
+      B[o] = -(Am[o] + Jp[o]) * ch;
+      C[o] = -(Ap[o] - Jn[o]) * ch;
     }
 #pragma unroll
   for (int rr = 0; rr < N; ++rr)
