@@ -135,28 +135,40 @@ __device__ __forceinline__ int bc_map(int x, int nx, int bin, int bout, bool& fl
   return x;
 }
 
-// cell index in the full array of interior-relative (i, j, k), mapping
-// through the boundaries; flips[a] reports a sign change of the axis-a
-// momentum
-__device__ __forceinline__ int mapped_cell(const FusedArgs& a, int i, int j, int k, bool* flips) {
-  flips[0] = flips[1] = flips[2] = false;
+// offset within an x1-x2 plane of the full array of interior-relative
+// (i, j), mapping through the x1 / x2 boundaries (flips[0], flips[1])
+__device__ __forceinline__ int plane_cell(const FusedArgs& a, int i, int j, bool* flips) {
+  flips[0] = flips[1] = false;
   int lim = a.nx1 + a.ng - 1;
   i = i > lim ? lim : i;
   i = bc_map(i, a.nx1, a.bc[0], a.bc[1], flips[0]);
-  int qi = i + a.ng, qj = 0, qk = 0;
+  int qj = 0;
   if (a.n2 > 1) {
     int l2 = a.nx2 + a.ng - 1;
     j = j > l2 ? l2 : j;
     j = bc_map(j, a.nx2, a.bc[2], a.bc[3], flips[1]);
     qj = j + a.ng;
   }
-  if (a.n3 > 1) {
-    int l3 = a.nx3 + a.ng - 1;
-    k = k > l3 ? l3 : k;
-    k = bc_map(k, a.nx3, a.bc[4], a.bc[5], flips[2]);
-    qk = k + a.ng;
-  }
-  return (qk * a.n2 + qj) * a.n1 + qi;
+  return qj * a.n1 + i + a.ng;
+}
+
+// full-array index of the first cell of x3 slice k (interior-relative),
+// mapped through the x3 boundaries (flip: the x3 momentum changes sign)
+__device__ __forceinline__ int slice_start(const FusedArgs& a, int k, bool& flip) {
+  flip = false;
+  if (a.n3 <= 1) return 0;
+  int l3 = a.nx3 + a.ng - 1;
+  k = k > l3 ? l3 : k;
+  k = bc_map(k, a.nx3, a.bc[4], a.bc[5], flip);
+  return (k + a.ng) * a.n2 * a.n1;
+}
+
+// cell index in the full array of interior-relative (i, j, k), mapping
+// through the boundaries; flips[a] reports a sign change of the axis-a
+// momentum
+__device__ __forceinline__ int mapped_cell(const FusedArgs& a, int i, int j, int k, bool* flips) {
+  const int p = plane_cell(a, i, j, flips);
+  return slice_start(a, k, flips[2]) + p;
 }
 
 // reciprocals of the fused kernels: the FP32 hardware reciprocal (this unit
@@ -1052,57 +1064,89 @@ __global__ void __launch_bounds__(kRhsThreads, 4)
     for (int q = 0; q < 4; ++q) part[q][v] = 0.f;
   }
 
+  // the own cell's offset in its plane (interior in x1 and x2)
+  const int own_pq = (o2 + cy + a.ng) * a.n1 + o1 + cx + a.ng;
+
   for (int kk = o3 - 3; kk <= o3 + ez + 2; ++kk) {
     float* ws = smem + ((kk - o3 + 3) & 1) * NP * HS;
-    // ---- halo'd x1-x2 plane of slice kk: conserved -> primitive; x1 ghost
-    //      refill of the tile's own columns as in rhs_tile ----
-    for (int l = threadIdx.x; l < HS; l += blockDim.x) {
-      int x;
-      const int y = dH1.div(l, x);
-      const int gi = o1 + x - 3, gj = o2 + y - 3;
-      bool fl[3];
-      const int q = mapped_cell(a, gi, gj, kk, fl);
-      float U[5], W[5];
+    bool f3;
+    const int s0 = slice_start(a, kk, f3);
+    if (kk >= o3 && kk < o3 + ez) {
+      // ---- halo'd x1-x2 plane of slice kk: conserved -> primitive; x1
+      //      ghost refill of the tile's own columns as in rhs_tile ----
+      // cell l of the plane from source q (flips f1, f2 and the slice's
+      // f3) at interior-relative x1 index gi; qb >= 0: refilled x1 ghost
+      // whose interior base cell is qb
+      auto load_cell = [&](int l, int q, bool f1, bool f2, int gi, int qb, bool upper) {
+        float U[5], W[5];
 #pragma unroll
-      for (int v = 0; v < 5; ++v) U[v] = u[v * ncell + q];
-      if (fl[0]) U[IVX] = -U[IVX];
-      if (fl[1]) U[IVY] = -U[IVY];
-      if (fl[2]) U[IVZ] = -U[IVZ];
-      if (kBg) {
-        float rtot;
-        bg_load_prim(a, BV, U, gi + a.ng, W, rtot);
-        if (refill && (gi < 0 || gi >= a.nx1) && (unsigned)(y - 3) < (unsigned)e2) {
-          bool flb[3];
-          const bool upper = gi >= a.nx1;
-          const int qb = mapped_cell(a, upper ? a.nx1 - 1 : 0, gj, kk, flb);
-          bg_refill_prim(a, BV, u, ncell, qb, gi, W, rtot);
+        for (int v = 0; v < 5; ++v) U[v] = u[v * ncell + q];
+        if (f1) U[IVX] = -U[IVX];
+        if (f2) U[IVY] = -U[IVY];
+        if (f3) U[IVZ] = -U[IVZ];
+        if (kBg) {
+          float rtot;
+          bg_load_prim(a, BV, U, gi + a.ng, W, rtot);
+          if (qb >= 0) bg_refill_prim(a, BV, u, ncell, qb, gi, W, rtot);
+          ws[5 * HS + l] = rtot;
+        } else {
+          cons_to_prim(U, gammad, W);
+          if (qb >= 0) {
+            float Ub[5], Wb[5];
+#pragma unroll
+            for (int v = 0; v < 5; ++v) Ub[v] = u[v * ncell + qb];
+            cons_to_prim(Ub, gammad, Wb);  // rho and p do not depend on the flips
+            const IsentropicRefill R(Wb[IDN], Wb[IPR], gammad, gdz, upper ? 1 : -1);
+            R.layer(upper ? gi - a.nx1 + 1 : -gi, W[IDN], W[IPR]);
+          }
         }
-        ws[5 * HS + l] = rtot;
-      } else {
-        cons_to_prim(U, gammad, W);
-        if (refill && (gi < 0 || gi >= a.nx1) && (unsigned)(y - 3) < (unsigned)e2) {
-          const bool upper = gi >= a.nx1;
-          bool flb[3];
-          const int qb = mapped_cell(a, upper ? a.nx1 - 1 : 0, gj, kk, flb);
-          float Ub[5], Wb[5];
 #pragma unroll
-          for (int v = 0; v < 5; ++v) Ub[v] = u[v * ncell + qb];
-          cons_to_prim(Ub, gammad, Wb);  // rho and p do not depend on the flips
-          const IsentropicRefill R(Wb[IDN], Wb[IPR], gammad, gdz, upper ? 1 : -1);
-          R.layer(upper ? gi - a.nx1 + 1 : -gi, W[IDN], W[IPR]);
+        for (int v = 0; v < 5; ++v) ws[v * HS + l] = W[v];
+      };
+      for (int l = threadIdx.x; l < HS; l += blockDim.x) {
+        int x;
+        const int y = dH1.div(l, x);
+        const int gi = o1 + x - 3, gj = o2 + y - 3;
+        bool fl[2];
+        const int q = s0 + plane_cell(a, gi, gj, fl);
+        int qb = -1;
+        const bool upper = gi >= a.nx1;
+        if (refill && (gi < 0 || gi >= a.nx1) && (unsigned)(y - 3) < (unsigned)e2) {
+          bool flb[2];
+          qb = s0 + plane_cell(a, upper ? a.nx1 - 1 : 0, gj, flb);
+        }
+        load_cell(l, q, fl[0], fl[1], gi, qb, upper);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+#pragma unroll
+        for (int q = 0; q < 5; ++q) win[v][q] = win[v][q + 1];
+        win[v][5] = own ? ws[v * HS + lin_own] : 1.f;
+      }
+    } else {
+      // ---- x3 halo slice: only the own column's cell is used (x3 faces) ----
+      float W[5] = {1.f, 1.f, 1.f, 1.f, 1.f};
+      if (own) {
+        float U[5];
+        const int q = s0 + own_pq;
+#pragma unroll
+        for (int v = 0; v < 5; ++v) U[v] = u[v * ncell + q];
+        if (f3) U[IVZ] = -U[IVZ];
+        if (kBg) {
+          float rtot;
+          bg_load_prim(a, BV, U, o1 + cx + a.ng, W, rtot);
+        } else {
+          cons_to_prim(U, gammad, W);
         }
       }
 #pragma unroll
-      for (int v = 0; v < 5; ++v) ws[v * HS + l] = W[v];
-    }
-    __syncthreads();
+      for (int v = 0; v < 5; ++v) {
 #pragma unroll
-    for (int v = 0; v < 5; ++v) {
-#pragma unroll
-      for (int q = 0; q < 5; ++q) win[v][q] = win[v][q + 1];
-      win[v][5] = own ? ws[v * HS + lin_own] : 1.f;
+        for (int q = 0; q < 5; ++q) win[v][q] = win[v][q + 1];
+        win[v][5] = W[v];
+      }
     }
-
     // ---- x3 face f = kk - 2 (below slice f) from the own column ----
     const int f = kk - 2;
     float f3hi[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
@@ -2399,6 +2443,8 @@ static int make_march_args(const snapx_config* cfg, double dt, FusedArgs* out) {
   a.merged = 0;
   SNAPX_CHECK_ARG(a.tx1 >= 1 && a.tx2 >= 1 && a.tx3 >= 1 && a.tx1 * a.tx2 <= kRhsThreads,
                   "fused path: bad x1-x2 tile");
+  // the kernel keeps the halo'd x1 positions of its plane cells in 8 bits
+  SNAPX_CHECK_ARG(a.tx1 + 6 <= 255, "fused path: x1 tile too long for the marching kernel");
   SNAPX_CHECK_ARG(march_smem_bytes(a) <= 64 * 1024, "fused path: tile needs more than 64 KB LDS");
   *out = a;
   return 0;

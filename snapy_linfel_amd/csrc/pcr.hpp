@@ -160,6 +160,51 @@ __device__ __forceinline__ void pcr_solve(double* A, double* B, double* C, doubl
     for (int e = 0; e < N; ++e) r[e] = 0.;
   }
   for (int sft = 1; sft < nl; sft <<= 1) {
+    if constexpr (N == 3) {
+      if (2 * sft >= nl) {
+        // Last level: nl <= 2 sft, so a row couples to row i - sft or to
+        // row i + sft, never to both, and the coupling of that row back to
+        // it is its only one (its B is zero below sft and its C from sft
+        // on).  Row j < sft publishes A^-1 C_j for row j + sft, row j >= sft
+        // publishes A^-1 B_j for row j - sft: one product instead of two,
+        // and one elimination instead of two with the new couplings (which
+        // are zero).  N = 5 keeps the general level: its registers are at
+        // the limit.
+        const bool lo = i < sft;
+        double X[NN], inv[NN], P[NN], p[N];
+#pragma unroll
+        for (int e = 0; e < NN; ++e) X[e] = lo ? C[e] : B[e];
+        inverse3<double, Fast>(A, inv);
+        matmul<double, 3>(inv, X, P);
+        matvec<double, 3>(inv, r, p);
+#pragma unroll
+        for (int e = 0; e < NN; ++e) sh[e * T + t] = P[e];
+#pragma unroll
+        for (int e = 0; e < N; ++e) sh[(NN + e) * T + t] = p[e];
+        column_barrier(wave);
+        const bool hm = act && i - sft >= 0;
+        const bool hp = act && i + sft < nl;
+        if (hm || hp) {
+          const int tn = hm ? t - sft : t + sft;
+          double y[NN];
+#pragma unroll
+          for (int e = 0; e < NN; ++e) {
+            P[e] = sh[e * T + tn];
+            X[e] = hm ? B[e] : C[e];
+          }
+#pragma unroll
+          for (int e = 0; e < N; ++e) p[e] = sh[(NN + e) * T + tn];
+          matmul<double, N>(X, P, y);
+#pragma unroll
+          for (int e = 0; e < NN; ++e) A[e] -= y[e];
+          matvec<double, N>(X, p, y);
+#pragma unroll
+          for (int e = 0; e < N; ++e) r[e] -= y[e];
+        }
+        column_barrier(wave);
+        break;
+      }
+    }
     pcr_publish<N, Fast>(A, B, C, r, sh, T, t);
     column_barrier(wave);
     const bool hm = act && i - sft >= 0;
