@@ -1072,50 +1072,64 @@ __global__ void __launch_bounds__(kRhsThreads, 4)
     bool f3;
     const int s0 = slice_start(a, kk, f3);
     if (kk >= o3 && kk < o3 + ez) {
-      // ---- halo'd x1-x2 plane of slice kk: conserved -> primitive; x1
-      //      ghost refill of the tile's own columns as in rhs_tile ----
-      // cell l of the plane from source q (flips f1, f2 and the slice's
-      // f3) at interior-relative x1 index gi; qb >= 0: refilled x1 ghost
-      // whose interior base cell is qb
-      auto load_cell = [&](int l, int q, bool f1, bool f2, int gi, int qb, bool upper) {
-        float U[5], W[5];
-#pragma unroll
-        for (int v = 0; v < 5; ++v) U[v] = u[v * ncell + q];
-        if (f1) U[IVX] = -U[IVX];
-        if (f2) U[IVY] = -U[IVY];
-        if (f3) U[IVZ] = -U[IVZ];
-        if (kBg) {
-          float rtot;
-          bg_load_prim(a, BV, U, gi + a.ng, W, rtot);
-          if (qb >= 0) bg_refill_prim(a, BV, u, ncell, qb, gi, W, rtot);
-          ws[5 * HS + l] = rtot;
-        } else {
-          cons_to_prim(U, gammad, W);
-          if (qb >= 0) {
-            float Ub[5], Wb[5];
-#pragma unroll
-            for (int v = 0; v < 5; ++v) Ub[v] = u[v * ncell + qb];
-            cons_to_prim(Ub, gammad, Wb);  // rho and p do not depend on the flips
-            const IsentropicRefill R(Wb[IDN], Wb[IPR], gammad, gdz, upper ? 1 : -1);
-            R.layer(upper ? gi - a.nx1 + 1 : -gi, W[IDN], W[IPR]);
-          }
-        }
-#pragma unroll
-        for (int v = 0; v < 5; ++v) ws[v * HS + l] = W[v];
-      };
+      // ---- halo'd x1-x2 plane of slice kk: conserved -> primitive ----
       for (int l = threadIdx.x; l < HS; l += blockDim.x) {
         int x;
         const int y = dH1.div(l, x);
         const int gi = o1 + x - 3, gj = o2 + y - 3;
         bool fl[2];
         const int q = s0 + plane_cell(a, gi, gj, fl);
-        int qb = -1;
-        const bool upper = gi >= a.nx1;
-        if (refill && (gi < 0 || gi >= a.nx1) && (unsigned)(y - 3) < (unsigned)e2) {
-          bool flb[2];
-          qb = s0 + plane_cell(a, upper ? a.nx1 - 1 : 0, gj, flb);
+        // a physical x1 ghost of the tile's own rows: its density and
+        // pressure come from the refill pass below (velocities from the
+        // mirror source)
+        const bool rf = refill && (gi < 0 || gi >= a.nx1) && (unsigned)(y - 3) < (unsigned)e2;
+        float U[5], W[5];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) U[v] = u[v * ncell + q];
+        if (fl[0]) U[IVX] = -U[IVX];
+        if (fl[1]) U[IVY] = -U[IVY];
+        if (f3) U[IVZ] = -U[IVZ];
+        if (kBg) {
+          float rtot;
+          bg_load_prim(a, BV, U, gi + a.ng, W, rtot);
+          if (!rf) ws[5 * HS + l] = rtot;
+        } else {
+          cons_to_prim(U, gammad, W);
         }
-        load_cell(l, q, fl[0], fl[1], gi, qb, upper);
+#pragma unroll
+        for (int v = 0; v < 5; ++v)
+          if (!rf || (v != IDN && v != IPR)) ws[v * HS + l] = W[v];
+      }
+      // ---- x1 ghost refill (src/hydro/hydro.cpp:231-269) of the tile's own
+      //      rows, from the last thread down: these cells are few, and one
+      //      pass keeps the refill's branch out of the other wavefronts ----
+      if (refill) {
+        const int nlo = max(0, 3 - o1), nhi = max(0, H1 - (a.nx1 - o1 + 3));
+        const int per = nlo + nhi;
+        for (int k = blockDim.x - 1 - threadIdx.x; k < per * e2; k += blockDim.x) {
+          const int row = k / per, c = k - row * per;
+          const bool upper = c >= nlo;
+          const int x = upper ? a.nx1 - o1 + 3 + (c - nlo) : c, y = row + 3;
+          const int l = y * H1 + x, gi = o1 + x - 3, gj = o2 + row;
+          bool flb[2];
+          const int qb = s0 + plane_cell(a, upper ? a.nx1 - 1 : 0, gj, flb);
+          if (kBg) {
+            float W[5], rtot;
+            bg_refill_prim(a, BV, u, ncell, qb, gi, W, rtot);
+            ws[IDN * HS + l] = W[IDN];
+            ws[IPR * HS + l] = W[IPR];
+            ws[5 * HS + l] = rtot;
+          } else {
+            float Ub[5], Wb[5], rho, pre;
+#pragma unroll
+            for (int v = 0; v < 5; ++v) Ub[v] = u[v * ncell + qb];
+            cons_to_prim(Ub, gammad, Wb);  // rho and p do not depend on the flips
+            const IsentropicRefill R(Wb[IDN], Wb[IPR], gammad, gdz, upper ? 1 : -1);
+            R.layer(upper ? gi - a.nx1 + 1 : -gi, rho, pre);
+            ws[IDN * HS + l] = rho;
+            ws[IPR * HS + l] = pre;
+          }
+        }
       }
       __syncthreads();
 #pragma unroll
@@ -1791,26 +1805,54 @@ __device__ __forceinline__ void column_pcr2(float* __restrict__ u, float* __rest
     double wm0[5], wp0[5], wp1[5];  // below row i0, above row i0, above row i1
 #pragma unroll
     for (int v = 0; v < 5; ++v) wm0[v] = wp0[v] = wp1[v] = w1[v];
-    if (act0) {
-      if (m > 0) {
+    if (act0 && m > 0) {
 #pragma unroll
-        for (int v = 0; v < 5; ++v) wm0[v] = sh[(5 + v) * T + t - 1];
-      } else {
-        float_prim_at<kBg>(Um, flm, a, lm, wm0);
-        refill_ghost<false, kBg>(w0, wm0, a, -1, Uc0);
-      }
-      if (top0) {
-        float_prim_at<kBg>(Up, flp, a, lp, wp0);
-        refill_ghost<false, kBg>(w0, wp0, a, +1, Uc0);
-      }
+      for (int v = 0; v < 5; ++v) wm0[v] = sh[(5 + v) * T + t - 1];
     }
-    if (act1) {
-      if (!top1) {
+    if (act1 && !top1) {
 #pragma unroll
-        for (int v = 0; v < 5; ++v) wp1[v] = sh[v * T + t + 1];
-      } else {
-        float_prim_at<kBg>(Up, flp, a, lp, wp1);
-        refill_ghost<false, kBg>(w1, wp1, a, +1, Uc1);
+      for (int v = 0; v < 5; ++v) wp1[v] = sh[v * T + t + 1];
+    }
+    {
+      // The physical ghosts of the column ends in one pass for the
+      // wavefront (a column is one wavefront, so its bottom and top lanes
+      // would otherwise take the refill's branch one after the other): lane
+      // 0 the ghost below row 0, the top lane the ghost above row nl - 1.
+      const bool lo = act0 && m == 0, hi = top0 || top1;
+      if (lo || hi) {
+        const bool b1 = !lo && top1;  // the boundary row is i1 (else i0)
+        float Us[5], Ubd[5];
+        double wb[5], g[5];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+          Us[v] = lo ? Um[v] : Up[v];
+          Ubd[v] = b1 ? Uc1[v] : Uc0[v];
+          wb[v] = b1 ? w1[v] : w0[v];
+        }
+        float_prim_at<kBg>(Us, lo ? flm : flp, a, lo ? lm : lp, g);
+        refill_ghost<false, kBg>(wb, g, a, lo ? -1 : +1, Ubd);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+          if (lo)
+            wm0[v] = g[v];
+          else if (top0)
+            wp0[v] = g[v];
+          else
+            wp1[v] = g[v];
+        }
+      }
+      if (lo && hi) {
+        // columns of one or two cells: lane 0 has the top ghost as well
+        double g[5];
+        float_prim_at<kBg>(Up, flp, a, lp, g);
+        refill_ghost<false, kBg>(top0 ? w0 : w1, g, a, +1, top0 ? Uc0 : Uc1);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+          if (top0)
+            wp0[v] = g[v];
+          else
+            wp1[v] = g[v];
+        }
       }
     }
     column_barrier(wave);
