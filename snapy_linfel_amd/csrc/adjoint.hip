@@ -1540,8 +1540,10 @@ static const double kStageW[3][3] = {
 // Forward recompute of one step from u_in into the tape: stage inputs, their
 // primitives (revised ghosts), explicit tendencies, VIC blocks and delta.
 // With u_out, also the step's output (third stage combined, boundaries).
-// The explicit part runs on the plain path in the storage type; the column
-// solve is this file's FP64 one for either type.
+// The explicit part runs in the storage type: FP64 on the plain path, FP32
+// through the fused path's K1 (the arithmetic of the fused forward that the
+// FP32 tapes replay) where the fused path takes the block, else on the plain
+// path; the column solve is this file's FP64 one for either type.
 template <typename S, bool kSph>
 static int adjoint_record(const snapx_config* cfg, double dt, const S* u_in, S* u_out,
                           const AdjSpace<S>& s, hipStream_t st) {
@@ -1555,12 +1557,27 @@ static int adjoint_record(const snapx_config* cfg, double dt, const S* u_in, S* 
   const auto& W = kStageW;
   const int N = ((cfg->implicit_scheme >> 3) & 1) ? 5 : 3;
   const int ncol = (g.n2 > 1 ? cfg->nx2 : 1) * (g.n3 > 1 ? cfg->nx3 : 1);
+  const bool fused_rhs = sizeof(S) == sizeof(float) && fused_rhs_ok(cfg);
   TraceRange range("snapx adjoint record");
   SNAPX_HIP(hipMemcpyAsync(s.us[0], u_in, fb, hipMemcpyDeviceToDevice, st));
   for (int k = 0; k < 3; ++k) {
-    if (int e = snapx_hydro_forward(&c0, dt, s.us[k], s.w[k], s.dux[k], nullptr, nullptr,
-                                    nullptr, s.hydro_ws, s.hydro_bytes, dtype, st))
+    if (fused_rhs) {
+      // primitives with the revised x1 ghosts, as the plain forward leaves
+      // them, and the tendency of the interior from K1; the ghost entries
+      // (the plain path's body forces of the ghost cells, which the
+      // boundary functions overwrite) are zero
+      if (int e = snapx_ideal_gas_cons2prim(cfg, s.us[k], s.w[k], dtype, st)) return e;
+      if (g.n1 > 1 && cfg->grav1 != 0.)
+        if (int e = snapx_revise_x1_ghost(cfg, s.w[k], dtype, st)) return e;
+      hipLaunchKernelGGL(zero_kernel<S>, blocks_for(n, kT), dim3(kT), 0, st, s.dux[k], n);
+      SNAPX_LAUNCH_CHECK();
+      if (int e = fused_explicit_tendency(cfg, dt, reinterpret_cast<const float*>(s.us[k]),
+                                          reinterpret_cast<float*>(s.dux[k]), st))
+        return e;
+    } else if (int e = snapx_hydro_forward(&c0, dt, s.us[k], s.w[k], s.dux[k], nullptr, nullptr,
+                                           nullptr, s.hydro_ws, s.hydro_bytes, dtype, st)) {
       return e;
+    }
     // du of the stage: the column solve on a copy of the explicit tendency
     // (without u_out, the third stage's solve matters only for the blocks and
     // delta it leaves for the reverse sweep)

@@ -97,6 +97,14 @@ struct TraceRange {
 // Validates a config for the kernels of this library.
 int validate_config(const snapx_config* c);
 
+// The fused path's explicit tendency (its K1) for another unit of the
+// library: du <- dt times the flux divergence and the sources of the FP32
+// state u, interior cells only, enqueued on st.  fused_rhs_ok tells, without
+// setting an error, whether the fused path takes the block's options.
+bool fused_rhs_ok(const snapx_config* c);
+int fused_explicit_tendency(const snapx_config* c, double dt, const float* u, float* du,
+                            hipStream_t st);
+
 // Quotient and remainder of a small non-negative index by a positive
 // divisor through the FP32 reciprocal and one correction step (an integer
 // division by a runtime divisor costs some forty instructions).  Exact for

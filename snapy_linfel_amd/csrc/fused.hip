@@ -2367,26 +2367,35 @@ static size_t rhs_smem_bytes(const FusedArgs& a) {
   return sizeof(float) * (5 * (H + rhs_face_count(a, a.merged)) + fp + bgp);
 }
 
-static int check_fused(const snapx_config* c) {
-  if (int e = validate_config(c)) return e;
+// The option of a block that the fused path does not take, or null.
+static const char* fused_unsupported(const snapx_config* c) {
   if (c->coord != SNAPX_COORD_CARTESIAN) {
     // the curvilinear K2 is the cell-parallel column kernel only
     const int N = ((c->implicit_scheme >> 3) & 1) ? 5 : 3;
-    SNAPX_CHECK_ARG(c->nx1 <= (N == 3 ? PcrShape<3>::kThreads : PcrShape<5>::kThreads),
-                    "fused path: curvilinear columns of at most 256 (3x3) / 128 (5x5) cells");
+    if (c->nx1 > (N == 3 ? PcrShape<3>::kThreads : PcrShape<5>::kThreads))
+      return "fused path: curvilinear columns of at most 256 (3x3) / 128 (5x5) cells";
   }
-  SNAPX_CHECK_ARG(c->nghost == 3, "fused path: nghost must be 3");
+  if (c->nghost != 3) return "fused path: nghost must be 3";
   // the reference advances custom ghost cells with the stage combine and the
   // body forces; the fused kernels never update ghost cells within a step
   for (int b = 0; b < 6; ++b)
-    SNAPX_CHECK_ARG(c->bc[b] != SNAPX_BC_CUSTOM,
-                    "fused path: custom boundaries are not supported (use the plain path)");
-  SNAPX_CHECK_ARG(c->nx1 > 1, "fused path: x1 must be active");
-  SNAPX_CHECK_ARG(!c->limiter, "fused path: the EOS limiter is not supported (use the plain path)");
-  SNAPX_CHECK_ARG(c->non_hydrostatic == 1.0,
-                  "fused path: non-hydrostatic < 1 is not supported (use the plain path)");
+    if (c->bc[b] == SNAPX_BC_CUSTOM)
+      return "fused path: custom boundaries are not supported (use the plain path)";
+  if (c->nx1 <= 1) return "fused path: x1 must be active";
+  if (c->limiter) return "fused path: the EOS limiter is not supported (use the plain path)";
+  if (c->non_hydrostatic != 1.0)
+    return "fused path: non-hydrostatic < 1 is not supported (use the plain path)";
   for (int d = 0; d < 3; ++d)
-    SNAPX_CHECK_ARG(!c->disable_flux[d], "fused path: disable-flux is not supported");
+    if (c->disable_flux[d]) return "fused path: disable-flux is not supported";
+  return nullptr;
+}
+
+static int check_fused(const snapx_config* c) {
+  if (int e = validate_config(c)) return e;
+  if (const char* m = fused_unsupported(c)) {
+    set_error(m);
+    return SNAPX_EINVAL;
+  }
   return 0;
 }
 
@@ -2458,8 +2467,18 @@ static int make_fused_args(const snapx_config* cfg, Tile t, double dt, FusedArgs
 
 // 2.5-D K1 for 3-D blocks (fused_rhs_march_kernel): x1-x2 tiles of at most
 // kRhsThreads cells (32 x 8 where the block allows) marching through slabs of
-// up to kMarchSlab slices along x3.
-constexpr int kMarchSlab = 16;  // measured best of 4 .. 128 on a 256 x 256 x 128 block
+// 16, 8 or 4 slices along x3: the deepest slab that gives at least
+// kMarchGroups (tile, slab) workgroups, else 4 slices.  Blocks with fewer
+// than kMarchMinGroups workgroups at 4 slices take the 16 x 4 x 2 tiles.
+// K1 per launch (us; slab 16 / 8 / 4 / tiled) on blocks nx1 x nx2 x nx3:
+// 128x256x256 470 / 478 / - / -, 64x256x256 237 / 243 / 254 / -,
+// 128x128x128 172 / 128 / 133 / 171, 96x128x128 168 / 109 / 116 / 238,
+// 64x128x128 152 / 94 / 72 / 177, 128x96x96 - / 104 / 100 / -,
+// 256x64x64 - / 94 / 71 / -, 128x64x64 151 / 82 / 53 / 95,
+// 64x64x64 150 / 80 / 46 / 52, 32x64x64 152 / 78 / 43 / 33.
+constexpr int kMarchSlabs[3] = {16, 8, 4};
+constexpr int kMarchGroups = 768;
+constexpr int kMarchMinGroups = 256;
 
 static bool march_ok(const snapx_config* cfg) {
   return cfg->nx3 > 1 && cfg->nx2 > 1 && cfg->coord == SNAPX_COORD_CARTESIAN;
@@ -2480,7 +2499,15 @@ static int make_march_args(const snapx_config* cfg, double dt, FusedArgs* out) {
   Tile t;
   t.tx1 = std::min(32, cfg->nx1);
   t.tx2 = std::min(cfg->nx2, kRhsThreads / t.tx1);
-  t.tx3 = std::min(kMarchSlab, cfg->nx3);
+  // the deepest slab with kMarchGroups workgroups, else the shallowest
+  const int ntile = ((cfg->nx1 + t.tx1 - 1) / t.tx1) * ((cfg->nx2 + t.tx2 - 1) / t.tx2);
+  int slab = kMarchSlabs[2];
+  for (int k = 0; k < 3; ++k)
+    if ((int64_t)ntile * ((cfg->nx3 + kMarchSlabs[k] - 1) / kMarchSlabs[k]) >= kMarchGroups) {
+      slab = kMarchSlabs[k];
+      break;
+    }
+  t.tx3 = std::min(slab, cfg->nx3);
   FusedArgs a = fill_fused_args(cfg, t, dt);
   a.merged = 0;
   SNAPX_CHECK_ARG(a.tx1 >= 1 && a.tx2 >= 1 && a.tx3 >= 1 && a.tx1 * a.tx2 <= kRhsThreads,
@@ -2584,6 +2611,90 @@ static int pcr_block(const ColArgs& c, int N) {
   return cpw * c.lanes;
 }
 
+// K1 of one block: the kernel (marching tiles for 3-D blocks with enough
+// (tile, slab) workgroups, curvilinear, perturbation-path or 1-D / 2-D /
+// small 3-D tiles), its arguments and launch shape.
+struct RhsPlan {
+  FusedArgs a;
+  bool march, sph, gno;
+  const snapx_config* cfg;
+  int nblk;
+  size_t smem;
+};
+
+// bg: device tables of the hydrostatic background (null: total state); dtp:
+// the time step in device memory (replaces dt)
+static int rhs_plan(const snapx_config* cfg, double dt, const void* bg, const double* dtp,
+                    RhsPlan* p) {
+  Grid g = make_grid(cfg);
+  bool march = march_ok(cfg);
+  const bool sph = cfg->coord != SNAPX_COORD_CARTESIAN;  // curvilinear kernels
+  const bool gno = cfg->coord == SNAPX_COORD_GNOMONIC_EQUIANGLE;
+  FusedArgs a;
+  if (march) {
+    if (int e = make_march_args(cfg, dt, &a)) return e;
+    // the marching kernel needs enough (tile, slab) workgroups to fill the
+    // GPU; below kMarchMinGroups the merged 16x4x2 tiles win
+    if (a.nt1 * a.nt2 * a.nt3 < kMarchMinGroups) march = false;
+  }
+  if (!march) {
+    Tile t = choose_tile(g);
+    // 3-D tiles: 16x4x2, every face of the tile in one LDS pass
+    if (g.n3 > 1) t = {16, 4, 2};
+    // the perturbation path runs merged tiles only
+    if (bg && g.n3 == 1 && g.n2 > 1) t = {16, 4, 1};
+    if (int e = make_fused_args(cfg, t, dt, &a)) return e;
+  }
+  if (bg) {
+    a.bgd = reinterpret_cast<const double*>(bg);
+    a.bgt = reinterpret_cast<const float*>(a.bgd + 2 * g.n1);
+    SNAPX_CHECK_ARG(march || a.merged, "perturbation path: tile without a merged face pass");
+  }
+  a.dtp = dtp;
+  size_t smem = march ? march_smem_bytes(a) : rhs_smem_bytes(a);
+  SNAPX_CHECK_ARG(smem <= 64 * 1024, "fused path: tile needs more than 64 KB LDS");
+  p->a = a;
+  p->march = march;
+  p->sph = sph;
+  p->gno = gno;
+  p->cfg = cfg;
+  p->nblk = a.nt1 * a.nt2 * a.nt3;
+  p->smem = smem;
+  return 0;
+}
+
+// du <- the explicit tendency of u (dt times the flux divergence and the
+// sources, interior cells); ra / rb: optional events stamped with the start
+// and end of the dispatch
+static void rhs_launch(const RhsPlan& p, const float* u, float* du, hipStream_t st,
+                       hipEvent_t ra = nullptr, hipEvent_t rb = nullptr) {
+  const uint32_t smem = (uint32_t)p.smem;
+  const dim3 grid(p.nblk), block(kRhsThreads);
+  if (p.sph)
+    hipExtLaunchKernelGGL(p.gno ? fused_rhs_gno_kernel : fused_rhs_sph_kernel, grid, block, smem,
+                          st, ra, rb, 0u, u, du, p.a);
+  else if (p.march)
+    hipExtLaunchKernelGGL(march_kernel_for(p.cfg, p.a.bgt != nullptr), grid, block, smem, st, ra,
+                          rb, 0u, u, du, p.a);
+  else if (p.a.bgt)
+    hipExtLaunchKernelGGL(fused_rhs_bg_kernel, grid, block, smem, st, ra, rb, 0u, u, du, p.a);
+  else if (p.a.merged)
+    hipExtLaunchKernelGGL(fused_rhs_kernel<1>, grid, block, smem, st, ra, rb, 0u, u, du, p.a);
+  else
+    hipExtLaunchKernelGGL(fused_rhs_kernel<0>, grid, block, smem, st, ra, rb, 0u, u, du, p.a);
+}
+
+bool fused_rhs_ok(const snapx_config* c) { return fused_unsupported(c) == nullptr; }
+
+int fused_explicit_tendency(const snapx_config* c, double dt, const float* u, float* du,
+                            hipStream_t st) {
+  RhsPlan rp;
+  if (int e = rhs_plan(c, dt, nullptr, nullptr, &rp)) return e;
+  rhs_launch(rp, u, du, st);
+  SNAPX_LAUNCH_CHECK();
+  return 0;
+}
+
 struct EventPair {
   hipEvent_t a, b;
 };
@@ -2603,39 +2714,13 @@ static int fused_step(const snapx_config* cfg, double dt, float* u, void* worksp
   float* u0 = reinterpret_cast<float*>(base + field);
   double* scr = reinterpret_cast<double*>(base + 2 * field);
 
-  bool march = march_ok(cfg);
-  const bool sph = cfg->coord != SNAPX_COORD_CARTESIAN;  // curvilinear kernels
-  const bool gno = cfg->coord == SNAPX_COORD_GNOMONIC_EQUIANGLE;
-  FusedArgs a;
-  if (march) {
-    if (int e = make_march_args(cfg, dt, &a)) return e;
-    // the marching kernel needs enough (tile, slab) workgroups to fill the
-    // GPU; below 512 the merged 16x4x2 tiles win (K1 per launch: 64x128x128
-    // 191 against 289 us, 128x128x128 356 against 342, 64x256x256 705
-    // against 432)
-    if (a.nt1 * a.nt2 * a.nt3 < 512) march = false;
-  }
-  if (!march) {
-    Tile t = choose_tile(g);
-    // 3-D tiles: 16x4x2, every face of the tile in one LDS pass
-    if (g.n3 > 1) t = {16, 4, 2};
-    // the perturbation path runs merged tiles only
-    if (bg && g.n3 == 1 && g.n2 > 1) t = {16, 4, 1};
-    if (int e = make_fused_args(cfg, t, dt, &a)) return e;
-  }
-  if (bg) {
-    a.bgd = reinterpret_cast<const double*>(bg);
-    a.bgt = reinterpret_cast<const float*>(a.bgd + 2 * g.n1);
-    SNAPX_CHECK_ARG(march || a.merged, "perturbation path: tile without a merged face pass");
-  }
-  a.dtp = dtp;
-  size_t smem = march ? march_smem_bytes(a) : rhs_smem_bytes(a);
-  SNAPX_CHECK_ARG(smem <= 64 * 1024, "fused path: tile needs more than 64 KB LDS");
-  int nblk = a.nt1 * a.nt2 * a.nt3;
+  RhsPlan rp;
+  if (int e = rhs_plan(cfg, dt, bg, dtp, &rp)) return e;
+  const bool sph = rp.sph, gno = rp.gno;
   ColArgs c;
   if (int e = make_col_args(cfg, dt, &c)) return e;
-  c.bgd = a.bgd;
-  c.bgt = a.bgt;
+  c.bgd = rp.a.bgd;
+  c.bgt = rp.a.bgt;
   c.dtp = dtp;
   int N = ((cfg->implicit_scheme >> 3) & 1) ? 5 : 3;
   int cblk = (c.ncol + kColThreads - 1) / kColThreads;
@@ -2654,22 +2739,7 @@ static int fused_step(const snapx_config* cfg, double dt, float* u, void* worksp
   for (int s = 0; s < 3; ++s) {
     hipEvent_t ra = ev ? ev[2 * s].a : nullptr, rb = ev ? ev[2 * s].b : nullptr;
     roctxRangePushA(kRhsRange[s]);
-    if (sph)
-      hipExtLaunchKernelGGL(gno ? fused_rhs_gno_kernel : fused_rhs_sph_kernel, dim3(nblk),
-                            dim3(kRhsThreads), (uint32_t)smem, st, ra, rb, 0u, (const float*)u,
-                            du, a);
-    else if (march)
-      hipExtLaunchKernelGGL(march_kernel_for(cfg, bg != nullptr), dim3(nblk), dim3(kRhsThreads),
-                            (uint32_t)smem, st, ra, rb, 0u, (const float*)u, du, a);
-    else if (bg)
-      hipExtLaunchKernelGGL(fused_rhs_bg_kernel, dim3(nblk), dim3(kRhsThreads), (uint32_t)smem,
-                            st, ra, rb, 0u, (const float*)u, du, a);
-    else if (a.merged)
-      hipExtLaunchKernelGGL(fused_rhs_kernel<1>, dim3(nblk), dim3(kRhsThreads), (uint32_t)smem, st,
-                            ra, rb, 0u, (const float*)u, du, a);
-    else
-      hipExtLaunchKernelGGL(fused_rhs_kernel<0>, dim3(nblk), dim3(kRhsThreads), (uint32_t)smem, st,
-                            ra, rb, 0u, (const float*)u, du, a);
+    rhs_launch(rp, (const float*)u, du, st, ra, rb);
     roctxRangePop();
     SNAPX_LAUNCH_CHECK();
     TraceRange col_range(kColRange[s]);

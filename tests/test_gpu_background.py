@@ -104,8 +104,8 @@ def test_box3d_background_path_vs_oracle(riemann):
 
 
 def test_marching_background_path_vs_oracle():
-    """x3-marching K1 (512 or more slab workgroups) with two rows per lane in
-    the column solve: 70 x 65 x 460 cells, one step."""
+    """x3-marching K1 (16-slice slabs, 783 workgroups) with two rows per lane
+    in the column solve: 70 x 65 x 460 cells, one step."""
     oc, w, u = _box3d(n=(70, 65, 460))
     oc.x1max, oc.x2max, oc.x3max = 7000.0, 6500.0, 46000.0
     w, u = O.initialize(oc, w)

@@ -132,11 +132,14 @@ def test_batch_mixed_column_heights_exact():
         assert torch.equal(b.member(u, m), single), m
 
 
-@pytest.mark.parametrize("n", [(37, 21, 45), (33, 65, 460)], ids=["tiled", "marching"])
+@pytest.mark.parametrize("n", [(37, 21, 45), (33, 65, 460), (33, 65, 130)],
+                         ids=["tiled", "marching", "marching4"])
 def test_fused_3d_marching_rhs_matches_tiled_rhs(n):
     """3-D blocks: the single-block path's RHS kernel (merged 16x4x2 tiles for
-    the small block, the x3-marching kernel once it has 512 workgroups or
-    more) and the batch path's tiled RHS kernel perform the same operations in
+    the small block with 72 (tile, 4-slice slab) workgroups, the x3-marching
+    kernel with 8-slice slabs for 33 x 65 x 460 (1044 workgroups) and with
+    4-slice slabs for 33 x 65 x 130 (594)) and the batch path's tiled RHS
+    kernel perform the same operations in
     the same order; the compiler contracts a few multiply-adds differently in
     the kernels, so they agree to FP32 rounding (not bit for bit), and both
     meet the oracle's FP32 bar.  Ragged tiles in every direction, reflecting
