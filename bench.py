@@ -300,7 +300,7 @@ def c5_sensitivity(capi, dev, nsteps=200, every=20, ndir=2, wind=(0.0, 0.0), ric
     torch.cuda.synchronize()
     t_grad = time.perf_counter() - t0
     t_back = time.perf_counter() - t1
-    rels = []
+    rels, fds, errs = [], [], []
     for k in range(ndir):
         d = torch.randn(cshape, generator=gen, dtype=torch.float64).to(dev)
 
@@ -313,6 +313,8 @@ def c5_sensitivity(capi, dev, nsteps=200, every=20, ndir=2, wind=(0.0, 0.0), ric
         fd = (4 * central(eps) - central(2 * eps)) / 3 if richardson else central(eps)
         ad = float((grad * d).sum())
         rels.append(abs(ad - fd) / abs(fd))
+        fds.append(fd)
+        errs.append(abs(ad - fd))
         if ndir > 4 and (k + 1) % 8 == 0:
             print(f"c5 finite differences: {k + 1}/{ndir} directions, max rel err "
                   f"{max(rels):.3e}", file=sys.stderr, flush=True)
@@ -329,6 +331,17 @@ def c5_sensitivity(capi, dev, nsteps=200, every=20, ndir=2, wind=(0.0, 0.0), ric
            "fd_directions": ndir,
            "fd_rel_err_max": max(rels) if rels else None,
            "fd_rel_err_median": sorted(rels)[len(rels) // 2] if rels else None}
+    if rels:
+        # g.d over random Gaussian directions is itself Gaussian, so among
+        # many directions a few have a small directional derivative by
+        # cancellation, and their relative error is large at the same
+        # absolute error: also the error against the RMS directional
+        # derivative of the sampled directions
+        rms = (sum(f * f for f in fds) / len(fds)) ** 0.5
+        out["fd_err_over_rms_fd_max"] = max(errs) / rms
+        out["fd_err_over_rms_fd_median"] = sorted(errs)[len(errs) // 2] / rms
+        out["fd_min_abs_fd_over_rms"] = min(abs(f) for f in fds) / rms
+        out["fd_abs_fd_over_rms_at_max_rel_err"] = abs(fds[rels.index(max(rels))]) / rms
     return (out, grad) if return_grad else out
 
 

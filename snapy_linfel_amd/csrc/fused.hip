@@ -1039,7 +1039,7 @@ __global__ void __launch_bounds__(kRhsThreads, 4)
   const int sol2 = P::r2 >= 0 ? P::r2 : a.riemann[1];
   const int sol3 = P::r3 >= 0 ? P::r3 : a.riemann[2];
   const int nf0 = (e1 + 1) * e2, ftot = nf0 + e1 * (e2 + 1);
-  float* fs = smem + 2 * NP * HS;  // x1 then x2 face fluxes of one slice [5][ftot]
+  float* fs = smem + NP * HS;  // x1 then x2 face fluxes of one slice [5][ftot]
   // perturbation path: the tables of the tile's H1 levels and tx1 + 1 x1
   // faces, copied to LDS after the face fluxes
   BgView BV;
@@ -1068,7 +1068,10 @@ __global__ void __launch_bounds__(kRhsThreads, 4)
   const int own_pq = (o2 + cy + a.ng) * a.n1 + o1 + cx + a.ng;
 
   for (int kk = o3 - 3; kk <= o3 + ez + 2; ++kk) {
-    float* ws = smem + ((kk - o3 + 3) & 1) * NP * HS;
+    // one plane buffer: a slice's plane is read only between the barrier
+    // after its load and the barrier after its faces, so no thread writes
+    // the next plane while another still reads this one
+    float* ws = smem;
     bool f3;
     const int s0 = slice_start(a, kk, f3);
     if (kk >= o3 && kk < o3 + ez) {
@@ -2489,10 +2492,10 @@ static size_t march_smem_bytes(const FusedArgs& a) {
   size_t ftot = (size_t)(a.tx1 + 1) * a.tx2 + (size_t)a.tx1 * (a.tx2 + 1);
   // perturbation path: a sixth plane per slice and the tile's background
   // tables (5 per level, 10 per x1 face)
-  const size_t bgt = a.bgt ? 2 * HS + kBgLevelRows * (size_t)(a.tx1 + 6) +
+  const size_t bgt = a.bgt ? HS + kBgLevelRows * (size_t)(a.tx1 + 6) +
                                  kBgFaceRows * (size_t)(a.tx1 + 1)
                            : 0;
-  return sizeof(float) * (10 * HS + 5 * ftot + bgt);
+  return sizeof(float) * (5 * HS + 5 * ftot + bgt);
 }
 
 static int make_march_args(const snapx_config* cfg, double dt, FusedArgs* out) {
