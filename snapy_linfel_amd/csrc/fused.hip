@@ -1053,7 +1053,7 @@ __global__ void __launch_bounds__(kRhsThreads, 4)
   const float dt = step_dt(a);
 
   float win[5][6];   // own column, primitives of slices kk - 5 .. kk
-  float part[4][5];  // x1 + x2 flux differences of slices kk .. kk - 3
+  float part[3][5];  // x1 + x2 flux differences of slices kk - 1 .. kk - 3
   float f3lo[5];     // x3 flux at the lower face of the cell finished next
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
@@ -1061,7 +1061,7 @@ __global__ void __launch_bounds__(kRhsThreads, 4)
 #pragma unroll
     for (int q = 0; q < 6; ++q) win[v][q] = 1.f;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) part[q][v] = 0.f;
+    for (int q = 0; q < 3; ++q) part[q][v] = 0.f;
   }
 
   // the own cell's offset in its plane (interior in x1 and x2)
@@ -1191,6 +1191,44 @@ __global__ void __launch_bounds__(kRhsThreads, 4)
       for (int v = 0; v < 5; ++v) f3hi[vmap[v]] = flx[v];
     }
 
+    // ---- finish cell kk - 3 before the faces of slice kk (its x1 + x2
+    //      part is the oldest in the ring): x3 flux difference, du = -dt D
+    //      + gravity ----
+    const int cz = kk - 3;
+    if (own && cz >= o3 && cz < o3 + ez) {
+      float d5[5];
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        float acc = part[2][v];
+        acc += area3 * (f3hi[v] - f3lo[v]);
+        d5[v] = -dt * (acc / vol);
+      }
+      float rho = win[IDN][2];
+      if (kBg) {
+        // the background's flux divergence and gravity are the residual
+        const int lv = o1 + cx + a.ng;
+        d5[IVX] += dt * (rho * a.grav1 + BV.lv(BG_RES, lv));
+        rho += BV.lv(BG_RHO_C, lv);
+        d5[IPR] += dt * rho * win[IVX][2] * a.grav1;
+      } else if (a.grav1 != 0.f) {
+        d5[IVX] += dt * rho * a.grav1;
+        d5[IPR] += dt * rho * win[IVX][2] * a.grav1;
+      }
+      if (a.grav2 != 0.f) {
+        d5[IVY] += dt * rho * a.grav2;
+        d5[IPR] += dt * rho * win[IVY][2] * a.grav2;
+      }
+      if (a.grav3 != 0.f) {
+        d5[IVZ] += dt * rho * a.grav3;
+        d5[IPR] += dt * rho * win[IVZ][2] * a.grav3;
+      }
+      const int q = ((cz + a.ng) * a.n2 + o2 + cy + a.ng) * a.n1 + o1 + cx + a.ng;
+#pragma unroll
+      for (int v = 0; v < 5; ++v) du[v * ncell + q] = d5[v];
+    }
+#pragma unroll
+    for (int v = 0; v < 5; ++v) f3lo[v] = f3hi[v];
+
     // ---- x1 and x2 faces of slice kk, then their flux differences ----
     float pn[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
     if (kk >= o3 && kk < o3 + ez) {
@@ -1255,47 +1293,11 @@ __global__ void __launch_bounds__(kRhsThreads, 4)
     }
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
-      part[3][v] = part[2][v];
       part[2][v] = part[1][v];
       part[1][v] = part[0][v];
       part[0][v] = pn[v];
     }
 
-    // ---- finish cell kk - 3: x3 flux difference, du = -dt D + gravity ----
-    const int cz = kk - 3;
-    if (own && cz >= o3 && cz < o3 + ez) {
-      float d5[5];
-#pragma unroll
-      for (int v = 0; v < 5; ++v) {
-        float acc = part[3][v];
-        acc += area3 * (f3hi[v] - f3lo[v]);
-        d5[v] = -dt * (acc / vol);
-      }
-      float rho = win[IDN][2];
-      if (kBg) {
-        // the background's flux divergence and gravity are the residual
-        const int lv = o1 + cx + a.ng;
-        d5[IVX] += dt * (rho * a.grav1 + BV.lv(BG_RES, lv));
-        rho += BV.lv(BG_RHO_C, lv);
-        d5[IPR] += dt * rho * win[IVX][2] * a.grav1;
-      } else if (a.grav1 != 0.f) {
-        d5[IVX] += dt * rho * a.grav1;
-        d5[IPR] += dt * rho * win[IVX][2] * a.grav1;
-      }
-      if (a.grav2 != 0.f) {
-        d5[IVY] += dt * rho * a.grav2;
-        d5[IPR] += dt * rho * win[IVY][2] * a.grav2;
-      }
-      if (a.grav3 != 0.f) {
-        d5[IVZ] += dt * rho * a.grav3;
-        d5[IPR] += dt * rho * win[IVZ][2] * a.grav3;
-      }
-      const int q = ((cz + a.ng) * a.n2 + o2 + cy + a.ng) * a.n1 + o1 + cx + a.ng;
-#pragma unroll
-      for (int v = 0; v < 5; ++v) du[v * ncell + q] = d5[v];
-    }
-#pragma unroll
-    for (int v = 0; v < 5; ++v) f3lo[v] = f3hi[v];
   }
 }
 
