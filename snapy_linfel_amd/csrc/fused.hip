@@ -517,6 +517,81 @@ __device__ __forceinline__ void sph_cell_du(const FusedArgs& a, const float* ws,
   }
 }
 
+// Face options fixed at compile time (-1: read from the arguments at run
+// time): the density interpolant and the interpolant of velocities and
+// pressure for x1 (kTy0, kTy20) and for x2 / x3 (kTy1, kTy21), and the
+// Riemann solver per direction.  A fixed instance has the scale option off.
+// The instances of the configurations in use compile one interpolant pair and
+// one solver per direction, without the run-time dispatch (the marching K1
+// and rhs_tile).
+template <int kTy0, int kTy20, int kTy1, int kTy21, int kR1, int kR2, int kR3>
+struct MarchFaces {
+  static constexpr int ty0 = kTy0, ty20 = kTy20, ty1 = kTy1, ty21 = kTy21;
+  static constexpr int r1 = kR1, r2 = kR2, r3 = kR3;
+};
+using MarchGeneric = MarchFaces<-1, -1, -1, -1, -1, -1, -1>;
+// the compiled instances: WENO5 density with CP5 velocities and pressure on
+// both axis groups, with LMARS in x1 and Roe in x2 / x3, or one solver in
+// every direction
+using FacesLmRoRo = MarchFaces<SNAPX_INTERP_WENO5, SNAPX_INTERP_CP5, SNAPX_INTERP_WENO5,
+                               SNAPX_INTERP_CP5, SNAPX_RIEMANN_LMARS, SNAPX_RIEMANN_ROE,
+                               SNAPX_RIEMANN_ROE>;
+using FacesLm = MarchFaces<SNAPX_INTERP_WENO5, SNAPX_INTERP_CP5, SNAPX_INTERP_WENO5,
+                           SNAPX_INTERP_CP5, SNAPX_RIEMANN_LMARS, SNAPX_RIEMANN_LMARS,
+                           SNAPX_RIEMANN_LMARS>;
+using FacesRo = MarchFaces<SNAPX_INTERP_WENO5, SNAPX_INTERP_CP5, SNAPX_INTERP_WENO5,
+                           SNAPX_INTERP_CP5, SNAPX_RIEMANN_ROE, SNAPX_RIEMANN_ROE,
+                           SNAPX_RIEMANN_ROE>;
+using FacesHl = MarchFaces<SNAPX_INTERP_WENO5, SNAPX_INTERP_CP5, SNAPX_INTERP_WENO5,
+                           SNAPX_INTERP_CP5, SNAPX_RIEMANN_HLLC, SNAPX_RIEMANN_HLLC,
+                           SNAPX_RIEMANN_HLLC>;
+enum { kFacesGeneric = 0, kFacesLmRoRo, kFacesLm, kFacesRo, kFacesHl };
+
+// The compiled face-option instance that matches a block (kFacesGeneric:
+// none; the solvers of inactive directions do not matter).
+static int face_policy(const snapx_config* c) {
+  constexpr int W5 = SNAPX_INTERP_WENO5;
+  constexpr int RO = SNAPX_RIEMANN_ROE, LM = SNAPX_RIEMANN_LMARS, HL = SNAPX_RIEMANN_HLLC;
+  const bool weno5_cp5 = c->recon_type[0] == W5 && c->recon_type[1] == W5 && !c->recon_shock[0] &&
+                         !c->recon_shock[1] && !c->recon_scale[0] && !c->recon_scale[1];
+  if (!weno5_cp5) return kFacesGeneric;
+  const int r1 = c->riemann[0];
+  const int r2 = c->nx2 > 1 ? c->riemann[1] : r1;
+  const int r3 = c->nx3 > 1 ? c->riemann[2] : r2;
+  if (r1 == LM && r2 == RO && r3 == RO) return kFacesLmRoRo;
+  if (r1 == LM && r2 == LM && r3 == LM) return kFacesLm;
+  if (r1 == RO && r2 == RO && r3 == RO) return kFacesRo;
+  if (r1 == HL && r2 == HL && r3 == HL) return kFacesHl;
+  return kFacesGeneric;
+}
+
+
+// The face options of direction d (row r = 0 for x1, 1 for x2 / x3): the
+// constants of P where it fixes them, else the argument fields passed in
+// (by value: a reference to the by-value argument struct would place it in
+// scratch memory).
+template <class P>
+__device__ __forceinline__ int face_type(int r, int t0, int t1) {
+  if (P::ty0 >= 0 && P::ty1 >= 0) return r ? P::ty1 : P::ty0;
+  return r ? t1 : t0;
+}
+template <class P>
+__device__ __forceinline__ int face_scale(int r, int s0, int s1) {
+  if (P::ty0 >= 0 && P::ty1 >= 0) return 0;
+  return r ? s1 : s0;
+}
+template <class P>
+__device__ __forceinline__ int face_type2(int r, int type, int shock0, int shock1) {
+  if (P::ty20 >= 0 && P::ty21 >= 0) return r ? P::ty21 : P::ty20;
+  const int shock = r ? shock1 : shock0;
+  return shock ? type : interp2_type(type);
+}
+template <class P>
+__device__ __forceinline__ int face_solver(int d, int r0, int r1, int r2) {
+  if (P::r1 >= 0 && P::r2 >= 0 && P::r3 >= 0) return d == 0 ? P::r1 : (d == 1 ? P::r2 : P::r3);
+  return d == 0 ? r0 : (d == 1 ? r1 : r2);
+}
+
 // K1 body for one tile; every thread of the workgroup calls it with the same
 // tile (it synchronises the workgroup internally).  The arguments come by
 // value: a reference to the kernel argument makes the compiler keep a copy
@@ -536,7 +611,7 @@ __device__ __forceinline__ void sph_cell_du(const FusedArgs& a, const float* ws,
 template <int kSph>
 using RhsArgs = std::conditional_t<kSph != 0, const FusedArgs&, const FusedArgs>;
 
-template <int kMerged, int kSph = 0, bool kBg = false>
+template <int kMerged, int kSph = 0, bool kBg = false, class P = MarchGeneric>
 __device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __restrict__ du,
                                          RhsArgs<kSph> a, int b, float* smem, float dt) {
   static_assert(!kBg || (kMerged == 1 && !kSph), "background path: merged Cartesian tiles");
@@ -683,12 +758,11 @@ __device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __r
       const int sd = d == 0 ? 1 : (d == 1 ? H1 : H1 * H2);  // LDS stride along d
       const int r = d == 0 ? 0 : 1;
       // selects, not a runtime index into the argument struct (which would
-      // place it in scratch memory)
-      const int type = r ? a.recon_type[1] : a.recon_type[0];
-      const int scale = r ? a.recon_scale[1] : a.recon_scale[0];
-      const int shock = r ? a.recon_shock[1] : a.recon_shock[0];
-      const int type2 = shock ? type : interp2_type(type);
-      const int solver = d == 0 ? a.riemann[0] : (d == 1 ? a.riemann[1] : a.riemann[2]);
+      // place it in scratch memory); the options P fixes are constants
+      const int type = face_type<P>(r, a.recon_type[0], a.recon_type[1]);
+      const int scale = face_scale<P>(r, a.recon_scale[0], a.recon_scale[1]);
+      const int type2 = face_type2<P>(r, type, a.recon_shock[0], a.recon_shock[1]);
+      const int solver = face_solver<P>(d, a.riemann[0], a.riemann[1], a.riemann[2]);
       int fx, fy, fz;
       if (d == 0) {
         fz = de2.div(dq1.div(f, fx), fy);
@@ -825,12 +899,11 @@ __device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __r
       int sd = d == 0 ? 1 : (d == 1 ? H1 : H1 * H2);  // LDS stride along d
       int r = d == 0 ? 0 : 1;
       // selects, not a runtime index into the argument struct (which would
-      // place it in scratch memory)
-      int type = r ? a.recon_type[1] : a.recon_type[0];
-      int scale = r ? a.recon_scale[1] : a.recon_scale[0];
-      int shock = r ? a.recon_shock[1] : a.recon_shock[0];
-      int type2 = shock ? type : interp2_type(type);
-      int solver = d == 0 ? a.riemann[0] : (d == 1 ? a.riemann[1] : a.riemann[2]);
+      // place it in scratch memory); the options P fixes are constants
+      const int type = face_type<P>(r, a.recon_type[0], a.recon_type[1]);
+      const int scale = face_scale<P>(r, a.recon_scale[0], a.recon_scale[1]);
+      const int type2 = face_type2<P>(r, type, a.recon_shock[0], a.recon_shock[1]);
+      const int solver = face_solver<P>(d, a.riemann[0], a.riemann[1], a.riemann[2]);
       const DivMod df1(f1), df2(f2);
       for (int f = threadIdx.x; f < nf; f += blockDim.x) {
         int fx, fy;
@@ -927,18 +1000,19 @@ __device__ __forceinline__ void rhs_tile(const float* __restrict__ u, float* __r
   }
 }
 
-template <int kMerged>
+template <int kMerged, class P = MarchGeneric>
 __global__ void __launch_bounds__(kRhsThreads)
     fused_rhs_kernel(const float* __restrict__ u, float* __restrict__ du, FusedArgs a) {
   extern __shared__ float smem[];
-  rhs_tile<kMerged>(u, du, a, blockIdx.x, smem, step_dt(a));
+  rhs_tile<kMerged, 0, false, P>(u, du, a, blockIdx.x, smem, step_dt(a));
 }
 
 // K1 of the perturbation path (merged face pass, Cartesian)
+template <class P = MarchGeneric>
 __global__ void __launch_bounds__(kRhsThreads)
     fused_rhs_bg_kernel(const float* __restrict__ u, float* __restrict__ du, FusedArgs a) {
   extern __shared__ float smem[];
-  rhs_tile<1, 0, true>(u, du, a, blockIdx.x, smem, step_dt(a));
+  rhs_tile<1, 0, true, P>(u, du, a, blockIdx.x, smem, step_dt(a));
 }
 
 // K1 of spherical-polar blocks (merged face pass, geometric sources)
@@ -970,18 +1044,6 @@ __global__ void __launch_bounds__(kRhsThreads)
 // (four wavefronts per SIMD: 128 registers, a few of the slice window spilled;
 // 0.81 against 0.85 ms per launch at three wavefronts and no spills)
 //
-// Face options fixed at compile time (-1: read from the arguments at run
-// time): the density interpolant and the interpolant of velocities and
-// pressure for x1 (kTy0, kTy20) and for x2 / x3 (kTy1, kTy21), and the
-// Riemann solver per direction.  A fixed instance has the scale option off.
-// The instances of the configurations in use compile one interpolant pair and
-// one solver per direction, without the run-time dispatch.
-template <int kTy0, int kTy20, int kTy1, int kTy21, int kR1, int kR2, int kR3>
-struct MarchFaces {
-  static constexpr int ty0 = kTy0, ty20 = kTy20, ty1 = kTy1, ty21 = kTy21;
-  static constexpr int r1 = kR1, r2 = kR2, r3 = kR3;
-};
-using MarchGeneric = MarchFaces<-1, -1, -1, -1, -1, -1, -1>;
 
 // Interface states of one face in the face frame of direction kD (normal
 // velocity in slot IVX) from the pencil of each variable, p_v = ws + vmap[v]
@@ -2251,7 +2313,7 @@ __device__ __forceinline__ int find_member(const BatchMember* m, int nm, int ite
 
 // persistent K1 over the (member, tile) work list; kSph: every member is a
 // spherical-polar block (merged face pass with geometric sources)
-template <int kSph>
+template <int kSph, class P = MarchGeneric>
 __global__ void __launch_bounds__(kRhsThreads)
     batch_rhs_kernel(const float* __restrict__ u, float* __restrict__ du,
                      const BatchMember* __restrict__ mem, int nm, int total) {
@@ -2261,9 +2323,29 @@ __global__ void __launch_bounds__(kRhsThreads)
     // (curvilinear members read their arguments in place, Cartesian ones
     // from a copy: either way without a copy in scratch memory)
     RhsArgs<kSph> a = mem[m].a;
-    rhs_tile<kSph ? 1 : -1, kSph>(u + mem[m].off, du + mem[m].off, a, item - mem[m].tile0, smem,
-                                  a.dt);
+    rhs_tile<kSph ? 1 : -1, kSph, false, P>(u + mem[m].off, du + mem[m].off, a,
+                                            item - mem[m].tile0, smem, a.dt);
     __syncthreads();
+  }
+}
+
+using BatchRhsKernel = void (*)(const float*, float*, const BatchMember*, int, int);
+
+// K1 of a batch: the instance of the members' coordinate system, and for
+// Cartesian members the compiled face-option instance that every member
+// matches (else the generic one)
+static BatchRhsKernel batch_rhs_for(const snapx_config* members, int nm) {
+  if (members[0].coord == SNAPX_COORD_GNOMONIC_EQUIANGLE) return batch_rhs_kernel<2>;
+  if (members[0].coord != SNAPX_COORD_CARTESIAN) return batch_rhs_kernel<1>;
+  int pol = face_policy(&members[0]);
+  for (int m = 1; m < nm && pol != kFacesGeneric; ++m)
+    if (face_policy(&members[m]) != pol) pol = kFacesGeneric;
+  switch (pol) {
+    case kFacesLmRoRo: return batch_rhs_kernel<0, FacesLmRoRo>;
+    case kFacesLm: return batch_rhs_kernel<0, FacesLm>;
+    case kFacesRo: return batch_rhs_kernel<0, FacesRo>;
+    case kFacesHl: return batch_rhs_kernel<0, FacesHl>;
+    default: return batch_rhs_kernel<0>;
   }
 }
 
@@ -2531,28 +2613,38 @@ using MarchKernel = void (*)(const float*, float*, FusedArgs);
 // Roe in x2 / x3, and one solver in every direction.  Other options run the
 // generic instance.
 static MarchKernel march_kernel_for(const snapx_config* c, bool bg) {
-  constexpr int W5 = SNAPX_INTERP_WENO5, C5 = SNAPX_INTERP_CP5;
-  constexpr int RO = SNAPX_RIEMANN_ROE, LM = SNAPX_RIEMANN_LMARS, HL = SNAPX_RIEMANN_HLLC;
-  const bool weno5_cp5 = c->recon_type[0] == W5 && c->recon_type[1] == W5 && !c->recon_shock[0] &&
-                         !c->recon_shock[1] && !c->recon_scale[0] && !c->recon_scale[1];
-  if (bg) {
+  const int pol = face_policy(c);
+  if (bg)
     // perturbation path: the 3-D workload's solvers, else the generic instance
-    if (weno5_cp5 && c->riemann[0] == LM && c->riemann[1] == RO && c->riemann[2] == RO)
-      return fused_rhs_march_kernel<MarchFaces<W5, C5, W5, C5, LM, RO, RO>, true>;
-    return fused_rhs_march_kernel<MarchGeneric, true>;
+    return pol == kFacesLmRoRo ? fused_rhs_march_kernel<FacesLmRoRo, true>
+                               : fused_rhs_march_kernel<MarchGeneric, true>;
+  switch (pol) {
+    case kFacesLmRoRo: return fused_rhs_march_kernel<FacesLmRoRo>;
+    case kFacesLm: return fused_rhs_march_kernel<FacesLm>;
+    case kFacesRo: return fused_rhs_march_kernel<FacesRo>;
+    case kFacesHl: return fused_rhs_march_kernel<FacesHl>;
+    default: return fused_rhs_march_kernel<MarchGeneric>;
   }
-  if (weno5_cp5) {
-    const int r1 = c->riemann[0], r2 = c->riemann[1], r3 = c->riemann[2];
-    if (r1 == LM && r2 == RO && r3 == RO)
-      return fused_rhs_march_kernel<MarchFaces<W5, C5, W5, C5, LM, RO, RO>>;
-    if (r1 == LM && r2 == LM && r3 == LM)
-      return fused_rhs_march_kernel<MarchFaces<W5, C5, W5, C5, LM, LM, LM>>;
-    if (r1 == RO && r2 == RO && r3 == RO)
-      return fused_rhs_march_kernel<MarchFaces<W5, C5, W5, C5, RO, RO, RO>>;
-    if (r1 == HL && r2 == HL && r3 == HL)
-      return fused_rhs_march_kernel<MarchFaces<W5, C5, W5, C5, HL, HL, HL>>;
+}
+
+// The tiled K1 (merged or per-direction face passes, or the perturbation
+// path's merged tiles) for a block's face options.
+template <int kMerged>
+static MarchKernel rhs_kernel_for(int pol) {
+  switch (pol) {
+    case kFacesLmRoRo: return fused_rhs_kernel<kMerged, FacesLmRoRo>;
+    case kFacesLm: return fused_rhs_kernel<kMerged, FacesLm>;
+    case kFacesRo: return fused_rhs_kernel<kMerged, FacesRo>;
+    case kFacesHl: return fused_rhs_kernel<kMerged, FacesHl>;
+    default: return fused_rhs_kernel<kMerged>;
   }
-  return fused_rhs_march_kernel<MarchGeneric>;
+}
+static MarchKernel rhs_bg_kernel_for(int pol) {
+  switch (pol) {
+    case kFacesLmRoRo: return fused_rhs_bg_kernel<FacesLmRoRo>;
+    case kFacesLm: return fused_rhs_bg_kernel<FacesLm>;
+    default: return fused_rhs_bg_kernel<>;
+  }
 }
 
 static int make_col_args(const snapx_config* cfg, double dt, ColArgs* out) {
@@ -2682,11 +2774,14 @@ static void rhs_launch(const RhsPlan& p, const float* u, float* du, hipStream_t 
     hipExtLaunchKernelGGL(march_kernel_for(p.cfg, p.a.bgt != nullptr), grid, block, smem, st, ra,
                           rb, 0u, u, du, p.a);
   else if (p.a.bgt)
-    hipExtLaunchKernelGGL(fused_rhs_bg_kernel, grid, block, smem, st, ra, rb, 0u, u, du, p.a);
+    hipExtLaunchKernelGGL(rhs_bg_kernel_for(face_policy(p.cfg)), grid, block, smem, st, ra, rb, 0u,
+                          u, du, p.a);
   else if (p.a.merged)
-    hipExtLaunchKernelGGL(fused_rhs_kernel<1>, grid, block, smem, st, ra, rb, 0u, u, du, p.a);
+    hipExtLaunchKernelGGL(rhs_kernel_for<1>(face_policy(p.cfg)), grid, block, smem, st, ra, rb, 0u,
+                          u, du, p.a);
   else
-    hipExtLaunchKernelGGL(fused_rhs_kernel<0>, grid, block, smem, st, ra, rb, 0u, u, du, p.a);
+    hipExtLaunchKernelGGL(rhs_kernel_for<0>(face_policy(p.cfg)), grid, block, smem, st, ra, rb, 0u,
+                          u, du, p.a);
 }
 
 bool fused_rhs_ok(const snapx_config* c) { return fused_unsupported(c) == nullptr; }
@@ -3207,6 +3302,7 @@ extern "C" int snapx_batch_rk3_steps(const snapx_config* members, const snapx_ba
   double* scr = reinterpret_cast<double*>(base + batch_table_bytes(nm) + 2 * field);
   // persistent grids: two workgroups per CU for K1, enough wavefronts for K2
   int rhs_grid = std::min(info->total_tiles, 512);
+  const BatchRhsKernel rhs_kernel = batch_rhs_for(members, nm);
   int col_grid = std::min((info->total_cols + kColThreads - 1) / kColThreads, 2048);
   EventPair ev[2];
   if (timing)
@@ -3218,9 +3314,8 @@ extern "C" int snapx_batch_rk3_steps(const snapx_config* members, const snapx_ba
     TraceRange step_range("snapx batch step");
     for (int s = 0; s < 3; ++s) {
       if (timing) SNAPX_HIP(hipEventRecord(ev[0].a, st));
-      hipLaunchKernelGGL(gno ? batch_rhs_kernel<2> : (sph ? batch_rhs_kernel<1> : batch_rhs_kernel<0>),
-                         dim3(rhs_grid), dim3(kRhsThreads), info->rhs_smem, st, u, du, table, nm,
-                         info->total_tiles);
+      hipLaunchKernelGGL(rhs_kernel, dim3(rhs_grid), dim3(kRhsThreads), info->rhs_smem, st, u, du,
+                         table, nm, info->total_tiles);
       SNAPX_LAUNCH_CHECK();
       if (timing) {
         SNAPX_HIP(hipEventRecord(ev[0].b, st));
@@ -3445,9 +3540,7 @@ extern "C" int snapx_batch_autotune(const snapx_config* members, int nmembers, d
     }
     int grid = std::min(inf.total_tiles, 512);
     // K1 reads u and writes du only: the state is not changed
-    auto rhs = members[0].coord == SNAPX_COORD_GNOMONIC_EQUIANGLE ? batch_rhs_kernel<2>
-               : members[0].coord != SNAPX_COORD_CARTESIAN        ? batch_rhs_kernel<1>
-                                                                   : batch_rhs_kernel<0>;
+    auto rhs = batch_rhs_for(members, nmembers);
     hipLaunchKernelGGL(rhs, dim3(grid), dim3(kRhsThreads), inf.rhs_smem, st, u, du, table,
                        nmembers, inf.total_tiles);
     (void)hipEventRecord(e0, st);
