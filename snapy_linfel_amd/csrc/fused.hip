@@ -1859,7 +1859,8 @@ __device__ __forceinline__ void column_pcr2(float* __restrict__ u, float* __rest
   float_prim_at<kBg>(Uc0, false, a, a.ng + (act0 ? i0 : 0), w0);
   float_prim_at<kBg>(Uc1, false, a, a.ng + (act1 ? i1 : 0), w1);
   double s_a2 = 1., s_b = 0.;
-  const bool wave = half == 64;
+  // every column within one wavefront (half <= 64 lanes, aligned)
+  const bool wave = half <= 64;
   if (a.scheme) {
     double Pb0[NN], Pc0[NN], pr0[N];  // the even row, normalised
     // ---- primitives of the neighbours ----
@@ -2673,10 +2674,10 @@ static int make_col_args(const snapx_config* cfg, double dt, ColArgs* out) {
   const bool sph = cfg->coord == SNAPX_COORD_SPHERICAL_POLAR;
   c.zin1 = sph ? sph_dx1v(c.geo, g.ng - 1) : g.dx1;
   c.zout1 = sph ? sph_dx1v(c.geo, g.n1 - g.ng - 1) : g.dx1;
-  // 3x3 columns of more than 64 cells: two rows per lane (column_pcr2;
-  // Cartesian blocks)
+  // 3x3 columns of more than 32 cells: two rows per lane (column_pcr2;
+  // Cartesian blocks), 32 lanes or more per column
   c.rows2 = (cfg->coord == SNAPX_COORD_CARTESIAN && !((cfg->implicit_scheme >> 3) & 1) &&
-             c.lanes >= 128) ? 1 : 0;
+             c.nl > 32) ? 1 : 0;
   if (c.rows2) c.lanes >>= 1, c.lshift -= 1;
   SNAPX_CHECK_ARG(!c.scheme || c.grav1 != 0.,
                   "[ImplicitHydro] forcing does not have const-gravity");

@@ -134,9 +134,12 @@ __device__ __forceinline__ void column_barrier(bool wave) {
   }
 }
 
-__device__ __forceinline__ double wave_sum(double x) {
+// Sum over the aligned groups of `width` lanes (a power of two, at most 64)
+// of a wavefront.
+__device__ __forceinline__ double wave_sum(double x, int width = 64) {
 #pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) x += __shfl_xor(x, m);
+  for (int m = 32; m >= 1; m >>= 1)
+    if (m < width) x += __shfl_xor(x, m);
   return x;
 }
 
@@ -270,16 +273,17 @@ __device__ __forceinline__ void pcr_solve(double* A, double* B, double* C, doubl
   }
 }
 
-// Sum of x over the lanes of each column (columns of L = 2^lshift >= 64
-// lanes, whole wavefronts): wave reductions, then, for taller columns, the
-// per-wavefront partials through sh[] in a fixed order.  Every thread calls
-// it; sh must not be in use by other lanes (a barrier precedes the partial
-// writes).  Deterministic.
+// Sum of x over the lanes of each column (columns of L = 2^lshift >= 32
+// lanes: aligned parts of a wavefront or whole wavefronts): wave reductions,
+// then, for columns taller than a wavefront, the per-wavefront partials
+// through sh[] in a fixed order.  Every thread calls it; sh must not be in
+// use by other lanes (a barrier precedes the partial writes).  Deterministic.
 template <int K>
 __device__ __forceinline__ void column_sums(double* x, double* __restrict__ sh, int t,
                                             int lanes, int lshift) {
+  const int width = lanes < 64 ? lanes : 64;
 #pragma unroll
-  for (int k = 0; k < K; ++k) x[k] = wave_sum(x[k]);
+  for (int k = 0; k < K; ++k) x[k] = wave_sum(x[k], width);
   if (lanes > 64) {
     const int wv = t >> 6, nwv = lanes >> 6;
     __syncthreads();
