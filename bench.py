@@ -722,7 +722,7 @@ def main():
         ms0 = us0[0][Is].sum()
         n2 = cs.n_interior
         ks1 = ("fused_rhs_kernel", rs["rhs_ms"], 2 * NV * 4 * n2)
-        ks2 = ("fused_column_pcr_kernel<3>", rs["col_ms"], 4 * NV * 4 * n2)
+        ks2 = ("fused_column_pcr2_kernel", rs["col_ms"], 4 * NV * 4 * n2)
         doms = ks1 if ks1[1] >= ks2[1] else ks2
         extra["straka_2d_512x64_fp32"] = {
             "cell_updates_per_s": round(n2 * 1000 / rs["t32"], 1), "steps": 1000, "dt_s": 0.1,
