@@ -1033,16 +1033,16 @@ __global__ void __launch_bounds__(kRhsThreads)
 //
 // For 3-D blocks the x1-x2 tile (tx1 x tx2 <= 256 cells, one per thread)
 // marches along x3 through a slab of tx3 slices.  Per slice the halo'd x1-x2
-// plane of primitives goes to LDS once (two planes, alternating), the x1 and
-// x2 faces of the slice are computed from LDS, and every thread keeps its own
-// column's last six slices in registers for the x3 faces.  Halo loads per
+// plane of primitives goes to LDS once (one buffer), the x1 and x2 faces of
+// the slice are computed from LDS, and every thread keeps its own column's
+// last six slices in registers for the x3 faces; the three halo slices at
+// each end of the slab load only the thread's own cell.  Halo loads per
 // interior cell fall from 8.6 (16 x 4 x 4 tiles of rhs_tile) to
-// (tx1 + 6)(tx2 + 6)(tx3 + 6) / (tx1 tx2 tx3), 2.9 for 32 x 8 x 16.  The
-// operations (interface states, fluxes, the order of the flux differences,
-// the ghost refill) are those of rhs_tile; the two agree to FP32 rounding
-// (the compiler contracts a few multiply-adds differently).
-// (four wavefronts per SIMD: 128 registers, a few of the slice window spilled;
-// 0.81 against 0.85 ms per launch at three wavefronts and no spills)
+// ((tx1 + 6)(tx2 + 6) tx3 + 6 tx1 tx2) / (tx1 tx2 tx3), 2.45 for 32 x 8 x 16.
+// The operations (interface states, fluxes, the order of the flux
+// differences, the ghost refill) are those of rhs_tile; the two agree to FP32
+// rounding (the compiler contracts a few multiply-adds differently).
+// (four wavefronts per SIMD: 128 registers, a few spilled)
 //
 
 // Interface states of one face in the face frame of direction kD (normal
