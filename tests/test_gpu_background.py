@@ -103,11 +103,14 @@ def test_box3d_background_path_vs_oracle(riemann):
     assert (own < 1e-5).all(), own
 
 
-def test_marching_background_path_vs_oracle():
-    """x3-marching K1 (16-slice slabs, 783 workgroups) with two rows per lane
-    in the column solve: 70 x 65 x 460 cells, one step."""
-    oc, w, u = _box3d(n=(70, 65, 460))
-    oc.x1max, oc.x2max, oc.x3max = 7000.0, 6500.0, 46000.0
+@pytest.mark.parametrize("n", [(70, 65, 460), (33, 65, 130)], ids=["slab16", "slab4"])
+def test_marching_background_path_vs_oracle(n):
+    """x3-marching K1 on the perturbation path with two rows per lane in the
+    column solve, one step: 70 x 65 x 460 cells (16-slice slabs, 783
+    workgroups, 70-cell columns) and 33 x 65 x 130 cells (ragged 4-slice
+    slabs, 594 workgroups, 33-cell columns of 17 lanes)."""
+    oc, w, u = _box3d(n=n)
+    oc.x1max, oc.x2max, oc.x3max = 100.0 * n[0], 100.0 * n[1], 100.0 * n[2]
     w, u = O.initialize(oc, w)
     rb, pb = _box_background(oc)
     dt = 0.2

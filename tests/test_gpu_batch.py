@@ -111,11 +111,12 @@ def test_fused_graph_replays_match_eager():
 
 
 def test_batch_mixed_column_heights_exact():
-    """Members whose columns take one wavefront, two and four (64-, 128- and
-    256-lane column groups in one launch) reproduce the single-block path
-    exactly."""
+    """Members of every column form in one launch reproduce the single-block
+    path exactly: 20-cell columns (one row per lane, 64 lanes), 40- and
+    64-cell columns (two rows per lane, 32-lane groups, two columns per
+    wavefront), 90 and 150 cells (two rows per lane, 64 and 128 lanes)."""
     mem = []
-    for nx2, nx1 in ((12, 20), (8, 90), (6, 150), (10, 64)):
+    for nx2, nx1 in ((12, 20), (8, 90), (6, 150), (10, 64), (9, 40)):
         oc = O.straka_config(nx2=nx2, nx1=nx1, implicit_scheme=1, riemann="hllc")
         oc.x2max = 50.0 * nx2
         oc.x1max = 50.0 * nx1
