@@ -46,6 +46,9 @@ struct AdjArgs {
   int bc[6];
   int scheme;
   int lanes, lshift;  // cell-parallel column kernels: lanes per column, log2
+  // 3x3 columns of 33 .. 256 cells: two rows per lane, `half` lanes per
+  // column (log2 hshift); 0 for one row per lane
+  int half, hshift;
   // spherical-polar blocks: per-cell areas, volumes and source factors
   // (sph_cell), per-layer ghost distances (sph_dx1v)
   int sph;
@@ -530,6 +533,244 @@ __global__ void __launch_bounds__(PcrShape<N>::kThreads)
     lam_dx[IVY * ncell + L.q] = y[2] / a.dt;
     lam_dx[IVZ * ncell + L.q] = y[3] / a.dt;
     lam_dx[IPR * ncell + L.q] = y[4] / a.dt;
+  }
+}
+
+// Solves M x = rhs (transpose false) or M^T x = rhs (transpose true) for the
+// rows 2m and 2m + 1 of column col (3 x 3 blocks of the tape) on lane m of
+// the column's `half` lanes: one cyclic-reduction level inside the lane
+// removes the even rows (the odd row eliminates the even row below from its
+// own registers and the one above from the next lane through LDS), PCR
+// solves the odd rows, and the even rows follow by back substitution.
+// Inactive rows are identity rows with a zero right-hand side.  Every lane
+// of the workgroup calls it.
+__device__ __forceinline__ void column_solve2(const double* __restrict__ abc, int64_t col, int m,
+                                              int nl, int half, bool transpose, bool act0,
+                                              bool act1, const double* r0, const double* r1,
+                                              double* x0, double* x1, double* __restrict__ sh,
+                                              int T, int t, bool wave) {
+  constexpr int N = 3, NN = 9;
+  double Pb0[NN], Pc0[NN], pr0[N];  // the even row, normalised
+  {
+    double A[NN], B[NN], C[NN], rr[N], inv[NN];
+    if (act0) {
+      vic_row_blocks<N>(abc, col, 2 * m, nl, transpose, A, B, C);
+    } else {
+#pragma unroll
+      for (int e = 0; e < NN; ++e) {
+        A[e] = (e % (N + 1)) == 0 ? 1. : 0.;
+        B[e] = C[e] = 0.;
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < N; ++e) rr[e] = act0 ? r0[e] : 0.;
+    inverse3<double, false>(A, inv);
+    matmul<double, N>(inv, B, Pb0);
+    matmul<double, N>(inv, C, Pc0);
+    matvec<double, N>(inv, rr, pr0);
+  }
+#pragma unroll
+  for (int e = 0; e < NN; ++e) {
+    sh[e * T + t] = Pb0[e];
+    sh[(NN + e) * T + t] = Pc0[e];
+  }
+#pragma unroll
+  for (int e = 0; e < N; ++e) sh[(2 * NN + e) * T + t] = pr0[e];
+  // the odd row
+  double A1[NN], B1[NN], C1[NN], q1[N];
+  if (act1) {
+    vic_row_blocks<N>(abc, col, 2 * m + 1, nl, transpose, A1, B1, C1);
+  } else {
+#pragma unroll
+    for (int e = 0; e < NN; ++e) {
+      A1[e] = (e % (N + 1)) == 0 ? 1. : 0.;
+      B1[e] = C1[e] = 0.;
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < N; ++e) q1[e] = act1 ? r1[e] : 0.;
+  column_barrier(wave);
+  {
+    double nB[NN], nC[NN], y[NN];
+    // the even row below (own registers) couples the odd row to row 2m - 1
+    matmul<double, N>(B1, Pb0, y);
+#pragma unroll
+    for (int e = 0; e < NN; ++e) nB[e] = -y[e];
+    matmul<double, N>(B1, Pc0, y);
+#pragma unroll
+    for (int e = 0; e < NN; ++e) A1[e] -= y[e];
+    matvec<double, N>(B1, pr0, y);
+#pragma unroll
+    for (int e = 0; e < N; ++e) q1[e] -= y[e];
+    // the even row above (next lane) couples it to row 2m + 3
+#pragma unroll
+    for (int e = 0; e < NN; ++e) nC[e] = 0.;
+    if (m + 1 < half) {
+      double Pb[NN], Pc[NN], pr[N];
+#pragma unroll
+      for (int e = 0; e < NN; ++e) {
+        Pb[e] = sh[e * T + t + 1];
+        Pc[e] = sh[(NN + e) * T + t + 1];
+      }
+#pragma unroll
+      for (int e = 0; e < N; ++e) pr[e] = sh[(2 * NN + e) * T + t + 1];
+      matmul<double, N>(C1, Pc, y);
+#pragma unroll
+      for (int e = 0; e < NN; ++e) nC[e] = -y[e];
+      matmul<double, N>(C1, Pb, y);
+#pragma unroll
+      for (int e = 0; e < NN; ++e) A1[e] -= y[e];
+      matvec<double, N>(C1, pr, y);
+#pragma unroll
+      for (int e = 0; e < N; ++e) q1[e] -= y[e];
+    }
+#pragma unroll
+    for (int e = 0; e < NN; ++e) {
+      B1[e] = nB[e];
+      C1[e] = nC[e];
+    }
+  }
+  column_barrier(wave);
+  pcr_solve<N>(A1, B1, C1, q1, x1, sh, T, t, m, nl / 2, act1, wave);
+  // the even rows by back substitution
+#pragma unroll
+  for (int e = 0; e < N; ++e) sh[e * T + t] = x1[e];
+  column_barrier(wave);
+  double xb[N];
+#pragma unroll
+  for (int e = 0; e < N; ++e) xb[e] = m > 0 ? sh[e * T + t - 1] : 0.;
+#pragma unroll
+  for (int rr = 0; rr < N; ++rr) {
+    double v = pr0[rr];
+#pragma unroll
+    for (int c = 0; c < N; ++c) v -= Pb0[rr * N + c] * xb[c] + Pc0[rr * N + c] * x1[c];
+    x0[rr] = v;
+  }
+  column_barrier(wave);
+}
+
+// Lane geometry of the two-rows-per-lane column kernels: lane m of column
+// col serves layers 2m and 2m + 1.
+struct ColLane2 {
+  int col, m, qb;
+  bool act0, act1;
+};
+
+__device__ __forceinline__ ColLane2 col_lane2(const AdjArgs& a, int ncol) {
+  ColLane2 L;
+  L.col = blockIdx.x * (blockDim.x >> a.hshift) + (threadIdx.x >> a.hshift);
+  L.m = threadIdx.x & (a.half - 1);
+  const bool colok = L.col < ncol;
+  L.act0 = colok && 2 * L.m < a.nx1;
+  L.act1 = colok && 2 * L.m + 1 < a.nx1;
+  L.qb = colok ? col_q0(a, L.col) : 0;
+  return L;
+}
+
+// vic_fwd_solve_kernel with two rows per lane (3x3 blocks)
+template <typename S, bool kSph>
+__global__ void __launch_bounds__(PcrShape<3>::kThreads, 2)
+    vic_fwd_solve2_kernel(const S* __restrict__ w, const S* __restrict__ dux,
+                          const double* __restrict__ abc, S* __restrict__ du,
+                          double* __restrict__ dl, AdjArgs a, int ncol) {
+  extern __shared__ double sh[];
+  constexpr int N = 3;
+  const int ncell = a.n1 * a.n2 * a.n3, nl = a.nx1;
+  const ColLane2 L = col_lane2(a, ncol);
+  const bool act[2] = {L.act0, L.act1};
+  double r[2][N], x[2][N], vol[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int i = 2 * L.m + h;
+    vol[h] = !kSph ? a.dx3 * (a.dx2 * a.dx1) : (act[h] ? sph_layer(a, L.col, i).vol : 1.);
+#pragma unroll
+    for (int e = 0; e < N; ++e) r[h][e] = 0.;
+    if (act[h]) {
+      vic_pick<N>(dux, ncell, L.qb + i, r[h]);
+#pragma unroll
+      for (int e = 0; e < N; ++e) r[h][e] /= a.dt;
+    }
+  }
+  column_solve2(abc, L.col, L.m, nl, a.half, false, L.act0, L.act1, r[0], r[1], x[0], x[1], sh,
+                blockDim.x, threadIdx.x, a.half <= 64);
+  double rho[2], xr[2], ai[2];
+  double sums[2] = {0., 0.};
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int q = L.qb + 2 * L.m + h;
+    rho[h] = act[h] ? w[IDN * ncell + q] : 0.;
+    xr[h] = act[h] ? dux[IDN * ncell + q] : 0.;
+    ai[h] = rho[h] * vol[h];
+    sums[0] += ai[h] * ai[h];
+    sums[1] += act[h] ? (xr[h] - x[h][0]) * 1.0 * vol[h] : 0.;
+  }
+  column_sums<2>(sums, sh, threadIdx.x, a.half, a.hshift);
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    if (!act[h]) continue;
+    const int i = 2 * L.m + h, q = L.qb + i;
+    double* cdl = dl + ((int64_t)L.col * nl + i) * 2 * N;
+#pragma unroll
+    for (int e = 0; e < N; ++e) cdl[e] = x[h][e];
+    du[IDN * ncell + q] = (xr[h] + (x[h][0] - xr[h])) + rho[h] * ai[h] * sums[1] / sums[0];
+    du[IVX * ncell + q] = x[h][1];
+    du[IPR * ncell + q] = x[h][2];
+  }
+}
+
+// vic_adj_solve_kernel with two rows per lane (3x3 blocks)
+template <typename S, bool kSph>
+__global__ void __launch_bounds__(PcrShape<3>::kThreads, 2)
+    vic_adj_solve2_kernel(const S* __restrict__ w, const S* __restrict__ dux,
+                          const S* __restrict__ lam_du, const double* __restrict__ abc,
+                          S* __restrict__ lam_dx, S* __restrict__ lam_w,
+                          double* __restrict__ dl, AdjArgs a, int ncol) {
+  extern __shared__ double sh[];
+  constexpr int N = 3;
+  const int ncell = a.n1 * a.n2 * a.n3, nl = a.nx1;
+  const ColLane2 L = col_lane2(a, ncol);
+  const bool act[2] = {L.act0, L.act1};
+  double vol[2], rho[2], lr[2];
+  double sums[3] = {0., 0., 0.};
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int i = 2 * L.m + h, q = L.qb + i;
+    vol[h] = !kSph ? a.dx3 * (a.dx2 * a.dx1) : (act[h] ? sph_layer(a, L.col, i).vol : 1.);
+    const double d0 = act[h] ? dl[((int64_t)L.col * nl + i) * 2 * N] : 0.;
+    rho[h] = act[h] ? w[IDN * ncell + q] : 0.;
+    lr[h] = act[h] ? lam_du[IDN * ncell + q] : 0.;
+    const double ai = rho[h] * vol[h];
+    sums[0] += ai * ai;
+    sums[1] += act[h] ? ((double)dux[IDN * ncell + q] - d0) * 1.0 * vol[h] : 0.;
+    sums[2] += lr[h] * rho[h] * ai;
+  }
+  column_sums<3>(sums, sh, threadIdx.x, a.half, a.hshift);
+  const double Ssum = sums[0], Bd = sums[1], G = sums[2] / Ssum;
+  const double lamS = -G * Bd / Ssum;
+  double ld[2][N], y[2][N];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+#pragma unroll
+    for (int e = 0; e < N; ++e) ld[h][e] = 0.;
+    if (!act[h]) continue;
+    const int q = L.qb + 2 * L.m + h;
+    vic_pick<N>(lam_du, ncell, q, ld[h]);
+    ld[h][0] = lr[h] - G * vol[h];
+    lam_w[IDN * ncell + q] += lr[h] * 2. * rho[h] * vol[h] * Bd / Ssum +
+                              lamS * 2. * rho[h] * vol[h] * vol[h];
+  }
+  column_solve2(abc, L.col, L.m, nl, a.half, true, L.act0, L.act1, ld[0], ld[1], y[0], y[1], sh,
+                blockDim.x, threadIdx.x, a.half <= 64);
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    if (!act[h]) continue;
+    const int i = 2 * L.m + h, q = L.qb + i;
+    double* cdl = dl + ((int64_t)L.col * nl + i) * 2 * N;
+#pragma unroll
+    for (int e = 0; e < N; ++e) cdl[N + e] = y[h][e];
+    lam_dx[IDN * ncell + q] = G * vol[h] + y[h][0] / a.dt;
+    lam_dx[IVX * ncell + q] = y[h][1] / a.dt;
+    lam_dx[IPR * ncell + q] = y[h][2] / a.dt;
   }
 }
 
@@ -1360,6 +1601,9 @@ static AdjArgs adj_args(const snapx_config* c, double dt) {
   a.scheme = c->implicit_scheme;
   a.lanes = 64, a.lshift = 6;
   while (a.lanes < c->nx1) a.lanes <<= 1, a.lshift += 1;
+  const bool n3 = !((c->implicit_scheme >> 3) & 1);
+  a.half = (n3 && c->nx1 > 32 && a.lanes <= PcrShape<3>::kThreads) ? a.lanes / 2 : 0;
+  a.hshift = a.half ? a.lshift - 1 : 0;
   a.sph = c->coord == SNAPX_COORD_SPHERICAL_POLAR;
   a.geo = make_geometry(c);
   return a;
@@ -1483,6 +1727,14 @@ static int launch_vic_fwd(const AdjArgs& a, const AdjSpace<S>& s, int k, S* du, 
   hipLaunchKernelGGL((vic_asm_kernel<N, S, kSph>), blocks_for((int64_t)ncol * a.nx1, kT), dim3(kT), 0,
                      st, s.w[k], s.abc[k], a, ncol);
   SNAPX_LAUNCH_CHECK();
+  if (N == 3 && a.half) {
+    const int T3 = PcrShape<3>::kThreads, cpw = T3 >> a.hshift;
+    hipLaunchKernelGGL((vic_fwd_solve2_kernel<S, kSph>), dim3((ncol + cpw - 1) / cpw), dim3(T3),
+                       sizeof(double) * T3 * PcrShape<3>::kPub, st, s.w[k], s.dux[k], s.abc[k], du,
+                       s.dl[k], a, ncol);
+    SNAPX_LAUNCH_CHECK();
+    return 0;
+  }
   int cpw = T >> a.lshift;
   hipLaunchKernelGGL((vic_fwd_solve_kernel<N, S, kSph>), dim3((ncol + cpw - 1) / cpw), dim3(T),
                      sizeof(double) * T * PcrShape<N>::kPub, st, s.w[k], s.dux[k], s.abc[k], du,
@@ -1495,7 +1747,13 @@ template <int N, typename S, bool kSph>
 static int launch_vic_adj(const AdjArgs& a, const AdjSpace<S>& s, int k, int ncol,
                           hipStream_t st) {
   const int T = PcrShape<N>::kThreads;
-  if (adj_pcr(a, N)) {
+  if (N == 3 && a.half) {
+    // two rows per lane; blocks and delta come from the forward recompute
+    const int T3 = PcrShape<3>::kThreads, cpw = T3 >> a.hshift;
+    hipLaunchKernelGGL((vic_adj_solve2_kernel<S, kSph>), dim3((ncol + cpw - 1) / cpw), dim3(T3),
+                       sizeof(double) * T3 * PcrShape<3>::kPub, st, s.w[k], s.dux[k], s.lam_du,
+                       s.abc[k], s.lam_dx, s.lam_w, s.dl[k], a, ncol);
+  } else if (adj_pcr(a, N)) {
     // blocks and delta of this stage come from the forward recompute
     int cpw = T >> a.lshift;
     hipLaunchKernelGGL((vic_adj_solve_kernel<N, S, kSph>), dim3((ncol + cpw - 1) / cpw), dim3(T),

@@ -48,6 +48,13 @@ def _case(name):
         nx1 = 100 if name.endswith("1") else 150
         oc, w, u = _straka_small(int(name[-1]), nx2=24, nx1=nx1, wind=10.0)
         return oc, u, 0.2
+    if name.startswith("straka_mid"):
+        # columns of 44 cells: two rows per lane in 32-lane groups (two
+        # columns per wavefront).  (At 45 and 48 cells a face of the flow
+        # sits on an upwind switch, which central differences straddle; the
+        # odd height is in the autograd comparison.)
+        oc, w, u = _straka_small(int(name[-1]), nx2=24, nx1=44, wind=10.0)
+        return oc, u, 0.2
     if name.startswith("straka"):
         oc, w, u = _straka_small(int(name[6]), nx2=24, nx1=16, wind=10.0)
         return oc, u, 0.2
@@ -67,7 +74,8 @@ def _case(name):
         # periodic) of tests/test_gpu_spherical.py: "sph<dims>_<solver><scheme>"
         from test_gpu_spherical import _sector
         dims, rest = name[3:].split("_")
-        n = (14, 10, 1) if dims == "2d" else (10, 8, 6)
+        # 2dtall: 40-cell radial columns (two rows per lane)
+        n = {"2d": (14, 10, 1), "2dtall": (40, 10, 1), "3d": (10, 8, 6)}[dims]
         oc, w, u = _sector(rest[:-1], int(rest[-1]), n=n, seed=9)
         return oc, u, 0.01
     if name == "sod":
@@ -102,11 +110,12 @@ def _bc_consistent(cfg, v):
     return v
 
 
-CASES = ["straka0", "straka1", "straka9", "straka_tall1", "straka_tall9", "box_lmars_roe",
-         "box_hllc", "box_nograv", "sod"]
+CASES = ["straka0", "straka1", "straka9", "straka_mid1", "straka_tall1", "straka_tall9",
+         "box_lmars_roe", "box_hllc", "box_nograv", "sod"]
 # spherical-polar blocks: per-cell areas and volumes, the geometric sources
 # (with the x1 face pressure), per-layer column geometry and ghost distances
-SPH_CASES = ["sph3d_hllc1", "sph3d_lmars9", "sph3d_roe0", "sph2d_hllc1", "sph2d_roe9"]
+SPH_CASES = ["sph3d_hllc1", "sph3d_lmars9", "sph3d_roe0", "sph2d_hllc1", "sph2d_roe9",
+             "sph2dtall_hllc1"]
 
 
 @pytest.mark.parametrize("case", CASES + SPH_CASES)
@@ -220,15 +229,19 @@ def _box_autograd(n=(24, 32, 32)):
     return _box3d(n=n)
 
 
-@pytest.mark.parametrize("case", ["box_32x32x24", "straka9"])
+@pytest.mark.parametrize("case", ["box_32x32x24", "straka9", "straka1_45"])
 def test_adjoint_matches_torch_autograd(case):
     """The hand-written adjoint (GPU) against libtorch autograd through the
     differentiable restatement of the step (oracle/torch_oracle.py, CPU FP64,
     itself pinned to the C oracle): full gradients, <= 1e-9 of the largest
-    entry per variable."""
+    entry per variable.  straka1_45: 45-cell 3x3 columns, two rows per lane
+    in 32-lane groups with an odd row count."""
     import torch_oracle as T
     if case == "straka9":
         oc, w, u_np = _straka_small(9, nx2=24, nx1=16, wind=10.0)
+        dt = 0.2
+    elif case == "straka1_45":
+        oc, w, u_np = _straka_small(1, nx2=12, nx1=45, wind=10.0)
         dt = 0.2
     else:
         oc, w, u_np = _box_autograd()

@@ -23,8 +23,8 @@ from test_gpu_plain import cfg_of, to_dev
 
 pytestmark = pytest.mark.gpu
 
-CASES = ["straka1", "straka9", "box_lmars_roe", "box_hllc", "straka_tall1", "sph3d_hllc1",
-         "sph2d_roe9"]
+CASES = ["straka1", "straka9", "box_lmars_roe", "box_hllc", "straka_mid1", "straka_tall1",
+         "sph3d_hllc1", "sph2d_roe9", "sph2dtall_hllc1"]
 
 
 def _lam(u0, seed):
