@@ -84,11 +84,13 @@ def test_fused_straka_multistep_error_and_mass():
 
 
 @pytest.mark.parametrize("scheme", [1, 9])
-@pytest.mark.parametrize("nx1", [64, 100, 200, 300])
+@pytest.mark.parametrize("nx1", [64, 65, 99, 100, 128, 200, 300])
 def test_fused_column_heights(nx1, scheme):
     """Columns of one wavefront, of several (cross-wavefront PCR barriers and
     column sums), and taller than the column workgroup (the lane-per-column
-    block Thomas kernel), with a ragged x2 tile (6 columns, tiles of 4)."""
+    block Thomas kernel), with a ragged x2 tile (6 columns, tiles of 4).
+    Scheme 1 at 65-128 cells: one wavefront of two rows per lane (65 and 99
+    cells leave an odd row count after the in-lane reduction)."""
     oc, w, u = _straka_small(scheme, nx2=6, nx1=nx1, wind=10.0)  # dx1 = 6.4 km / nx1
     dt = 0.2
     u_ref, _ = O.rk3_step(oc, dt, u)
@@ -134,3 +136,18 @@ def test_fused_ragged_last_x1_tile_refill(nx1):
     capi.fused_rk3_step(cfg_of(oc), 0.2, uf)
     e = var_errors(host(uf), u_ref, O.interior(oc))
     assert (e < 1e-5).all(), e
+
+
+@pytest.mark.parametrize("nx1", [66, 99, 128])
+def test_fused_packed_columns_full_workgroups(nx1):
+    """2048 columns, so that K2 runs workgroups of four one-wavefront
+    columns of two rows per lane (one step against the FP64 oracle, the 1e-5
+    bar)."""
+    oc, w, u = _straka_small(1, nx2=2048, nx1=nx1, wind=10.0)
+    dt = 0.2
+    u_ref, _ = O.rk3_step(oc, dt, u)
+    cfg = cfg_of(oc)
+    uf = to_dev(u, torch.float32)
+    capi.fused_rk3_step(cfg, dt, uf)
+    e = var_errors(host(uf), u_ref, O.interior(oc))
+    assert (e < 1e-5).all(), (nx1, e)
